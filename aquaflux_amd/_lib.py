@@ -1,0 +1,154 @@
+"""ctypes binding of ``libaquaflux_amd.so`` (the C ABI declared in ``include/aquaflux_amd.h``).
+
+The library is loaded lazily on first use.  There is no fallback: when the shared library is
+missing or a GPU is not available, every compute entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+
+import torch
+
+_LIB = None
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "libaquaflux_amd.so"
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_int32_p = ctypes.POINTER(ctypes.c_int32)
+c_int64_p = ctypes.POINTER(ctypes.c_int64)
+c_void_p = ctypes.c_void_p
+
+
+class AfxGeometry(ctypes.Structure):
+    _fields_ = [
+        ("area", c_void_p),
+        ("normal", c_void_p),
+        ("face_centroid", c_void_p),
+        ("volume", c_void_p),
+        ("cell_centroid", c_void_p),
+    ]
+
+
+class AfxFlowConfig(ctypes.Structure):
+    _fields_ = [
+        ("advection", ctypes.c_int32),
+        ("gradient", ctypes.c_int32),
+        ("pressure_pin", ctypes.c_int64),
+        ("pressure_pin_value", ctypes.c_double),
+        ("body_force", ctypes.c_double * 3),
+        ("limiter_k", ctypes.c_double),
+        ("face_kind", c_void_p),
+    ]
+
+
+class AfxFlowInputs(ctypes.Structure):
+    _fields_ = [
+        ("state", c_void_p),
+        ("mu", c_void_p),
+        ("rho", c_void_p),
+        ("u_b", c_void_p),
+        ("p_b", c_void_p),
+        ("gamma_b", c_void_p),
+    ]
+
+
+class AfxFlowCotangents(ctypes.Structure):
+    _fields_ = [
+        ("state", c_void_p),
+        ("mu", c_void_p),
+        ("rho", c_void_p),
+        ("u_b", c_void_p),
+        ("p_b", c_void_p),
+        ("gamma_b", c_void_p),
+    ]
+
+
+# Declared entry points: name -> (argtypes). Every function returns int except afx_last_error.
+_SIGNATURES = {
+    "afx_mesh_create": [
+        ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+        c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_void_p),
+    ],
+    "afx_mesh_destroy": [c_void_p],
+    "afx_mesh_info": [c_void_p, c_void_p],
+    "afx_mesh_cell_faces": [c_void_p, c_void_p, c_void_p, c_void_p],
+    "afx_geometry_compute": [c_void_p, c_void_p, ctypes.POINTER(AfxGeometry), c_void_p],
+    "afx_flow_workspace_bytes": [c_void_p, ctypes.c_int, c_int64_p],
+    "afx_flow_residual": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxFlowConfig), ctypes.POINTER(AfxFlowInputs),
+        c_void_p, c_void_p, ctypes.c_int64, c_void_p,
+    ],
+    "afx_flow_jvp": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxFlowConfig), ctypes.POINTER(AfxFlowInputs),
+        ctypes.POINTER(AfxFlowInputs), c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p,
+    ],
+    "afx_flow_vjp": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxFlowConfig), ctypes.POINTER(AfxFlowInputs),
+        c_void_p, ctypes.POINTER(AfxFlowCotangents), c_void_p, ctypes.c_int64, c_void_p,
+    ],
+    "afx_flow_fields": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxFlowConfig), ctypes.POINTER(AfxFlowInputs),
+        c_void_p, c_void_p, c_void_p, c_void_p,
+    ],
+}
+
+EXTRA_SIGNATURES: dict = {}
+
+
+def lib():
+    """The loaded library (raises when it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        path = os.environ.get("AFX_LIBRARY", str(LIB_PATH))
+        if not os.path.exists(path):
+            raise RuntimeError(
+                f"aquaflux_amd HIP library not found at {path}; build it with "
+                "`python -m aquaflux_amd.build` (hipcc --offload-arch=gfx950)"
+            )
+        handle = ctypes.CDLL(path)
+        handle.afx_last_error.restype = ctypes.c_char_p
+        handle.afx_last_error.argtypes = []
+        for name, argtypes in {**_SIGNATURES, **EXTRA_SIGNATURES}.items():
+            fn = getattr(handle, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = argtypes
+        _LIB = handle
+    return _LIB
+
+
+def exported_symbols() -> list[str]:
+    """Every entry point this binding declares (the checks compare it with the header)."""
+    return ["afx_last_error", *_SIGNATURES, *EXTRA_SIGNATURES]
+
+
+def check(rc: int, what: str) -> None:
+    if rc == 0:
+        return
+    msg = lib().afx_last_error().decode(errors="replace")
+    if rc == 2:
+        raise ValueError(f"{what}: {msg}")
+    raise RuntimeError(f"{what}: {msg}")
+
+
+def ptr(t) -> int | None:
+    """Device (or host) address of a tensor / ndarray, None for None."""
+    if t is None:
+        return None
+    if isinstance(t, torch.Tensor):
+        return t.data_ptr()
+    return t.ctypes.data
+
+
+def stream_of(device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def require_cuda(t: torch.Tensor, name: str) -> None:
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ValueError(f"{name} must be a CUDA tensor")
+    if t.dtype != torch.float64:
+        raise ValueError(f"{name} must be float64, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")

@@ -1,0 +1,254 @@
+// Shared device/host helpers for the aquaflux MI355X kernels (gfx950, wave64).
+//
+// * `dual`  -- a value/tangent pair.  The forward kernels are templates over the scalar type, so
+//   instantiating them with `dual` gives the exact tangent (JVP) of the same arithmetic.  The
+//   non-smooth primitives follow JAX's conventions, because the reference differentiates with JAX:
+//   `maximum`/`minimum` split the tangent 1/2-1/2 on a tie, `where` sends it to the taken branch,
+//   `abs` has derivative sign(x) (0 at 0).
+// * `In<S>` / `Out<S>` -- structure-of-arrays views: a value pointer plus an optional tangent
+//   pointer (null = zero tangent).
+// * error plumbing -- every C entry point returns an int status; the message of the last failure
+//   on the calling thread is kept for `afx_last_error`.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+#include <string>
+
+#ifndef AFX_DEV
+#define AFX_DEV __device__ __forceinline__
+#endif
+#define AFX_HD __host__ __device__ __forceinline__
+
+namespace afx {
+
+// ---------------------------------------------------------------------------------------------
+// dual numbers
+struct dual {
+  double v, d;
+};
+
+AFX_HD dual mkd(double v, double d) { return dual{v, d}; }
+AFX_HD double val(double x) { return x; }
+AFX_HD double val(const dual& x) { return x.v; }
+AFX_HD double tan_of(double) { return 0.0; }
+AFX_HD double tan_of(const dual& x) { return x.d; }
+
+AFX_HD dual operator+(dual a, dual b) { return {a.v + b.v, a.d + b.d}; }
+AFX_HD dual operator-(dual a, dual b) { return {a.v - b.v, a.d - b.d}; }
+AFX_HD dual operator*(dual a, dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+AFX_HD dual operator/(dual a, dual b) {
+  double q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+AFX_HD dual operator-(dual a) { return {-a.v, -a.d}; }
+AFX_HD dual operator+(dual a, double b) { return {a.v + b, a.d}; }
+AFX_HD dual operator+(double a, dual b) { return {a + b.v, b.d}; }
+AFX_HD dual operator-(dual a, double b) { return {a.v - b, a.d}; }
+AFX_HD dual operator-(double a, dual b) { return {a - b.v, -b.d}; }
+AFX_HD dual operator*(dual a, double b) { return {a.v * b, a.d * b}; }
+AFX_HD dual operator*(double a, dual b) { return {a * b.v, a * b.d}; }
+AFX_HD dual operator/(dual a, double b) { return {a.v / b, a.d / b}; }
+AFX_HD dual operator/(double a, dual b) {
+  double q = a / b.v;
+  return {q, -q * b.d / b.v};
+}
+AFX_HD dual& operator+=(dual& a, dual b) { a.v += b.v; a.d += b.d; return a; }
+AFX_HD dual& operator-=(dual& a, dual b) { a.v -= b.v; a.d -= b.d; return a; }
+
+AFX_HD double dsqrt(double x) { return sqrt(x); }
+AFX_HD dual dsqrt(dual x) {
+  double s = sqrt(x.v);
+  return {s, x.d * 0.5 / s};
+}
+AFX_HD double dtanh(double x) { return tanh(x); }
+AFX_HD dual dtanh(dual x) {
+  double t = tanh(x.v);
+  return {t, x.d * (1.0 - t * t)};
+}
+AFX_HD double dlog(double x) { return log(x); }
+AFX_HD dual dlog(dual x) { return {log(x.v), x.d / x.v}; }
+AFX_HD double dexp(double x) { return exp(x); }
+AFX_HD dual dexp(dual x) {
+  double e = exp(x.v);
+  return {e, x.d * e};
+}
+AFX_HD double dabs(double x) { return fabs(x); }
+AFX_HD dual dabs(dual x) {
+  double s = x.v > 0.0 ? 1.0 : (x.v < 0.0 ? -1.0 : 0.0);
+  return {fabs(x.v), s * x.d};
+}
+
+// JAX `maximum` / `minimum`: tangent split evenly on a tie.
+AFX_HD double dmax(double a, double b) { return a > b ? a : b; }
+AFX_HD double dmin(double a, double b) { return a < b ? a : b; }
+AFX_HD dual dmax(dual a, dual b) {
+  if (a.v > b.v) return a;
+  if (a.v < b.v) return b;
+  return {a.v, 0.5 * (a.d + b.d)};
+}
+AFX_HD dual dmin(dual a, dual b) {
+  if (a.v < b.v) return a;
+  if (a.v > b.v) return b;
+  return {a.v, 0.5 * (a.d + b.d)};
+}
+AFX_HD dual dmax(dual a, double b) { return dmax(a, dual{b, 0.0}); }
+AFX_HD dual dmin(dual a, double b) { return dmin(a, dual{b, 0.0}); }
+AFX_HD dual dmax(double a, dual b) { return dmax(dual{a, 0.0}, b); }
+AFX_HD dual dmin(double a, dual b) { return dmin(dual{a, 0.0}, b); }
+
+// d max(x, 0) / dx with the tie convention (used by the hand-written adjoints).
+AFX_HD double dmax0_dx(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? 0.0 : 0.5); }
+
+template <class S>
+AFX_HD S sconst(double v) { return S(v); }
+template <>
+AFX_HD dual sconst<dual>(double v) { return dual{v, 0.0}; }
+
+// ---------------------------------------------------------------------------------------------
+// structure-of-arrays views
+template <class S>
+struct In;
+template <>
+struct In<double> {
+  const double* v = nullptr;
+  const double* t = nullptr;
+  AFX_DEV double operator[](int64_t i) const { return v[i]; }
+};
+template <>
+struct In<dual> {
+  const double* v = nullptr;
+  const double* t = nullptr;
+  AFX_DEV dual operator[](int64_t i) const { return dual{v[i], t ? t[i] : 0.0}; }
+};
+
+template <class S>
+struct Out;
+template <>
+struct Out<double> {
+  double* v = nullptr;
+  double* t = nullptr;
+  AFX_DEV void set(int64_t i, double x) const { v[i] = x; }
+};
+template <>
+struct Out<dual> {
+  double* v = nullptr;
+  double* t = nullptr;
+  AFX_DEV void set(int64_t i, dual x) const {
+    v[i] = x.v;
+    t[i] = x.d;
+  }
+};
+
+// Read-only geometry/topology of a mesh, as device pointers (all SoA: vec[j * n + i]).
+struct MeshView {
+  int32_t C, F;
+  const int32_t* owner;
+  const int32_t* nb;     // -1 on boundary faces
+  const int32_t* cptr;   // [C+1] cell -> face CSR
+  const int32_t* cface;  // f (cell owns f) or ~f (cell is f's neighbour)
+  const double* area;    // [F]
+  const double* nrm;     // [d][F] owner-outward unit normal
+  const double* fc;      // [d][F] face centroid
+  const double* vol;     // [C]
+  const double* cc;      // [d][C] cell centroid
+};
+
+// Per-face geometry seen from a face's owner (P) and neighbour (N).
+template <int D>
+struct FaceGeo {
+  double A, n[D], xf[D], xP[D], xN[D];
+  double g;      // interpolation factor (0 on boundary faces)
+  double dvec[D];  // xN - xP (0 on boundary faces)
+  double dist;   // normal distance: d.n (interior) or (xf - xP).n (boundary)
+  double skew[D];  // xf - (xP + g d)
+  double dp[D], dn[D];  // xf - xP, xf - xN
+  double dpn, dnn;
+};
+
+template <int D>
+AFX_DEV double vdot(const double* a, const double* b) {
+  double s = a[0] * b[0];
+#pragma unroll
+  for (int j = 1; j < D; ++j) s = s + a[j] * b[j];
+  return s;
+}
+
+// Loads the geometry of face f; `P`/`N` are its owner and neighbour (N < 0 on a boundary face).
+template <int D>
+AFX_DEV void load_face_geo(const MeshView& m, int32_t f, int32_t P, int32_t N, FaceGeo<D>& q) {
+  q.A = m.area[f];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    q.n[j] = m.nrm[(int64_t)j * m.F + f];
+    q.xf[j] = m.fc[(int64_t)j * m.F + f];
+    q.xP[j] = m.cc[(int64_t)j * m.C + P];
+    q.xN[j] = N >= 0 ? m.cc[(int64_t)j * m.C + N] : q.xP[j];
+    q.dvec[j] = q.xN[j] - q.xP[j];
+    q.dp[j] = q.xf[j] - q.xP[j];
+    q.dn[j] = q.xf[j] - q.xN[j];
+  }
+  q.dpn = vdot<D>(q.dp, q.n);
+  q.dnn = vdot<D>(q.dn, q.n);
+  if (N >= 0) {
+    double dd = vdot<D>(q.dvec, q.dvec);
+    q.g = vdot<D>(q.dp, q.dvec) / dd;
+    q.dist = vdot<D>(q.dvec, q.n);
+  } else {
+    q.g = 0.0;
+    q.dist = q.dpn;
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) q.skew[j] = q.xf[j] - (q.xP[j] + q.dvec[j] * q.g);
+}
+
+// Decodes one entry of the cell -> face list.
+struct FaceRef {
+  int32_t f, P, N;  // face, owner, neighbour (-1 boundary)
+  bool own;         // the visiting cell is the owner
+};
+AFX_DEV FaceRef decode_face(const MeshView& m, int32_t e) {
+  FaceRef r;
+  r.own = e >= 0;
+  r.f = r.own ? e : ~e;
+  r.P = m.owner[r.f];
+  r.N = m.nb[r.f];
+  return r;
+}
+
+constexpr int kBlock = 256;
+inline dim3 grid_for(int64_t n, int block = kBlock) { return dim3((unsigned)((n + block - 1) / block)); }
+
+}  // namespace afx
+
+// ---------------------------------------------------------------------------------------------
+// host-side error plumbing (one message per calling thread)
+void afx_set_error(const std::string& msg);
+const char* afx_get_error();
+
+#define AFX_CHECK_HIP(expr)                                                          \
+  do {                                                                               \
+    hipError_t _e = (expr);                                                          \
+    if (_e != hipSuccess) {                                                          \
+      afx_set_error(std::string(#expr) + ": " + hipGetErrorString(_e));              \
+      return 1;                                                                      \
+    }                                                                                \
+  } while (0)
+
+#define AFX_CHECK_LAUNCH()                                                           \
+  do {                                                                               \
+    hipError_t _e = hipGetLastError();                                               \
+    if (_e != hipSuccess) {                                                          \
+      afx_set_error(std::string("kernel launch: ") + hipGetErrorString(_e));          \
+      return 1;                                                                      \
+    }                                                                                \
+  } while (0)
+
+#define AFX_REQUIRE(cond, msg)                                                       \
+  do {                                                                               \
+    if (!(cond)) {                                                                   \
+      afx_set_error(msg);                                                            \
+      return 2;                                                                      \
+    }                                                                                \
+  } while (0)

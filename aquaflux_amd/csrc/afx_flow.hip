@@ -1,0 +1,926 @@
+// Coupled (u, v[, w], p) residual R(state), its tangent (JVP) and its adjoint (VJP).
+//
+// Three cell-centric passes, one thread per cell walking its face list (fixed order, no atomics):
+//   G  gradients of u_i and p (compact Green-Gauss or least squares) + Venkatakrishnan limiter
+//   D  momentum diagonal a_P (flow/momentum.py:596-635, flow/rhie_chow.py:81-175)
+//   R  face fluxes (diffusion, pressure, advection, Rhie-Chow mass flux) summed per cell
+// An interior face is evaluated by both of its cells with identical inputs, so the owner's +flux
+// and the neighbour's -flux are the same number: conservation holds and no face array is stored.
+// The JVP instantiates the same passes on `dual`.  The VJP recomputes G and D, then runs the
+// hand-written reverse passes R', D', G' (each again one thread per cell, gather form).
+#include <type_traits>
+
+#include "afx_common.h"
+#include "afx_mesh.h"
+#include "afx_flow_face.h"
+
+using namespace afx;
+
+namespace {
+
+template <class S>
+struct FlowArgs {
+  const int8_t* kind;
+  In<S> x, ub, pb, gB, mu, rho;  // state [(D+1)][C], ub [D][F], pb [F], gB [F] (optional)
+  In<S> G, aP, psi;              // pass products (read in later passes)
+  int32_t pin;
+  double pin_value;
+  double bf[3];
+  double lim_k;
+};
+
+template <int D, class S>
+AFX_DEV void load_state(const FlowArgs<S>& a, int32_t C, int32_t c, FlowCell<D, S>& X) {
+#pragma unroll
+  for (int i = 0; i < D; ++i) X.u[i] = a.x[(int64_t)i * C + c];
+  X.p = a.x[(int64_t)D * C + c];
+}
+
+template <int D, class S>
+AFX_DEV void load_grad(const FlowArgs<S>& a, int32_t C, int32_t c, FlowCell<D, S>& X) {
+  constexpr int NV = D + 1;
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) X.Gu[i][j] = a.G[(int64_t)(i * D + j) * C + c];
+#pragma unroll
+  for (int j = 0; j < D; ++j) X.Gp[j] = a.G[(int64_t)((NV - 1) * D + j) * C + c];
+}
+
+template <int D, class S>
+AFX_DEV void load_cell_full(const FlowArgs<S>& a, const MeshView& m, int32_t c, bool need_ap, bool need_psi,
+                            FlowCell<D, S>& X) {
+  load_state<D, S>(a, m.C, c, X);
+  load_grad<D, S>(a, m.C, c, X);
+  X.mu = a.mu[c];
+  X.rho = a.rho[c];
+  X.V = m.vol[c];
+  X.aP = need_ap ? a.aP[c] : sconst<S>(1.0);
+#pragma unroll
+  for (int i = 0; i < D; ++i) X.psi[i] = need_psi ? a.psi[(int64_t)i * m.C + c] : sconst<S>(1.0);
+}
+
+// Flow boundary value of variable v (u_0..u_{D-1}, p) on boundary face f owned by cell P.
+template <int D, class S>
+AFX_DEV S flow_bval(const FlowArgs<S>& a, int kind, int v, int32_t F, int32_t f, const S* xP) {
+  if (v < D) {
+    if (kind == kFlowWall || kind == kFlowInlet) return a.ub[(int64_t)v * F + f];
+    if (kind == kFlowOutlet) return xP[v];
+    return sconst<S>(0.0);
+  }
+  if (kind == kFlowWall || kind == kFlowInlet) return xP[D];
+  if (kind == kFlowOutlet) return a.pb[f];
+  return sconst<S>(0.0);
+}
+
+// Least-squares normal matrix of cell c (geometry only): M = sum_f w dx dx^T, w = 1/|dx|^2.
+template <int D>
+AFX_DEV void lsq_matrix(const MeshView& m, int32_t c, double M[D][D]) {
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int b = 0; b < D; ++b) M[a][b] = 0.0;
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    int32_t o = r.own ? r.N : r.P;
+    double dx[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+      dx[j] = (o >= 0 ? m.cc[(int64_t)j * m.C + o] : m.fc[(int64_t)j * m.F + r.f]) - m.cc[(int64_t)j * m.C + c];
+    double w = 1.0 / vdot<D>(dx, dx);
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int b = 0; b < D; ++b) M[a][b] += w * dx[a] * dx[b];
+  }
+}
+
+// Inverse of a symmetric positive-definite D x D matrix (adjugate / determinant).
+template <int D>
+AFX_DEV void sym_inverse(const double M[D][D], double I[D][D]) {
+  if (D == 2) {
+    double det = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+    I[0][0] = M[1][1] / det;
+    I[1][1] = M[0][0] / det;
+    I[0][1] = -M[0][1] / det;
+    I[1][0] = -M[1][0] / det;
+  } else {
+    double c00 = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+    double c01 = M[1][2] * M[2][0] - M[1][0] * M[2][2];
+    double c02 = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+    double det = M[0][0] * c00 + M[0][1] * c01 + M[0][2] * c02;
+    double inv = 1.0 / det;
+    I[0][0] = c00 * inv;
+    I[1][0] = c01 * inv;
+    I[2][0] = c02 * inv;
+    I[0][1] = (M[0][2] * M[2][1] - M[0][1] * M[2][2]) * inv;
+    I[1][1] = (M[0][0] * M[2][2] - M[0][2] * M[2][0]) * inv;
+    I[2][1] = (M[0][1] * M[2][0] - M[0][0] * M[2][1]) * inv;
+    I[0][2] = (M[0][1] * M[1][2] - M[0][2] * M[1][1]) * inv;
+    I[1][2] = (M[0][2] * M[1][0] - M[0][0] * M[1][2]) * inv;
+    I[2][2] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) * inv;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// pass G: gradients (+ limiter)
+template <int D, class S, int GRAD, bool LIM>
+__global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Out<S> G, Out<S> psi_out,
+                                                   int32_t* __restrict__ ext) {
+  constexpr int NV = D + 1;
+  int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.C) return;
+  S xc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) xc[v] = a.x[(int64_t)v * m.C + c];
+  S acc[NV][D];
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int j = 0; j < D; ++j) acc[v][j] = sconst<S>(0.0);
+  double M[D][D];
+  if (GRAD == 1) {
+#pragma unroll
+    for (int p = 0; p < D; ++p)
+#pragma unroll
+      for (int q = 0; q < D; ++q) M[p][q] = 0.0;
+  }
+  S mx[D], mn[D];
+  int32_t amx[D], amn[D];
+  if (LIM) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      mx[i] = xc[i];
+      mn[i] = xc[i];
+      amx[i] = c;
+      amn[i] = c;
+    }
+  }
+  const int32_t kb = m.cptr[c], ke = m.cptr[c + 1];
+  for (int32_t k = kb; k < ke; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    if (o >= 0) {
+      S xo[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) xo[v] = a.x[(int64_t)v * m.C + o];
+      if (LIM) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+          if (val(xo[i]) > val(mx[i])) {
+            mx[i] = xo[i];
+            amx[i] = o;
+          }
+          if (val(xo[i]) < val(mn[i])) {
+            mn[i] = xo[i];
+            amn[i] = o;
+          }
+        }
+      }
+      if (GRAD == 0) {
+        FaceGeo<D> q;
+        load_face_geo<D>(m, r.f, r.P, r.N, q);
+        const double w = 1.0 - q.g;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          S phiP = r.own ? xc[v] : xo[v];
+          S phiN = r.own ? xo[v] : xc[v];
+          S phi = w * phiP + q.g * phiN;
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+            S t = phi * (q.n[j] * q.A);
+            acc[v][j] = r.own ? acc[v][j] + t : acc[v][j] - t;
+          }
+        }
+      } else {
+        double dx[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dx[j] = m.cc[(int64_t)j * m.C + o] - m.cc[(int64_t)j * m.C + c];
+        const double wgt = 1.0 / vdot<D>(dx, dx);
+#pragma unroll
+        for (int p = 0; p < D; ++p)
+#pragma unroll
+          for (int q2 = 0; q2 < D; ++q2) M[p][q2] += wgt * dx[p] * dx[q2];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          S dphi = xo[v] - xc[v];
+#pragma unroll
+          for (int j = 0; j < D; ++j) acc[v][j] += (wgt * dx[j]) * dphi;
+        }
+      }
+    } else {
+      const int kind = a.kind[r.f];
+      if (GRAD == 0) {
+        const double A = m.area[r.f];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          S phi = flow_bval<D, S>(a, kind, v, m.F, r.f, xc);
+#pragma unroll
+          for (int j = 0; j < D; ++j) acc[v][j] += phi * (m.nrm[(int64_t)j * m.F + r.f] * A);
+        }
+      } else {
+        double dx[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dx[j] = m.fc[(int64_t)j * m.F + r.f] - m.cc[(int64_t)j * m.C + c];
+        const double wgt = 1.0 / vdot<D>(dx, dx);
+#pragma unroll
+        for (int p = 0; p < D; ++p)
+#pragma unroll
+          for (int q2 = 0; q2 < D; ++q2) M[p][q2] += wgt * dx[p] * dx[q2];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          S dphi = flow_bval<D, S>(a, kind, v, m.F, r.f, xc) - xc[v];
+#pragma unroll
+          for (int j = 0; j < D; ++j) acc[v][j] += (wgt * dx[j]) * dphi;
+        }
+      }
+    }
+  }
+  S grad[NV][D];
+  if (GRAD == 0) {
+    const double iv = 1.0 / m.vol[c];
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int j = 0; j < D; ++j) grad[v][j] = acc[v][j] * iv;
+  } else {
+    double I[D][D];
+    sym_inverse<D>(M, I);
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int p = 0; p < D; ++p) {
+        S s = I[p][0] * acc[v][0];
+#pragma unroll
+        for (int q2 = 1; q2 < D; ++q2) s = s + I[p][q2] * acc[v][q2];
+        grad[v][p] = s;
+      }
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int j = 0; j < D; ++j) G.set((int64_t)(v * D + j) * m.C + c, grad[v][j]);
+  if (LIM) {
+    const double e2 = m.vol[c] * (a.lim_k * a.lim_k * a.lim_k);
+    double xcen[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) xcen[j] = m.cc[(int64_t)j * m.C + c];
+    S lo[D];
+    bool first = true;
+    for (int32_t k = kb; k < ke; ++k) {
+      int32_t e = m.cface[k];
+      int32_t f = e >= 0 ? e : ~e;
+      double dxf[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) dxf[j] = m.fc[(int64_t)j * m.F + f] - xcen[j];
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        S dm = grad[i][0] * dxf[0];
+#pragma unroll
+        for (int j = 1; j < D; ++j) dm = dm + grad[i][j] * dxf[j];
+        S vf = venkat_face<S>(dm, mx[i] - xc[i], mn[i] - xc[i], e2);
+        if (first || val(vf) < val(lo[i])) lo[i] = vf;
+      }
+      first = false;
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      S ps = dmin(dmax(lo[i], 0.0), 1.0);
+      psi_out.set((int64_t)i * m.C + c, ps);
+      if (ext) {
+        ext[(int64_t)i * m.C + c] = amx[i];
+        ext[(int64_t)(D + i) * m.C + c] = amn[i];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// pass D: momentum diagonal a_P
+template <int D, class S, int ADV>
+__global__ void __launch_bounds__(256) k_flow_ap(MeshView m, FlowArgs<S> a, Out<S> aP) {
+  int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.C) return;
+  FlowCell<D, S> X;
+  load_state<D, S>(a, m.C, c, X);
+  load_grad<D, S>(a, m.C, c, X);
+  X.mu = a.mu[c];
+  X.rho = a.rho[c];
+  X.V = m.vol[c];
+  S acc = sconst<S>(0.0);
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    FaceGeo<D> q;
+    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    if (o >= 0) {
+      FlowCell<D, S> O;
+      load_state<D, S>(a, m.C, o, O);
+      load_grad<D, S>(a, m.C, o, O);
+      O.mu = a.mu[o];
+      O.rho = a.rho[o];
+      O.V = m.vol[o];
+      S oc, nc;
+      if (r.own)
+        ap_interior<D, ADV, S>(q, X, O, oc, nc);
+      else
+        ap_interior<D, ADV, S>(q, O, X, oc, nc);
+      acc += r.own ? oc : nc;
+    } else {
+      S gb = a.gB.v ? a.gB[r.f] : X.mu;
+      acc += ap_boundary<D, ADV, S>(q, a.kind[r.f], X, gb);
+    }
+  }
+  aP.set(c, acc);
+}
+
+// ---------------------------------------------------------------------------------------------
+// pass R: residual
+template <int D, class S, int ADV>
+__global__ void __launch_bounds__(256) k_flow_res(MeshView m, FlowArgs<S> a, Out<S> R) {
+  constexpr int NV = D + 1;
+  int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.C) return;
+  FlowCell<D, S> X;
+  load_cell_full<D, S>(a, m, c, true, ADV == kAdvLimited, X);
+  S acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = sconst<S>(0.0);
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    FaceGeo<D> q;
+    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    S out[NV];
+    if (o >= 0) {
+      FlowCell<D, S> O;
+      load_cell_full<D, S>(a, m, o, true, ADV == kAdvLimited, O);
+      if (r.own)
+        flow_flux_interior<D, ADV, S>(q, X, O, out);
+      else
+        flow_flux_interior<D, ADV, S>(q, O, X, out);
+    } else {
+      S ub[D];
+#pragma unroll
+      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.f];
+      S gb = a.gB.v ? a.gB[r.f] : X.mu;
+      flow_flux_boundary<D, ADV, S>(q, a.kind[r.f], X, ub, a.pb[r.f], gb, out);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = r.own ? acc[v] + out[v] : acc[v] - out[v];
+  }
+#pragma unroll
+  for (int i = 0; i < D; ++i) R.set((int64_t)i * m.C + c, acc[i] - a.bf[i] * X.V);
+  S cont = acc[D];
+  if (c == a.pin) cont = X.p - a.pin_value;
+  R.set((int64_t)D * m.C + c, cont);
+}
+
+// ---------------------------------------------------------------------------------------------
+// reverse pass R': cotangent of the residual -> cell-local cotangents of state, gradients, a_P,
+// mu, rho, psi (written), and of the boundary-face data of faces this cell owns.
+struct FlowBars {
+  double* x;      // [(D+1)][C]
+  double* G;      // [(D+1)D][C]
+  double* aP;     // [C]
+  double* psi;    // [D][C]
+  double* mu;     // [C]
+  double* rho;    // [C]
+  double* ub;     // [D][F]
+  double* pb;     // [F]
+  double* gB;     // [F] (only with a boundary-coefficient override)
+  double* ext;    // [2D][C] cotangents of the limiter stencil extrema
+};
+
+template <int D, int ADV>
+__global__ void __launch_bounds__(256) k_flow_res_adj(MeshView m, FlowArgs<double> a, const double* __restrict__ lam,
+                                                      FlowBars bar) {
+  constexpr int NV = D + 1;
+  int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.C) return;
+  FlowCell<D, double> X;
+  load_cell_full<D, double>(a, m, c, true, ADV == kAdvLimited, X);
+  FlowCellBar<D> B;
+  B.zero();
+  double lc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) lc[v] = lam[(int64_t)v * m.C + c];
+  const double lc_cont = (c == a.pin) ? 0.0 : lc[D];
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    FaceGeo<D> q;
+    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    if (o >= 0) {
+      FlowCell<D, double> O;
+      load_cell_full<D, double>(a, m, o, true, ADV == kAdvLimited, O);
+      double lf[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        double lo = lam[(int64_t)v * m.C + o];
+        if (v == D && o == a.pin) lo = 0.0;
+        double lcv = v == D ? lc_cont : lc[v];
+        lf[v] = r.own ? lcv - lo : lo - lcv;
+      }
+      if (r.own)
+        adj_flux_interior<D, ADV>(q, X, O, lf, 0, B);
+      else
+        adj_flux_interior<D, ADV>(q, O, X, lf, 1, B);
+      if (r.own) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) bar.ub[(int64_t)i * m.F + r.f] = 0.0;
+        bar.pb[r.f] = 0.0;
+        if (bar.gB) bar.gB[r.f] = 0.0;
+      }
+    } else {
+      double lf[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) lf[v] = v == D ? lc_cont : lc[v];
+      double ub[D], ubb[D], pbb, gbb;
+#pragma unroll
+      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.f];
+      const double gb = a.gB.v ? a.gB[r.f] : X.mu;
+      adj_flux_boundary<D, ADV>(q, a.kind[r.f], X, ub, a.pb[r.f], gb, lf, B, ubb, pbb, gbb);
+#pragma unroll
+      for (int i = 0; i < D; ++i) bar.ub[(int64_t)i * m.F + r.f] = ubb[i];
+      bar.pb[r.f] = pbb;
+      if (a.gB.v) {
+        if (bar.gB) bar.gB[r.f] = gbb;
+      } else {
+        B.mu += gbb;
+      }
+    }
+  }
+  if (c == a.pin) B.p += lc[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) bar.x[(int64_t)i * m.C + c] = B.u[i];
+  bar.x[(int64_t)D * m.C + c] = B.p;
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) bar.G[(int64_t)(i * D + j) * m.C + c] = B.Gu[i][j];
+#pragma unroll
+  for (int j = 0; j < D; ++j) bar.G[(int64_t)(D * D + j) * m.C + c] = B.Gp[j];
+  bar.aP[c] = B.aP;
+  bar.mu[c] = B.mu;
+  bar.rho[c] = B.rho;
+  if (ADV == kAdvLimited) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) bar.psi[(int64_t)i * m.C + c] = B.psi[i];
+  }
+}
+
+// reverse pass D': a_P cotangent (+ limiter) -> state / gradient / mu / rho cotangents; then the
+// gradient cotangent is pre-multiplied by the cell's reconstruction operator (1/V or M^-1).
+template <int D, int ADV, int GRAD>
+__global__ void __launch_bounds__(256) k_flow_ap_adj(MeshView m, FlowArgs<double> a, const int32_t* __restrict__ ext,
+                                                     FlowBars bar) {
+  constexpr int NV = D + 1;
+  int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.C) return;
+  FlowCell<D, double> X;
+  load_state<D, double>(a, m.C, c, X);
+  load_grad<D, double>(a, m.C, c, X);
+  X.mu = a.mu[c];
+  X.rho = a.rho[c];
+  X.V = m.vol[c];
+  FlowCellBar<D> B;
+  B.zero();
+  const double apb_c = bar.aP[c];
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    FaceGeo<D> q;
+    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    if (o >= 0) {
+      FlowCell<D, double> O;
+      load_state<D, double>(a, m.C, o, O);
+      load_grad<D, double>(a, m.C, o, O);
+      O.mu = a.mu[o];
+      O.rho = a.rho[o];
+      O.V = m.vol[o];
+      const double apb_o = bar.aP[o];
+      if (r.own)
+        adj_ap_interior<D, ADV>(q, X, O, apb_c, apb_o, 0, B);
+      else
+        adj_ap_interior<D, ADV>(q, O, X, apb_o, apb_c, 1, B);
+    } else {
+      const double gb = a.gB.v ? a.gB[r.f] : X.mu;
+      double gbb;
+      adj_ap_boundary<D, ADV>(q, a.kind[r.f], X, gb, apb_c, B, gbb);
+      if (a.gB.v) {
+        if (bar.gB) bar.gB[r.f] += gbb;
+      } else {
+        B.mu += gbb;
+      }
+    }
+  }
+  // limiter: psi = clip(min_f venkat(dm_f, head_f, e2), 0, 1)
+  if (ADV == kAdvLimited) {
+    const double e2 = X.V * (a.lim_k * a.lim_k * a.lim_k);
+    double xcen[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) xcen[j] = m.cc[(int64_t)j * m.C + c];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const double pbar = bar.psi[(int64_t)i * m.C + c];
+      const int32_t amx = ext[(int64_t)i * m.C + c], amn = ext[(int64_t)(D + i) * m.C + c];
+      const double mxv = a.x[(int64_t)i * m.C + amx], mnv = a.x[(int64_t)i * m.C + amn];
+      // forward recompute: minimum over faces and its first argmin
+      double lo = 0.0;
+      int32_t kmin = -1;
+      for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+        int32_t e = m.cface[k];
+        int32_t f = e >= 0 ? e : ~e;
+        // recompute with the same association as the forward pass
+        double dmf = X.Gu[i][0] * (m.fc[f] - xcen[0]);
+#pragma unroll
+        for (int j = 1; j < D; ++j) dmf = dmf + X.Gu[i][j] * (m.fc[(int64_t)j * m.F + f] - xcen[j]);
+        double vf = venkat_face<double>(dmf, mxv - X.u[i], mnv - X.u[i], e2);
+        if (kmin < 0 || vf < lo) {
+          lo = vf;
+          kmin = k;
+        }
+      }
+      // d clip / d lo  (JAX minimum(maximum(x, 0), 1) tie convention)
+      double dclip = (lo > 0.0 && lo < 1.0) ? 1.0 : ((lo == 0.0 || lo == 1.0) ? 0.5 : 0.0);
+      const double vbar = pbar * dclip;
+      double mxb = 0.0, mnb = 0.0;
+      if (vbar != 0.0 && kmin >= 0) {
+        int32_t e = m.cface[kmin];
+        int32_t f = e >= 0 ? e : ~e;
+        double dxf[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dxf[j] = m.fc[(int64_t)j * m.F + f] - xcen[j];
+        double dmf = X.Gu[i][0] * dxf[0];
+#pragma unroll
+        for (int j = 1; j < D; ++j) dmf = dmf + X.Gu[i][j] * dxf[j];
+        // partial derivatives by two seeded tangent evaluations
+        dual r_dm = venkat_face<dual>(dual{dmf, 1.0}, dual{mxv - X.u[i], 0.0}, dual{mnv - X.u[i], 0.0}, e2);
+        dual r_hx = venkat_face<dual>(dual{dmf, 0.0}, dual{mxv - X.u[i], 1.0}, dual{mnv - X.u[i], 0.0}, e2);
+        dual r_hn = venkat_face<dual>(dual{dmf, 0.0}, dual{mxv - X.u[i], 0.0}, dual{mnv - X.u[i], 1.0}, e2);
+        const double dmb = vbar * r_dm.d;
+#pragma unroll
+        for (int j = 0; j < D; ++j) B.Gu[i][j] += dmb * dxf[j];
+        mxb = vbar * r_hx.d;
+        mnb = vbar * r_hn.d;
+        B.u[i] -= mxb + mnb;
+        if (amx == c) {
+          B.u[i] += mxb;
+          mxb = 0.0;
+        }
+        if (amn == c) {
+          B.u[i] += mnb;
+          mnb = 0.0;
+        }
+      }
+      bar.ext[(int64_t)i * m.C + c] = mxb;
+      bar.ext[(int64_t)(D + i) * m.C + c] = mnb;
+    }
+  }
+  // accumulate onto the R' cotangents
+#pragma unroll
+  for (int i = 0; i < D; ++i) bar.x[(int64_t)i * m.C + c] += B.u[i];
+  bar.mu[c] += B.mu;
+  bar.rho[c] += B.rho;
+  // total gradient cotangent, pre-multiplied by the reconstruction operator
+  double gb[NV][D];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) gb[i][j] = bar.G[(int64_t)(i * D + j) * m.C + c] + B.Gu[i][j];
+#pragma unroll
+  for (int j = 0; j < D; ++j) gb[D][j] = bar.G[(int64_t)(D * D + j) * m.C + c] + B.Gp[j];
+  if (GRAD == 0) {
+    const double iv = 1.0 / X.V;
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int j = 0; j < D; ++j) bar.G[(int64_t)(v * D + j) * m.C + c] = gb[v][j] * iv;
+  } else {
+    double M[D][D], I[D][D];
+    lsq_matrix<D>(m, c, M);
+    sym_inverse<D>(M, I);
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int p = 0; p < D; ++p) {
+        double s = 0.0;
+#pragma unroll
+        for (int q2 = 0; q2 < D; ++q2) s += I[p][q2] * gb[v][q2];
+        bar.G[(int64_t)(v * D + p) * m.C + c] = s;
+      }
+  }
+}
+
+// reverse pass G': scaled gradient cotangents (and limiter extrema cotangents) -> state and
+// boundary-value cotangents.
+template <int D, int GRAD, bool LIM>
+__global__ void __launch_bounds__(256) k_flow_grad_adj(MeshView m, FlowArgs<double> a, const int32_t* __restrict__ ext,
+                                                       FlowBars bar) {
+  constexpr int NV = D + 1;
+  int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.C) return;
+  double xb[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) xb[v] = 0.0;
+  double Bc[NV][D];
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int j = 0; j < D; ++j) Bc[v][j] = bar.G[(int64_t)(v * D + j) * m.C + c];
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    if (o >= 0) {
+      if (GRAD == 0) {
+        FaceGeo<D> q;
+        load_face_geo<D>(m, r.f, r.P, r.N, q);
+        const double ws = r.own ? 1.0 - q.g : q.g;
+        const int32_t P = r.P, N = r.N;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          double pb_ = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j)
+            pb_ += (q.n[j] * q.A) * (bar.G[(int64_t)(v * D + j) * m.C + P] - bar.G[(int64_t)(v * D + j) * m.C + N]);
+          xb[v] += ws * pb_;
+        }
+      } else {
+        double dx[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dx[j] = m.cc[(int64_t)j * m.C + o] - m.cc[(int64_t)j * m.C + c];
+        const double wgt = 1.0 / vdot<D>(dx, dx);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          double own_dot = 0.0, oth_dot = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+            own_dot += dx[j] * Bc[v][j];
+            oth_dot += dx[j] * bar.G[(int64_t)(v * D + j) * m.C + o];
+          }
+          // own gradient: b_c += w dx (x_o - x_c);  other's: b_o += w (-dx) (x_c - x_o)
+          xb[v] += -wgt * own_dot - wgt * oth_dot;
+        }
+      }
+      if (LIM) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+          if (ext[(int64_t)i * m.C + o] == c) xb[i] += bar.ext[(int64_t)i * m.C + o];
+          if (ext[(int64_t)(D + i) * m.C + o] == c) xb[i] += bar.ext[(int64_t)(D + i) * m.C + o];
+        }
+      }
+    } else {
+      const int kind = a.kind[r.f];
+      double phib[NV];
+      if (GRAD == 0) {
+        const double A = m.area[r.f];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          double s = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) s += (m.nrm[(int64_t)j * m.F + r.f] * A) * Bc[v][j];
+          phib[v] = s;
+        }
+      } else {
+        double dx[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dx[j] = m.fc[(int64_t)j * m.F + r.f] - m.cc[(int64_t)j * m.C + c];
+        const double wgt = 1.0 / vdot<D>(dx, dx);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          double s = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) s += dx[j] * Bc[v][j];
+          phib[v] = wgt * s;
+          xb[v] -= wgt * s;
+        }
+      }
+      // route the boundary-value cotangents through the closures
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        if (kind == kFlowWall || kind == kFlowInlet)
+          bar.ub[(int64_t)i * m.F + r.f] += phib[i];
+        else if (kind == kFlowOutlet)
+          xb[i] += phib[i];
+      }
+      if (kind == kFlowWall || kind == kFlowInlet)
+        xb[D] += phib[D];
+      else if (kind == kFlowOutlet)
+        bar.pb[r.f] += phib[D];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) bar.x[(int64_t)v * m.C + c] += xb[v];
+}
+
+// ---------------------------------------------------------------------------------------------
+// host side
+
+struct Work {
+  double *G, *aP, *psi;
+  double *Gt, *aPt, *psit;
+  double *Gb, *aPb, *psib, *extb;
+  int32_t* ext;
+};
+
+int64_t work_bytes(int D, int64_t C, int mode) {
+  const int NV = D + 1;
+  int64_t n = (int64_t)(NV * D + 1 + D) * C;  // G, aP, psi
+  if (mode == 1) n *= 2;
+  if (mode == 2) n += (int64_t)(NV * D + 1 + D + 2 * D) * C + (int64_t)D * C;  // bars + ext (2D int32)
+  return n * 8 + 256;
+}
+
+Work carve(void* base, int D, int64_t C, int mode) {
+  const int NV = D + 1;
+  double* p = (double*)base;
+  Work w{};
+  w.G = p;
+  p += (int64_t)NV * D * C;
+  w.aP = p;
+  p += C;
+  w.psi = p;
+  p += (int64_t)D * C;
+  if (mode == 1) {
+    w.Gt = p;
+    p += (int64_t)NV * D * C;
+    w.aPt = p;
+    p += C;
+    w.psit = p;
+    p += (int64_t)D * C;
+  }
+  if (mode == 2) {
+    w.Gb = p;
+    p += (int64_t)NV * D * C;
+    w.aPb = p;
+    p += C;
+    w.psib = p;
+    p += (int64_t)D * C;
+    w.extb = p;
+    p += (int64_t)2 * D * C;
+    w.ext = (int32_t*)p;
+  }
+  return w;
+}
+
+template <class S>
+FlowArgs<S> make_args(const afx_flow_config* cfg, const afx_flow_inputs* in, const afx_flow_inputs* tan) {
+  FlowArgs<S> a{};
+  a.kind = cfg->face_kind;
+  a.x.v = in->state;
+  a.ub.v = in->u_b;
+  a.pb.v = in->p_b;
+  a.gB.v = in->gamma_b;
+  a.mu.v = in->mu;
+  a.rho.v = in->rho;
+  if (tan) {
+    a.x.t = tan->state;
+    a.ub.t = tan->u_b;
+    a.pb.t = tan->p_b;
+    a.gB.t = tan->gamma_b;
+    a.mu.t = tan->mu;
+    a.rho.t = tan->rho;
+  }
+  a.pin = cfg->pressure_pin >= 0 ? (int32_t)cfg->pressure_pin : -1;
+  a.pin_value = cfg->pressure_pin_value;
+  for (int i = 0; i < 3; ++i) a.bf[i] = cfg->body_force[i];
+  a.lim_k = cfg->limiter_k;
+  return a;
+}
+
+template <int D, class S, int ADV, int GRAD>
+int run_forward(const afx_mesh* mh, const afx_geometry* g, FlowArgs<S> a, Out<S> R, double* G, double* Gt,
+                double* aP, double* aPt, double* psi, double* psit, int32_t* ext, hipStream_t s) {
+  MeshView m = afx_view(mh, g);
+  Out<S> oG{G, Gt}, oP{psi, psit}, oA{aP, aPt};
+  if (ADV == kAdvLimited)
+    hipLaunchKernelGGL((k_flow_grad<D, S, GRAD, true>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oG, oP, ext);
+  else
+    hipLaunchKernelGGL((k_flow_grad<D, S, GRAD, false>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oG, oP, ext);
+  AFX_CHECK_LAUNCH();
+  a.G.v = G;
+  a.G.t = Gt;
+  hipLaunchKernelGGL((k_flow_ap<D, S, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oA);
+  AFX_CHECK_LAUNCH();
+  if (R.v == nullptr) return 0;
+  a.aP.v = aP;
+  a.aP.t = aPt;
+  a.psi.v = psi;
+  a.psi.t = psit;
+  hipLaunchKernelGGL((k_flow_res<D, S, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, R);
+  AFX_CHECK_LAUNCH();
+  return 0;
+}
+
+template <int D, int ADV, int GRAD>
+int run_vjp(const afx_mesh* mh, const afx_geometry* g, FlowArgs<double> a, const double* lam, FlowBars bar,
+            const Work& w, hipStream_t s) {
+  MeshView m = afx_view(mh, g);
+  int rc = run_forward<D, double, ADV, GRAD>(mh, g, a, Out<double>{nullptr, nullptr}, w.G, nullptr, w.aP, nullptr,
+                                             w.psi, nullptr, w.ext, s);
+  if (rc) return rc;
+  a.G.v = w.G;
+  a.aP.v = w.aP;
+  a.psi.v = w.psi;
+  hipLaunchKernelGGL((k_flow_res_adj<D, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, lam, bar);
+  AFX_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_flow_ap_adj<D, ADV, GRAD>), grid_for(m.C), dim3(kBlock), 0, s, m, a, w.ext, bar);
+  AFX_CHECK_LAUNCH();
+  if (ADV == kAdvLimited)
+    hipLaunchKernelGGL((k_flow_grad_adj<D, GRAD, true>), grid_for(m.C), dim3(kBlock), 0, s, m, a, w.ext, bar);
+  else
+    hipLaunchKernelGGL((k_flow_grad_adj<D, GRAD, false>), grid_for(m.C), dim3(kBlock), 0, s, m, a, w.ext, bar);
+  AFX_CHECK_LAUNCH();
+  return 0;
+}
+
+// Dispatch on (dim, advection, gradient) to a templated body.
+template <class Fn>
+int dispatch(int D, int adv, int grad, Fn&& fn) {
+#define AFX_CASE(DD, AA, GG) \
+  if (D == DD && adv == AA && grad == GG) return fn(std::integral_constant<int, DD>{}, std::integral_constant<int, AA>{}, std::integral_constant<int, GG>{});
+  AFX_CASE(2, 0, 0) AFX_CASE(2, 1, 0) AFX_CASE(2, 2, 0) AFX_CASE(2, 0, 1) AFX_CASE(2, 1, 1) AFX_CASE(2, 2, 1)
+  AFX_CASE(3, 0, 0) AFX_CASE(3, 1, 0) AFX_CASE(3, 2, 0) AFX_CASE(3, 0, 1) AFX_CASE(3, 1, 1) AFX_CASE(3, 2, 1)
+#undef AFX_CASE
+  afx_set_error("unsupported (dim, advection, gradient) combination");
+  return 2;
+}
+
+int check_common(const afx_mesh* m, const afx_geometry* g, const afx_flow_config* cfg, const afx_flow_inputs* in,
+                 void* work, int64_t work_size, int mode) {
+  AFX_REQUIRE(m && g && cfg && in, "flow: null argument");
+  AFX_REQUIRE(in->state && in->mu && in->rho && in->u_b && in->p_b && cfg->face_kind,
+              "flow: a required input pointer is null");
+  AFX_REQUIRE(cfg->advection >= 0 && cfg->advection <= 2, "flow: advection must be 0, 1 or 2");
+  AFX_REQUIRE(cfg->gradient == 0 || cfg->gradient == 1, "flow: gradient must be 0 or 1");
+  AFX_REQUIRE(cfg->pressure_pin < (int64_t)m->C, "flow: pressure_pin out of range");
+  AFX_REQUIRE(work != nullptr && work_size >= work_bytes(m->dim, m->C, mode), "flow: workspace too small");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int afx_flow_workspace_bytes(const afx_mesh* m, int mode, int64_t* bytes) {
+  AFX_REQUIRE(m && bytes, "afx_flow_workspace_bytes: null argument");
+  AFX_REQUIRE(mode >= 0 && mode <= 2, "afx_flow_workspace_bytes: mode must be 0, 1 or 2");
+  *bytes = work_bytes(m->dim, m->C, mode);
+  return 0;
+}
+
+extern "C" int afx_flow_residual(const afx_mesh* m, const afx_geometry* g, const afx_flow_config* cfg,
+                                 const afx_flow_inputs* in, double* residual, void* work, int64_t work_size,
+                                 hipStream_t s) {
+  int rc = check_common(m, g, cfg, in, work, work_size, 0);
+  if (rc) return rc;
+  AFX_REQUIRE(residual, "afx_flow_residual: null residual");
+  Work w = carve(work, m->dim, m->C, 0);
+  FlowArgs<double> a = make_args<double>(cfg, in, nullptr);
+  return dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
+    return run_forward<decltype(Dc)::value, double, decltype(Ac)::value, decltype(Gc)::value>(
+        m, g, a, Out<double>{residual, nullptr}, w.G, nullptr, w.aP, nullptr, w.psi, nullptr, nullptr, s);
+  });
+}
+
+extern "C" int afx_flow_jvp(const afx_mesh* m, const afx_geometry* g, const afx_flow_config* cfg,
+                            const afx_flow_inputs* in, const afx_flow_inputs* tangent, double* residual,
+                            double* residual_tangent, void* work, int64_t work_size, hipStream_t s) {
+  int rc = check_common(m, g, cfg, in, work, work_size, 1);
+  if (rc) return rc;
+  AFX_REQUIRE(tangent && residual && residual_tangent, "afx_flow_jvp: null argument");
+  Work w = carve(work, m->dim, m->C, 1);
+  FlowArgs<dual> a = make_args<dual>(cfg, in, tangent);
+  return dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
+    return run_forward<decltype(Dc)::value, dual, decltype(Ac)::value, decltype(Gc)::value>(
+        m, g, a, Out<dual>{residual, residual_tangent}, w.G, w.Gt, w.aP, w.aPt, w.psi, w.psit, nullptr, s);
+  });
+}
+
+extern "C" int afx_flow_vjp(const afx_mesh* m, const afx_geometry* g, const afx_flow_config* cfg,
+                            const afx_flow_inputs* in, const double* cotangent, afx_flow_cotangents* out,
+                            void* work, int64_t work_size, hipStream_t s) {
+  int rc = check_common(m, g, cfg, in, work, work_size, 2);
+  if (rc) return rc;
+  AFX_REQUIRE(cotangent && out && out->state && out->mu && out->rho && out->u_b && out->p_b,
+              "afx_flow_vjp: null cotangent argument");
+  AFX_REQUIRE(in->gamma_b == nullptr || out->gamma_b != nullptr,
+              "afx_flow_vjp: gamma_b given without a gamma_b cotangent buffer");
+  Work w = carve(work, m->dim, m->C, 2);
+  FlowArgs<double> a = make_args<double>(cfg, in, nullptr);
+  FlowBars bar{out->state, w.Gb, w.aPb, w.psib, out->mu, out->rho, out->u_b, out->p_b, out->gamma_b, w.extb};
+  return dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
+    return run_vjp<decltype(Dc)::value, decltype(Ac)::value, decltype(Gc)::value>(m, g, a, cotangent, bar, w, s);
+  });
+}
+
+extern "C" int afx_flow_fields(const afx_mesh* m, const afx_geometry* g, const afx_flow_config* cfg,
+                               const afx_flow_inputs* in, double* gradients, double* a_p, double* psi,
+                               hipStream_t s) {
+  AFX_REQUIRE(m && g && cfg && in && gradients && a_p, "afx_flow_fields: null argument");
+  AFX_REQUIRE(cfg->advection != 2 || psi, "afx_flow_fields: limited advection needs a psi buffer");
+  FlowArgs<double> a = make_args<double>(cfg, in, nullptr);
+  return dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
+    return run_forward<decltype(Dc)::value, double, decltype(Ac)::value, decltype(Gc)::value>(
+        m, g, a, Out<double>{nullptr, nullptr}, gradients, nullptr, a_p, nullptr, psi, nullptr, nullptr, s);
+  });
+}

@@ -1,0 +1,343 @@
+// Mesh handle and device geometry (forward + adjoint with respect to node coordinates).
+//
+// Geometry follows the reference pipeline (aquaflux/mesh/mesh.py:365-387):
+//   1. face area / centroid / winding normal -- 2D edge (mesh/face.py:217-230) or 3D centre fan
+//      with the signed-area centroid (mesh/face.py:292-349);
+//   2. approximate cell centroids = mean of the cell's face centroids (mesh/cell.py:84-112);
+//   3. owner-outward orientation of the normal (mesh/face.py:196-211);
+//   4. cell volume and centroid by the divergence theorem, recentred on the approximate centroid
+//      (mesh/cell.py:123-170).
+// Each step is one thread per face or per cell; cell steps walk the cell -> face list, so every
+// reduction is a fixed-order register sum (no atomics, bitwise reproducible).
+#include "afx_common.h"
+#include "afx_mesh.h"
+
+#include <vector>
+#include <algorithm>
+
+static thread_local std::string g_err;
+void afx_set_error(const std::string& msg) { g_err = msg; }
+const char* afx_get_error() { return g_err.c_str(); }
+
+using namespace afx;
+
+extern "C" const char* afx_last_error(void) { return afx_get_error(); }
+
+extern "C" int afx_mesh_create(int dim, int64_t n_cells, int64_t n_faces, int64_t n_nodes,
+                               const int32_t* owner, const int32_t* neighbour,
+                               const int64_t* face_node_offsets, const int32_t* face_node_indices,
+                               afx_mesh** out) {
+  AFX_REQUIRE(out != nullptr, "afx_mesh_create: null output handle");
+  *out = nullptr;
+  AFX_REQUIRE(dim == 2 || dim == 3, "afx_mesh_create: dim must be 2 or 3");
+  AFX_REQUIRE(n_cells > 0 && n_faces > 0 && n_nodes > 0, "afx_mesh_create: empty mesh");
+  AFX_REQUIRE(n_cells < INT32_MAX && n_faces < INT32_MAX / 2 && n_nodes < INT32_MAX,
+              "afx_mesh_create: mesh exceeds 32-bit indexing");
+  // host-side build of the cell -> face CSR (owner entries f, neighbour entries ~f), face order
+  std::vector<int32_t> count(n_cells + 1, 0);
+  for (int64_t f = 0; f < n_faces; ++f) {
+    int32_t P = owner[f], N = neighbour[f];
+    AFX_REQUIRE(P >= 0 && P < n_cells, "afx_mesh_create: owner index out of range");
+    AFX_REQUIRE(N >= -1 && N < n_cells, "afx_mesh_create: neighbour index out of range");
+    count[P + 1]++;
+    if (N >= 0) count[N + 1]++;
+  }
+  for (int64_t c = 0; c < n_cells; ++c) count[c + 1] += count[c];
+  std::vector<int32_t> cface(count[n_cells]);
+  std::vector<int32_t> fill(count.begin(), count.end() - 1);
+  int32_t max_faces = 0;
+  for (int64_t f = 0; f < n_faces; ++f) {
+    cface[fill[owner[f]]++] = (int32_t)f;
+    if (neighbour[f] >= 0) cface[fill[neighbour[f]]++] = ~(int32_t)f;
+  }
+  for (int64_t c = 0; c < n_cells; ++c) max_faces = std::max(max_faces, count[c + 1] - count[c]);
+  // node -> incidence CSR (for the node-coordinate adjoint gather)
+  int64_t n_inc = face_node_offsets[n_faces];
+  std::vector<int32_t> ncount(n_nodes + 1, 0);
+  for (int64_t i = 0; i < n_inc; ++i) {
+    int32_t v = face_node_indices[i];
+    AFX_REQUIRE(v >= 0 && v < n_nodes, "afx_mesh_create: face_node_indices out of range");
+    ncount[v + 1]++;
+  }
+  for (int64_t v = 0; v < n_nodes; ++v) ncount[v + 1] += ncount[v];
+  std::vector<int32_t> ninc(n_inc);
+  {
+    std::vector<int32_t> nfill(ncount.begin(), ncount.end() - 1);
+    for (int64_t i = 0; i < n_inc; ++i) ninc[nfill[face_node_indices[i]]++] = (int32_t)i;
+  }
+  std::vector<int32_t> inc_face(n_inc);
+  for (int64_t f = 0; f < n_faces; ++f)
+    for (int64_t i = face_node_offsets[f]; i < face_node_offsets[f + 1]; ++i) inc_face[i] = (int32_t)f;
+  std::vector<int32_t> foff32(n_faces + 1);
+  for (int64_t f = 0; f <= n_faces; ++f) foff32[f] = (int32_t)face_node_offsets[f];
+
+  afx_mesh* m = new afx_mesh();
+  m->dim = dim;
+  m->C = (int32_t)n_cells;
+  m->F = (int32_t)n_faces;
+  m->N = (int32_t)n_nodes;
+  m->n_inc = (int32_t)n_inc;
+  m->n_cface = (int32_t)cface.size();
+  m->max_faces = max_faces;
+  AFX_CHECK_HIP(hipGetDevice(&m->device));
+  auto up = [&](void** dst, const void* src, size_t bytes) -> hipError_t {
+    hipError_t e = hipMalloc(dst, bytes > 0 ? bytes : 4);
+    if (e != hipSuccess) return e;
+    return bytes ? hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+  };
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = up((void**)&m->owner, owner, n_faces * 4);
+  if (e == hipSuccess) e = up((void**)&m->nb, neighbour, n_faces * 4);
+  if (e == hipSuccess) e = up((void**)&m->cptr, count.data(), (n_cells + 1) * 4);
+  if (e == hipSuccess) e = up((void**)&m->cface, cface.data(), cface.size() * 4);
+  if (e == hipSuccess) e = up((void**)&m->foff, foff32.data(), (n_faces + 1) * 4);
+  if (e == hipSuccess) e = up((void**)&m->fidx, face_node_indices, n_inc * 4);
+  if (e == hipSuccess) e = up((void**)&m->inc_face, inc_face.data(), n_inc * 4);
+  if (e == hipSuccess) e = up((void**)&m->nptr, ncount.data(), (n_nodes + 1) * 4);
+  if (e == hipSuccess) e = up((void**)&m->ninc, ninc.data(), n_inc * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&m->approx, (size_t)n_cells * dim * 8);
+  if (e == hipSuccess) e = hipMalloc((void**)&m->normal_raw, (size_t)n_faces * dim * 8);
+  if (e != hipSuccess) {
+    afx_set_error(std::string("afx_mesh_create: ") + hipGetErrorString(e));
+    afx_mesh_destroy(m);
+    return 1;
+  }
+  *out = m;
+  return 0;
+}
+
+extern "C" int afx_mesh_destroy(afx_mesh* m) {
+  if (!m) return 0;
+  void* ptrs[] = {m->owner, m->nb,  m->cptr,   m->cface,      m->foff,   m->fidx,
+                  m->inc_face, m->nptr, m->ninc, m->approx, m->normal_raw};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete m;
+  return 0;
+}
+
+extern "C" int afx_mesh_info(const afx_mesh* m, int64_t* out) {
+  AFX_REQUIRE(m && out, "afx_mesh_info: null argument");
+  out[0] = m->C;
+  out[1] = m->F;
+  out[2] = m->N;
+  out[3] = m->dim;
+  out[4] = m->n_cface;
+  out[5] = m->max_faces;
+  out[6] = m->device;
+  return 0;
+}
+
+extern "C" int afx_mesh_cell_faces(const afx_mesh* m, int32_t* cptr, int32_t* cface, hipStream_t s) {
+  AFX_REQUIRE(m && cptr && cface, "afx_mesh_cell_faces: null argument");
+  AFX_CHECK_HIP(hipMemcpyAsync(cptr, m->cptr, (size_t)(m->C + 1) * 4, hipMemcpyDeviceToDevice, s));
+  AFX_CHECK_HIP(hipMemcpyAsync(cface, m->cface, (size_t)m->n_cface * 4, hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------------------------
+// face geometry from nodes (unoriented normal)
+template <int D>
+__global__ void k_face_geometry(int32_t F, int32_t Nn, const int32_t* __restrict__ foff,
+                                const int32_t* __restrict__ fidx, const double* __restrict__ X,
+                                double* __restrict__ area, double* __restrict__ nrm,
+                                double* __restrict__ fc) {
+  int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  int32_t b = foff[f], e = foff[f + 1];
+  if (D == 2) {
+    int32_t a0 = fidx[b], a1 = fidx[b + 1];
+    double p0[2] = {X[a0], X[Nn + a0]}, p1[2] = {X[a1], X[Nn + a1]};
+    double d0 = p1[0] - p0[0], d1 = p1[1] - p0[1];
+    double sq = d0 * d0 + d1 * d1;
+    area[f] = sq > 0.0 ? sqrt(sq) : 0.0;
+    fc[f] = 0.5 * (p0[0] + p1[0]);
+    fc[F + f] = 0.5 * (p0[1] + p1[1]);
+    double v0 = d1, v1 = -d0;
+    double s2 = v0 * v0 + v1 * v1;
+    double inv = s2 > 0.0 ? sqrt(s2) : 1.0;
+    nrm[f] = s2 > 0.0 ? v0 / inv : 0.0;
+    nrm[F + f] = s2 > 0.0 ? v1 / inv : 0.0;
+  } else {
+    int32_t k = e - b;
+    double ctr[3] = {0.0, 0.0, 0.0};
+    for (int32_t i = b; i < e; ++i) {
+      int32_t a = fidx[i];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) ctr[j] += X[(int64_t)j * Nn + a];
+    }
+    double invk = 1.0 / (double)k;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) ctr[j] = ctr[j] * invk;
+    double S[3] = {0.0, 0.0, 0.0};
+    for (int32_t i = b; i < e; ++i) {
+      int32_t a = fidx[i], c = fidx[i + 1 < e ? i + 1 : b];
+      double u[3], w[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        u[j] = X[(int64_t)j * Nn + a] - ctr[j];
+        w[j] = X[(int64_t)j * Nn + c] - ctr[j];
+      }
+      S[0] += 0.5 * (u[1] * w[2] - u[2] * w[1]);
+      S[1] += 0.5 * (u[2] * w[0] - u[0] * w[2]);
+      S[2] += 0.5 * (u[0] * w[1] - u[1] * w[0]);
+    }
+    double sq = S[0] * S[0] + S[1] * S[1] + S[2] * S[2];
+    double A = sq > 0.0 ? sqrt(sq) : 0.0;
+    double nn[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) nn[j] = sq > 0.0 ? S[j] / A : 0.0;
+    double cs[3] = {0.0, 0.0, 0.0};
+    for (int32_t i = b; i < e; ++i) {
+      int32_t a = fidx[i], c = fidx[i + 1 < e ? i + 1 : b];
+      double pb[3], pc[3], u[3], w[3], dir[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        pb[j] = X[(int64_t)j * Nn + a];
+        pc[j] = X[(int64_t)j * Nn + c];
+        u[j] = pb[j] - ctr[j];
+        w[j] = pc[j] - ctr[j];
+      }
+      dir[0] = 0.5 * (u[1] * w[2] - u[2] * w[1]);
+      dir[1] = 0.5 * (u[2] * w[0] - u[0] * w[2]);
+      dir[2] = 0.5 * (u[0] * w[1] - u[1] * w[0]);
+      double sgn = dir[0] * nn[0] + dir[1] * nn[1] + dir[2] * nn[2];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) cs[j] += sgn * ((ctr[j] + pb[j] + pc[j]) / 3.0);
+    }
+    double safe = A > 0.0 ? A : 1.0;
+    area[f] = A;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      nrm[(int64_t)j * F + f] = nn[j];
+      fc[(int64_t)j * F + f] = cs[j] / safe;
+    }
+  }
+}
+
+template <int D>
+__global__ void k_approx_centroid(MeshView m, double* __restrict__ approx) {
+  int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.C) return;
+  double s[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) s[j] = 0.0;
+  int32_t b = m.cptr[c], e = m.cptr[c + 1];
+  for (int32_t k = b; k < e; ++k) {
+    int32_t ent = m.cface[k];
+    int32_t f = ent >= 0 ? ent : ~ent;
+#pragma unroll
+    for (int j = 0; j < D; ++j) s[j] += m.fc[(int64_t)j * m.F + f];
+  }
+  double inv = 1.0 / (double)(e - b);
+#pragma unroll
+  for (int j = 0; j < D; ++j) approx[(int64_t)j * m.C + c] = s[j] * inv;
+}
+
+template <int D>
+__global__ void k_orient(MeshView m, const double* __restrict__ approx, const double* __restrict__ raw,
+                         double* __restrict__ nrm) {
+  int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.F) return;
+  int32_t P = m.owner[f];
+  double proj = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j)
+    proj += (m.fc[(int64_t)j * m.F + f] - approx[(int64_t)j * m.C + P]) * raw[(int64_t)j * m.F + f];
+  double sgn = proj < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) nrm[(int64_t)j * m.F + f] = raw[(int64_t)j * m.F + f] * sgn;
+}
+
+template <int D>
+__global__ void k_cell_volume(MeshView m, const double* __restrict__ approx, double* __restrict__ vol,
+                              double* __restrict__ cc) {
+  int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m.C) return;
+  double ap[D], v = 0.0, cs[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    ap[j] = approx[(int64_t)j * m.C + c];
+    cs[j] = 0.0;
+  }
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    int32_t ent = m.cface[k];
+    int32_t f = ent >= 0 ? ent : ~ent;
+    double loc[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) loc[j] = m.fc[(int64_t)j * m.F + f] - ap[j];
+    double nl[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) nl[j] = m.nrm[(int64_t)j * m.F + f];
+    double flux = vdot<D>(loc, nl) * m.area[f];
+    if (ent >= 0) {
+      v += flux;
+#pragma unroll
+      for (int j = 0; j < D; ++j) cs[j] += loc[j] * flux;
+    } else {
+      v += -flux;
+#pragma unroll
+      for (int j = 0; j < D; ++j) cs[j] += -(loc[j] * flux);
+    }
+  }
+  double V = v / (double)D;
+  vol[c] = V;
+  double inv = 1.0 / ((double)(D + 1) * V);
+#pragma unroll
+  for (int j = 0; j < D; ++j) cc[(int64_t)j * m.C + c] = cs[j] * inv + ap[j];
+}
+
+}  // namespace
+
+afx::MeshView afx_view(const afx_mesh* m, const afx_geometry* g) {
+  afx::MeshView v;
+  v.C = m->C;
+  v.F = m->F;
+  v.owner = m->owner;
+  v.nb = m->nb;
+  v.cptr = m->cptr;
+  v.cface = m->cface;
+  v.area = g ? g->area : nullptr;
+  v.nrm = g ? g->normal : nullptr;
+  v.fc = g ? g->face_centroid : nullptr;
+  v.vol = g ? g->volume : nullptr;
+  v.cc = g ? g->cell_centroid : nullptr;
+  return v;
+}
+
+extern "C" int afx_geometry_compute(const afx_mesh* m, const double* node_coords, afx_geometry* g,
+                                    hipStream_t s) {
+  AFX_REQUIRE(m && node_coords && g, "afx_geometry_compute: null argument");
+  AFX_REQUIRE(g->area && g->normal && g->face_centroid && g->volume && g->cell_centroid,
+              "afx_geometry_compute: geometry output pointer missing");
+  double* area = const_cast<double*>(g->area);
+  double* nrm = const_cast<double*>(g->normal);
+  double* fc = const_cast<double*>(g->face_centroid);
+  double* vol = const_cast<double*>(g->volume);
+  double* cc = const_cast<double*>(g->cell_centroid);
+  afx::MeshView v = afx_view(m, g);
+  if (m->dim == 2) {
+    hipLaunchKernelGGL(k_face_geometry<2>, grid_for(m->F), dim3(kBlock), 0, s, m->F, m->N, m->foff,
+                       m->fidx, node_coords, area, m->normal_raw, fc);
+    AFX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_approx_centroid<2>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx);
+    AFX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_orient<2>, grid_for(m->F), dim3(kBlock), 0, s, v, m->approx, m->normal_raw, nrm);
+    AFX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_cell_volume<2>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx, vol, cc);
+    AFX_CHECK_LAUNCH();
+  } else {
+    hipLaunchKernelGGL(k_face_geometry<3>, grid_for(m->F), dim3(kBlock), 0, s, m->F, m->N, m->foff,
+                       m->fidx, node_coords, area, m->normal_raw, fc);
+    AFX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_approx_centroid<3>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx);
+    AFX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_orient<3>, grid_for(m->F), dim3(kBlock), 0, s, v, m->approx, m->normal_raw, nrm);
+    AFX_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_cell_volume<3>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx, vol, cc);
+    AFX_CHECK_LAUNCH();
+  }
+  return 0;
+}

@@ -1,0 +1,393 @@
+"""Coupled momentum + Rhie-Chow continuity residual on the GPU (``aquaflux/flow``).
+
+:class:`MomentumContinuity` keeps the reference's constructor and methods
+(``aquaflux/flow/momentum.py:233-924``): ``build(mesh, geometry, properties, gradient_scheme,
+boundary, *, advection_scheme, pressure_pin, pressure_pin_value, body_force, sources)``,
+``residual(state)``, ``unpack``/``pack``/``initial_state``, ``with_eddy_viscosity``.  The residual
+is a ``torch.autograd.Function`` whose backward is the hand-written HIP adjoint and whose forward
+derivative is the dual-number HIP tangent, so ``torch.autograd`` (reverse) and
+``torch.autograd.forward_ad`` (forward) both differentiate through it with respect to the state,
+the per-cell viscosity / density, and the boundary velocity / pressure.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .boundary import BoundaryConditions
+from .discretization import AdvectionScheme, LimitedUpwind
+
+FLOW_NONE, FLOW_WALL, FLOW_INLET, FLOW_OUTLET = 0, 1, 2, 3
+
+
+# --- flow boundary bundles (flow/boundary.py:123-446) ----------------------------------------------
+
+
+class FlowBoundary:
+    kind = FLOW_NONE
+
+    def prescribed_velocity(self, dim):
+        return None
+
+    def prescribed_pressure(self):
+        return None
+
+    def shears_flow(self) -> bool:
+        return False
+
+
+class NoSlipWall(FlowBoundary):
+    """Zero velocity, zero-gradient pressure, no through-flow (``flow/boundary.py:278-305``)."""
+
+    kind = FLOW_WALL
+
+    def prescribed_velocity(self, dim):
+        return torch.zeros(dim, dtype=torch.float64)
+
+    def shears_flow(self):
+        return True
+
+
+class MovingWall(FlowBoundary):
+    """Prescribed tangential velocity, no through-flow (``flow/boundary.py:308-346``)."""
+
+    kind = FLOW_WALL
+
+    def __init__(self, velocity):
+        self.velocity = velocity
+
+    def prescribed_velocity(self, dim):
+        return _as_vector(self.velocity, dim)
+
+    def shears_flow(self):
+        return True
+
+
+class VelocityInlet(FlowBoundary):
+    """Prescribed velocity, zero-gradient pressure, ``mdot = rho (u_in . n) A``
+    (``flow/boundary.py:349-384``).  ``velocity`` may be a tensor requiring grad."""
+
+    kind = FLOW_INLET
+
+    def __init__(self, velocity):
+        self.velocity = velocity
+
+    def prescribed_velocity(self, dim):
+        return _as_vector(self.velocity, dim)
+
+
+class PressureOutlet(FlowBoundary):
+    """Prescribed pressure, zero-gradient velocity, Rhie-Chow mass flux toward ``p_b``
+    (``flow/boundary.py:387-446``)."""
+
+    kind = FLOW_OUTLET
+
+    def __init__(self, pressure=0.0):
+        self.pressure = pressure
+
+    def prescribed_pressure(self):
+        return self.pressure
+
+
+def _as_vector(v, dim):
+    if callable(v) and not isinstance(v, torch.Tensor):
+        raise ValueError("velocity profiles (callables) are not supported; pass a constant vector")
+    t = v if isinstance(v, torch.Tensor) else torch.tensor(list(v), dtype=torch.float64)
+    t = t.to(torch.float64)
+    if t.shape != (dim,):
+        raise ValueError(f"velocity must have shape ({dim},), got {tuple(t.shape)}")
+    return t
+
+
+# --- the residual operator ---------------------------------------------------------------------
+
+
+class _Raw:
+    """Binds the C ABI calls for one configured flow problem."""
+
+    def __init__(self, mesh, geometry, cfg: _lib.AfxFlowConfig, kind: torch.Tensor):
+        self.mesh = mesh
+        self.dm = mesh.device_mesh(geometry.face.area.device)
+        self.geometry = geometry
+        self.gs = geometry.c_struct()
+        self.cfg = cfg
+        self.kind = kind  # keeps the device buffer alive
+        L = _lib.lib()
+        self.work_bytes = []
+        for mode in range(3):
+            n = ctypes.c_int64()
+            _lib.check(L.afx_flow_workspace_bytes(self.dm.handle, mode, ctypes.byref(n)), "afx_flow_workspace_bytes")
+            self.work_bytes.append(int(n.value))
+
+    def inputs(self, state, mu, rho, ub, pb, gb=None):
+        return _lib.AfxFlowInputs(
+            _lib.ptr(state), _lib.ptr(mu), _lib.ptr(rho), _lib.ptr(ub), _lib.ptr(pb), _lib.ptr(gb)
+        )
+
+    def residual(self, state, mu, rho, ub, pb, gb=None, out=None):
+        R = torch.empty_like(state) if out is None else out
+        w = self.dm.workspace("flow0", self.work_bytes[0])
+        ins = self.inputs(state, mu, rho, ub, pb, gb)
+        _lib.check(
+            _lib.lib().afx_flow_residual(
+                self.dm.handle, ctypes.byref(self.gs), ctypes.byref(self.cfg), ctypes.byref(ins),
+                R.data_ptr(), w.data_ptr(), w.numel(), _lib.stream_of(self.dm.device),
+            ),
+            "afx_flow_residual",
+        )
+        return R
+
+    def jvp(self, primals, tangents):
+        state = primals[0]
+        R = torch.empty_like(state)
+        dR = torch.empty_like(state)
+        w = self.dm.workspace("flow1", self.work_bytes[1])
+        ins = self.inputs(*primals)
+        tns = self.inputs(*tangents)
+        _lib.check(
+            _lib.lib().afx_flow_jvp(
+                self.dm.handle, ctypes.byref(self.gs), ctypes.byref(self.cfg), ctypes.byref(ins), ctypes.byref(tns),
+                R.data_ptr(), dR.data_ptr(), w.data_ptr(), w.numel(), _lib.stream_of(self.dm.device),
+            ),
+            "afx_flow_jvp",
+        )
+        return R, dR
+
+    def vjp(self, primals, cot):
+        state, mu, rho, ub, pb = primals[:5]
+        gb = primals[5] if len(primals) > 5 else None
+        bars = [torch.empty_like(state), torch.empty_like(mu), torch.empty_like(rho), torch.empty_like(ub), torch.empty_like(pb)]
+        gbb = torch.empty_like(gb) if gb is not None else None
+        w = self.dm.workspace("flow2", self.work_bytes[2])
+        ins = self.inputs(state, mu, rho, ub, pb, gb)
+        outs = _lib.AfxFlowCotangents(*[b.data_ptr() for b in bars], _lib.ptr(gbb))
+        _lib.check(
+            _lib.lib().afx_flow_vjp(
+                self.dm.handle, ctypes.byref(self.gs), ctypes.byref(self.cfg), ctypes.byref(ins), cot.data_ptr(),
+                ctypes.byref(outs), w.data_ptr(), w.numel(), _lib.stream_of(self.dm.device),
+            ),
+            "afx_flow_vjp",
+        )
+        return bars + ([gbb] if gb is not None else [])
+
+    def fields(self, state, mu, rho, ub, pb, gb=None):
+        d, C = self.mesh.dim, self.mesh.n_cells
+        dev = state.device
+        G = torch.empty(((d + 1) * d, C), dtype=torch.float64, device=dev)
+        aP = torch.empty(C, dtype=torch.float64, device=dev)
+        psi = torch.empty((d, C), dtype=torch.float64, device=dev) if self.cfg.advection == 2 else None
+        ins = self.inputs(state, mu, rho, ub, pb, gb)
+        _lib.check(
+            _lib.lib().afx_flow_fields(
+                self.dm.handle, ctypes.byref(self.gs), ctypes.byref(self.cfg), ctypes.byref(ins),
+                G.data_ptr(), aP.data_ptr(), _lib.ptr(psi), _lib.stream_of(dev),
+            ),
+            "afx_flow_fields",
+        )
+        return G, aP, psi
+
+
+class _FlowResidualFn(torch.autograd.Function):
+    """R(state, mu, rho, u_b, p_b[, gamma_b]) with HIP forward, tangent and adjoint."""
+
+    @staticmethod
+    def forward(ctx, raw, state, mu, rho, ub, pb, gb):
+        ctx.raw = raw
+        ctx.has_gb = gb is not None
+        prim = (state, mu, rho, ub, pb) + ((gb,) if gb is not None else ())
+        ctx.save_for_backward(*prim)
+        ctx.save_for_forward(*prim)
+        return raw.residual(state, mu, rho, ub, pb, gb)
+
+    @staticmethod
+    def backward(ctx, gR):
+        prim = ctx.saved_tensors
+        bars = ctx.raw.vjp(prim, gR.contiguous())
+        gbb = bars[5] if ctx.has_gb else None
+        return (None, bars[0], bars[1], bars[2], bars[3], bars[4], gbb)
+
+    @staticmethod
+    def jvp(ctx, _raw_t, dstate, dmu, drho, dub, dpb, dgb):
+        prim = ctx.saved_tensors
+        tans = [t.contiguous() if t is not None else None for t in (dstate, dmu, drho, dub, dpb)]
+        if ctx.has_gb:
+            tans.append(dgb.contiguous() if dgb is not None else None)
+        _, dR = ctx.raw.jvp(prim, tans)
+        return dR
+
+
+class MomentumContinuity:
+    """Coupled momentum + Rhie-Chow continuity residual (``aquaflux/flow/momentum.py:158-924``)."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+    @classmethod
+    def build(
+        cls,
+        mesh,
+        geometry,
+        properties,
+        gradient_scheme,
+        boundary: BoundaryConditions,
+        *,
+        advection_scheme: AdvectionScheme | None = None,
+        pressure_pin: int | None = None,
+        pressure_pin_value: float = 0.0,
+        body_force=None,
+        sources: tuple = (),
+    ) -> "MomentumContinuity":
+        properties.require("viscosity", "density")
+        if sources:
+            raise ValueError("momentum sources are not supported by the GPU residual")
+        d = mesh.dim
+        dev = geometry.face.area.device
+        bound = boundary.resolve(mesh.face_patches)
+        kind = np.zeros(mesh.n_faces, dtype=np.int8)
+        for name, bc, faces in bound.items():
+            if not isinstance(bc, FlowBoundary):
+                raise ValueError(f"patch '{name}': {type(bc).__name__} is not a flow boundary")
+            if np.any(mesh.neighbour[faces] >= 0):
+                raise ValueError(f"patch '{name}' names interior faces; baffles are not supported")
+            kind[faces] = bc.kind
+        cfg = _lib.AfxFlowConfig()
+        cfg.advection = 0 if advection_scheme is None else int(advection_scheme.code)
+        cfg.gradient = int(gradient_scheme.code)
+        if cfg.gradient not in (0, 1):
+            raise ValueError(f"unsupported gradient scheme {type(gradient_scheme).__name__}")
+        cfg.pressure_pin = -1 if pressure_pin is None else int(pressure_pin)
+        cfg.pressure_pin_value = float(pressure_pin_value)
+        bf = [0.0, 0.0, 0.0]
+        if body_force is not None:
+            for i, v in enumerate(np.asarray(body_force, dtype=float).ravel()[:d]):
+                bf[i] = float(v)
+        cfg.body_force = (ctypes.c_double * 3)(*bf)
+        cfg.limiter_k = advection_scheme.limiter_k() if isinstance(advection_scheme, LimitedUpwind) else 5.0
+        kind_dev = torch.from_numpy(kind).to(dev)
+        cfg.face_kind = kind_dev.data_ptr()
+        raw = _Raw(mesh, geometry, cfg, kind_dev)
+        return cls(
+            mesh=mesh,
+            geometry=geometry,
+            properties=properties,
+            gradient_scheme=gradient_scheme,
+            advection_scheme=advection_scheme,
+            boundary=bound,
+            pressure_pin=pressure_pin,
+            pressure_pin_value=pressure_pin_value,
+            body_force=bf[:d],
+            eddy_viscosity=None,
+            wall_eddy_viscosity=None,
+            _raw=raw,
+        )
+
+    # --- layout (flow/state.py:17-73) ------------------------------------------------------------
+    @property
+    def dim(self):
+        return self.mesh.dim
+
+    @property
+    def device(self):
+        return self.geometry.face.area.device
+
+    def unpack(self, state):
+        blocks = state.reshape(self.dim + 1, self.mesh.n_cells)
+        return blocks[: self.dim].T, blocks[self.dim]
+
+    def pack(self, velocity, pressure):
+        return torch.cat([velocity.T, pressure[None, :]]).reshape(-1)
+
+    def initial_state(self):
+        return torch.zeros((self.dim + 1) * self.mesh.n_cells, dtype=torch.float64, device=self.device)
+
+    # --- properties ------------------------------------------------------------------------------
+    def with_eddy_viscosity(self, eddy_viscosity, wall_eddy_viscosity=None) -> "MomentumContinuity":
+        """Copy carrying a kinematic eddy viscosity (``flow/momentum.py:314-349``)."""
+        new = MomentumContinuity(**self.__dict__)
+        new.eddy_viscosity = eddy_viscosity
+        new.wall_eddy_viscosity = wall_eddy_viscosity
+        return new
+
+    def material(self):
+        props = self.properties.evaluate(self.mesh.n_cells, self.device)
+        return props["viscosity"], props["density"]
+
+    @property
+    def viscosity(self):
+        mu, rho = self.material()
+        return mu if self.eddy_viscosity is None else mu + rho * self.eddy_viscosity
+
+    @property
+    def density(self):
+        return self.material()[1]
+
+    def shear_mask(self):
+        mask = torch.zeros(self.mesh.n_faces, dtype=torch.bool)
+        for _, bc, faces in self.boundary.items():
+            if bc.shears_flow():
+                mask[torch.from_numpy(faces)] = True
+        return mask.to(self.device)
+
+    def boundary_data(self):
+        """Per-face prescribed velocity ``(d, F)`` and outlet pressure ``(F,)`` (differentiable)."""
+        d, F = self.dim, self.mesh.n_faces
+        ub = torch.zeros((d, F), dtype=torch.float64, device=self.device)
+        pb = torch.zeros(F, dtype=torch.float64, device=self.device)
+        for _, bc, faces in self.boundary.items():
+            idx = torch.from_numpy(faces).to(self.device)
+            v = bc.prescribed_velocity(d)
+            if v is not None:
+                v = v.to(self.device)
+                ub = ub.index_put((slice(None), idx), v[:, None].expand(d, idx.numel()))
+            p = bc.prescribed_pressure()
+            if p is not None:
+                pv = p if isinstance(p, torch.Tensor) else torch.tensor(float(p), dtype=torch.float64)
+                pb = pb.index_put((idx,), pv.to(self.device).reshape(()).expand(idx.numel()))
+        return ub.contiguous(), pb.contiguous()
+
+    def _inputs(self):
+        mu, rho = self.material()
+        mu_eff = mu if self.eddy_viscosity is None else mu + rho * self.eddy_viscosity
+        ub, pb = self.boundary_data()
+        gb = None
+        if self.wall_eddy_viscosity is not None:
+            own = self.mesh.device_mesh(self.device).owner
+            wall_mu = mu[own] + rho[own] * self.wall_eddy_viscosity
+            gb = torch.where(self.shear_mask(), wall_mu, mu_eff[own]).contiguous()
+        return mu_eff.contiguous(), rho.contiguous(), ub, pb, gb
+
+    # --- residual --------------------------------------------------------------------------------
+    def residual(self, state):
+        """Coupled residual, same shape as ``state`` (``flow/momentum.py:880-882``)."""
+        _lib.require_cuda(state, "state")
+        mu, rho, ub, pb, gb = self._inputs()
+        return _FlowResidualFn.apply(self._raw, state.contiguous(), mu, rho, ub, pb, gb)
+
+    __call__ = residual
+
+    def residual_jvp(self, state, tangent):
+        """(R, J tangent) with respect to the state only (the Newton-Krylov matvec)."""
+        mu, rho, ub, pb, gb = self._inputs()
+        prim = [state.contiguous(), mu.detach(), rho.detach(), ub.detach(), pb.detach()]
+        if gb is not None:
+            prim.append(gb.detach())
+        tans = [tangent.contiguous(), None, None, None, None] + ([None] if gb is not None else [])
+        return self._raw.jvp(prim, tans)
+
+    def residual_vjp(self, state, cotangent):
+        """J^T cotangent with respect to the state (the adjoint-Krylov matvec)."""
+        mu, rho, ub, pb, gb = self._inputs()
+        prim = [state.contiguous(), mu.detach(), rho.detach(), ub.detach(), pb.detach()]
+        if gb is not None:
+            prim.append(gb.detach())
+        return self._raw.vjp(prim, cotangent.contiguous())[0]
+
+    def fields(self, state):
+        """Gradients ``((d+1) d, C)``, ``a_P`` ``(C,)`` and limiter ``psi`` for a state."""
+        mu, rho, ub, pb, gb = self._inputs()
+        return self._raw.fields(state.contiguous(), mu.detach(), rho.detach(), ub.detach(), pb.detach(),
+                                None if gb is None else gb.detach())

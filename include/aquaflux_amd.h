@@ -1,0 +1,132 @@
+/* aquaflux_amd -- C ABI of the MI355X (gfx950) finite-volume residual library.
+ *
+ * Every entry point returns 0 on success, 1 on a HIP runtime failure and 2 on an invalid argument;
+ * `afx_last_error()` returns the calling thread's last message.  Nothing throws across the ABI.
+ * All array arguments are DEVICE pointers on the handle's device unless stated otherwise, laid
+ * out structure-of-arrays: a per-face vector field is `v[j * n_faces + f]`, a per-cell one
+ * `v[j * n_cells + c]`.  The caller owns every buffer; the mesh handle owns only its topology
+ * copies and scratch.  Work is enqueued on `stream` and is stream-ordered; calls on one handle
+ * from several host threads must use distinct workspaces.
+ *
+ * The state vector keeps the reference's field-major layout
+ * [u-block, v-block, (w-block), p-block, ...], each block n_cells long
+ * (reference: aquaflux/flow/state.py:36-69).
+ */
+#ifndef AQUAFLUX_AMD_H
+#define AQUAFLUX_AMD_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------------------------- */
+/* errors                                                                                        */
+
+/* Message of the calling thread's last failure ("" if none). */
+const char* afx_last_error(void);
+
+/* ------------------------------------------------------------------------------------------- */
+/* mesh                                                                                          */
+
+typedef struct afx_mesh afx_mesh;
+
+/* Builds a device mesh handle on the current device from HOST arrays.
+ * Replaces the connectivity construction of Mesh.from_csr / FaceCellConnectivity /
+ * FaceNodeConnectivity (reference: aquaflux/mesh/mesh.py:137-206,
+ * aquaflux/mesh/connectivity.py:114-136, :399-459): owner[f], neighbour[f] (-1 = boundary face),
+ * face -> node CSR (face_node_offsets[n_faces + 1], face_node_indices[...]). */
+int afx_mesh_create(int dim, int64_t n_cells, int64_t n_faces, int64_t n_nodes, const int32_t* owner,
+                    const int32_t* neighbour, const int64_t* face_node_offsets,
+                    const int32_t* face_node_indices, afx_mesh** out);
+int afx_mesh_destroy(afx_mesh* mesh);
+/* out[7] = {n_cells, n_faces, n_nodes, dim, n_cell_face_entries, max_faces_per_cell, device} */
+int afx_mesh_info(const afx_mesh* mesh, int64_t* out);
+/* Copies the cell -> face list (entries f when the cell owns f, ~f when it is f's neighbour). */
+int afx_mesh_cell_faces(const afx_mesh* mesh, int32_t* cell_ptr, int32_t* cell_faces, hipStream_t stream);
+
+/* ------------------------------------------------------------------------------------------- */
+/* geometry                                                                                      */
+
+typedef struct {
+  const double* area;          /* [F]    */
+  const double* normal;        /* [d][F] owner-outward unit normal */
+  const double* face_centroid; /* [d][F] */
+  const double* volume;        /* [C]    */
+  const double* cell_centroid; /* [d][C] */
+} afx_geometry;
+
+/* Face/cell geometry from node coordinates node_coords[d][N] into the (writable) buffers of `out`.
+ * Replaces Mesh.geometry / _fused_geometry (reference: aquaflux/mesh/mesh.py:341-387,
+ * aquaflux/mesh/face.py:196-349, aquaflux/mesh/cell.py:84-170). */
+int afx_geometry_compute(const afx_mesh* mesh, const double* node_coords, afx_geometry* out, hipStream_t stream);
+
+/* ------------------------------------------------------------------------------------------- */
+/* coupled (u, v[, w], p) flow residual                                                          */
+
+enum { AFX_FLOW_NONE = 0, AFX_FLOW_WALL = 1, AFX_FLOW_INLET = 2, AFX_FLOW_OUTLET = 3 };
+enum { AFX_ADV_NONE = 0, AFX_ADV_UPWIND = 1, AFX_ADV_LIMITED = 2 };
+enum { AFX_GRAD_GREEN_GAUSS = 0, AFX_GRAD_LEAST_SQUARES = 1 };
+
+typedef struct {
+  int32_t advection;          /* AFX_ADV_*: None (Stokes) / FirstOrderUpwind / LimitedUpwind(Venkatakrishnan) */
+  int32_t gradient;           /* AFX_GRAD_*: CompactGreenGauss / least squares */
+  int64_t pressure_pin;       /* cell whose continuity row becomes p - value; -1 = none */
+  double pressure_pin_value;
+  double body_force[3];       /* uniform force per unit volume */
+  double limiter_k;           /* Venkatakrishnan K (eps^2 = V K^3) */
+  const int8_t* face_kind;    /* [F] AFX_FLOW_* per face (interior faces: AFX_FLOW_NONE) */
+} afx_flow_config;
+
+typedef struct {
+  const double* state;   /* [(d+1)][C] field-major u, v, (w), p */
+  const double* mu;      /* [C] dynamic (effective) viscosity */
+  const double* rho;     /* [C] density */
+  const double* u_b;     /* [d][F] prescribed velocity on wall / inlet faces */
+  const double* p_b;     /* [F] prescribed pressure on outlet faces */
+  const double* gamma_b; /* [F] optional boundary-face momentum viscosity override (NULL = owner mu) */
+} afx_flow_inputs;
+
+typedef struct {
+  double* state;   /* [(d+1)][C] */
+  double* mu;      /* [C] */
+  double* rho;     /* [C] */
+  double* u_b;     /* [d][F] */
+  double* p_b;     /* [F] */
+  double* gamma_b; /* [F] (required iff inputs.gamma_b is set) */
+} afx_flow_cotangents;
+
+/* Workspace (device bytes) for mode 0 = residual, 1 = JVP, 2 = VJP. */
+int afx_flow_workspace_bytes(const afx_mesh* mesh, int mode, int64_t* bytes);
+
+/* R(state).  Replaces MomentumContinuity.residual (reference: aquaflux/flow/momentum.py:880-882,
+ * with flow_fields :822-857 and residual_from_fields :859-878). */
+int afx_flow_residual(const afx_mesh* mesh, const afx_geometry* geom, const afx_flow_config* cfg,
+                      const afx_flow_inputs* in, double* residual, void* work, int64_t work_bytes,
+                      hipStream_t stream);
+
+/* R and dR = J . tangent (tangent pointers may be NULL = zero).  Replaces jax.jvp of the residual
+ * (reference: aquaflux/solve/newton.py:108-121). */
+int afx_flow_jvp(const afx_mesh* mesh, const afx_geometry* geom, const afx_flow_config* cfg,
+                 const afx_flow_inputs* in, const afx_flow_inputs* tangent, double* residual,
+                 double* residual_tangent, void* work, int64_t work_bytes, hipStream_t stream);
+
+/* Cotangents J^T . cotangent with respect to state, mu, rho, u_b, p_b (gamma_b).  Replaces jax.vjp
+ * of the residual (reference: aquaflux/solve/implicit.py:733-761). */
+int afx_flow_vjp(const afx_mesh* mesh, const afx_geometry* geom, const afx_flow_config* cfg,
+                 const afx_flow_inputs* in, const double* cotangent, afx_flow_cotangents* out, void* work,
+                 int64_t work_bytes, hipStream_t stream);
+
+/* Intermediate fields: gradients [(d+1)d][C], a_P [C], psi [d][C] (limited advection only).
+ * Replaces MomentumContinuity.flow_fields' gradient / momentum_matrix_diagonal products
+ * (reference: aquaflux/flow/momentum.py:822-857). */
+int afx_flow_fields(const afx_mesh* mesh, const afx_geometry* geom, const afx_flow_config* cfg,
+                    const afx_flow_inputs* in, double* gradients, double* a_p, double* psi, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AQUAFLUX_AMD_H */

@@ -1,0 +1,310 @@
+"""Pins the CPU oracle against the reference's own closed-form unit-test facts (CPU only).
+
+Each test cites the reference test it restates.  These run without a GPU.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from aquaflux_amd.mesh import Mesh, perturbed_grid_2d, perturbed_grid_3d, structured_grid_2d, structured_grid_3d, tet_grid_3d
+from oracle import fv
+from oracle.connectivity import structured_2d, structured_3d
+
+F64 = torch.float64
+
+
+def _geom(mesh):
+    m = fv.OMesh.from_mesh(mesh)
+    return m, fv.geometry(m)
+
+
+# --- geometry: tests/unit/test_mesh.py closed forms --------------------------------------------
+
+
+def test_two_unit_squares():
+    """test_mesh.py:276-305: unit edges, midpoints, +x shared normal, unit volumes/centroids."""
+    nodes = [[0, 0], [1, 0], [2, 0], [0, 1], [1, 1], [2, 1]]
+    faces = [[1, 4], [0, 1], [4, 3], [3, 0], [1, 2], [2, 5], [5, 4]]
+    mesh = Mesh.from_faces(nodes, faces, [0, 0, 0, 0, 1, 1, 1], [1, -1, -1, -1, -1, -1, -1], n_cells=2)
+    m, g = _geom(mesh)
+    assert torch.allclose(g.area, torch.ones(7, dtype=F64))
+    assert torch.allclose(g.face_centroid[0], torch.tensor([1.0, 0.5], dtype=F64))
+    assert torch.allclose(g.normal[0], torch.tensor([1.0, 0.0], dtype=F64))
+    assert torch.allclose(g.volume, torch.ones(2, dtype=F64))
+    assert torch.allclose(g.cell_centroid, torch.tensor([[0.5, 0.5], [1.5, 0.5]], dtype=F64))
+
+
+def test_triangle_and_parallelogram():
+    """test_mesh.py:307-321."""
+    tri = Mesh.from_faces([[0, 0], [1, 0], [0, 1]], [[0, 1], [1, 2], [2, 0]], [0, 0, 0], [-1, -1, -1], n_cells=1)
+    _, g = _geom(tri)
+    assert torch.allclose(g.volume, torch.tensor([0.5], dtype=F64))
+    assert torch.allclose(g.cell_centroid, torch.tensor([[1 / 3, 1 / 3]], dtype=F64))
+    par = Mesh.from_faces([[0, 0], [2, 0], [3, 1], [1, 1]], [[0, 1], [1, 2], [2, 3], [3, 0]], [0] * 4, [-1] * 4, n_cells=1)
+    _, g = _geom(par)
+    assert torch.allclose(g.volume, torch.tensor([2.0], dtype=F64))
+    assert torch.allclose(g.cell_centroid, torch.tensor([[1.5, 0.5]], dtype=F64))
+
+
+CUBE_NODES = [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]]
+CUBE_FACES = [[0, 1, 2, 3], [4, 5, 6, 7], [0, 1, 5, 4], [3, 2, 6, 7], [0, 3, 7, 4], [1, 2, 6, 5]]
+
+
+def test_cube_tet_box():
+    """test_mesh.py:324-362: unit cube, tetrahedron, 2x3x4 box."""
+    cube = Mesh.from_faces(CUBE_NODES, CUBE_FACES, [0] * 6, [-1] * 6, n_cells=1)
+    _, g = _geom(cube)
+    assert torch.allclose(g.area, torch.ones(6, dtype=F64))
+    assert torch.allclose(g.volume, torch.tensor([1.0], dtype=F64))
+    assert torch.allclose(g.cell_centroid, torch.tensor([[0.5, 0.5, 0.5]], dtype=F64))
+    out = g.face_centroid - g.cell_centroid[0]
+    assert bool(torch.all((out * g.normal).sum(1) > 0))
+    tet = Mesh.from_faces(
+        [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], [[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]], [0] * 4, [-1] * 4, n_cells=1
+    )
+    _, g = _geom(tet)
+    assert torch.allclose(g.volume, torch.tensor([1 / 6], dtype=F64))
+    assert torch.allclose(g.cell_centroid, torch.tensor([[0.25, 0.25, 0.25]], dtype=F64))
+    box = Mesh.from_faces(np.array(CUBE_NODES) * [2.0, 3.0, 4.0], CUBE_FACES, [0] * 6, [-1] * 6, n_cells=1)
+    _, g = _geom(box)
+    assert torch.allclose(g.volume, torch.tensor([24.0], dtype=F64))
+    assert torch.allclose(g.cell_centroid, torch.tensor([[1.0, 1.5, 2.0]], dtype=F64))
+    assert torch.allclose(torch.sort(g.area).values, torch.tensor([6.0, 6.0, 8.0, 8.0, 12.0, 12.0], dtype=F64))
+
+
+def test_prism_mixed_faces():
+    """test_mesh.py:375-394: triangular prism, volume 1/2, centroid (1/3, 1/3, 1/2)."""
+    nodes = [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [0, 1, 1]]
+    faces = [[0, 1, 2], [3, 4, 5], [0, 1, 4, 3], [1, 2, 5, 4], [2, 0, 3, 5]]
+    mesh = Mesh.from_faces(nodes, faces, [0] * 5, [-1] * 5, n_cells=1)
+    _, g = _geom(mesh)
+    assert torch.allclose(g.area[:2], torch.tensor([0.5, 0.5], dtype=F64))
+    assert torch.allclose(g.volume, torch.tensor([0.5], dtype=F64))
+    assert torch.allclose(g.cell_centroid, torch.tensor([[1 / 3, 1 / 3, 0.5]], dtype=F64))
+
+
+def test_far_from_origin_centroid():
+    """test_mesh.py:663-683: recentred accumulation keeps a tiny far cell's centroid accurate."""
+    s = 1e-3
+    off = np.array([1.7e6, -5.1e5, 3.2e5])
+    mesh = Mesh.from_faces(np.array(CUBE_NODES) * s + off, CUBE_FACES, [0] * 6, [-1] * 6, n_cells=1)
+    _, g = _geom(mesh)
+    assert abs(float(g.volume[0]) - s**3) <= 1e-6 * s**3
+    err = np.linalg.norm(g.cell_centroid[0].numpy() - (off + 0.5 * s)) / s
+    assert err < 1e-3
+
+
+def test_structured_volumes_sum_and_tets():
+    """Volumes tile the domain; every face normal is unit; tet split volumes are h^3/6."""
+    for mesh, total in (
+        (perturbed_grid_2d(8, 5, 2.0, 1.0, perturb=0.2, seed=0), 2.0),
+        (perturbed_grid_3d(4, 3, 3, perturb=0.2, seed=1), 1.0),
+        (tet_grid_3d(3, perturb=0.15, seed=3), 1.0),
+    ):
+        _, g = _geom(mesh)
+        assert abs(float(g.volume.sum()) - total) < 1e-12
+        assert torch.allclose(g.normal.norm(dim=1), torch.ones(mesh.n_faces, dtype=F64))
+        assert bool(torch.all(g.volume > 0))
+    _, g = _geom(tet_grid_3d(2))
+    assert torch.allclose(g.volume, torch.full((48,), 1 / 48, dtype=F64))
+
+
+def test_structured_connectivity_bit_exact():
+    """Vectorised generators == loop restatement of mesh/structured.py (faces, owner, nb, patches)."""
+    m = structured_grid_2d(6, 4, named_boundaries=True)
+    r, o, n, p = structured_2d(6, 4)
+    assert np.array_equal(m.face_node_indices.reshape(-1, 2), np.array(r))
+    assert np.array_equal(m.owner, o) and np.array_equal(m.neighbour, n)
+    for k, v in p.items():
+        assert np.array_equal(m.face_patches.indices(k), v)
+    m = structured_grid_3d(4, 3, 2, named_boundaries=True)
+    r, o, n, p = structured_3d(4, 3, 2)
+    assert np.array_equal(m.face_node_indices.reshape(-1, 4), np.array(r))
+    assert np.array_equal(m.owner, o) and np.array_equal(m.neighbour, n)
+    for k, v in p.items():
+        assert np.array_equal(m.face_patches.indices(k), v)
+    # sizes of the survey's configurations (face counts of the scope table)
+    assert structured_grid_2d(64, 64).n_faces == 8320
+    assert structured_grid_2d(512, 256).n_faces == 262912
+
+
+# --- fluxes: tests/unit/test_diffusion.py, test_advection.py closed forms ---------------------------
+
+
+def _one_face(phi_o, phi_n, *, grad=(0.0, 0.0), interior=True, bval=0.0, gamma=(1.0, 1.0)):
+    """test_diffusion.py:34-74: owner (0.5,0), neighbour (1.5,0), face (1,0), n=x, A=0.5."""
+    nc = 2 if interior else 1
+    m = fv.OMesh(
+        node_coords=torch.zeros((2, 2), dtype=F64),
+        owner=torch.tensor([0]),
+        neighbour=torch.tensor([1 if interior else -1]),
+        offsets=np.array([0, 2]),
+        indices=np.array([0, 1]),
+        n_cells=nc,
+        patches={},
+    )
+    g = fv.OGeometry(
+        area=torch.tensor([0.5], dtype=F64),
+        face_centroid=torch.tensor([[1.0, 0.0]], dtype=F64),
+        normal=torch.tensor([[1.0, 0.0]], dtype=F64),
+        volume=torch.ones(nc, dtype=F64),
+        cell_centroid=torch.tensor([[0.5, 0.0], [1.5, 0.0]], dtype=F64)[:nc],
+    )
+    phi = torch.tensor([phi_o, phi_n][:nc], dtype=F64)
+    grad_t = torch.tensor([list(grad)] * nc, dtype=F64)
+    gam = torch.tensor(list(gamma)[:nc], dtype=F64)
+    return m, g, phi, grad_t, gam, torch.tensor([bval], dtype=F64)
+
+
+def test_diffusion_closed_forms():
+    m, g, phi, grad, gam, bv = _one_face(1.0, 3.0)
+    assert abs(float(fv.diffusion_flux(m, g, phi, grad, gam, bv)[0]) + 0.5 * 2.0) < 1e-13  # :77-82
+    m, g, phi, grad, gam, bv = _one_face(1.0, 0.0, interior=False, bval=2.0)
+    assert abs(float(fv.diffusion_flux(m, g, phi, grad, gam, bv)[0]) + 1.0 * (2.0 - 1.0) / 0.5 * 0.5) < 1e-13  # :118-122
+    m, g, phi, grad, gam, bv = _one_face(0.0, 0.0, gamma=(2.0, 8.0))
+    cond = fv.conductance(m, g, gam)
+    assert abs(float(cond[0]) - 2 * 2 * 8 / 10 * 0.5) < 1e-13  # :85-97 harmonic mean
+    # conductance == d(owner flux)/d(phi_P)  (:100-115)
+    phi = torch.tensor([1.0, 0.0], dtype=F64, requires_grad=True)
+    flux = fv.diffusion_flux(m, g, phi, grad, gam, bv)[0]
+    (dphi,) = torch.autograd.grad(flux, phi)
+    assert abs(float(dphi[0]) - float(cond[0])) < 1e-13
+
+
+def test_upwind_selection():
+    """test_advection.py:26-60: owner value on outflow, neighbour on inflow, weak value at inlets."""
+    m, g, phi, grad, gam, bv = _one_face(1.0, 3.0)
+    assert float(fv.upwind_face_value(m, phi, torch.tensor([2.0], dtype=F64), bv)[0]) == 1.0
+    assert float(fv.upwind_face_value(m, phi, torch.tensor([-2.0], dtype=F64), bv)[0]) == 3.0
+    m, g, phi, grad, gam, bv = _one_face(1.0, 0.0, interior=False, bval=7.0)
+    assert float(fv.upwind_face_value(m, phi, torch.tensor([-1.0], dtype=F64), bv)[0]) == 7.0
+    assert float(fv.upwind_face_value(m, phi, torch.tensor([1.0], dtype=F64), bv)[0]) == 1.0
+
+
+def test_limited_reconstruction_closed_form():
+    """test_advection.py:63-100: phi_C + psi grad.(x_f - x_C) = 3 + 2 * 0.5 with psi = 1."""
+    m, g, phi, grad, gam, bv = _one_face(3.0, 0.0, grad=(2.0, 0.0))
+    psi = torch.ones(2, dtype=F64)
+    v = fv.limited_face_value(m, g, phi, grad, psi, torch.tensor([1.0], dtype=F64), bv)
+    assert abs(float(v[0]) - 4.0) < 1e-13
+
+
+# --- gradients / limiter -------------------------------------------------------------------------
+
+
+def _linear(g, c=(0.3, -1.2, 0.7)):
+    x = g.cell_centroid
+    return 1.5 + sum(c[j] * x[:, j] for j in range(x.shape[1]))
+
+
+@pytest.mark.parametrize("scheme", ["gg", "lsq"])
+def test_gradient_linear_exactness(scheme):
+    """test_gradient.py:57-110: linear fields are reconstructed exactly (GG: orthogonal grid)."""
+    mesh = structured_grid_2d(6, 5, 1.0, 1.0) if scheme == "gg" else perturbed_grid_2d(6, 5, perturb=0.2, seed=0)
+    m, g = _geom(mesh)
+    c = (0.3, -1.2)
+    phi = _linear(g, c)
+    bv = 1.5 + c[0] * g.face_centroid[:, 0] + c[1] * g.face_centroid[:, 1]
+    grad = fv.compact_green_gauss(m, g, phi, bv) if scheme == "gg" else fv.least_squares_gradient(m, g, phi, bv)
+    interior_cells = torch.ones(m.n_cells, dtype=torch.bool)
+    if scheme == "gg":  # orthogonal uniform grid: exact in every cell
+        pass
+    assert torch.allclose(grad[interior_cells], torch.tensor(c, dtype=F64).expand(m.n_cells, 2), atol=1e-10)
+
+
+def test_lsq_matches_bruteforce_lstsq():
+    """LSQ (no reference counterpart) == numpy.linalg.lstsq per cell on a skewed 3D mesh."""
+    mesh = perturbed_grid_3d(3, 3, 2, perturb=0.2, seed=4)
+    m, g = _geom(mesh)
+    rng = np.random.default_rng(0)
+    phi = torch.tensor(rng.normal(size=m.n_cells), dtype=F64)
+    bv = torch.tensor(rng.normal(size=m.n_faces), dtype=F64)
+    grad = fv.least_squares_gradient(m, g, phi, bv).numpy()
+    own, nb = mesh.owner, mesh.neighbour
+    cc, fc = g.cell_centroid.numpy(), g.face_centroid.numpy()
+    for c in range(m.n_cells):
+        rows, rhs = [], []
+        for f in np.nonzero((own == c) | (nb == c))[0]:
+            o = nb[f] if own[f] == c else own[f]
+            dx = (cc[o] if o >= 0 else fc[f]) - cc[c]
+            dphi = (phi[o] if o >= 0 else bv[f]) - phi[c]
+            w = 1.0 / np.sqrt(dx @ dx)
+            rows.append(w * dx)
+            rhs.append(w * float(dphi))
+        ref = np.linalg.lstsq(np.array(rows), np.array(rhs), rcond=None)[0]
+        assert np.allclose(grad[c], ref, atol=1e-10)
+
+
+def _psi_24(field):
+    """test_limiter.py:10-25: 24x24 grid, Green-Gauss gradient (orthogonal grid), K = 5."""
+    mesh = structured_grid_2d(24, 24)
+    m, g = _geom(mesh)
+    phi, bv = field(g.cell_centroid), field(g.face_centroid)
+    grad = fv.compact_green_gauss(m, g, phi, bv)
+    psi = fv.venkatakrishnan(m, g, phi, grad, 5.0)
+    bcells = set(mesh.owner[mesh.neighbour < 0].tolist())
+    interior = torch.tensor([c not in bcells for c in range(m.n_cells)])
+    return psi, interior, g.cell_centroid
+
+
+def test_limiter_linear_field_is_unlimited():
+    """test_limiter.py:31-34: psi > 0.999 in the interior on a linear field."""
+    psi, interior, _ = _psi_24(lambda x: 2.0 * x[:, 0] - 3.0 * x[:, 1] + 1.0)
+    assert float(psi[interior].min()) > 0.999
+
+
+def test_limiter_activates_near_step():
+    """test_limiter.py:37-53: psi < 0.9 near a step, > 0.99 far from it, and within [0, 1]."""
+    step = lambda x: torch.where(x[:, 0] < 0.5, 1.0, 0.0).to(F64)  # noqa: E731
+    psi, _, cc = _psi_24(step)
+    near = (cc[:, 0] - 0.5).abs() < 0.1
+    far = (cc[:, 0] - 0.5).abs() > 0.3
+    assert float(psi[near].min()) < 0.9
+    assert float(psi[far].min()) > 0.99
+    assert float(psi.min()) >= 0.0 and float(psi.max()) <= 1.0 + 1e-12
+
+
+# --- coupled flow: conservation and Rhie-Chow properties ---------------------------------------
+
+
+def _cavity_setup(m, n_faces, top_faces):
+    kind = torch.full((n_faces,), fv.FLOW_NONE, dtype=torch.long)
+    kind[~m.interior] = fv.FLOW_WALL
+    ub = torch.zeros((n_faces, m.dim), dtype=F64)
+    ub[top_faces, 0] = 1.0
+    return fv.FlowSetup(kind=kind, u_b=ub, p_b=torch.zeros(n_faces, dtype=F64), advection="upwind", pressure_pin=0)
+
+
+def test_flow_continuity_sums_to_zero_on_closed_domain():
+    """Sum of continuity rows (excluding the pin) == -(pinned cell's flux); mdot = 0 on walls."""
+    mesh = structured_grid_2d(8, 8, named_boundaries=True)
+    m, g = _geom(mesh)
+    s = _cavity_setup(m, mesh.n_faces, torch.tensor(mesh.face_patches.indices("top")))
+    s.pressure_pin = None
+    rng = np.random.default_rng(0)
+    state = torch.tensor(rng.uniform(-1, 1, 3 * m.n_cells), dtype=F64)
+    mu = torch.full((m.n_cells,), 0.01, dtype=F64)
+    rho = torch.ones(m.n_cells, dtype=F64)
+    R = fv.flow_residual(m, g, s, state, mu, rho)
+    assert abs(float(R[2 * m.n_cells:].sum())) < 1e-12
+
+
+def test_rhie_chow_cancels_on_linear_pressure():
+    """test_rhie_chow.py:19-69: with a linear p and uniform u, mdot = rho u.n A on interior faces."""
+    mesh = perturbed_grid_2d(6, 6, perturb=0.2, seed=2, named_boundaries=True)
+    m, g = _geom(mesh)
+    s = _cavity_setup(m, mesh.n_faces, torch.tensor([], dtype=torch.long))
+    s.pressure_pin = None
+    s.gradient = "least_squares"
+    C = m.n_cells
+    x = g.cell_centroid
+    u = torch.stack([torch.full((C,), 0.7, dtype=F64), torch.full((C,), -0.2, dtype=F64)], 1)
+    p = 0.4 * x[:, 0] - 1.1 * x[:, 1]
+    # impose the exact linear boundary pressure through an outlet-like closure on every face
+    s.kind[~m.interior] = fv.FLOW_OUTLET
+    s.p_b = 0.4 * g.face_centroid[:, 0] - 1.1 * g.face_centroid[:, 1]
+    state = torch.cat([u[:, 0], u[:, 1], p])
+    f = fv.flow_fields(m, g, s, state, torch.full((C,), 0.01, dtype=F64), torch.ones(C, dtype=F64))
+    ref = (u[m.owner] * g.normal).sum(1) * g.area
+    assert torch.allclose(f["mdot"][m.interior], ref[m.interior], atol=1e-12)
