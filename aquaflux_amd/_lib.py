@@ -88,6 +88,10 @@ _SIGNATURES = {
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxFlowConfig), ctypes.POINTER(AfxFlowInputs),
         c_void_p, ctypes.POINTER(AfxFlowCotangents), c_void_p, ctypes.c_int64, c_void_p,
     ],
+    "afx_profile_enable": [ctypes.c_int64],
+    "afx_profile_collect": [c_void_p, c_void_p],
+    "afx_profile_kernel_count": [],
+    "afx_profile_kernel_name": [ctypes.c_int],
     "afx_flow_fields": [
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxFlowConfig), ctypes.POINTER(AfxFlowInputs),
         c_void_p, c_void_p, c_void_p, c_void_p,
@@ -114,6 +118,7 @@ def lib():
             fn = getattr(handle, name)
             fn.restype = ctypes.c_int
             fn.argtypes = argtypes
+        handle.afx_profile_kernel_name.restype = ctypes.c_char_p
         _LIB = handle
     return _LIB
 
@@ -152,3 +157,27 @@ def require_cuda(t: torch.Tensor, name: str) -> None:
         raise ValueError(f"{name} must be float64, got {t.dtype}")
     if not t.is_contiguous():
         raise ValueError(f"{name} must be contiguous")
+
+
+class KernelTimer:
+    """Per-kernel durations from event pairs recorded inside the library on each launch stream."""
+
+    def __init__(self, max_launches: int = 100000):
+        self.max_launches = max_launches
+
+    def __enter__(self):
+        check(lib().afx_profile_enable(self.max_launches), "afx_profile_enable")
+        return self
+
+    def collect(self) -> dict:
+        """{kernel name: (total ms, launches)} of everything recorded since enabling."""
+        L = lib()
+        n = L.afx_profile_kernel_count()
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int64 * n)()
+        check(L.afx_profile_collect(ctypes.addressof(ms), ctypes.addressof(cnt)), "afx_profile_collect")
+        return {L.afx_profile_kernel_name(i).decode(): (ms[i], cnt[i]) for i in range(n) if cnt[i]}
+
+    def __exit__(self, *exc):
+        lib().afx_profile_enable(0)
+        return False

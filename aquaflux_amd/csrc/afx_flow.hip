@@ -13,6 +13,7 @@
 #include "afx_common.h"
 #include "afx_mesh.h"
 #include "afx_flow_face.h"
+#include "afx_profile.h"
 
 using namespace afx;
 
@@ -795,20 +796,20 @@ int run_forward(const afx_mesh* mh, const afx_geometry* g, FlowArgs<S> a, Out<S>
   MeshView m = afx_view(mh, g);
   Out<S> oG{G, Gt}, oP{psi, psit}, oA{aP, aPt};
   if (ADV == kAdvLimited)
-    hipLaunchKernelGGL((k_flow_grad<D, S, GRAD, true>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oG, oP, ext);
+    AFX_LAUNCH(kKFlowGrad, (k_flow_grad<D, S, GRAD, true>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oG, oP, ext);
   else
-    hipLaunchKernelGGL((k_flow_grad<D, S, GRAD, false>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oG, oP, ext);
+    AFX_LAUNCH(kKFlowGrad, (k_flow_grad<D, S, GRAD, false>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oG, oP, ext);
   AFX_CHECK_LAUNCH();
   a.G.v = G;
   a.G.t = Gt;
-  hipLaunchKernelGGL((k_flow_ap<D, S, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oA);
+  AFX_LAUNCH(kKFlowAp, (k_flow_ap<D, S, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oA);
   AFX_CHECK_LAUNCH();
   if (R.v == nullptr) return 0;
   a.aP.v = aP;
   a.aP.t = aPt;
   a.psi.v = psi;
   a.psi.t = psit;
-  hipLaunchKernelGGL((k_flow_res<D, S, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, R);
+  AFX_LAUNCH(kKFlowRes, (k_flow_res<D, S, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, R);
   AFX_CHECK_LAUNCH();
   return 0;
 }
@@ -823,14 +824,14 @@ int run_vjp(const afx_mesh* mh, const afx_geometry* g, FlowArgs<double> a, const
   a.G.v = w.G;
   a.aP.v = w.aP;
   a.psi.v = w.psi;
-  hipLaunchKernelGGL((k_flow_res_adj<D, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, lam, bar);
+  AFX_LAUNCH(kKFlowResAdj, (k_flow_res_adj<D, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, lam, bar);
   AFX_CHECK_LAUNCH();
-  hipLaunchKernelGGL((k_flow_ap_adj<D, ADV, GRAD>), grid_for(m.C), dim3(kBlock), 0, s, m, a, w.ext, bar);
+  AFX_LAUNCH(kKFlowApAdj, (k_flow_ap_adj<D, ADV, GRAD>), grid_for(m.C), dim3(kBlock), 0, s, m, a, w.ext, bar);
   AFX_CHECK_LAUNCH();
   if (ADV == kAdvLimited)
-    hipLaunchKernelGGL((k_flow_grad_adj<D, GRAD, true>), grid_for(m.C), dim3(kBlock), 0, s, m, a, w.ext, bar);
+    AFX_LAUNCH(kKFlowGradAdj, (k_flow_grad_adj<D, GRAD, true>), grid_for(m.C), dim3(kBlock), 0, s, m, a, w.ext, bar);
   else
-    hipLaunchKernelGGL((k_flow_grad_adj<D, GRAD, false>), grid_for(m.C), dim3(kBlock), 0, s, m, a, w.ext, bar);
+    AFX_LAUNCH(kKFlowGradAdj, (k_flow_grad_adj<D, GRAD, false>), grid_for(m.C), dim3(kBlock), 0, s, m, a, w.ext, bar);
   AFX_CHECK_LAUNCH();
   return 0;
 }

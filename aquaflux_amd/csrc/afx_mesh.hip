@@ -11,6 +11,7 @@
 // reduction is a fixed-order register sum (no atomics, bitwise reproducible).
 #include "afx_common.h"
 #include "afx_mesh.h"
+#include "afx_profile.h"
 
 #include <vector>
 #include <algorithm>
@@ -319,24 +320,24 @@ extern "C" int afx_geometry_compute(const afx_mesh* m, const double* node_coords
   double* cc = const_cast<double*>(g->cell_centroid);
   afx::MeshView v = afx_view(m, g);
   if (m->dim == 2) {
-    hipLaunchKernelGGL(k_face_geometry<2>, grid_for(m->F), dim3(kBlock), 0, s, m->F, m->N, m->foff,
+    AFX_LAUNCH(kKGeomFace, k_face_geometry<2>, grid_for(m->F), dim3(kBlock), 0, s, m->F, m->N, m->foff,
                        m->fidx, node_coords, area, m->normal_raw, fc);
     AFX_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_approx_centroid<2>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx);
+    AFX_LAUNCH(kKGeomApprox, k_approx_centroid<2>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx);
     AFX_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_orient<2>, grid_for(m->F), dim3(kBlock), 0, s, v, m->approx, m->normal_raw, nrm);
+    AFX_LAUNCH(kKGeomOrient, k_orient<2>, grid_for(m->F), dim3(kBlock), 0, s, v, m->approx, m->normal_raw, nrm);
     AFX_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_cell_volume<2>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx, vol, cc);
+    AFX_LAUNCH(kKGeomVolume, k_cell_volume<2>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx, vol, cc);
     AFX_CHECK_LAUNCH();
   } else {
-    hipLaunchKernelGGL(k_face_geometry<3>, grid_for(m->F), dim3(kBlock), 0, s, m->F, m->N, m->foff,
+    AFX_LAUNCH(kKGeomFace, k_face_geometry<3>, grid_for(m->F), dim3(kBlock), 0, s, m->F, m->N, m->foff,
                        m->fidx, node_coords, area, m->normal_raw, fc);
     AFX_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_approx_centroid<3>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx);
+    AFX_LAUNCH(kKGeomApprox, k_approx_centroid<3>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx);
     AFX_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_orient<3>, grid_for(m->F), dim3(kBlock), 0, s, v, m->approx, m->normal_raw, nrm);
+    AFX_LAUNCH(kKGeomOrient, k_orient<3>, grid_for(m->F), dim3(kBlock), 0, s, v, m->approx, m->normal_raw, nrm);
     AFX_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_cell_volume<3>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx, vol, cc);
+    AFX_LAUNCH(kKGeomVolume, k_cell_volume<3>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx, vol, cc);
     AFX_CHECK_LAUNCH();
   }
   return 0;

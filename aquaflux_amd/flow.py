@@ -103,6 +103,23 @@ def _as_vector(v, dim):
     return t
 
 
+def _boundary_data(boundary, d, F, device):
+    """Prescribed face velocity ``(d, F)`` and pressure ``(F,)`` folded patch by patch
+    (``boundary/collection.py:97-137``); tensor-valued closures stay differentiable."""
+    ub = torch.zeros((F, d), dtype=torch.float64, device=device)
+    pb = torch.zeros(F, dtype=torch.float64, device=device)
+    for _, bc, faces in boundary.items():
+        idx = torch.from_numpy(faces).to(device)
+        v = bc.prescribed_velocity(d)
+        if v is not None:
+            ub = ub.index_put((idx,), v.to(device)[None, :].expand(idx.numel(), d))
+        p = bc.prescribed_pressure()
+        if p is not None:
+            pv = p if isinstance(p, torch.Tensor) else torch.tensor(float(p), dtype=torch.float64)
+            pb = pb.index_put((idx,), pv.to(device).reshape(()).expand(idx.numel()))
+    return ub.T.contiguous(), pb.contiguous()
+
+
 # --- the residual operator ---------------------------------------------------------------------
 
 
@@ -334,20 +351,7 @@ class MomentumContinuity:
 
     def boundary_data(self):
         """Per-face prescribed velocity ``(d, F)`` and outlet pressure ``(F,)`` (differentiable)."""
-        d, F = self.dim, self.mesh.n_faces
-        ub = torch.zeros((d, F), dtype=torch.float64, device=self.device)
-        pb = torch.zeros(F, dtype=torch.float64, device=self.device)
-        for _, bc, faces in self.boundary.items():
-            idx = torch.from_numpy(faces).to(self.device)
-            v = bc.prescribed_velocity(d)
-            if v is not None:
-                v = v.to(self.device)
-                ub = ub.index_put((slice(None), idx), v[:, None].expand(d, idx.numel()))
-            p = bc.prescribed_pressure()
-            if p is not None:
-                pv = p if isinstance(p, torch.Tensor) else torch.tensor(float(p), dtype=torch.float64)
-                pb = pb.index_put((idx,), pv.to(self.device).reshape(()).expand(idx.numel()))
-        return ub.contiguous(), pb.contiguous()
+        return _boundary_data(self.boundary, self.dim, self.mesh.n_faces, self.device)
 
     def _inputs(self):
         mu, rho = self.material()

@@ -29,6 +29,17 @@ extern "C" {
 const char* afx_last_error(void);
 
 /* ------------------------------------------------------------------------------------------- */
+/* kernel timing (no reference counterpart: the reference has no profiler hook)                  */
+
+/* Enables per-launch event timing with a pool of max_launches event pairs (<= 0 disables). */
+int afx_profile_enable(int64_t max_launches);
+/* Per-kernel total milliseconds and launch counts since enabling (arrays of
+ * afx_profile_kernel_count() entries); waits for the recorded events and clears them. */
+int afx_profile_collect(double* total_ms, int64_t* count);
+int afx_profile_kernel_count(void);
+const char* afx_profile_kernel_name(int id);
+
+/* ------------------------------------------------------------------------------------------- */
 /* mesh                                                                                          */
 
 typedef struct afx_mesh afx_mesh;
