@@ -145,6 +145,25 @@ def cpu_baseline(mesh, state, lam, reps=2):
     }
 
 
+def calibrate_copy(mib: int, reps: int = 5) -> int:
+    """Copies ``mib`` MiB device to device ``reps`` times: FETCH_SIZE / WRITE_SIZE of these
+    dispatches should read ``mib`` MiB each, which calibrates the counters' units."""
+    import torch
+
+    n = mib * 1024 * 1024 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"copy_mib": mib, "reps": reps + 1, "copy_GBps": 2 * n * 8 / dt / 1e9}), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,7 +171,17 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=2)
+    ap.add_argument(
+        "--calibrate-copy",
+        type=int,
+        default=0,
+        metavar="MIB",
+        help="only run plain device-to-device copies of this many MiB (a known-traffic reference "
+        "for hardware byte counters) and exit",
+    )
     args = ap.parse_args(argv)
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
         print(
             f"bench.py measures one GPU only: this build's path runs on a single MI355X "
@@ -207,6 +236,13 @@ def main(argv=None):
         "avg_ms": round(avg_ms, 4),
         "algorithmic_bytes": int(nbytes),
     }
+    traffic_file = os.path.join(ROOT, "profiles", "flow_hex128x64x64_traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as fh:
+            tk = json.load(fh).get("kernels", {}).get(dom)
+        if tk:
+            roofline["traffic"] = tk["hbm_bytes_per_launch"]
+            roofline["traffic_source"] = "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated (profiles/)"
     kernels = {k: round(v[0] / v[1], 4) for k, v in sorted(kern.items())}
     out = {
         "metric": "Gfaces/s for residual+VJP",
