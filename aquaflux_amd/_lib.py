@@ -88,6 +88,15 @@ _SIGNATURES = {
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxFlowConfig), ctypes.POINTER(AfxFlowInputs),
         c_void_p, ctypes.POINTER(AfxFlowCotangents), c_void_p, ctypes.c_int64, c_void_p,
     ],
+    "afx_bell_slots": [c_void_p, c_void_p],
+    "afx_flow_jacobian": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxFlowConfig), ctypes.POINTER(AfxFlowInputs),
+        ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+    ],
+    "afx_bell_spmv_t": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "afx_bell_spmv": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "afx_pbj_setup": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    "afx_pbj_apply": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "afx_profile_enable": [ctypes.c_int64],
     "afx_profile_collect": [c_void_p, c_void_p],
     "afx_profile_kernel_count": [],

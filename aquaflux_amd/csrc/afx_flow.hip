@@ -129,7 +129,7 @@ template <int D, class S, int GRAD, bool LIM>
 __global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Out<S> G, Out<S> psi_out,
                                                    int32_t* __restrict__ ext) {
   constexpr int NV = D + 1;
-  int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t c = xcd_thread_index();
   if (c >= m.C) return;
   S xc[NV];
 #pragma unroll
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Ou
 // pass D: momentum diagonal a_P
 template <int D, class S, int ADV>
 __global__ void __launch_bounds__(256) k_flow_ap(MeshView m, FlowArgs<S> a, Out<S> aP) {
-  int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t c = xcd_thread_index();
   if (c >= m.C) return;
   FlowCell<D, S> X;
   load_state<D, S>(a, m.C, c, X);
@@ -340,7 +340,7 @@ __global__ void __launch_bounds__(256) k_flow_ap(MeshView m, FlowArgs<S> a, Out<
 template <int D, class S, int ADV>
 __global__ void __launch_bounds__(256) k_flow_res(MeshView m, FlowArgs<S> a, Out<S> R) {
   constexpr int NV = D + 1;
-  int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t c = xcd_thread_index();
   if (c >= m.C) return;
   FlowCell<D, S> X;
   load_cell_full<D, S>(a, m, c, true, ADV == kAdvLimited, X);
@@ -397,7 +397,7 @@ template <int D, int ADV>
 __global__ void __launch_bounds__(256) k_flow_res_adj(MeshView m, FlowArgs<double> a, const double* __restrict__ lam,
                                                       FlowBars bar) {
   constexpr int NV = D + 1;
-  int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t c = xcd_thread_index();
   if (c >= m.C) return;
   FlowCell<D, double> X;
   load_cell_full<D, double>(a, m, c, true, ADV == kAdvLimited, X);
@@ -477,7 +477,7 @@ template <int D, int ADV, int GRAD>
 __global__ void __launch_bounds__(256) k_flow_ap_adj(MeshView m, FlowArgs<double> a, const int32_t* __restrict__ ext,
                                                      FlowBars bar) {
   constexpr int NV = D + 1;
-  int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t c = xcd_thread_index();
   if (c >= m.C) return;
   FlowCell<D, double> X;
   load_state<D, double>(a, m.C, c, X);
@@ -620,7 +620,7 @@ template <int D, int GRAD, bool LIM>
 __global__ void __launch_bounds__(256) k_flow_grad_adj(MeshView m, FlowArgs<double> a, const int32_t* __restrict__ ext,
                                                        FlowBars bar) {
   constexpr int NV = D + 1;
-  int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t c = xcd_thread_index();
   if (c >= m.C) return;
   double xb[NV];
 #pragma unroll

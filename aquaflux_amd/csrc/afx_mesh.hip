@@ -145,7 +145,7 @@ __global__ void k_face_geometry(int32_t F, int32_t Nn, const int32_t* __restrict
                                 const int32_t* __restrict__ fidx, const double* __restrict__ X,
                                 double* __restrict__ area, double* __restrict__ nrm,
                                 double* __restrict__ fc) {
-  int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t f = xcd_thread_index();
   if (f >= F) return;
   int32_t b = foff[f], e = foff[f + 1];
   if (D == 2) {
@@ -220,7 +220,7 @@ __global__ void k_face_geometry(int32_t F, int32_t Nn, const int32_t* __restrict
 
 template <int D>
 __global__ void k_approx_centroid(MeshView m, double* __restrict__ approx) {
-  int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t c = xcd_thread_index();
   if (c >= m.C) return;
   double s[D];
 #pragma unroll
@@ -240,7 +240,7 @@ __global__ void k_approx_centroid(MeshView m, double* __restrict__ approx) {
 template <int D>
 __global__ void k_orient(MeshView m, const double* __restrict__ approx, const double* __restrict__ raw,
                          double* __restrict__ nrm) {
-  int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t f = xcd_thread_index();
   if (f >= m.F) return;
   int32_t P = m.owner[f];
   double proj = 0.0;
@@ -255,7 +255,7 @@ __global__ void k_orient(MeshView m, const double* __restrict__ approx, const do
 template <int D>
 __global__ void k_cell_volume(MeshView m, const double* __restrict__ approx, double* __restrict__ vol,
                               double* __restrict__ cc) {
-  int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t c = xcd_thread_index();
   if (c >= m.C) return;
   double ap[D], v = 0.0, cs[D];
 #pragma unroll

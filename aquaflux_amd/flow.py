@@ -373,6 +373,12 @@ class MomentumContinuity:
 
     __call__ = residual
 
+    def residual_raw(self, state):
+        """R(state) with no autograd bookkeeping (Newton iterations, line searches)."""
+        mu, rho, ub, pb, gb = self._inputs()
+        return self._raw.residual(state.contiguous(), mu.detach(), rho.detach(), ub.detach(), pb.detach(),
+                                  None if gb is None else gb.detach())
+
     def residual_jvp(self, state, tangent):
         """(R, J tangent) with respect to the state only (the Newton-Krylov matvec)."""
         mu, rho, ub, pb, gb = self._inputs()

@@ -1,0 +1,362 @@
+"""Newton-Krylov drivers on the GPU (``aquaflux/solve/newton.py``, ``linear.py``, ``implicit.py``).
+
+* :func:`gmres` -- restarted, right-preconditioned GMRES; ``flexible=True`` gives FGMRES (the
+  preconditioner may change between iterations, e.g. an inner Krylov solve).  Orthogonalisation is
+  classical Gram-Schmidt applied twice (CGS2), two matrix-vector products against the basis per
+  step, which keeps the basis orthogonal to working precision while running as BLAS-2 on the GPU.
+  Stops when ``||r|| <= atol + rtol ||b||`` (``solve/linear.py:199-298``).
+* :class:`DampedNewtonStep` -- Newton correction from GMRES on the exact FP64 tangent (the HIP JVP),
+  then a backtracking ladder ``alpha in {1, 1/2, ..., 2**-steps}`` accepting the first strict
+  descent (``solve/implicit.py:270-455``).  The Krylov solve is preconditioned by the compact block
+  Jacobian: point-block Jacobi, or an inner GMRES(30) on that Jacobian, in FP32 or FP64 -- the FP32
+  inner solve is an iterative refinement under the FP64 outer FGMRES.
+* :class:`ImplicitNewtonSolver` -- drives Newton to ``||R|| <= atol + rtol ||R0||``
+  (``solve/implicit.py:568-635``, ``forward_step.py:352-354``) and differentiates the root by the
+  implicit-function theorem: the backward pass solves ``J^T lambda = g`` with the HIP adjoint and
+  returns ``-(dR/dtheta)^T lambda`` for the parameter tensors (``solve/implicit.py:733-761``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+@dataclasses.dataclass
+class KrylovInfo:
+    iterations: int = 0
+    restarts: int = 0
+    residual_norm: float = math.inf
+    converged: bool = False
+
+
+def gmres(matvec, b, *, x0=None, rtol=1e-10, atol=0.0, restart=30, max_iter=1000, precond=None, flexible=False):
+    """Solve ``A x = b``; returns ``(x, KrylovInfo)``.  ``precond`` is applied on the right."""
+    dtype, dev = b.dtype, b.device
+    n = b.numel()
+    bnorm = float(torch.linalg.vector_norm(b))
+    tol = atol + rtol * bnorm
+    x = torch.zeros_like(b) if x0 is None else x0.clone()
+    info = KrylovInfo()
+    if bnorm == 0.0:
+        info.residual_norm, info.converged = 0.0, True
+        return x, info
+    m = int(restart)
+    V = torch.empty((m + 1, n), dtype=dtype, device=dev)
+    Z = torch.empty((m, n), dtype=dtype, device=dev) if (flexible and precond is not None) else None
+    while True:
+        r = b - matvec(x) if (x0 is not None or info.restarts > 0) else b.clone()
+        beta = float(torch.linalg.vector_norm(r))
+        info.residual_norm = beta
+        if beta <= tol:
+            info.converged = True
+            return x, info
+        if info.iterations >= max_iter:
+            return x, info
+        V[0] = r / beta
+        H = np.zeros((m + 1, m))
+        cs = np.zeros(m)
+        sn = np.zeros(m)
+        g = np.zeros(m + 1)
+        g[0] = beta
+        k = 0
+        for j in range(m):
+            z = V[j] if precond is None else precond(V[j])
+            if Z is not None:
+                Z[j] = z
+            w = matvec(z)
+            Vj = V[: j + 1]
+            h = Vj @ w
+            w = w - Vj.T @ h
+            h2 = Vj @ w
+            w = w - Vj.T @ h2
+            h = h + h2
+            hn = torch.linalg.vector_norm(w)
+            hv = torch.cat([h, hn.reshape(1)]).double().cpu().numpy()
+            H[: j + 2, j] = hv
+            if hv[j + 1] > 0:
+                V[j + 1] = w / hn
+            for i in range(j):  # apply previous rotations
+                t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
+                H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
+                H[i, j] = t
+            den = math.hypot(H[j, j], H[j + 1, j])
+            cs[j], sn[j] = (1.0, 0.0) if den == 0.0 else (H[j, j] / den, H[j + 1, j] / den)
+            H[j, j] = den
+            H[j + 1, j] = 0.0
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = cs[j] * g[j]
+            k = j + 1
+            info.iterations += 1
+            if abs(g[j + 1]) <= tol or info.iterations >= max_iter or hv[j + 1] == 0.0:
+                break
+        y = np.linalg.solve(np.triu(H[:k, :k]) + np.diag(np.where(np.diag(H[:k, :k]) == 0, 1.0, 0.0)), g[:k])
+        yt = torch.tensor(y, dtype=dtype, device=dev)
+        if Z is not None:
+            x = x + Z[:k].T @ yt
+        else:
+            upd = V[:k].T @ yt
+            x = x + (upd if precond is None else precond(upd))
+        info.restarts += 1
+        info.residual_norm = abs(g[k])
+
+
+class BlockJacobian:
+    """Compact block Jacobian of a flow residual at a state (block-ELL, FP32 or FP64)."""
+
+    def __init__(self, op, state, precision: str = "fp32"):
+        self.op = op
+        raw = op._raw
+        dm = raw.dm
+        self.dm = dm
+        self.C = op.mesh.n_cells
+        self.B = op.mesh.dim + 1
+        self.prec = 0 if precision == "fp32" else 1
+        self.dtype = torch.float32 if self.prec == 0 else torch.float64
+        ns = ctypes.c_int32()
+        _lib.check(_lib.lib().afx_bell_slots(dm.handle, ctypes.byref(ns)), "afx_bell_slots")
+        self.nslot = int(ns.value)
+        dev = dm.device
+        self.vals = torch.empty(self.nslot * self.B * self.B * self.C, dtype=self.dtype, device=dev)
+        self.cols = torch.empty(self.nslot * self.C, dtype=torch.int32, device=dev)
+        self.tslot = torch.empty(self.nslot * self.C, dtype=torch.int32, device=dev)
+        fields = torch.empty(((self.B * (self.B - 1)) + self.B) * self.C, dtype=torch.float64, device=dev)
+        mu, rho, ub, pb, gb = op._inputs()
+        ins = raw.inputs(state.contiguous(), mu.detach(), rho.detach(), ub.detach(), pb.detach(),
+                         None if gb is None else gb.detach())
+        st = _lib.stream_of(dev)
+        _lib.check(
+            _lib.lib().afx_flow_jacobian(dm.handle, ctypes.byref(raw.gs), ctypes.byref(raw.cfg), ctypes.byref(ins),
+                                         self.prec, self.vals.data_ptr(), self.cols.data_ptr(),
+                                         self.tslot.data_ptr(), fields.data_ptr(), st),
+            "afx_flow_jacobian",
+        )
+        self.inv = torch.empty(self.B * self.B * self.C, dtype=self.dtype, device=dev)
+        flag = torch.zeros(1, dtype=torch.int32, device=dev)
+        _lib.check(
+            _lib.lib().afx_pbj_setup(self.C, self.B, self.prec, self.vals.data_ptr(), self.inv.data_ptr(),
+                                     flag.data_ptr(), st),
+            "afx_pbj_setup",
+        )
+        self.singular = bool(flag.item())
+
+    # vectors: field-major FP64 (residual layout) <-> cell-major in the Jacobian's precision
+    def to_cell_major(self, v):
+        return v.reshape(self.B, self.C).T.contiguous().reshape(-1).to(self.dtype)
+
+    def from_cell_major(self, v):
+        return v.reshape(self.C, self.B).T.contiguous().reshape(-1).to(torch.float64)
+
+    def matvec(self, x):
+        y = torch.empty_like(x)
+        _lib.check(
+            _lib.lib().afx_bell_spmv(self.C, self.B, self.nslot, self.prec, self.vals.data_ptr(), self.cols.data_ptr(),
+                                     x.data_ptr(), y.data_ptr(), _lib.stream_of(x.device)),
+            "afx_bell_spmv",
+        )
+        return y
+
+    def rmatvec(self, x):
+        """``y = A^T x``."""
+        y = torch.empty_like(x)
+        _lib.check(
+            _lib.lib().afx_bell_spmv_t(self.C, self.B, self.nslot, self.prec, self.vals.data_ptr(),
+                                       self.cols.data_ptr(), self.tslot.data_ptr(), x.data_ptr(), y.data_ptr(),
+                                       _lib.stream_of(x.device)),
+            "afx_bell_spmv_t",
+        )
+        return y
+
+    def pbj(self, x):
+        y = torch.empty_like(x)
+        _lib.check(
+            _lib.lib().afx_pbj_apply(self.C, self.B, self.prec, self.inv.data_ptr(), x.data_ptr(), y.data_ptr(),
+                                     _lib.stream_of(x.device)),
+            "afx_pbj_apply",
+        )
+        return y
+
+    def pbj_transpose(self, x):
+        """``y = D^-T x`` (the adjoint solve's preconditioner)."""
+        inv = self.inv.reshape(self.B, self.B, self.C)  # inv[r, k, c]
+        return torch.einsum("rkc,cr->ck", inv, x.reshape(self.C, self.B)).reshape(-1).contiguous()
+
+
+@dataclasses.dataclass
+class DampedNewtonStep:
+    """Newton correction + backtracking line search (``solve/implicit.py:458-565``).
+
+    The linear solve stops at a global relative residual ``||J delta + R|| <= rtol ||R||`` -- the
+    reference's robust forward stop (``relative_residual_gmres``, ``solve/linear.py:139-200``), not
+    lineax's componentwise test whose absolute floor would freeze Newton once ``||R||`` falls below
+    it.
+
+    ``preconditioner``: ``None`` (plain GMRES on the JVP), ``"pbj"`` (point-block Jacobi of the
+    compact Jacobian) or ``"inner"`` (FGMRES whose preconditioner is GMRES(``inner_restart``) on the
+    compact Jacobian with point-block Jacobi); ``precision`` is the compact Jacobian's arithmetic.
+    """
+
+    preconditioner: str | None = "inner"
+    precision: str = "fp32"
+    line_search: int = 10
+    rtol: float = 1e-3
+    atol: float = 0.0
+    restart: int = 30
+    max_iter: int = 300
+    inner_restart: int = 30
+    inner_iters: int = 30
+    inner_rtol: float = 1e-2
+
+    def correction(self, op, phi, r):
+        J = None
+        precond = None
+        flexible = False
+        if self.preconditioner is not None:
+            J = BlockJacobian(op, phi, self.precision)
+            if self.preconditioner == "pbj":
+                precond = lambda v: J.from_cell_major(J.pbj(J.to_cell_major(v)))  # noqa: E731
+            elif self.preconditioner == "inner":
+                flexible = True
+
+                def precond(v):
+                    vc = J.to_cell_major(v)
+                    z, _ = gmres(J.matvec, vc, rtol=self.inner_rtol, restart=self.inner_restart,
+                                 max_iter=self.inner_iters, precond=J.pbj)
+                    return J.from_cell_major(z)
+            else:
+                raise ValueError(f"unknown preconditioner {self.preconditioner!r}")
+
+        def matvec(v):
+            return op.residual_jvp(phi, v)[1]
+
+        delta, info = gmres(matvec, -r, rtol=self.rtol, atol=self.atol, restart=self.restart, max_iter=self.max_iter,
+                            precond=precond, flexible=flexible)
+        return delta, info
+
+
+def backtracking_line_search(residual, phi, delta, reference_norm, steps, norm=None):
+    """First (largest) ``alpha`` in ``{1, 1/2, ..., 2**-steps}`` with ``||R(phi + alpha delta)||`` below
+    ``reference_norm``; else the smallest rung with a finite residual (``solve/implicit.py:270-455``)."""
+    norm = norm or (lambda v: float(torch.linalg.vector_norm(v)))
+    fallback = None
+    for i in range(steps + 1):
+        alpha = 0.5**i
+        cand = phi + alpha * delta
+        r = residual(cand)
+        rn = norm(r)
+        if math.isfinite(rn):
+            fallback = (cand, r, rn, alpha)
+            if rn < reference_norm:
+                return cand, r, rn, alpha
+    if fallback is None:
+        raise FloatingPointError("line search: every rung produced a non-finite residual")
+    return fallback
+
+
+@dataclasses.dataclass
+class NewtonReport:
+    steps: int = 0
+    residual_norms: list = dataclasses.field(default_factory=list)
+    krylov_iterations: list = dataclasses.field(default_factory=list)
+    alphas: list = dataclasses.field(default_factory=list)
+    seconds: float = 0.0
+    converged: bool = False
+
+
+@dataclasses.dataclass
+class ImplicitNewtonSolver:
+    """Newton to ``||R|| <= atol + rtol ||R0||`` with an IFT adjoint (``solve/implicit.py:767-864``)."""
+
+    rtol: float = 1e-10
+    atol: float = 1e-12
+    max_steps: int = 50
+    forward_step: DampedNewtonStep = dataclasses.field(default_factory=DampedNewtonStep)
+    adjoint_rtol: float = 1e-10
+    adjoint_restart: int = 50
+    adjoint_max_iter: int = 2000
+
+    def forward(self, op, phi0):
+        """Runs Newton from ``phi0`` (no autograd); returns ``(phi, NewtonReport)``."""
+        rep = NewtonReport()
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            phi = phi0.detach().clone().contiguous()
+            r = op.residual_raw(phi)
+            r0 = float(torch.linalg.vector_norm(r))
+            rn = r0
+            rep.residual_norms.append(rn)
+            target = self.atol + self.rtol * r0
+            while rn > target:
+                if rep.steps >= self.max_steps:
+                    torch.cuda.synchronize()
+                    rep.seconds = time.perf_counter() - t0
+                    raise RuntimeError(
+                        f"Newton did not converge in {self.max_steps} steps: ||R|| = {rn:.3e} > {target:.3e}"
+                    )
+                delta, kinfo = self.forward_step.correction(op, phi, r)
+                phi, r, rn, alpha = backtracking_line_search(op.residual_raw, phi, delta, rn, self.forward_step.line_search)
+                if not math.isfinite(rn):
+                    raise FloatingPointError("non-finite residual in Newton")
+                rep.steps += 1
+                rep.residual_norms.append(rn)
+                rep.krylov_iterations.append(kinfo.iterations)
+                rep.alphas.append(alpha)
+            torch.cuda.synchronize()
+        rep.seconds = time.perf_counter() - t0
+        rep.converged = True
+        return phi, rep
+
+    def solve(self, op, phi0, params=()):
+        """Root ``phi*`` of ``op``; differentiable w.r.t. ``params`` (tensors ``op`` depends on)."""
+        phi, rep = self.forward(op, phi0)
+        self.last_report = rep
+        params = tuple(params)
+        if not any(isinstance(p, torch.Tensor) and p.requires_grad for p in params):
+            return phi
+        return _ImplicitRoot.apply(self, op, phi, *params)
+
+    def adjoint(self, op, phi, cotangent):
+        """``lambda`` solving ``J(phi)^T lambda = cotangent``: FGMRES on the HIP adjoint, preconditioned
+        by GMRES on the transposed compact Jacobian with transposed point-block Jacobi (the forward
+        preconditioner, transposed, as ``solve/implicit.py:709-730`` does)."""
+        J = BlockJacobian(op, phi, "fp64")
+        step = self.forward_step
+
+        def precond(v):
+            vc = J.to_cell_major(v)
+            z, _ = gmres(J.rmatvec, vc, rtol=step.inner_rtol, restart=step.inner_restart,
+                         max_iter=step.inner_iters, precond=J.pbj_transpose)
+            return J.from_cell_major(z)
+
+        lam, info = gmres(lambda w: op.residual_vjp(phi, w), cotangent, rtol=self.adjoint_rtol,
+                          restart=self.adjoint_restart, max_iter=self.adjoint_max_iter, precond=precond, flexible=True)
+        if not info.converged:
+            raise RuntimeError(f"adjoint GMRES did not converge: ||r|| = {info.residual_norm:.3e}")
+        return lam
+
+
+class _ImplicitRoot(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, solver, op, phi, *params):
+        ctx.solver, ctx.op = solver, op
+        ctx.save_for_backward(phi, *params)
+        return phi.clone()
+
+    @staticmethod
+    def backward(ctx, g):
+        phi, *params = ctx.saved_tensors
+        lam = ctx.solver.adjoint(ctx.op, phi, g.contiguous())
+        with torch.enable_grad():
+            leaves = [p for p in params if p.requires_grad]
+            R = ctx.op.residual(phi.detach())
+            grads = torch.autograd.grad(R, leaves, -lam, allow_unused=True)
+        it = iter(grads)
+        out = [next(it) if p.requires_grad else None for p in params]
+        return (None, None, None, *out)

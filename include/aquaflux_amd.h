@@ -136,6 +136,39 @@ int afx_flow_vjp(const afx_mesh* mesh, const afx_geometry* geom, const afx_flow_
 int afx_flow_fields(const afx_mesh* mesh, const afx_geometry* geom, const afx_flow_config* cfg,
                     const afx_flow_inputs* in, double* gradients, double* a_p, double* psi, hipStream_t stream);
 
+/* ------------------------------------------------------------------------------------------- */
+/* Newton-Krylov inner operators: compact block Jacobian, SpMV, point-block Jacobi              */
+
+/* Block-ELL slot count per row (1 + max faces per cell). */
+int afx_bell_slots(const afx_mesh* mesh, int32_t* nslot);
+
+/* Reach-1 block Jacobian of the flow residual at `in` (gradients, a_P and limiter frozen), stored
+ * block-ELL cell-major: vals[(slot*B*B + r*B + k)*C + c] (float if precision == 0, double if 1),
+ * cols[slot*C + c]; B = d + 1.  `tslot` (nullable, [nslot*C]) receives, per slot, the slot of the
+ * same face in the column cell's row (for afx_bell_spmv_t).  `fields` is scratch of
+ * ((d+1)d + 1 + d) * C doubles.
+ * Replaces materialize_block_jacobian's coloured-JVP probing (reference:
+ * aquaflux/solve/sparse_jacobian.py:722-927) with direct dual-number assembly. */
+int afx_flow_jacobian(const afx_mesh* mesh, const afx_geometry* geom, const afx_flow_config* cfg,
+                      const afx_flow_inputs* in, int precision, void* vals, int32_t* cols, int32_t* tslot,
+                      double* fields, hipStream_t stream);
+
+/* y = A x for a block-ELL matrix with cell-major vectors x[c*B + v] (B = 3 or 4). */
+int afx_bell_spmv(int64_t n_rows, int32_t block, int32_t nslot, int precision, const void* vals,
+                  const int32_t* cols, const void* x, void* y, hipStream_t stream);
+
+/* y = A^T x using the transpose slot map of afx_flow_jacobian. */
+int afx_bell_spmv_t(int64_t n_rows, int32_t block, int32_t nslot, int precision, const void* vals,
+                    const int32_t* cols, const int32_t* tslot, const void* x, void* y, hipStream_t stream);
+
+/* Inverses of the diagonal blocks (slot 0) into inv[(r*B + k)*C + c]; sets *singular = 1 (device
+ * int) when a block is singular (that block is replaced by the identity). */
+int afx_pbj_setup(int64_t n_rows, int32_t block, int precision, const void* vals, void* inv, int32_t* singular,
+                  hipStream_t stream);
+/* y = D^-1 x. */
+int afx_pbj_apply(int64_t n_rows, int32_t block, int precision, const void* inv, const void* x, void* y,
+                  hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -399,8 +399,11 @@ def advective_momentum_flux(m, g, vel, rho, fac, grad_vel):
     return dot(mf, g.normal)
 
 
-def flow_fields(m, g, setup: FlowSetup, state, mu, rho, nu_t=None, wall_nu_t=None, shear_mask=None):
-    """``MomentumContinuity.flow_fields`` (``flow/momentum.py:822-857``) -- returns a dict."""
+def flow_fields(m, g, setup: FlowSetup, state, mu, rho, nu_t=None, wall_nu_t=None, shear_mask=None, frozen=None):
+    """``MomentumContinuity.flow_fields`` (``flow/momentum.py:822-857``) -- returns a dict.
+
+    ``frozen`` (a dict from an earlier call) substitutes its gradients and ``a_p``: the reach-1
+    linearisation the block Jacobian of the Newton preconditioner is built from."""
     d, C = m.dim, m.n_cells
     blocks = state.reshape(d + 1, C)
     vel, p = blocks[:d].T, blocks[d]
@@ -408,6 +411,8 @@ def flow_fields(m, g, setup: FlowSetup, state, mu, rho, nu_t=None, wall_nu_t=Non
     ub, pb = flow_boundary_fields(m, g, setup, vel, p)
     grad_vel = torch.stack([_gradient(m, g, setup, vel[:, i], ub[:, i]) for i in range(d)], dim=1)
     grad_p = _gradient(m, g, setup, p, pb)
+    if frozen is not None:
+        grad_vel, grad_p = frozen["grad_vel"].detach(), frozen["grad_p"].detach()
     fac = interpolation_factor(m, g)
     dist = normal_distance(m, g)
     own = m.owner
@@ -436,6 +441,8 @@ def flow_fields(m, g, setup: FlowSetup, state, mu, rho, nu_t=None, wall_nu_t=Non
         nc = nc + torch.maximum(-mdot_est, torch.zeros_like(mdot_est))
     oc = torch.where(m.interior, oc, bcoef)
     a_p = scatter(m, oc, nc)
+    if frozen is not None:
+        a_p = frozen["a_p"].detach()
     d_coeff = (g.volume / a_p)[:, None].expand(C, d)
     # _mass_flux (momentum.py:531-554) + interior_mass_flux (rhie_chow.py:274-353)
     mom_n = advective_momentum_flux(m, g, vel, rho, fac, grad_vel)
@@ -493,12 +500,22 @@ def flow_residual_from_fields(m, g, setup: FlowSetup, f, psi_vel=None):
     return torch.cat(cols + [cont])
 
 
-def flow_residual(m, g, setup: FlowSetup, state, mu, rho):
-    """``MomentumContinuity.residual`` (``flow/momentum.py:880-882``)."""
-    f = flow_fields(m, g, setup, state, mu, rho)
+def flow_residual(m, g, setup: FlowSetup, state, mu, rho, frozen_state=None):
+    """``MomentumContinuity.residual`` (``flow/momentum.py:880-882``).
+
+    With ``frozen_state`` the gradients, ``a_p`` and limiter are those of ``frozen_state`` (held
+    constant): differentiating that at ``state == frozen_state`` gives the reach-1 block Jacobian."""
+    frozen = None
+    if frozen_state is not None:
+        with torch.no_grad():
+            frozen = flow_fields(m, g, setup, frozen_state, mu, rho)
+    f = flow_fields(m, g, setup, state, mu, rho, frozen=frozen)
     psi = None
     if setup.advection == "limited":
-        psi = [venkatakrishnan(m, g, f["vel"][:, i], f["grad_vel"][:, i], setup.limiter_k) for i in range(m.dim)]
+        src = frozen if frozen is not None else f
+        psi = [venkatakrishnan(m, g, src["vel"][:, i], src["grad_vel"][:, i], setup.limiter_k) for i in range(m.dim)]
+        if frozen is not None:
+            psi = [p.detach() for p in psi]
     return flow_residual_from_fields(m, g, setup, f, psi)
 
 
