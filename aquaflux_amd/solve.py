@@ -183,6 +183,21 @@ class BlockJacobian:
         )
         return y
 
+    def add_diagonal(self, shift):
+        """Adds a per-row shift (field-major FP64 vector) to the diagonal blocks and refreshes the
+        point-block Jacobi inverses."""
+        B, C = self.B, self.C
+        sc = shift.reshape(B, C).to(self.dtype)
+        diag = self.vals[: B * B * C].view(B, B, C)
+        for r in range(B):
+            diag[r, r] += sc[r]
+        flag = torch.zeros(1, dtype=torch.int32, device=self.vals.device)
+        _lib.check(
+            _lib.lib().afx_pbj_setup(C, B, self.prec, self.vals.data_ptr(), self.inv.data_ptr(), flag.data_ptr(),
+                                     _lib.stream_of(self.vals.device)),
+            "afx_pbj_setup",
+        )
+
     def pbj_transpose(self, x):
         """``y = D^-T x`` (the adjoint solve's preconditioner)."""
         inv = self.inv.reshape(self.B, self.B, self.C)  # inv[r, k, c]
@@ -213,13 +228,28 @@ class DampedNewtonStep:
     inner_restart: int = 30
     inner_iters: int = 30
     inner_rtol: float = 1e-2
+    # pseudo-transient continuation (switched evolution relaxation): None = plain Newton
+    ptc_cfl0: float | None = None
+    ptc_cfl_max: float = 1e12
 
-    def correction(self, op, phi, r):
+    def pseudo_time_shift(self, op, phi, cfl):
+        """Diagonal ``a_P / CFL`` on the momentum rows (zero on continuity): a local pseudo-time step
+        scaled by the momentum diagonal, as the reference's pseudo-transient shift basis is."""
+        _, aP, _ = op.fields(phi)
+        d = op.mesh.dim
+        shift = torch.zeros_like(phi).reshape(d + 1, -1)
+        shift[:d] = (aP / cfl)[None, :]
+        return shift.reshape(-1)
+
+    def correction(self, op, phi, r, cfl=None):
         J = None
         precond = None
         flexible = False
+        shift = None if cfl is None or cfl >= self.ptc_cfl_max else self.pseudo_time_shift(op, phi, cfl)
         if self.preconditioner is not None:
             J = BlockJacobian(op, phi, self.precision)
+            if shift is not None:
+                J.add_diagonal(shift)
             if self.preconditioner == "pbj":
                 precond = lambda v: J.from_cell_major(J.pbj(J.to_cell_major(v)))  # noqa: E731
             elif self.preconditioner == "inner":
@@ -234,7 +264,8 @@ class DampedNewtonStep:
                 raise ValueError(f"unknown preconditioner {self.preconditioner!r}")
 
         def matvec(v):
-            return op.residual_jvp(phi, v)[1]
+            jv = op.residual_jvp(phi, v)[1]
+            return jv if shift is None else jv + shift * v
 
         delta, info = gmres(matvec, -r, rtol=self.rtol, atol=self.atol, restart=self.restart, max_iter=self.max_iter,
                             precond=precond, flexible=flexible)
@@ -281,6 +312,7 @@ class ImplicitNewtonSolver:
     adjoint_rtol: float = 1e-10
     adjoint_restart: int = 50
     adjoint_max_iter: int = 2000
+    progress: object = None  # optional callable(step, ||R||, krylov_iterations, alpha, seconds)
 
     def forward(self, op, phi0):
         """Runs Newton from ``phi0`` (no autograd); returns ``(phi, NewtonReport)``."""
@@ -300,14 +332,23 @@ class ImplicitNewtonSolver:
                     raise RuntimeError(
                         f"Newton did not converge in {self.max_steps} steps: ||R|| = {rn:.3e} > {target:.3e}"
                     )
-                delta, kinfo = self.forward_step.correction(op, phi, r)
-                phi, r, rn, alpha = backtracking_line_search(op.residual_raw, phi, delta, rn, self.forward_step.line_search)
+                fs = self.forward_step
+                if fs.ptc_cfl0 is not None:
+                    # switched evolution relaxation: CFL grows as the residual falls
+                    cfl = min(fs.ptc_cfl_max, fs.ptc_cfl0 * r0 / rn)
+                    delta, kinfo = fs.correction(op, phi, r, cfl)
+                    phi, r, rn, alpha = backtracking_line_search(op.residual_raw, phi, delta, 4.0 * rn, fs.line_search)
+                else:
+                    delta, kinfo = fs.correction(op, phi, r)
+                    phi, r, rn, alpha = backtracking_line_search(op.residual_raw, phi, delta, rn, fs.line_search)
                 if not math.isfinite(rn):
                     raise FloatingPointError("non-finite residual in Newton")
                 rep.steps += 1
                 rep.residual_norms.append(rn)
                 rep.krylov_iterations.append(kinfo.iterations)
                 rep.alphas.append(alpha)
+                if self.progress is not None:
+                    self.progress(rep.steps, rn / r0, kinfo.iterations, alpha, time.perf_counter() - t0)
             torch.cuda.synchronize()
         rep.seconds = time.perf_counter() - t0
         rep.converged = True

@@ -145,6 +145,61 @@ def cpu_baseline(mesh, state, lam, reps=2):
     }
 
 
+def newton_compare(device, nx=200, ny=100, nz=100, mu=1e-3, max_steps=80, cfl0=10.0):
+    """Newton to ||R|| <= 1e-12 + 1e-10 ||R0|| on the Re = 1000 hex box (u, v, w, p), from the seeded
+    state, with the inner Krylov solve on the compact block Jacobian in FP32 and in FP64."""
+    import torch
+
+    from aquaflux_amd.boundary import BoundaryConditions
+    from aquaflux_amd.discretization import FirstOrderUpwind
+    from aquaflux_amd.flow import MomentumContinuity, NoSlipWall, PressureOutlet, VelocityInlet
+    from aquaflux_amd.mesh import structured_grid_3d
+    from aquaflux_amd.properties import Constant, PropertyModel
+    from aquaflux_amd.schemes import LeastSquaresGradient
+    from aquaflux_amd.solve import DampedNewtonStep, ImplicitNewtonSolver
+
+    mesh = structured_grid_3d(nx, ny, nz, named_boundaries=True)
+    geom = mesh.geometry(device=device)
+    bcs = BoundaryConditions({"left": VelocityInlet((1.0, 0.0, 0.0)), "right": PressureOutlet(0.0),
+                              "bottom": NoSlipWall(), "top": NoSlipWall(), "back": NoSlipWall(), "front": NoSlipWall()})
+    mc = MomentumContinuity.build(mesh, geom, PropertyModel({"viscosity": Constant(mu), "density": Constant(1.0)}),
+                                  LeastSquaresGradient(), bcs, advection_scheme=FirstOrderUpwind())
+    rng = np.random.default_rng(2)
+    C = mesh.n_cells
+    x0 = torch.tensor(np.concatenate([1.0 + 0.05 * rng.normal(size=C), 0.05 * rng.normal(size=C),
+                                      0.05 * rng.normal(size=C), rng.normal(0.0, 0.1, C)]), dtype=torch.float64,
+                      device=device)
+    t_setup = time.perf_counter()
+    for prec in ("fp32", "fp64"):  # warm up every kernel and allocation outside the timed solves
+        DampedNewtonStep(preconditioner="inner", precision=prec, max_iter=2, inner_iters=2).correction(
+            mc, x0, mc.residual_raw(x0))
+    torch.cuda.synchronize()
+    print(f"newton: setup + warmup {time.perf_counter() - t_setup:.2f} s", file=sys.stderr, flush=True)
+    out = {"workload": f"hex box {nx}x{ny}x{nz} ({C} cells, {4 * C} unknowns), Re={1.0 / mu:g}, first-order upwind, "
+                       "least-squares gradients, seeded state; pseudo-transient continuation (SER, CFL0="
+                       f"{cfl0:g}) into Newton; stop at ||R|| <= 1e-12 + 1e-10 ||R0||"}
+    roots = {}
+    for prec in ("fp32", "fp64"):
+        def progress(step, rel, kits, alpha, secs, prec=prec):
+            print(f"newton[{prec}] step {step}: |R|/|R0| = {rel:.3e}, krylov {kits}, alpha {alpha:g}, "
+                  f"{secs:.2f} s", file=sys.stderr, flush=True)
+
+        solver = ImplicitNewtonSolver(max_steps=max_steps, progress=progress,
+                                      forward_step=DampedNewtonStep(preconditioner="inner", precision=prec,
+                                                                    ptc_cfl0=cfl0))
+        torch.cuda.synchronize()
+        phi, rep = solver.forward(mc, x0)
+        roots[prec] = phi
+        out[prec] = {"seconds": round(rep.seconds, 3), "newton_steps": rep.steps,
+                     "krylov_iterations": rep.krylov_iterations, "final_relative_residual":
+                     rep.residual_norms[-1] / rep.residual_norms[0]}
+        print(json.dumps({"newton": prec, **out[prec]}), file=sys.stderr, flush=True)
+    d = float((roots["fp32"] - roots["fp64"]).abs().max() / roots["fp64"].abs().max())
+    out["state_difference_fp32_vs_fp64"] = d
+    out["speedup_fp32_inner"] = round(out["fp64"]["seconds"] / out["fp32"]["seconds"], 3)
+    return out
+
+
 def calibrate_copy(mib: int, reps: int = 5) -> int:
     """Copies ``mib`` MiB device to device ``reps`` times: FETCH_SIZE / WRITE_SIZE of these
     dispatches should read ``mib`` MiB each, which calibrates the counters' units."""
@@ -171,6 +226,10 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=2)
+    ap.add_argument("--no-newton", action="store_true", help="skip the Newton time-to-1e-10 comparison")
+    ap.add_argument("--newton-only", action="store_true", help="only the Newton comparison")
+    ap.add_argument("--newton-size", type=int, nargs=3, default=(200, 100, 100), metavar=("NX", "NY", "NZ"))
+    ap.add_argument("--newton-cfl0", type=float, default=10.0)
     ap.add_argument(
         "--calibrate-copy",
         type=int,
@@ -195,6 +254,9 @@ def main(argv=None):
     from aquaflux_amd import _lib
 
     dev = torch.device("cuda", 0)
+    if args.newton_only:
+        print(json.dumps(newton_compare(dev, *args.newton_size, cfl0=args.newton_cfl0)), flush=True)
+        return 0
     mesh, geom, mc, x, lam, state_np, lam_np = build_workload(dev)
     raw = mc._raw
     mu, rho, ub, pb, _ = mc._inputs()
@@ -269,6 +331,8 @@ def main(argv=None):
         "roofline": roofline,
         "cpu_baseline": None,
     }
+    if not args.no_newton:
+        out["newton"] = newton_compare(dev)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mesh, state_np, lam_np, reps=args.cpu_reps)
     print(json.dumps(out), flush=True)
