@@ -1,0 +1,120 @@
+"""Pins the SST part of the oracle against the reference's closed-form unit tests (CPU only)."""
+
+import math
+
+import numpy as np
+import torch
+
+from oracle import fv
+from oracle import sst as S
+
+F64 = torch.float64
+M = S.SSTModel()
+
+
+def c(*v):
+    return torch.tensor([float(x) for x in v], dtype=F64)
+
+
+def test_blend_f2_f1_eddy_viscosity():
+    """tests/unit/test_sst.py:27-94."""
+    assert torch.allclose(M.blend(c(1.0), 0.85, 1.0), c(0.85))
+    assert torch.allclose(M.blend(c(0.5), 0.85, 1.0), c(0.925))
+    assert torch.allclose(M.f2(c(0.81), c(100.0), c(0.0), c(1.0)), c(math.tanh(0.2**2)))
+    assert torch.allclose(M.f2(c(0.81), c(100.0), c(0.06), c(1.0)), c(math.tanh(0.3**2)))
+    z = torch.zeros((1, 2), dtype=F64)
+    assert torch.allclose(M.f1(c(0.81), c(100.0), c(0.0), c(1.0), z, z), c(math.tanh(0.1**4)))
+    gr = torch.tensor([[100.0, 0.0]], dtype=F64)
+    assert float(M.f1(c(0.81), c(100.0), c(0.0), c(1.0), gr, gr)) < float(M.f1(c(0.81), c(100.0), c(0.0), c(1.0), z, z))
+    assert torch.allclose(M.eddy_viscosity(c(2.0), c(4.0), c(0.0), c(1e-3), c(1.0)), c(0.5))
+    k, om, s, nu, d = c(1.0), c(1.0), c(100.0), c(0.0), c(1.0)
+    assert torch.allclose(M.eddy_viscosity(k, om, s, nu, d), M.a1 * k / (s * M.f2(k, om, nu, d)))
+
+
+def test_wall_closures():
+    """tests/unit/test_turbulence_boundary.py:37-300."""
+    nu, d = 1e-5, torch.logspace(-5.0, -3.0, 7, dtype=F64)
+    om = S.omega_wall_value(torch.full_like(d, nu), d, M)
+    a = om * d**2
+    res = nu * 6.0 * a / d**4 - M.beta_1 * om**2
+    assert float(res.abs().max()) < 1e-8 * float((M.beta_1 * om**2).max())
+    y = M.wall_y_star_lam
+    assert abs(y - math.log(M.e_wall * y) / M.kappa) < 1e-6 and 10.0 < y < 13.0
+    d1, k1 = c(1e-5), c(0.01)
+    assert torch.equal(S.nut_wall(torch.full_like(d1, nu), d1, k1, M), torch.zeros(1, dtype=F64))
+    d2, k2 = c(3e-3), c(0.3)
+    ys = M.beta_star**0.25 * torch.sqrt(k2) * d2 / nu
+    assert float(ys) > y
+    assert torch.allclose(S.nut_wall(torch.full_like(d2, nu), d2, k2, M), nu * (ys * M.kappa / torch.log(M.e_wall * ys) - 1.0))
+    u_tau = M.beta_star**0.25 * torch.sqrt(k2)
+    assert torch.allclose(S.log_layer_shear_rate(d2, k2, M), u_tau / (M.kappa * d2))
+    shear = c(900.0)
+    u_log = M.kappa * shear * d2 / torch.log(M.e_wall * ys)
+    tau = (nu + S.nut_wall(torch.full_like(d2, nu), d2, k2, M)) * shear
+    assert torch.allclose(tau, u_tau * u_log)
+    # omega_wall is the quadrature blend of the two branches (p = 2): sqrt(vis^2 + log^2)
+    nu_t, d_t, k_t = c(1e-5), c(1e-3), c(0.05)
+    vis = S.omega_wall_value(nu_t, d_t, M)
+    log = torch.sqrt(k_t) / (M.beta_star**0.25 * M.kappa * d_t)
+    assert torch.allclose(S.omega_wall(nu_t, d_t, k_t, M), torch.sqrt(vis**2 + log**2))
+
+
+def test_strain_rate():
+    """tests/unit/test_strain.py:12-30."""
+    assert torch.allclose(S.strain_rate_magnitude(torch.tensor([[[0.0, 2.0], [0.0, 0.0]]], dtype=F64)), c(2.0))
+    assert torch.allclose(S.strain_rate_magnitude(torch.tensor([[[3.0, 0.0], [0.0, 0.0]]], dtype=F64)), c(math.sqrt(2.0) * 3.0))
+    assert torch.allclose(S.strain_rate_magnitude(torch.zeros((4, 3, 3), dtype=F64)), torch.zeros(4, dtype=F64))
+
+
+def test_omega_wall_gradient_is_the_analytical_chain_rule():
+    """tests/unit/test_turbulence_boundary.py:390-414 (p = 2: omega = sqrt(vis^2 + log^2))."""
+    nu, d, k = c(1e-5), c(2e-3), c(0.04)
+    dg, kg = torch.tensor([[0.0, 1.0]], dtype=F64), torch.tensor([[0.5, -2.0]], dtype=F64)
+    got = S.omega_wall_gradient(nu, d, k, dg, kg, M)
+    vis = S.omega_wall_value(nu, d, M)
+    log = torch.sqrt(k) / (M.beta_star**0.25 * M.kappa * d)
+    om = torch.sqrt(vis**2 + log**2)
+    dod = -(2 * vis**2 + log**2) / (d * om)
+    dok = log**2 / (2 * k * om)
+    assert torch.allclose(got, dod[:, None] * dg + dok[:, None] * kg, rtol=1e-12)
+
+
+def cavity_rans(n=6, gradient="green_gauss"):
+    """The SST cavity of tests/unit/test_coupled_rans.py:78-118 (nu = 1e-2, FOU, walls k = 0)."""
+    from aquaflux_amd.mesh import structured_grid_2d
+
+    mesh = structured_grid_2d(n, n, named_boundaries=True)
+    m = fv.OMesh.from_mesh(mesh)
+    g = fv.geometry(m)
+    F = mesh.n_faces
+    kind = torch.full((F,), fv.FLOW_NONE, dtype=torch.long)
+    kind[~m.interior] = fv.FLOW_WALL
+    ub = torch.zeros((F, 2), dtype=F64)
+    ub[torch.from_numpy(mesh.face_patches.indices("top")), 0] = 1.0
+    s = fv.FlowSetup(kind=kind, u_b=ub, p_b=torch.zeros(F, dtype=F64), advection="upwind", gradient=gradient,
+                     pressure_pin=0)
+    walls = np.concatenate([mesh.face_patches.indices(w) for w in ("top", "bottom", "left", "right")])
+    kbc = fv.ScalarSetup(kind=torch.where(m.interior, 0, fv.BC_DIRICHLET), value=torch.zeros(F, dtype=F64),
+                         t_inf=torch.zeros(F, dtype=F64))
+    wbc = fv.ScalarSetup(kind=torch.where(m.interior, 0, fv.BC_ZERO_GRADIENT), value=torch.zeros(F, dtype=F64),
+                         t_inf=torch.zeros(F, dtype=F64))
+    t = S.build_sst_setup(m, g, gradient, M, torch.full((m.n_cells,), 1e-2, dtype=F64), 1.0, walls, kbc, wbc)
+    return mesh, m, g, s, t
+
+
+def test_rans_jvp_matches_finite_difference():
+    """tests/unit/test_coupled_rans.py:516-528: JVP vs central FD, rel < 1e-6, healthy state."""
+    mesh, m, g, s, t = cavity_rans()
+    C = m.n_cells
+    rng = np.random.default_rng(0)
+    state = torch.tensor(np.concatenate([0.1 * rng.normal(size=2 * C), 0.1 * rng.normal(size=C),
+                                         0.05 + 0.01 * rng.uniform(size=C), 10.0 + rng.uniform(size=C)]))
+    mu, rho = torch.full((C,), 1e-2, dtype=F64), torch.ones(C, dtype=F64)
+    f = lambda z: S.rans_residual(m, g, s, t, z, mu, rho, "upwind")  # noqa: E731
+    v = torch.tensor(rng.normal(size=state.numel()))
+    v = v / v.norm()
+    _, jv = torch.func.jvp(f, (state,), (v,))
+    eps = 1e-5
+    fd = (f(state + eps * v) - f(state - eps * v)) / (2 * eps)
+    assert float((fd - jv).norm() / jv.norm()) < 1e-6
+    assert bool(torch.isfinite(f(state)).all())
