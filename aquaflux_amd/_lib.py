@@ -65,6 +65,46 @@ class AfxFlowCotangents(ctypes.Structure):
     ]
 
 
+class AfxRansConfig(ctypes.Structure):
+    _fields_ = [
+        ("flow", AfxFlowConfig),
+        ("scalar_advection", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("density", ctypes.c_double),
+        ("sst", ctypes.c_double * 16),
+        ("k_kind", c_void_p),
+        ("omega_kind", c_void_p),
+        ("wall_face", c_void_p),
+        ("wall_cell", c_void_p),
+        ("wall_distance", c_void_p),
+        ("wall_distance_gradient", c_void_p),
+    ]
+
+
+class AfxRansInputs(ctypes.Structure):
+    _fields_ = [
+        ("state", c_void_p),
+        ("mu", c_void_p),
+        ("rho", c_void_p),
+        ("nu", c_void_p),
+        ("u_b", c_void_p),
+        ("p_b", c_void_p),
+        ("k_value", c_void_p),
+        ("omega_value", c_void_p),
+    ]
+
+
+class AfxRansCotangents(ctypes.Structure):
+    _fields_ = [
+        ("state", c_void_p),
+        ("mu", c_void_p),
+        ("rho", c_void_p),
+        ("nu", c_void_p),
+        ("u_b", c_void_p),
+        ("p_b", c_void_p),
+    ]
+
+
 # Declared entry points: name -> (argtypes). Every function returns int except afx_last_error.
 _SIGNATURES = {
     "afx_mesh_create": [
@@ -104,6 +144,26 @@ _SIGNATURES = {
     "afx_flow_fields": [
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxFlowConfig), ctypes.POINTER(AfxFlowInputs),
         c_void_p, c_void_p, c_void_p, c_void_p,
+    ],
+    "afx_scalar_gradient": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+    ],
+    "afx_rans_workspace_bytes": [c_void_p, ctypes.c_int, c_int64_p],
+    "afx_rans_residual": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxRansConfig), ctypes.POINTER(AfxRansInputs),
+        c_void_p, c_void_p, ctypes.c_int64, c_void_p,
+    ],
+    "afx_rans_jvp": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxRansConfig), ctypes.POINTER(AfxRansInputs),
+        ctypes.POINTER(AfxRansInputs), c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p,
+    ],
+    "afx_rans_vjp": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxRansConfig), ctypes.POINTER(AfxRansInputs),
+        c_void_p, ctypes.POINTER(AfxRansCotangents), c_void_p, ctypes.c_int64, c_void_p,
+    ],
+    "afx_rans_fields": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxRansConfig), ctypes.POINTER(AfxRansInputs),
+        c_void_p, c_void_p,
     ],
 }
 

@@ -217,6 +217,55 @@ AFX_DEV FaceRef decode_face(const MeshView& m, int32_t e) {
   return r;
 }
 
+// Least-squares normal matrix of cell c (geometry only): M = sum_f w dx dx^T, w = 1/|dx|^2.
+template <int D>
+AFX_DEV void lsq_matrix(const MeshView& m, int32_t c, double M[D][D]) {
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int b = 0; b < D; ++b) M[a][b] = 0.0;
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    int32_t o = r.own ? r.N : r.P;
+    double dx[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+      dx[j] = (o >= 0 ? m.cc[(int64_t)j * m.C + o] : m.fc[(int64_t)j * m.F + r.f]) - m.cc[(int64_t)j * m.C + c];
+    double w = 1.0 / vdot<D>(dx, dx);
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int b = 0; b < D; ++b) M[a][b] += w * dx[a] * dx[b];
+  }
+}
+
+// Inverse of a symmetric positive-definite D x D matrix (adjugate / determinant).
+template <int D>
+AFX_DEV void sym_inverse(const double M[D][D], double I[D][D]) {
+  if (D == 2) {
+    double det = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+    I[0][0] = M[1][1] / det;
+    I[1][1] = M[0][0] / det;
+    I[0][1] = -M[0][1] / det;
+    I[1][0] = -M[1][0] / det;
+  } else {
+    double c00 = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+    double c01 = M[1][2] * M[2][0] - M[1][0] * M[2][2];
+    double c02 = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+    double det = M[0][0] * c00 + M[0][1] * c01 + M[0][2] * c02;
+    double inv = 1.0 / det;
+    I[0][0] = c00 * inv;
+    I[1][0] = c01 * inv;
+    I[2][0] = c02 * inv;
+    I[0][1] = (M[0][2] * M[2][1] - M[0][1] * M[2][2]) * inv;
+    I[1][1] = (M[0][0] * M[2][2] - M[0][2] * M[2][0]) * inv;
+    I[2][1] = (M[0][1] * M[2][0] - M[0][0] * M[2][1]) * inv;
+    I[0][2] = (M[0][1] * M[1][2] - M[0][2] * M[1][1]) * inv;
+    I[1][2] = (M[0][2] * M[1][0] - M[0][0] * M[1][2]) * inv;
+    I[2][2] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) * inv;
+  }
+}
+
 constexpr int kBlock = 256;
 constexpr int kNumXcd = 8;
 

@@ -1,0 +1,1321 @@
+// Coupled RANS residual R(u, v[, w], p, k, omega) with the k-omega SST closure: forward, tangent
+// (dual instantiation) and adjoint (hand-written reverse passes).
+//
+// Reference: CoupledRANS.residual (turbulence/coupled.py:437-473) = closure_fields
+// (turbulence/transport.py:508-546) -> momentum.with_eddy_viscosity(nu_t, nu_t,wall) ->
+// flow_fields / residual_from_fields (flow/momentum.py:822-878) -> k_residual / omega_residual
+// (turbulence/transport.py:581-718) on the Rhie-Chow mass flux over the density.
+//
+// Passes (one thread per cell, gather over the cell's faces, no atomics):
+//   G  gradients of u_i, p, k, omega (+ Venkatakrishnan psi of u_i, k, omega)
+//   C  cell-local SST closure: nu_t, F1, strain, cross-diffusion, wall-function nu_t, wall shear rate
+//   D  momentum diagonal with mu_eff = mu + rho nu_t and the wall-function boundary viscosity
+//   R  momentum / continuity / k / omega fluxes, SST sources, omega wall-cell fixation
+// The adjoint runs G, C, D forward, then R', D', C' (closure and limiter adjoints by seeded duals,
+// face fluxes by hand), G'.
+#include <type_traits>
+
+#include "afx_common.h"
+#include "afx_mesh.h"
+#include "afx_flow_face.h"
+#include "afx_scalar_face.h"
+#include "afx_sst.h"
+#include "afx_profile.h"
+
+using namespace afx;
+
+namespace {
+
+template <class S>
+struct RansArgs {
+  const int8_t* kind;   // flow boundary kind per face
+  const int8_t* kbc;    // k closure per face (Dirichlet / ZeroGradient)
+  const int8_t* wbc;    // omega closure per face
+  const int8_t* wallf;  // 1 on the SST wall faces
+  const int8_t* wallc;  // 1 on wall-adjacent cells
+  In<S> x, ub, pb, mu, rho, nu, kval, wval;
+  const double* dwall;
+  const double* dgrad;
+  In<S> G, psi, nut, f1, Sr, cd, nutw, tau, mueff, aP;
+  int32_t pin;
+  double pin_value;
+  double bf[3];
+  double lim_k;
+  SstConst sc;
+};
+
+// closure arrays (written by pass C)
+struct ClosureOut {
+  double *nut, *f1, *Sr, *cd, *nutw, *tau, *mueff;
+  double *nut_t, *f1_t, *Sr_t, *cd_t, *nutw_t, *tau_t, *mueff_t;
+};
+
+template <int D, class S>
+AFX_DEV S rans_bval_lead(const RansArgs<S>& a, int v, int32_t F, int32_t f, const S* xc) {
+  // leading-order boundary value (zero owner gradient) of variable v
+  if (v <= D) {
+    const int kind = a.kind[f];
+    if (v < D) {
+      if (kind == kFlowWall || kind == kFlowInlet) return a.ub[(int64_t)v * F + f];
+      if (kind == kFlowOutlet) return xc[v];
+      return sconst<S>(0.0);
+    }
+    if (kind == kFlowWall || kind == kFlowInlet) return xc[D];
+    if (kind == kFlowOutlet) return a.pb[f];
+    return sconst<S>(0.0);
+  }
+  const int8_t bk = v == D + 1 ? a.kbc[f] : a.wbc[f];
+  if (bk == kBcDirichlet) return v == D + 1 ? a.kval[f] : a.wval[f];
+  if (bk == kBcZeroGradient) return xc[v];
+  return sconst<S>(0.0);
+}
+
+// limiter slot of variable v (-1 when not limited): u_i -> i, k -> D, omega -> D + 1
+template <int D>
+AFX_DEV int lim_slot(int v, bool limf, bool lims) {
+  if (v < D) return limf ? v : -1;
+  if (v == D) return -1;
+  return lims ? v - 1 : -1;
+}
+
+// ---------------------------------------------------------------------------------------------
+// pass G
+template <int D, class S, int GRAD, bool LIMF, bool LIMS>
+__global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Out<S> G, Out<S> psi_out,
+                                                   int32_t* __restrict__ ext) {
+  constexpr int NV = D + 3, NL = D + 2;
+  const int32_t c = xcd_thread_index();
+  if (c >= m.C) return;
+  S xc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) xc[v] = a.x[(int64_t)v * m.C + c];
+  S acc[NV][D];
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int j = 0; j < D; ++j) acc[v][j] = sconst<S>(0.0);
+  double M[D][D];
+#pragma unroll
+  for (int p = 0; p < D; ++p)
+#pragma unroll
+    for (int q = 0; q < D; ++q) M[p][q] = 0.0;
+  S mx[NL], mn[NL];
+  int32_t amx[NL], amn[NL];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const int l = lim_slot<D>(v, LIMF, LIMS);
+    if (l >= 0) {
+      mx[l] = xc[v];
+      mn[l] = xc[v];
+      amx[l] = c;
+      amn[l] = c;
+    }
+  }
+  const int32_t kb = m.cptr[c], ke = m.cptr[c + 1];
+  for (int32_t k = kb; k < ke; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    if (o >= 0) {
+      S xo[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) xo[v] = a.x[(int64_t)v * m.C + o];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const int l = lim_slot<D>(v, LIMF, LIMS);
+        if (l >= 0) {
+          if (val(xo[v]) > val(mx[l])) {
+            mx[l] = xo[v];
+            amx[l] = o;
+          }
+          if (val(xo[v]) < val(mn[l])) {
+            mn[l] = xo[v];
+            amn[l] = o;
+          }
+        }
+      }
+      if (GRAD == 0) {
+        FaceGeo<D> q;
+        load_face_geo<D>(m, r.f, r.P, r.N, q);
+        const double w = 1.0 - q.g;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          S phi = w * (r.own ? xc[v] : xo[v]) + q.g * (r.own ? xo[v] : xc[v]);
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+            S t = phi * (q.n[j] * q.A);
+            acc[v][j] = r.own ? acc[v][j] + t : acc[v][j] - t;
+          }
+        }
+      } else {
+        double dx[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dx[j] = m.cc[(int64_t)j * m.C + o] - m.cc[(int64_t)j * m.C + c];
+        const double wgt = 1.0 / vdot<D>(dx, dx);
+#pragma unroll
+        for (int p = 0; p < D; ++p)
+#pragma unroll
+          for (int q2 = 0; q2 < D; ++q2) M[p][q2] += wgt * dx[p] * dx[q2];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          S dphi = xo[v] - xc[v];
+#pragma unroll
+          for (int j = 0; j < D; ++j) acc[v][j] += (wgt * dx[j]) * dphi;
+        }
+      }
+    } else {
+      if (GRAD == 0) {
+        const double A = m.area[r.f];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          S phi = rans_bval_lead<D, S>(a, v, m.F, r.f, xc);
+#pragma unroll
+          for (int j = 0; j < D; ++j) acc[v][j] += phi * (m.nrm[(int64_t)j * m.F + r.f] * A);
+        }
+      } else {
+        double dx[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dx[j] = m.fc[(int64_t)j * m.F + r.f] - m.cc[(int64_t)j * m.C + c];
+        const double wgt = 1.0 / vdot<D>(dx, dx);
+#pragma unroll
+        for (int p = 0; p < D; ++p)
+#pragma unroll
+          for (int q2 = 0; q2 < D; ++q2) M[p][q2] += wgt * dx[p] * dx[q2];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          S dphi = rans_bval_lead<D, S>(a, v, m.F, r.f, xc) - xc[v];
+#pragma unroll
+          for (int j = 0; j < D; ++j) acc[v][j] += (wgt * dx[j]) * dphi;
+        }
+      }
+    }
+  }
+  S grad[NV][D];
+  if (GRAD == 0) {
+    const double iv = 1.0 / m.vol[c];
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int j = 0; j < D; ++j) grad[v][j] = acc[v][j] * iv;
+  } else {
+    double I[D][D];
+    sym_inverse<D>(M, I);
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int p = 0; p < D; ++p) {
+        S s = I[p][0] * acc[v][0];
+#pragma unroll
+        for (int q2 = 1; q2 < D; ++q2) s = s + I[p][q2] * acc[v][q2];
+        grad[v][p] = s;
+      }
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int j = 0; j < D; ++j) G.set((int64_t)(v * D + j) * m.C + c, grad[v][j]);
+  if (LIMF || LIMS) {
+    const double e2 = m.vol[c] * (a.lim_k * a.lim_k * a.lim_k);
+    double xcen[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) xcen[j] = m.cc[(int64_t)j * m.C + c];
+    S lo[NL];
+    bool first = true;
+    for (int32_t k = kb; k < ke; ++k) {
+      const int32_t e = m.cface[k];
+      const int32_t f = e >= 0 ? e : ~e;
+      double dxf[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) dxf[j] = m.fc[(int64_t)j * m.F + f] - xcen[j];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const int l = lim_slot<D>(v, LIMF, LIMS);
+        if (l < 0) continue;
+        S dm = grad[v][0] * dxf[0];
+#pragma unroll
+        for (int j = 1; j < D; ++j) dm = dm + grad[v][j] * dxf[j];
+        S vf = venkat_face<S>(dm, mx[l] - xc[v], mn[l] - xc[v], e2);
+        if (first || val(vf) < val(lo[l])) lo[l] = vf;
+      }
+      first = false;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const int l = lim_slot<D>(v, LIMF, LIMS);
+      if (l < 0) continue;
+      psi_out.set((int64_t)l * m.C + c, dmin(dmax(lo[l], 0.0), 1.0));
+      if (ext) {
+        ext[(int64_t)l * m.C + c] = amx[l];
+        ext[(int64_t)(NL + l) * m.C + c] = amn[l];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// pass C: closure
+template <int D, class S>
+AFX_DEV void load_closure_inputs(const RansArgs<S>& a, const MeshView& m, int32_t c, S& k, S& w, S (&Gu)[D][D],
+                                 S (&Gk)[D], S (&Gw)[D]) {
+  k = a.x[(int64_t)(D + 1) * m.C + c];
+  w = a.x[(int64_t)(D + 2) * m.C + c];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) Gu[i][j] = a.G[(int64_t)(i * D + j) * m.C + c];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    Gk[j] = a.G[(int64_t)((D + 1) * D + j) * m.C + c];
+    Gw[j] = a.G[(int64_t)((D + 2) * D + j) * m.C + c];
+  }
+}
+
+// Wall shear rate of a wall cell: area-weighted |u_P - u_b| / d over its SST wall faces.
+template <int D, class S>
+AFX_DEV S wall_shear_rate(const RansArgs<S>& a, const MeshView& m, int32_t c, double d) {
+  S num = sconst<S>(0.0);
+  double den = 0.0;
+  S xc[D + 1];
+#pragma unroll
+  for (int v = 0; v <= D; ++v) xc[v] = a.x[(int64_t)v * m.C + c];
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    const int32_t e = m.cface[k];
+    if (e < 0 || !a.wallf[e]) continue;
+    S s2 = sconst<S>(0.0);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      S sl = xc[j] - rans_bval_lead<D, S>(a, j, m.F, e, xc);
+      s2 = j == 0 ? sl * sl : s2 + sl * sl;
+    }
+    const double A = m.area[e];
+    num = num + (safe_sqrt(s2) / d) * A;
+    den += A;
+  }
+  return den > 0.0 ? num / den : sconst<S>(0.0);
+}
+
+template <int D, class S>
+__global__ void __launch_bounds__(256) k_rans_closure(MeshView m, RansArgs<S> a, ClosureOut out) {
+  const int32_t c = xcd_thread_index();
+  if (c >= m.C) return;
+  S k, w, Gu[D][D], Gk[D], Gw[D];
+  load_closure_inputs<D, S>(a, m, c, k, w, Gu, Gk, Gw);
+  const bool wall = a.wallc[c] != 0;
+  const double d = a.dwall[c];
+  double dg[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) dg[j] = wall ? a.dgrad[(int64_t)j * m.C + c] : 0.0;
+  const S nu = a.nu[c];
+  SstClosure<S> o = sst_closure<D, S>(a.sc, k, w, Gu, Gk, Gw, nu, d, dg, wall);
+  S tau = wall ? wall_shear_rate<D, S>(a, m, c, d) : sconst<S>(0.0);
+  S mueff = a.mu[c] + a.rho[c] * o.nut;
+  Out<S>{out.nut, out.nut_t}.set(c, o.nut);
+  Out<S>{out.f1, out.f1_t}.set(c, o.f1);
+  Out<S>{out.Sr, out.Sr_t}.set(c, o.strain);
+  Out<S>{out.cd, out.cd_t}.set(c, o.cd);
+  Out<S>{out.nutw, out.nutw_t}.set(c, o.nutw);
+  Out<S>{out.tau, out.tau_t}.set(c, tau);
+  Out<S>{out.mueff, out.mueff_t}.set(c, mueff);
+}
+
+// ---------------------------------------------------------------------------------------------
+// shared loaders for passes D / R
+template <int D, class S>
+AFX_DEV void load_flow(const RansArgs<S>& a, const MeshView& m, int32_t c, bool with_ap, bool with_psi,
+                       FlowCell<D, S>& X) {
+#pragma unroll
+  for (int i = 0; i < D; ++i) X.u[i] = a.x[(int64_t)i * m.C + c];
+  X.p = a.x[(int64_t)D * m.C + c];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) X.Gu[i][j] = a.G[(int64_t)(i * D + j) * m.C + c];
+#pragma unroll
+  for (int j = 0; j < D; ++j) X.Gp[j] = a.G[(int64_t)(D * D + j) * m.C + c];
+  X.mu = a.mueff[c];
+  X.rho = a.rho[c];
+  X.V = m.vol[c];
+  X.aP = with_ap ? a.aP[c] : sconst<S>(1.0);
+#pragma unroll
+  for (int i = 0; i < D; ++i) X.psi[i] = with_psi ? a.psi[(int64_t)i * m.C + c] : sconst<S>(1.0);
+}
+
+// The momentum boundary-face viscosity: mu + rho nu_t,wall on shearing walls (flow/momentum.py:351-418).
+template <class S>
+AFX_DEV S rans_gamma_b(const RansArgs<S>& a, int kind, int32_t f, int32_t P, S mueffP) {
+  if (kind == kFlowWall) return a.mu[P] + a.rho[P] * (a.wallf[f] ? a.nutw[P] : sconst<S>(0.0));
+  return mueffP;
+}
+
+template <int D, class S>
+struct ScalarCell {
+  S phi, G[D], psi, gam;
+};
+
+template <int D, class S>
+AFX_DEV void load_scalar(const RansArgs<S>& a, const MeshView& m, int32_t c, int var, bool with_psi, double s1,
+                         double s2, ScalarCell<D, S>& Z) {
+  Z.phi = a.x[(int64_t)var * m.C + c];
+#pragma unroll
+  for (int j = 0; j < D; ++j) Z.G[j] = a.G[(int64_t)(var * D + j) * m.C + c];
+  Z.psi = with_psi ? a.psi[(int64_t)(var - 1) * m.C + c] : sconst<S>(1.0);
+  Z.gam = a.nu[c] + sst_blend(a.f1[c], s1, s2) * a.nut[c];
+}
+
+// ---------------------------------------------------------------------------------------------
+// pass D
+template <int D, class S, int ADV>
+__global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<S> aP) {
+  const int32_t c = xcd_thread_index();
+  if (c >= m.C) return;
+  FlowCell<D, S> X;
+  load_flow<D, S>(a, m, c, false, false, X);
+  S acc = sconst<S>(0.0);
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    FaceGeo<D> q;
+    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    if (o >= 0) {
+      FlowCell<D, S> O;
+      load_flow<D, S>(a, m, o, false, false, O);
+      S oc, nc;
+      if (r.own)
+        ap_interior<D, ADV, S>(q, X, O, oc, nc);
+      else
+        ap_interior<D, ADV, S>(q, O, X, oc, nc);
+      acc += r.own ? oc : nc;
+    } else {
+      const int kind = a.kind[r.f];
+      acc += ap_boundary<D, ADV, S>(q, kind, X, rans_gamma_b<S>(a, kind, r.f, c, X.mu));
+    }
+  }
+  aP.set(c, acc);
+}
+
+// ---------------------------------------------------------------------------------------------
+// pass R
+template <int D, class S, int ADVF, int ADVS>
+__global__ void __launch_bounds__(256) k_rans_res(MeshView m, RansArgs<S> a, Out<S> R) {
+  constexpr int NV = D + 1;
+  const SstConst& sc = a.sc;
+  const int32_t c = xcd_thread_index();
+  if (c >= m.C) return;
+  FlowCell<D, S> X;
+  load_flow<D, S>(a, m, c, true, ADVF == kAdvLimited, X);
+  ScalarCell<D, S> Kc, Wc;
+  load_scalar<D, S>(a, m, c, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Kc);
+  load_scalar<D, S>(a, m, c, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wc);
+  S acc[NV], acck = sconst<S>(0.0), accw = sconst<S>(0.0);
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = sconst<S>(0.0);
+  const double irho = 1.0 / sc.density;
+  const double dP = a.dwall[c];
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    FaceGeo<D> q;
+    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    S out[NV], fk, fw;
+    if (o >= 0) {
+      FlowCell<D, S> O;
+      load_flow<D, S>(a, m, o, true, ADVF == kAdvLimited, O);
+      ScalarCell<D, S> Ko, Wo;
+      load_scalar<D, S>(a, m, o, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Ko);
+      load_scalar<D, S>(a, m, o, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wo);
+      if (r.own)
+        flow_flux_interior<D, ADVF, S>(q, X, O, out);
+      else
+        flow_flux_interior<D, ADVF, S>(q, O, X, out);
+      S Qf = out[D] * irho;
+      const ScalarCell<D, S>& KP = r.own ? Kc : Ko;
+      const ScalarCell<D, S>& KN = r.own ? Ko : Kc;
+      const ScalarCell<D, S>& WP = r.own ? Wc : Wo;
+      const ScalarCell<D, S>& WN = r.own ? Wo : Wc;
+      fk = scalar_flux_interior<D, ADVS, S>(q, Qf, KP.phi, KN.phi, KP.G, KN.G, KP.psi, KN.psi, KP.gam, KN.gam);
+      fw = scalar_flux_interior<D, ADVS, S>(q, Qf, WP.phi, WN.phi, WP.G, WN.G, WP.psi, WN.psi, WP.gam, WN.gam);
+    } else {
+      S ub[D];
+#pragma unroll
+      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.f];
+      const int kind = a.kind[r.f];
+      flow_flux_boundary<D, ADVF, S>(q, kind, X, ub, a.pb[r.f], rans_gamma_b<S>(a, kind, r.f, c, X.mu), out);
+      S Qf = out[D] * irho;
+      S zero = sconst<S>(0.0);
+      S bk = scalar_bval<D, S>(a.kbc[r.f], q, Kc.phi, Kc.G, a.kval[r.f], zero, zero);
+      S gk = Kc.gam;
+      if (a.wallf[r.f]) gk = (1.0 - wall_function_weight<S>(sc, a.nu[c], dP, Kc.phi)) * Kc.gam;
+      fk = scalar_flux_boundary<D, ADVS, S>(q, Qf, Kc.phi, Kc.G, Kc.psi, bk, gk);
+      S bw = scalar_bval<D, S>(a.wbc[r.f], q, Wc.phi, Wc.G, a.wval[r.f], zero, zero);
+      fw = scalar_flux_boundary<D, ADVS, S>(q, Qf, Wc.phi, Wc.G, Wc.psi, bw, Wc.gam);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = r.own ? acc[v] + out[v] : acc[v] - out[v];
+    acck = r.own ? acck + fk : acck - fk;
+    accw = r.own ? accw + fw : accw - fw;
+  }
+#pragma unroll
+  for (int i = 0; i < D; ++i) R.set((int64_t)i * m.C + c, acc[i] - a.bf[i] * X.V);
+  S cont = acc[D];
+  if (c == a.pin) cont = X.p - a.pin_value;
+  R.set((int64_t)D * m.C + c, cont);
+  const bool wall = a.wallc[c] != 0;
+  S pk, dk, pw, dw, cw;
+  sst_sources<S>(sc, Kc.phi, Wc.phi, a.nut[c], a.Sr[c], a.f1[c], a.cd[c], a.tau[c], a.nu[c], dP, wall, X.V, pk, dk,
+                 pw, dw, cw);
+  R.set((int64_t)(D + 1) * m.C + c, (acck - pk) - dk);
+  S rw = ((accw - pw) - dw) - cw;
+  if (wall) rw = Wc.phi - omega_wall_k<S>(sc, a.nu[c], dP, Kc.phi);
+  R.set((int64_t)(D + 2) * m.C + c, rw);
+}
+
+// ---------------------------------------------------------------------------------------------
+// adjoint storage
+struct RansBars {
+  double* x;      // [(D+3)][C]   (output)
+  double* mu;     // [C]          (output: molecular dynamic viscosity)
+  double* rho;    // [C]          (output)
+  double* nu;     // [C]          (output: molecular kinematic viscosity)
+  double* ub;     // [D][F]       (output)
+  double* pb;     // [F]          (output)
+  double* G;      // [(D+3)D][C]  scratch
+  double* aP;     // [C]
+  double* psi;    // [D+2][C]
+  double* ext;    // [2(D+2)][C]
+  double* mueff;  // [C]
+  double* nut;    // [C]
+  double* f1;
+  double* Sr;
+  double* cd;
+  double* tau;
+  double* nutw;
+};
+
+template <int D>
+struct RansCellBar {
+  FlowCellBar<D> fl;  // fl.mu = mu_eff cotangent
+  ScalarBar<D> k, w;
+  double nut, f1, Sr, cd, tau, nutw, mum, nu;
+  AFX_DEV void zero() {
+    fl.zero();
+    k.zero();
+    w.zero();
+    nut = f1 = Sr = cd = tau = nutw = mum = nu = 0.0;
+  }
+};
+
+// gamma = nu + blend(f1, s1, s2) nu_t
+AFX_DEV void adj_gamma(double gam_bar, double f1, double nut, double s1, double s2, double& nu_b, double& f1_b,
+                       double& nut_b) {
+  nu_b += gam_bar;
+  f1_b += gam_bar * (s1 - s2) * nut;
+  nut_b += gam_bar * (f1 * s1 + (1.0 - f1) * s2);
+}
+
+// Routes the cotangent of the momentum boundary viscosity.
+template <int D>
+AFX_DEV void adj_gamma_b(const RansArgs<double>& a, int kind, int32_t f, int32_t P, double gbb, RansCellBar<D>& B) {
+  if (kind == kFlowWall) {
+    B.mum += gbb;
+    const double nw = a.wallf[f] ? a.nutw[P] : 0.0;
+    B.fl.rho += gbb * nw;
+    if (a.wallf[f]) B.nutw += gbb * a.rho[P];
+  } else {
+    B.fl.mu += gbb;
+  }
+}
+
+// reverse pass R'
+template <int D, int ADVF, int ADVS>
+__global__ void __launch_bounds__(256) k_rans_res_adj(MeshView m, RansArgs<double> a, const double* __restrict__ lam,
+                                                      RansBars bar) {
+  constexpr int NV = D + 1, NA = D + 3;
+  const SstConst& sc = a.sc;
+  const int32_t c = xcd_thread_index();
+  if (c >= m.C) return;
+  FlowCell<D, double> X;
+  load_flow<D, double>(a, m, c, true, ADVF == kAdvLimited, X);
+  ScalarCell<D, double> Kc, Wc;
+  load_scalar<D, double>(a, m, c, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Kc);
+  load_scalar<D, double>(a, m, c, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wc);
+  RansCellBar<D> B;
+  B.zero();
+  const bool wall = a.wallc[c] != 0;
+  const double irho = 1.0 / sc.density;
+  const double dP = a.dwall[c];
+  double lc[NA];
+#pragma unroll
+  for (int v = 0; v < NA; ++v) lc[v] = lam[(int64_t)v * m.C + c];
+  const double lc_cont = c == a.pin ? 0.0 : lc[D];
+  const double lc_w = wall ? 0.0 : lc[D + 2];  // flux / source part of the omega row
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    FaceGeo<D> q;
+    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    double lf[NV];
+    if (o >= 0) {
+      FlowCell<D, double> O;
+      load_flow<D, double>(a, m, o, true, ADVF == kAdvLimited, O);
+      ScalarCell<D, double> Ko, Wo;
+      load_scalar<D, double>(a, m, o, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Ko);
+      load_scalar<D, double>(a, m, o, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wo);
+      double out[NV];
+      const FlowCell<D, double>& P = r.own ? X : O;
+      const FlowCell<D, double>& N = r.own ? O : X;
+      flow_flux_interior<D, ADVF, double>(q, P, N, out);
+      const double Qf = out[D] * irho;
+      const bool wo = a.wallc[o] != 0;
+      const double lk_o = lam[(int64_t)(D + 1) * m.C + o];
+      const double lw_o = wo ? 0.0 : lam[(int64_t)(D + 2) * m.C + o];
+      const double lkf = r.own ? lc[D + 1] - lk_o : lk_o - lc[D + 1];
+      const double lwf = r.own ? lc_w - lw_o : lw_o - lc_w;
+      const ScalarCell<D, double>& KP = r.own ? Kc : Ko;
+      const ScalarCell<D, double>& KN = r.own ? Ko : Kc;
+      const ScalarCell<D, double>& WP = r.own ? Wc : Wo;
+      const ScalarCell<D, double>& WN = r.own ? Wo : Wc;
+      const int side = r.own ? 0 : 1;
+      double Qb = adj_scalar_flux_interior<D, ADVS>(q, Qf, KP.phi, KN.phi, KP.G, KN.G, KP.psi, KN.psi, KP.gam, KN.gam,
+                                                    lkf, side, B.k);
+      Qb += adj_scalar_flux_interior<D, ADVS>(q, Qf, WP.phi, WN.phi, WP.G, WN.G, WP.psi, WN.psi, WP.gam, WN.gam, lwf,
+                                              side, B.w);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        double lo = lam[(int64_t)v * m.C + o];
+        if (v == D && o == a.pin) lo = 0.0;
+        const double lcv = v == D ? lc_cont : lc[v];
+        lf[v] = r.own ? lcv - lo : lo - lcv;
+      }
+      lf[D] += Qb * irho;
+      adj_flux_interior<D, ADVF>(q, P, N, lf, side, B.fl);
+      if (r.own) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) bar.ub[(int64_t)i * m.F + r.f] = 0.0;
+        bar.pb[r.f] = 0.0;
+      }
+    } else {
+      const int kind = a.kind[r.f];
+      double ub[D], out[NV];
+#pragma unroll
+      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.f];
+      const double gB = rans_gamma_b<double>(a, kind, r.f, c, X.mu);
+      flow_flux_boundary<D, ADVF, double>(q, kind, X, ub, a.pb[r.f], gB, out);
+      const double Qf = out[D] * irho;
+      // k face
+      const double bk = scalar_bval<D, double>(a.kbc[r.f], q, Kc.phi, Kc.G, a.kval[r.f], 0.0, 0.0);
+      double wgt = 0.0, gk = Kc.gam;
+      if (a.wallf[r.f]) {
+        wgt = wall_function_weight<double>(sc, a.nu[c], dP, Kc.phi);
+        gk = (1.0 - wgt) * Kc.gam;
+      }
+      double bkb, gkb, vb;
+      double Qb = adj_scalar_flux_boundary<D, ADVS>(q, Qf, Kc.phi, Kc.G, Kc.psi, bk, gk, lc[D + 1], B.k, bkb, gkb);
+      adj_scalar_bval<D>(a.kbc[r.f], q, bkb, B.k, vb);
+      if (a.wallf[r.f]) {
+        B.k.gam += (1.0 - wgt) * gkb;
+        const double wb = -Kc.gam * gkb;
+        dual wk = wall_function_weight<dual>(sc, dual{a.nu[c], 0.0}, dP, dual{Kc.phi, 1.0});
+        dual wn = wall_function_weight<dual>(sc, dual{a.nu[c], 1.0}, dP, dual{Kc.phi, 0.0});
+        B.k.phi += wb * wk.d;
+        B.nu += wb * wn.d;
+      } else {
+        B.k.gam += gkb;
+      }
+      // omega face
+      const double bw = scalar_bval<D, double>(a.wbc[r.f], q, Wc.phi, Wc.G, a.wval[r.f], 0.0, 0.0);
+      double bwb, gwb;
+      Qb += adj_scalar_flux_boundary<D, ADVS>(q, Qf, Wc.phi, Wc.G, Wc.psi, bw, Wc.gam, lc_w, B.w, bwb, gwb);
+      adj_scalar_bval<D>(a.wbc[r.f], q, bwb, B.w, vb);
+      B.w.gam += gwb;
+      // flow face
+#pragma unroll
+      for (int v = 0; v < NV; ++v) lf[v] = v == D ? lc_cont : lc[v];
+      lf[D] += Qb * irho;
+      double ubb[D], pbb, gbb;
+      adj_flux_boundary<D, ADVF>(q, kind, X, ub, a.pb[r.f], gB, lf, B.fl, ubb, pbb, gbb);
+      adj_gamma_b<D>(a, kind, r.f, c, gbb, B);
+#pragma unroll
+      for (int i = 0; i < D; ++i) bar.ub[(int64_t)i * m.F + r.f] = ubb[i];
+      bar.pb[r.f] = pbb;
+    }
+  }
+  // sources (cell-local): R_k -= prod_k + dest_k, R_w -= prod_w + dest_w + cross_w (non-wall)
+  {
+    const double lk = -lc[D + 1], lw = -lc_w;
+    double inp[8] = {Kc.phi, Wc.phi, a.nut[c], a.Sr[c], a.f1[c], a.cd[c], a.tau[c], a.nu[c]};
+    double grad[8];
+    for (int s = 0; s < 8; ++s) {
+      dual in[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) in[t] = dual{inp[t], t == s ? 1.0 : 0.0};
+      dual pk, dk, pw, dw, cw;
+      sst_sources<dual>(sc, in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], dP, wall, X.V, pk, dk, pw, dw, cw);
+      grad[s] = lk * (pk.d + dk.d) + lw * (pw.d + dw.d + cw.d);
+    }
+    B.k.phi += grad[0];
+    B.w.phi += grad[1];
+    B.nut += grad[2];
+    B.Sr += grad[3];
+    B.f1 += grad[4];
+    B.cd += grad[5];
+    B.tau += grad[6];
+    B.nu += grad[7];
+  }
+  if (wall) {  // R_w = w - omega_wall(nu, d, k)
+    const double l = lc[D + 2];
+    B.w.phi += l;
+    dual ok = omega_wall_k<dual>(sc, dual{a.nu[c], 0.0}, dP, dual{Kc.phi, 1.0});
+    dual on = omega_wall_k<dual>(sc, dual{a.nu[c], 1.0}, dP, dual{Kc.phi, 0.0});
+    B.k.phi -= l * ok.d;
+    B.nu -= l * on.d;
+  }
+  if (c == a.pin) B.fl.p += lc[D];
+  // diffusivities gamma_k, gamma_w of this cell
+  adj_gamma(B.k.gam, a.f1[c], a.nut[c], sc.sk1, sc.sk2, B.nu, B.f1, B.nut);
+  adj_gamma(B.w.gam, a.f1[c], a.nut[c], sc.sw1, sc.sw2, B.nu, B.f1, B.nut);
+  // write
+#pragma unroll
+  for (int i = 0; i < D; ++i) bar.x[(int64_t)i * m.C + c] = B.fl.u[i];
+  bar.x[(int64_t)D * m.C + c] = B.fl.p;
+  bar.x[(int64_t)(D + 1) * m.C + c] = B.k.phi;
+  bar.x[(int64_t)(D + 2) * m.C + c] = B.w.phi;
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) bar.G[(int64_t)(i * D + j) * m.C + c] = B.fl.Gu[i][j];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    bar.G[(int64_t)(D * D + j) * m.C + c] = B.fl.Gp[j];
+    bar.G[(int64_t)((D + 1) * D + j) * m.C + c] = B.k.G[j];
+    bar.G[(int64_t)((D + 2) * D + j) * m.C + c] = B.w.G[j];
+  }
+  bar.aP[c] = B.fl.aP;
+  if (ADVF == kAdvLimited) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) bar.psi[(int64_t)i * m.C + c] = B.fl.psi[i];
+  }
+  if (ADVS == kAdvLimited) {
+    bar.psi[(int64_t)D * m.C + c] = B.k.psi;
+    bar.psi[(int64_t)(D + 1) * m.C + c] = B.w.psi;
+  }
+  bar.mueff[c] = B.fl.mu;
+  bar.rho[c] = B.fl.rho;
+  bar.mu[c] = B.mum;
+  bar.nu[c] = B.nu;
+  bar.nut[c] = B.nut;
+  bar.f1[c] = B.f1;
+  bar.Sr[c] = B.Sr;
+  bar.cd[c] = B.cd;
+  bar.tau[c] = B.tau;
+  bar.nutw[c] = B.nutw;
+}
+
+// reverse pass D'
+template <int D, int ADV>
+__global__ void __launch_bounds__(256) k_rans_ap_adj(MeshView m, RansArgs<double> a, RansBars bar) {
+  const int32_t c = xcd_thread_index();
+  if (c >= m.C) return;
+  FlowCell<D, double> X;
+  load_flow<D, double>(a, m, c, false, false, X);
+  RansCellBar<D> B;
+  B.zero();
+  const double apb_c = bar.aP[c];
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    FaceGeo<D> q;
+    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    if (o >= 0) {
+      FlowCell<D, double> O;
+      load_flow<D, double>(a, m, o, false, false, O);
+      const double apb_o = bar.aP[o];
+      if (r.own)
+        adj_ap_interior<D, ADV>(q, X, O, apb_c, apb_o, 0, B.fl);
+      else
+        adj_ap_interior<D, ADV>(q, O, X, apb_o, apb_c, 1, B.fl);
+    } else {
+      const int kind = a.kind[r.f];
+      const double gB = rans_gamma_b<double>(a, kind, r.f, c, X.mu);
+      double gbb;
+      adj_ap_boundary<D, ADV>(q, kind, X, gB, apb_c, B.fl, gbb);
+      adj_gamma_b<D>(a, kind, r.f, c, gbb, B);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < D; ++i) bar.x[(int64_t)i * m.C + c] += B.fl.u[i];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) bar.G[(int64_t)(i * D + j) * m.C + c] += B.fl.Gu[i][j];
+  bar.mueff[c] += B.fl.mu;
+  bar.rho[c] += B.fl.rho;
+  bar.mu[c] += B.mum;
+  bar.nutw[c] += B.nutw;
+}
+
+// reverse pass C': closure (seeded duals), wall shear rate, limiters; then the gradient cotangent
+// is pre-multiplied by the cell's reconstruction operator.
+template <int D, int ADVF, int ADVS, int GRAD>
+__global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<double> a,
+                                                          const int32_t* __restrict__ ext, RansBars bar) {
+  constexpr int NA = D + 3, NL = D + 2;
+  const SstConst& sc = a.sc;
+  const int32_t c = xcd_thread_index();
+  if (c >= m.C) return;
+  const bool wall = a.wallc[c] != 0;
+  const double d = a.dwall[c];
+  double dg[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) dg[j] = wall ? a.dgrad[(int64_t)j * m.C + c] : 0.0;
+  // total closure-output cotangents (mu_eff = mu + rho nu_t)
+  const double me_b = bar.mueff[c];
+  const double nut_b = bar.nut[c] + a.rho[c] * me_b;
+  double rho_b = bar.rho[c] + a.nut[c] * me_b;
+  double mu_b = bar.mu[c] + me_b;
+  const double f1_b = bar.f1[c], Sr_b = bar.Sr[c], cd_b = bar.cd[c], nutw_b = bar.nutw[c];
+  double nu_b = bar.nu[c];
+  double k0, w0, Gu0[D][D], Gk0[D], Gw0[D];
+  load_closure_inputs<D, double>(a, m, c, k0, w0, Gu0, Gk0, Gw0);
+  const double nu0 = a.nu[c];
+  // inputs: 0 k, 1 w, 2 nu, then Gu (D*D), Gk (D), Gw (D)
+  constexpr int NI = 3 + D * D + 2 * D;
+  double gin[NI];
+  for (int s = 0; s < NI; ++s) {
+    if (wall && s >= 3 + D * D + D) {  // the wall-imposed omega gradient replaces the reconstructed one
+      gin[s] = 0.0;
+      continue;
+    }
+    auto sd = [&](int t) { return t == s ? 1.0 : 0.0; };
+    dual k = {k0, sd(0)}, w = {w0, sd(1)}, nu = {nu0, sd(2)};
+    dual Gu[D][D], Gk[D], Gw[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) Gu[i][j] = dual{Gu0[i][j], sd(3 + i * D + j)};
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      Gk[j] = dual{Gk0[j], sd(3 + D * D + j)};
+      Gw[j] = dual{Gw0[j], sd(3 + D * D + D + j)};
+    }
+    SstClosure<dual> o = sst_closure<D, dual>(sc, k, w, Gu, Gk, Gw, nu, d, dg, wall);
+    gin[s] = nut_b * o.nut.d + f1_b * o.f1.d + Sr_b * o.strain.d + cd_b * o.cd.d + nutw_b * o.nutw.d;
+  }
+  double xb_k = gin[0], xb_w = gin[1];
+  nu_b += gin[2];
+  double Gb[NA][D];
+#pragma unroll
+  for (int v = 0; v < NA; ++v)
+#pragma unroll
+    for (int j = 0; j < D; ++j) Gb[v][j] = bar.G[(int64_t)(v * D + j) * m.C + c];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) Gb[i][j] += gin[3 + i * D + j];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    Gb[D + 1][j] += gin[3 + D * D + j];
+    Gb[D + 2][j] += gin[3 + D * D + D + j];
+  }
+  double xb[NA];
+#pragma unroll
+  for (int v = 0; v < NA; ++v) xb[v] = 0.0;
+  xb[D + 1] = xb_k;
+  xb[D + 2] = xb_w;
+  // wall shear rate: tau = sum_f (|u - u_b| / d) A_f / sum_f A_f
+  const double tau_b = bar.tau[c];
+  if (wall && tau_b != 0.0) {
+    double xcl[D + 1];
+#pragma unroll
+    for (int v = 0; v <= D; ++v) xcl[v] = a.x[(int64_t)v * m.C + c];
+    double den = 0.0;
+    for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+      const int32_t e = m.cface[k];
+      if (e >= 0 && a.wallf[e]) den += m.area[e];
+    }
+    for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+      const int32_t e = m.cface[k];
+      if (e < 0 || !a.wallf[e]) continue;
+      double sl[D], s2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        sl[j] = xcl[j] - rans_bval_lead<D, double>(a, j, m.F, e, xcl);
+        s2 += sl[j] * sl[j];
+      }
+      if (s2 <= 0.0) continue;
+      const double rate_b = tau_b * m.area[e] / den;
+      const double mag = sqrt(s2);
+      const int kind = a.kind[e];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        const double sb = rate_b / d * sl[j] / mag;
+        xb[j] += sb;  // d slip / d u_P
+        if (kind == kFlowWall || kind == kFlowInlet)
+          bar.ub[(int64_t)j * m.F + e] -= sb;
+        else if (kind == kFlowOutlet)
+          xb[j] -= sb;
+      }
+    }
+  }
+  // limiters (u_i if the momentum advection is limited, k and omega if theirs is)
+  {
+    const double e2 = m.vol[c] * (a.lim_k * a.lim_k * a.lim_k);
+    double xcen[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) xcen[j] = m.cc[(int64_t)j * m.C + c];
+#pragma unroll
+    for (int v = 0; v < NA; ++v) {
+      const int l = lim_slot<D>(v, ADVF == kAdvLimited, ADVS == kAdvLimited);
+      if (l < 0) continue;
+      const double pbar = bar.psi[(int64_t)l * m.C + c];
+      const int32_t amx = ext[(int64_t)l * m.C + c], amn = ext[(int64_t)(NL + l) * m.C + c];
+      const double xv = a.x[(int64_t)v * m.C + c];
+      const double mxv = a.x[(int64_t)v * m.C + amx], mnv = a.x[(int64_t)v * m.C + amn];
+      double Gv[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) Gv[j] = a.G[(int64_t)(v * D + j) * m.C + c];
+      double lo = 0.0;
+      int32_t kmin = -1;
+      for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+        const int32_t e = m.cface[k];
+        const int32_t f = e >= 0 ? e : ~e;
+        double dmf = Gv[0] * (m.fc[f] - xcen[0]);
+#pragma unroll
+        for (int j = 1; j < D; ++j) dmf = dmf + Gv[j] * (m.fc[(int64_t)j * m.F + f] - xcen[j]);
+        const double vf = venkat_face<double>(dmf, mxv - xv, mnv - xv, e2);
+        if (kmin < 0 || vf < lo) {
+          lo = vf;
+          kmin = k;
+        }
+      }
+      const double dclip = (lo > 0.0 && lo < 1.0) ? 1.0 : ((lo == 0.0 || lo == 1.0) ? 0.5 : 0.0);
+      const double vbar = pbar * dclip;
+      double mxb = 0.0, mnb = 0.0;
+      if (vbar != 0.0 && kmin >= 0) {
+        const int32_t e = m.cface[kmin];
+        const int32_t f = e >= 0 ? e : ~e;
+        double dxf[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dxf[j] = m.fc[(int64_t)j * m.F + f] - xcen[j];
+        double dmf = Gv[0] * dxf[0];
+#pragma unroll
+        for (int j = 1; j < D; ++j) dmf = dmf + Gv[j] * dxf[j];
+        dual r_dm = venkat_face<dual>(dual{dmf, 1.0}, dual{mxv - xv, 0.0}, dual{mnv - xv, 0.0}, e2);
+        dual r_hx = venkat_face<dual>(dual{dmf, 0.0}, dual{mxv - xv, 1.0}, dual{mnv - xv, 0.0}, e2);
+        dual r_hn = venkat_face<dual>(dual{dmf, 0.0}, dual{mxv - xv, 0.0}, dual{mnv - xv, 1.0}, e2);
+#pragma unroll
+        for (int j = 0; j < D; ++j) Gb[v][j] += vbar * r_dm.d * dxf[j];
+        mxb = vbar * r_hx.d;
+        mnb = vbar * r_hn.d;
+        xb[v] -= mxb + mnb;
+        if (amx == c) {
+          xb[v] += mxb;
+          mxb = 0.0;
+        }
+        if (amn == c) {
+          xb[v] += mnb;
+          mnb = 0.0;
+        }
+      }
+      bar.ext[(int64_t)l * m.C + c] = mxb;
+      bar.ext[(int64_t)(NL + l) * m.C + c] = mnb;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < NA; ++v) bar.x[(int64_t)v * m.C + c] += xb[v];
+  bar.nu[c] = nu_b;
+  bar.mu[c] = mu_b;
+  bar.rho[c] = rho_b;
+  // pre-multiply the gradient cotangent by the reconstruction operator
+  if (GRAD == 0) {
+    const double iv = 1.0 / m.vol[c];
+#pragma unroll
+    for (int v = 0; v < NA; ++v)
+#pragma unroll
+      for (int j = 0; j < D; ++j) bar.G[(int64_t)(v * D + j) * m.C + c] = Gb[v][j] * iv;
+  } else {
+    double M[D][D], I[D][D];
+    lsq_matrix<D>(m, c, M);
+    sym_inverse<D>(M, I);
+#pragma unroll
+    for (int v = 0; v < NA; ++v)
+#pragma unroll
+      for (int p = 0; p < D; ++p) {
+        double s = 0.0;
+#pragma unroll
+        for (int q2 = 0; q2 < D; ++q2) s += I[p][q2] * Gb[v][q2];
+        bar.G[(int64_t)(v * D + p) * m.C + c] = s;
+      }
+  }
+}
+
+// reverse pass G'
+template <int D, int GRAD, bool LIMF, bool LIMS>
+__global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<double> a, const int32_t* __restrict__ ext,
+                                                       RansBars bar) {
+  constexpr int NA = D + 3, NL = D + 2;
+  const int32_t c = xcd_thread_index();
+  if (c >= m.C) return;
+  double xb[NA];
+#pragma unroll
+  for (int v = 0; v < NA; ++v) xb[v] = 0.0;
+  double Bc[NA][D];
+#pragma unroll
+  for (int v = 0; v < NA; ++v)
+#pragma unroll
+    for (int j = 0; j < D; ++j) Bc[v][j] = bar.G[(int64_t)(v * D + j) * m.C + c];
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    if (o >= 0) {
+      if (GRAD == 0) {
+        FaceGeo<D> q;
+        load_face_geo<D>(m, r.f, r.P, r.N, q);
+        const double ws = r.own ? 1.0 - q.g : q.g;
+#pragma unroll
+        for (int v = 0; v < NA; ++v) {
+          double pb_ = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j)
+            pb_ += (q.n[j] * q.A) * (bar.G[(int64_t)(v * D + j) * m.C + r.P] - bar.G[(int64_t)(v * D + j) * m.C + r.N]);
+          xb[v] += ws * pb_;
+        }
+      } else {
+        double dx[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dx[j] = m.cc[(int64_t)j * m.C + o] - m.cc[(int64_t)j * m.C + c];
+        const double wgt = 1.0 / vdot<D>(dx, dx);
+#pragma unroll
+        for (int v = 0; v < NA; ++v) {
+          double own_dot = 0.0, oth_dot = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+            own_dot += dx[j] * Bc[v][j];
+            oth_dot += dx[j] * bar.G[(int64_t)(v * D + j) * m.C + o];
+          }
+          xb[v] += -wgt * own_dot - wgt * oth_dot;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < NA; ++v) {
+        const int l = lim_slot<D>(v, LIMF, LIMS);
+        if (l < 0) continue;
+        if (ext[(int64_t)l * m.C + o] == c) xb[v] += bar.ext[(int64_t)l * m.C + o];
+        if (ext[(int64_t)(NL + l) * m.C + o] == c) xb[v] += bar.ext[(int64_t)(NL + l) * m.C + o];
+      }
+    } else {
+      const int kind = a.kind[r.f];
+      double phib[NA];
+      if (GRAD == 0) {
+        const double A = m.area[r.f];
+#pragma unroll
+        for (int v = 0; v < NA; ++v) {
+          double s = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) s += (m.nrm[(int64_t)j * m.F + r.f] * A) * Bc[v][j];
+          phib[v] = s;
+        }
+      } else {
+        double dx[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dx[j] = m.fc[(int64_t)j * m.F + r.f] - m.cc[(int64_t)j * m.C + c];
+        const double wgt = 1.0 / vdot<D>(dx, dx);
+#pragma unroll
+        for (int v = 0; v < NA; ++v) {
+          double s = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) s += dx[j] * Bc[v][j];
+          phib[v] = wgt * s;
+          xb[v] -= wgt * s;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        if (kind == kFlowWall || kind == kFlowInlet)
+          bar.ub[(int64_t)i * m.F + r.f] += phib[i];
+        else if (kind == kFlowOutlet)
+          xb[i] += phib[i];
+      }
+      if (kind == kFlowWall || kind == kFlowInlet)
+        xb[D] += phib[D];
+      else if (kind == kFlowOutlet)
+        bar.pb[r.f] += phib[D];
+      if (a.kbc[r.f] == kBcZeroGradient) xb[D + 1] += phib[D + 1];
+      if (a.wbc[r.f] == kBcZeroGradient) xb[D + 2] += phib[D + 2];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < NA; ++v) bar.x[(int64_t)v * m.C + c] += xb[v];
+}
+
+// ---------------------------------------------------------------------------------------------
+// host side
+
+int64_t rans_work_doubles(int D, int64_t C, int mode) {
+  const int NA = D + 3, NL = D + 2;
+  int64_t fwd = (int64_t)(NA * D + NL + 7 + 1) * C;  // G, psi, closure(7), aP
+  if (mode == 1) return 2 * fwd;
+  if (mode == 2) return fwd + (int64_t)(NA * D + 1 + NL + 2 * NL + 7) * C + (int64_t)NL * C;  // bars + ext ints
+  return fwd;
+}
+
+struct RWork {
+  double *G, *psi, *aP, *cl;      // cl: 7 closure arrays
+  double *Gt, *psit, *aPt, *clt;  // tangents
+  double *Gb, *aPb, *psib, *extb, *mueffb, *nutb, *f1b, *Srb, *cdb, *taub, *nutwb;
+  int32_t* ext;
+};
+
+RWork rcarve(void* base, int D, int64_t C, int mode) {
+  const int NA = D + 3, NL = D + 2;
+  double* p = (double*)base;
+  RWork w{};
+  auto take = [&](int64_t n) {
+    double* r = p;
+    p += n;
+    return r;
+  };
+  w.G = take((int64_t)NA * D * C);
+  w.psi = take((int64_t)NL * C);
+  w.aP = take(C);
+  w.cl = take(7 * C);
+  if (mode == 1) {
+    w.Gt = take((int64_t)NA * D * C);
+    w.psit = take((int64_t)NL * C);
+    w.aPt = take(C);
+    w.clt = take(7 * C);
+  }
+  if (mode == 2) {
+    w.Gb = take((int64_t)NA * D * C);
+    w.aPb = take(C);
+    w.psib = take((int64_t)NL * C);
+    w.extb = take((int64_t)2 * NL * C);
+    w.mueffb = take(C);
+    w.nutb = take(C);
+    w.f1b = take(C);
+    w.Srb = take(C);
+    w.cdb = take(C);
+    w.taub = take(C);
+    w.nutwb = take(C);
+    w.ext = (int32_t*)take((int64_t)NL * C);
+  }
+  return w;
+}
+
+SstConst make_sst(const afx_rans_config* cfg) {
+  SstConst s;
+  const double* k = cfg->sst;
+  s.a1 = k[0];
+  s.beta_star = k[1];
+  s.alpha1 = k[2];
+  s.beta1 = k[3];
+  s.sk1 = k[4];
+  s.sw1 = k[5];
+  s.alpha2 = k[6];
+  s.beta2 = k[7];
+  s.sk2 = k[8];
+  s.sw2 = k[9];
+  s.kappa = k[10];
+  s.e_wall = k[11];
+  s.p_exp = k[12];
+  s.c_vis = k[13];
+  s.bsq = k[14];
+  s.ylam = k[15];
+  s.density = cfg->density;
+  return s;
+}
+
+template <class S>
+RansArgs<S> rargs(const afx_rans_config* cfg, const afx_rans_inputs* in, const afx_rans_inputs* tan) {
+  RansArgs<S> a{};
+  a.kind = cfg->flow.face_kind;
+  a.kbc = cfg->k_kind;
+  a.wbc = cfg->omega_kind;
+  a.wallf = cfg->wall_face;
+  a.wallc = cfg->wall_cell;
+  a.x.v = in->state;
+  a.ub.v = in->u_b;
+  a.pb.v = in->p_b;
+  a.mu.v = in->mu;
+  a.rho.v = in->rho;
+  a.nu.v = in->nu;
+  a.kval.v = in->k_value;
+  a.wval.v = in->omega_value;
+  if (tan) {
+    a.x.t = tan->state;
+    a.ub.t = tan->u_b;
+    a.pb.t = tan->p_b;
+    a.mu.t = tan->mu;
+    a.rho.t = tan->rho;
+    a.nu.t = tan->nu;
+    a.kval.t = tan->k_value;
+    a.wval.t = tan->omega_value;
+  }
+  a.dwall = cfg->wall_distance;
+  a.dgrad = cfg->wall_distance_gradient;
+  a.pin = cfg->flow.pressure_pin >= 0 ? (int32_t)cfg->flow.pressure_pin : -1;
+  a.pin_value = cfg->flow.pressure_pin_value;
+  for (int i = 0; i < 3; ++i) a.bf[i] = cfg->flow.body_force[i];
+  a.lim_k = cfg->flow.limiter_k;
+  a.sc = make_sst(cfg);
+  return a;
+}
+
+template <class S>
+void bind_closure(RansArgs<S>& a, double* cl, double* clt, int64_t C) {
+  In<S>* dst[7] = {&a.nut, &a.f1, &a.Sr, &a.cd, &a.nutw, &a.tau, &a.mueff};
+  for (int i = 0; i < 7; ++i) {
+    dst[i]->v = cl + i * C;
+    dst[i]->t = clt ? clt + i * C : nullptr;
+  }
+}
+
+template <int D, class S, int ADVF, int ADVS, int GRAD>
+int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S> R, const RWork& w, double* Gt,
+                 double* psit, double* aPt, double* clt, int32_t* ext, hipStream_t s) {
+  MeshView m = afx_view(mh, g);
+  const int64_t C = m.C;
+  Out<S> oG{w.G, Gt}, oP{w.psi, psit};
+  constexpr bool LF = ADVF == kAdvLimited, LS = ADVS == kAdvLimited;
+  AFX_LAUNCH(kKFlowGrad, (k_rans_grad<D, S, GRAD, LF, LS>), grid_for(C), dim3(kBlock), 0, s, m, a, oG, oP, ext);
+  AFX_CHECK_LAUNCH();
+  a.G.v = w.G;
+  a.G.t = Gt;
+  a.psi.v = w.psi;
+  a.psi.t = psit;
+  ClosureOut co{};
+  double** vv[7] = {&co.nut, &co.f1, &co.Sr, &co.cd, &co.nutw, &co.tau, &co.mueff};
+  double** tt[7] = {&co.nut_t, &co.f1_t, &co.Sr_t, &co.cd_t, &co.nutw_t, &co.tau_t, &co.mueff_t};
+  for (int i = 0; i < 7; ++i) {
+    *vv[i] = w.cl + i * C;
+    *tt[i] = clt ? clt + i * C : nullptr;
+  }
+  AFX_LAUNCH(kKSstClosure, (k_rans_closure<D, S>), grid_for(C), dim3(kBlock), 0, s, m, a, co);
+  AFX_CHECK_LAUNCH();
+  bind_closure<S>(a, w.cl, clt, C);
+  AFX_LAUNCH(kKFlowAp, (k_rans_ap<D, S, ADVF>), grid_for(C), dim3(kBlock), 0, s, m, a, Out<S>{w.aP, aPt});
+  AFX_CHECK_LAUNCH();
+  if (R.v == nullptr) return 0;
+  a.aP.v = w.aP;
+  a.aP.t = aPt;
+  AFX_LAUNCH(kKSstRes, (k_rans_res<D, S, ADVF, ADVS>), grid_for(C), dim3(kBlock), 0, s, m, a, R);
+  AFX_CHECK_LAUNCH();
+  return 0;
+}
+
+template <class Fn>
+int rdispatch(int D, int advf, int advs, int grad, Fn&& fn) {
+#define AFX_R(DD, AF, AS, GG)                                                                            \
+  if (D == DD && advf == AF && advs == AS && grad == GG)                                                 \
+    return fn(std::integral_constant<int, DD>{}, std::integral_constant<int, AF>{},                       \
+              std::integral_constant<int, AS>{}, std::integral_constant<int, GG>{});
+  AFX_R(2, 1, 1, 0) AFX_R(2, 2, 2, 0) AFX_R(2, 1, 1, 1) AFX_R(2, 2, 2, 1) AFX_R(2, 1, 2, 1) AFX_R(2, 2, 1, 1)
+  AFX_R(3, 1, 1, 0) AFX_R(3, 2, 2, 0) AFX_R(3, 1, 1, 1) AFX_R(3, 2, 2, 1) AFX_R(3, 1, 2, 1) AFX_R(3, 2, 1, 1)
+#undef AFX_R
+  afx_set_error("rans: unsupported (dim, advection, scalar advection, gradient) combination");
+  return 2;
+}
+
+int rcheck(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg, const afx_rans_inputs* in,
+           void* work, int64_t work_size, int mode) {
+  AFX_REQUIRE(m && g && cfg && in, "rans: null argument");
+  AFX_REQUIRE(in->state && in->mu && in->rho && in->nu && in->u_b && in->p_b && in->k_value && in->omega_value,
+              "rans: a required input pointer is null");
+  AFX_REQUIRE(cfg->flow.face_kind && cfg->k_kind && cfg->omega_kind && cfg->wall_face && cfg->wall_cell &&
+                  cfg->wall_distance && cfg->wall_distance_gradient,
+              "rans: a required configuration pointer is null");
+  AFX_REQUIRE(cfg->density > 0.0, "rans: density must be positive");
+  AFX_REQUIRE(work && work_size >= rans_work_doubles(m->dim, m->C, mode) * 8, "rans: workspace too small");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int afx_rans_workspace_bytes(const afx_mesh* m, int mode, int64_t* bytes) {
+  AFX_REQUIRE(m && bytes && mode >= 0 && mode <= 2, "afx_rans_workspace_bytes: bad argument");
+  *bytes = rans_work_doubles(m->dim, m->C, mode) * 8 + 256;
+  return 0;
+}
+
+extern "C" int afx_rans_residual(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg,
+                                 const afx_rans_inputs* in, double* residual, void* work, int64_t work_size,
+                                 hipStream_t s) {
+  int rc = rcheck(m, g, cfg, in, work, work_size, 0);
+  if (rc) return rc;
+  AFX_REQUIRE(residual, "afx_rans_residual: null residual");
+  RWork w = rcarve(work, m->dim, m->C, 0);
+  RansArgs<double> a = rargs<double>(cfg, in, nullptr);
+  return rdispatch(m->dim, cfg->flow.advection, cfg->scalar_advection, cfg->flow.gradient,
+                   [&](auto Dc, auto Fc, auto Sc, auto Gc) {
+                     return rans_forward<decltype(Dc)::value, double, decltype(Fc)::value, decltype(Sc)::value,
+                                         decltype(Gc)::value>(m, g, a, Out<double>{residual, nullptr}, w, nullptr,
+                                                              nullptr, nullptr, nullptr, nullptr, s);
+                   });
+}
+
+extern "C" int afx_rans_jvp(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg,
+                            const afx_rans_inputs* in, const afx_rans_inputs* tangent, double* residual,
+                            double* residual_tangent, void* work, int64_t work_size, hipStream_t s) {
+  int rc = rcheck(m, g, cfg, in, work, work_size, 1);
+  if (rc) return rc;
+  AFX_REQUIRE(tangent && residual && residual_tangent, "afx_rans_jvp: null argument");
+  RWork w = rcarve(work, m->dim, m->C, 1);
+  RansArgs<dual> a = rargs<dual>(cfg, in, tangent);
+  return rdispatch(m->dim, cfg->flow.advection, cfg->scalar_advection, cfg->flow.gradient,
+                   [&](auto Dc, auto Fc, auto Sc, auto Gc) {
+                     return rans_forward<decltype(Dc)::value, dual, decltype(Fc)::value, decltype(Sc)::value,
+                                         decltype(Gc)::value>(m, g, a, Out<dual>{residual, residual_tangent}, w,
+                                                              w.Gt, w.psit, w.aPt, w.clt, nullptr, s);
+                   });
+}
+
+extern "C" int afx_rans_vjp(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg,
+                            const afx_rans_inputs* in, const double* cotangent, afx_rans_cotangents* out, void* work,
+                            int64_t work_size, hipStream_t s) {
+  int rc = rcheck(m, g, cfg, in, work, work_size, 2);
+  if (rc) return rc;
+  AFX_REQUIRE(cotangent && out && out->state && out->mu && out->rho && out->nu && out->u_b && out->p_b,
+              "afx_rans_vjp: null cotangent argument");
+  RWork w = rcarve(work, m->dim, m->C, 2);
+  RansArgs<double> a = rargs<double>(cfg, in, nullptr);
+  RansBars bar{out->state, out->mu,  out->rho, out->nu, out->u_b, out->p_b, w.Gb,  w.aPb, w.psib,
+               w.extb,     w.mueffb, w.nutb,   w.f1b,   w.Srb,    w.cdb,    w.taub, w.nutwb};
+  return rdispatch(m->dim, cfg->flow.advection, cfg->scalar_advection, cfg->flow.gradient,
+                   [&](auto Dc, auto Fc, auto Sc, auto Gc) {
+                     constexpr int D = decltype(Dc)::value, AF = decltype(Fc)::value, AS = decltype(Sc)::value,
+                                   GR = decltype(Gc)::value;
+                     constexpr bool LF = AF == kAdvLimited, LS = AS == kAdvLimited;
+                     int r2 = rans_forward<D, double, AF, AS, GR>(m, g, a, Out<double>{nullptr, nullptr}, w, nullptr,
+                                                                  nullptr, nullptr, nullptr, w.ext, s);
+                     if (r2) return r2;
+                     MeshView v = afx_view(m, g);
+                     RansArgs<double> b = a;
+                     b.G.v = w.G;
+                     b.psi.v = w.psi;
+                     b.aP.v = w.aP;
+                     bind_closure<double>(b, w.cl, nullptr, m->C);
+                     AFX_LAUNCH(kKSstResAdj, (k_rans_res_adj<D, AF, AS>), grid_for(m->C), dim3(kBlock), 0, s, v, b,
+                                cotangent, bar);
+                     AFX_CHECK_LAUNCH();
+                     AFX_LAUNCH(kKFlowApAdj, (k_rans_ap_adj<D, AF>), grid_for(m->C), dim3(kBlock), 0, s, v, b, bar);
+                     AFX_CHECK_LAUNCH();
+                     AFX_LAUNCH(kKSstClosureAdj, (k_rans_closure_adj<D, AF, AS, GR>), grid_for(m->C), dim3(kBlock), 0,
+                                s, v, b, w.ext, bar);
+                     AFX_CHECK_LAUNCH();
+                     AFX_LAUNCH(kKFlowGradAdj, (k_rans_grad_adj<D, GR, LF, LS>), grid_for(m->C), dim3(kBlock), 0, s,
+                                v, b, w.ext, bar);
+                     AFX_CHECK_LAUNCH();
+                     return 0;
+                   });
+}
+
+extern "C" int afx_rans_fields(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg,
+                               const afx_rans_inputs* in, double* fields, hipStream_t s) {
+  AFX_REQUIRE(m && g && cfg && in && fields, "afx_rans_fields: null argument");
+  RWork w = rcarve(fields, m->dim, m->C, 0);
+  RansArgs<double> a = rargs<double>(cfg, in, nullptr);
+  return rdispatch(m->dim, cfg->flow.advection, cfg->scalar_advection, cfg->flow.gradient,
+                   [&](auto Dc, auto Fc, auto Sc, auto Gc) {
+                     return rans_forward<decltype(Dc)::value, double, decltype(Fc)::value, decltype(Sc)::value,
+                                         decltype(Gc)::value>(m, g, a, Out<double>{nullptr, nullptr}, w, nullptr,
+                                                              nullptr, nullptr, nullptr, nullptr, s);
+                   });
+}

@@ -1,0 +1,180 @@
+// k-omega SST closure, wall treatment and sources as cell-local device functions, templated on the
+// scalar type (so `dual` gives tangents; the adjoints seed them one input at a time).
+//
+// Reference (file:line): SSTModel.f1 / f2 / eddy_viscosity / blend (turbulence/sst.py:84-239);
+// safe_sqrt, strain_rate_magnitude (turbulence/strain.py:32-66); omega_wall, _power_mean, nut_wall,
+// wall_y_star, wall_function_weight, log_layer_shear_rate, k_wall_production, omega_wall_gradient
+// (turbulence/boundary.py:45-589); KProduction, KDestruction, NearWallKClosure, OmegaProduction,
+// OmegaDestruction, OmegaCrossDiffusion (turbulence/sources.py:60-412); SSTTurbulence.strain_rate,
+// closure_fields, k_residual, omega_residual (turbulence/transport.py:369-718).
+#pragma once
+
+#include "afx_common.h"
+
+namespace afx {
+
+struct SstConst {
+  double a1, beta_star, alpha1, beta1, sk1, sw1, alpha2, beta2, sk2, sw2, kappa, e_wall, p_exp, c_vis;
+  double bsq;   // beta_star ** 0.25
+  double ylam;  // laminar / log crossover y*_lam
+  double density;
+};
+
+constexpr double kCrossDiffusionFloor = 1e-10;
+constexpr double kProductionLimitRatio = 10.0;
+constexpr double kEddyViscosityFloor = 1e-30;
+
+// safe_sqrt: zero value and zero derivative at a non-positive argument.
+template <class S>
+AFX_DEV S safe_sqrt(S x) {
+  return val(x) > 0.0 ? dsqrt(x) : sconst<S>(0.0);
+}
+
+template <class S>
+AFX_DEV S kpos(S k) {
+  return dmax(k, 0.0);
+}
+
+template <class S>
+AFX_DEV S sst_blend(S f1, double inner, double outer) {
+  return f1 * inner + (1.0 - f1) * outer;
+}
+
+template <class S>
+AFX_DEV S wall_y_star(const SstConst& c, S nu, double d, S k) {
+  return c.bsq * safe_sqrt(kpos(k)) * d / nu;
+}
+
+template <class S>
+AFX_DEV S wall_function_weight(const SstConst& c, S nu, double d, S k) {
+  S r = wall_y_star(c, nu, d, k) / c.ylam;
+  S r2 = r * r;
+  return dtanh(r2 * r2);
+}
+
+template <class S>
+AFX_DEV S log_layer_shear_rate(const SstConst& c, double d, S k) {
+  return c.bsq * safe_sqrt(kpos(k)) / (c.kappa * d);
+}
+
+template <class S>
+AFX_DEV S nut_wall(const SstConst& c, S nu, double d, S k) {
+  S ys = wall_y_star(c, nu, d, k);
+  S arg = c.e_wall * dmax(ys, c.ylam);
+  S r = ys * c.kappa / dlog(arg) - 1.0;
+  return val(ys) > c.ylam ? nu * r : sconst<S>(0.0);
+}
+
+template <class S>
+AFX_DEV S pow_p(S x, double p) {
+  if (p == 2.0) return x * x;
+  return dexp(p * dlog(x));
+}
+
+// omega_wall and its partial derivatives d/dd, d/dk (the imposed wall gradient of omega).
+template <class S>
+AFX_DEV S omega_wall(const SstConst& c, S nu, S d, S k, S* dodd = nullptr, S* dodk = nullptr) {
+  const double p = c.p_exp;
+  S a = c.c_vis * (6.0 * nu / (c.beta1 * (d * d)));
+  S sk = safe_sqrt(kpos(k));
+  S b = sk / (c.bsq * c.kappa * d);
+  S m = dmax(a, b);
+  S ra = a / m, rb = b / m;
+  S bt = val(rb) > 0.0 ? pow_p(rb, p) : sconst<S>(0.0);
+  S T = pow_p(ra, p) + bt;
+  S om = p == 2.0 ? m * dsqrt(T) : m * dexp(dlog(T) / p);
+  if (dodd) {
+    // (a/om)^(p-1) da + (b/om)^(p-1) db, with da/dd = -2a/d, db/dd = -b/d, db/dk = b/(2k) (k > 0)
+    S wa = p == 2.0 ? a / om : dexp((p - 1.0) * dlog(a / om));
+    S wb = val(b) > 0.0 ? (p == 2.0 ? b / om : dexp((p - 1.0) * dlog(b / om))) : sconst<S>(0.0);
+    *dodd = wa * (-2.0 * a / d) + wb * (-b / d);
+    S dbdk = val(k) > 0.0 ? 0.5 / (sk * (c.bsq * c.kappa * d)) : sconst<S>(0.0);
+    *dodk = wb * dbdk;
+  }
+  return om;
+}
+
+template <class S>
+AFX_DEV S omega_wall_k(const SstConst& c, S nu, double d, S k) {
+  return omega_wall<S>(c, nu, sconst<S>(d), k);
+}
+
+// Outputs of the cell-local closure.
+template <class S>
+struct SstClosure {
+  S nut, f1, strain, cd, nutw;
+};
+
+// Closure fields of one cell (transport.py:369-546): strain rate (with the wall log-layer blend),
+// F1 from the (wall-imposed) omega gradient, eddy viscosity with the F2 limiter, the cross-diffusion
+// value 2 (1 - F1) sigma_w2 grad k . grad w / w, and the wall-function eddy viscosity of its wall faces.
+template <int D, class S>
+AFX_DEV SstClosure<S> sst_closure(const SstConst& c, S k, S w, const S (&Gu)[D][D], const S (&Gk)[D],
+                                  const S (&Gw_rec)[D], S nu, double d, const double (&dgrad)[D], bool wall) {
+  SstClosure<S> o;
+  // strain magnitude sqrt(2 S_ij S_ij)
+  S ss = sconst<S>(0.0);
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      S sij = 0.5 * (Gu[i][j] + Gu[j][i]);
+      ss = ss + sij * sij;
+    }
+  S strain = safe_sqrt(2.0 * ss);
+  S Gw[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) Gw[j] = Gw_rec[j];
+  if (wall) {
+    S wgt = wall_function_weight(c, nu, d, k);
+    strain = (1.0 - wgt) * strain + wgt * log_layer_shear_rate(c, d, k);
+    S dodd, dodk;
+    omega_wall<S>(c, nu, sconst<S>(d), k, &dodd, &dodk);
+#pragma unroll
+    for (int j = 0; j < D; ++j) Gw[j] = dodd * dgrad[j] + dodk * Gk[j];
+  }
+  S kw = Gk[0] * Gw[0];
+#pragma unroll
+  for (int j = 1; j < D; ++j) kw = kw + Gk[j] * Gw[j];
+  // F1 (sst.py:152-195)
+  S cdk = dmax(2.0 * c.sw2 * kw / w, kCrossDiffusionFloor);
+  S pk = kpos(k);
+  S sq = safe_sqrt(pk);
+  S arg1 = dmin(dmax(sq / (c.beta_star * w * d), 500.0 * nu / ((d * d) * w)), 4.0 * c.sw2 * pk / (cdk * (d * d)));
+  S a2 = arg1 * arg1;
+  o.f1 = dtanh(a2 * a2);
+  // F2 and eddy viscosity (sst.py:125-150, 197-239)
+  S arg2 = dmax(2.0 * sq / (c.beta_star * w * d), 500.0 * nu / ((d * d) * w));
+  S f2 = dtanh(arg2 * arg2);
+  o.nut = c.a1 * pk / dmax(c.a1 * w, strain * f2);
+  o.strain = strain;
+  o.cd = 2.0 * (1.0 - o.f1) * c.sw2 * kw / w;
+  o.nutw = wall ? nut_wall(c, nu, d, k) : sconst<S>(0.0);
+  return o;
+}
+
+// Volume-integrated sources of one cell: out_k = KProduction + KDestruction, out_w = OmegaProduction
+// + OmegaDestruction + OmegaCrossDiffusion (each production positive; the residual subtracts them
+// in this order).  ``tau`` is the wall shear rate of a wall cell.
+template <class S>
+AFX_DEV void sst_sources(const SstConst& c, S k, S w, S nut, S strain, S f1, S cd, S tau, S nu, double d,
+                         bool wall, double V, S& prod_k, S& dest_k, S& prod_w, S& dest_w, S& cross_w) {
+  S limited = dmin(nut * (strain * strain), kProductionLimitRatio * c.beta_star * kpos(k) * w);
+  S wdis = w;
+  if (wall) {
+    S lw = wall_function_weight(c, nu, d, k);
+    S wv = (nu + nut_wall(c, nu, d, k)) * tau * log_layer_shear_rate(c, d, k);
+    limited = (1.0 - lw) * limited + lw * wv;
+    wdis = omega_wall_k(c, nu, d, k);
+  }
+  prod_k = limited * V;
+  dest_k = -c.beta_star * k * wdis * V;
+  S alpha = sst_blend(f1, c.alpha1, c.alpha2);
+  S cap = kProductionLimitRatio * c.beta_star * kpos(k) * w / dmax(nut, kEddyViscosityFloor);
+  prod_w = alpha * dmin(strain * strain, cap) * V;
+  S beta = sst_blend(f1, c.beta1, c.beta2);
+  dest_w = -beta * (w * w) * V;
+  cross_w = cd * V;
+}
+
+}  // namespace afx

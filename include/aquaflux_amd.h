@@ -137,6 +137,87 @@ int afx_flow_fields(const afx_mesh* mesh, const afx_geometry* geom, const afx_fl
                     const afx_flow_inputs* in, double* gradients, double* a_p, double* psi, hipStream_t stream);
 
 /* ------------------------------------------------------------------------------------------- */
+/* scalar fields                                                                                 */
+
+/* Cell gradient grad[j][C] of phi by `gradient` (AFX_GRAD_*) with leading-order boundary values:
+ * kind[f] == AFX_BC_DIRICHLET -> value[f], any other closure -> the owner value.  Replaces
+ * ResidualAssembler.gradient for Dirichlet / zero-gradient closures (reference:
+ * aquaflux/discretization/residual.py:253-296), e.g. the SST wall-distance gradient
+ * (aquaflux/turbulence/transport.py:86-110). */
+int afx_scalar_gradient(const afx_mesh* mesh, const afx_geometry* geom, int gradient, const int8_t* kind,
+                        const double* value, const double* phi, double* grad, hipStream_t stream);
+
+/* ------------------------------------------------------------------------------------------- */
+/* coupled RANS (u, v[, w], p, k, omega) residual with the k-omega SST closure                   */
+
+enum { AFX_BC_NONE = 0, AFX_BC_DIRICHLET = 1, AFX_BC_ZERO_GRADIENT = 2, AFX_BC_NEUMANN = 3, AFX_BC_CONVECTIVE = 4 };
+
+typedef struct {
+  afx_flow_config flow;               /* momentum / continuity configuration */
+  int32_t scalar_advection;           /* AFX_ADV_* of the k and omega equations */
+  int32_t reserved;
+  double density;                     /* the SST assembler's density (k/omega volume flux mdot / density) */
+  double sst[16];                     /* a1, beta*, alpha1, beta1, sigma_k1, sigma_w1, alpha2, beta2,
+                                         sigma_k2, sigma_w2, kappa, E, wall-omega exponent,
+                                         wall-omega viscous coeff, beta*^0.25, y*_lam */
+  const int8_t* k_kind;               /* [F] AFX_BC_DIRICHLET / AFX_BC_ZERO_GRADIENT on boundary faces */
+  const int8_t* omega_kind;           /* [F] */
+  const int8_t* wall_face;            /* [F] 1 on the SST wall-patch faces */
+  const int8_t* wall_cell;            /* [C] 1 on their owner cells (the omega fixation set) */
+  const double* wall_distance;        /* [C] */
+  const double* wall_distance_gradient; /* [d][C] */
+} afx_rans_config;
+
+typedef struct {
+  const double* state;       /* [(d+3)][C] field-major u, v, (w), p, k, omega */
+  const double* mu;          /* [C] molecular dynamic viscosity */
+  const double* rho;         /* [C] density */
+  const double* nu;          /* [C] molecular kinematic viscosity of the SST assembler */
+  const double* u_b;         /* [d][F] */
+  const double* p_b;         /* [F] */
+  const double* k_value;     /* [F] Dirichlet values of k */
+  const double* omega_value; /* [F] Dirichlet values of omega */
+} afx_rans_inputs;
+
+typedef struct {
+  double* state; /* [(d+3)][C] */
+  double* mu;    /* [C] */
+  double* rho;   /* [C] */
+  double* nu;    /* [C] */
+  double* u_b;   /* [d][F] */
+  double* p_b;   /* [F] */
+} afx_rans_cotangents;
+
+/* Workspace (device bytes) for mode 0 = residual, 1 = JVP, 2 = VJP. */
+int afx_rans_workspace_bytes(const afx_mesh* mesh, int mode, int64_t* bytes);
+
+/* R(u, p, k, omega).  Replaces CoupledRANS.residual with DirectScalars (reference:
+ * aquaflux/turbulence/coupled.py:437-473, closure_fields / k_residual / omega_residual in
+ * aquaflux/turbulence/transport.py:548-718). */
+int afx_rans_residual(const afx_mesh* mesh, const afx_geometry* geom, const afx_rans_config* cfg,
+                      const afx_rans_inputs* in, double* residual, void* work, int64_t work_bytes,
+                      hipStream_t stream);
+
+/* R and J . tangent (tangent pointers may be NULL = zero).  Replaces jax.jvp of CoupledRANS.residual
+ * (reference: aquaflux/solve/newton.py:108-121). */
+int afx_rans_jvp(const afx_mesh* mesh, const afx_geometry* geom, const afx_rans_config* cfg,
+                 const afx_rans_inputs* in, const afx_rans_inputs* tangent, double* residual,
+                 double* residual_tangent, void* work, int64_t work_bytes, hipStream_t stream);
+
+/* J^T . cotangent with respect to state, mu, rho, nu, u_b, p_b.  Replaces jax.vjp of
+ * CoupledRANS.residual (reference: aquaflux/solve/implicit.py:733-761). */
+int afx_rans_vjp(const afx_mesh* mesh, const afx_geometry* geom, const afx_rans_config* cfg,
+                 const afx_rans_inputs* in, const double* cotangent, afx_rans_cotangents* out, void* work,
+                 int64_t work_bytes, hipStream_t stream);
+
+/* Forward intermediates into `fields` (the mode-0 workspace layout): gradients [(d+3)d][C],
+ * limiters [d+2][C], a_P [C], closure [7][C] = nu_t, F1, strain rate, cross-diffusion,
+ * wall-function nu_t, wall shear rate, mu_eff.  Replaces SSTTurbulence.closure_fields (reference:
+ * aquaflux/turbulence/transport.py:548-579). */
+int afx_rans_fields(const afx_mesh* mesh, const afx_geometry* geom, const afx_rans_config* cfg,
+                    const afx_rans_inputs* in, double* fields, hipStream_t stream);
+
+/* ------------------------------------------------------------------------------------------- */
 /* Newton-Krylov inner operators: compact block Jacobian, SpMV, point-block Jacobi              */
 
 /* Block-ELL slot count per row (1 + max faces per cell). */

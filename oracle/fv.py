@@ -94,6 +94,24 @@ class OMesh:
         return torch.where(self.interior, self.neighbour, torch.full_like(self.neighbour, self.n_cells))
 
 
+# --- max / min with JAX's derivative rule ---------------------------------------------------
+
+
+def maximum(a, b):
+    """``jnp.maximum``: value ``max(a, b)``; the derivative takes the larger operand's (half of each
+    on a tie).  Written with ``torch.where`` because torch's own forward-mode rule for
+    ``torch.maximum`` (``t_b + mask * (t_a - t_b)``) loses the selected tangent to cancellation when
+    the other operand's tangent is many orders larger, which JAX's per-operand weighting does not."""
+    a, b = torch.broadcast_tensors(torch.as_tensor(a, dtype=F64), torch.as_tensor(b, dtype=F64))
+    return torch.where(a > b, a, torch.where(a < b, b, 0.5 * (a + b)))
+
+
+def minimum(a, b):
+    """``jnp.minimum`` with the same derivative rule as :func:`maximum`."""
+    a, b = torch.broadcast_tensors(torch.as_tensor(a, dtype=F64), torch.as_tensor(b, dtype=F64))
+    return torch.where(a < b, a, torch.where(a > b, b, 0.5 * (a + b)))
+
+
 # --- scatter primitives (aquaflux/mesh/connectivity.py:236-342) ------------------------------
 
 
@@ -289,7 +307,7 @@ def venkatakrishnan(m: OMesh, g: OGeometry, phi, grad, k=5.0):
 
     psi = scatter_min(m, face_limiter(own), face_limiter(nb))
     # jnp.clip(x, 0, 1) == minimum(maximum(x, 0), 1): ties split the derivative
-    return torch.minimum(torch.maximum(psi, torch.zeros_like(psi)), torch.ones_like(psi))
+    return minimum(maximum(psi, 0.0), 1.0)
 
 
 # --- face fluxes (aquaflux/discretization) --------------------------------------------------------
@@ -430,15 +448,15 @@ def flow_fields(m, g, setup: FlowSetup, state, mu, rho, nu_t=None, wall_nu_t=Non
         mdot_est = advective_momentum_flux(m, g, vel, rho, fac, grad_vel) * g.area
     k = setup.kind
     visc_b = bvisc * g.area / dist
-    conv_b = torch.maximum(mdot_est, torch.zeros_like(mdot_est)) if mdot_est is not None else torch.zeros(m.n_faces, dtype=F64)
+    conv_b = maximum(mdot_est, 0.0) if mdot_est is not None else torch.zeros(m.n_faces, dtype=F64)
     bcoef = torch.where(k == FLOW_WALL, visc_b, torch.zeros_like(visc_b))
     bcoef = torch.where(k == FLOW_INLET, visc_b + conv_b, bcoef)
     bcoef = torch.where(k == FLOW_OUTLET, conv_b, bcoef)
     visc = conductance(m, g, mu_eff)
     oc, nc = visc, visc
     if mdot_est is not None:
-        oc = oc + torch.maximum(mdot_est, torch.zeros_like(mdot_est))
-        nc = nc + torch.maximum(-mdot_est, torch.zeros_like(mdot_est))
+        oc = oc + maximum(mdot_est, 0.0)
+        nc = nc + maximum(-mdot_est, 0.0)
     oc = torch.where(m.interior, oc, bcoef)
     a_p = scatter(m, oc, nc)
     if frozen is not None:
@@ -545,8 +563,12 @@ def scalar_boundary_values(m, g, s: ScalarSetup, phi, grad, gamma):
     out = torch.zeros(m.n_faces, dtype=F64)
     out = torch.where(k == BC_DIRICHLET, s.value, out)
     out = torch.where(k == BC_ZERO_GRADIENT, phi[own] + corr, out)
-    out = torch.where(k == BC_NEUMANN, phi[own] + corr - s.value / gamma[own] * dn, out)
-    beta = s.value / gamma[own] * dn
+    # the Neumann / Convective branches divide by the owner diffusivity; faces of other kinds read a
+    # unit stand-in there so that no unselected 0/0 reaches the reverse pass of torch.where
+    robin = (k == BC_NEUMANN) | (k == BC_CONVECTIVE)
+    gam = torch.where(robin, gamma[own], torch.ones_like(gamma[own]))
+    out = torch.where(k == BC_NEUMANN, phi[own] + corr - s.value / gam * dn, out)
+    beta = s.value / gam * dn
     out = torch.where(k == BC_CONVECTIVE, (phi[own] + corr + beta * s.t_inf) / (1.0 + beta), out)
     return out
 

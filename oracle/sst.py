@@ -59,28 +59,28 @@ class SSTModel:
 
     def f2(self, k, omega, nu, d):
         """``sst.py:125-150``."""
-        arg2 = torch.maximum(
-            2.0 * safe_sqrt(torch.maximum(k, torch.zeros_like(k))) / (self.beta_star * omega * d),
+        arg2 = fv.maximum(
+            2.0 * safe_sqrt(fv.maximum(k, 0.0)) / (self.beta_star * omega * d),
             500.0 * nu / (d**2 * omega),
         )
         return torch.tanh(arg2**2)
 
     def f1(self, k, omega, nu, d, grad_k, grad_omega):
         """``sst.py:152-195``."""
-        cd = torch.maximum(
+        cd = fv.maximum(
             2.0 * self.sigma_omega2 * fv.dot(grad_k, grad_omega) / omega,
             torch.full_like(omega, CROSS_DIFFUSION_FLOOR),
         )
-        pk = torch.maximum(k, torch.zeros_like(k))
-        arg1 = torch.minimum(
-            torch.maximum(safe_sqrt(pk) / (self.beta_star * omega * d), 500.0 * nu / (d**2 * omega)),
+        pk = fv.maximum(k, 0.0)
+        arg1 = fv.minimum(
+            fv.maximum(safe_sqrt(pk) / (self.beta_star * omega * d), 500.0 * nu / (d**2 * omega)),
             4.0 * self.sigma_omega2 * pk / (cd * d**2),
         )
         return torch.tanh(arg1**4)
 
     def eddy_viscosity(self, k, omega, strain, nu, d):
         """``sst.py:197-239``: ``a1 max(k,0) / max(a1 omega, S F2)``."""
-        return self.a1 * torch.maximum(k, torch.zeros_like(k)) / torch.maximum(
+        return self.a1 * fv.maximum(k, 0.0) / fv.maximum(
             self.a1 * omega, strain * self.f2(k, omega, nu, d)
         )
 
@@ -105,7 +105,7 @@ def omega_wall_value(nu, d, model):
 
 
 def _power_mean(a, b, p):
-    m = torch.maximum(a, b)
+    m = fv.maximum(a, b)
     ra, rb = a / m, b / m
     pos = rb > 0.0
     rbs = torch.where(pos, rb, torch.ones_like(rb))
@@ -115,18 +115,18 @@ def _power_mean(a, b, p):
 
 def omega_wall(nu, d, k, model):
     vis = model.wall_omega_viscous_coeff * omega_wall_value(nu, d, model)
-    log = safe_sqrt(torch.maximum(k, torch.zeros_like(k))) / (model.beta_star**0.25 * model.kappa * d)
+    log = safe_sqrt(fv.maximum(k, 0.0)) / (model.beta_star**0.25 * model.kappa * d)
     return _power_mean(vis, log, model.wall_omega_exponent)
 
 
 def wall_y_star(nu, d, k, model):
-    return model.beta_star**0.25 * safe_sqrt(torch.maximum(k, torch.zeros_like(k))) * d / nu
+    return model.beta_star**0.25 * safe_sqrt(fv.maximum(k, 0.0)) * d / nu
 
 
 def nut_wall(nu, d, k, model):
     ys = wall_y_star(nu, d, k, model)
     yl = model.wall_y_star_lam
-    arg = model.e_wall * torch.maximum(ys, torch.full_like(ys, yl))
+    arg = model.e_wall * fv.maximum(ys, yl)
     r = ys * model.kappa / torch.log(arg) - 1.0
     return nu * torch.where(ys > yl, r, torch.zeros_like(r))
 
@@ -136,7 +136,7 @@ def wall_function_weight(nu, d, k, model):
 
 
 def log_layer_shear_rate(d, k, model):
-    return model.beta_star**0.25 * safe_sqrt(torch.maximum(k, torch.zeros_like(k))) / (model.kappa * d)
+    return model.beta_star**0.25 * safe_sqrt(fv.maximum(k, 0.0)) / (model.kappa * d)
 
 
 def k_wall_production(nu, d, k, shear_rate, model):
@@ -275,8 +275,8 @@ def rans_residual(m, g, s: fv.FlowSetup, t: SSTSetup, state, mu, rho, scalar_adv
     face_gamma[t.wall_faces] = (1.0 - wall_function_weight(t.nu[wo], dw[wo], k[wo], model)) * face_gamma[t.wall_faces]
     Rk = _scalar_balance(m, g, s.gradient, t.k_bc, k, Q, gk, face_gamma, scalar_advection, s.limiter_k)
     prod = cl["nu_t"] * cl["strain"] ** 2
-    limit = PRODUCTION_LIMIT_RATIO * model.beta_star * torch.maximum(k, torch.zeros_like(k)) * omega
-    limited = torch.minimum(prod, limit).clone()
+    limit = PRODUCTION_LIMIT_RATIO * model.beta_star * fv.maximum(k, 0.0) * omega
+    limited = fv.minimum(prod, limit).clone()
     kw = k[wall]
     lw = wall_function_weight(t.nu[wall], dw[wall], kw, model)
     wv = k_wall_production(t.nu[wall], dw[wall], kw, cl["tau"], model)
@@ -289,9 +289,9 @@ def rans_residual(m, g, s: fv.FlowSetup, t: SSTSetup, state, mu, rho, scalar_adv
     gw = t.nu + model.blend(cl["f1"], model.sigma_omega1, model.sigma_omega2) * cl["nu_t"]
     Rw = _scalar_balance(m, g, s.gradient, t.omega_bc, omega, Q, gw, None, scalar_advection, s.limiter_k)
     alpha = model.blend(cl["f1"], model.alpha_1, model.alpha_2)
-    cap = (PRODUCTION_LIMIT_RATIO * model.beta_star * torch.maximum(k, torch.zeros_like(k)) * omega
-           / torch.maximum(cl["nu_t"], torch.full_like(k, EDDY_VISCOSITY_FLOOR)))
-    Rw = Rw - alpha * torch.minimum(cl["strain"] ** 2, cap) * V
+    cap = (PRODUCTION_LIMIT_RATIO * model.beta_star * fv.maximum(k, 0.0) * omega
+           / fv.maximum(cl["nu_t"], EDDY_VISCOSITY_FLOOR))
+    Rw = Rw - alpha * fv.minimum(cl["strain"] ** 2, cap) * V
     beta = model.blend(cl["f1"], model.beta_1, model.beta_2)
     Rw = Rw - (-beta * omega**2 * V)
     cross = 2.0 * (1.0 - cl["f1"]) * model.sigma_omega2 * fv.dot(cl["grad_k"], cl["grad_omega"]) / omega

@@ -118,3 +118,52 @@ def test_rans_jvp_matches_finite_difference():
     fd = (f(state + eps * v) - f(state - eps * v)) / (2 * eps)
     assert float((fd - jv).norm() / jv.norm()) < 1e-6
     assert bool(torch.isfinite(f(state)).all())
+
+
+def test_rans_jvp_is_additive_and_matches_richardson_fd():
+    """Every field block of the oracle's RANS tangent agrees with Richardson-extrapolated central
+    differences to 1e-10 of the block's scale, and the tangent is additive over the k and omega
+    blocks.  (A forward rule that mixes an unselected max/min operand's huge tangent into the
+    selected one -- F1's cross-diffusion floor makes such tangents ~1e10 -- fails both.)"""
+    mesh, m, g, s, t = cavity_rans(12)
+    C = m.n_cells
+    rng = np.random.default_rng(1)
+    state = torch.tensor(np.concatenate([rng.uniform(-0.5, 0.5, 2 * C), rng.normal(0.0, 0.1, C),
+                                         np.exp(rng.uniform(np.log(1e-5), np.log(1e-2), C)),
+                                         np.exp(rng.uniform(0.0, np.log(1e3), C))]))
+    mu, rho = torch.full((C,), 1e-2, dtype=F64), torch.ones(C, dtype=F64)
+    f = lambda z: S.rans_residual(m, g, s, t, z, mu, rho, "upwind")  # noqa: E731
+    v = torch.tensor(rng.normal(size=state.numel())) * state.abs().clamp_min(1e-3)
+    _, jv = torch.func.jvp(f, (state,), (v,))
+
+    def cd(e):
+        return (f(state + e * v) - f(state - e * v)) / (2 * e)
+
+    rich = (4.0 * cd(5e-5) - cd(1e-4)) / 3.0
+    for b in range(5):
+        a_, r_ = jv.reshape(5, C)[b], rich.reshape(5, C)[b]
+        assert float((a_ - r_).abs().max() / r_.abs().max()) < 1e-10, b
+    vk = torch.zeros_like(v)
+    vk[3 * C : 4 * C] = v[3 * C : 4 * C]
+    vw = torch.zeros_like(v)
+    vw[4 * C :] = v[4 * C :]
+    jk = torch.func.jvp(f, (state,), (vk,))[1]
+    jw = torch.func.jvp(f, (state,), (vw,))[1]
+    jkw = torch.func.jvp(f, (state,), (vk + vw,))[1]
+    assert float((jkw - jk - jw).abs().max() / jkw.abs().max()) < 1e-13
+
+
+def test_max_min_tangent_rule():
+    """fv.maximum / fv.minimum: JAX's tie split, and the selected operand's tangent kept exactly
+    beside an unselected tangent 1e12 times larger."""
+    a = torch.tensor([1.0, 2.0, 3.0], dtype=F64)
+    b = torch.tensor([2.0, 2.0, 1.0], dtype=F64)
+    ta = torch.tensor([1.0, 1.0, 1.0], dtype=F64)
+    tb = torch.tensor([1e12, 3.0, 1e12], dtype=F64)
+    _, t = torch.func.jvp(fv.maximum, (a, b), (ta, tb))
+    assert t.tolist() == [1e12, 2.0, 1.0]
+    _, t = torch.func.jvp(fv.minimum, (a, b), (ta, tb))
+    assert t.tolist() == [1.0, 2.0, 1e12]
+    x = torch.tensor([0.0, -1.0, 2.0], dtype=F64, requires_grad=True)
+    (gx,) = torch.autograd.grad(fv.maximum(x, 0.0).sum(), x)
+    assert gx.tolist() == [0.5, 0.0, 1.0]
