@@ -32,6 +32,10 @@ enum KernelId : int {
   kKPbjSetup,
   kKPbjApply,
   kKBlas,
+  kKRansGrad,
+  kKRansAp,
+  kKRansApAdj,
+  kKRansGradAdj,
   kKNumKernels
 };
 

@@ -1175,7 +1175,7 @@ int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S
   const int64_t C = m.C;
   Out<S> oG{w.G, Gt}, oP{w.psi, psit};
   constexpr bool LF = ADVF == kAdvLimited, LS = ADVS == kAdvLimited;
-  AFX_LAUNCH(kKFlowGrad, (k_rans_grad<D, S, GRAD, LF, LS>), grid_for(C), dim3(kBlock), 0, s, m, a, oG, oP, ext);
+  AFX_LAUNCH(kKRansGrad, (k_rans_grad<D, S, GRAD, LF, LS>), grid_for(C), dim3(kBlock), 0, s, m, a, oG, oP, ext);
   AFX_CHECK_LAUNCH();
   a.G.v = w.G;
   a.G.t = Gt;
@@ -1191,7 +1191,7 @@ int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S
   AFX_LAUNCH(kKSstClosure, (k_rans_closure<D, S>), grid_for(C), dim3(kBlock), 0, s, m, a, co);
   AFX_CHECK_LAUNCH();
   bind_closure<S>(a, w.cl, clt, C);
-  AFX_LAUNCH(kKFlowAp, (k_rans_ap<D, S, ADVF>), grid_for(C), dim3(kBlock), 0, s, m, a, Out<S>{w.aP, aPt});
+  AFX_LAUNCH(kKRansAp, (k_rans_ap<D, S, ADVF>), grid_for(C), dim3(kBlock), 0, s, m, a, Out<S>{w.aP, aPt});
   AFX_CHECK_LAUNCH();
   if (R.v == nullptr) return 0;
   a.aP.v = w.aP;
@@ -1295,12 +1295,12 @@ extern "C" int afx_rans_vjp(const afx_mesh* m, const afx_geometry* g, const afx_
                      AFX_LAUNCH(kKSstResAdj, (k_rans_res_adj<D, AF, AS>), grid_for(m->C), dim3(kBlock), 0, s, v, b,
                                 cotangent, bar);
                      AFX_CHECK_LAUNCH();
-                     AFX_LAUNCH(kKFlowApAdj, (k_rans_ap_adj<D, AF>), grid_for(m->C), dim3(kBlock), 0, s, v, b, bar);
+                     AFX_LAUNCH(kKRansApAdj, (k_rans_ap_adj<D, AF>), grid_for(m->C), dim3(kBlock), 0, s, v, b, bar);
                      AFX_CHECK_LAUNCH();
                      AFX_LAUNCH(kKSstClosureAdj, (k_rans_closure_adj<D, AF, AS, GR>), grid_for(m->C), dim3(kBlock), 0,
                                 s, v, b, w.ext, bar);
                      AFX_CHECK_LAUNCH();
-                     AFX_LAUNCH(kKFlowGradAdj, (k_rans_grad_adj<D, GR, LF, LS>), grid_for(m->C), dim3(kBlock), 0, s,
+                     AFX_LAUNCH(kKRansGradAdj, (k_rans_grad_adj<D, GR, LF, LS>), grid_for(m->C), dim3(kBlock), 0, s,
                                 v, b, w.ext, bar);
                      AFX_CHECK_LAUNCH();
                      return 0;

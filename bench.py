@@ -1,8 +1,10 @@
 """Benchmark: residual + VJP throughput (Gfaces/s) of the face-loop residual on one MI355X.
 
 One step = one residual evaluation R(state) plus one adjoint J^T lambda (cotangents of the state,
-viscosity, density and boundary values), all inputs resident in HBM.  The workload is the 3D hex
-channel of 128 x 64 x 64 cells (524,288 cells, 1,593,344 faces) with a seeded turbulent-like state.
+viscosity, density and boundary values), all inputs resident in HBM.  The default workload is the
+coupled RANS (u, v, w, p, k, omega with the k-omega SST closure) on the 3D hex channel of
+128 x 64 x 64 cells (524,288 cells, 1,593,344 faces) with a seeded turbulent-like state;
+``--workload flow`` runs the laminar (u, v, w, p) residual on the same mesh.
 
 Prints ONE JSON line.  ``roofline`` is the dominant kernel's algorithmic bytes per launch over its
 average launch time, timed with events recorded by the library on the launch stream during the
@@ -25,123 +27,204 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+S8 = 8  # bytes per FP64 value
 
-# Algorithmic (compulsory) bytes per launch of each kernel: every array the kernel must touch,
-# counted once.  Units: per cell (c), per face (f), per cell-face entry (e), per boundary face (b).
-# D = 3, NV = 4 flow variables, least-squares gradient, limited advection.
-def _kernel_bytes(D=3, NV=4, limited=True):
-    S = 8
-    G = NV * D * S
-    psi = D * S if limited else 0
-    topo = dict(c=4, e=4, f=8)  # cell_ptr, cell->face list, owner+neighbour
-    cgeo = (1 + D) * S  # volume + centroid
-    fgeo = (1 + 2 * D) * S  # area + normal + centroid
-    bnd = 1 + D * S + S  # kind + u_b + p_b
-    state = NV * S
-    m = {}
-    m["flow_grad"] = dict(c=topo["c"] + state + cgeo + G + psi, e=topo["e"], f=topo["f"] + fgeo, b=bnd)
-    m["flow_ap"] = dict(c=topo["c"] + state + G + 2 * S + cgeo + S, e=topo["e"], f=topo["f"] + fgeo, b=1)
-    m["flow_res"] = dict(
-        c=topo["c"] + state + G + 3 * S + psi + cgeo + state, e=topo["e"], f=topo["f"] + fgeo, b=bnd
-    )
-    m["flow_res_adj"] = dict(
-        c=topo["c"] + state + G + 3 * S + psi + cgeo + state + state + G + 3 * S + psi,
-        e=topo["e"],
-        f=topo["f"] + fgeo + (D + 1) * S,
-        b=bnd,
-    )
-    m["flow_ap_adj"] = dict(
-        c=topo["c"] + state + G + 2 * S + cgeo + S + (state + 2 * S) * 2 + G * 2 + psi + 2 * D * S + 2 * D * 4,
-        e=topo["e"],
-        f=topo["f"] + fgeo,
-        b=1,
-    )
-    m["flow_grad_adj"] = dict(
-        c=topo["c"] + cgeo + G + state * 2 + 2 * D * S + 2 * D * 4, e=topo["e"], f=topo["f"] + fgeo, b=bnd + (D + 1) * S
-    )
-    return m
+# Whole-step algorithmic bytes per face of residual + VJP (pass-structured compulsory traffic of the
+# recompute design: 3 forward face passes, 3 reverse passes) for the two 3D hex workloads.
+STEP_BYTES_PER_FACE = {"sst": 1386.0}
 
 
-def algorithmic_bytes(name, C, F, E, Fb):
-    k = _kernel_bytes()[name]
-    return k["c"] * C + k["f"] * F + k["e"] * E + k["b"] * Fb
+def _face_pass(D):
+    """One face pass: owner + neighbour (2 x int32), area, unit normal and centroid (FP64), and the
+    cell -> face list entries that reach the face from both sides (2 x int32)."""
+    return 2 * 4 + S8 * (1 + 2 * D) + 2 * 4
 
 
-def build_workload(device, nx=128, ny=64, nz=64):
+def _flow_kernel_bytes(D=3, limited=True):
+    """Compulsory bytes of each laminar-flow kernel, as per-cell (c), per-face (f) and
+    per-boundary-face (b) coefficients: every array a kernel reads or writes counted once."""
+    NV = D + 1
+    G = NV * D * S8
+    psi = D * S8 if limited else 0
+    state = NV * S8
+    cgeo = (1 + D) * S8
+    fp = _face_pass(D)
+    bnd = 1 + D * S8 + S8
+    return {
+        "flow_grad": dict(c=4 + state + cgeo + G + psi, f=fp, b=bnd),
+        "flow_ap": dict(c=4 + D * S8 + D * D * S8 + 2 * S8 + cgeo + S8, f=fp, b=1),
+        "flow_res": dict(c=4 + state + G + 3 * S8 + psi + cgeo + state, f=fp, b=bnd),
+        "flow_res_adj": dict(c=4 + 2 * state + G + 3 * S8 + psi + cgeo + state + G + 3 * S8 + psi,
+                             f=fp + (D + 1) * S8, b=bnd),
+        "flow_ap_adj": dict(c=4 + D * S8 + D * D * S8 + 2 * S8 + cgeo + S8 + 2 * (D * S8 + 2 * S8)
+                            + 2 * D * D * S8 + psi + 2 * D * S8 + 2 * D * 4, f=fp, b=1),
+        "flow_grad_adj": dict(c=4 + cgeo + G + 2 * state + 2 * D * S8 + 2 * D * 4, f=fp, b=bnd + (D + 1) * S8),
+    }
+
+
+def _rans_kernel_bytes(D=3, limited=True):
+    """Compulsory bytes of each coupled-RANS kernel (same convention as the flow model; ``w`` is per
+    wall-adjacent cell, ``wf`` per wall face).  Limited advection of u_i, k, omega (NL = D + 2)."""
+    NA, NL = D + 3, D + 2
+    state, G = NA * S8, NA * D * S8
+    psi = NL * S8 if limited else 0
+    ext = 2 * NL * 4 if limited else 0
+    cgeo = (1 + D) * S8
+    fp = _face_pass(D)
+    bnd = 3 + (D + 1) * S8 + 2 * S8 + 1  # kinds, u_b, p_b, k/omega values, wall flag
+    closure_in = 2 * S8 + (D * D + 2 * D) * S8 + 4 * S8 + 1  # k, w, grad u/k/w, nu mu rho d, wall flag
+    cell_sst = 5 * S8  # nu_t, F1, strain, cross-diffusion, wall shear rate
+    res_in = state + G + psi + S8 + 3 * S8 + cell_sst + S8 + 1 + cgeo  # ... + a_P, mu_eff rho nu, d, flag
+    bars10 = 10 * S8
+    return {
+        # launched twice per step (residual, and the recompute inside the VJP, which also writes
+        # the extremum owners): the average of the two
+        "rans_grad": dict(c=4 + state + cgeo + G + psi + ext / 2, f=fp, b=bnd),
+        "rans_closure": dict(c=4 + closure_in + 7 * S8, f=0, b=0, w=D * S8 + D * S8, wf=1 + D * S8 + S8 + 4),
+        "rans_ap": dict(c=4 + D * S8 + D * D * S8 + 2 * S8 + cgeo + S8, f=fp, b=2, w=2 * S8),
+        "rans_res": dict(c=4 + res_in + state, f=fp, b=bnd, w=2 * S8),
+        "rans_res_adj": dict(c=4 + res_in + state + state + G + S8 + psi + bars10, f=fp + (D + 1) * S8, b=bnd,
+                             w=2 * S8),
+        "rans_ap_adj": dict(c=4 + D * S8 + D * D * S8 + 4 * S8 + cgeo + S8 + 2 * D * S8 + 2 * D * D * S8
+                            + 2 * 4 * S8, f=fp, b=2),
+        "rans_closure_adj": dict(c=4 + closure_in + 2 * S8 + bars10 + 2 * G + psi + ext + 2 * state + 3 * S8
+                                 + 2 * NL * S8 + 2 * NL * S8 + cgeo, f=2 * 4 + D * S8, b=0,
+                                 w=D * S8 + D * S8, wf=1 + 2 * D * S8 + S8),
+        "rans_grad_adj": dict(c=4 + G + ext + 2 * NL * S8 + 2 * state + cgeo, f=fp, b=3 + 2 * (D + 1) * S8),
+    }
+
+
+def algorithmic_bytes(name, counts, workload="sst"):
+    """Bytes per launch of kernel ``name`` for mesh counts {C, F, Fb, Cw, Fw}."""
+    model = (_rans_kernel_bytes() if workload == "sst" else _flow_kernel_bytes())[name]
+    return (model.get("c", 0) * counts["C"] + model.get("f", 0) * counts["F"] + model.get("b", 0) * counts["Fb"]
+            + model.get("w", 0) * counts.get("Cw", 0) + model.get("wf", 0) * counts.get("Fw", 0))
+
+
+CHANNEL_BCS = (("left", 2), ("right", 3), ("bottom", 1), ("top", 1), ("back", 1), ("front", 1))
+WALLS = ("bottom", "top", "back", "front")
+
+
+def _channel_mesh(nx, ny, nz):
+    from aquaflux_amd.mesh import structured_grid_3d
+
+    return structured_grid_3d(nx, ny, nz, lx=2.0, named_boundaries=True)
+
+
+def seeded_state(C, turbulent, seed=2):
+    """u = 1 + 0.05 N(0,1), v, w = 0.05 N(0,1), p = N(0, 0.1) [, k ~ LogU(1e-5, 1e-2),
+    omega ~ LogU(1, 1e3)], drawn in that order from ``default_rng(seed)``."""
+    rng = np.random.default_rng(seed)
+    parts = [1.0 + 0.05 * rng.normal(size=C), 0.05 * rng.normal(size=C), 0.05 * rng.normal(size=C),
+             rng.normal(0.0, 0.1, C)]
+    if turbulent:
+        parts += [np.exp(rng.uniform(np.log(1e-5), np.log(1e-2), C)), np.exp(rng.uniform(0.0, np.log(1e3), C))]
+    state = np.concatenate(parts)
+    return state, rng.normal(size=state.size)
+
+
+def build_workload(device, workload="sst", nx=128, ny=64, nz=64, nu=1e-5):
+    """The 3D hex channel: inlet u = (1, 0, 0) on the left, pressure outlet on the right, no-slip
+    walls elsewhere; least-squares gradients and Venkatakrishnan-limited upwind advection.  The SST
+    variant adds k = 1e-3, omega = 1 at the inlet, k = 0 and the omega wall fixation on the walls,
+    zero gradient at the outlet, nu = 1e-5."""
     import torch
 
-    from aquaflux_amd.boundary import BoundaryConditions
+    from aquaflux_amd.boundary import BoundaryConditions, Dirichlet, ZeroGradient
     from aquaflux_amd.discretization import LimitedUpwind
     from aquaflux_amd.flow import MomentumContinuity, NoSlipWall, PressureOutlet, VelocityInlet
-    from aquaflux_amd.mesh import structured_grid_3d
     from aquaflux_amd.properties import Constant, PropertyModel
     from aquaflux_amd.schemes import LeastSquaresGradient, VenkatakrishnanLimiter
 
-    mesh = structured_grid_3d(nx, ny, nz, named_boundaries=True)
+    mesh = _channel_mesh(nx, ny, nz)
     geom = mesh.geometry(device=device)
-    bcs = BoundaryConditions(
-        {
-            "left": VelocityInlet((1.0, 0.0, 0.0)),
-            "right": PressureOutlet(0.0),
-            "bottom": NoSlipWall(),
-            "top": NoSlipWall(),
-            "back": NoSlipWall(),
-            "front": NoSlipWall(),
-        }
-    )
-    props = PropertyModel({"viscosity": Constant(1e-5), "density": Constant(1.0)})
-    mc = MomentumContinuity.build(
-        mesh, geom, props, LeastSquaresGradient(), bcs, advection_scheme=LimitedUpwind(VenkatakrishnanLimiter(5.0))
-    )
-    rng = np.random.default_rng(2)
+    bcs = BoundaryConditions({"left": VelocityInlet((1.0, 0.0, 0.0)), "right": PressureOutlet(0.0),
+                              **{w: NoSlipWall() for w in WALLS}})
+    props = PropertyModel({"viscosity": Constant(nu), "density": Constant(1.0)})
+    adv = LimitedUpwind(VenkatakrishnanLimiter(5.0))
+    mc = MomentumContinuity.build(mesh, geom, props, LeastSquaresGradient(), bcs, advection_scheme=adv)
     C = mesh.n_cells
-    state = np.concatenate(
-        [1.0 + 0.05 * rng.normal(size=C), 0.05 * rng.normal(size=C), 0.05 * rng.normal(size=C), rng.normal(0.0, 0.1, C)]
-    )
-    lam = rng.normal(size=state.size)
+    state, lam = seeded_state(C, workload == "sst")
     x = torch.tensor(state, dtype=torch.float64, device=device)
     lam_t = torch.tensor(lam, dtype=torch.float64, device=device)
-    return mesh, geom, mc, x, lam_t, state, lam
+    if workload != "sst":
+        return mesh, mc, x, lam_t, state, lam
+    from aquaflux_amd.turbulence import CoupledRANS, SSTModel, SSTTurbulence
+
+    kb = BoundaryConditions({"left": Dirichlet(1e-3), "right": ZeroGradient(), **{w: Dirichlet(0.0) for w in WALLS}})
+    wb = BoundaryConditions({"left": Dirichlet(1.0), "right": ZeroGradient(), **{w: ZeroGradient() for w in WALLS}})
+    turb = SSTTurbulence.build(SSTModel(), mesh, geom, LeastSquaresGradient(), adv, 1.0,
+                               torch.full((C,), nu, dtype=torch.float64), list(WALLS), kb, wb)
+    return mesh, CoupledRANS.build(mc, turb), x, lam_t, state, lam
 
 
-def cpu_baseline(mesh, state, lam, reps=2):
-    """The CPU oracle (torch float64, reference pass structure) on the same workload."""
+def cpu_baseline(workload, nx, ny, nz, reps=2, nu=1e-5):
+    """The CPU oracle (torch float64, reference pass structure) timed on a bounded sample: the same
+    channel workload on an nx x ny x nz mesh, residual + VJP (torch autograd) ``reps`` times."""
     import torch
 
     from oracle import fv
+    from oracle import sst as S
 
-    threads = os.cpu_count() or 1
+    threads = min(os.cpu_count() or 1, 16)
     torch.set_num_threads(threads)
+    mesh = _channel_mesh(nx, ny, nz)
     m = fv.OMesh.from_mesh(mesh)
     g = fv.geometry(m)
     F, d, C = mesh.n_faces, mesh.dim, mesh.n_cells
     kind = torch.zeros(F, dtype=torch.long)
     ub = torch.zeros((F, d), dtype=torch.float64)
-    for name, k in (("left", 2), ("right", 3), ("top", 1), ("bottom", 1), ("back", 1), ("front", 1)):
+    for name, k in CHANNEL_BCS:
         idx = torch.from_numpy(mesh.face_patches.indices(name))
         kind[idx] = k
         if name == "left":
             ub[idx, 0] = 1.0
     s = fv.FlowSetup(kind=kind, u_b=ub, p_b=torch.zeros(F, dtype=torch.float64), advection="limited",
                      gradient="least_squares")
-    x = torch.tensor(state, dtype=torch.float64, requires_grad=True)
-    mu = torch.full((C,), 1e-5, dtype=torch.float64)
+    mu = torch.full((C,), nu, dtype=torch.float64)
     rho = torch.ones(C, dtype=torch.float64)
+    state, lam = seeded_state(C, workload == "sst")
+    x = torch.tensor(state, dtype=torch.float64, requires_grad=True)
     lam_t = torch.tensor(lam, dtype=torch.float64)
+    if workload == "sst":
+        wf = np.concatenate([mesh.face_patches.indices(w) for w in WALLS]).astype(np.int64)
+
+        def scalar_bc(inlet, walls_dirichlet):
+            kk = torch.zeros(F, dtype=torch.long)
+            vv = torch.zeros(F, dtype=torch.float64)
+            for name, _ in CHANNEL_BCS:
+                idx = torch.from_numpy(mesh.face_patches.indices(name))
+                if name == "left":
+                    kk[idx], vv[idx] = fv.BC_DIRICHLET, inlet
+                elif name in WALLS and walls_dirichlet:
+                    kk[idx] = fv.BC_DIRICHLET
+                else:
+                    kk[idx] = fv.BC_ZERO_GRADIENT
+            return fv.ScalarSetup(kind=kk, value=vv, t_inf=torch.zeros(F, dtype=torch.float64))
+
+        t = S.build_sst_setup(m, g, "least_squares", S.SSTModel(), torch.full((C,), nu, dtype=torch.float64), 1.0,
+                              wf, scalar_bc(1e-3, True), scalar_bc(1.0, False))
+
+        def fn(z):
+            return S.rans_residual(m, g, s, t, z, mu, rho, "limited")
+    else:
+        def fn(z):
+            return fv.flow_residual(m, g, s, z, mu, rho)
+
     times = []
     for _ in range(reps):
         t0 = time.perf_counter()
-        R = fv.flow_residual(m, g, s, x, mu, rho)
+        R = fn(x)
         torch.autograd.grad(R, (x,), lam_t)
         times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
+    tm = float(np.median(times))
     return {
-        "value": F / t / 1e9,
+        "value": F / tm / 1e9,
         "unit": "Gfaces/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{reps} residual+VJP evaluations of the full workload ({F} faces) by the torch-CPU "
-        f"float64 oracle, median {t:.2f} s each",
+        "sample": f"{reps} residual+VJP evaluations by the torch-CPU float64 oracle of the same channel "
+        f"workload at {nx}x{ny}x{nz} ({C} cells, {F} faces), median {tm:.2f} s each",
     }
 
 
@@ -224,8 +307,11 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", choices=("sst", "flow"), default="sst")
+    ap.add_argument("--size", type=int, nargs=3, default=(128, 64, 64), metavar=("NX", "NY", "NZ"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=2)
+    ap.add_argument("--cpu-size", type=int, nargs=3, default=(128, 64, 16), metavar=("NX", "NY", "NZ"))
     ap.add_argument("--no-newton", action="store_true", help="skip the Newton time-to-1e-10 comparison")
     ap.add_argument("--newton-only", action="store_true", help="only the Newton comparison")
     ap.add_argument("--newton-size", type=int, nargs=3, default=(200, 100, 100), metavar=("NX", "NY", "NZ"))
@@ -257,13 +343,16 @@ def main(argv=None):
     if args.newton_only:
         print(json.dumps(newton_compare(dev, *args.newton_size, cfl0=args.newton_cfl0)), flush=True)
         return 0
-    mesh, geom, mc, x, lam, state_np, lam_np = build_workload(dev)
-    raw = mc._raw
-    mu, rho, ub, pb, _ = mc._inputs()
-    prim = [x, mu, rho, ub, pb]
+    wl = args.workload
+    nx, ny, nz = args.size
+    mesh, op, x, lam, _, _ = build_workload(dev, wl, nx, ny, nz)
+    raw = op._raw
+    prim = op._detached(x) if wl == "sst" else [x] + [t.detach() for t in op._inputs()[:4]]
+    print(f"bench: {wl} workload {nx}x{ny}x{nz} built ({mesh.n_cells} cells, {mesh.n_faces} faces)",
+          file=sys.stderr, flush=True)
 
     def step():
-        raw.residual(x, mu, rho, ub, pb)
+        raw.residual(*prim)
         raw.vjp(prim, lam)
 
     for _ in range(args.warmup):
@@ -280,12 +369,16 @@ def main(argv=None):
         kern = timer.collect()
     F, C = mesh.n_faces, mesh.n_cells
     Fb = int(np.count_nonzero(mesh.neighbour < 0))
-    E = 2 * (F - Fb) + Fb
+    counts = {"C": C, "F": F, "Fb": Fb}
+    if wl == "sst":
+        wf = op.turbulence.wall_faces
+        counts["Fw"] = int(wf.numel())
+        counts["Cw"] = int(op.turbulence.wall_cells.numel())
     ms_per_step = elapsed / args.steps * 1e3
     value = F * args.steps / elapsed / 1e9
     dom = max(kern, key=lambda k: kern[k][0])
     avg_ms = kern[dom][0] / kern[dom][1]
-    nbytes = algorithmic_bytes(dom, C, F, E, Fb)
+    nbytes = algorithmic_bytes(dom, counts, wl)
     achieved = nbytes / (avg_ms * 1e-3) / 1e9
     roofline = {
         "bound": "hbm",
@@ -298,7 +391,12 @@ def main(argv=None):
         "avg_ms": round(avg_ms, 4),
         "algorithmic_bytes": int(nbytes),
     }
-    traffic_file = os.path.join(ROOT, "profiles", "flow_hex128x64x64_traffic.json")
+    if wl in STEP_BYTES_PER_FACE:
+        step_gbs = STEP_BYTES_PER_FACE[wl] * F / (ms_per_step * 1e-3) / 1e9
+        roofline["step_bytes_per_face"] = STEP_BYTES_PER_FACE[wl]
+        roofline["step_frac"] = round(step_gbs / HBM_PEAK_GBS, 4)
+    tag = "rans" if wl == "sst" else "flow"
+    traffic_file = os.path.join(ROOT, "profiles", f"{tag}_hex{nx}x{ny}x{nz}_traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as fh:
             tk = json.load(fh).get("kernels", {}).get(dom)
@@ -306,6 +404,10 @@ def main(argv=None):
             roofline["traffic"] = tk["hbm_bytes_per_launch"]
             roofline["traffic_source"] = "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated (profiles/)"
     kernels = {k: round(v[0] / v[1], 4) for k, v in sorted(kern.items())}
+    desc = {
+        "sst": "coupled u,v,w,p,k,omega with the k-omega SST closure (wall functions, omega wall fixation)",
+        "flow": "coupled u,v,w,p (laminar)",
+    }[wl]
     out = {
         "metric": "Gfaces/s for residual+VJP",
         "value": round(value, 4),
@@ -318,12 +420,13 @@ def main(argv=None):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded state u=1+0.05N, v,w=0.05N, p=N(0,0.1); seeded cotangent N(0,1))",
+        "data": "synthetic (seeded state u=1+0.05N, v,w=0.05N, p=N(0,0.1)"
+        + (", k~LogU(1e-5,1e-2), omega~LogU(1,1e3)" if wl == "sst" else "")
+        + "; seeded cotangent N(0,1))",
         "config": {
-            "workload": "3D hex channel 128x64x64 (524,288 cells, 1,593,344 faces), coupled u,v,w,p, "
-            "least-squares gradients, Venkatakrishnan-limited upwind, inlet/outlet/walls; "
-            "residual + VJP w.r.t. state, viscosity, density, boundary values (k-omega SST closure not "
-            "in this step)",
+            "workload": f"3D hex channel {nx}x{ny}x{nz} ({C:,} cells, {F:,} faces), {desc}, least-squares "
+            "gradients, Venkatakrishnan-limited upwind, inlet/outlet/walls; residual + VJP w.r.t. state, "
+            "viscosity, density, boundary values",
             "cells": C,
             "faces": F,
         },
@@ -332,9 +435,9 @@ def main(argv=None):
         "cpu_baseline": None,
     }
     if not args.no_newton:
-        out["newton"] = newton_compare(dev)
+        out["newton"] = newton_compare(dev, *args.newton_size, cfl0=args.newton_cfl0)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(mesh, state_np, lam_np, reps=args.cpu_reps)
+        out["cpu_baseline"] = cpu_baseline(wl, *args.cpu_size, reps=args.cpu_reps)
     print(json.dumps(out), flush=True)
     return 0
 
