@@ -105,6 +105,46 @@ class AfxRansCotangents(ctypes.Structure):
     ]
 
 
+class AfxGeometryCotangents(ctypes.Structure):
+    _fields_ = [
+        ("area", c_void_p),
+        ("normal", c_void_p),
+        ("face_centroid", c_void_p),
+        ("volume", c_void_p),
+        ("cell_centroid", c_void_p),
+    ]
+
+
+class AfxScalarConfig(ctypes.Structure):
+    _fields_ = [
+        ("advection", ctypes.c_int32),
+        ("gradient", ctypes.c_int32),
+        ("limiter_k", ctypes.c_double),
+        ("bc_kind", c_void_p),
+    ]
+
+
+class AfxScalarInputs(ctypes.Structure):
+    _fields_ = [
+        ("phi", c_void_p),
+        ("gamma", c_void_p),
+        ("flux", c_void_p),
+        ("value", c_void_p),
+        ("t_inf", c_void_p),
+    ]
+
+
+class AfxScalarCotangents(ctypes.Structure):
+    _fields_ = [
+        ("phi", c_void_p),
+        ("gamma", c_void_p),
+        ("flux", c_void_p),
+        ("value", c_void_p),
+        ("t_inf", c_void_p),
+        ("geometry", AfxGeometryCotangents),
+    ]
+
+
 # Declared entry points: name -> (argtypes). Every function returns int except afx_last_error.
 _SIGNATURES = {
     "afx_mesh_create": [
@@ -147,6 +187,24 @@ _SIGNATURES = {
     ],
     "afx_scalar_gradient": [
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+    ],
+    "afx_geometry_vjp_workspace_bytes": [c_void_p, c_int64_p],
+    "afx_geometry_vjp": [
+        c_void_p, c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxGeometryCotangents), c_void_p, c_void_p,
+        ctypes.c_int64, c_void_p,
+    ],
+    "afx_scalar_workspace_bytes": [c_void_p, ctypes.c_int, c_int64_p],
+    "afx_scalar_residual": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxScalarConfig), ctypes.POINTER(AfxScalarInputs),
+        c_void_p, c_void_p, ctypes.c_int64, c_void_p,
+    ],
+    "afx_scalar_jvp": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxScalarConfig), ctypes.POINTER(AfxScalarInputs),
+        ctypes.POINTER(AfxScalarInputs), c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p,
+    ],
+    "afx_scalar_vjp": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxScalarConfig), ctypes.POINTER(AfxScalarInputs),
+        c_void_p, ctypes.POINTER(AfxScalarCotangents), c_void_p, ctypes.c_int64, c_void_p,
     ],
     "afx_rans_workspace_bytes": [c_void_p, ctypes.c_int, c_int64_p],
     "afx_rans_residual": [

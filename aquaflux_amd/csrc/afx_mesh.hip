@@ -290,6 +290,245 @@ __global__ void k_cell_volume(MeshView m, const double* __restrict__ approx, dou
   for (int j = 0; j < D; ++j) cc[(int64_t)j * m.C + c] = cs[j] * inv + ap[j];
 }
 
+// ---------------------------------------------------------------------------------------------
+// geometry adjoint (reverse of the four passes above), from cotangents of (area, oriented normal,
+// face centroid, volume, cell centroid) to node-coordinate cotangents.
+
+// Cell pass: cotangents of the divergence-theorem sums (csum [d], S = d V) and of the approximate
+// centroid, per cell: cellbar[(j)*C + c] for csum_j, [d*C + c] for S, [(d+1+j)*C + c] for approx_j.
+template <int D>
+__global__ void k_geom_cell_adj(MeshView m, const double* __restrict__ approx, const double* __restrict__ vol_b,
+                                const double* __restrict__ cc_b, double* __restrict__ cellbar) {
+  int32_t c = xcd_thread_index();
+  if (c >= m.C) return;
+  const double V = m.vol[c];
+  double ap[D], csb[D], apb[D];
+  double Vb = vol_b ? vol_b[c] : 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    ap[j] = approx[(int64_t)j * m.C + c];
+    const double ccb = cc_b ? cc_b[(int64_t)j * m.C + c] : 0.0;
+    csb[j] = ccb / ((double)(D + 1) * V);
+    Vb -= ccb * (m.cc[(int64_t)j * m.C + c] - ap[j]) / V;
+    apb[j] = ccb;
+  }
+  const double Sb = Vb / (double)D;
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    const int32_t ent = m.cface[k];
+    const int32_t f = ent >= 0 ? ent : ~ent;
+    const double sg = ent >= 0 ? 1.0 : -1.0;
+    const double A = m.area[f];
+    double l[D], n[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      l[j] = m.fc[(int64_t)j * m.F + f] - ap[j];
+      n[j] = m.nrm[(int64_t)j * m.F + f];
+    }
+    const double phi = vdot<D>(l, n) * A;
+    const double phib = sg * (Sb + vdot<D>(csb, l));
+#pragma unroll
+    for (int j = 0; j < D; ++j) apb[j] -= sg * csb[j] * phi + phib * A * n[j];
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    cellbar[(int64_t)j * m.C + c] = csb[j];
+    cellbar[(int64_t)(D + 1 + j) * m.C + c] = apb[j];
+  }
+  cellbar[(int64_t)D * m.C + c] = Sb;
+}
+
+AFX_DEV void cross3(const double* a, const double* b, double* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Face pass: total cotangents of (area, oriented normal, centroid) of each face, then the reverse of
+// its node construction into per-incidence node cotangents inc_b[j * n_inc + i].
+template <int D>
+__global__ void k_geom_face_adj(MeshView m, int32_t Nn, const int32_t* __restrict__ foff,
+                                const int32_t* __restrict__ fidx, const double* __restrict__ X,
+                                const double* __restrict__ approx, const double* __restrict__ cellbar,
+                                const double* __restrict__ area_b, const double* __restrict__ nrm_b,
+                                const double* __restrict__ fc_b, double* __restrict__ inc_b, int32_t n_inc) {
+  int32_t f = xcd_thread_index();
+  if (f >= m.F) return;
+  const int32_t P = m.owner[f], N = m.nb[f];
+  const double A = m.area[f];
+  double n[D], xf[D], nb[D], xfb[D];
+  double Ab = area_b ? area_b[f] : 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    n[j] = m.nrm[(int64_t)j * m.F + f];
+    xf[j] = m.fc[(int64_t)j * m.F + f];
+    nb[j] = nrm_b ? nrm_b[(int64_t)j * m.F + f] : 0.0;
+    xfb[j] = fc_b ? fc_b[(int64_t)j * m.F + f] : 0.0;
+  }
+  for (int side = 0; side < 2; ++side) {
+    const int32_t c = side == 0 ? P : N;
+    if (c < 0) continue;
+    const double sg = side == 0 ? 1.0 : -1.0;
+    double l[D], csb[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      l[j] = xf[j] - approx[(int64_t)j * m.C + c];
+      csb[j] = cellbar[(int64_t)j * m.C + c];
+    }
+    const double Sb = cellbar[(int64_t)D * m.C + c];
+    const double ln = vdot<D>(l, n);
+    const double phi = ln * A;
+    const double phib = sg * (Sb + vdot<D>(csb, l));
+    const double inv_nf = 1.0 / (double)(m.cptr[c + 1] - m.cptr[c]);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      nb[j] += phib * A * l[j];
+      xfb[j] += sg * csb[j] * phi + phib * A * n[j] + cellbar[(int64_t)(D + 1 + j) * m.C + c] * inv_nf;
+    }
+    Ab += phib * ln;
+  }
+  const int32_t b = foff[f], e = foff[f + 1];
+  // orientation sign of the stored normal relative to the node-winding normal
+  if (D == 2) {
+    const int32_t a0 = fidx[b], a1 = fidx[b + 1];
+    const double d0 = X[a1] - X[a0], d1 = X[Nn + a1] - X[Nn + a0];
+    const double v0 = d1, v1 = -d0;
+    const double vv = sqrt(v0 * v0 + v1 * v1);
+    const double r0 = v0 / vv, r1 = v1 / vv;
+    const double s = (r0 * n[0] + r1 * n[1]) < 0.0 ? -1.0 : 1.0;  // n = s * raw
+    const double rb0 = s * nb[0], rb1 = s * nb[1];
+    const double proj = rb0 * r0 + rb1 * r1;
+    const double vb0 = (rb0 - proj * r0) / vv, vb1 = (rb1 - proj * r1) / vv;
+    // area = |d|, raw normal = unit(d1, -d0), centroid = (p0 + p1) / 2
+    const double db0 = Ab * d0 / A - vb1, db1 = Ab * d1 / A + vb0;
+    inc_b[b] = -db0 + 0.5 * xfb[0];
+    inc_b[n_inc + b] = -db1 + 0.5 * xfb[1];
+    inc_b[b + 1] = db0 + 0.5 * xfb[0];
+    inc_b[n_inc + b + 1] = db1 + 0.5 * xfb[1];
+  } else {
+    const int32_t k = e - b;
+    const double invk = 1.0 / (double)k;
+    double ctr[3] = {0.0, 0.0, 0.0};
+    for (int32_t i = b; i < e; ++i) {
+      const int32_t a = fidx[i];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) ctr[j] += X[(int64_t)j * Nn + a];
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) ctr[j] = ctr[j] * invk;
+    double Sv[3] = {0.0, 0.0, 0.0};
+    for (int32_t i = b; i < e; ++i) {
+      const int32_t a = fidx[i], c2 = fidx[i + 1 < e ? i + 1 : b];
+      double u[3], w[3], dir[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        u[j] = X[(int64_t)j * Nn + a] - ctr[j];
+        w[j] = X[(int64_t)j * Nn + c2] - ctr[j];
+      }
+      cross3(u, w, dir);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) Sv[j] += 0.5 * dir[j];
+    }
+    const double Ar = sqrt(Sv[0] * Sv[0] + Sv[1] * Sv[1] + Sv[2] * Sv[2]);
+    double nh[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) nh[j] = Sv[j] / Ar;
+    const double s = (nh[0] * n[0] + nh[1] * n[1] + nh[2] * n[2]) < 0.0 ? -1.0 : 1.0;
+    double nhb[3], csb[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      nhb[j] = s * nb[j];
+      csb[j] = xfb[j] / Ar;
+    }
+    // centroid = cs / A
+    Ab -= (xfb[0] * xf[0] + xfb[1] * xf[1] + xfb[2] * xf[2]) / Ar;
+    // sgn_i = dir_i . nh contributes sum_i sgn_b_i dir_i to nh's cotangent
+    for (int32_t i = b; i < e; ++i) {
+      const int32_t a = fidx[i], c2 = fidx[i + 1 < e ? i + 1 : b];
+      double pb_[3], pc_[3], u[3], w[3], dir[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        pb_[j] = X[(int64_t)j * Nn + a];
+        pc_[j] = X[(int64_t)j * Nn + c2];
+        u[j] = pb_[j] - ctr[j];
+        w[j] = pc_[j] - ctr[j];
+      }
+      cross3(u, w, dir);
+      double sgnb = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) sgnb += csb[j] * ((ctr[j] + pb_[j] + pc_[j]) / 3.0);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) nhb[j] += sgnb * 0.5 * dir[j];
+    }
+    // nh = Sv / |Sv|, A = |Sv|
+    const double pr = nhb[0] * nh[0] + nhb[1] * nh[1] + nhb[2] * nh[2];
+    double Svb[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Svb[j] = Ab * nh[j] + (nhb[j] - pr * nh[j]) / Ar;
+    double ctrb[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      for (int32_t i = b; i < e; ++i) inc_b[(int64_t)j * n_inc + i] = 0.0;
+    for (int32_t i = b; i < e; ++i) {
+      const int32_t a = fidx[i];
+      const int32_t inext = i + 1 < e ? i + 1 : b;
+      const int32_t c2 = fidx[inext];
+      double pb_[3], pc_[3], u[3], w[3], dir[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        pb_[j] = X[(int64_t)j * Nn + a];
+        pc_[j] = X[(int64_t)j * Nn + c2];
+        u[j] = pb_[j] - ctr[j];
+        w[j] = pc_[j] - ctr[j];
+      }
+      cross3(u, w, dir);
+      double sgn = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) sgn += 0.5 * dir[j] * nh[j];
+      double dirb[3], ub[3], wb[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) dirb[j] = 0.5 * (Svb[j] + (0.0));
+      // sgn_i = dir_i . nh (with dir_i = u x w / 2): cotangent sgnb_i adds sgnb_i nh / 2 to (u x w)'s
+      double sgnb = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) sgnb += csb[j] * ((ctr[j] + pb_[j] + pc_[j]) / 3.0);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) dirb[j] += 0.5 * sgnb * nh[j];
+      // d(u x w) = du x w + u x dw:  u_b = w x dirb, w_b = dirb x u
+      cross3(w, dirb, ub);
+      cross3(dirb, u, wb);
+      double tcb[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        tcb[j] = sgn * csb[j] / 3.0;
+        inc_b[(int64_t)j * n_inc + i] += ub[j] + tcb[j];
+        inc_b[(int64_t)j * n_inc + inext] += wb[j] + tcb[j];
+        ctrb[j] += tcb[j] - ub[j] - wb[j];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      for (int32_t i = b; i < e; ++i) inc_b[(int64_t)j * n_inc + i] += ctrb[j] * invk;
+  }
+}
+
+// Node pass: node cotangent = sum over the node's incidences.
+template <int D>
+__global__ void k_geom_node_adj(int32_t Nn, const int32_t* __restrict__ nptr, const int32_t* __restrict__ ninc,
+                                const double* __restrict__ inc_b, int32_t n_inc, double* __restrict__ node_b) {
+  int32_t p = xcd_thread_index();
+  if (p >= Nn) return;
+  double s[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) s[j] = 0.0;
+  for (int32_t k = nptr[p]; k < nptr[p + 1]; ++k) {
+    const int32_t i = ninc[k];
+#pragma unroll
+    for (int j = 0; j < D; ++j) s[j] += inc_b[(int64_t)j * n_inc + i];
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) node_b[(int64_t)j * Nn + p] = s[j];
+}
+
 }  // namespace
 
 afx::MeshView afx_view(const afx_mesh* m, const afx_geometry* g) {
@@ -338,6 +577,56 @@ extern "C" int afx_geometry_compute(const afx_mesh* m, const double* node_coords
     AFX_LAUNCH(kKGeomOrient, k_orient<3>, grid_for(m->F), dim3(kBlock), 0, s, v, m->approx, m->normal_raw, nrm);
     AFX_CHECK_LAUNCH();
     AFX_LAUNCH(kKGeomVolume, k_cell_volume<3>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx, vol, cc);
+    AFX_CHECK_LAUNCH();
+  }
+  return 0;
+}
+
+extern "C" int afx_geometry_vjp_workspace_bytes(const afx_mesh* m, int64_t* bytes) {
+  AFX_REQUIRE(m && bytes, "afx_geometry_vjp_workspace_bytes: null argument");
+  const int64_t D = m->dim;
+  *bytes = ((D + (2 * D + 1)) * (int64_t)m->C + D * (int64_t)m->n_inc) * 8 + 256;
+  return 0;
+}
+
+extern "C" int afx_geometry_vjp(const afx_mesh* m, const double* node_coords, const afx_geometry* g,
+                                const afx_geometry_cotangents* gb, double* node_b, void* work, int64_t work_bytes,
+                                hipStream_t s) {
+  AFX_REQUIRE(m && node_coords && g && gb && node_b && work, "afx_geometry_vjp: null argument");
+  AFX_REQUIRE(g->area && g->normal && g->face_centroid && g->volume && g->cell_centroid,
+              "afx_geometry_vjp: geometry pointer missing");
+  int64_t need = 0;
+  afx_geometry_vjp_workspace_bytes(m, &need);
+  AFX_REQUIRE(work_bytes >= need - 256, "afx_geometry_vjp: workspace too small");
+  AFX_REQUIRE(m->C > 0 && m->F > 0 && m->N > 0, "afx_geometry_vjp: empty mesh");
+  const int D = m->dim;
+  double* approx = (double*)work;
+  double* cellbar = approx + (int64_t)D * m->C;
+  double* inc_b = cellbar + (int64_t)(2 * D + 1) * m->C;
+  afx::MeshView v = afx_view(m, g);
+  if (D == 2) {
+    AFX_LAUNCH(kKGeomAdj, k_approx_centroid<2>, grid_for(m->C), dim3(kBlock), 0, s, v, approx);
+    AFX_CHECK_LAUNCH();
+    AFX_LAUNCH(kKGeomAdj, k_geom_cell_adj<2>, grid_for(m->C), dim3(kBlock), 0, s, v, approx, gb->volume,
+               gb->cell_centroid, cellbar);
+    AFX_CHECK_LAUNCH();
+    AFX_LAUNCH(kKGeomAdj, k_geom_face_adj<2>, grid_for(m->F), dim3(kBlock), 0, s, v, m->N, m->foff, m->fidx,
+               node_coords, approx, cellbar, gb->area, gb->normal, gb->face_centroid, inc_b, m->n_inc);
+    AFX_CHECK_LAUNCH();
+    AFX_LAUNCH(kKGeomAdj, k_geom_node_adj<2>, grid_for(m->N), dim3(kBlock), 0, s, m->N, m->nptr, m->ninc, inc_b,
+               m->n_inc, node_b);
+    AFX_CHECK_LAUNCH();
+  } else {
+    AFX_LAUNCH(kKGeomAdj, k_approx_centroid<3>, grid_for(m->C), dim3(kBlock), 0, s, v, approx);
+    AFX_CHECK_LAUNCH();
+    AFX_LAUNCH(kKGeomAdj, k_geom_cell_adj<3>, grid_for(m->C), dim3(kBlock), 0, s, v, approx, gb->volume,
+               gb->cell_centroid, cellbar);
+    AFX_CHECK_LAUNCH();
+    AFX_LAUNCH(kKGeomAdj, k_geom_face_adj<3>, grid_for(m->F), dim3(kBlock), 0, s, v, m->N, m->foff, m->fidx,
+               node_coords, approx, cellbar, gb->area, gb->normal, gb->face_centroid, inc_b, m->n_inc);
+    AFX_CHECK_LAUNCH();
+    AFX_LAUNCH(kKGeomAdj, k_geom_node_adj<3>, grid_for(m->N), dim3(kBlock), 0, s, m->N, m->nptr, m->ninc, inc_b,
+               m->n_inc, node_b);
     AFX_CHECK_LAUNCH();
   }
   return 0;

@@ -36,6 +36,7 @@ enum KernelId : int {
   kKRansAp,
   kKRansApAdj,
   kKRansGradAdj,
+  kKScalarLimAdj,
   kKNumKernels
 };
 

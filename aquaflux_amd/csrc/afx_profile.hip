@@ -15,7 +15,7 @@ const char* const kKernelNames[kKNumKernels] = {
     "scalar_grad",   "scalar_res",    "scalar_res_adj",  "scalar_grad_adj", "geom_adj",
     "rans_closure",  "rans_res",      "rans_res_adj",    "rans_closure_adj", "bcsr_assemble",
     "bcsr_spmv",     "pbj_setup",     "pbj_apply",       "blas",
-    "rans_grad",     "rans_ap",       "rans_ap_adj",     "rans_grad_adj",
+    "rans_grad",     "rans_ap",       "rans_ap_adj",     "rans_grad_adj",   "scalar_lim_adj",
 };
 
 namespace {

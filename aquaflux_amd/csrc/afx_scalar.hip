@@ -1,11 +1,25 @@
-// Cell gradient of a scalar field with Dirichlet / zero-gradient boundary closures (leading-order
-// boundary values), by compact Green-Gauss or least squares.
+// Scalar advection-diffusion residual R(phi) of ScalarTransport on a given face flux: forward,
+// tangent (dual instantiation) and adjoint -- with respect to phi, the diffusivity, the face flux,
+// the boundary values and the geometry (face areas, normals, centroids, cell volumes and
+// centroids), so that a node-coordinate adjoint follows from afx_geometry_vjp.
 //
-// Reference: ResidualAssembler.gradient (discretization/residual.py:253-296) with
-// CompactGreenGauss / LeastSquaresGradient (schemes/gradient.py) -- used for the SST wall-distance
-// gradient (turbulence/transport.py:86-110).
+// Reference: ScalarTransport.assembler / residual (transport/scalar.py:185-223) ->
+// ResidualAssembler.residual (discretization/residual.py:252-412) with (AdvectionFlux, DiffusionFlux)
+// summed in that order; CompactGreenGauss (schemes/gradient.py:129-153); VenkatakrishnanLimiter
+// (schemes/limiter.py:78-117); boundary closures (boundary/conditions.py:46-201).
+//
+// Passes (one thread per cell, gather over the cell's faces, no atomics):
+//   G  leading-order boundary values -> cell gradient (+ limiter psi and stencil-extremum owners)
+//   R  face fluxes Q phi_f - Gamma grad_n phi A summed per cell
+// Adjoint: G forward, then R' (state, flux, boundary-value and geometry cotangents of the face
+// fluxes), L' (limiter; gradient cotangent premultiplied by the reconstruction operator), G'.
+// Face-array cotangents are written by the face's owner thread only; each cell keeps its own
+// centroid / volume cotangent, so every output element has exactly one writer per pass.
+#include <type_traits>
+
 #include "afx_common.h"
 #include "afx_mesh.h"
+#include "afx_flow_face.h"
 #include "afx_scalar_face.h"
 #include "afx_profile.h"
 
@@ -13,8 +27,804 @@ using namespace afx;
 
 namespace {
 
-AFX_DEV double lead_bval(int8_t kind, double val, double phiP) {
-  if (kind == kBcDirichlet) return val;
+constexpr int kGradNone = 2;
+
+template <class S>
+struct ScArgs {
+  const int8_t* kind;  // boundary closure per face (AFX_BC_*)
+  In<S> phi, gam, flux, val, tinf;
+  In<S> G, psi;
+  double lim_k;
+};
+
+template <int D, class S>
+AFX_DEV void load_grad_row(const In<S>& G, int32_t C, int32_t c, S (&g)[D]) {
+#pragma unroll
+  for (int j = 0; j < D; ++j) g[j] = G[(int64_t)j * C + c];
+}
+
+// ---------------------------------------------------------------------------------------------
+// pass G
+template <int D, class S, int GRAD, bool LIM>
+__global__ void __launch_bounds__(256) k_sc_grad(MeshView m, ScArgs<S> a, Out<S> G, Out<S> psi_out,
+                                                 int32_t* __restrict__ ext) {
+  const int32_t c = xcd_thread_index();
+  if (c >= m.C) return;
+  const S xc = a.phi[c];
+  S acc[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) acc[j] = sconst<S>(0.0);
+  double M[D][D];
+#pragma unroll
+  for (int p = 0; p < D; ++p)
+#pragma unroll
+    for (int q = 0; q < D; ++q) M[p][q] = 0.0;
+  S mx = xc, mn = xc;
+  int32_t amx = c, amn = c;
+  const int32_t kb = m.cptr[c], ke = m.cptr[c + 1];
+  S zero[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) zero[j] = sconst<S>(0.0);
+  for (int32_t k = kb; k < ke; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    if (o >= 0) {
+      const S xo = a.phi[o];
+      if (LIM) {
+        if (val(xo) > val(mx)) {
+          mx = xo;
+          amx = o;
+        }
+        if (val(xo) < val(mn)) {
+          mn = xo;
+          amn = o;
+        }
+      }
+      if (GRAD == 0) {
+        FaceGeo<D> q;
+        load_face_geo<D>(m, r.f, r.P, r.N, q);
+        const double w = 1.0 - q.g;
+        const S pf = w * (r.own ? xc : xo) + q.g * (r.own ? xo : xc);
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          const S t = pf * (q.n[j] * q.A);
+          acc[j] = r.own ? acc[j] + t : acc[j] - t;
+        }
+      } else if (GRAD == 1) {
+        double dx[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dx[j] = m.cc[(int64_t)j * m.C + o] - m.cc[(int64_t)j * m.C + c];
+        const double wgt = 1.0 / vdot<D>(dx, dx);
+#pragma unroll
+        for (int p = 0; p < D; ++p)
+#pragma unroll
+          for (int q2 = 0; q2 < D; ++q2) M[p][q2] += wgt * dx[p] * dx[q2];
+        const S dphi = xo - xc;
+#pragma unroll
+        for (int j = 0; j < D; ++j) acc[j] += (wgt * dx[j]) * dphi;
+      }
+    } else if (GRAD != kGradNone) {
+      FaceGeo<D> q;
+      load_face_geo<D>(m, r.f, r.P, r.N, q);
+      const S pb = scalar_bval<D, S>(a.kind[r.f], q, xc, zero, a.val[r.f], a.gam[c], a.tinf[r.f]);
+      if (GRAD == 0) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) acc[j] += pb * (q.n[j] * q.A);
+      } else {
+        double dx[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dx[j] = q.dp[j];
+        const double wgt = 1.0 / vdot<D>(dx, dx);
+#pragma unroll
+        for (int p = 0; p < D; ++p)
+#pragma unroll
+          for (int q2 = 0; q2 < D; ++q2) M[p][q2] += wgt * dx[p] * dx[q2];
+        const S dphi = pb - xc;
+#pragma unroll
+        for (int j = 0; j < D; ++j) acc[j] += (wgt * dx[j]) * dphi;
+      }
+    }
+  }
+  S grad[D];
+  if (GRAD == 0) {
+    const double iv = 1.0 / m.vol[c];
+#pragma unroll
+    for (int j = 0; j < D; ++j) grad[j] = acc[j] * iv;
+  } else if (GRAD == 1) {
+    double I[D][D];
+    sym_inverse<D>(M, I);
+#pragma unroll
+    for (int p = 0; p < D; ++p) {
+      S s = I[p][0] * acc[0];
+#pragma unroll
+      for (int q2 = 1; q2 < D; ++q2) s = s + I[p][q2] * acc[q2];
+      grad[p] = s;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < D; ++j) grad[j] = sconst<S>(0.0);
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) G.set((int64_t)j * m.C + c, grad[j]);
+  if (LIM) {
+    const double e2 = m.vol[c] * (a.lim_k * a.lim_k * a.lim_k);
+    double xcen[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) xcen[j] = m.cc[(int64_t)j * m.C + c];
+    S lo = sconst<S>(0.0);
+    bool first = true;
+    for (int32_t k = kb; k < ke; ++k) {
+      const int32_t e = m.cface[k];
+      const int32_t f = e >= 0 ? e : ~e;
+      S dm = grad[0] * (m.fc[f] - xcen[0]);
+#pragma unroll
+      for (int j = 1; j < D; ++j) dm = dm + grad[j] * (m.fc[(int64_t)j * m.F + f] - xcen[j]);
+      const S vf = venkat_face<S>(dm, mx - xc, mn - xc, e2);
+      if (first || val(vf) < val(lo)) lo = vf;
+      first = false;
+    }
+    psi_out.set(c, dmin(dmax(lo, 0.0), 1.0));
+    if (ext) {
+      ext[c] = amx;
+      ext[(int64_t)m.C + c] = amn;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// pass R
+template <int D, class S, int ADV>
+__global__ void __launch_bounds__(256) k_sc_res(MeshView m, ScArgs<S> a, Out<S> R) {
+  const int32_t c = xcd_thread_index();
+  if (c >= m.C) return;
+  const S xc = a.phi[c], gc = a.gam[c];
+  const S pc = ADV == kAdvLimited ? a.psi[c] : sconst<S>(1.0);
+  S Gc[D];
+  load_grad_row<D, S>(a.G, m.C, c, Gc);
+  S acc = sconst<S>(0.0);
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    FaceGeo<D> q;
+    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    const S Qf = a.flux[r.f];
+    S fl;
+    if (o >= 0) {
+      const S xo = a.phi[o], go = a.gam[o];
+      const S po = ADV == kAdvLimited ? a.psi[o] : sconst<S>(1.0);
+      S Go[D];
+      load_grad_row<D, S>(a.G, m.C, o, Go);
+      if (r.own)
+        fl = scalar_flux_interior<D, ADV, S>(q, Qf, xc, xo, Gc, Go, pc, po, gc, go);
+      else
+        fl = scalar_flux_interior<D, ADV, S>(q, Qf, xo, xc, Go, Gc, po, pc, go, gc);
+    } else {
+      const S bv = scalar_bval<D, S>(a.kind[r.f], q, xc, Gc, a.val[r.f], gc, a.tinf[r.f]);
+      fl = scalar_flux_boundary<D, ADV, S>(q, Qf, xc, Gc, pc, bv, gc);
+    }
+    acc = r.own ? acc + fl : acc - fl;
+  }
+  R.set(c, acc);
+}
+
+// ---------------------------------------------------------------------------------------------
+// adjoint helpers with geometry cotangents
+
+// Cotangents of the quantities a face's physics reads from its FaceGeo.
+template <int D>
+struct FaceGeoBar {
+  double A, n[D], xf[D], xP[D], xN[D], g, dp[D], dn[D], dpn, dnn;
+  AFX_DEV void zero() {
+    A = g = dpn = dnn = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) n[j] = xf[j] = xP[j] = xN[j] = dp[j] = dn[j] = 0.0;
+  }
+};
+
+// Folds the derived cotangents (g, dp, dn, dpn, dnn) into the primitive ones (A, n, xf, xP, xN),
+// reversing load_face_geo.  On a boundary face xN aliases xP.
+template <int D>
+AFX_DEV void adj_face_geo(const FaceGeo<D>& q, bool interior, FaceGeoBar<D>& b) {
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    b.dp[j] += b.dpn * q.n[j];
+    b.n[j] += b.dpn * q.dp[j] + b.dnn * q.dn[j];
+    b.dn[j] += b.dnn * q.n[j];
+  }
+  if (interior) {
+    const double dd = vdot<D>(q.dvec, q.dvec);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      b.dp[j] += b.g * q.dvec[j] / dd;
+      const double dvb = b.g * (q.dp[j] - 2.0 * q.g * q.dvec[j]) / dd;
+      b.xN[j] += dvb;
+      b.xP[j] -= dvb;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    b.xf[j] += b.dp[j] + b.dn[j];
+    b.xP[j] -= b.dp[j];
+    b.xN[j] -= b.dn[j];
+    if (!interior) {
+      b.xP[j] += b.xN[j];
+      b.xN[j] = 0.0;
+    }
+    b.dp[j] = b.dn[j] = 0.0;
+  }
+  b.dpn = b.dnn = b.g = 0.0;
+}
+
+// corr = G . (d - n (d.n)) for d = dp (side 0) or dn (side 1), cotangent cb: the gradient part ...
+template <int D>
+AFX_DEV void adj_corr_state(const FaceGeo<D>& q, int side, double cb, double* Gb) {
+  const double* d = side == 0 ? q.dp : q.dn;
+  const double dn_ = side == 0 ? q.dpn : q.dnn;
+#pragma unroll
+  for (int j = 0; j < D; ++j) Gb[j] += cb * (d[j] - q.n[j] * dn_);
+}
+
+// ... and the geometry part.
+template <int D>
+AFX_DEV void adj_corr_geo(const FaceGeo<D>& q, int side, const double* G, double cb, FaceGeoBar<D>& qb) {
+  const double dn_ = side == 0 ? q.dpn : q.dnn;
+  double gn = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) gn += G[j] * q.n[j];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    if (side == 0)
+      qb.dp[j] += cb * G[j];
+    else
+      qb.dn[j] += cb * G[j];
+    qb.n[j] -= cb * G[j] * dn_;
+  }
+  if (side == 0)
+    qb.dpn -= cb * gn;
+  else
+    qb.dnn -= cb * gn;
+}
+
+// Adjoint of scalar_bval (owner P, owner gradient G); bb = cotangent of the boundary value.
+template <int D, bool GEO>
+AFX_DEV void adj_sc_bval(int kind, const FaceGeo<D>& q, double phiP, const double* G, double v, double gam,
+                         double tinf, double bb, ScalarBar<D>& B, double& val_b, double& tinf_b,
+                         FaceGeoBar<D>& qb) {
+  if (kind == kBcDirichlet) {
+    val_b += bb;
+    return;
+  }
+  if (kind == kBcNone) return;
+  double corr = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) corr += G[j] * (q.dp[j] - q.n[j] * q.dpn);
+  double ub = bb;  // cotangent of phiP + corr
+  if (kind == kBcNeumann) {
+    // phiP + corr - v / gam * dpn
+    val_b -= bb * q.dpn / gam;
+    B.gam += bb * v * q.dpn / (gam * gam);
+    if (GEO) qb.dpn -= bb * v / gam;
+  } else if (kind == kBcConvective) {
+    const double beta = v / gam * q.dpn;
+    const double u = phiP + corr + beta * tinf;
+    const double b = u / (1.0 + beta);
+    ub = bb / (1.0 + beta);
+    const double beta_b = ub * (tinf - b);
+    tinf_b += ub * beta;
+    val_b += beta_b * q.dpn / gam;
+    B.gam -= beta_b * v * q.dpn / (gam * gam);
+    if (GEO) qb.dpn += beta_b * v / gam;
+  }
+  B.phi += ub;
+  adj_corr_state<D>(q, 0, ub, B.G);
+  if (GEO) adj_corr_geo<D>(q, 0, G, ub, qb);
+}
+
+// Adjoint of the interior-face flux for the side `side` (0 = owner P, 1 = neighbour N); the face
+// cotangent is lb.  Accumulates that side's state cotangents into B and, with GEO, the full face
+// geometry cotangent into qb (identical on both sides).  Returns the cotangent of Q.
+template <int D, int ADV, bool GEO>
+AFX_DEV double adj_sc_interior(const FaceGeo<D>& q, double Qf, double phiP, double phiN, const double* GP,
+                               const double* GN, double psiP, double psiN, double gamP, double gamN, double lb,
+                               int side, ScalarBar<D>& B, FaceGeoBar<D>& qb) {
+  double Q_bar = 0.0;
+  if (ADV != kAdvNone) {
+    const bool outflow = Qf >= 0.0;
+    const double fv = scalar_face_value<D, ADV, double>(q, outflow, phiP, phiN, GP, GN, psiP, psiN);
+    Q_bar = lb * fv;
+    const double fb = lb * Qf;
+    const bool mine = (side == 0) == outflow;
+    if (mine) B.phi += fb;
+    if (ADV == kAdvLimited) {
+      const double* xu = outflow ? q.xP : q.xN;
+      const double* Gu = outflow ? GP : GN;
+      const double ps = outflow ? psiP : psiN;
+      double t = 0.0;
+#pragma unroll
+      for (int j = 0; j < D; ++j) t += Gu[j] * (q.xf[j] - xu[j]);
+      if (mine) {
+        B.psi += fb * t;
+#pragma unroll
+        for (int j = 0; j < D; ++j) B.G[j] += fb * ps * (q.xf[j] - xu[j]);
+      }
+      if (GEO) {
+        const double tb = fb * ps;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          qb.xf[j] += tb * Gu[j];
+          if (outflow)
+            qb.xP[j] -= tb * Gu[j];
+          else
+            qb.xN[j] -= tb * Gu[j];
+        }
+      }
+    }
+  }
+  double cp = 0.0, cn = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    cp += GP[j] * (q.dp[j] - q.n[j] * q.dpn);
+    cn += GN[j] * (q.dn[j] - q.n[j] * q.dnn);
+  }
+  const double ratio = gamP / gamN;
+  const double denom = q.dpn - ratio * q.dnn;
+  const double ng = (((phiN - phiP) + cn) - cp) / denom;
+  const double ng_bar = lb * (-gamP) * q.A;
+  const double num_bar = ng_bar / denom;
+  const double denom_bar = -ng_bar * ng / denom;
+  const double ratio_bar = -denom_bar * q.dnn;
+  if (side == 0) {
+    B.gam += -lb * ng * q.A + ratio_bar / gamN;
+    B.phi -= num_bar;
+    adj_corr_state<D>(q, 0, -num_bar, B.G);
+  } else {
+    B.gam -= ratio_bar * ratio / gamN;
+    B.phi += num_bar;
+    adj_corr_state<D>(q, 1, num_bar, B.G);
+  }
+  if (GEO) {
+    qb.A += -lb * gamP * ng;
+    qb.dpn += denom_bar;
+    qb.dnn -= denom_bar * ratio;
+    adj_corr_geo<D>(q, 0, GP, -num_bar, qb);
+    adj_corr_geo<D>(q, 1, GN, num_bar, qb);
+  }
+  return Q_bar;
+}
+
+// Adjoint of the boundary-face flux (owner side) including its boundary value.  Returns Q's cotangent.
+template <int D, int ADV, bool GEO>
+AFX_DEV double adj_sc_boundary(int kind, const FaceGeo<D>& q, double Qf, double phiP, const double* GP, double psiP,
+                               double gam, double v, double tinf, double lb, ScalarBar<D>& B, double& val_b,
+                               double& tinf_b, FaceGeoBar<D>& qb) {
+  const double bval = scalar_bval<D, double>(kind, q, phiP, GP, v, gam, tinf);
+  double bval_b = 0.0, Q_bar = 0.0;
+  if (ADV != kAdvNone) {
+    const bool outflow = Qf >= 0.0;
+    const double fb = lb * Qf;
+    if (outflow) {
+      if (ADV == kAdvLimited) {
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) t += GP[j] * (q.xf[j] - q.xP[j]);
+        Q_bar = lb * (phiP + psiP * t);
+        B.phi += fb;
+        B.psi += fb * t;
+#pragma unroll
+        for (int j = 0; j < D; ++j) B.G[j] += fb * psiP * (q.xf[j] - q.xP[j]);
+        if (GEO) {
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+            qb.xf[j] += fb * psiP * GP[j];
+            qb.xP[j] -= fb * psiP * GP[j];
+          }
+        }
+      } else {
+        Q_bar = lb * phiP;
+        B.phi += fb;
+      }
+    } else {
+      Q_bar = lb * bval;
+      bval_b += fb;
+    }
+  }
+  double cp = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) cp += GP[j] * (q.dp[j] - q.n[j] * q.dpn);
+  const double ng = ((bval - phiP) - cp) / q.dpn;
+  B.gam += -lb * ng * q.A;
+  const double ng_bar = lb * (-gam) * q.A;
+  const double num_bar = ng_bar / q.dpn;
+  bval_b += num_bar;
+  B.phi -= num_bar;
+  adj_corr_state<D>(q, 0, -num_bar, B.G);
+  if (GEO) {
+    adj_corr_geo<D>(q, 0, GP, -num_bar, qb);
+    qb.A += -lb * gam * ng;
+    qb.dpn -= ng_bar * ng / q.dpn;
+  }
+  adj_sc_bval<D, GEO>(kind, q, phiP, GP, v, gam, tinf, bval_b, B, val_b, tinf_b, qb);
+  return Q_bar;
+}
+
+struct ScBars {
+  double* phi;   // [C]
+  double* gam;   // [C]
+  double* flux;  // [F]
+  double* val;   // [F]
+  double* tinf;  // [F]
+  double* G;     // [D][C] scratch
+  double* psi;   // [C]    scratch
+  double* ext;   // [2][C] scratch: cotangents of the stencil extrema
+  double* limx;  // [D][C] scratch: limiter's face-centroid cotangent (at its argmin face)
+  int32_t* limf; // [C]    scratch: limiter argmin face (-1 none)
+  // geometry cotangents (null when not requested)
+  double *area, *nrm, *fc, *vol, *cc;
+};
+
+template <int D, bool GEO>
+AFX_DEV void write_face_geo_bar(const ScBars& bar, int32_t F, int32_t f, const FaceGeoBar<D>& qb, bool accumulate) {
+  if (!GEO) return;
+  if (accumulate) {
+    bar.area[f] += qb.A;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      bar.nrm[(int64_t)j * F + f] += qb.n[j];
+      bar.fc[(int64_t)j * F + f] += qb.xf[j];
+    }
+  } else {
+    bar.area[f] = qb.A;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      bar.nrm[(int64_t)j * F + f] = qb.n[j];
+      bar.fc[(int64_t)j * F + f] = qb.xf[j];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// reverse pass R'
+template <int D, int ADV, bool GEO>
+__global__ void __launch_bounds__(256) k_sc_res_adj(MeshView m, ScArgs<double> a, const double* __restrict__ lam,
+                                                    ScBars bar) {
+  const int32_t c = xcd_thread_index();
+  if (c >= m.C) return;
+  const double xc = a.phi[c], gc = a.gam[c];
+  const double pc = ADV == kAdvLimited ? a.psi[c] : 1.0;
+  double Gc[D];
+  load_grad_row<D, double>(a.G, m.C, c, Gc);
+  const double lc = lam[c];
+  ScalarBar<D> B;
+  B.zero();
+  double ccb[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) ccb[j] = 0.0;
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    FaceGeo<D> q;
+    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    FaceGeoBar<D> qb;
+    qb.zero();
+    const double Qf = a.flux[r.f];
+    if (o >= 0) {
+      const double xo = a.phi[o], go = a.gam[o];
+      const double po = ADV == kAdvLimited ? a.psi[o] : 1.0;
+      double Go[D];
+      load_grad_row<D, double>(a.G, m.C, o, Go);
+      const double lo = lam[o];
+      const double lf = r.own ? lc - lo : lo - lc;
+      double Qb;
+      if (r.own)
+        Qb = adj_sc_interior<D, ADV, GEO>(q, Qf, xc, xo, Gc, Go, pc, po, gc, go, lf, 0, B, qb);
+      else
+        Qb = adj_sc_interior<D, ADV, GEO>(q, Qf, xo, xc, Go, Gc, po, pc, go, gc, lf, 1, B, qb);
+      if (GEO) adj_face_geo<D>(q, true, qb);
+      if (r.own) {
+        bar.flux[r.f] = Qb;
+        bar.val[r.f] = 0.0;
+        bar.tinf[r.f] = 0.0;
+        write_face_geo_bar<D, GEO>(bar, m.F, r.f, qb, false);
+      }
+#pragma unroll
+      for (int j = 0; j < D; ++j) ccb[j] += r.own ? qb.xP[j] : qb.xN[j];
+    } else {
+      double vb = 0.0, tb = 0.0;
+      const double Qb = adj_sc_boundary<D, ADV, GEO>(a.kind[r.f], q, Qf, xc, Gc, pc, gc, a.val[r.f], a.tinf[r.f],
+                                                     lc, B, vb, tb, qb);
+      if (GEO) adj_face_geo<D>(q, false, qb);
+      bar.flux[r.f] = Qb;
+      bar.val[r.f] = vb;
+      bar.tinf[r.f] = tb;
+      write_face_geo_bar<D, GEO>(bar, m.F, r.f, qb, false);
+#pragma unroll
+      for (int j = 0; j < D; ++j) ccb[j] += qb.xP[j];
+    }
+  }
+  bar.phi[c] = B.phi;
+  bar.gam[c] = B.gam;
+  bar.psi[c] = B.psi;
+#pragma unroll
+  for (int j = 0; j < D; ++j) bar.G[(int64_t)j * m.C + c] = B.G[j];
+  if (GEO) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) bar.cc[(int64_t)j * m.C + c] = ccb[j];
+    bar.vol[c] = 0.0;
+  }
+}
+
+// Venkatakrishnan face value with a differentiable e2 (for the volume cotangent).
+AFX_DEV dual venkat_face_de2(double dm0, double hmax, double hmin, double e2) {
+  const double sgn = dm0 >= 0.0 ? 1.0 : -1.0;
+  const double dm = sgn * (fabs(dm0) + 1e-12);
+  const double head = dm > 0.0 ? hmax : hmin;
+  const dual E{e2, 1.0};
+  const double h2 = head * head, dm2 = dm * dm;
+  const dual num = (h2 + E) * dm + 2.0 * dm2 * head;
+  const dual den = (h2 + 2.0 * dm2 + head * dm + E) * dm;
+  return num / den;
+}
+
+// reverse pass L': limiter, then the gradient cotangent is premultiplied by the reconstruction
+// operator (GG: 1/V, LSQ: M^-1); GG also yields the volume cotangent -Gbar.G / V.
+template <int D, int GRAD, bool LIM, bool GEO>
+__global__ void __launch_bounds__(256) k_sc_lim_adj(MeshView m, ScArgs<double> a, const int32_t* __restrict__ ext,
+                                                    ScBars bar) {
+  const int32_t c = xcd_thread_index();
+  if (c >= m.C) return;
+  double Gb[D], Gc[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) Gb[j] = bar.G[(int64_t)j * m.C + c];
+  load_grad_row<D, double>(a.G, m.C, c, Gc);
+  double xb = 0.0, volb = 0.0, xcb[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) xcb[j] = 0.0;
+  if (LIM) {
+    const double V = m.vol[c];
+    const double K3 = a.lim_k * a.lim_k * a.lim_k;
+    const double e2 = V * K3;
+    double xcen[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) xcen[j] = m.cc[(int64_t)j * m.C + c];
+    const double pbar = bar.psi[c];
+    const int32_t amx = ext[c], amn = ext[(int64_t)m.C + c];
+    const double xv = a.phi[c], mxv = a.phi[amx], mnv = a.phi[amn];
+    double lo = 0.0;
+    int32_t kmin = -1;
+    for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+      const int32_t e = m.cface[k];
+      const int32_t f = e >= 0 ? e : ~e;
+      double dmf = Gc[0] * (m.fc[f] - xcen[0]);
+#pragma unroll
+      for (int j = 1; j < D; ++j) dmf = dmf + Gc[j] * (m.fc[(int64_t)j * m.F + f] - xcen[j]);
+      const double vf = venkat_face<double>(dmf, mxv - xv, mnv - xv, e2);
+      if (kmin < 0 || vf < lo) {
+        lo = vf;
+        kmin = k;
+      }
+    }
+    const double dclip = (lo > 0.0 && lo < 1.0) ? 1.0 : ((lo == 0.0 || lo == 1.0) ? 0.5 : 0.0);
+    const double vbar = pbar * dclip;
+    double mxb = 0.0, mnb = 0.0;
+    int32_t fmin = -1;
+    double xfb[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) xfb[j] = 0.0;
+    if (vbar != 0.0 && kmin >= 0) {
+      const int32_t e = m.cface[kmin];
+      const int32_t f = e >= 0 ? e : ~e;
+      double dxf[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) dxf[j] = m.fc[(int64_t)j * m.F + f] - xcen[j];
+      double dmf = Gc[0] * dxf[0];
+#pragma unroll
+      for (int j = 1; j < D; ++j) dmf = dmf + Gc[j] * dxf[j];
+      const dual r_dm = venkat_face<dual>(dual{dmf, 1.0}, dual{mxv - xv, 0.0}, dual{mnv - xv, 0.0}, e2);
+      const dual r_hx = venkat_face<dual>(dual{dmf, 0.0}, dual{mxv - xv, 1.0}, dual{mnv - xv, 0.0}, e2);
+      const dual r_hn = venkat_face<dual>(dual{dmf, 0.0}, dual{mxv - xv, 0.0}, dual{mnv - xv, 1.0}, e2);
+      const double dmb = vbar * r_dm.d;
+#pragma unroll
+      for (int j = 0; j < D; ++j) Gb[j] += dmb * dxf[j];
+      if (GEO) {
+        fmin = f;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          xfb[j] = dmb * Gc[j];
+          xcb[j] -= dmb * Gc[j];
+        }
+        volb += vbar * venkat_face_de2(dmf, mxv - xv, mnv - xv, e2).d * K3;
+      }
+      mxb = vbar * r_hx.d;
+      mnb = vbar * r_hn.d;
+      xb -= mxb + mnb;
+      if (amx == c) {
+        xb += mxb;
+        mxb = 0.0;
+      }
+      if (amn == c) {
+        xb += mnb;
+        mnb = 0.0;
+      }
+    }
+    bar.ext[c] = mxb;
+    bar.ext[(int64_t)m.C + c] = mnb;
+    if (GEO) {
+      bar.limf[c] = fmin;
+#pragma unroll
+      for (int j = 0; j < D; ++j) bar.limx[(int64_t)j * m.C + c] = xfb[j];
+    }
+  }
+  bar.phi[c] += xb;
+  if (GRAD == 0) {
+    const double iv = 1.0 / m.vol[c];
+    if (GEO) {
+      double gg = 0.0;
+#pragma unroll
+      for (int j = 0; j < D; ++j) gg += Gb[j] * Gc[j];
+      volb -= gg * iv;
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j) bar.G[(int64_t)j * m.C + c] = Gb[j] * iv;
+  } else if (GRAD == 1) {
+    double M[D][D], I[D][D];
+    lsq_matrix<D>(m, c, M);
+    sym_inverse<D>(M, I);
+#pragma unroll
+    for (int p = 0; p < D; ++p) {
+      double s = 0.0;
+#pragma unroll
+      for (int q2 = 0; q2 < D; ++q2) s += I[p][q2] * Gb[q2];
+      bar.G[(int64_t)p * m.C + c] = s;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < D; ++j) bar.G[(int64_t)j * m.C + c] = 0.0;
+  }
+  if (GEO) {
+    bar.vol[c] += volb;
+#pragma unroll
+    for (int j = 0; j < D; ++j) bar.cc[(int64_t)j * m.C + c] += xcb[j];
+  }
+}
+
+// reverse pass G'
+template <int D, int GRAD, bool LIM, bool GEO>
+__global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> a, const int32_t* __restrict__ ext,
+                                                     ScBars bar) {
+  const int32_t c = xcd_thread_index();
+  if (c >= m.C) return;
+  double Bc[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) Bc[j] = bar.G[(int64_t)j * m.C + c];
+  const double xc = a.phi[c];
+  double xb = 0.0, gamb = 0.0, ccb[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) ccb[j] = 0.0;
+  double zero[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) zero[j] = 0.0;
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    FaceGeoBar<D> qb;
+    qb.zero();
+    bool have_geo = false;
+    if (o >= 0) {
+      if (GRAD == 0) {
+        FaceGeo<D> q;
+        load_face_geo<D>(m, r.f, r.P, r.N, q);
+        double BP[D], BN[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          BP[j] = bar.G[(int64_t)j * m.C + r.P];
+          BN[j] = bar.G[(int64_t)j * m.C + r.N];
+        }
+        double pb_ = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) pb_ += (q.n[j] * q.A) * (BP[j] - BN[j]);
+        xb += (r.own ? 1.0 - q.g : q.g) * pb_;
+        if (GEO) {
+          const double phP = a.phi[r.P], phN = a.phi[r.N];
+          const double phf = (1.0 - q.g) * phP + q.g * phN;
+          double dn = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+            qb.n[j] += phf * q.A * (BP[j] - BN[j]);
+            dn += q.n[j] * (BP[j] - BN[j]);
+          }
+          qb.A += phf * dn;
+          qb.g += pb_ * (phN - phP);
+          adj_face_geo<D>(q, true, qb);
+          have_geo = true;
+        }
+      } else if (GRAD == 1) {
+        double dx[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dx[j] = m.cc[(int64_t)j * m.C + o] - m.cc[(int64_t)j * m.C + c];
+        const double wgt = 1.0 / vdot<D>(dx, dx);
+        double own_dot = 0.0, oth_dot = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          own_dot += dx[j] * Bc[j];
+          oth_dot += dx[j] * bar.G[(int64_t)j * m.C + o];
+        }
+        xb += -wgt * own_dot - wgt * oth_dot;
+      }
+      if (LIM) {
+        if (ext[o] == c) xb += bar.ext[o];
+        if (ext[(int64_t)m.C + o] == c) xb += bar.ext[(int64_t)m.C + o];
+      }
+    } else if (GRAD != kGradNone) {
+      FaceGeo<D> q;
+      load_face_geo<D>(m, r.f, r.P, r.N, q);
+      double phib;
+      if (GRAD == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) s += (q.n[j] * q.A) * Bc[j];
+        phib = s;
+        if (GEO) {
+          const double pbv = scalar_bval<D, double>(a.kind[r.f], q, xc, zero, a.val[r.f], a.gam[c], a.tinf[r.f]);
+#pragma unroll
+          for (int j = 0; j < D; ++j) qb.n[j] += pbv * q.A * Bc[j];
+          double dn = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) dn += q.n[j] * Bc[j];
+          qb.A += pbv * dn;
+        }
+      } else {
+        const double wgt = 1.0 / vdot<D>(q.dp, q.dp);
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) s += q.dp[j] * Bc[j];
+        phib = wgt * s;
+        xb -= wgt * s;
+      }
+      ScalarBar<D> Bl;
+      Bl.zero();
+      double vb = 0.0, tb = 0.0;
+      adj_sc_bval<D, GEO>(a.kind[r.f], q, xc, zero, a.val[r.f], a.gam[c], a.tinf[r.f], phib, Bl, vb, tb, qb);
+      xb += Bl.phi;
+      gamb += Bl.gam;
+      bar.val[r.f] += vb;
+      bar.tinf[r.f] += tb;
+      if (GEO) {
+        adj_face_geo<D>(q, false, qb);
+        have_geo = true;
+      }
+    }
+    if (GEO) {
+      // the limiters of this face's cells that took their minimum here
+      if (r.own) {
+        if (LIM && bar.limf[r.P] == r.f) {
+#pragma unroll
+          for (int j = 0; j < D; ++j) qb.xf[j] += bar.limx[(int64_t)j * m.C + r.P];
+        }
+        if (LIM && r.N >= 0 && bar.limf[r.N] == r.f) {
+#pragma unroll
+          for (int j = 0; j < D; ++j) qb.xf[j] += bar.limx[(int64_t)j * m.C + r.N];
+        }
+        write_face_geo_bar<D, GEO>(bar, m.F, r.f, qb, true);
+      }
+      if (have_geo) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) ccb[j] += r.own ? qb.xP[j] : qb.xN[j];
+      }
+    }
+  }
+  bar.phi[c] += xb;
+  bar.gam[c] += gamb;
+  if (GEO) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) bar.cc[(int64_t)j * m.C + c] += ccb[j];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// gradient of a scalar with Dirichlet / zero-gradient leading-order boundary values (used for the
+// SST wall-distance gradient)
+AFX_DEV double lead_bval(int8_t kind, double v, double phiP) {
+  if (kind == kBcDirichlet) return v;
   if (kind == kBcNone) return 0.0;
   return phiP;
 }
@@ -100,6 +910,120 @@ __global__ void __launch_bounds__(256) k_scalar_grad(MeshView m, const int8_t* _
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// host side
+
+// Workspace in doubles: mode 0 = G [D][C], psi [C], ext (2 int32 per cell); mode 1 adds tangents of
+// G and psi; mode 2 adds the adjoint scratch (G, psi, ext, limiter face cotangent, argmin face).
+int64_t sc_work_doubles(int D, int64_t C, int mode) {
+  int64_t n = (int64_t)(D + 1 + 1) * C;
+  if (mode == 1) n += (int64_t)(D + 1) * C;
+  if (mode == 2) n += (int64_t)(D + 1 + 2 + D + 1) * C;
+  return n;
+}
+
+struct ScWork {
+  double *G, *psi, *Gt, *psit;
+  int32_t* ext;
+  double *Gb, *psib, *extb, *limx;
+  int32_t* limf;
+};
+
+ScWork sc_carve(void* base, int D, int64_t C, int mode) {
+  double* p = (double*)base;
+  ScWork w{};
+  w.G = p;
+  p += (int64_t)D * C;
+  w.psi = p;
+  p += C;
+  w.ext = (int32_t*)p;
+  p += C;
+  if (mode == 1) {
+    w.Gt = p;
+    p += (int64_t)D * C;
+    w.psit = p;
+    p += C;
+  }
+  if (mode == 2) {
+    w.Gb = p;
+    p += (int64_t)D * C;
+    w.psib = p;
+    p += C;
+    w.extb = p;
+    p += 2 * C;
+    w.limx = p;
+    p += (int64_t)D * C;
+    w.limf = (int32_t*)p;
+    p += C;
+  }
+  return w;
+}
+
+template <class S>
+ScArgs<S> sc_args(const afx_scalar_config* cfg, const afx_scalar_inputs* in, const afx_scalar_inputs* tan) {
+  ScArgs<S> a{};
+  a.kind = cfg->bc_kind;
+  a.phi.v = in->phi;
+  a.gam.v = in->gamma;
+  a.flux.v = in->flux;
+  a.val.v = in->value;
+  a.tinf.v = in->t_inf;
+  if (tan) {
+    a.phi.t = tan->phi;
+    a.gam.t = tan->gamma;
+    a.flux.t = tan->flux;
+    a.val.t = tan->value;
+    a.tinf.t = tan->t_inf;
+  }
+  a.lim_k = cfg->limiter_k;
+  return a;
+}
+
+template <class Fn>
+int sc_dispatch(int D, int adv, int grad, Fn&& fn) {
+#define AFX_S(DD, AA, GG)                                                                               \
+  if (D == DD && adv == AA && grad == GG)                                                               \
+    return fn(std::integral_constant<int, DD>{}, std::integral_constant<int, AA>{},                      \
+              std::integral_constant<int, GG>{});
+  AFX_S(2, 0, 0) AFX_S(2, 1, 0) AFX_S(2, 2, 0) AFX_S(2, 0, 1) AFX_S(2, 1, 1) AFX_S(2, 2, 1)
+  AFX_S(2, 0, 2) AFX_S(2, 1, 2) AFX_S(2, 2, 2)
+  AFX_S(3, 0, 0) AFX_S(3, 1, 0) AFX_S(3, 2, 0) AFX_S(3, 0, 1) AFX_S(3, 1, 1) AFX_S(3, 2, 1)
+  AFX_S(3, 0, 2) AFX_S(3, 1, 2) AFX_S(3, 2, 2)
+#undef AFX_S
+  afx_set_error("scalar: unsupported (dim, advection, gradient) combination");
+  return 2;
+}
+
+template <int D, class S, int ADV, int GRAD>
+int sc_forward(const afx_mesh* mh, const afx_geometry* g, ScArgs<S> a, Out<S> R, const ScWork& w, double* Gt,
+               double* psit, int32_t* ext, hipStream_t s) {
+  MeshView m = afx_view(mh, g);
+  constexpr bool LIM = ADV == kAdvLimited;
+  AFX_LAUNCH(kKScalarGrad, (k_sc_grad<D, S, GRAD, LIM>), grid_for(m.C), dim3(kBlock), 0, s, m, a,
+             Out<S>{w.G, Gt}, Out<S>{w.psi, psit}, ext);
+  AFX_CHECK_LAUNCH();
+  if (R.v == nullptr) return 0;
+  a.G.v = w.G;
+  a.G.t = Gt;
+  a.psi.v = w.psi;
+  a.psi.t = psit;
+  AFX_LAUNCH(kKScalarRes, (k_sc_res<D, S, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, R);
+  AFX_CHECK_LAUNCH();
+  return 0;
+}
+
+int sc_check(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cfg, const afx_scalar_inputs* in,
+             void* work, int64_t work_size, int mode) {
+  AFX_REQUIRE(m && g && cfg && in, "scalar: null argument");
+  AFX_REQUIRE(in->phi && in->gamma && in->flux && in->value && in->t_inf, "scalar: a required input pointer is null");
+  AFX_REQUIRE(cfg->bc_kind, "scalar: bc_kind is null");
+  AFX_REQUIRE(cfg->advection >= 0 && cfg->advection <= 2, "scalar: unknown advection scheme");
+  AFX_REQUIRE(cfg->gradient >= 0 && cfg->gradient <= 2, "scalar: unknown gradient scheme");
+  AFX_REQUIRE(m->C > 0, "scalar: empty mesh");
+  AFX_REQUIRE(work && work_size >= sc_work_doubles(m->dim, m->C, mode) * 8, "scalar: workspace too small");
+  return 0;
+}
+
 }  // namespace
 
 extern "C" int afx_scalar_gradient(const afx_mesh* mh, const afx_geometry* g, int gradient, const int8_t* kind,
@@ -119,4 +1043,85 @@ extern "C" int afx_scalar_gradient(const afx_mesh* mh, const afx_geometry* g, in
 #undef AFX_SG
   afx_set_error("afx_scalar_gradient: unsupported dimension");
   return 2;
+}
+
+extern "C" int afx_scalar_workspace_bytes(const afx_mesh* m, int mode, int64_t* bytes) {
+  AFX_REQUIRE(m && bytes && mode >= 0 && mode <= 2, "afx_scalar_workspace_bytes: bad argument");
+  *bytes = sc_work_doubles(m->dim, m->C, mode) * 8 + 256;
+  return 0;
+}
+
+extern "C" int afx_scalar_residual(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cfg,
+                                   const afx_scalar_inputs* in, double* residual, void* work, int64_t work_size,
+                                   hipStream_t s) {
+  int rc = sc_check(m, g, cfg, in, work, work_size, 0);
+  if (rc) return rc;
+  AFX_REQUIRE(residual, "afx_scalar_residual: null residual");
+  ScWork w = sc_carve(work, m->dim, m->C, 0);
+  ScArgs<double> a = sc_args<double>(cfg, in, nullptr);
+  return sc_dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
+    return sc_forward<decltype(Dc)::value, double, decltype(Ac)::value, decltype(Gc)::value>(
+        m, g, a, Out<double>{residual, nullptr}, w, nullptr, nullptr, nullptr, s);
+  });
+}
+
+extern "C" int afx_scalar_jvp(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cfg,
+                              const afx_scalar_inputs* in, const afx_scalar_inputs* tangent, double* residual,
+                              double* residual_tangent, void* work, int64_t work_size, hipStream_t s) {
+  int rc = sc_check(m, g, cfg, in, work, work_size, 1);
+  if (rc) return rc;
+  AFX_REQUIRE(tangent && residual && residual_tangent, "afx_scalar_jvp: null argument");
+  ScWork w = sc_carve(work, m->dim, m->C, 1);
+  ScArgs<dual> a = sc_args<dual>(cfg, in, tangent);
+  return sc_dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
+    return sc_forward<decltype(Dc)::value, dual, decltype(Ac)::value, decltype(Gc)::value>(
+        m, g, a, Out<dual>{residual, residual_tangent}, w, w.Gt, w.psit, nullptr, s);
+  });
+}
+
+extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cfg,
+                              const afx_scalar_inputs* in, const double* cotangent, afx_scalar_cotangents* out,
+                              void* work, int64_t work_size, hipStream_t s) {
+  int rc = sc_check(m, g, cfg, in, work, work_size, 2);
+  if (rc) return rc;
+  AFX_REQUIRE(cotangent && out && out->phi && out->gamma && out->flux && out->value && out->t_inf,
+              "afx_scalar_vjp: null cotangent argument");
+  const afx_geometry_cotangents& gb = out->geometry;
+  const bool geo = gb.area || gb.normal || gb.face_centroid || gb.volume || gb.cell_centroid;
+  AFX_REQUIRE(!geo || (gb.area && gb.normal && gb.face_centroid && gb.volume && gb.cell_centroid),
+              "afx_scalar_vjp: geometry cotangents need all five arrays");
+  AFX_REQUIRE(!geo || cfg->gradient != 1,
+              "afx_scalar_vjp: geometry cotangents are available for Green-Gauss or no gradient reconstruction");
+  ScWork w = sc_carve(work, m->dim, m->C, 2);
+  ScArgs<double> a = sc_args<double>(cfg, in, nullptr);
+  ScBars bar{out->phi, out->gamma, out->flux, out->value, out->t_inf, w.Gb, w.psib, w.extb, w.limx, w.limf,
+             gb.area, gb.normal, gb.face_centroid, gb.volume, gb.cell_centroid};
+  return sc_dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
+    constexpr int D = decltype(Dc)::value, ADV = decltype(Ac)::value, GR = decltype(Gc)::value;
+    constexpr bool LIM = ADV == kAdvLimited;
+    int r2 = sc_forward<D, double, ADV, GR>(m, g, a, Out<double>{nullptr, nullptr}, w, nullptr, nullptr, w.ext, s);
+    if (r2) return r2;
+    MeshView v = afx_view(m, g);
+    ScArgs<double> b = a;
+    b.G.v = w.G;
+    b.psi.v = w.psi;
+    auto run = [&](auto Geo) {
+      constexpr bool GEO = decltype(Geo)::value;
+      AFX_LAUNCH(kKScalarResAdj, (k_sc_res_adj<D, ADV, GEO>), grid_for(m->C), dim3(kBlock), 0, s, v, b, cotangent,
+                 bar);
+      AFX_CHECK_LAUNCH();
+      AFX_LAUNCH(kKScalarLimAdj, (k_sc_lim_adj<D, GR, LIM, GEO>), grid_for(m->C), dim3(kBlock), 0, s, v, b, w.ext,
+                 bar);
+      AFX_CHECK_LAUNCH();
+      AFX_LAUNCH(kKScalarGradAdj, (k_sc_grad_adj<D, GR, LIM, GEO>), grid_for(m->C), dim3(kBlock), 0, s, v, b,
+                 w.ext, bar);
+      AFX_CHECK_LAUNCH();
+      return 0;
+    };
+    if constexpr (GR == 1) {
+      return run(std::false_type{});
+    } else {
+      return geo ? run(std::true_type{}) : run(std::false_type{});
+    }
+  });
 }

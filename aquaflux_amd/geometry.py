@@ -58,6 +58,7 @@ class MeshGeometry:
 
 
 def _nodes_soa(mesh, node_coords, device):
+    """Node coordinates as a (dim, n_nodes) float64 device tensor; a tensor input keeps its graph."""
     if node_coords is None:
         x = torch.as_tensor(mesh.node_coords, dtype=torch.float64)
         return x.T.contiguous().to(device)
@@ -65,28 +66,58 @@ def _nodes_soa(mesh, node_coords, device):
         node_coords = torch.as_tensor(node_coords, dtype=torch.float64)
     if node_coords.shape != (mesh.n_nodes, mesh.dim):
         raise ValueError(f"node_coords must have shape {(mesh.n_nodes, mesh.dim)}, got {tuple(node_coords.shape)}")
-    return node_coords.detach().to(device=device, dtype=torch.float64).T.contiguous()
+    return node_coords.to(device=device, dtype=torch.float64).T.contiguous()
+
+
+def _empty_geometry(d, F, C, dev):
+    e = lambda *shape: torch.empty(shape, dtype=torch.float64, device=dev)  # noqa: E731
+    return e(F), e(d, F), e(d, F), e(C), e(d, C)
+
+
+class _GeometryFn(torch.autograd.Function):
+    """Node coordinates (dim, N) -> (area, normal, face centroid, volume, cell centroid), with the
+    hand-written HIP adjoint (afx_geometry_vjp) as the backward."""
+
+    @staticmethod
+    def forward(ctx, dm, X):
+        d, F, C = dm.dim, dm.n_faces, dm.n_cells
+        area, nrm, fc, vol, cc = _empty_geometry(d, F, C, X.device)
+        gs = _lib.AfxGeometry(area.data_ptr(), nrm.data_ptr(), fc.data_ptr(), vol.data_ptr(), cc.data_ptr())
+        _lib.check(
+            _lib.lib().afx_geometry_compute(dm.handle, X.data_ptr(), ctypes.byref(gs), _lib.stream_of(X.device)),
+            "afx_geometry_compute",
+        )
+        ctx.dm = dm
+        ctx.save_for_backward(X, area, nrm, fc, vol, cc)
+        return area, nrm, fc, vol, cc
+
+    @staticmethod
+    def backward(ctx, gA, gN, gF, gV, gC):
+        X, area, nrm, fc, vol, cc = ctx.saved_tensors
+        dm = ctx.dm
+        bars = [None if g is None else g.contiguous() for g in (gA, gN, gF, gV, gC)]
+        gs = _lib.AfxGeometry(area.data_ptr(), nrm.data_ptr(), fc.data_ptr(), vol.data_ptr(), cc.data_ptr())
+        gb = _lib.AfxGeometryCotangents(*[_lib.ptr(b) for b in bars])
+        nbytes = ctypes.c_int64()
+        L = _lib.lib()
+        _lib.check(L.afx_geometry_vjp_workspace_bytes(dm.handle, ctypes.byref(nbytes)), "afx_geometry_vjp_workspace_bytes")
+        work = dm.workspace("geom_vjp", int(nbytes.value))
+        out = torch.empty_like(X)
+        _lib.check(
+            L.afx_geometry_vjp(dm.handle, X.data_ptr(), ctypes.byref(gs), ctypes.byref(gb), out.data_ptr(),
+                               work.data_ptr(), work.numel(), _lib.stream_of(X.device)),
+            "afx_geometry_vjp",
+        )
+        return None, out
 
 
 def compute_geometry(mesh, node_coords=None, device=None) -> MeshGeometry:
-    """Derive face/cell geometry from node coordinates with the HIP kernels."""
+    """Derive face/cell geometry from node coordinates with the HIP kernels.  With ``node_coords``
+    a tensor that requires grad, the geometry tensors carry the hand-written adjoint."""
     dm = mesh.device_mesh(device)
     dev = dm.device
     X = _nodes_soa(mesh, node_coords, dev)
-    d, F, C = mesh.dim, mesh.n_faces, mesh.n_cells
-    face = FaceGeometry(
-        area=torch.empty(F, dtype=torch.float64, device=dev),
-        centroid_soa=torch.empty((d, F), dtype=torch.float64, device=dev),
-        normal_soa=torch.empty((d, F), dtype=torch.float64, device=dev),
-    )
-    cell = CellGeometry(
-        volume=torch.empty(C, dtype=torch.float64, device=dev),
-        centroid_soa=torch.empty((d, C), dtype=torch.float64, device=dev),
-    )
-    geom = MeshGeometry(face=face, cell=cell, node_coords_soa=X)
-    gs = geom.c_struct()
-    _lib.check(
-        _lib.lib().afx_geometry_compute(dm.handle, X.data_ptr(), ctypes.byref(gs), _lib.stream_of(dev)),
-        "afx_geometry_compute",
-    )
-    return geom
+    area, nrm, fc, vol, cc = _GeometryFn.apply(dm, X)
+    face = FaceGeometry(area=area, centroid_soa=fc, normal_soa=nrm)
+    cell = CellGeometry(volume=vol, centroid_soa=cc)
+    return MeshGeometry(face=face, cell=cell, node_coords_soa=X)

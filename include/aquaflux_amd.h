@@ -74,6 +74,27 @@ typedef struct {
  * aquaflux/mesh/face.py:196-349, aquaflux/mesh/cell.py:84-170). */
 int afx_geometry_compute(const afx_mesh* mesh, const double* node_coords, afx_geometry* out, hipStream_t stream);
 
+/* Cotangents of the five geometry arrays (written, not accumulated; NULL = not requested). */
+typedef struct {
+  double* area;          /* [F]    */
+  double* normal;        /* [d][F] */
+  double* face_centroid; /* [d][F] */
+  double* volume;        /* [C]    */
+  double* cell_centroid; /* [d][C] */
+} afx_geometry_cotangents;
+
+/* Device workspace of afx_geometry_vjp. */
+int afx_geometry_vjp_workspace_bytes(const afx_mesh* mesh, int64_t* bytes);
+
+/* node_b[d][N] = (d geometry / d node_coords)^T . geometry cotangents, recomputing the face
+ * construction (centre-fan triangles / edges), the approximate centroids and the divergence-theorem
+ * sums from the nodes and the forward geometry `geom`.  Replaces jax.vjp through Mesh.geometry
+ * (reference: aquaflux/mesh/mesh.py:341-387, aquaflux/mesh/face.py:196-349,
+ * aquaflux/mesh/cell.py:84-170). */
+int afx_geometry_vjp(const afx_mesh* mesh, const double* node_coords, const afx_geometry* geom,
+                     const afx_geometry_cotangents* geom_cotangents, double* node_b, void* work,
+                     int64_t work_bytes, hipStream_t stream);
+
 /* ------------------------------------------------------------------------------------------- */
 /* coupled (u, v[, w], p) flow residual                                                          */
 
@@ -146,6 +167,53 @@ int afx_flow_fields(const afx_mesh* mesh, const afx_geometry* geom, const afx_fl
  * (aquaflux/turbulence/transport.py:86-110). */
 int afx_scalar_gradient(const afx_mesh* mesh, const afx_geometry* geom, int gradient, const int8_t* kind,
                         const double* value, const double* phi, double* grad, hipStream_t stream);
+
+enum { AFX_GRAD_NONE = 2 };
+
+typedef struct {
+  int32_t advection;     /* AFX_ADV_* */
+  int32_t gradient;      /* AFX_GRAD_GREEN_GAUSS / AFX_GRAD_LEAST_SQUARES / AFX_GRAD_NONE (orthogonal grids) */
+  double limiter_k;      /* Venkatakrishnan K */
+  const int8_t* bc_kind; /* [F] AFX_BC_* on boundary faces */
+} afx_scalar_config;
+
+typedef struct {
+  const double* phi;   /* [C] transported scalar */
+  const double* gamma; /* [C] effective diffusivity */
+  const double* flux;  /* [F] owner-outward volumetric face flux Q_f */
+  const double* value; /* [F] Dirichlet value / Neumann flux q / convective coefficient h */
+  const double* t_inf; /* [F] convective ambient value */
+} afx_scalar_inputs;
+
+typedef struct {
+  double* phi;   /* [C] */
+  double* gamma; /* [C] */
+  double* flux;  /* [F] */
+  double* value; /* [F] */
+  double* t_inf; /* [F] */
+  afx_geometry_cotangents geometry; /* all five set, or all NULL (least squares: NULL only) */
+} afx_scalar_cotangents;
+
+/* Workspace (device bytes) for mode 0 = residual, 1 = JVP, 2 = VJP. */
+int afx_scalar_workspace_bytes(const afx_mesh* mesh, int mode, int64_t* bytes);
+
+/* R(phi) = sum of (Q phi_f + diffusion) face fluxes.  Replaces ScalarTransport.residual(flux)(phi)
+ * (reference: aquaflux/transport/scalar.py:185-223, ResidualAssembler.residual in
+ * aquaflux/discretization/residual.py:252-412). */
+int afx_scalar_residual(const afx_mesh* mesh, const afx_geometry* geom, const afx_scalar_config* cfg,
+                        const afx_scalar_inputs* in, double* residual, void* work, int64_t work_bytes,
+                        hipStream_t stream);
+
+/* R and J . tangent (tangent pointers may be NULL = zero).  Replaces jax.jvp of the scalar residual. */
+int afx_scalar_jvp(const afx_mesh* mesh, const afx_geometry* geom, const afx_scalar_config* cfg,
+                   const afx_scalar_inputs* in, const afx_scalar_inputs* tangent, double* residual,
+                   double* residual_tangent, void* work, int64_t work_bytes, hipStream_t stream);
+
+/* Cotangents of phi, gamma, the face flux, the boundary values and (optionally) the geometry.
+ * Replaces jax.vjp of the scalar residual, including through MeshGeometry. */
+int afx_scalar_vjp(const afx_mesh* mesh, const afx_geometry* geom, const afx_scalar_config* cfg,
+                   const afx_scalar_inputs* in, const double* cotangent, afx_scalar_cotangents* out, void* work,
+                   int64_t work_bytes, hipStream_t stream);
 
 /* ------------------------------------------------------------------------------------------- */
 /* coupled RANS (u, v[, w], p, k, omega) residual with the k-omega SST closure                   */

@@ -1,0 +1,197 @@
+"""GPU parity of the scalar-transport residual, tangent and adjoint (including the adjoint with
+respect to mesh node coordinates through the geometry) against the CPU oracle.
+
+Tolerances: residual 1e-12 and derivatives 1e-11 relative to the largest entry of each compared
+array (register sums on the GPU and index_add scatters in the oracle associate differently).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fv
+from tests import scalar_cases as SC
+
+pytestmark = pytest.mark.gpu
+F64 = torch.float64
+TOL_R = 1e-12
+TOL_D = 1e-11
+
+CASES = [
+    ("tet3", "limited", "gg"),
+    ("tet3", "upwind", "gg"),
+    ("tet3", "limited", "lsq"),
+    ("tet3", "upwind", "none"),
+    ("quad_mixed", "limited", "gg"),
+    ("quad_mixed", "upwind", "lsq"),
+    ("quad_mixed", "none", "gg"),
+    ("quad_mixed", "limited", "none"),
+]
+GEO_CASES = [c for c in CASES if c[2] != "lsq"]
+
+
+def _ids(c):
+    return "-".join(c)
+
+
+def rel(a, b):
+    a = a.detach().double().cpu()
+    b = b.detach().double().cpu()
+    return float((a - b).abs().max() / max(float(b.abs().max()), 1e-300))
+
+
+def _setup(case, cuda, seed=0):
+    name, adv, grad = case
+    print(f"case {_ids(case)} seed {seed}", flush=True)
+    mesh, u, bcs = SC.make(name)
+    phi, gamma = SC.seeded(mesh, seed)
+    m = fv.OMesh.from_mesh(mesh)
+    g = fv.geometry(m)
+    s = SC.oracle_setup(mesh, bcs, adv, grad)
+    geom, st = SC.build_gpu(mesh, bcs, adv, grad, gamma, cuda)
+    return mesh, u, bcs, phi, gamma, (m, g, s), geom, st
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_scalar_residual_matches_oracle(case, cuda):
+    mesh, u, bcs, phi, gamma, (m, g, s), geom, st = _setup(case, cuda)
+    ref = fv.scalar_residual(m, g, s, torch.tensor(phi), SC.oracle_flux(g, u), torch.tensor(gamma))
+    got = st.residual(SC.gpu_flux(geom, u))(torch.tensor(phi, device=cuda))
+    err = rel(got, ref)
+    print("residual err", err)
+    assert err < TOL_R
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_scalar_jvp_matches_oracle(case, cuda):
+    mesh, u, bcs, phi, gamma, (m, g, s), geom, st = _setup(case, cuda, seed=1)
+    rng = np.random.default_rng(2)
+    C, F = mesh.n_cells, mesh.n_faces
+    t_phi = torch.tensor(rng.normal(size=C))
+    t_gam = torch.tensor(rng.normal(size=C)) * 0.001
+    t_flux = torch.tensor(rng.normal(size=F)) * 0.01
+    t_val = torch.tensor(rng.normal(size=F))
+    t_tinf = torch.tensor(rng.normal(size=F))
+    flux_o = SC.oracle_flux(g, u)
+
+    def f(ph, ga, fl, va, ti):
+        s2 = fv.ScalarSetup(kind=s.kind, value=va, t_inf=ti, advection=s.advection, gradient=s.gradient,
+                            limiter=s.limiter, limiter_k=s.limiter_k)
+        return fv.scalar_residual(m, g, s2, ph, fl, ga)
+
+    _, ref = torch.func.jvp(f, (torch.tensor(phi), torch.tensor(gamma), flux_o, s.value, s.t_inf),
+                            (t_phi, t_gam, t_flux, t_val, t_tinf))
+    prim = (torch.tensor(phi, device=cuda), torch.tensor(gamma, device=cuda),
+            SC.gpu_flux(geom, u).detach().contiguous(), st._val, st._tinf)
+    geo = tuple(t.detach() for t in st._geometry())
+    _, dR = st._raw.jvp(prim, tuple(t.to(cuda) for t in (t_phi, t_gam, t_flux, t_val, t_tinf)), geo)
+    err = rel(dR, ref)
+    print("jvp err", err)
+    assert err < TOL_D
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_scalar_vjp_matches_oracle(case, cuda):
+    mesh, u, bcs, phi, gamma, (m, g, s), geom, st = _setup(case, cuda, seed=2)
+    rng = np.random.default_rng(4)
+    lam = torch.tensor(rng.normal(size=mesh.n_cells))
+    ph = torch.tensor(phi, requires_grad=True)
+    ga = torch.tensor(gamma, requires_grad=True)
+    fl = SC.oracle_flux(g, u).detach().clone().requires_grad_(True)
+    va = s.value.clone().requires_grad_(True)
+    ti = s.t_inf.clone().requires_grad_(True)
+    s2 = fv.ScalarSetup(kind=s.kind, value=va, t_inf=ti, advection=s.advection, gradient=s.gradient,
+                        limiter=s.limiter, limiter_k=s.limiter_k)
+    R = fv.scalar_residual(m, g, s2, ph, fl, ga)
+    refs = torch.autograd.grad(R, (ph, ga, fl, va, ti), lam, allow_unused=True)
+    bars, _ = st.residual_vjp(SC.gpu_flux(geom, u), torch.tensor(phi, device=cuda), lam.to(cuda))
+    bnd = torch.from_numpy(mesh.neighbour < 0)
+    for name, got, ref in zip(("phi", "gamma", "flux", "value", "t_inf"), bars, refs):
+        ref = torch.zeros_like(got.cpu()) if ref is None else ref
+        if name in ("value", "t_inf"):
+            got, ref = got.cpu()[bnd], ref[bnd]
+        scale = max(float(ref.abs().max()), 1e-300)
+        err = float((got.cpu() - ref).abs().max()) / scale
+        print(name, err, scale)
+        assert err < TOL_D or float(ref.abs().max()) == 0.0 and float(got.abs().max()) < 1e-14, name
+
+
+@pytest.mark.parametrize("case", GEO_CASES, ids=_ids)
+def test_scalar_node_coordinate_adjoint_matches_oracle(case, cuda):
+    """d <lambda, R(phi; geometry(X), flux(geometry(X)))> / dX through the hand-written residual and
+    geometry adjoints, against torch autograd of the oracle."""
+    name, adv, grad = case
+    mesh, u, bcs = SC.make(name)
+    phi, gamma = SC.seeded(mesh, 3)
+    rng = np.random.default_rng(5)
+    lam = torch.tensor(rng.normal(size=mesh.n_cells))
+    # oracle
+    Xo = torch.tensor(mesh.node_coords, dtype=F64, requires_grad=True)
+    m = fv.OMesh.from_mesh(mesh, node_coords=Xo)
+    g = fv.geometry(m)
+    s = SC.oracle_setup(mesh, bcs, adv, grad)
+    R = fv.scalar_residual(m, g, s, torch.tensor(phi), SC.oracle_flux(g, u), torch.tensor(gamma))
+    (ref,) = torch.autograd.grad((R * lam).sum(), Xo)
+    # HIP path
+    X = torch.tensor(mesh.node_coords, dtype=F64, device=cuda, requires_grad=True)
+    geom, st = SC.build_gpu(mesh, bcs, adv, grad, gamma, cuda, node_coords=X)
+    Rg = st.residual(SC.gpu_flux(geom, u))(torch.tensor(phi, device=cuda))
+    (got,) = torch.autograd.grad((Rg * lam.to(cuda)).sum(), X)
+    err = rel(got, ref)
+    print("node adjoint err", err, float(ref.abs().max()))
+    assert err < TOL_D
+
+
+def test_geometry_adjoint_matches_oracle(cuda):
+    """Cotangents of all five geometry arrays back to the nodes (tet and quad meshes)."""
+    for name in ("tet3", "quad_mixed"):
+        mesh, _, _ = SC.make(name)
+        rng = np.random.default_rng(9)
+        Xo = torch.tensor(mesh.node_coords, dtype=F64, requires_grad=True)
+        g = fv.geometry(fv.OMesh.from_mesh(mesh, node_coords=Xo))
+        w = [torch.tensor(rng.normal(size=t.shape)) for t in (g.area, g.normal, g.face_centroid, g.volume,
+                                                              g.cell_centroid)]
+        L = sum((a * b).sum() for a, b in zip((g.area, g.normal, g.face_centroid, g.volume, g.cell_centroid), w))
+        (ref,) = torch.autograd.grad(L, Xo)
+        X = torch.tensor(mesh.node_coords, dtype=F64, device=cuda, requires_grad=True)
+        geom = mesh.geometry(node_coords=X, device=cuda)
+        arrs = (geom.face.area, geom.face.normal, geom.face.centroid, geom.cell.volume, geom.cell.centroid)
+        Lg = sum((a * b.to(cuda)).sum() for a, b in zip(arrs, w))
+        (got,) = torch.autograd.grad(Lg, X)
+        err = rel(got, ref)
+        print(name, "geometry adjoint err", err)
+        assert err < TOL_D
+
+
+def test_scalar_node_adjoint_finite_difference(cuda):
+    """The node-coordinate adjoint against central differences of the GPU residual itself.  Only
+    interior nodes move: the wall faces then stay parallel to the flow (Q = 0 exactly), where the
+    upwind switch makes the residual non-differentiable in the node coordinates."""
+    mesh, u, bcs = SC.make("tet3")
+    phi, gamma = SC.seeded(mesh, 6)
+    rng = np.random.default_rng(7)
+    lam = torch.tensor(rng.normal(size=mesh.n_cells), device=cuda)
+    X0 = torch.tensor(mesh.node_coords, dtype=F64, device=cuda)
+    on_boundary = np.zeros(mesh.n_nodes, dtype=bool)
+    off = mesh.face_node_offsets
+    for f in np.nonzero(mesh.neighbour < 0)[0]:
+        on_boundary[mesh.face_node_indices[off[f]:off[f + 1]]] = True
+    dX = torch.tensor(rng.normal(size=X0.shape), device=cuda) * 1e-2 / 3.0
+    dX[torch.from_numpy(on_boundary).to(cuda)] = 0.0
+
+    def L(X):
+        geom, st = SC.build_gpu(mesh, bcs, "limited", "gg", gamma, cuda, node_coords=X)
+        R = st.residual(SC.gpu_flux(geom, u))(torch.tensor(phi, device=cuda))
+        return (R * lam).sum()
+
+    X = X0.clone().requires_grad_(True)
+    (gx,) = torch.autograd.grad(L(X), X)
+    an = float((gx * dX).sum())
+
+    def cd(e):
+        with torch.no_grad():
+            return float((L(X0 + e * dX) - L(X0 - e * dX)) / (2 * e))
+
+    fd = (4.0 * cd(5e-5) - cd(1e-4)) / 3.0
+    print("fd", fd, "adjoint", an)
+    assert abs(fd - an) <= 1e-7 * max(abs(an), 1e-12)

@@ -308,3 +308,89 @@ def test_rhie_chow_cancels_on_linear_pressure():
     f = fv.flow_fields(m, g, s, state, torch.full((C,), 0.01, dtype=F64), torch.ones(C, dtype=F64))
     ref = (u[m.owner] * g.normal).sum(1) * g.area
     assert torch.allclose(f["mdot"][m.interior], ref[m.interior], atol=1e-12)
+
+
+def test_geometry_gradient_pins():
+    """tests/unit/test_mesh.py:730-748 (d total volume / d node vs central FD), plus invariants the
+    node-coordinate adjoint must honour: the total volume does not change when an interior node
+    moves, and a random functional of all five geometry arrays matches FD on every coordinate of a
+    small jittered tet mesh."""
+    from aquaflux_amd.mesh import structured_grid_3d, tet_grid_3d
+
+    mesh = structured_grid_3d(2, 3, 4, lx=2.0, ly=3.0, lz=4.0)
+    X = torch.tensor(mesh.node_coords, dtype=F64, requires_grad=True)
+    total = fv.geometry(fv.OMesh.from_mesh(mesh, node_coords=X)).volume.sum()
+    (gx,) = torch.autograd.grad(total, X)
+
+    def vol(Xn):
+        return float(fv.geometry(fv.OMesh.from_mesh(mesh, node_coords=Xn)).volume.sum())
+
+    h = 1e-6
+    Xp, Xm = X.detach().clone(), X.detach().clone()
+    Xp[6, 0] += h
+    Xm[6, 0] -= h
+    assert abs(float(gx[6, 0]) - (vol(Xp) - vol(Xm)) / (2 * h)) < 1e-5
+    tm = tet_grid_3d(2, perturb=0.15, seed=3)
+    Xt = torch.tensor(tm.node_coords, dtype=F64, requires_grad=True)
+    g = fv.geometry(fv.OMesh.from_mesh(tm, node_coords=Xt))
+    (gt,) = torch.autograd.grad(g.volume.sum(), Xt)
+    on_b = np.zeros(tm.n_nodes, dtype=bool)
+    for f in np.nonzero(tm.neighbour < 0)[0]:
+        on_b[tm.face_node_indices[tm.face_node_offsets[f]:tm.face_node_offsets[f + 1]]] = True
+    assert float(gt[torch.from_numpy(~on_b)].abs().max()) < 1e-13
+    rng = np.random.default_rng(0)
+    arrays = (g.area, g.normal, g.face_centroid, g.volume, g.cell_centroid)
+    w = [torch.tensor(rng.normal(size=a.shape)) for a in arrays]
+
+    def functional(Xn):
+        gg = fv.geometry(fv.OMesh.from_mesh(tm, node_coords=Xn))
+        return sum((a * b).sum() for a, b in zip((gg.area, gg.normal, gg.face_centroid, gg.volume,
+                                                  gg.cell_centroid), w))
+
+    (ga,) = torch.autograd.grad(functional(Xt), Xt)
+    Xd = Xt.detach()
+    for node in range(0, tm.n_nodes, 5):
+        for ax in range(3):
+            e = torch.zeros_like(Xd)
+            e[node, ax] = 1.0
+            fd = (4 * (functional(Xd + 5e-6 * e) - functional(Xd - 5e-6 * e)) / 1e-5
+                  - (functional(Xd + 1e-5 * e) - functional(Xd - 1e-5 * e)) / 2e-5) / 3
+            assert abs(float(fd) - float(ga[node, ax])) < 1e-7 * max(1.0, abs(float(ga[node, ax])))
+
+
+def test_scalar_residual_jvp_matches_richardson_fd():
+    """The oracle's scalar-transport residual tangent (phi, diffusivity, flux) against Richardson-
+    extrapolated central differences on a jittered tet mesh with limited advection."""
+    from aquaflux_amd.mesh import tet_grid_3d
+
+    mesh = tet_grid_3d(3, perturb=0.15, seed=3)
+    m = fv.OMesh.from_mesh(mesh)
+    g = fv.geometry(m)
+    F, C = mesh.n_faces, mesh.n_cells
+    kind = torch.where(m.interior, 0, fv.BC_ZERO_GRADIENT)
+    left = torch.tensor(mesh.face_patches.indices("left"), dtype=torch.long)
+    kind[left] = fv.BC_DIRICHLET
+    value = torch.zeros(F, dtype=F64)
+    value[left] = 1.0
+    s = fv.ScalarSetup(kind=kind, value=value, t_inf=torch.zeros(F, dtype=F64), advection="limited",
+                       gradient="green_gauss")
+    rng = np.random.default_rng(1)
+    phi = torch.tensor(rng.uniform(size=C))
+    gam = torch.full((C,), 0.01, dtype=F64)
+    flux = g.normal[:, 0] * g.area
+    # no flux tangent where the flux is exactly zero (wall faces parallel to the flow): the upwind
+    # switch is not differentiable there
+    tan = (torch.tensor(rng.normal(size=C)), torch.tensor(rng.normal(size=C)) * 1e-3,
+           torch.tensor(rng.normal(size=F)) * 1e-2 * (flux.abs() > 1e-12))
+
+    def f(p, ga, fl):
+        return fv.scalar_residual(m, g, s, p, fl, ga)
+
+    _, jv = torch.func.jvp(f, (phi, gam, flux), tan)
+
+    def cd(e):
+        return (f(phi + e * tan[0], gam + e * tan[1], flux + e * tan[2])
+                - f(phi - e * tan[0], gam - e * tan[1], flux - e * tan[2])) / (2 * e)
+
+    rich = (4 * cd(5e-6) - cd(1e-5)) / 3
+    assert float((jv - rich).abs().max() / jv.abs().max()) < 1e-8
