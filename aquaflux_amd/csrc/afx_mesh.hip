@@ -605,27 +605,27 @@ extern "C" int afx_geometry_vjp(const afx_mesh* m, const double* node_coords, co
   double* inc_b = cellbar + (int64_t)(2 * D + 1) * m->C;
   afx::MeshView v = afx_view(m, g);
   if (D == 2) {
-    AFX_LAUNCH(kKGeomAdj, k_approx_centroid<2>, grid_for(m->C), dim3(kBlock), 0, s, v, approx);
+    AFX_LAUNCH(kKGeomApprox, k_approx_centroid<2>, grid_for(m->C), dim3(kBlock), 0, s, v, approx);
     AFX_CHECK_LAUNCH();
     AFX_LAUNCH(kKGeomAdj, k_geom_cell_adj<2>, grid_for(m->C), dim3(kBlock), 0, s, v, approx, gb->volume,
                gb->cell_centroid, cellbar);
     AFX_CHECK_LAUNCH();
-    AFX_LAUNCH(kKGeomAdj, k_geom_face_adj<2>, grid_for(m->F), dim3(kBlock), 0, s, v, m->N, m->foff, m->fidx,
+    AFX_LAUNCH(kKGeomAdjFace, k_geom_face_adj<2>, grid_for(m->F), dim3(kBlock), 0, s, v, m->N, m->foff, m->fidx,
                node_coords, approx, cellbar, gb->area, gb->normal, gb->face_centroid, inc_b, m->n_inc);
     AFX_CHECK_LAUNCH();
-    AFX_LAUNCH(kKGeomAdj, k_geom_node_adj<2>, grid_for(m->N), dim3(kBlock), 0, s, m->N, m->nptr, m->ninc, inc_b,
+    AFX_LAUNCH(kKGeomAdjNode, k_geom_node_adj<2>, grid_for(m->N), dim3(kBlock), 0, s, m->N, m->nptr, m->ninc, inc_b,
                m->n_inc, node_b);
     AFX_CHECK_LAUNCH();
   } else {
-    AFX_LAUNCH(kKGeomAdj, k_approx_centroid<3>, grid_for(m->C), dim3(kBlock), 0, s, v, approx);
+    AFX_LAUNCH(kKGeomApprox, k_approx_centroid<3>, grid_for(m->C), dim3(kBlock), 0, s, v, approx);
     AFX_CHECK_LAUNCH();
     AFX_LAUNCH(kKGeomAdj, k_geom_cell_adj<3>, grid_for(m->C), dim3(kBlock), 0, s, v, approx, gb->volume,
                gb->cell_centroid, cellbar);
     AFX_CHECK_LAUNCH();
-    AFX_LAUNCH(kKGeomAdj, k_geom_face_adj<3>, grid_for(m->F), dim3(kBlock), 0, s, v, m->N, m->foff, m->fidx,
+    AFX_LAUNCH(kKGeomAdjFace, k_geom_face_adj<3>, grid_for(m->F), dim3(kBlock), 0, s, v, m->N, m->foff, m->fidx,
                node_coords, approx, cellbar, gb->area, gb->normal, gb->face_centroid, inc_b, m->n_inc);
     AFX_CHECK_LAUNCH();
-    AFX_LAUNCH(kKGeomAdj, k_geom_node_adj<3>, grid_for(m->N), dim3(kBlock), 0, s, m->N, m->nptr, m->ninc, inc_b,
+    AFX_LAUNCH(kKGeomAdjNode, k_geom_node_adj<3>, grid_for(m->N), dim3(kBlock), 0, s, m->N, m->nptr, m->ninc, inc_b,
                m->n_inc, node_b);
     AFX_CHECK_LAUNCH();
   }

@@ -37,6 +37,8 @@ enum KernelId : int {
   kKRansApAdj,
   kKRansGradAdj,
   kKScalarLimAdj,
+  kKGeomAdjFace,
+  kKGeomAdjNode,
   kKNumKernels
 };
 

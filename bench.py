@@ -94,11 +94,42 @@ def _rans_kernel_bytes(D=3, limited=True):
     }
 
 
+def _scalar_kernel_bytes(D=3):
+    """Compulsory bytes of the scalar-transport and geometry kernels (same convention; ``n`` per
+    node, ``i`` per face-node incidence).  Limited advection, Green-Gauss gradients."""
+    fp = _face_pass(D)
+    cgeo = (1 + D) * S8
+    fgeo = (1 + 2 * D) * S8
+    bnd = 1 + 2 * S8
+    return {
+        # launched twice per step (residual, and the recompute inside the VJP that also writes the
+        # stencil-extremum owners)
+        "scalar_grad": dict(c=4 + S8 + cgeo + D * S8 + S8 + 4, f=fp, b=bnd),
+        "scalar_res": dict(c=4 + 3 * S8 + D * S8 + cgeo + S8, f=fp + S8, b=bnd),
+        "scalar_res_adj": dict(c=4 + 4 * S8 + D * S8 + cgeo + 3 * S8 + D * S8 + D * S8 + S8, f=fp + 2 * S8 + fgeo,
+                               b=bnd + 2 * S8),
+        "scalar_lim_adj": dict(c=4 + 2 * D * S8 + D * S8 + 4 * S8 + S8 + 8 + 2 * S8 + D * S8 + 4 + cgeo
+                               + 2 * S8 + 2 * D * S8, f=4 * 2 + D * S8),
+        "scalar_grad_adj": dict(c=4 + D * S8 + S8 + 2 * S8 + S8 + 2 * S8 + 8 + 2 * S8 + 4 + D * S8 + cgeo
+                                + 2 * D * S8, f=fp + 2 * fgeo, b=bnd + 4 * S8),
+        "geom_face": dict(f=4 + fgeo, i=4 + D * S8),
+        "geom_approx": dict(c=4 + D * S8, e=4 + D * S8),
+        "geom_orient": dict(f=4 + 2 * D * S8 + D * S8, c=D * S8),
+        "geom_volume": dict(c=4 + 2 * D * S8 + S8, f=fp),
+        "geom_adj_cell": dict(c=4 + cgeo + 2 * D * S8 + S8 + D * S8 + (2 * D + 1) * S8, f=fp),
+        "geom_adj_face": dict(f=4 + 4 + fgeo + fgeo, c=(2 * D + 1) * S8 + D * S8 + 4, i=4 + D * S8 + D * S8),
+        "geom_adj_node": dict(n=4 + D * S8, i=4 + D * S8),
+    }
+
+
 def algorithmic_bytes(name, counts, workload="sst"):
-    """Bytes per launch of kernel ``name`` for mesh counts {C, F, Fb, Cw, Fw}."""
-    model = (_rans_kernel_bytes() if workload == "sst" else _flow_kernel_bytes())[name]
+    """Bytes per launch of kernel ``name`` for mesh counts {C, F, Fb, Cw, Fw, N, I, E}."""
+    models = {"sst": _rans_kernel_bytes, "flow": _flow_kernel_bytes, "tet": _scalar_kernel_bytes}
+    model = models[workload]()[name]
     return (model.get("c", 0) * counts["C"] + model.get("f", 0) * counts["F"] + model.get("b", 0) * counts["Fb"]
-            + model.get("w", 0) * counts.get("Cw", 0) + model.get("wf", 0) * counts.get("Fw", 0))
+            + model.get("w", 0) * counts.get("Cw", 0) + model.get("wf", 0) * counts.get("Fw", 0)
+            + model.get("n", 0) * counts.get("N", 0) + model.get("i", 0) * counts.get("I", 0)
+            + model.get("e", 0) * counts.get("E", 0))
 
 
 CHANNEL_BCS = (("left", 2), ("right", 3), ("bottom", 1), ("top", 1), ("back", 1), ("front", 1))
@@ -228,6 +259,120 @@ def cpu_baseline(workload, nx, ny, nz, reps=2, nu=1e-5):
     }
 
 
+TET_BCS = ("left", "right", "bottom", "top", "back", "front")
+
+
+def tet_workload(device, n=96, steps=10, warmup=2):
+    """Scalar advection-diffusion on the jittered 6-tet split of an n^3 unit cube (Pe = 100, u =
+    (1, 0, 0) face flux (u . n) A, Venkatakrishnan-limited upwind, Green-Gauss gradients, Dirichlet 1
+    on the left, zero gradient elsewhere).  One step = geometry from the node coordinates, the face
+    flux, the residual, and the reverse pass to the node coordinates and phi through the hand-written
+    residual and geometry adjoints (torch autograd composes them)."""
+    import torch
+
+    from aquaflux_amd import _lib
+    from aquaflux_amd.boundary import BoundaryConditions, Dirichlet, ZeroGradient
+    from aquaflux_amd.discretization import LimitedUpwind
+    from aquaflux_amd.mesh import tet_grid_3d
+    from aquaflux_amd.properties import Constant
+    from aquaflux_amd.schemes import CompactGreenGauss, VenkatakrishnanLimiter
+    from aquaflux_amd.transport import ScalarTransport, face_mass_flux
+
+    t0 = time.perf_counter()
+    mesh = tet_grid_3d(n, perturb=0.15, seed=3)
+    X = torch.tensor(mesh.node_coords, dtype=torch.float64, device=device, requires_grad=True)
+    bcs = BoundaryConditions({p: (Dirichlet(1.0) if p == "left" else ZeroGradient()) for p in TET_BCS})
+    geom = mesh.geometry(node_coords=X, device=device)
+    st = ScalarTransport.build(mesh, geom, Constant(0.01), bcs, LimitedUpwind(VenkatakrishnanLimiter(5.0)),
+                               gradient_scheme=CompactGreenGauss())
+    rng = np.random.default_rng(3)
+    C, F = mesh.n_cells, mesh.n_faces
+    phi = torch.tensor(rng.uniform(0.0, 1.0, C), device=device, requires_grad=True)
+    lam = torch.tensor(rng.normal(size=C), device=device)
+    print(f"bench: tet workload {n}^3 x 6 built ({C} cells, {F} faces) in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    u = (1.0, 0.0, 0.0)
+
+    def step():
+        g = mesh.geometry(node_coords=X, device=device)
+        st.geometry = g
+        R = st.residual(face_mass_flux(g, u))(phi)
+        return torch.autograd.grad((R * lam).sum(), (X, phi))
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    timer = _lib.KernelTimer(max_launches=32 * (steps + 1))
+    with timer:
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t1
+        kern = timer.collect()
+    Fb = int(np.count_nonzero(mesh.neighbour < 0))
+    counts = {"C": C, "F": F, "Fb": Fb, "N": mesh.n_nodes, "I": int(mesh.face_node_indices.size),
+              "E": 2 * F - Fb}
+    dom = max(kern, key=lambda k: kern[k][0])
+    avg_ms = kern[dom][0] / kern[dom][1]
+    nbytes = algorithmic_bytes(dom, counts, "tet")
+    achieved = nbytes / (avg_ms * 1e-3) / 1e9
+    return mesh, {
+        "metric": "Gfaces/s for residual+VJP",
+        "value": round(F * steps / elapsed / 1e9, 4),
+        "unit": "Gfaces/s",
+        "ms_per_step": round(elapsed / steps * 1e3, 4),
+        "steps": steps,
+        "dtype": "f64",
+        "config": {"workload": f"3D jittered tetrahedra, 6 x {n}^3 ({C:,} cells, {F:,} faces), scalar "
+                   "advection-diffusion Pe=100, Venkatakrishnan-limited upwind, Green-Gauss; step = geometry + "
+                   "flux + residual + adjoint to node coordinates and phi", "cells": C, "faces": F},
+        "kernel_avg_ms": {k: round(v[0] / v[1], 4) for k, v in sorted(kern.items())},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_ms": round(avg_ms, 4), "algorithmic_bytes": int(nbytes)},
+    }
+
+
+def tet_cpu_baseline(n=32, reps=5):
+    """The CPU oracle on a bounded sample of the tet workload: geometry, flux, residual and torch
+    autograd to node coordinates and phi on the 6 x n^3 mesh."""
+    import torch
+
+    from aquaflux_amd.mesh import tet_grid_3d
+    from oracle import fv
+
+    threads = min(os.cpu_count() or 1, 16)
+    torch.set_num_threads(threads)
+    mesh = tet_grid_3d(n, perturb=0.15, seed=3)
+    F, C = mesh.n_faces, mesh.n_cells
+    rng = np.random.default_rng(3)
+    phi = torch.tensor(rng.uniform(0.0, 1.0, C), requires_grad=True)
+    lam = torch.tensor(rng.normal(size=C))
+    kind = torch.where(torch.from_numpy(mesh.neighbour >= 0), 0, fv.BC_ZERO_GRADIENT)
+    left = torch.from_numpy(np.asarray(mesh.face_patches.indices("left"), dtype=np.int64))
+    kind[left] = fv.BC_DIRICHLET
+    value = torch.zeros(F, dtype=torch.float64)
+    value[left] = 1.0
+    s = fv.ScalarSetup(kind=kind, value=value, t_inf=torch.zeros(F, dtype=torch.float64), advection="limited",
+                       gradient="green_gauss")
+    gam = torch.full((C,), 0.01, dtype=torch.float64)
+    times = []
+    for _ in range(reps):
+        X = torch.tensor(mesh.node_coords, dtype=torch.float64, requires_grad=True)
+        t0 = time.perf_counter()
+        m = fv.OMesh.from_mesh(mesh, node_coords=X)
+        g = fv.geometry(m)
+        R = fv.scalar_residual(m, g, s, phi, g.normal[:, 0] * g.area, gam)
+        torch.autograd.grad((R * lam).sum(), (X, phi))
+        times.append(time.perf_counter() - t0)
+    tm = float(np.median(times))
+    return {"value": F / tm / 1e9, "unit": "Gfaces/s", "cores": threads, "kind": "port",
+            "sample": f"{reps} evaluations by the torch-CPU float64 oracle of the same tet workload at 6 x {n}^3 "
+            f"({C} cells, {F} faces), median {tm:.2f} s each"}
+
+
 def newton_compare(device, nx=200, ny=100, nz=100, mu=1e-3, max_steps=80, cfl0=10.0):
     """Newton to ||R|| <= 1e-12 + 1e-10 ||R0|| on the Re = 1000 hex box (u, v, w, p), from the seeded
     state, with the inner Krylov solve on the compact block Jacobian in FP32 and in FP64."""
@@ -307,7 +452,9 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=("sst", "flow"), default="sst")
+    ap.add_argument("--workload", choices=("sst", "flow", "tet"), default="sst")
+    ap.add_argument("--no-tet", action="store_true", help="skip the tetrahedral scalar-transport line")
+    ap.add_argument("--tet-n", type=int, default=96)
     ap.add_argument("--size", type=int, nargs=3, default=(128, 64, 64), metavar=("NX", "NY", "NZ"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=2)
@@ -344,6 +491,13 @@ def main(argv=None):
         print(json.dumps(newton_compare(dev, *args.newton_size, cfl0=args.newton_cfl0)), flush=True)
         return 0
     wl = args.workload
+    if wl == "tet":
+        _, tres = tet_workload(dev, args.tet_n, steps=args.steps, warmup=args.warmup)
+        tres.update({"n_gpus": 1, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+                     "vs_baseline": None, "data": "synthetic (jittered mesh seed 3, phi ~ U(0,1), cotangent N(0,1))",
+                     "cpu_baseline": None if args.no_cpu_baseline else tet_cpu_baseline()})
+        print(json.dumps(tres), flush=True)
+        return 0
     nx, ny, nz = args.size
     mesh, op, x, lam, _, _ = build_workload(dev, wl, nx, ny, nz)
     raw = op._raw
@@ -434,6 +588,11 @@ def main(argv=None):
         "roofline": roofline,
         "cpu_baseline": None,
     }
+    if not args.no_tet:
+        _, tres = tet_workload(dev, args.tet_n, steps=10, warmup=2)
+        out["tet_scalar"] = {k: tres[k] for k in ("value", "unit", "ms_per_step", "config", "kernel_avg_ms",
+                                                  "roofline")}
+        torch.cuda.empty_cache()
     if not args.no_newton:
         out["newton"] = newton_compare(dev, *args.newton_size, cfl0=args.newton_cfl0)
     if not args.no_cpu_baseline:
