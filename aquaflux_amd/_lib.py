@@ -219,6 +219,10 @@ _SIGNATURES = {
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxRansConfig), ctypes.POINTER(AfxRansInputs),
         c_void_p, ctypes.POINTER(AfxRansCotangents), c_void_p, ctypes.c_int64, c_void_p,
     ],
+    "afx_rans_residual_vjp": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxRansConfig), ctypes.POINTER(AfxRansInputs),
+        c_void_p, c_void_p, ctypes.POINTER(AfxRansCotangents), c_void_p, ctypes.c_int64, c_void_p,
+    ],
     "afx_rans_fields": [
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxRansConfig), ctypes.POINTER(AfxRansInputs),
         c_void_p, c_void_p,

@@ -39,6 +39,8 @@ enum KernelId : int {
   kKScalarLimAdj,
   kKGeomAdjFace,
   kKGeomAdjNode,
+  kKRansScalarRes,
+  kKRansFlowResAdj,
   kKNumKernels
 };
 

@@ -13,10 +13,10 @@ const char* const kKernelNames[kKNumKernels] = {
     "geom_face",     "geom_approx",   "geom_orient",     "geom_volume",  "flow_grad",
     "flow_ap",       "flow_res",      "flow_res_adj",    "flow_ap_adj",  "flow_grad_adj",
     "scalar_grad",   "scalar_res",    "scalar_res_adj",  "scalar_grad_adj", "geom_adj_cell",
-    "rans_closure",  "rans_res",      "rans_res_adj",    "rans_closure_adj", "bcsr_assemble",
+    "rans_closure",  "rans_flow_res", "rans_scalar_res_adj", "rans_closure_adj", "bcsr_assemble",
     "bcsr_spmv",     "pbj_setup",     "pbj_apply",       "blas",
     "rans_grad",     "rans_ap",       "rans_ap_adj",     "rans_grad_adj",   "scalar_lim_adj",
-    "geom_adj_face", "geom_adj_node",
+    "geom_adj_face", "geom_adj_node",   "rans_scalar_res", "rans_flow_res_adj",
 };
 
 namespace {

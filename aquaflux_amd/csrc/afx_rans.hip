@@ -393,21 +393,88 @@ __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<
 }
 
 // ---------------------------------------------------------------------------------------------
-// pass R
-template <int D, class S, int ADVF, int ADVS>
-__global__ void __launch_bounds__(256) k_rans_res(MeshView m, RansArgs<S> a, Out<S> R) {
+// pass R, flow part: momentum and continuity rows, and the Rhie-Chow face mass flux (written by
+// each face's owner) that the k / omega fluxes advect on.
+template <int D, class S, int ADVF>
+__global__ void __launch_bounds__(256) k_rans_fres(MeshView m, RansArgs<S> a, Out<S> R, Out<S> mdot) {
   constexpr int NV = D + 1;
-  const SstConst& sc = a.sc;
   const int32_t c = xcd_thread_index();
   if (c >= m.C) return;
   FlowCell<D, S> X;
   load_flow<D, S>(a, m, c, true, ADVF == kAdvLimited, X);
+  S acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = sconst<S>(0.0);
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    FaceGeo<D> q;
+    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    S out[NV];
+    if (o >= 0) {
+      FlowCell<D, S> O;
+      load_flow<D, S>(a, m, o, true, ADVF == kAdvLimited, O);
+      if (r.own)
+        flow_flux_interior<D, ADVF, S>(q, X, O, out);
+      else
+        flow_flux_interior<D, ADVF, S>(q, O, X, out);
+    } else {
+      S ub[D];
+#pragma unroll
+      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.f];
+      const int kind = a.kind[r.f];
+      flow_flux_boundary<D, ADVF, S>(q, kind, X, ub, a.pb[r.f], rans_gamma_b<S>(a, kind, r.f, c, X.mu), out);
+    }
+    if (r.own) mdot.set(r.f, out[D]);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = r.own ? acc[v] + out[v] : acc[v] - out[v];
+  }
+  if (R.v == nullptr) return;
+#pragma unroll
+  for (int i = 0; i < D; ++i) R.set((int64_t)i * m.C + c, acc[i] - a.bf[i] * X.V);
+  S cont = acc[D];
+  if (c == a.pin) cont = X.p - a.pin_value;
+  R.set((int64_t)D * m.C + c, cont);
+}
+
+// k and omega face fluxes of one face seen from cell c (all scalar operands in registers).
+template <int D, class S, int ADVS>
+AFX_DEV void rans_scalar_face(const RansArgs<S>& a, const MeshView& m, const FaceRef& r, int32_t o,
+                              const FaceGeo<D>& q, S Qf, const ScalarCell<D, S>& Kc, const ScalarCell<D, S>& Wc,
+                              double dP, S& fk, S& fw) {
+  const SstConst& sc = a.sc;
+  if (o >= 0) {
+    ScalarCell<D, S> Ko, Wo;
+    load_scalar<D, S>(a, m, o, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Ko);
+    load_scalar<D, S>(a, m, o, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wo);
+    if (r.own) {
+      fk = scalar_flux_interior<D, ADVS, S>(q, Qf, Kc.phi, Ko.phi, Kc.G, Ko.G, Kc.psi, Ko.psi, Kc.gam, Ko.gam);
+      fw = scalar_flux_interior<D, ADVS, S>(q, Qf, Wc.phi, Wo.phi, Wc.G, Wo.G, Wc.psi, Wo.psi, Wc.gam, Wo.gam);
+    } else {
+      fk = scalar_flux_interior<D, ADVS, S>(q, Qf, Ko.phi, Kc.phi, Ko.G, Kc.G, Ko.psi, Kc.psi, Ko.gam, Kc.gam);
+      fw = scalar_flux_interior<D, ADVS, S>(q, Qf, Wo.phi, Wc.phi, Wo.G, Wc.G, Wo.psi, Wc.psi, Wo.gam, Wc.gam);
+    }
+  } else {
+    const S zero = sconst<S>(0.0);
+    const S bk = scalar_bval<D, S>(a.kbc[r.f], q, Kc.phi, Kc.G, a.kval[r.f], zero, zero);
+    S gk = Kc.gam;
+    if (a.wallf[r.f]) gk = (1.0 - wall_function_weight<S>(sc, a.nu[r.P], dP, Kc.phi)) * Kc.gam;
+    fk = scalar_flux_boundary<D, ADVS, S>(q, Qf, Kc.phi, Kc.G, Kc.psi, bk, gk);
+    const S bw = scalar_bval<D, S>(a.wbc[r.f], q, Wc.phi, Wc.G, a.wval[r.f], zero, zero);
+    fw = scalar_flux_boundary<D, ADVS, S>(q, Qf, Wc.phi, Wc.G, Wc.psi, bw, Wc.gam);
+  }
+}
+
+// pass R, k / omega part: transport on mdot / density, SST sources, omega wall fixation.
+template <int D, class S, int ADVS>
+__global__ void __launch_bounds__(256) k_rans_sres(MeshView m, RansArgs<S> a, In<S> mdot, Out<S> R) {
+  const SstConst& sc = a.sc;
+  const int32_t c = xcd_thread_index();
+  if (c >= m.C) return;
   ScalarCell<D, S> Kc, Wc;
   load_scalar<D, S>(a, m, c, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Kc);
   load_scalar<D, S>(a, m, c, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wc);
-  S acc[NV], acck = sconst<S>(0.0), accw = sconst<S>(0.0);
-#pragma unroll
-  for (int v = 0; v < NV; ++v) acc[v] = sconst<S>(0.0);
+  S acck = sconst<S>(0.0), accw = sconst<S>(0.0);
   const double irho = 1.0 / sc.density;
   const double dP = a.dwall[c];
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
@@ -415,53 +482,16 @@ __global__ void __launch_bounds__(256) k_rans_res(MeshView m, RansArgs<S> a, Out
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
-    S out[NV], fk, fw;
-    if (o >= 0) {
-      FlowCell<D, S> O;
-      load_flow<D, S>(a, m, o, true, ADVF == kAdvLimited, O);
-      ScalarCell<D, S> Ko, Wo;
-      load_scalar<D, S>(a, m, o, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Ko);
-      load_scalar<D, S>(a, m, o, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wo);
-      if (r.own)
-        flow_flux_interior<D, ADVF, S>(q, X, O, out);
-      else
-        flow_flux_interior<D, ADVF, S>(q, O, X, out);
-      S Qf = out[D] * irho;
-      const ScalarCell<D, S>& KP = r.own ? Kc : Ko;
-      const ScalarCell<D, S>& KN = r.own ? Ko : Kc;
-      const ScalarCell<D, S>& WP = r.own ? Wc : Wo;
-      const ScalarCell<D, S>& WN = r.own ? Wo : Wc;
-      fk = scalar_flux_interior<D, ADVS, S>(q, Qf, KP.phi, KN.phi, KP.G, KN.G, KP.psi, KN.psi, KP.gam, KN.gam);
-      fw = scalar_flux_interior<D, ADVS, S>(q, Qf, WP.phi, WN.phi, WP.G, WN.G, WP.psi, WN.psi, WP.gam, WN.gam);
-    } else {
-      S ub[D];
-#pragma unroll
-      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.f];
-      const int kind = a.kind[r.f];
-      flow_flux_boundary<D, ADVF, S>(q, kind, X, ub, a.pb[r.f], rans_gamma_b<S>(a, kind, r.f, c, X.mu), out);
-      S Qf = out[D] * irho;
-      S zero = sconst<S>(0.0);
-      S bk = scalar_bval<D, S>(a.kbc[r.f], q, Kc.phi, Kc.G, a.kval[r.f], zero, zero);
-      S gk = Kc.gam;
-      if (a.wallf[r.f]) gk = (1.0 - wall_function_weight<S>(sc, a.nu[c], dP, Kc.phi)) * Kc.gam;
-      fk = scalar_flux_boundary<D, ADVS, S>(q, Qf, Kc.phi, Kc.G, Kc.psi, bk, gk);
-      S bw = scalar_bval<D, S>(a.wbc[r.f], q, Wc.phi, Wc.G, a.wval[r.f], zero, zero);
-      fw = scalar_flux_boundary<D, ADVS, S>(q, Qf, Wc.phi, Wc.G, Wc.psi, bw, Wc.gam);
-    }
-#pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] = r.own ? acc[v] + out[v] : acc[v] - out[v];
+    const S Qf = mdot[r.f] * irho;
+    S fk, fw;
+    rans_scalar_face<D, S, ADVS>(a, m, r, o, q, Qf, Kc, Wc, dP, fk, fw);
     acck = r.own ? acck + fk : acck - fk;
     accw = r.own ? accw + fw : accw - fw;
   }
-#pragma unroll
-  for (int i = 0; i < D; ++i) R.set((int64_t)i * m.C + c, acc[i] - a.bf[i] * X.V);
-  S cont = acc[D];
-  if (c == a.pin) cont = X.p - a.pin_value;
-  R.set((int64_t)D * m.C + c, cont);
   const bool wall = a.wallc[c] != 0;
   S pk, dk, pw, dw, cw;
-  sst_sources<S>(sc, Kc.phi, Wc.phi, a.nut[c], a.Sr[c], a.f1[c], a.cd[c], a.tau[c], a.nu[c], dP, wall, X.V, pk, dk,
-                 pw, dw, cw);
+  sst_sources<S>(sc, Kc.phi, Wc.phi, a.nut[c], a.Sr[c], a.f1[c], a.cd[c], a.tau[c], a.nu[c], dP, wall, m.vol[c], pk,
+                 dk, pw, dw, cw);
   R.set((int64_t)(D + 1) * m.C + c, (acck - pk) - dk);
   S rw = ((accw - pw) - dw) - cw;
   if (wall) rw = Wc.phi - omega_wall_k<S>(sc, a.nu[c], dP, Kc.phi);
@@ -511,9 +541,9 @@ AFX_DEV void adj_gamma(double gam_bar, double f1, double nut, double s1, double 
   nut_b += gam_bar * (f1 * s1 + (1.0 - f1) * s2);
 }
 
-// Routes the cotangent of the momentum boundary viscosity.
-template <int D>
-AFX_DEV void adj_gamma_b(const RansArgs<double>& a, int kind, int32_t f, int32_t P, double gbb, RansCellBar<D>& B) {
+// Routes the cotangent of the momentum boundary viscosity (B has fl, mum, nutw).
+template <int D, class BarT>
+AFX_DEV void adj_gamma_b(const RansArgs<double>& a, int kind, int32_t f, int32_t P, double gbb, BarT& B) {
   if (kind == kFlowWall) {
     B.mum += gbb;
     const double nw = a.wallf[f] ? a.nutw[P] : 0.0;
@@ -524,82 +554,85 @@ AFX_DEV void adj_gamma_b(const RansArgs<double>& a, int kind, int32_t f, int32_t
   }
 }
 
-// reverse pass R'
-template <int D, int ADVF, int ADVS>
-__global__ void __launch_bounds__(256) k_rans_res_adj(MeshView m, RansArgs<double> a, const double* __restrict__ lam,
-                                                      RansBars bar) {
-  constexpr int NV = D + 1, NA = D + 3;
+template <int D>
+struct FlowBarExt {
+  FlowCellBar<D> fl;
+  double mum, nutw;
+};
+
+template <int D>
+struct ScalarBarExt {
+  ScalarBar<D> k, w;
+  double nut, f1, Sr, cd, tau, nu;
+  AFX_DEV void zero() {
+    k.zero();
+    w.zero();
+    nut = f1 = Sr = cd = tau = nu = 0.0;
+  }
+};
+
+// reverse pass R', k / omega part: cotangents of the scalar fluxes, sources and fixation; each
+// face's owner writes the mass-flux cotangent mdot_bar[f] the flow part consumes.  With R set, the
+// k / omega residual rows come out of the same face evaluations.
+template <int D, int ADVS>
+__global__ void __launch_bounds__(256) k_rans_sres_adj(MeshView m, RansArgs<double> a,
+                                                       const double* __restrict__ mdot,
+                                                       const double* __restrict__ lam, RansBars bar,
+                                                       double* __restrict__ mdot_bar, double* __restrict__ R) {
   const SstConst& sc = a.sc;
   const int32_t c = xcd_thread_index();
   if (c >= m.C) return;
-  FlowCell<D, double> X;
-  load_flow<D, double>(a, m, c, true, ADVF == kAdvLimited, X);
   ScalarCell<D, double> Kc, Wc;
   load_scalar<D, double>(a, m, c, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Kc);
   load_scalar<D, double>(a, m, c, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wc);
-  RansCellBar<D> B;
+  ScalarBarExt<D> B;
   B.zero();
   const bool wall = a.wallc[c] != 0;
   const double irho = 1.0 / sc.density;
   const double dP = a.dwall[c];
-  double lc[NA];
-#pragma unroll
-  for (int v = 0; v < NA; ++v) lc[v] = lam[(int64_t)v * m.C + c];
-  const double lc_cont = c == a.pin ? 0.0 : lc[D];
-  const double lc_w = wall ? 0.0 : lc[D + 2];  // flux / source part of the omega row
+  const double V = m.vol[c];
+  const double lck = lam[(int64_t)(D + 1) * m.C + c];
+  const double lcw = wall ? 0.0 : lam[(int64_t)(D + 2) * m.C + c];  // flux / source part of the omega row
+  double acck = 0.0, accw = 0.0;
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
     FaceRef r = decode_face(m, m.cface[k]);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
-    double lf[NV];
+    const double Qf = mdot[r.f] * irho;
+    double Qb;
     if (o >= 0) {
-      FlowCell<D, double> O;
-      load_flow<D, double>(a, m, o, true, ADVF == kAdvLimited, O);
       ScalarCell<D, double> Ko, Wo;
       load_scalar<D, double>(a, m, o, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Ko);
       load_scalar<D, double>(a, m, o, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wo);
-      double out[NV];
-      const FlowCell<D, double>& P = r.own ? X : O;
-      const FlowCell<D, double>& N = r.own ? O : X;
-      flow_flux_interior<D, ADVF, double>(q, P, N, out);
-      const double Qf = out[D] * irho;
-      const bool wo = a.wallc[o] != 0;
       const double lk_o = lam[(int64_t)(D + 1) * m.C + o];
-      const double lw_o = wo ? 0.0 : lam[(int64_t)(D + 2) * m.C + o];
-      const double lkf = r.own ? lc[D + 1] - lk_o : lk_o - lc[D + 1];
-      const double lwf = r.own ? lc_w - lw_o : lw_o - lc_w;
-      const ScalarCell<D, double>& KP = r.own ? Kc : Ko;
-      const ScalarCell<D, double>& KN = r.own ? Ko : Kc;
-      const ScalarCell<D, double>& WP = r.own ? Wc : Wo;
-      const ScalarCell<D, double>& WN = r.own ? Wo : Wc;
-      const int side = r.own ? 0 : 1;
-      double Qb = adj_scalar_flux_interior<D, ADVS>(q, Qf, KP.phi, KN.phi, KP.G, KN.G, KP.psi, KN.psi, KP.gam, KN.gam,
-                                                    lkf, side, B.k);
-      Qb += adj_scalar_flux_interior<D, ADVS>(q, Qf, WP.phi, WN.phi, WP.G, WN.G, WP.psi, WN.psi, WP.gam, WN.gam, lwf,
-                                              side, B.w);
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        double lo = lam[(int64_t)v * m.C + o];
-        if (v == D && o == a.pin) lo = 0.0;
-        const double lcv = v == D ? lc_cont : lc[v];
-        lf[v] = r.own ? lcv - lo : lo - lcv;
-      }
-      lf[D] += Qb * irho;
-      adj_flux_interior<D, ADVF>(q, P, N, lf, side, B.fl);
+      const double lw_o = a.wallc[o] ? 0.0 : lam[(int64_t)(D + 2) * m.C + o];
+      const double lkf = r.own ? lck - lk_o : lk_o - lck;
+      const double lwf = r.own ? lcw - lw_o : lw_o - lcw;
       if (r.own) {
-#pragma unroll
-        for (int i = 0; i < D; ++i) bar.ub[(int64_t)i * m.F + r.f] = 0.0;
-        bar.pb[r.f] = 0.0;
+        Qb = adj_scalar_flux_interior<D, ADVS>(q, Qf, Kc.phi, Ko.phi, Kc.G, Ko.G, Kc.psi, Ko.psi, Kc.gam, Ko.gam, lkf,
+                                               0, B.k);
+        Qb += adj_scalar_flux_interior<D, ADVS>(q, Qf, Wc.phi, Wo.phi, Wc.G, Wo.G, Wc.psi, Wo.psi, Wc.gam, Wo.gam,
+                                                lwf, 0, B.w);
+        if (R) {
+          acck += scalar_flux_interior<D, ADVS, double>(q, Qf, Kc.phi, Ko.phi, Kc.G, Ko.G, Kc.psi, Ko.psi, Kc.gam,
+                                                        Ko.gam);
+          accw += scalar_flux_interior<D, ADVS, double>(q, Qf, Wc.phi, Wo.phi, Wc.G, Wo.G, Wc.psi, Wo.psi, Wc.gam,
+                                                        Wo.gam);
+        }
+      } else {
+        Qb = adj_scalar_flux_interior<D, ADVS>(q, Qf, Ko.phi, Kc.phi, Ko.G, Kc.G, Ko.psi, Kc.psi, Ko.gam, Kc.gam, lkf,
+                                               1, B.k);
+        Qb += adj_scalar_flux_interior<D, ADVS>(q, Qf, Wo.phi, Wc.phi, Wo.G, Wc.G, Wo.psi, Wc.psi, Wo.gam, Wc.gam,
+                                                lwf, 1, B.w);
+        if (R) {
+          acck -= scalar_flux_interior<D, ADVS, double>(q, Qf, Ko.phi, Kc.phi, Ko.G, Kc.G, Ko.psi, Kc.psi, Ko.gam,
+                                                        Kc.gam);
+          accw -= scalar_flux_interior<D, ADVS, double>(q, Qf, Wo.phi, Wc.phi, Wo.G, Wc.G, Wo.psi, Wc.psi, Wo.gam,
+                                                        Wc.gam);
+        }
       }
     } else {
-      const int kind = a.kind[r.f];
-      double ub[D], out[NV];
-#pragma unroll
-      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.f];
-      const double gB = rans_gamma_b<double>(a, kind, r.f, c, X.mu);
-      flow_flux_boundary<D, ADVF, double>(q, kind, X, ub, a.pb[r.f], gB, out);
-      const double Qf = out[D] * irho;
       // k face
       const double bk = scalar_bval<D, double>(a.kbc[r.f], q, Kc.phi, Kc.G, a.kval[r.f], 0.0, 0.0);
       double wgt = 0.0, gk = Kc.gam;
@@ -608,7 +641,7 @@ __global__ void __launch_bounds__(256) k_rans_res_adj(MeshView m, RansArgs<doubl
         gk = (1.0 - wgt) * Kc.gam;
       }
       double bkb, gkb, vb;
-      double Qb = adj_scalar_flux_boundary<D, ADVS>(q, Qf, Kc.phi, Kc.G, Kc.psi, bk, gk, lc[D + 1], B.k, bkb, gkb);
+      Qb = adj_scalar_flux_boundary<D, ADVS>(q, Qf, Kc.phi, Kc.G, Kc.psi, bk, gk, lck, B.k, bkb, gkb);
       adj_scalar_bval<D>(a.kbc[r.f], q, bkb, B.k, vb);
       if (a.wallf[r.f]) {
         B.k.gam += (1.0 - wgt) * gkb;
@@ -623,13 +656,133 @@ __global__ void __launch_bounds__(256) k_rans_res_adj(MeshView m, RansArgs<doubl
       // omega face
       const double bw = scalar_bval<D, double>(a.wbc[r.f], q, Wc.phi, Wc.G, a.wval[r.f], 0.0, 0.0);
       double bwb, gwb;
-      Qb += adj_scalar_flux_boundary<D, ADVS>(q, Qf, Wc.phi, Wc.G, Wc.psi, bw, Wc.gam, lc_w, B.w, bwb, gwb);
+      Qb += adj_scalar_flux_boundary<D, ADVS>(q, Qf, Wc.phi, Wc.G, Wc.psi, bw, Wc.gam, lcw, B.w, bwb, gwb);
       adj_scalar_bval<D>(a.wbc[r.f], q, bwb, B.w, vb);
       B.w.gam += gwb;
-      // flow face
+      if (R) {
+        acck += scalar_flux_boundary<D, ADVS, double>(q, Qf, Kc.phi, Kc.G, Kc.psi, bk, gk);
+        accw += scalar_flux_boundary<D, ADVS, double>(q, Qf, Wc.phi, Wc.G, Wc.psi, bw, Wc.gam);
+      }
+    }
+    if (r.own) mdot_bar[r.f] = Qb * irho;
+  }
+  // sources (cell-local): R_k -= prod_k + dest_k, R_w -= prod_w + dest_w + cross_w (non-wall)
+  {
+    const double lk = -lck, lw = -lcw;
+    if (wall) {  // the near-wall blends: seeded tangents of the source function
+      double inp[8] = {Kc.phi, Wc.phi, a.nut[c], a.Sr[c], a.f1[c], a.cd[c], a.tau[c], a.nu[c]};
+      double grad[8];
+      for (int s = 0; s < 8; ++s) {
+        dual in[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) in[t] = dual{inp[t], t == s ? 1.0 : 0.0};
+        dual pk, dk, pw, dw, cw;
+        sst_sources<dual>(sc, in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], dP, wall, V, pk, dk, pw, dw, cw);
+        grad[s] = lk * (pk.d + dk.d) + lw * (pw.d + dw.d + cw.d);
+      }
+      B.k.phi += grad[0];
+      B.w.phi += grad[1];
+      B.nut += grad[2];
+      B.Sr += grad[3];
+      B.f1 += grad[4];
+      B.cd += grad[5];
+      B.tau += grad[6];
+      B.nu += grad[7];
+    } else {
+      adj_sst_sources_open(sc, Kc.phi, Wc.phi, a.nut[c], a.Sr[c], a.f1[c], V, lk, lw, B.k.phi, B.w.phi, B.nut, B.Sr,
+                           B.f1, B.cd);
+    }
+  }
+  if (wall) {  // R_w = w - omega_wall(nu, d, k)
+    const double l = lam[(int64_t)(D + 2) * m.C + c];
+    B.w.phi += l;
+    dual ok = omega_wall_k<dual>(sc, dual{a.nu[c], 0.0}, dP, dual{Kc.phi, 1.0});
+    dual on = omega_wall_k<dual>(sc, dual{a.nu[c], 1.0}, dP, dual{Kc.phi, 0.0});
+    B.k.phi -= l * ok.d;
+    B.nu -= l * on.d;
+  }
+  if (R) {
+    double pk, dk, pw, dw, cw;
+    sst_sources<double>(sc, Kc.phi, Wc.phi, a.nut[c], a.Sr[c], a.f1[c], a.cd[c], a.tau[c], a.nu[c], dP, wall, V, pk,
+                        dk, pw, dw, cw);
+    R[(int64_t)(D + 1) * m.C + c] = (acck - pk) - dk;
+    R[(int64_t)(D + 2) * m.C + c] = wall ? Wc.phi - omega_wall_k<double>(sc, a.nu[c], dP, Kc.phi)
+                                         : ((accw - pw) - dw) - cw;
+  }
+  // diffusivities gamma_k, gamma_w of this cell
+  adj_gamma(B.k.gam, a.f1[c], a.nut[c], sc.sk1, sc.sk2, B.nu, B.f1, B.nut);
+  adj_gamma(B.w.gam, a.f1[c], a.nut[c], sc.sw1, sc.sw2, B.nu, B.f1, B.nut);
+  bar.x[(int64_t)(D + 1) * m.C + c] = B.k.phi;
+  bar.x[(int64_t)(D + 2) * m.C + c] = B.w.phi;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    bar.G[(int64_t)((D + 1) * D + j) * m.C + c] = B.k.G[j];
+    bar.G[(int64_t)((D + 2) * D + j) * m.C + c] = B.w.G[j];
+  }
+  if (ADVS == kAdvLimited) {
+    bar.psi[(int64_t)D * m.C + c] = B.k.psi;
+    bar.psi[(int64_t)(D + 1) * m.C + c] = B.w.psi;
+  }
+  bar.nu[c] = B.nu;
+  bar.nut[c] = B.nut;
+  bar.f1[c] = B.f1;
+  bar.Sr[c] = B.Sr;
+  bar.cd[c] = B.cd;
+  bar.tau[c] = B.tau;
+}
+
+// reverse pass R', flow part: momentum / continuity flux cotangents plus the scalar part's
+// mass-flux cotangent.
+template <int D, int ADVF>
+__global__ void __launch_bounds__(256) k_rans_fres_adj(MeshView m, RansArgs<double> a, const double* __restrict__ lam,
+                                                       const double* __restrict__ mdot_bar, RansBars bar) {
+  constexpr int NV = D + 1;
+  const int32_t c = xcd_thread_index();
+  if (c >= m.C) return;
+  FlowCell<D, double> X;
+  load_flow<D, double>(a, m, c, true, ADVF == kAdvLimited, X);
+  FlowBarExt<D> B;
+  B.fl.zero();
+  B.mum = B.nutw = 0.0;
+  double lc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) lc[v] = lam[(int64_t)v * m.C + c];
+  const double lc_cont = c == a.pin ? 0.0 : lc[D];
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, m.cface[k]);
+    const int32_t o = r.own ? r.N : r.P;
+    FaceGeo<D> q;
+    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    double lf[NV];
+    if (o >= 0) {
+      FlowCell<D, double> O;
+      load_flow<D, double>(a, m, o, true, ADVF == kAdvLimited, O);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        double lo = lam[(int64_t)v * m.C + o];
+        if (v == D && o == a.pin) lo = 0.0;
+        const double lcv = v == D ? lc_cont : lc[v];
+        lf[v] = r.own ? lcv - lo : lo - lcv;
+      }
+      lf[D] += mdot_bar[r.f];
+      if (r.own)
+        adj_flux_interior<D, ADVF>(q, X, O, lf, 0, B.fl);
+      else
+        adj_flux_interior<D, ADVF>(q, O, X, lf, 1, B.fl);
+      if (r.own) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) bar.ub[(int64_t)i * m.F + r.f] = 0.0;
+        bar.pb[r.f] = 0.0;
+      }
+    } else {
+      const int kind = a.kind[r.f];
+      double ub[D];
+#pragma unroll
+      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.f];
+      const double gB = rans_gamma_b<double>(a, kind, r.f, c, X.mu);
 #pragma unroll
       for (int v = 0; v < NV; ++v) lf[v] = v == D ? lc_cont : lc[v];
-      lf[D] += Qb * irho;
+      lf[D] += mdot_bar[r.f];
       double ubb[D], pbb, gbb;
       adj_flux_boundary<D, ADVF>(q, kind, X, ub, a.pb[r.f], gB, lf, B.fl, ubb, pbb, gbb);
       adj_gamma_b<D>(a, kind, r.f, c, gbb, B);
@@ -638,74 +791,24 @@ __global__ void __launch_bounds__(256) k_rans_res_adj(MeshView m, RansArgs<doubl
       bar.pb[r.f] = pbb;
     }
   }
-  // sources (cell-local): R_k -= prod_k + dest_k, R_w -= prod_w + dest_w + cross_w (non-wall)
-  {
-    const double lk = -lc[D + 1], lw = -lc_w;
-    double inp[8] = {Kc.phi, Wc.phi, a.nut[c], a.Sr[c], a.f1[c], a.cd[c], a.tau[c], a.nu[c]};
-    double grad[8];
-    for (int s = 0; s < 8; ++s) {
-      dual in[8];
-#pragma unroll
-      for (int t = 0; t < 8; ++t) in[t] = dual{inp[t], t == s ? 1.0 : 0.0};
-      dual pk, dk, pw, dw, cw;
-      sst_sources<dual>(sc, in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], dP, wall, X.V, pk, dk, pw, dw, cw);
-      grad[s] = lk * (pk.d + dk.d) + lw * (pw.d + dw.d + cw.d);
-    }
-    B.k.phi += grad[0];
-    B.w.phi += grad[1];
-    B.nut += grad[2];
-    B.Sr += grad[3];
-    B.f1 += grad[4];
-    B.cd += grad[5];
-    B.tau += grad[6];
-    B.nu += grad[7];
-  }
-  if (wall) {  // R_w = w - omega_wall(nu, d, k)
-    const double l = lc[D + 2];
-    B.w.phi += l;
-    dual ok = omega_wall_k<dual>(sc, dual{a.nu[c], 0.0}, dP, dual{Kc.phi, 1.0});
-    dual on = omega_wall_k<dual>(sc, dual{a.nu[c], 1.0}, dP, dual{Kc.phi, 0.0});
-    B.k.phi -= l * ok.d;
-    B.nu -= l * on.d;
-  }
   if (c == a.pin) B.fl.p += lc[D];
-  // diffusivities gamma_k, gamma_w of this cell
-  adj_gamma(B.k.gam, a.f1[c], a.nut[c], sc.sk1, sc.sk2, B.nu, B.f1, B.nut);
-  adj_gamma(B.w.gam, a.f1[c], a.nut[c], sc.sw1, sc.sw2, B.nu, B.f1, B.nut);
-  // write
 #pragma unroll
   for (int i = 0; i < D; ++i) bar.x[(int64_t)i * m.C + c] = B.fl.u[i];
   bar.x[(int64_t)D * m.C + c] = B.fl.p;
-  bar.x[(int64_t)(D + 1) * m.C + c] = B.k.phi;
-  bar.x[(int64_t)(D + 2) * m.C + c] = B.w.phi;
 #pragma unroll
   for (int i = 0; i < D; ++i)
 #pragma unroll
     for (int j = 0; j < D; ++j) bar.G[(int64_t)(i * D + j) * m.C + c] = B.fl.Gu[i][j];
 #pragma unroll
-  for (int j = 0; j < D; ++j) {
-    bar.G[(int64_t)(D * D + j) * m.C + c] = B.fl.Gp[j];
-    bar.G[(int64_t)((D + 1) * D + j) * m.C + c] = B.k.G[j];
-    bar.G[(int64_t)((D + 2) * D + j) * m.C + c] = B.w.G[j];
-  }
+  for (int j = 0; j < D; ++j) bar.G[(int64_t)(D * D + j) * m.C + c] = B.fl.Gp[j];
   bar.aP[c] = B.fl.aP;
   if (ADVF == kAdvLimited) {
 #pragma unroll
     for (int i = 0; i < D; ++i) bar.psi[(int64_t)i * m.C + c] = B.fl.psi[i];
   }
-  if (ADVS == kAdvLimited) {
-    bar.psi[(int64_t)D * m.C + c] = B.k.psi;
-    bar.psi[(int64_t)(D + 1) * m.C + c] = B.w.psi;
-  }
   bar.mueff[c] = B.fl.mu;
   bar.rho[c] = B.fl.rho;
   bar.mu[c] = B.mum;
-  bar.nu[c] = B.nu;
-  bar.nut[c] = B.nut;
-  bar.f1[c] = B.f1;
-  bar.Sr[c] = B.Sr;
-  bar.cd[c] = B.cd;
-  bar.tau[c] = B.tau;
   bar.nutw[c] = B.nutw;
 }
 
@@ -776,27 +879,21 @@ __global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<d
   double k0, w0, Gu0[D][D], Gk0[D], Gw0[D];
   load_closure_inputs<D, double>(a, m, c, k0, w0, Gu0, Gk0, Gw0);
   const double nu0 = a.nu[c];
-  // inputs: 0 k, 1 w, 2 nu, then Gu (D*D), Gk (D), Gw (D)
-  constexpr int NI = 3 + D * D + 2 * D;
-  double gin[NI];
-  for (int s = 0; s < NI; ++s) {
-    if (wall && s >= 3 + D * D + D) {  // the wall-imposed omega gradient replaces the reconstructed one
+  // The closure depends on the gradients only through the strain magnitude and the products
+  // (A, B) of sst_closure_core: seed those six scalars, then chain to the gradients by hand.
+  const double strain0 = strain_magnitude<D, double>(Gu0);
+  double A0, B0;
+  closure_products<D, double>(Gk0, Gw0, dg, wall, A0, B0);
+  double gin[6];
+  const int nseed = wall ? 6 : 5;
+  for (int s = 0; s < 6; ++s) {
+    if (s >= nseed) {
       gin[s] = 0.0;
       continue;
     }
     auto sd = [&](int t) { return t == s ? 1.0 : 0.0; };
-    dual k = {k0, sd(0)}, w = {w0, sd(1)}, nu = {nu0, sd(2)};
-    dual Gu[D][D], Gk[D], Gw[D];
-#pragma unroll
-    for (int i = 0; i < D; ++i)
-#pragma unroll
-      for (int j = 0; j < D; ++j) Gu[i][j] = dual{Gu0[i][j], sd(3 + i * D + j)};
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-      Gk[j] = dual{Gk0[j], sd(3 + D * D + j)};
-      Gw[j] = dual{Gw0[j], sd(3 + D * D + D + j)};
-    }
-    SstClosure<dual> o = sst_closure<D, dual>(sc, k, w, Gu, Gk, Gw, nu, d, dg, wall);
+    SstClosure<dual> o = sst_closure_core<dual>(sc, dual{k0, sd(0)}, dual{w0, sd(1)}, dual{strain0, sd(3)},
+                                                dual{A0, sd(4)}, dual{B0, sd(5)}, dual{nu0, sd(2)}, d, wall);
     gin[s] = nut_b * o.nut.d + f1_b * o.f1.d + Sr_b * o.strain.d + cd_b * o.cd.d + nutw_b * o.nutw.d;
   }
   double xb_k = gin[0], xb_w = gin[1];
@@ -806,14 +903,23 @@ __global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<d
   for (int v = 0; v < NA; ++v)
 #pragma unroll
     for (int j = 0; j < D; ++j) Gb[v][j] = bar.G[(int64_t)(v * D + j) * m.C + c];
+  // strain0 = sqrt(2 S:S): d strain0 / d Gu_ij = 2 S_ij / strain0 (zero at strain0 = 0)
+  if (strain0 > 0.0) {
+    const double sb = gin[3] / strain0;
 #pragma unroll
-  for (int i = 0; i < D; ++i)
+    for (int i = 0; i < D; ++i)
 #pragma unroll
-    for (int j = 0; j < D; ++j) Gb[i][j] += gin[3 + i * D + j];
+      for (int j = 0; j < D; ++j) Gb[i][j] += sb * (Gu0[i][j] + Gu0[j][i]);
+  }
+  if (wall) {  // A = grad k . grad d, B = |grad k|^2
 #pragma unroll
-  for (int j = 0; j < D; ++j) {
-    Gb[D + 1][j] += gin[3 + D * D + j];
-    Gb[D + 2][j] += gin[3 + D * D + D + j];
+    for (int j = 0; j < D; ++j) Gb[D + 1][j] += gin[4] * dg[j] + 2.0 * gin[5] * Gk0[j];
+  } else {  // A = grad k . grad omega
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      Gb[D + 1][j] += gin[4] * Gw0[j];
+      Gb[D + 2][j] += gin[4] * Gk0[j];
+    }
   }
   double xb[NA];
 #pragma unroll
@@ -1049,22 +1155,22 @@ __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<doub
 // ---------------------------------------------------------------------------------------------
 // host side
 
-int64_t rans_work_doubles(int D, int64_t C, int mode) {
+int64_t rans_work_doubles(int D, int64_t C, int64_t F, int mode) {
   const int NA = D + 3, NL = D + 2;
-  int64_t fwd = (int64_t)(NA * D + NL + 7 + 1) * C;  // G, psi, closure(7), aP
+  int64_t fwd = (int64_t)(NA * D + NL + 7 + 1) * C + F;  // G, psi, closure(7), aP, face mass flux
   if (mode == 1) return 2 * fwd;
-  if (mode == 2) return fwd + (int64_t)(NA * D + 1 + NL + 2 * NL + 7) * C + (int64_t)NL * C;  // bars + ext ints
+  if (mode == 2) return fwd + (int64_t)(NA * D + 1 + NL + 2 * NL + 7) * C + (int64_t)NL * C + F;  // bars, ext, mdot_bar
   return fwd;
 }
 
 struct RWork {
-  double *G, *psi, *aP, *cl;      // cl: 7 closure arrays
-  double *Gt, *psit, *aPt, *clt;  // tangents
-  double *Gb, *aPb, *psib, *extb, *mueffb, *nutb, *f1b, *Srb, *cdb, *taub, *nutwb;
+  double *G, *psi, *aP, *cl, *mdot;  // cl: 7 closure arrays
+  double *Gt, *psit, *aPt, *clt, *mdott;  // tangents
+  double *Gb, *aPb, *psib, *extb, *mueffb, *nutb, *f1b, *Srb, *cdb, *taub, *nutwb, *mdotb;
   int32_t* ext;
 };
 
-RWork rcarve(void* base, int D, int64_t C, int mode) {
+RWork rcarve(void* base, int D, int64_t C, int64_t F, int mode) {
   const int NA = D + 3, NL = D + 2;
   double* p = (double*)base;
   RWork w{};
@@ -1077,11 +1183,13 @@ RWork rcarve(void* base, int D, int64_t C, int mode) {
   w.psi = take((int64_t)NL * C);
   w.aP = take(C);
   w.cl = take(7 * C);
+  w.mdot = take(F);
   if (mode == 1) {
     w.Gt = take((int64_t)NA * D * C);
     w.psit = take((int64_t)NL * C);
     w.aPt = take(C);
     w.clt = take(7 * C);
+    w.mdott = take(F);
   }
   if (mode == 2) {
     w.Gb = take((int64_t)NA * D * C);
@@ -1096,6 +1204,7 @@ RWork rcarve(void* base, int D, int64_t C, int mode) {
     w.taub = take(C);
     w.nutwb = take(C);
     w.ext = (int32_t*)take((int64_t)NL * C);
+    w.mdotb = take(F);
   }
   return w;
 }
@@ -1168,9 +1277,11 @@ void bind_closure(RansArgs<S>& a, double* cl, double* clt, int64_t C) {
   }
 }
 
+// Forward passes: G, closure, a_P; then, for `R` set, the flow and k / omega residual rows; for
+// `mdot_only`, just the face mass flux (and the flow rows when R is set) for the reverse passes.
 template <int D, class S, int ADVF, int ADVS, int GRAD>
 int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S> R, const RWork& w, double* Gt,
-                 double* psit, double* aPt, double* clt, int32_t* ext, hipStream_t s) {
+                 double* psit, double* aPt, double* clt, double* mdott, int32_t* ext, bool mdot_only, hipStream_t s) {
   MeshView m = afx_view(mh, g);
   const int64_t C = m.C;
   Out<S> oG{w.G, Gt}, oP{w.psi, psit};
@@ -1193,10 +1304,16 @@ int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S
   bind_closure<S>(a, w.cl, clt, C);
   AFX_LAUNCH(kKRansAp, (k_rans_ap<D, S, ADVF>), grid_for(C), dim3(kBlock), 0, s, m, a, Out<S>{w.aP, aPt});
   AFX_CHECK_LAUNCH();
-  if (R.v == nullptr) return 0;
+  if (R.v == nullptr && !mdot_only) return 0;
   a.aP.v = w.aP;
   a.aP.t = aPt;
-  AFX_LAUNCH(kKSstRes, (k_rans_res<D, S, ADVF, ADVS>), grid_for(C), dim3(kBlock), 0, s, m, a, R);
+  AFX_LAUNCH(kKSstRes, (k_rans_fres<D, S, ADVF>), grid_for(C), dim3(kBlock), 0, s, m, a, R, Out<S>{w.mdot, mdott});
+  AFX_CHECK_LAUNCH();
+  if (mdot_only) return 0;
+  In<S> md;
+  md.v = w.mdot;
+  md.t = mdott;
+  AFX_LAUNCH(kKRansScalarRes, (k_rans_sres<D, S, ADVS>), grid_for(C), dim3(kBlock), 0, s, m, a, md, R);
   AFX_CHECK_LAUNCH();
   return 0;
 }
@@ -1223,7 +1340,7 @@ int rcheck(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg,
                   cfg->wall_distance && cfg->wall_distance_gradient,
               "rans: a required configuration pointer is null");
   AFX_REQUIRE(cfg->density > 0.0, "rans: density must be positive");
-  AFX_REQUIRE(work && work_size >= rans_work_doubles(m->dim, m->C, mode) * 8, "rans: workspace too small");
+  AFX_REQUIRE(work && work_size >= rans_work_doubles(m->dim, m->C, m->F, mode) * 8, "rans: workspace too small");
   return 0;
 }
 
@@ -1231,7 +1348,7 @@ int rcheck(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg,
 
 extern "C" int afx_rans_workspace_bytes(const afx_mesh* m, int mode, int64_t* bytes) {
   AFX_REQUIRE(m && bytes && mode >= 0 && mode <= 2, "afx_rans_workspace_bytes: bad argument");
-  *bytes = rans_work_doubles(m->dim, m->C, mode) * 8 + 256;
+  *bytes = rans_work_doubles(m->dim, m->C, m->F, mode) * 8 + 256;
   return 0;
 }
 
@@ -1241,13 +1358,13 @@ extern "C" int afx_rans_residual(const afx_mesh* m, const afx_geometry* g, const
   int rc = rcheck(m, g, cfg, in, work, work_size, 0);
   if (rc) return rc;
   AFX_REQUIRE(residual, "afx_rans_residual: null residual");
-  RWork w = rcarve(work, m->dim, m->C, 0);
+  RWork w = rcarve(work, m->dim, m->C, m->F, 0);
   RansArgs<double> a = rargs<double>(cfg, in, nullptr);
   return rdispatch(m->dim, cfg->flow.advection, cfg->scalar_advection, cfg->flow.gradient,
                    [&](auto Dc, auto Fc, auto Sc, auto Gc) {
                      return rans_forward<decltype(Dc)::value, double, decltype(Fc)::value, decltype(Sc)::value,
                                          decltype(Gc)::value>(m, g, a, Out<double>{residual, nullptr}, w, nullptr,
-                                                              nullptr, nullptr, nullptr, nullptr, s);
+                                                              nullptr, nullptr, nullptr, nullptr, nullptr, false, s);
                    });
 }
 
@@ -1257,24 +1374,28 @@ extern "C" int afx_rans_jvp(const afx_mesh* m, const afx_geometry* g, const afx_
   int rc = rcheck(m, g, cfg, in, work, work_size, 1);
   if (rc) return rc;
   AFX_REQUIRE(tangent && residual && residual_tangent, "afx_rans_jvp: null argument");
-  RWork w = rcarve(work, m->dim, m->C, 1);
+  RWork w = rcarve(work, m->dim, m->C, m->F, 1);
   RansArgs<dual> a = rargs<dual>(cfg, in, tangent);
   return rdispatch(m->dim, cfg->flow.advection, cfg->scalar_advection, cfg->flow.gradient,
                    [&](auto Dc, auto Fc, auto Sc, auto Gc) {
                      return rans_forward<decltype(Dc)::value, dual, decltype(Fc)::value, decltype(Sc)::value,
                                          decltype(Gc)::value>(m, g, a, Out<dual>{residual, residual_tangent}, w,
-                                                              w.Gt, w.psit, w.aPt, w.clt, nullptr, s);
+                                                              w.Gt, w.psit, w.aPt, w.clt, w.mdott, nullptr, false, s);
                    });
 }
 
-extern "C" int afx_rans_vjp(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg,
-                            const afx_rans_inputs* in, const double* cotangent, afx_rans_cotangents* out, void* work,
-                            int64_t work_size, hipStream_t s) {
+namespace {
+
+// The reverse passes, after one forward recompute of G, the closure and a_P; `residual` (nullable)
+// also receives R from the same forward products.
+int rans_vjp_impl(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg, const afx_rans_inputs* in,
+                  const double* cotangent, afx_rans_cotangents* out, double* residual, void* work, int64_t work_size,
+                  hipStream_t s) {
   int rc = rcheck(m, g, cfg, in, work, work_size, 2);
   if (rc) return rc;
   AFX_REQUIRE(cotangent && out && out->state && out->mu && out->rho && out->nu && out->u_b && out->p_b,
               "afx_rans_vjp: null cotangent argument");
-  RWork w = rcarve(work, m->dim, m->C, 2);
+  RWork w = rcarve(work, m->dim, m->C, m->F, 2);
   RansArgs<double> a = rargs<double>(cfg, in, nullptr);
   RansBars bar{out->state, out->mu,  out->rho, out->nu, out->u_b, out->p_b, w.Gb,  w.aPb, w.psib,
                w.extb,     w.mueffb, w.nutb,   w.f1b,   w.Srb,    w.cdb,    w.taub, w.nutwb};
@@ -1283,8 +1404,9 @@ extern "C" int afx_rans_vjp(const afx_mesh* m, const afx_geometry* g, const afx_
                      constexpr int D = decltype(Dc)::value, AF = decltype(Fc)::value, AS = decltype(Sc)::value,
                                    GR = decltype(Gc)::value;
                      constexpr bool LF = AF == kAdvLimited, LS = AS == kAdvLimited;
-                     int r2 = rans_forward<D, double, AF, AS, GR>(m, g, a, Out<double>{nullptr, nullptr}, w, nullptr,
-                                                                  nullptr, nullptr, nullptr, w.ext, s);
+                     int r2 = rans_forward<D, double, AF, AS, GR>(m, g, a, Out<double>{residual, nullptr}, w,
+                                                                  nullptr, nullptr, nullptr, nullptr, nullptr, w.ext,
+                                                                  true, s);
                      if (r2) return r2;
                      MeshView v = afx_view(m, g);
                      RansArgs<double> b = a;
@@ -1292,8 +1414,11 @@ extern "C" int afx_rans_vjp(const afx_mesh* m, const afx_geometry* g, const afx_
                      b.psi.v = w.psi;
                      b.aP.v = w.aP;
                      bind_closure<double>(b, w.cl, nullptr, m->C);
-                     AFX_LAUNCH(kKSstResAdj, (k_rans_res_adj<D, AF, AS>), grid_for(m->C), dim3(kBlock), 0, s, v, b,
-                                cotangent, bar);
+                     AFX_LAUNCH(kKSstResAdj, (k_rans_sres_adj<D, AS>), grid_for(m->C), dim3(kBlock), 0, s, v, b,
+                                w.mdot, cotangent, bar, w.mdotb, residual);
+                     AFX_CHECK_LAUNCH();
+                     AFX_LAUNCH(kKRansFlowResAdj, (k_rans_fres_adj<D, AF>), grid_for(m->C), dim3(kBlock), 0, s, v, b,
+                                cotangent, w.mdotb, bar);
                      AFX_CHECK_LAUNCH();
                      AFX_LAUNCH(kKRansApAdj, (k_rans_ap_adj<D, AF>), grid_for(m->C), dim3(kBlock), 0, s, v, b, bar);
                      AFX_CHECK_LAUNCH();
@@ -1307,15 +1432,30 @@ extern "C" int afx_rans_vjp(const afx_mesh* m, const afx_geometry* g, const afx_
                    });
 }
 
+}  // namespace
+
+extern "C" int afx_rans_vjp(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg,
+                            const afx_rans_inputs* in, const double* cotangent, afx_rans_cotangents* out, void* work,
+                            int64_t work_size, hipStream_t s) {
+  return rans_vjp_impl(m, g, cfg, in, cotangent, out, nullptr, work, work_size, s);
+}
+
+extern "C" int afx_rans_residual_vjp(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg,
+                                     const afx_rans_inputs* in, const double* cotangent, double* residual,
+                                     afx_rans_cotangents* out, void* work, int64_t work_size, hipStream_t s) {
+  AFX_REQUIRE(residual, "afx_rans_residual_vjp: null residual");
+  return rans_vjp_impl(m, g, cfg, in, cotangent, out, residual, work, work_size, s);
+}
+
 extern "C" int afx_rans_fields(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg,
                                const afx_rans_inputs* in, double* fields, hipStream_t s) {
   AFX_REQUIRE(m && g && cfg && in && fields, "afx_rans_fields: null argument");
-  RWork w = rcarve(fields, m->dim, m->C, 0);
+  RWork w = rcarve(fields, m->dim, m->C, m->F, 0);
   RansArgs<double> a = rargs<double>(cfg, in, nullptr);
   return rdispatch(m->dim, cfg->flow.advection, cfg->scalar_advection, cfg->flow.gradient,
                    [&](auto Dc, auto Fc, auto Sc, auto Gc) {
                      return rans_forward<decltype(Dc)::value, double, decltype(Fc)::value, decltype(Sc)::value,
                                          decltype(Gc)::value>(m, g, a, Out<double>{nullptr, nullptr}, w, nullptr,
-                                                              nullptr, nullptr, nullptr, nullptr, s);
+                                                              nullptr, nullptr, nullptr, nullptr, nullptr, false, s);
                    });
 }

@@ -105,14 +105,9 @@ struct SstClosure {
   S nut, f1, strain, cd, nutw;
 };
 
-// Closure fields of one cell (transport.py:369-546): strain rate (with the wall log-layer blend),
-// F1 from the (wall-imposed) omega gradient, eddy viscosity with the F2 limiter, the cross-diffusion
-// value 2 (1 - F1) sigma_w2 grad k . grad w / w, and the wall-function eddy viscosity of its wall faces.
+// Strain-rate magnitude sqrt(2 S_ij S_ij) of a velocity gradient (strain.py:46-66).
 template <int D, class S>
-AFX_DEV SstClosure<S> sst_closure(const SstConst& c, S k, S w, const S (&Gu)[D][D], const S (&Gk)[D],
-                                  const S (&Gw_rec)[D], S nu, double d, const double (&dgrad)[D], bool wall) {
-  SstClosure<S> o;
-  // strain magnitude sqrt(2 S_ij S_ij)
+AFX_DEV S strain_magnitude(const S (&Gu)[D][D]) {
   S ss = sconst<S>(0.0);
 #pragma unroll
   for (int i = 0; i < D; ++i)
@@ -121,21 +116,26 @@ AFX_DEV SstClosure<S> sst_closure(const SstConst& c, S k, S w, const S (&Gu)[D][
       S sij = 0.5 * (Gu[i][j] + Gu[j][i]);
       ss = ss + sij * sij;
     }
-  S strain = safe_sqrt(2.0 * ss);
-  S Gw[D];
-#pragma unroll
-  for (int j = 0; j < D; ++j) Gw[j] = Gw_rec[j];
+  return safe_sqrt(2.0 * ss);
+}
+
+// Closure of one cell given the raw strain magnitude and grad k . grad omega in factored form:
+// away from walls kw = A (= grad k . grad omega); at a wall cell the omega gradient is the imposed
+// d omega_wall/dd grad d + d omega_wall/dk grad k, so kw = dodd A + dodk B with A = grad k . grad d,
+// B = |grad k|^2.  (transport.py:369-546, sst.py:125-239)
+template <class S>
+AFX_DEV SstClosure<S> sst_closure_core(const SstConst& c, S k, S w, S strain0, S A, S B, S nu, double d,
+                                       bool wall) {
+  SstClosure<S> o;
+  S strain = strain0;
+  S kw = A;
   if (wall) {
     S wgt = wall_function_weight(c, nu, d, k);
-    strain = (1.0 - wgt) * strain + wgt * log_layer_shear_rate(c, d, k);
+    strain = (1.0 - wgt) * strain0 + wgt * log_layer_shear_rate(c, d, k);
     S dodd, dodk;
     omega_wall<S>(c, nu, sconst<S>(d), k, &dodd, &dodk);
-#pragma unroll
-    for (int j = 0; j < D; ++j) Gw[j] = dodd * dgrad[j] + dodk * Gk[j];
+    kw = dodd * A + dodk * B;
   }
-  S kw = Gk[0] * Gw[0];
-#pragma unroll
-  for (int j = 1; j < D; ++j) kw = kw + Gk[j] * Gw[j];
   // F1 (sst.py:152-195)
   S cdk = dmax(2.0 * c.sw2 * kw / w, kCrossDiffusionFloor);
   S pk = kpos(k);
@@ -151,6 +151,37 @@ AFX_DEV SstClosure<S> sst_closure(const SstConst& c, S k, S w, const S (&Gu)[D][
   o.cd = 2.0 * (1.0 - o.f1) * c.sw2 * kw / w;
   o.nutw = wall ? nut_wall(c, nu, d, k) : sconst<S>(0.0);
   return o;
+}
+
+// The factored inputs (A, B) of sst_closure_core.
+template <int D, class S>
+AFX_DEV void closure_products(const S (&Gk)[D], const S (&Gw_rec)[D], const double (&dgrad)[D], bool wall, S& A,
+                              S& B) {
+  if (wall) {
+    A = Gk[0] * dgrad[0];
+    B = Gk[0] * Gk[0];
+#pragma unroll
+    for (int j = 1; j < D; ++j) {
+      A = A + Gk[j] * dgrad[j];
+      B = B + Gk[j] * Gk[j];
+    }
+  } else {
+    A = Gk[0] * Gw_rec[0];
+#pragma unroll
+    for (int j = 1; j < D; ++j) A = A + Gk[j] * Gw_rec[j];
+    B = sconst<S>(0.0);
+  }
+}
+
+// Closure fields of one cell (transport.py:369-546): strain rate (with the wall log-layer blend),
+// F1 from the (wall-imposed) omega gradient, eddy viscosity with the F2 limiter, the cross-diffusion
+// value 2 (1 - F1) sigma_w2 grad k . grad w / w, and the wall-function eddy viscosity of its wall faces.
+template <int D, class S>
+AFX_DEV SstClosure<S> sst_closure(const SstConst& c, S k, S w, const S (&Gu)[D][D], const S (&Gk)[D],
+                                  const S (&Gw_rec)[D], S nu, double d, const double (&dgrad)[D], bool wall) {
+  S A, B;
+  closure_products<D, S>(Gk, Gw_rec, dgrad, wall, A, B);
+  return sst_closure_core<S>(c, k, w, strain_magnitude<D, S>(Gu), A, B, nu, d, wall);
 }
 
 // Volume-integrated sources of one cell: out_k = KProduction + KDestruction, out_w = OmegaProduction
@@ -175,6 +206,62 @@ AFX_DEV void sst_sources(const SstConst& c, S k, S w, S nut, S strain, S f1, S c
   S beta = sst_blend(f1, c.beta1, c.beta2);
   dest_w = -beta * (w * w) * V;
   cross_w = cd * V;
+}
+
+// d min(a, b) / d a with the JAX tie convention (half on a tie).
+AFX_DEV double dmin_da(double a, double b) { return a < b ? 1.0 : (a > b ? 0.0 : 0.5); }
+
+// Hand-written adjoint of sst_sources away from walls, for cotangents lk of (prod_k + dest_k) and
+// lw of (prod_w + dest_w + cross_w); accumulates into the k, omega, nu_t, strain, F1 and
+// cross-diffusion cotangents.
+AFX_DEV void adj_sst_sources_open(const SstConst& c, double k, double w, double nut, double strain, double f1,
+                                  double V, double lk, double lw, double& k_b, double& w_b, double& nut_b,
+                                  double& strain_b, double& f1_b, double& cd_b) {
+  const double pk = k > 0.0 ? k : 0.0;
+  const double dpk = k > 0.0 ? 1.0 : (k < 0.0 ? 0.0 : 0.5);
+  const double s2 = strain * strain;
+  const double r = kProductionLimitRatio * c.beta_star;
+  double pk_b = 0.0;
+  // prod_k = min(nut S^2, r k+ w) V
+  {
+    const double a = nut * s2, b = r * pk * w;
+    const double lb = lk * V;
+    const double sa = dmin_da(a, b);
+    const double ab = lb * sa, bb = lb * (1.0 - sa);
+    nut_b += ab * s2;
+    strain_b += ab * 2.0 * nut * strain;
+    pk_b += bb * r * w;
+    w_b += bb * r * pk;
+  }
+  // dest_k = -beta* k w V
+  k_b += lk * (-c.beta_star * w * V);
+  w_b += lk * (-c.beta_star * k * V);
+  // prod_w = alpha(F1) min(S^2, cap) V,  cap = r k+ w / max(nut, floor)
+  {
+    const double alpha = sst_blend(f1, c.alpha1, c.alpha2);
+    const double den = nut > kEddyViscosityFloor ? nut : kEddyViscosityFloor;
+    const double cap = r * pk * w / den;
+    const double m2 = s2 < cap ? s2 : cap;
+    f1_b += lw * m2 * V * (c.alpha1 - c.alpha2);
+    const double m2b = lw * alpha * V;
+    const double ss = dmin_da(s2, cap);
+    strain_b += m2b * ss * 2.0 * strain;
+    const double capb = m2b * (1.0 - ss);
+    const double numb = capb / den;
+    const double denb = -capb * cap / den;
+    nut_b += denb * (nut > kEddyViscosityFloor ? 1.0 : (nut < kEddyViscosityFloor ? 0.0 : 0.5));
+    pk_b += numb * r * w;
+    w_b += numb * r * pk;
+  }
+  // dest_w = -beta(F1) w^2 V
+  {
+    const double beta = sst_blend(f1, c.beta1, c.beta2);
+    f1_b += lw * (-(w * w) * V) * (c.beta1 - c.beta2);
+    w_b += lw * (-beta * 2.0 * w * V);
+  }
+  // cross_w = cd V
+  cd_b += lw * V;
+  k_b += pk_b * dpk;
 }
 
 }  // namespace afx

@@ -298,6 +298,22 @@ class _RansRaw:
         )
         return bars
 
+    def residual_vjp(self, primals, cot):
+        """(R, cotangents) with the forward passes shared between the two."""
+        R = torch.empty_like(primals[0])
+        bars = [torch.empty_like(p) for p in primals]
+        w = self.dm.workspace("rans2", self.work_bytes[2])
+        ins = self.inputs(*primals)
+        outs = _lib.AfxRansCotangents(*[b.data_ptr() for b in bars])
+        _lib.check(
+            _lib.lib().afx_rans_residual_vjp(
+                self.dm.handle, ctypes.byref(self.gs), ctypes.byref(self.cfg), ctypes.byref(ins), cot.data_ptr(),
+                R.data_ptr(), ctypes.byref(outs), w.data_ptr(), w.numel(), _lib.stream_of(self.dm.device),
+            ),
+            "afx_rans_residual_vjp",
+        )
+        return R, bars
+
     def fields(self, state, mu, rho, nu, ub, pb):
         d, C = self.mesh.dim, self.mesh.n_cells
         n = ((d + 3) * d + (d + 2) + 1 + 7) * C

@@ -278,6 +278,12 @@ int afx_rans_vjp(const afx_mesh* mesh, const afx_geometry* geom, const afx_rans_
                  const afx_rans_inputs* in, const double* cotangent, afx_rans_cotangents* out, void* work,
                  int64_t work_bytes, hipStream_t stream);
 
+/* R and J^T . cotangent in one call, the forward passes shared (the pair jax.vjp returns: the
+ * primal output and its pullback applied to `cotangent`).  Workspace as for mode 2. */
+int afx_rans_residual_vjp(const afx_mesh* mesh, const afx_geometry* geom, const afx_rans_config* cfg,
+                          const afx_rans_inputs* in, const double* cotangent, double* residual,
+                          afx_rans_cotangents* out, void* work, int64_t work_bytes, hipStream_t stream);
+
 /* Forward intermediates into `fields` (the mode-0 workspace layout): gradients [(d+3)d][C],
  * limiters [d+2][C], a_P [C], closure [7][C] = nu_t, F1, strain rate, cross-diffusion,
  * wall-function nu_t, wall shear rate, mu_eff.  Replaces SSTTurbulence.closure_fields (reference:

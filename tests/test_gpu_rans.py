@@ -152,6 +152,22 @@ def test_rans_vjp_matches_oracle(case, cuda):
         assert err < TOL_D, name
 
 
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_rans_fused_residual_vjp_matches_separate_calls(case, cuda):
+    """afx_rans_residual_vjp returns the cotangents of afx_rans_vjp bitwise and the residual of
+    afx_rans_residual to rounding (its k / omega rows come from the adjoint kernel's face loop)."""
+    mesh, rans, oset, st = _setup(case, cuda, seed=8)
+    rng = np.random.default_rng(12)
+    lam = torch.tensor(rng.normal(size=st.numel()), device=cuda)
+    prim = rans._detached(st.to(cuda))
+    R0 = rans._raw.residual(*prim)
+    b0 = rans._raw.vjp(prim, lam)
+    R1, b1 = rans._raw.residual_vjp(prim, lam)
+    assert max(RC.block_errors(R1, R0, mesh.dim + 3)) < 1e-14
+    for x, y in zip(b0, b1):
+        assert torch.equal(x, y)
+
+
 @pytest.mark.parametrize("case", CASES[:2], ids=_ids)
 def test_rans_adjoint_dot_product(case, cuda):
     """<lambda, J v> = <J^T lambda, v> on the GPU alone, to rounding."""
