@@ -40,6 +40,9 @@ class AfxFlowConfig(ctypes.Structure):
         ("body_force", ctypes.c_double * 3),
         ("limiter_k", ctypes.c_double),
         ("face_kind", c_void_p),
+        ("gradient_sweeps", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("gradient_relaxation", ctypes.c_double),
     ]
 
 
@@ -121,6 +124,9 @@ class AfxScalarConfig(ctypes.Structure):
         ("gradient", ctypes.c_int32),
         ("limiter_k", ctypes.c_double),
         ("bc_kind", c_void_p),
+        ("gradient_sweeps", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("gradient_relaxation", ctypes.c_double),
     ]
 
 
@@ -186,7 +192,8 @@ _SIGNATURES = {
         c_void_p, c_void_p, c_void_p, c_void_p,
     ],
     "afx_scalar_gradient": [
-        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.c_int, ctypes.c_int, ctypes.c_double, c_void_p, c_void_p,
+        c_void_p, c_void_p, c_void_p,
     ],
     "afx_geometry_vjp_workspace_bytes": [c_void_p, c_int64_p],
     "afx_geometry_vjp": [

@@ -14,6 +14,7 @@
 #include "afx_mesh.h"
 #include "afx_flow_face.h"
 #include "afx_profile.h"
+#include "afx_cgg.h"
 
 using namespace afx;
 
@@ -543,7 +544,12 @@ __global__ void __launch_bounds__(256) k_flow_ap_adj(MeshView m, FlowArgs<double
     for (int j = 0; j < D; ++j) gb[i][j] = bar.G[(int64_t)(i * D + j) * m.C + c] + B.Gu[i][j];
 #pragma unroll
   for (int j = 0; j < D; ++j) gb[D][j] = bar.G[(int64_t)(D * D + j) * m.C + c] + B.Gp[j];
-  if (GRAD == 0) {
+  if (GRAD == kGradCorrected) {  // the swept solve's reverse pass forms the cotangent of b
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int j = 0; j < D; ++j) bar.G[(int64_t)(v * D + j) * m.C + c] = gb[v][j];
+  } else if (GRAD == 0) {
     const double iv = 1.0 / X.V;
 #pragma unroll
     for (int v = 0; v < NV; ++v)
@@ -674,6 +680,7 @@ struct Work {
   double *Gt, *aPt, *psit;
   double *Gb, *aPb, *psib, *extb;
   int32_t* ext;
+  double* cgg;  // corrected Green-Gauss sweep scratch [2 (D+1) D][C]
 };
 
 int64_t work_bytes(int D, int64_t C, int mode) {
@@ -681,6 +688,7 @@ int64_t work_bytes(int D, int64_t C, int mode) {
   int64_t n = (int64_t)(NV * D + 1 + D) * C;  // G, aP, psi
   if (mode == 1) n *= 2;
   if (mode == 2) n += (int64_t)(NV * D + 1 + D + 2 * D) * C + (int64_t)D * C;  // bars + ext (2D int32)
+  n += (int64_t)2 * NV * D * C;  // corrected Green-Gauss sweep scratch
   return n * 8 + 256;
 }
 
@@ -712,7 +720,9 @@ Work carve(void* base, int D, int64_t C, int mode) {
     w.extb = p;
     p += (int64_t)2 * D * C;
     w.ext = (int32_t*)p;
+    p += (int64_t)D * C;
   }
+  w.cgg = p;
   return w;
 }
 
@@ -743,16 +753,35 @@ FlowArgs<S> make_args(const afx_flow_config* cfg, const afx_flow_inputs* in, con
 
 template <int D, class S, int ADV, int GRAD>
 int run_forward(const afx_mesh* mh, const afx_geometry* g, FlowArgs<S> a, Out<S> R, double* G, double* Gt,
-                double* aP, double* aPt, double* psi, double* psit, int32_t* ext, hipStream_t s) {
+                double* aP, double* aPt, double* psi, double* psit, int32_t* ext, double* cgg, int sweeps,
+                double relax, hipStream_t s) {
   MeshView m = afx_view(mh, g);
   Out<S> oG{G, Gt}, oP{psi, psit}, oA{aP, aPt};
-  if (ADV == kAdvLimited)
-    AFX_LAUNCH(kKFlowGrad, (k_flow_grad<D, S, GRAD, true>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oG, oP, ext);
+  constexpr int GK = GRAD == kGradCorrected ? 0 : GRAD;  // the compact pass supplies V^-1 b
+  if (ADV == kAdvLimited && GRAD != kGradCorrected)
+    AFX_LAUNCH(kKFlowGrad, (k_flow_grad<D, S, GK, true>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oG, oP, ext);
   else
-    AFX_LAUNCH(kKFlowGrad, (k_flow_grad<D, S, GRAD, false>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oG, oP, ext);
+    AFX_LAUNCH(kKFlowGrad, (k_flow_grad<D, S, GK, false>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oG, oP, ext);
   AFX_CHECK_LAUNCH();
   a.G.v = G;
   a.G.t = Gt;
+  if (GRAD == kGradCorrected) {
+    AFX_REQUIRE(cgg, "flow: corrected Green-Gauss needs sweep scratch");
+    int rc = cgg_sweeps(m, D, D + 1, G, cgg, sweeps, relax, s);
+    if (rc) return rc;
+    if (Gt) {
+      rc = cgg_sweeps(m, D, D + 1, Gt, cgg, sweeps, relax, s);
+      if (rc) return rc;
+    }
+    if (ADV == kAdvLimited) {
+      LimiterSlots sl{};
+      sl.nl = D;
+      sl.ext_stride = D;
+      for (int i = 0; i < D; ++i) sl.var[i] = sl.slot[i] = i;
+      rc = venkat_pass(m, D, sl, a.x, a.G, a.lim_k, oP, ext, s);
+      if (rc) return rc;
+    }
+  }
   AFX_LAUNCH(kKFlowAp, (k_flow_ap<D, S, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oA);
   AFX_CHECK_LAUNCH();
   if (R.v == nullptr) return 0;
@@ -767,10 +796,10 @@ int run_forward(const afx_mesh* mh, const afx_geometry* g, FlowArgs<S> a, Out<S>
 
 template <int D, int ADV, int GRAD>
 int run_vjp(const afx_mesh* mh, const afx_geometry* g, FlowArgs<double> a, const double* lam, FlowBars bar,
-            const Work& w, hipStream_t s) {
+            const Work& w, int sweeps, double relax, hipStream_t s) {
   MeshView m = afx_view(mh, g);
   int rc = run_forward<D, double, ADV, GRAD>(mh, g, a, Out<double>{nullptr, nullptr}, w.G, nullptr, w.aP, nullptr,
-                                             w.psi, nullptr, w.ext, s);
+                                             w.psi, nullptr, w.ext, w.cgg, sweeps, relax, s);
   if (rc) return rc;
   a.G.v = w.G;
   a.aP.v = w.aP;
@@ -779,10 +808,15 @@ int run_vjp(const afx_mesh* mh, const afx_geometry* g, FlowArgs<double> a, const
   AFX_CHECK_LAUNCH();
   AFX_LAUNCH(kKFlowApAdj, (k_flow_ap_adj<D, ADV, GRAD>), grid_for(m.C), dim3(kBlock), 0, s, m, a, w.ext, bar);
   AFX_CHECK_LAUNCH();
+  constexpr int GK = GRAD == kGradCorrected ? 0 : GRAD;
+  if (GRAD == kGradCorrected) {
+    rc = cgg_sweeps_adjoint(m, D, D + 1, bar.G, w.cgg, sweeps, relax, s);
+    if (rc) return rc;
+  }
   if (ADV == kAdvLimited)
-    AFX_LAUNCH(kKFlowGradAdj, (k_flow_grad_adj<D, GRAD, true>), grid_for(m.C), dim3(kBlock), 0, s, m, a, w.ext, bar);
+    AFX_LAUNCH(kKFlowGradAdj, (k_flow_grad_adj<D, GK, true>), grid_for(m.C), dim3(kBlock), 0, s, m, a, w.ext, bar);
   else
-    AFX_LAUNCH(kKFlowGradAdj, (k_flow_grad_adj<D, GRAD, false>), grid_for(m.C), dim3(kBlock), 0, s, m, a, w.ext, bar);
+    AFX_LAUNCH(kKFlowGradAdj, (k_flow_grad_adj<D, GK, false>), grid_for(m.C), dim3(kBlock), 0, s, m, a, w.ext, bar);
   AFX_CHECK_LAUNCH();
   return 0;
 }
@@ -794,6 +828,7 @@ int dispatch(int D, int adv, int grad, Fn&& fn) {
   if (D == DD && adv == AA && grad == GG) return fn(std::integral_constant<int, DD>{}, std::integral_constant<int, AA>{}, std::integral_constant<int, GG>{});
   AFX_CASE(2, 0, 0) AFX_CASE(2, 1, 0) AFX_CASE(2, 2, 0) AFX_CASE(2, 0, 1) AFX_CASE(2, 1, 1) AFX_CASE(2, 2, 1)
   AFX_CASE(3, 0, 0) AFX_CASE(3, 1, 0) AFX_CASE(3, 2, 0) AFX_CASE(3, 0, 1) AFX_CASE(3, 1, 1) AFX_CASE(3, 2, 1)
+  AFX_CASE(2, 0, 3) AFX_CASE(2, 1, 3) AFX_CASE(2, 2, 3) AFX_CASE(3, 0, 3) AFX_CASE(3, 1, 3) AFX_CASE(3, 2, 3)
 #undef AFX_CASE
   afx_set_error("unsupported (dim, advection, gradient) combination");
   return 2;
@@ -805,7 +840,8 @@ int check_common(const afx_mesh* m, const afx_geometry* g, const afx_flow_config
   AFX_REQUIRE(in->state && in->mu && in->rho && in->u_b && in->p_b && cfg->face_kind,
               "flow: a required input pointer is null");
   AFX_REQUIRE(cfg->advection >= 0 && cfg->advection <= 2, "flow: advection must be 0, 1 or 2");
-  AFX_REQUIRE(cfg->gradient == 0 || cfg->gradient == 1, "flow: gradient must be 0 or 1");
+  AFX_REQUIRE(cfg->gradient == 0 || cfg->gradient == 1 || cfg->gradient == 3, "flow: gradient must be 0, 1 or 3");
+  AFX_REQUIRE(cfg->gradient != 3 || cfg->gradient_relaxation > 0.0, "flow: gradient relaxation must be positive");
   AFX_REQUIRE(cfg->pressure_pin < (int64_t)m->C, "flow: pressure_pin out of range");
   AFX_REQUIRE(work != nullptr && work_size >= work_bytes(m->dim, m->C, mode), "flow: workspace too small");
   return 0;
@@ -830,7 +866,8 @@ extern "C" int afx_flow_residual(const afx_mesh* m, const afx_geometry* g, const
   FlowArgs<double> a = make_args<double>(cfg, in, nullptr);
   return dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
     return run_forward<decltype(Dc)::value, double, decltype(Ac)::value, decltype(Gc)::value>(
-        m, g, a, Out<double>{residual, nullptr}, w.G, nullptr, w.aP, nullptr, w.psi, nullptr, nullptr, s);
+        m, g, a, Out<double>{residual, nullptr}, w.G, nullptr, w.aP, nullptr, w.psi, nullptr, nullptr, w.cgg,
+        cfg->gradient_sweeps, cfg->gradient_relaxation, s);
   });
 }
 
@@ -844,7 +881,8 @@ extern "C" int afx_flow_jvp(const afx_mesh* m, const afx_geometry* g, const afx_
   FlowArgs<dual> a = make_args<dual>(cfg, in, tangent);
   return dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
     return run_forward<decltype(Dc)::value, dual, decltype(Ac)::value, decltype(Gc)::value>(
-        m, g, a, Out<dual>{residual, residual_tangent}, w.G, w.Gt, w.aP, w.aPt, w.psi, w.psit, nullptr, s);
+        m, g, a, Out<dual>{residual, residual_tangent}, w.G, w.Gt, w.aP, w.aPt, w.psi, w.psit, nullptr, w.cgg,
+        cfg->gradient_sweeps, cfg->gradient_relaxation, s);
   });
 }
 
@@ -861,7 +899,8 @@ extern "C" int afx_flow_vjp(const afx_mesh* m, const afx_geometry* g, const afx_
   FlowArgs<double> a = make_args<double>(cfg, in, nullptr);
   FlowBars bar{out->state, w.Gb, w.aPb, w.psib, out->mu, out->rho, out->u_b, out->p_b, out->gamma_b, w.extb};
   return dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
-    return run_vjp<decltype(Dc)::value, decltype(Ac)::value, decltype(Gc)::value>(m, g, a, cotangent, bar, w, s);
+    return run_vjp<decltype(Dc)::value, decltype(Ac)::value, decltype(Gc)::value>(
+        m, g, a, cotangent, bar, w, cfg->gradient_sweeps, cfg->gradient_relaxation, s);
   });
 }
 
@@ -871,8 +910,14 @@ extern "C" int afx_flow_fields(const afx_mesh* m, const afx_geometry* g, const a
   AFX_REQUIRE(m && g && cfg && in && gradients && a_p, "afx_flow_fields: null argument");
   AFX_REQUIRE(cfg->advection != 2 || psi, "afx_flow_fields: limited advection needs a psi buffer");
   FlowArgs<double> a = make_args<double>(cfg, in, nullptr);
+  double* cgg = nullptr;
+  if (cfg->gradient == kGradCorrected) {
+    cgg = afx_mesh_scratch(const_cast<afx_mesh*>(m), (int64_t)2 * (m->dim + 1) * m->dim * m->C);
+    AFX_REQUIRE(cgg, "afx_flow_fields: could not allocate the corrected-gradient sweep scratch");
+  }
   return dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
     return run_forward<decltype(Dc)::value, double, decltype(Ac)::value, decltype(Gc)::value>(
-        m, g, a, Out<double>{nullptr, nullptr}, gradients, nullptr, a_p, nullptr, psi, nullptr, nullptr, s);
+        m, g, a, Out<double>{nullptr, nullptr}, gradients, nullptr, a_p, nullptr, psi, nullptr, nullptr, cgg,
+        cfg->gradient_sweeps, cfg->gradient_relaxation, s);
   });
 }

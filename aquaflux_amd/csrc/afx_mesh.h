@@ -19,6 +19,12 @@ struct afx_mesh {
   int32_t* ninc = nullptr;      // [n_inc]
   double* approx = nullptr;     // [d][C] scratch: approximate cell centroids
   double* normal_raw = nullptr; // [d][F] scratch: winding-order normals
+  double* scratch = nullptr;    // lazily grown scratch of entry points without a workspace argument
+  int64_t scratch_n = 0;
 };
+
+// Device scratch of at least n doubles owned by the handle (grown on demand, freed with it);
+// nullptr on allocation failure.  Not for concurrent use from several host threads.
+double* afx_mesh_scratch(afx_mesh* m, int64_t n);
 
 afx::MeshView afx_view(const afx_mesh* m, const afx_geometry* g);

@@ -109,8 +109,8 @@ extern "C" int afx_mesh_create(int dim, int64_t n_cells, int64_t n_faces, int64_
 
 extern "C" int afx_mesh_destroy(afx_mesh* m) {
   if (!m) return 0;
-  void* ptrs[] = {m->owner, m->nb,  m->cptr,   m->cface,      m->foff,   m->fidx,
-                  m->inc_face, m->nptr, m->ninc, m->approx, m->normal_raw};
+  void* ptrs[] = {m->owner, m->nb,  m->cptr,   m->cface,      m->foff,       m->fidx,
+                  m->inc_face, m->nptr, m->ninc, m->approx, m->normal_raw, m->scratch};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete m;
@@ -530,6 +530,21 @@ __global__ void k_geom_node_adj(int32_t Nn, const int32_t* __restrict__ nptr, co
 }
 
 }  // namespace
+
+double* afx_mesh_scratch(afx_mesh* m, int64_t n) {
+  if (m->scratch_n >= n) return m->scratch;
+  if (m->scratch) {
+    if (hipFree(m->scratch) != hipSuccess) return nullptr;
+    m->scratch = nullptr;
+    m->scratch_n = 0;
+  }
+  if (hipMalloc((void**)&m->scratch, n * 8) != hipSuccess) {
+    m->scratch = nullptr;
+    return nullptr;
+  }
+  m->scratch_n = n;
+  return m->scratch;
+}
 
 afx::MeshView afx_view(const afx_mesh* m, const afx_geometry* g) {
   afx::MeshView v;

@@ -41,6 +41,9 @@ enum KernelId : int {
   kKGeomAdjNode,
   kKRansScalarRes,
   kKRansFlowResAdj,
+  kKLimiter,
+  kKCggSweep,
+  kKCggSweepAdj,
   kKNumKernels
 };
 

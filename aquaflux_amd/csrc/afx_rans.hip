@@ -21,6 +21,7 @@
 #include "afx_scalar_face.h"
 #include "afx_sst.h"
 #include "afx_profile.h"
+#include "afx_cgg.h"
 
 using namespace afx;
 
@@ -42,6 +43,8 @@ struct RansArgs {
   double bf[3];
   double lim_k;
   SstConst sc;
+  int sweeps;    // corrected Green-Gauss sweep count
+  double relax;  // ... and relaxation
 };
 
 // closure arrays (written by pass C)
@@ -1030,8 +1033,14 @@ __global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<d
   bar.nu[c] = nu_b;
   bar.mu[c] = mu_b;
   bar.rho[c] = rho_b;
-  // pre-multiply the gradient cotangent by the reconstruction operator
-  if (GRAD == 0) {
+  // pre-multiply the gradient cotangent by the reconstruction operator (the corrected scheme's
+  // swept-solve reverse pass does it separately)
+  if (GRAD == kGradCorrected) {
+#pragma unroll
+    for (int v = 0; v < NA; ++v)
+#pragma unroll
+      for (int j = 0; j < D; ++j) bar.G[(int64_t)(v * D + j) * m.C + c] = Gb[v][j];
+  } else if (GRAD == 0) {
     const double iv = 1.0 / m.vol[c];
 #pragma unroll
     for (int v = 0; v < NA; ++v)
@@ -1158,9 +1167,10 @@ __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<doub
 int64_t rans_work_doubles(int D, int64_t C, int64_t F, int mode) {
   const int NA = D + 3, NL = D + 2;
   int64_t fwd = (int64_t)(NA * D + NL + 7 + 1) * C + F;  // G, psi, closure(7), aP, face mass flux
-  if (mode == 1) return 2 * fwd;
-  if (mode == 2) return fwd + (int64_t)(NA * D + 1 + NL + 2 * NL + 7) * C + (int64_t)NL * C + F;  // bars, ext, mdot_bar
-  return fwd;
+  const int64_t cgg = (int64_t)2 * NA * D * C;            // corrected Green-Gauss sweep scratch
+  if (mode == 1) return 2 * fwd + cgg;
+  if (mode == 2) return fwd + (int64_t)(NA * D + 1 + NL + 2 * NL + 7) * C + (int64_t)NL * C + F + cgg;
+  return fwd + cgg;
 }
 
 struct RWork {
@@ -1168,6 +1178,7 @@ struct RWork {
   double *Gt, *psit, *aPt, *clt, *mdott;  // tangents
   double *Gb, *aPb, *psib, *extb, *mueffb, *nutb, *f1b, *Srb, *cdb, *taub, *nutwb, *mdotb;
   int32_t* ext;
+  double* cgg;
 };
 
 RWork rcarve(void* base, int D, int64_t C, int64_t F, int mode) {
@@ -1206,6 +1217,7 @@ RWork rcarve(void* base, int D, int64_t C, int64_t F, int mode) {
     w.ext = (int32_t*)take((int64_t)NL * C);
     w.mdotb = take(F);
   }
+  w.cgg = take((int64_t)2 * NA * D * C);
   return w;
 }
 
@@ -1265,6 +1277,8 @@ RansArgs<S> rargs(const afx_rans_config* cfg, const afx_rans_inputs* in, const a
   for (int i = 0; i < 3; ++i) a.bf[i] = cfg->flow.body_force[i];
   a.lim_k = cfg->flow.limiter_k;
   a.sc = make_sst(cfg);
+  a.sweeps = cfg->flow.gradient_sweeps;
+  a.relax = cfg->flow.gradient_relaxation;
   return a;
 }
 
@@ -1285,11 +1299,39 @@ int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S
   MeshView m = afx_view(mh, g);
   const int64_t C = m.C;
   Out<S> oG{w.G, Gt}, oP{w.psi, psit};
+  constexpr bool CG = GRAD == kGradCorrected;
+  constexpr int GK = CG ? 0 : GRAD;
   constexpr bool LF = ADVF == kAdvLimited, LS = ADVS == kAdvLimited;
-  AFX_LAUNCH(kKRansGrad, (k_rans_grad<D, S, GRAD, LF, LS>), grid_for(C), dim3(kBlock), 0, s, m, a, oG, oP, ext);
+  AFX_LAUNCH(kKRansGrad, (k_rans_grad<D, S, GK, LF && !CG, LS && !CG>), grid_for(C), dim3(kBlock), 0, s, m, a, oG,
+             oP, ext);
   AFX_CHECK_LAUNCH();
   a.G.v = w.G;
   a.G.t = Gt;
+  if (CG) {
+    const int sw = a.sweeps;
+    const double rl = a.relax;
+    int rc = cgg_sweeps(m, D, D + 3, w.G, w.cgg, sw, rl, s);
+    if (rc) return rc;
+    if (Gt) {
+      rc = cgg_sweeps(m, D, D + 3, Gt, w.cgg, sw, rl, s);
+      if (rc) return rc;
+    }
+    LimiterSlots sl{};
+    sl.ext_stride = D + 2;
+    if (LF)
+      for (int i = 0; i < D; ++i) {
+        sl.var[sl.nl] = i;
+        sl.slot[sl.nl++] = i;
+      }
+    if (LS) {
+      sl.var[sl.nl] = D + 1;
+      sl.slot[sl.nl++] = D;
+      sl.var[sl.nl] = D + 2;
+      sl.slot[sl.nl++] = D + 1;
+    }
+    rc = venkat_pass(m, D, sl, a.x, a.G, a.lim_k, oP, ext, s);
+    if (rc) return rc;
+  }
   a.psi.v = w.psi;
   a.psi.t = psit;
   ClosureOut co{};
@@ -1326,6 +1368,7 @@ int rdispatch(int D, int advf, int advs, int grad, Fn&& fn) {
               std::integral_constant<int, AS>{}, std::integral_constant<int, GG>{});
   AFX_R(2, 1, 1, 0) AFX_R(2, 2, 2, 0) AFX_R(2, 1, 1, 1) AFX_R(2, 2, 2, 1) AFX_R(2, 1, 2, 1) AFX_R(2, 2, 1, 1)
   AFX_R(3, 1, 1, 0) AFX_R(3, 2, 2, 0) AFX_R(3, 1, 1, 1) AFX_R(3, 2, 2, 1) AFX_R(3, 1, 2, 1) AFX_R(3, 2, 1, 1)
+  AFX_R(2, 1, 1, 3) AFX_R(2, 2, 2, 3) AFX_R(3, 1, 1, 3) AFX_R(3, 2, 2, 3)
 #undef AFX_R
   afx_set_error("rans: unsupported (dim, advection, scalar advection, gradient) combination");
   return 2;
@@ -1340,6 +1383,8 @@ int rcheck(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg,
                   cfg->wall_distance && cfg->wall_distance_gradient,
               "rans: a required configuration pointer is null");
   AFX_REQUIRE(cfg->density > 0.0, "rans: density must be positive");
+  AFX_REQUIRE(cfg->flow.gradient != kGradCorrected || cfg->flow.gradient_relaxation > 0.0,
+              "rans: gradient relaxation must be positive");
   AFX_REQUIRE(work && work_size >= rans_work_doubles(m->dim, m->C, m->F, mode) * 8, "rans: workspace too small");
   return 0;
 }
@@ -1425,7 +1470,12 @@ int rans_vjp_impl(const afx_mesh* m, const afx_geometry* g, const afx_rans_confi
                      AFX_LAUNCH(kKSstClosureAdj, (k_rans_closure_adj<D, AF, AS, GR>), grid_for(m->C), dim3(kBlock), 0,
                                 s, v, b, w.ext, bar);
                      AFX_CHECK_LAUNCH();
-                     AFX_LAUNCH(kKRansGradAdj, (k_rans_grad_adj<D, GR, LF, LS>), grid_for(m->C), dim3(kBlock), 0, s,
+                     constexpr int GK = GR == kGradCorrected ? 0 : GR;
+                     if (GR == kGradCorrected) {
+                       int r3 = cgg_sweeps_adjoint(v, D, D + 3, bar.G, w.cgg, a.sweeps, a.relax, s);
+                       if (r3) return r3;
+                     }
+                     AFX_LAUNCH(kKRansGradAdj, (k_rans_grad_adj<D, GK, LF, LS>), grid_for(m->C), dim3(kBlock), 0, s,
                                 v, b, w.ext, bar);
                      AFX_CHECK_LAUNCH();
                      return 0;

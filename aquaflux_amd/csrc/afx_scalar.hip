@@ -22,6 +22,7 @@
 #include "afx_flow_face.h"
 #include "afx_scalar_face.h"
 #include "afx_profile.h"
+#include "afx_cgg.h"
 
 using namespace afx;
 
@@ -655,7 +656,10 @@ __global__ void __launch_bounds__(256) k_sc_lim_adj(MeshView m, ScArgs<double> a
     }
   }
   bar.phi[c] += xb;
-  if (GRAD == 0) {
+  if (GRAD == kGradCorrected) {  // the swept solve's reverse pass forms the cotangent of b
+#pragma unroll
+    for (int j = 0; j < D; ++j) bar.G[(int64_t)j * m.C + c] = Gb[j];
+  } else if (GRAD == 0) {
     const double iv = 1.0 / m.vol[c];
     if (GEO) {
       double gg = 0.0;
@@ -916,7 +920,7 @@ __global__ void __launch_bounds__(256) k_scalar_grad(MeshView m, const int8_t* _
 // Workspace in doubles: mode 0 = G [D][C], psi [C], ext (2 int32 per cell); mode 1 adds tangents of
 // G and psi; mode 2 adds the adjoint scratch (G, psi, ext, limiter face cotangent, argmin face).
 int64_t sc_work_doubles(int D, int64_t C, int mode) {
-  int64_t n = (int64_t)(D + 1 + 1) * C;
+  int64_t n = (int64_t)(D + 1 + 1) * C + (int64_t)2 * D * C;  // ... + corrected Green-Gauss sweep scratch
   if (mode == 1) n += (int64_t)(D + 1) * C;
   if (mode == 2) n += (int64_t)(D + 1 + 2 + D + 1) * C;
   return n;
@@ -927,6 +931,7 @@ struct ScWork {
   int32_t* ext;
   double *Gb, *psib, *extb, *limx;
   int32_t* limf;
+  double* cgg;
 };
 
 ScWork sc_carve(void* base, int D, int64_t C, int mode) {
@@ -956,6 +961,7 @@ ScWork sc_carve(void* base, int D, int64_t C, int mode) {
     w.limf = (int32_t*)p;
     p += C;
   }
+  w.cgg = p;
   return w;
 }
 
@@ -989,6 +995,7 @@ int sc_dispatch(int D, int adv, int grad, Fn&& fn) {
   AFX_S(2, 0, 2) AFX_S(2, 1, 2) AFX_S(2, 2, 2)
   AFX_S(3, 0, 0) AFX_S(3, 1, 0) AFX_S(3, 2, 0) AFX_S(3, 0, 1) AFX_S(3, 1, 1) AFX_S(3, 2, 1)
   AFX_S(3, 0, 2) AFX_S(3, 1, 2) AFX_S(3, 2, 2)
+  AFX_S(2, 0, 3) AFX_S(2, 1, 3) AFX_S(2, 2, 3) AFX_S(3, 0, 3) AFX_S(3, 1, 3) AFX_S(3, 2, 3)
 #undef AFX_S
   afx_set_error("scalar: unsupported (dim, advection, gradient) combination");
   return 2;
@@ -996,15 +1003,31 @@ int sc_dispatch(int D, int adv, int grad, Fn&& fn) {
 
 template <int D, class S, int ADV, int GRAD>
 int sc_forward(const afx_mesh* mh, const afx_geometry* g, ScArgs<S> a, Out<S> R, const ScWork& w, double* Gt,
-               double* psit, int32_t* ext, hipStream_t s) {
+               double* psit, int32_t* ext, int sweeps, double relax, hipStream_t s) {
   MeshView m = afx_view(mh, g);
   constexpr bool LIM = ADV == kAdvLimited;
-  AFX_LAUNCH(kKScalarGrad, (k_sc_grad<D, S, GRAD, LIM>), grid_for(m.C), dim3(kBlock), 0, s, m, a,
-             Out<S>{w.G, Gt}, Out<S>{w.psi, psit}, ext);
+  constexpr int GK = GRAD == kGradCorrected ? 0 : GRAD;
+  AFX_LAUNCH(kKScalarGrad, (k_sc_grad<D, S, GK, LIM && GRAD != kGradCorrected>), grid_for(m.C), dim3(kBlock), 0, s,
+             m, a, Out<S>{w.G, Gt}, Out<S>{w.psi, psit}, ext);
   AFX_CHECK_LAUNCH();
-  if (R.v == nullptr) return 0;
   a.G.v = w.G;
   a.G.t = Gt;
+  if (GRAD == kGradCorrected) {
+    int rc = cgg_sweeps(m, D, 1, w.G, w.cgg, sweeps, relax, s);
+    if (rc) return rc;
+    if (Gt) {
+      rc = cgg_sweeps(m, D, 1, Gt, w.cgg, sweeps, relax, s);
+      if (rc) return rc;
+    }
+    if (LIM) {
+      LimiterSlots sl{};
+      sl.nl = 1;
+      sl.ext_stride = 1;
+      rc = venkat_pass(m, D, sl, a.phi, a.G, a.lim_k, Out<S>{w.psi, psit}, ext, s);
+      if (rc) return rc;
+    }
+  }
+  if (R.v == nullptr) return 0;
   a.psi.v = w.psi;
   a.psi.t = psit;
   AFX_LAUNCH(kKScalarRes, (k_sc_res<D, S, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, R);
@@ -1018,7 +1041,8 @@ int sc_check(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* 
   AFX_REQUIRE(in->phi && in->gamma && in->flux && in->value && in->t_inf, "scalar: a required input pointer is null");
   AFX_REQUIRE(cfg->bc_kind, "scalar: bc_kind is null");
   AFX_REQUIRE(cfg->advection >= 0 && cfg->advection <= 2, "scalar: unknown advection scheme");
-  AFX_REQUIRE(cfg->gradient >= 0 && cfg->gradient <= 2, "scalar: unknown gradient scheme");
+  AFX_REQUIRE(cfg->gradient >= 0 && cfg->gradient <= 3, "scalar: unknown gradient scheme");
+  AFX_REQUIRE(cfg->gradient != 3 || cfg->gradient_relaxation > 0.0, "scalar: gradient relaxation must be positive");
   AFX_REQUIRE(m->C > 0, "scalar: empty mesh");
   AFX_REQUIRE(work && work_size >= sc_work_doubles(m->dim, m->C, mode) * 8, "scalar: workspace too small");
   return 0;
@@ -1026,18 +1050,24 @@ int sc_check(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* 
 
 }  // namespace
 
-extern "C" int afx_scalar_gradient(const afx_mesh* mh, const afx_geometry* g, int gradient, const int8_t* kind,
-                                   const double* value, const double* phi, double* grad, hipStream_t s) {
+extern "C" int afx_scalar_gradient(const afx_mesh* mh, const afx_geometry* g, int gradient, int sweeps,
+                                   double relaxation, const int8_t* kind, const double* value, const double* phi,
+                                   double* grad, hipStream_t s) {
   AFX_REQUIRE(mh && g && kind && value && phi && grad, "afx_scalar_gradient: null argument");
-  AFX_REQUIRE(gradient == 0 || gradient == 1, "afx_scalar_gradient: unknown gradient scheme");
+  AFX_REQUIRE(gradient == 0 || gradient == 1 || gradient == 3, "afx_scalar_gradient: unknown gradient scheme");
+  AFX_REQUIRE(gradient != 3 || relaxation > 0.0, "afx_scalar_gradient: relaxation must be positive");
   MeshView m = afx_view(mh, g);
   if (m.C == 0) return 0;
+  const int gk = gradient == 3 ? 0 : gradient;
 #define AFX_SG(DD, GG)                                                                                  \
-  if (mh->dim == DD && gradient == GG) {                                                                \
+  if (mh->dim == DD && gk == GG) {                                                                      \
     AFX_LAUNCH(kKScalarGrad, (k_scalar_grad<DD, GG>), grid_for(m.C), dim3(kBlock), 0, s, m, kind, value, \
                phi, grad);                                                                              \
     AFX_CHECK_LAUNCH();                                                                                 \
-    return 0;                                                                                           \
+    if (gradient != 3) return 0;                                                                        \
+    double* scratch = afx_mesh_scratch(const_cast<afx_mesh*>(mh), (int64_t)2 * DD * m.C);              \
+    AFX_REQUIRE(scratch, "afx_scalar_gradient: could not allocate the sweep scratch");                \
+    return cgg_sweeps(m, DD, 1, grad, scratch, sweeps, relaxation, s);                                  \
   }
   AFX_SG(2, 0) AFX_SG(2, 1) AFX_SG(3, 0) AFX_SG(3, 1)
 #undef AFX_SG
@@ -1061,7 +1091,8 @@ extern "C" int afx_scalar_residual(const afx_mesh* m, const afx_geometry* g, con
   ScArgs<double> a = sc_args<double>(cfg, in, nullptr);
   return sc_dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
     return sc_forward<decltype(Dc)::value, double, decltype(Ac)::value, decltype(Gc)::value>(
-        m, g, a, Out<double>{residual, nullptr}, w, nullptr, nullptr, nullptr, s);
+        m, g, a, Out<double>{residual, nullptr}, w, nullptr, nullptr, nullptr, cfg->gradient_sweeps,
+        cfg->gradient_relaxation, s);
   });
 }
 
@@ -1075,7 +1106,8 @@ extern "C" int afx_scalar_jvp(const afx_mesh* m, const afx_geometry* g, const af
   ScArgs<dual> a = sc_args<dual>(cfg, in, tangent);
   return sc_dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
     return sc_forward<decltype(Dc)::value, dual, decltype(Ac)::value, decltype(Gc)::value>(
-        m, g, a, Out<dual>{residual, residual_tangent}, w, w.Gt, w.psit, nullptr, s);
+        m, g, a, Out<dual>{residual, residual_tangent}, w, w.Gt, w.psit, nullptr, cfg->gradient_sweeps,
+        cfg->gradient_relaxation, s);
   });
 }
 
@@ -1090,8 +1122,9 @@ extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const af
   const bool geo = gb.area || gb.normal || gb.face_centroid || gb.volume || gb.cell_centroid;
   AFX_REQUIRE(!geo || (gb.area && gb.normal && gb.face_centroid && gb.volume && gb.cell_centroid),
               "afx_scalar_vjp: geometry cotangents need all five arrays");
-  AFX_REQUIRE(!geo || cfg->gradient != 1,
-              "afx_scalar_vjp: geometry cotangents are available for Green-Gauss or no gradient reconstruction");
+  AFX_REQUIRE(!geo || cfg->gradient == 0 || cfg->gradient == 2,
+              "afx_scalar_vjp: geometry cotangents are available for compact Green-Gauss or no gradient "
+              "reconstruction");
   ScWork w = sc_carve(work, m->dim, m->C, 2);
   ScArgs<double> a = sc_args<double>(cfg, in, nullptr);
   ScBars bar{out->phi, out->gamma, out->flux, out->value, out->t_inf, w.Gb, w.psib, w.extb, w.limx, w.limf,
@@ -1099,12 +1132,14 @@ extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const af
   return sc_dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
     constexpr int D = decltype(Dc)::value, ADV = decltype(Ac)::value, GR = decltype(Gc)::value;
     constexpr bool LIM = ADV == kAdvLimited;
-    int r2 = sc_forward<D, double, ADV, GR>(m, g, a, Out<double>{nullptr, nullptr}, w, nullptr, nullptr, w.ext, s);
+    int r2 = sc_forward<D, double, ADV, GR>(m, g, a, Out<double>{nullptr, nullptr}, w, nullptr, nullptr, w.ext,
+                                            cfg->gradient_sweeps, cfg->gradient_relaxation, s);
     if (r2) return r2;
     MeshView v = afx_view(m, g);
     ScArgs<double> b = a;
     b.G.v = w.G;
     b.psi.v = w.psi;
+    constexpr int GK = GR == kGradCorrected ? 0 : GR;
     auto run = [&](auto Geo) {
       constexpr bool GEO = decltype(Geo)::value;
       AFX_LAUNCH(kKScalarResAdj, (k_sc_res_adj<D, ADV, GEO>), grid_for(m->C), dim3(kBlock), 0, s, v, b, cotangent,
@@ -1113,12 +1148,16 @@ extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const af
       AFX_LAUNCH(kKScalarLimAdj, (k_sc_lim_adj<D, GR, LIM, GEO>), grid_for(m->C), dim3(kBlock), 0, s, v, b, w.ext,
                  bar);
       AFX_CHECK_LAUNCH();
-      AFX_LAUNCH(kKScalarGradAdj, (k_sc_grad_adj<D, GR, LIM, GEO>), grid_for(m->C), dim3(kBlock), 0, s, v, b,
+      if (GR == kGradCorrected) {
+        int r3 = cgg_sweeps_adjoint(v, D, 1, bar.G, w.cgg, cfg->gradient_sweeps, cfg->gradient_relaxation, s);
+        if (r3) return r3;
+      }
+      AFX_LAUNCH(kKScalarGradAdj, (k_sc_grad_adj<D, GK, LIM, GEO>), grid_for(m->C), dim3(kBlock), 0, s, v, b,
                  w.ext, bar);
       AFX_CHECK_LAUNCH();
       return 0;
     };
-    if constexpr (GR == 1) {
+    if constexpr (GR == 1 || GR == kGradCorrected) {
       return run(std::false_type{});
     } else {
       return geo ? run(std::true_type{}) : run(std::false_type{});

@@ -274,8 +274,10 @@ class MomentumContinuity:
         cfg = _lib.AfxFlowConfig()
         cfg.advection = 0 if advection_scheme is None else int(advection_scheme.code)
         cfg.gradient = int(gradient_scheme.code)
-        if cfg.gradient not in (0, 1):
+        if cfg.gradient not in (0, 1, 3):
             raise ValueError(f"unsupported gradient scheme {type(gradient_scheme).__name__}")
+        cfg.gradient_sweeps = int(getattr(gradient_scheme, "sweeps", 4))
+        cfg.gradient_relaxation = float(getattr(gradient_scheme, "relaxation", 1.0))
         cfg.pressure_pin = -1 if pressure_pin is None else int(pressure_pin)
         cfg.pressure_pin_value = float(pressure_pin_value)
         bf = [0.0, 0.0, 0.0]

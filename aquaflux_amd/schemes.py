@@ -20,6 +20,39 @@ class CompactGreenGauss(GradientScheme):
     code = 0
 
 
+@dataclasses.dataclass(frozen=True)
+class SweptGradientSolve:
+    """Fixed preconditioned-Richardson sweeps ``x <- x + r V^-1 (b - A_g x)`` from the peeled
+    ``x0 = r V^-1 b`` (``aquaflux/schemes/gradient.py:546-705``).  ``warn_tol`` is accepted for API
+    compatibility; the GPU path does not emit the under-resolution warning."""
+
+    sweeps: int = 4
+    relaxation: float = 1.0
+    warn_tol: float | None = None
+
+    def __post_init__(self):
+        if self.relaxation <= 0.0:
+            raise ValueError("SweptGradientSolve: relaxation must be positive")
+
+
+@dataclasses.dataclass(frozen=True)
+class CorrectedGreenGauss(GradientScheme):
+    """Green-Gauss with the non-orthogonal skewness correction, ``A_g G = B phi`` solved by swept
+    Richardson sweeps preconditioned by ``1/V`` (``aquaflux/schemes/gradient.py:1122-1337``).  Only
+    :class:`SweptGradientSolve` and the inverse-volume preconditioner are provided on the GPU."""
+
+    solver: SweptGradientSolve = dataclasses.field(default_factory=SweptGradientSolve)
+    code = 3
+
+    @property
+    def sweeps(self) -> int:
+        return int(self.solver.sweeps)
+
+    @property
+    def relaxation(self) -> float:
+        return float(self.solver.relaxation)
+
+
 class LeastSquaresGradient(GradientScheme):
     """Inverse-distance-squared weighted least squares over the face neighbours.
 

@@ -196,8 +196,10 @@ class ScalarTransport:
         cfg = _lib.AfxScalarConfig()
         cfg.advection = 0 if advection_scheme is None else int(advection_scheme.code)
         cfg.gradient = 2 if gradient_scheme is None else int(gradient_scheme.code)
-        if cfg.gradient not in (0, 1, 2):
+        if cfg.gradient not in (0, 1, 2, 3):
             raise ValueError(f"unsupported gradient scheme {type(gradient_scheme).__name__}")
+        cfg.gradient_sweeps = int(getattr(gradient_scheme, "sweeps", 4))
+        cfg.gradient_relaxation = float(getattr(gradient_scheme, "relaxation", 1.0))
         cfg.limiter_k = advection_scheme.limiter_k() if isinstance(advection_scheme, LimitedUpwind) else 5.0
         cfg.bc_kind = kind.data_ptr()
         return cls(mesh=mesh, geometry=geometry, diffusivity=diffusivity, boundary=bound,

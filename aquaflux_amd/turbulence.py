@@ -119,7 +119,8 @@ def scalar_gradient(mesh, geometry, gradient_scheme, phi, kind, value):
     gs = geometry.c_struct()
     _lib.check(
         _lib.lib().afx_scalar_gradient(
-            dm.handle, ctypes.byref(gs), int(gradient_scheme.code), kind.data_ptr(), value.data_ptr(),
+            dm.handle, ctypes.byref(gs), int(gradient_scheme.code), int(getattr(gradient_scheme, "sweeps", 4)),
+            float(getattr(gradient_scheme, "relaxation", 1.0)), kind.data_ptr(), value.data_ptr(),
             phi.contiguous().data_ptr(), G.data_ptr(), _lib.stream_of(dev),
         ),
         "afx_scalar_gradient",

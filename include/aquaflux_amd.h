@@ -100,16 +100,21 @@ int afx_geometry_vjp(const afx_mesh* mesh, const double* node_coords, const afx_
 
 enum { AFX_FLOW_NONE = 0, AFX_FLOW_WALL = 1, AFX_FLOW_INLET = 2, AFX_FLOW_OUTLET = 3 };
 enum { AFX_ADV_NONE = 0, AFX_ADV_UPWIND = 1, AFX_ADV_LIMITED = 2 };
-enum { AFX_GRAD_GREEN_GAUSS = 0, AFX_GRAD_LEAST_SQUARES = 1 };
+/* AFX_GRAD_CORRECTED: CorrectedGreenGauss with SweptGradientSolve (reference:
+ * aquaflux/schemes/gradient.py:618-705, :1122-1337). */
+enum { AFX_GRAD_GREEN_GAUSS = 0, AFX_GRAD_LEAST_SQUARES = 1, AFX_GRAD_CORRECTED = 3 };
 
 typedef struct {
   int32_t advection;          /* AFX_ADV_*: None (Stokes) / FirstOrderUpwind / LimitedUpwind(Venkatakrishnan) */
-  int32_t gradient;           /* AFX_GRAD_*: CompactGreenGauss / least squares */
+  int32_t gradient;           /* AFX_GRAD_*: CompactGreenGauss / least squares / CorrectedGreenGauss */
   int64_t pressure_pin;       /* cell whose continuity row becomes p - value; -1 = none */
   double pressure_pin_value;
   double body_force[3];       /* uniform force per unit volume */
   double limiter_k;           /* Venkatakrishnan K (eps^2 = V K^3) */
   const int8_t* face_kind;    /* [F] AFX_FLOW_* per face (interior faces: AFX_FLOW_NONE) */
+  int32_t gradient_sweeps;    /* AFX_GRAD_CORRECTED: swept-solve sweep count (reference default 4) */
+  int32_t reserved;
+  double gradient_relaxation; /* AFX_GRAD_CORRECTED: Richardson relaxation (reference default 1) */
 } afx_flow_config;
 
 typedef struct {
@@ -160,21 +165,25 @@ int afx_flow_fields(const afx_mesh* mesh, const afx_geometry* geom, const afx_fl
 /* ------------------------------------------------------------------------------------------- */
 /* scalar fields                                                                                 */
 
-/* Cell gradient grad[j][C] of phi by `gradient` (AFX_GRAD_*) with leading-order boundary values:
- * kind[f] == AFX_BC_DIRICHLET -> value[f], any other closure -> the owner value.  Replaces
- * ResidualAssembler.gradient for Dirichlet / zero-gradient closures (reference:
- * aquaflux/discretization/residual.py:253-296), e.g. the SST wall-distance gradient
- * (aquaflux/turbulence/transport.py:86-110). */
-int afx_scalar_gradient(const afx_mesh* mesh, const afx_geometry* geom, int gradient, const int8_t* kind,
-                        const double* value, const double* phi, double* grad, hipStream_t stream);
+/* Cell gradient grad[j][C] of phi by `gradient` (AFX_GRAD_*; `sweeps` and `relaxation` are read for
+ * AFX_GRAD_CORRECTED) with leading-order boundary values: kind[f] == AFX_BC_DIRICHLET -> value[f],
+ * any other closure -> the owner value.  Replaces ResidualAssembler.gradient for Dirichlet /
+ * zero-gradient closures (reference: aquaflux/discretization/residual.py:253-296), e.g. the SST
+ * wall-distance gradient (aquaflux/turbulence/transport.py:86-110). */
+int afx_scalar_gradient(const afx_mesh* mesh, const afx_geometry* geom, int gradient, int sweeps, double relaxation,
+                        const int8_t* kind, const double* value, const double* phi, double* grad,
+                        hipStream_t stream);
 
 enum { AFX_GRAD_NONE = 2 };
 
 typedef struct {
   int32_t advection;     /* AFX_ADV_* */
-  int32_t gradient;      /* AFX_GRAD_GREEN_GAUSS / AFX_GRAD_LEAST_SQUARES / AFX_GRAD_NONE (orthogonal grids) */
+  int32_t gradient;      /* AFX_GRAD_GREEN_GAUSS / _LEAST_SQUARES / _NONE (orthogonal grids) / _CORRECTED */
   double limiter_k;      /* Venkatakrishnan K */
   const int8_t* bc_kind; /* [F] AFX_BC_* on boundary faces */
+  int32_t gradient_sweeps;    /* AFX_GRAD_CORRECTED: swept-solve sweep count */
+  int32_t reserved;
+  double gradient_relaxation; /* AFX_GRAD_CORRECTED: Richardson relaxation */
 } afx_scalar_config;
 
 typedef struct {

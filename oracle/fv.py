@@ -264,6 +264,44 @@ def compact_green_gauss(m: OMesh, g: OGeometry, field, bvals):
     return scale(scatter_conservative(m, scale(area_vector, phi_face)), 1.0 / g.volume)
 
 
+def corrected_terms(m: OMesh, g: OGeometry):
+    """``CorrectedGreenGauss.terms`` (``schemes/gradient.py:1160-1185``): interpolation factor,
+    skewness offset x_ip - x_g, owner-outward area vector, volume."""
+    fac = interpolation_factor(m, g)
+    x_p = g.cell_centroid[m.owner]
+    d = g.cell_centroid[m.safe_nb] - x_p
+    skew = g.face_centroid - (x_p + scale(d, fac))
+    return fac, skew, scale(g.normal, g.area), g.volume
+
+
+def corrected_operator(m: OMesh, g: OGeometry, x):
+    """``A_g x = V x - correction(x)`` (``schemes/gradient.py:1187-1204``)."""
+    fac, skew, S, V = corrected_terms(m, g)
+    w = (1.0 - fac) * dot(skew, x[m.owner]) + fac * dot(skew, x[m.safe_nb])
+    corr = scatter_conservative(m, torch.where(m.interior[:, None], scale(S, w), torch.zeros_like(S)))
+    return scale(x, V) - corr
+
+
+def corrected_rhs(m: OMesh, g: OGeometry, field, bvals):
+    """``B phi`` (``schemes/gradient.py:1311-1319``): Green-Gauss sum of interpolated face values."""
+    fac, _, S, _ = corrected_terms(m, g)
+    phi_face = torch.where(m.interior, blend(m, field, field, fac), bvals)
+    return scatter_conservative(m, scale(S, phi_face))
+
+
+def corrected_green_gauss(m: OMesh, g: OGeometry, field, bvals, sweeps=4, relaxation=1.0):
+    """``CorrectedGreenGauss`` with ``SweptGradientSolve`` (``schemes/gradient.py:618-705``):
+    x0 = r V^-1 b (the peeled first sweep), then x <- x + r V^-1 (b - A_g x), ``sweeps - 1`` times."""
+    b = corrected_rhs(m, g, field, bvals)
+    inv = 1.0 / g.volume
+    if sweeps <= 0:
+        return torch.zeros_like(b)
+    x = relaxation * scale(b, inv)
+    for _ in range(sweeps - 1):
+        x = x + relaxation * scale(b - corrected_operator(m, g, x), inv)
+    return x
+
+
 def least_squares_gradient(m: OMesh, g: OGeometry, field, bvals):
     """Weighted least-squares cell gradient (no reference counterpart: a new ``GradientScheme``).
 
@@ -374,11 +412,13 @@ class FlowSetup:
     u_b: torch.Tensor  # (F, d) prescribed velocity (walls, inlets)
     p_b: torch.Tensor  # (F,) prescribed outlet pressure
     advection: str = "upwind"  # "none" | "upwind" | "limited"
-    gradient: str = "green_gauss"  # "green_gauss" | "least_squares"
+    gradient: str = "green_gauss"  # "green_gauss" | "least_squares" | "corrected"
     limiter_k: float = 5.0
     pressure_pin: int | None = None
     pressure_pin_value: float = 0.0
     body_force: tuple = (0.0, 0.0, 0.0)
+    gradient_sweeps: int = 4
+    gradient_relaxation: float = 1.0
 
 
 def flow_boundary_fields(m, g, setup: FlowSetup, vel, p):
@@ -397,6 +437,8 @@ def flow_boundary_fields(m, g, setup: FlowSetup, vel, p):
 def _gradient(m, g, setup, field, bvals):
     if setup.gradient == "least_squares":
         return least_squares_gradient(m, g, field, bvals)
+    if setup.gradient == "corrected":
+        return corrected_green_gauss(m, g, field, bvals, setup.gradient_sweeps, setup.gradient_relaxation)
     return compact_green_gauss(m, g, field, bvals)
 
 
@@ -549,6 +591,8 @@ class ScalarSetup:
     gradient: str = "green_gauss"  # "none" | "green_gauss" | "least_squares"
     limiter: bool = True
     limiter_k: float = 5.0
+    gradient_sweeps: int = 4
+    gradient_relaxation: float = 1.0
 
 
 def scalar_boundary_values(m, g, s: ScalarSetup, phi, grad, gamma):
@@ -585,6 +629,8 @@ def scalar_residual(m, g, s: ScalarSetup, phi, flux, gamma):
         lead = scalar_boundary_values(m, g, s, phi, zero, gamma)
         if s.gradient == "least_squares":
             grad = least_squares_gradient(m, g, phi, lead)
+        elif s.gradient == "corrected":
+            grad = corrected_green_gauss(m, g, phi, lead, s.gradient_sweeps, s.gradient_relaxation)
         else:
             grad = compact_green_gauss(m, g, phi, lead)
         bvals = scalar_boundary_values(m, g, s, phi, grad, gamma)

@@ -175,6 +175,13 @@ def wall_distance(m, g, wall_faces):
     return torch.sqrt(torch.tensor(dist, dtype=F64) ** 2)
 
 
+def gradient_spec(s: fv.FlowSetup):
+    """The flow setup's gradient scheme as scalar_gradient takes it."""
+    if s.gradient == "corrected":
+        return ("corrected", s.gradient_sweeps, s.gradient_relaxation)
+    return s.gradient
+
+
 def scalar_gradient(m, g, gradient, phi, bc: fv.ScalarSetup):
     """``ResidualAssembler.gradient`` for Dirichlet / ZeroGradient closures (leading-order values)."""
     zero = torch.zeros((m.n_cells, m.dim), dtype=F64)
@@ -182,6 +189,8 @@ def scalar_gradient(m, g, gradient, phi, bc: fv.ScalarSetup):
     lead = fv.scalar_boundary_values(m, g, bc, phi, zero, gamma)
     if gradient == "least_squares":
         return fv.least_squares_gradient(m, g, phi, lead)
+    if isinstance(gradient, tuple) and gradient[0] == "corrected":
+        return fv.corrected_green_gauss(m, g, phi, lead, gradient[1], gradient[2])
     return fv.compact_green_gauss(m, g, phi, lead)
 
 
@@ -249,13 +258,14 @@ def rans_residual(m, g, s: fv.FlowSetup, t: SSTSetup, state, mu, rho, scalar_adv
     """``CoupledRANS.residual`` (``turbulence/coupled.py:437-473``) with direct scalars."""
     d, C = m.dim, m.n_cells
     model = t.model
+    gspec = gradient_spec(s)
     blocks = state.reshape(d + 3, C)
     flow = blocks[: d + 1].reshape(-1)
     k, omega = blocks[d + 1], blocks[d + 2]
     vel, p = blocks[:d].T, blocks[d]
     ub, _ = fv.flow_boundary_fields(m, g, s, vel, p)
     grad_vel = torch.stack([fv._gradient(m, g, s, vel[:, i], ub[:, i]) for i in range(d)], dim=1)
-    cl = closure_fields(m, g, s.gradient, t, vel, ub, grad_vel, k, omega)
+    cl = closure_fields(m, g, gspec, t, vel, ub, grad_vel, k, omega)
     wall_nut = wall_face_eddy_viscosity(m, t, k)
     shear = s.kind == fv.FLOW_WALL
     f = fv.flow_fields(m, g, s, flow, mu, rho, nu_t=cl["nu_t"], wall_nu_t=wall_nut, shear_mask=shear)
@@ -273,7 +283,7 @@ def rans_residual(m, g, s: fv.FlowSetup, t: SSTSetup, state, mu, rho, scalar_adv
     face_gamma = gk[own].clone()
     wo = own[t.wall_faces]
     face_gamma[t.wall_faces] = (1.0 - wall_function_weight(t.nu[wo], dw[wo], k[wo], model)) * face_gamma[t.wall_faces]
-    Rk = _scalar_balance(m, g, s.gradient, t.k_bc, k, Q, gk, face_gamma, scalar_advection, s.limiter_k)
+    Rk = _scalar_balance(m, g, gspec, t.k_bc, k, Q, gk, face_gamma, scalar_advection, s.limiter_k)
     prod = cl["nu_t"] * cl["strain"] ** 2
     limit = PRODUCTION_LIMIT_RATIO * model.beta_star * fv.maximum(k, 0.0) * omega
     limited = fv.minimum(prod, limit).clone()
@@ -287,7 +297,7 @@ def rans_residual(m, g, s: fv.FlowSetup, t: SSTSetup, state, mu, rho, scalar_adv
     Rk = Rk - (-model.beta_star * k * om * V)
     # omega residual (transport.py:655-718)
     gw = t.nu + model.blend(cl["f1"], model.sigma_omega1, model.sigma_omega2) * cl["nu_t"]
-    Rw = _scalar_balance(m, g, s.gradient, t.omega_bc, omega, Q, gw, None, scalar_advection, s.limiter_k)
+    Rw = _scalar_balance(m, g, gspec, t.omega_bc, omega, Q, gw, None, scalar_advection, s.limiter_k)
     alpha = model.blend(cl["f1"], model.alpha_1, model.alpha_2)
     cap = (PRODUCTION_LIMIT_RATIO * model.beta_star * fv.maximum(k, 0.0) * omega
            / fv.maximum(cl["nu_t"], EDDY_VISCOSITY_FLOOR))

@@ -10,7 +10,13 @@ from aquaflux_amd.discretization import FirstOrderUpwind, LimitedUpwind
 from aquaflux_amd.flow import MomentumContinuity, MovingWall, NoSlipWall, PressureOutlet, VelocityInlet
 from aquaflux_amd.mesh import perturbed_grid_2d, perturbed_grid_3d, structured_grid_2d, structured_grid_3d
 from aquaflux_amd.properties import Constant, PropertyModel
-from aquaflux_amd.schemes import CompactGreenGauss, LeastSquaresGradient, VenkatakrishnanLimiter
+from aquaflux_amd.schemes import (
+    CompactGreenGauss,
+    CorrectedGreenGauss,
+    LeastSquaresGradient,
+    SweptGradientSolve,
+    VenkatakrishnanLimiter,
+)
 from oracle import fv
 
 F64 = torch.float64
@@ -49,10 +55,33 @@ def build(name, scheme="upwind", gradient="gg", mu=0.01, rho=1.0, device="cuda",
     geom = mesh.geometry(device=device)
     bcs, pin = flow_bcs(name, mesh.dim, inlet_velocity)
     adv = {"none": None, "upwind": FirstOrderUpwind(), "limited": LimitedUpwind(VenkatakrishnanLimiter(5.0))}[scheme]
-    gs = CompactGreenGauss() if gradient == "gg" else LeastSquaresGradient()
+    gs = gradient_scheme(gradient)
     props = PropertyModel({"viscosity": Constant(mu), "density": Constant(rho)})
     mc = MomentumContinuity.build(mesh, geom, props, gs, BoundaryConditions(bcs), advection_scheme=adv, pressure_pin=pin)
     return mesh, geom, mc
+
+
+def gradient_scheme(gradient):
+    """'gg' compact Green-Gauss, 'lsq' least squares, 'cgg' / 'cgg<n>' / 'cgg<n>r<relax>' corrected."""
+    if gradient == "gg":
+        return CompactGreenGauss()
+    if gradient == "lsq":
+        return LeastSquaresGradient()
+    sweeps, relax = gradient_sweeps(gradient)
+    return CorrectedGreenGauss(SweptGradientSolve(sweeps=sweeps, relaxation=relax))
+
+
+def gradient_sweeps(gradient):
+    rest = gradient[3:]
+    relax = 1.0
+    if "r" in rest:
+        rest, r = rest.split("r")
+        relax = float(r)
+    return (int(rest) if rest else 4), relax
+
+
+def oracle_gradient(gradient):
+    return {"gg": "green_gauss", "lsq": "least_squares"}.get(gradient, "corrected")
 
 
 def oracle_setup(mesh, mc, scheme, gradient):
@@ -75,8 +104,10 @@ def oracle_setup(mesh, mc, scheme, gradient):
         u_b=ub,
         p_b=pb,
         advection=scheme,
-        gradient="green_gauss" if gradient == "gg" else "least_squares",
+        gradient=oracle_gradient(gradient),
         pressure_pin=mc.pressure_pin,
+        **(dict(zip(("gradient_sweeps", "gradient_relaxation"), gradient_sweeps(gradient)))
+           if gradient.startswith("cgg") else {}),
     )
 
 

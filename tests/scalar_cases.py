@@ -9,7 +9,13 @@ from aquaflux_amd.boundary import BoundaryConditions, Convective, Dirichlet, Neu
 from aquaflux_amd.discretization import FirstOrderUpwind, LimitedUpwind
 from aquaflux_amd.mesh import perturbed_grid_2d, tet_grid_3d
 from aquaflux_amd.properties import FieldProperty
-from aquaflux_amd.schemes import CompactGreenGauss, LeastSquaresGradient, VenkatakrishnanLimiter
+from aquaflux_amd.schemes import (
+    CompactGreenGauss,
+    CorrectedGreenGauss,
+    LeastSquaresGradient,
+    SweptGradientSolve,
+    VenkatakrishnanLimiter,
+)
 from aquaflux_amd.transport import ScalarTransport, face_mass_flux
 from oracle import fv
 
@@ -29,8 +35,9 @@ CASES = {
 }
 
 ADV = {"none": None, "upwind": FirstOrderUpwind(), "limited": LimitedUpwind(VenkatakrishnanLimiter(5.0))}
-GRAD = {"gg": CompactGreenGauss(), "lsq": LeastSquaresGradient(), "none": None}
-ORACLE_GRAD = {"gg": "green_gauss", "lsq": "least_squares", "none": "none"}
+GRAD = {"gg": CompactGreenGauss(), "lsq": LeastSquaresGradient(), "none": None,
+        "cgg": CorrectedGreenGauss(SweptGradientSolve(sweeps=5, relaxation=0.9))}
+ORACLE_GRAD = {"gg": "green_gauss", "lsq": "least_squares", "none": "none", "cgg": "corrected"}
 
 
 def make(name):
@@ -50,7 +57,7 @@ def oracle_setup(mesh, bcs, adv, grad):
         value[idx] = float(v)
         t_inf[idx] = float(t)
     return fv.ScalarSetup(kind=kind, value=value, t_inf=t_inf, advection=adv, gradient=ORACLE_GRAD[grad],
-                          limiter=adv == "limited", limiter_k=5.0)
+                          limiter=adv == "limited", limiter_k=5.0, gradient_sweeps=5, gradient_relaxation=0.9)
 
 
 def oracle_flux(g, u):

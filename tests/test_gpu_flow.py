@@ -25,6 +25,10 @@ CASES = [
     ("channel3d", "upwind", "lsq"),
     ("channel3d", "limited", "lsq"),
     ("box3d", "upwind", "gg"),
+    ("channel2d", "limited", "cgg"),
+    ("channel2d", "upwind", "cgg9r0.8"),
+    ("channel3d", "limited", "cgg6"),
+    ("cavity", "none", "cgg1"),
 ]
 
 

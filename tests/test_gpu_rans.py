@@ -24,6 +24,8 @@ CASES = [
     ("channel3d_sst", "limited", "limited", "lsq", 1e-4),
     ("channel3d_sst", "limited", "upwind", "lsq", 1e-5),
     ("box3d_sst", "limited", "limited", "gg", 1e-3),
+    ("channel2d_sst", "limited", "limited", "cgg", 1e-5),
+    ("channel3d_sst", "upwind", "upwind", "cgg6r0.9", 1e-4),
 ]
 
 
@@ -65,7 +67,7 @@ def test_rans_closure_fields_match_oracle(case, cuda):
     vel, p = blocks[:d].T, blocks[d]
     ub, _ = fv.flow_boundary_fields(m, g, s, vel, p)
     gv = torch.stack([fv._gradient(m, g, s, vel[:, i], ub[:, i]) for i in range(d)], dim=1)
-    cl = S.closure_fields(m, g, s.gradient, t, vel, ub, gv, blocks[d + 1], blocks[d + 2])
+    cl = S.closure_fields(m, g, S.gradient_spec(s), t, vel, ub, gv, blocks[d + 1], blocks[d + 2])
     f = rans.closure_fields(st.to(cuda))
     for key, ref in (("nu_t", cl["nu_t"]), ("f1", cl["f1"]), ("strain_rate", cl["strain"])):
         err = float((f[key].cpu() - ref).abs().max() / ref.abs().max())

@@ -13,7 +13,7 @@ F64 = torch.float64
 
 
 @pytest.mark.parametrize("name,scheme,gradient", [("cavity", "upwind", "gg"), ("channel2d", "limited", "lsq"),
-                                                  ("channel3d", "upwind", "lsq")])
+                                                  ("channel3d", "upwind", "lsq"), ("channel2d", "limited", "cgg")])
 def test_block_jacobian_matches_frozen_oracle(cuda, name, scheme, gradient):
     """The block-ELL Jacobian equals d/dx of the residual with gradients, a_P, psi frozen."""
     mesh, geom, mc = build(name, scheme, gradient, device=cuda)

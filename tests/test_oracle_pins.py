@@ -394,3 +394,51 @@ def test_scalar_residual_jvp_matches_richardson_fd():
 
     rich = (4 * cd(5e-6) - cd(1e-5)) / 3
     assert float((jv - rich).abs().max() / jv.abs().max()) < 1e-8
+
+
+def test_corrected_green_gauss_pins():
+    """tests/unit/test_gradient.py:163-229: the swept corrected gradient equals compact Green-Gauss on
+    an orthogonal grid and after one sweep anywhere; with enough sweeps it is linear-exact on an
+    irregular grid (where compact is not) and equals the direct solve of A_g G = B phi assembled
+    densely by probing the operator."""
+    from aquaflux_amd.mesh import perturbed_grid_2d, structured_grid_2d
+
+    def lin(x):
+        return 1.5 * x[:, 0] - 0.7 * x[:, 1] + 0.3
+
+    orth = structured_grid_2d(12, 12)
+    mo = fv.OMesh.from_mesh(orth)
+    go = fv.geometry(mo)
+    rng = np.random.default_rng(0)
+    phi = torch.tensor(rng.normal(size=orth.n_cells))
+    bv = torch.tensor(rng.normal(size=orth.n_faces))
+    cg = fv.corrected_green_gauss(mo, go, phi, bv, sweeps=4)
+    assert float((cg - fv.compact_green_gauss(mo, go, phi, bv)).abs().max()) < 1e-12
+    pm = perturbed_grid_2d(16, 16, perturb=0.2, seed=0)
+    m = fv.OMesh.from_mesh(pm)
+    g = fv.geometry(m)
+    phi = lin(g.cell_centroid)
+    bv = lin(g.face_centroid)
+    assert torch.equal(fv.corrected_green_gauss(m, g, phi, bv, sweeps=1), fv.compact_green_gauss(m, g, phi, bv))
+    exact = torch.tensor([1.5, -0.7], dtype=F64)
+    interior = torch.ones(pm.n_cells, dtype=torch.bool)
+    interior[m.owner[~m.interior]] = False
+    err_c = float((fv.compact_green_gauss(m, g, phi, bv)[interior] - exact).abs().max())
+    err_s = float((fv.corrected_green_gauss(m, g, phi, bv, sweeps=40)[interior] - exact).abs().max())
+    assert err_c > 1e-3 and err_s < 1e-10
+    # direct solve on a small irregular grid
+    sm = perturbed_grid_2d(5, 4, perturb=0.2, seed=2)
+    ms = fv.OMesh.from_mesh(sm)
+    gs = fv.geometry(ms)
+    C = sm.n_cells
+    cols = []
+    for i in range(2 * C):
+        e = torch.zeros(2 * C, dtype=F64)
+        e[i] = 1.0
+        cols.append(fv.corrected_operator(ms, gs, e.reshape(C, 2)).reshape(-1))
+    A = torch.stack(cols, dim=1)
+    phis = torch.tensor(rng.normal(size=C))
+    bvs = torch.tensor(rng.normal(size=sm.n_faces))
+    direct = torch.linalg.solve(A, fv.corrected_rhs(ms, gs, phis, bvs).reshape(-1)).reshape(C, 2)
+    swept = fv.corrected_green_gauss(ms, gs, phis, bvs, sweeps=60)
+    assert float((swept - direct).abs().max() / direct.abs().max()) < 1e-12
