@@ -81,6 +81,8 @@ class AfxRansConfig(ctypes.Structure):
         ("wall_cell", c_void_p),
         ("wall_distance", c_void_p),
         ("wall_distance_gradient", c_void_p),
+        ("k_transform", ctypes.c_int32),
+        ("omega_transform", ctypes.c_int32),
     ]
 
 
@@ -125,8 +127,9 @@ class AfxScalarConfig(ctypes.Structure):
         ("limiter_k", ctypes.c_double),
         ("bc_kind", c_void_p),
         ("gradient_sweeps", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("transient", ctypes.c_int32),
         ("gradient_relaxation", ctypes.c_double),
+        ("dt", ctypes.c_double),
     ]
 
 
@@ -137,6 +140,8 @@ class AfxScalarInputs(ctypes.Structure):
         ("flux", c_void_p),
         ("value", c_void_p),
         ("t_inf", c_void_p),
+        ("phi_old", c_void_p),
+        ("phi_older", c_void_p),
     ]
 
 
@@ -148,6 +153,8 @@ class AfxScalarCotangents(ctypes.Structure):
         ("value", c_void_p),
         ("t_inf", c_void_p),
         ("geometry", AfxGeometryCotangents),
+        ("phi_old", c_void_p),
+        ("phi_older", c_void_p),
     ]
 
 

@@ -45,6 +45,7 @@ struct RansArgs {
   SstConst sc;
   int sweeps;    // corrected Green-Gauss sweep count
   double relax;  // ... and relaxation
+  int log_w;     // omega solved as log(omega): the wall fixation row is log(omega / omega_wall)
 };
 
 // closure arrays (written by pass C)
@@ -497,7 +498,10 @@ __global__ void __launch_bounds__(256) k_rans_sres(MeshView m, RansArgs<S> a, In
                  dk, pw, dw, cw);
   R.set((int64_t)(D + 1) * m.C + c, (acck - pk) - dk);
   S rw = ((accw - pw) - dw) - cw;
-  if (wall) rw = Wc.phi - omega_wall_k<S>(sc, a.nu[c], dP, Kc.phi);
+  if (wall) {
+    const S t = omega_wall_k<S>(sc, a.nu[c], dP, Kc.phi);
+    rw = a.log_w ? dlog(Wc.phi / t) : Wc.phi - t;
+  }
   R.set((int64_t)(D + 2) * m.C + c, rw);
 }
 
@@ -696,21 +700,26 @@ __global__ void __launch_bounds__(256) k_rans_sres_adj(MeshView m, RansArgs<doub
                            B.f1, B.cd);
     }
   }
-  if (wall) {  // R_w = w - omega_wall(nu, d, k)
+  if (wall) {  // R_w = w - omega_wall(nu, d, k), or log(w / omega_wall(nu, d, k))
     const double l = lam[(int64_t)(D + 2) * m.C + c];
-    B.w.phi += l;
     dual ok = omega_wall_k<dual>(sc, dual{a.nu[c], 0.0}, dP, dual{Kc.phi, 1.0});
     dual on = omega_wall_k<dual>(sc, dual{a.nu[c], 1.0}, dP, dual{Kc.phi, 0.0});
-    B.k.phi -= l * ok.d;
-    B.nu -= l * on.d;
+    const double lw_ = a.log_w ? l / Wc.phi : l, lt = a.log_w ? l / ok.v : l;
+    B.w.phi += lw_;
+    B.k.phi -= lt * ok.d;
+    B.nu -= lt * on.d;
   }
   if (R) {
     double pk, dk, pw, dw, cw;
     sst_sources<double>(sc, Kc.phi, Wc.phi, a.nut[c], a.Sr[c], a.f1[c], a.cd[c], a.tau[c], a.nu[c], dP, wall, V, pk,
                         dk, pw, dw, cw);
     R[(int64_t)(D + 1) * m.C + c] = (acck - pk) - dk;
-    R[(int64_t)(D + 2) * m.C + c] = wall ? Wc.phi - omega_wall_k<double>(sc, a.nu[c], dP, Kc.phi)
-                                         : ((accw - pw) - dw) - cw;
+    double rw = ((accw - pw) - dw) - cw;
+    if (wall) {
+      const double t = omega_wall_k<double>(sc, a.nu[c], dP, Kc.phi);
+      rw = a.log_w ? log(Wc.phi / t) : Wc.phi - t;
+    }
+    R[(int64_t)(D + 2) * m.C + c] = rw;
   }
   // diffusivities gamma_k, gamma_w of this cell
   adj_gamma(B.k.gam, a.f1[c], a.nut[c], sc.sk1, sc.sk2, B.nu, B.f1, B.nut);
@@ -1168,9 +1177,10 @@ int64_t rans_work_doubles(int D, int64_t C, int64_t F, int mode) {
   const int NA = D + 3, NL = D + 2;
   int64_t fwd = (int64_t)(NA * D + NL + 7 + 1) * C + F;  // G, psi, closure(7), aP, face mass flux
   const int64_t cgg = (int64_t)2 * NA * D * C;            // corrected Green-Gauss sweep scratch
-  if (mode == 1) return 2 * fwd + cgg;
-  if (mode == 2) return fwd + (int64_t)(NA * D + 1 + NL + 2 * NL + 7) * C + (int64_t)NL * C + F + cgg;
-  return fwd + cgg;
+  const int64_t phys = (int64_t)NA * C;                   // physical state of a transformed k / omega
+  if (mode == 1) return 2 * fwd + cgg + 2 * phys;
+  if (mode == 2) return fwd + (int64_t)(NA * D + 1 + NL + 2 * NL + 7) * C + (int64_t)NL * C + F + cgg + phys;
+  return fwd + cgg + phys;
 }
 
 struct RWork {
@@ -1179,6 +1189,7 @@ struct RWork {
   double *Gb, *aPb, *psib, *extb, *mueffb, *nutb, *f1b, *Srb, *cdb, *taub, *nutwb, *mdotb;
   int32_t* ext;
   double* cgg;
+  double *xs, *xst;  // physical state (and tangent) when k or omega is solved in log form
 };
 
 RWork rcarve(void* base, int D, int64_t C, int64_t F, int mode) {
@@ -1218,6 +1229,8 @@ RWork rcarve(void* base, int D, int64_t C, int64_t F, int mode) {
     w.mdotb = take(F);
   }
   w.cgg = take((int64_t)2 * NA * D * C);
+  w.xs = take((int64_t)NA * C);
+  if (mode == 1) w.xst = take((int64_t)NA * C);
   return w;
 }
 
@@ -1279,7 +1292,55 @@ RansArgs<S> rargs(const afx_rans_config* cfg, const afx_rans_inputs* in, const a
   a.sc = make_sst(cfg);
   a.sweeps = cfg->flow.gradient_sweeps;
   a.relax = cfg->flow.gradient_relaxation;
+  a.log_w = cfg->omega_transform == AFX_SCALAR_LOG;
   return a;
+}
+
+// Solved -> physical state (ScalarVariableTransform.to_physical, turbulence/coupled.py:379-392):
+// k (row D+1) and omega (row D+2) are e^w when solved in log form, every other row is copied; the
+// tangent is e^w dw.
+__global__ void __launch_bounds__(256) k_rans_to_physical(int64_t C, int NA, int logk_row, int logw_row,
+                                                          const double* __restrict__ x,
+                                                          const double* __restrict__ xt, double* __restrict__ xs,
+                                                          double* __restrict__ xst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)NA * C) return;
+  const int v = (int)(i / C);
+  const double w = x[i];
+  const bool lg = v == logk_row || v == logw_row;
+  const double e = lg ? exp(w) : w;
+  xs[i] = e;
+  if (xst) xst[i] = xt ? (lg ? e * xt[i] : xt[i]) : 0.0;
+}
+
+// Cotangent of the solved log-form rows: wbar = e^w * phibar (in place).
+__global__ void __launch_bounds__(256) k_rans_log_bar(int64_t C, int row, const double* __restrict__ xs,
+                                                      double* __restrict__ xb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C) return;
+  const int64_t j = (int64_t)row * C + i;
+  xb[j] *= xs[j];
+}
+
+bool rans_transformed(const afx_rans_config* cfg) {
+  return cfg->k_transform == AFX_SCALAR_LOG || cfg->omega_transform == AFX_SCALAR_LOG;
+}
+
+// Points `a` at the physical state; `xs` / `xst` receive it (`xst` only with a tangent).
+template <class S>
+int rans_bind_physical(const afx_mesh* m, const afx_rans_config* cfg, const afx_rans_inputs* in,
+                       const afx_rans_inputs* tan, double* xs, double* xst, RansArgs<S>& a, hipStream_t s) {
+  if (!rans_transformed(cfg)) return 0;
+  const int D = m->dim, NA = D + 3;
+  const int64_t C = m->C;
+  const double* xt = tan ? tan->state : nullptr;
+  k_rans_to_physical<<<grid_for((int64_t)NA * C), dim3(kBlock), 0, s>>>(
+      C, NA, cfg->k_transform == AFX_SCALAR_LOG ? D + 1 : -1, cfg->omega_transform == AFX_SCALAR_LOG ? D + 2 : -1,
+      in->state, xt, xs, tan ? xst : nullptr);
+  AFX_CHECK_LAUNCH();
+  a.x.v = xs;
+  a.x.t = tan ? xst : nullptr;
+  return 0;
 }
 
 template <class S>
@@ -1383,6 +1444,9 @@ int rcheck(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg,
                   cfg->wall_distance && cfg->wall_distance_gradient,
               "rans: a required configuration pointer is null");
   AFX_REQUIRE(cfg->density > 0.0, "rans: density must be positive");
+  AFX_REQUIRE(cfg->k_transform >= 0 && cfg->k_transform <= 1 && cfg->omega_transform >= 0 &&
+                  cfg->omega_transform <= 1,
+              "rans: k_transform / omega_transform must be AFX_SCALAR_DIRECT or AFX_SCALAR_LOG");
   AFX_REQUIRE(cfg->flow.gradient != kGradCorrected || cfg->flow.gradient_relaxation > 0.0,
               "rans: gradient relaxation must be positive");
   AFX_REQUIRE(work && work_size >= rans_work_doubles(m->dim, m->C, m->F, mode) * 8, "rans: workspace too small");
@@ -1405,6 +1469,8 @@ extern "C" int afx_rans_residual(const afx_mesh* m, const afx_geometry* g, const
   AFX_REQUIRE(residual, "afx_rans_residual: null residual");
   RWork w = rcarve(work, m->dim, m->C, m->F, 0);
   RansArgs<double> a = rargs<double>(cfg, in, nullptr);
+  rc = rans_bind_physical<double>(m, cfg, in, nullptr, w.xs, nullptr, a, s);
+  if (rc) return rc;
   return rdispatch(m->dim, cfg->flow.advection, cfg->scalar_advection, cfg->flow.gradient,
                    [&](auto Dc, auto Fc, auto Sc, auto Gc) {
                      return rans_forward<decltype(Dc)::value, double, decltype(Fc)::value, decltype(Sc)::value,
@@ -1421,6 +1487,8 @@ extern "C" int afx_rans_jvp(const afx_mesh* m, const afx_geometry* g, const afx_
   AFX_REQUIRE(tangent && residual && residual_tangent, "afx_rans_jvp: null argument");
   RWork w = rcarve(work, m->dim, m->C, m->F, 1);
   RansArgs<dual> a = rargs<dual>(cfg, in, tangent);
+  rc = rans_bind_physical<dual>(m, cfg, in, tangent, w.xs, w.xst, a, s);
+  if (rc) return rc;
   return rdispatch(m->dim, cfg->flow.advection, cfg->scalar_advection, cfg->flow.gradient,
                    [&](auto Dc, auto Fc, auto Sc, auto Gc) {
                      return rans_forward<decltype(Dc)::value, dual, decltype(Fc)::value, decltype(Sc)::value,
@@ -1442,6 +1510,8 @@ int rans_vjp_impl(const afx_mesh* m, const afx_geometry* g, const afx_rans_confi
               "afx_rans_vjp: null cotangent argument");
   RWork w = rcarve(work, m->dim, m->C, m->F, 2);
   RansArgs<double> a = rargs<double>(cfg, in, nullptr);
+  rc = rans_bind_physical<double>(m, cfg, in, nullptr, w.xs, nullptr, a, s);
+  if (rc) return rc;
   RansBars bar{out->state, out->mu,  out->rho, out->nu, out->u_b, out->p_b, w.Gb,  w.aPb, w.psib,
                w.extb,     w.mueffb, w.nutb,   w.f1b,   w.Srb,    w.cdb,    w.taub, w.nutwb};
   return rdispatch(m->dim, cfg->flow.advection, cfg->scalar_advection, cfg->flow.gradient,
@@ -1478,6 +1548,12 @@ int rans_vjp_impl(const afx_mesh* m, const afx_geometry* g, const afx_rans_confi
                      AFX_LAUNCH(kKRansGradAdj, (k_rans_grad_adj<D, GK, LF, LS>), grid_for(m->C), dim3(kBlock), 0, s,
                                 v, b, w.ext, bar);
                      AFX_CHECK_LAUNCH();
+                     for (int row = D + 1; row <= D + 2; ++row) {
+                       const int tr = row == D + 1 ? cfg->k_transform : cfg->omega_transform;
+                       if (tr != AFX_SCALAR_LOG) continue;
+                       k_rans_log_bar<<<grid_for(m->C), dim3(kBlock), 0, s>>>(m->C, row, w.xs, bar.x);
+                       AFX_CHECK_LAUNCH();
+                     }
                      return 0;
                    });
 }
@@ -1500,8 +1576,23 @@ extern "C" int afx_rans_residual_vjp(const afx_mesh* m, const afx_geometry* g, c
 extern "C" int afx_rans_fields(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg,
                                const afx_rans_inputs* in, double* fields, hipStream_t s) {
   AFX_REQUIRE(m && g && cfg && in && fields, "afx_rans_fields: null argument");
+  AFX_REQUIRE(cfg->k_transform >= 0 && cfg->k_transform <= 1 && cfg->omega_transform >= 0 &&
+                  cfg->omega_transform <= 1,
+              "afx_rans_fields: k_transform / omega_transform must be AFX_SCALAR_DIRECT or AFX_SCALAR_LOG");
+  // `fields` holds only the forward products (gradients, limiters, a_P, closure); the sweep
+  // scratch of the corrected gradient and the physical state of a log-form k / omega live in the
+  // mesh handle's scratch.
   RWork w = rcarve(fields, m->dim, m->C, m->F, 0);
+  const int NA = m->dim + 3;
+  const int64_t nscr = (int64_t)2 * NA * m->dim * m->C + (int64_t)NA * m->C;
+  double* scr = afx_mesh_scratch(const_cast<afx_mesh*>(m), nscr);
+  AFX_REQUIRE(scr, "afx_rans_fields: could not allocate scratch");
+  w.mdot = nullptr;
+  w.cgg = scr;
+  w.xs = scr + (int64_t)2 * NA * m->dim * m->C;
   RansArgs<double> a = rargs<double>(cfg, in, nullptr);
+  int rc = rans_bind_physical<double>(m, cfg, in, nullptr, w.xs, nullptr, a, s);
+  if (rc) return rc;
   return rdispatch(m->dim, cfg->flow.advection, cfg->scalar_advection, cfg->flow.gradient,
                    [&](auto Dc, auto Fc, auto Sc, auto Gc) {
                      return rans_forward<decltype(Dc)::value, double, decltype(Fc)::value, decltype(Sc)::value,

@@ -35,8 +35,19 @@ struct ScArgs {
   const int8_t* kind;  // boundary closure per face (AFX_BC_*)
   In<S> phi, gam, flux, val, tinf;
   In<S> G, psi;
+  In<S> phi_old, phi_older;  // previous time levels (transient term)
   double lim_k;
+  int bdf;     // 0 steady, 1 backward Euler, 2 BDF2
+  double dt;
 };
+
+// Accumulation V (phi - phi_old) / dt (BDF1) or V (1.5 phi - 2 phi_old + 0.5 phi_older) / dt
+// (transient.py:35-62).
+template <class S>
+AFX_DEV S accumulation(const ScArgs<S>& a, int32_t c, S phi, double V) {
+  if (a.bdf == 1) return V * (phi - a.phi_old[c]) / a.dt;
+  return V * ((1.5 * phi - 2.0 * a.phi_old[c]) + 0.5 * a.phi_older[c]) / a.dt;
+}
 
 template <int D, class S>
 AFX_DEV void load_grad_row(const In<S>& G, int32_t C, int32_t c, S (&g)[D]) {
@@ -205,6 +216,7 @@ __global__ void __launch_bounds__(256) k_sc_res(MeshView m, ScArgs<S> a, Out<S> 
     }
     acc = r.own ? acc + fl : acc - fl;
   }
+  if (a.bdf) acc = acc + accumulation<S>(a, c, xc, m.vol[c]);
   R.set(c, acc);
 }
 
@@ -449,6 +461,8 @@ AFX_DEV double adj_sc_boundary(int kind, const FaceGeo<D>& q, double Qf, double 
 }
 
 struct ScBars {
+  double* phi_old;    // [C] (transient only)
+  double* phi_older;  // [C] (BDF2 only)
   double* phi;   // [C]
   double* gam;   // [C]
   double* flux;  // [F]
@@ -542,6 +556,20 @@ __global__ void __launch_bounds__(256) k_sc_res_adj(MeshView m, ScArgs<double> a
       for (int j = 0; j < D; ++j) ccb[j] += qb.xP[j];
     }
   }
+  double volb = 0.0;
+  if (a.bdf) {  // accumulation term
+    const double V = m.vol[c], s_ = lc / a.dt;
+    if (a.bdf == 1) {
+      B.phi += s_ * V;
+      bar.phi_old[c] = -s_ * V;
+      volb = s_ * (xc - a.phi_old[c]);
+    } else {
+      B.phi += s_ * V * 1.5;
+      bar.phi_old[c] = -s_ * V * 2.0;
+      bar.phi_older[c] = s_ * V * 0.5;
+      volb = s_ * ((1.5 * xc - 2.0 * a.phi_old[c]) + 0.5 * a.phi_older[c]);
+    }
+  }
   bar.phi[c] = B.phi;
   bar.gam[c] = B.gam;
   bar.psi[c] = B.psi;
@@ -550,7 +578,7 @@ __global__ void __launch_bounds__(256) k_sc_res_adj(MeshView m, ScArgs<double> a
   if (GEO) {
 #pragma unroll
     for (int j = 0; j < D; ++j) bar.cc[(int64_t)j * m.C + c] = ccb[j];
-    bar.vol[c] = 0.0;
+    bar.vol[c] = volb;
   }
 }
 
@@ -981,7 +1009,15 @@ ScArgs<S> sc_args(const afx_scalar_config* cfg, const afx_scalar_inputs* in, con
     a.val.t = tan->value;
     a.tinf.t = tan->t_inf;
   }
+  a.phi_old.v = in->phi_old;
+  a.phi_older.v = in->phi_older;
+  if (tan) {
+    a.phi_old.t = tan->phi_old;
+    a.phi_older.t = tan->phi_older;
+  }
   a.lim_k = cfg->limiter_k;
+  a.bdf = cfg->transient;
+  a.dt = cfg->dt;
   return a;
 }
 
@@ -1044,6 +1080,9 @@ int sc_check(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* 
   AFX_REQUIRE(cfg->gradient >= 0 && cfg->gradient <= 3, "scalar: unknown gradient scheme");
   AFX_REQUIRE(cfg->gradient != 3 || cfg->gradient_relaxation > 0.0, "scalar: gradient relaxation must be positive");
   AFX_REQUIRE(m->C > 0, "scalar: empty mesh");
+  AFX_REQUIRE(cfg->transient >= 0 && cfg->transient <= 2, "scalar: transient must be 0 (steady), 1 or 2 (BDF order)");
+  AFX_REQUIRE(cfg->transient == 0 || (cfg->dt > 0.0 && in->phi_old), "scalar: the transient term needs dt > 0 and phi_old");
+  AFX_REQUIRE(cfg->transient != 2 || in->phi_older, "scalar: BDF2 needs phi_older");
   AFX_REQUIRE(work && work_size >= sc_work_doubles(m->dim, m->C, mode) * 8, "scalar: workspace too small");
   return 0;
 }
@@ -1127,8 +1166,10 @@ extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const af
               "reconstruction");
   ScWork w = sc_carve(work, m->dim, m->C, 2);
   ScArgs<double> a = sc_args<double>(cfg, in, nullptr);
-  ScBars bar{out->phi, out->gamma, out->flux, out->value, out->t_inf, w.Gb, w.psib, w.extb, w.limx, w.limf,
-             gb.area, gb.normal, gb.face_centroid, gb.volume, gb.cell_centroid};
+  AFX_REQUIRE(cfg->transient == 0 || out->phi_old, "afx_scalar_vjp: the transient term needs a phi_old cotangent");
+  AFX_REQUIRE(cfg->transient != 2 || out->phi_older, "afx_scalar_vjp: BDF2 needs a phi_older cotangent");
+  ScBars bar{out->phi_old, out->phi_older, out->phi, out->gamma, out->flux, out->value, out->t_inf, w.Gb, w.psib,
+             w.extb, w.limx, w.limf, gb.area, gb.normal, gb.face_centroid, gb.volume, gb.cell_centroid};
   return sc_dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
     constexpr int D = decltype(Dc)::value, ADV = decltype(Ac)::value, GR = decltype(Gc)::value;
     constexpr bool LIM = ADV == kAdvLimited;

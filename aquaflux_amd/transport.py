@@ -16,10 +16,11 @@ import ctypes
 
 import numpy as np
 import torch
+import torch.autograd.forward_ad as fwAD
 
 from . import _lib
 from .boundary import BC_DIRICHLET, BoundaryConditions, Convective, Dirichlet, Neumann, ZeroGradient
-from .discretization import AdvectionScheme, LimitedUpwind
+from .discretization import AdvectionScheme, LimitedUpwind, TransientTerm
 from .properties import FieldProperty
 
 F64 = torch.float64
@@ -92,19 +93,29 @@ class _ScalarRaw:
     def _gs(geo):
         return _lib.AfxGeometry(*[t.data_ptr() for t in geo])
 
-    def residual(self, phi, gam, flux, val, tinf, geo):
-        R = torch.empty_like(phi)
+    def _cfg(self, step):
+        """The configuration for one call: ``step`` is None (steady) or (bdf order, dt)."""
+        if step is None:
+            return self.cfg
+        cfg = _lib.AfxScalarConfig.from_buffer_copy(self.cfg)
+        cfg.transient, cfg.dt = step
+        return cfg
+
+    def residual(self, prim, geo, step=None):
+        """prim = (phi, gamma, flux, value, t_inf, phi_old, phi_older); the last two may be None."""
+        R = torch.empty_like(prim[0])
         w = self.dm.workspace("scalar0", self.work_bytes[0])
         gs = self._gs(geo)
-        ins = _lib.AfxScalarInputs(*[_lib.ptr(t) for t in (phi, gam, flux, val, tinf)])
+        ins = _lib.AfxScalarInputs(*[_lib.ptr(t) for t in prim])
         _lib.check(
-            _lib.lib().afx_scalar_residual(self.dm.handle, ctypes.byref(gs), ctypes.byref(self.cfg), ctypes.byref(ins),
-                                           R.data_ptr(), w.data_ptr(), w.numel(), _lib.stream_of(self.dm.device)),
+            _lib.lib().afx_scalar_residual(self.dm.handle, ctypes.byref(gs), ctypes.byref(self._cfg(step)),
+                                           ctypes.byref(ins), R.data_ptr(), w.data_ptr(), w.numel(),
+                                           _lib.stream_of(self.dm.device)),
             "afx_scalar_residual",
         )
         return R
 
-    def jvp(self, prim, tans, geo):
+    def jvp(self, prim, tans, geo, step=None):
         R = torch.empty_like(prim[0])
         dR = torch.empty_like(prim[0])
         w = self.dm.workspace("scalar1", self.work_bytes[1])
@@ -112,24 +123,25 @@ class _ScalarRaw:
         ins = _lib.AfxScalarInputs(*[_lib.ptr(t) for t in prim])
         tns = _lib.AfxScalarInputs(*[_lib.ptr(t) for t in tans])
         _lib.check(
-            _lib.lib().afx_scalar_jvp(self.dm.handle, ctypes.byref(gs), ctypes.byref(self.cfg), ctypes.byref(ins),
-                                      ctypes.byref(tns), R.data_ptr(), dR.data_ptr(), w.data_ptr(), w.numel(),
-                                      _lib.stream_of(self.dm.device)),
+            _lib.lib().afx_scalar_jvp(self.dm.handle, ctypes.byref(gs), ctypes.byref(self._cfg(step)),
+                                      ctypes.byref(ins), ctypes.byref(tns), R.data_ptr(), dR.data_ptr(), w.data_ptr(),
+                                      w.numel(), _lib.stream_of(self.dm.device)),
             "afx_scalar_jvp",
         )
         return R, dR
 
-    def vjp(self, prim, cot, geo, with_geometry):
-        bars = [torch.empty_like(t) for t in prim]
+    def vjp(self, prim, cot, geo, with_geometry, step=None):
+        bars = [None if t is None else torch.empty_like(t) for t in prim]
         gbars = [torch.empty_like(t) for t in geo] if with_geometry else [None] * 5
         w = self.dm.workspace("scalar2", self.work_bytes[2])
         gs = self._gs(geo)
         ins = _lib.AfxScalarInputs(*[_lib.ptr(t) for t in prim])
-        outs = _lib.AfxScalarCotangents(*[b.data_ptr() for b in bars],
-                                        _lib.AfxGeometryCotangents(*[_lib.ptr(b) for b in gbars]))
+        outs = _lib.AfxScalarCotangents(*[_lib.ptr(b) for b in bars[:5]],
+                                        _lib.AfxGeometryCotangents(*[_lib.ptr(b) for b in gbars]),
+                                        *[_lib.ptr(b) for b in bars[5:]])
         _lib.check(
-            _lib.lib().afx_scalar_vjp(self.dm.handle, ctypes.byref(gs), ctypes.byref(self.cfg), ctypes.byref(ins),
-                                      cot.data_ptr(), ctypes.byref(outs), w.data_ptr(), w.numel(),
+            _lib.lib().afx_scalar_vjp(self.dm.handle, ctypes.byref(gs), ctypes.byref(self._cfg(step)),
+                                      ctypes.byref(ins), cot.data_ptr(), ctypes.byref(outs), w.data_ptr(), w.numel(),
                                       _lib.stream_of(self.dm.device)),
             "afx_scalar_vjp",
         )
@@ -137,33 +149,43 @@ class _ScalarRaw:
 
 
 class _ScalarResidualFn(torch.autograd.Function):
-    """R(phi, gamma, flux, value, t_inf; area, normal, face centroid, volume, cell centroid)."""
+    """R(phi, gamma, flux, value, t_inf, phi_old, phi_older; area, normal, face centroid, volume,
+    cell centroid) at the time step ``step`` (None steady, else (bdf order, dt))."""
 
     @staticmethod
-    def forward(ctx, raw, phi, gam, flux, val, tinf, area, nrm, fc, vol, cc):
-        ctx.raw = raw
-        prim = (phi, gam, flux, val, tinf)
+    def forward(ctx, raw, step, phi, gam, flux, val, tinf, phi_old, phi_older, area, nrm, fc, vol, cc):
+        ctx.raw, ctx.step = raw, step
+        prim = (phi, gam, flux, val, tinf, phi_old, phi_older)
         geo = (area, nrm, fc, vol, cc)
-        ctx.save_for_backward(*prim, *geo)
-        ctx.save_for_forward(*prim, *geo)
-        return raw.residual(*prim, geo)
+        ctx.has_old = (phi_old is not None, phi_older is not None)
+        saved = [t for t in prim if t is not None] + list(geo)
+        ctx.save_for_backward(*saved)
+        ctx.save_for_forward(*saved)
+        return raw.residual(prim, geo, step)
+
+    @staticmethod
+    def _unpack(ctx):
+        saved = list(ctx.saved_tensors)
+        prim = saved[:5]
+        rest = saved[5:]
+        for present in ctx.has_old:
+            prim.append(rest.pop(0) if present else None)
+        return tuple(prim), tuple(rest)
 
     @staticmethod
     def backward(ctx, gR):
-        saved = ctx.saved_tensors
-        prim, geo = saved[:5], saved[5:]
-        with_geo = any(ctx.needs_input_grad[6:11])
-        bars, gbars = ctx.raw.vjp(prim, gR.contiguous(), geo, with_geo)
-        return (None, *bars, *gbars)
+        prim, geo = _ScalarResidualFn._unpack(ctx)
+        with_geo = any(ctx.needs_input_grad[9:14])
+        bars, gbars = ctx.raw.vjp(prim, gR.contiguous(), geo, with_geo, ctx.step)
+        return (None, None, *bars, *gbars)
 
     @staticmethod
-    def jvp(ctx, _raw_t, dphi, dgam, dflux, dval, dtinf, *dgeo):
-        if any(t is not None for t in dgeo):
-            raise NotImplementedError("forward-mode derivatives with respect to the geometry are not provided")
-        saved = ctx.saved_tensors
-        prim, geo = saved[:5], saved[5:]
-        tans = [None if t is None else t.contiguous() for t in (dphi, dgam, dflux, dval, dtinf)]
-        _, dR = ctx.raw.jvp(prim, tans, geo)
+    def jvp(ctx, _raw_t, _step_t, dphi, dgam, dflux, dval, dtinf, dold, dolder, *_dgeo):
+        # geometry tangents arrive as zeros (torch fills them for plain tensors); a geometry that
+        # carries a real tangent is refused before apply
+        prim, geo = _ScalarResidualFn._unpack(ctx)
+        tans = [None if t is None else t.contiguous() for t in (dphi, dgam, dflux, dval, dtinf, dold, dolder)]
+        _, dR = ctx.raw.jvp(prim, tans, geo, ctx.step)
         return dR
 
 
@@ -188,8 +210,8 @@ class ScalarTransport:
     ) -> "ScalarTransport":
         if sources:
             raise ValueError("volume sources are not supported by the GPU scalar residual")
-        if transient is not None:
-            raise ValueError("transient terms are not supported by the GPU scalar residual")
+        if transient is not None and not isinstance(transient, TransientTerm):
+            raise ValueError(f"unsupported transient term {type(transient).__name__}")
         dev = geometry.face.area.device
         bound = boundary.resolve(mesh.face_patches)
         kind, val, tinf = _boundary_arrays(bound, mesh, dev)
@@ -203,7 +225,7 @@ class ScalarTransport:
         cfg.limiter_k = advection_scheme.limiter_k() if isinstance(advection_scheme, LimitedUpwind) else 5.0
         cfg.bc_kind = kind.data_ptr()
         return cls(mesh=mesh, geometry=geometry, diffusivity=diffusivity, boundary=bound,
-                   advection_scheme=advection_scheme, gradient_scheme=gradient_scheme, sources=(), transient=None,
+                   advection_scheme=advection_scheme, gradient_scheme=gradient_scheme, sources=(), transient=transient,
                    _raw=_ScalarRaw(mesh, geometry, cfg, kind), _val=val, _tinf=tinf)
 
     def with_diffusivity(self, diffusivity) -> "ScalarTransport":
@@ -219,29 +241,65 @@ class ScalarTransport:
         g = self.geometry
         return (g.face.area, g.face.normal_soa, g.face.centroid_soa, g.cell.volume, g.cell.centroid_soa)
 
+    def _time_level(self, phi, phi_old, phi_older, dt, first_step):
+        """(step, phi_old, phi_older) of one call: the accumulation term of ``residual.py:141-147``
+        when a transient term was injected, nothing otherwise (history arguments are then ignored,
+        as in the reference)."""
+        if self.transient is None:
+            return None, None, None
+        if phi_old is None or dt is None:
+            raise ValueError("a transient residual needs phi_old and dt")
+        if isinstance(dt, torch.Tensor):
+            if dt.requires_grad:
+                raise ValueError("derivatives with respect to dt are not provided; pass dt as a float")
+            dt = float(dt)
+        dt = float(dt)
+        if not dt > 0.0:
+            raise ValueError(f"dt must be positive, got {dt}")
+        tensors = [phi_old] if first_step else [phi_old, phi_older]
+        for name, t in zip(("phi_old", "phi_older"), tensors):
+            if t is None:
+                raise ValueError(f"the BDF2 step needs {name}")
+            _lib.require_cuda(t, name)
+            if t.shape != phi.shape:
+                raise ValueError(f"{name} must have shape {tuple(phi.shape)}, got {tuple(t.shape)}")
+        if first_step:
+            return (1, dt), phi_old.contiguous(), None
+        return (2, dt), phi_old.contiguous(), phi_older.contiguous()
+
     def residual(self, flux):
-        """The residual function ``phi -> R(phi)`` for the owner-outward volumetric face flux ``flux``."""
+        """The residual function ``(phi, phi_old=None, phi_older=None, dt=None, first_step=False) ->
+        R`` for the owner-outward volumetric face flux ``flux`` (``residual.py:358-412``)."""
         _lib.require_cuda(flux, "flux")
         if flux.shape != (self.mesh.n_faces,):
             raise ValueError(f"flux must have shape ({self.mesh.n_faces},), got {tuple(flux.shape)}")
 
-        def fn(phi):
+        def fn(phi, phi_old=None, phi_older=None, dt=None, first_step=False):
             _lib.require_cuda(phi, "phi")
-            return _ScalarResidualFn.apply(self._raw, phi.contiguous(), self._gamma(), flux.contiguous(),
-                                           self._val, self._tinf, *self._geometry())
+            step, old, older = self._time_level(phi, phi_old, phi_older, dt, first_step)
+            geo = self._geometry()
+            if any(fwAD.unpack_dual(t).tangent is not None for t in geo):
+                raise NotImplementedError("forward-mode derivatives with respect to the geometry are not provided")
+            return _ScalarResidualFn.apply(self._raw, step, phi.contiguous(), self._gamma(), flux.contiguous(),
+                                           self._val, self._tinf, old, older, *geo)
 
         return fn
 
-    def residual_jvp(self, flux, phi, tangent):
-        """(R, J tangent) with respect to phi only (the Newton-Krylov matvec)."""
-        prim = (phi.contiguous(), self._gamma().detach(), flux.detach().contiguous(), self._val.detach(),
-                self._tinf.detach())
-        geo = tuple(t.detach() for t in self._geometry())
-        return self._raw.jvp(prim, (tangent.contiguous(), None, None, None, None), geo)
+    def _prim(self, flux, phi, old, older):
+        return (phi.contiguous(), self._gamma().detach(), flux.detach().contiguous(), self._val.detach(),
+                self._tinf.detach(), None if old is None else old.detach(), None if older is None else older.detach())
 
-    def residual_vjp(self, flux, phi, cotangent, with_geometry=False):
-        """All cotangents of the residual: ([phi, gamma, flux, value, t_inf], [geometry arrays])."""
-        prim = (phi.contiguous(), self._gamma().detach(), flux.detach().contiguous(), self._val.detach(),
-                self._tinf.detach())
+    def residual_jvp(self, flux, phi, tangent, phi_old=None, phi_older=None, dt=None, first_step=False):
+        """(R, J tangent) with respect to phi only (the Newton-Krylov matvec)."""
+        step, old, older = self._time_level(phi, phi_old, phi_older, dt, first_step)
         geo = tuple(t.detach() for t in self._geometry())
-        return self._raw.vjp(prim, cotangent.contiguous(), geo, with_geometry)
+        return self._raw.jvp(self._prim(flux, phi, old, older), (tangent.contiguous(),) + (None,) * 6, geo, step)
+
+    def residual_vjp(self, flux, phi, cotangent, with_geometry=False, phi_old=None, phi_older=None, dt=None,
+                     first_step=False):
+        """All cotangents of the residual: ([phi, gamma, flux, value, t_inf, phi_old, phi_older],
+        [geometry arrays]); the history cotangents are None for a steady equation (phi_older also on
+        the first step)."""
+        step, old, older = self._time_level(phi, phi_old, phi_older, dt, first_step)
+        geo = tuple(t.detach() for t in self._geometry())
+        return self._raw.vjp(self._prim(flux, phi, old, older), cotangent.contiguous(), geo, with_geometry, step)

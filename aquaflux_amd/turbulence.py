@@ -24,7 +24,7 @@ import torch
 
 from . import _lib
 from .boundary import BC_DIRICHLET, BC_ZERO_GRADIENT, BoundaryConditions, Dirichlet, ZeroGradient
-from .discretization import AdvectionScheme, LimitedUpwind
+from .discretization import AdvectionScheme, DifferenceRow, LimitedUpwind, LogRatioRow
 
 F64 = torch.float64
 
@@ -68,14 +68,49 @@ class SSTModel:
         ]
 
 
-class DirectScalars:
-    """The identity parametrization of a turbulence unknown (``turbulence/coupled.py:155-175``)."""
+class ScalarVariableTransform:
+    """Map between a solved turbulence unknown ``w`` and its physical field
+    (``turbulence/coupled.py:111-152``).  ``code`` is the C ABI's AFX_SCALAR_* value: the residual
+    kernels apply the map themselves, so these methods serve callers that build or read states."""
+
+    code = -1
+
+
+class DirectScalars(ScalarVariableTransform):
+    """The identity parametrization (``turbulence/coupled.py:155-175``)."""
+
+    code = 0
 
     def to_physical(self, w):
         return w
 
     def to_solved(self, phi):
         return phi
+
+    def jacobian_scale(self, phi):
+        return torch.ones_like(phi)
+
+    def fixation_row(self):
+        return DifferenceRow()
+
+
+class LogScalars(ScalarVariableTransform):
+    """``phi = e^w`` (``turbulence/coupled.py:178-211``): positive under any Newton step; the wall
+    fixation becomes the log-ratio row."""
+
+    code = 1
+
+    def to_physical(self, w):
+        return torch.exp(w)
+
+    def to_solved(self, phi):
+        return torch.log(phi)
+
+    def jacobian_scale(self, phi):
+        return phi
+
+    def fixation_row(self):
+        return LogRatioRow()
 
 
 def _scalar_face_data(bc: BoundaryConditions, mesh, what):
@@ -213,7 +248,7 @@ class SSTTurbulence:
 class _RansRaw:
     """Binds the C ABI calls of one configured coupled RANS problem."""
 
-    def __init__(self, momentum, turbulence: SSTTurbulence):
+    def __init__(self, momentum, turbulence: SSTTurbulence, k_transform=None, omega_transform=None):
         self.mesh = momentum.mesh
         self.dm = momentum._raw.dm
         self.gs = momentum._raw.gs
@@ -239,6 +274,8 @@ class _RansRaw:
         cfg.wall_cell = t._wall_cell.data_ptr()
         cfg.wall_distance = t.wall_distance.data_ptr()
         cfg.wall_distance_gradient = t.wall_distance_gradient.data_ptr()
+        cfg.k_transform = (k_transform or DirectScalars()).code
+        cfg.omega_transform = (omega_transform or DirectScalars()).code
         self.cfg = cfg
         self.keep = (t, momentum._raw)
         self.k_value = t._k_value
@@ -369,9 +406,11 @@ class CoupledRANS:
 
     @classmethod
     def build(cls, momentum, turbulence: SSTTurbulence, k_transform=None, omega_transform=None) -> "CoupledRANS":
+        """``CoupledRANS.build`` (``turbulence/coupled.py:306-355``); ``omega_transform=LogScalars()``
+        solves for ``log(omega)``."""
         for name, tr in (("k_transform", k_transform), ("omega_transform", omega_transform)):
-            if tr is not None and not isinstance(tr, DirectScalars):
-                raise ValueError(f"{name}: only DirectScalars is supported by the GPU residual")
+            if tr is not None and not isinstance(tr, (DirectScalars, LogScalars)):
+                raise ValueError(f"{name}: {type(tr).__name__} is not a supported ScalarVariableTransform")
         rho = momentum.density
         if not bool(torch.all(torch.isclose(rho, torch.tensor(turbulence.density, dtype=F64, device=rho.device)))):
             raise ValueError(
@@ -380,8 +419,11 @@ class CoupledRANS:
             )
         if momentum.eddy_viscosity is not None:
             raise ValueError("CoupledRANS.build expects the momentum assembler without an eddy viscosity")
-        return cls(momentum=momentum, turbulence=turbulence, k_transform=DirectScalars(),
-                   omega_transform=DirectScalars(), _raw=_RansRaw(momentum, turbulence))
+        k_transform = k_transform or DirectScalars()
+        omega_transform = omega_transform or DirectScalars()
+        return cls(momentum=momentum, turbulence=turbulence, k_transform=k_transform,
+                   omega_transform=omega_transform,
+                   _raw=_RansRaw(momentum, turbulence, k_transform, omega_transform))
 
     # --- layout (turbulence/coupled.py:56-120) ---------------------------------------------------
     @property
@@ -403,15 +445,19 @@ class CoupledRANS:
         C, d = self.n_cells, self.dim
         return state[: (d + 1) * C], state[(d + 1) * C : (d + 2) * C], state[(d + 2) * C :]
 
-    physical_fields = unpack
+    def physical_fields(self, state):
+        """Flow sub-vector and physical ``k``, ``omega`` (``turbulence/coupled.py:379-392``)."""
+        flow, k, omega = self.unpack(state)
+        return flow, self.k_transform.to_physical(k), self.omega_transform.to_physical(omega)
 
     def state_from_physical(self, flow, k, omega):
-        return self.pack_state(flow, k, omega)
+        """Inverse of :meth:`physical_fields` (``turbulence/coupled.py:394-404``)."""
+        return self.pack_state(flow, self.k_transform.to_solved(k), self.omega_transform.to_solved(omega))
 
     def with_scaled_molecular_viscosity(self, factor) -> "CoupledRANS":
         mom = MomentumScaled.scale(self.momentum, factor)
         turb = self.turbulence.with_scaled_molecular_viscosity(factor)
-        return CoupledRANS.build(mom, turb)
+        return CoupledRANS.build(mom, turb, self.k_transform, self.omega_transform)
 
     # --- inputs ----------------------------------------------------------------------------------
     def _inputs(self):

@@ -182,8 +182,9 @@ typedef struct {
   double limiter_k;      /* Venkatakrishnan K */
   const int8_t* bc_kind; /* [F] AFX_BC_* on boundary faces */
   int32_t gradient_sweeps;    /* AFX_GRAD_CORRECTED: swept-solve sweep count */
-  int32_t reserved;
+  int32_t transient;          /* 0 steady; 1 backward Euler (BDF1); 2 BDF2 (TransientTerm) */
   double gradient_relaxation; /* AFX_GRAD_CORRECTED: Richardson relaxation */
+  double dt;                  /* timestep of the transient term */
 } afx_scalar_config;
 
 typedef struct {
@@ -192,6 +193,8 @@ typedef struct {
   const double* flux;  /* [F] owner-outward volumetric face flux Q_f */
   const double* value; /* [F] Dirichlet value / Neumann flux q / convective coefficient h */
   const double* t_inf; /* [F] convective ambient value */
+  const double* phi_old;   /* [C] previous time level (transient only) */
+  const double* phi_older; /* [C] second-previous time level (BDF2 only) */
 } afx_scalar_inputs;
 
 typedef struct {
@@ -200,7 +203,9 @@ typedef struct {
   double* flux;  /* [F] */
   double* value; /* [F] */
   double* t_inf; /* [F] */
-  afx_geometry_cotangents geometry; /* all five set, or all NULL (least squares: NULL only) */
+  afx_geometry_cotangents geometry; /* all five set, or all NULL (compact Green-Gauss / no gradient only) */
+  double* phi_old;   /* [C] (transient only) */
+  double* phi_older; /* [C] (BDF2 only) */
 } afx_scalar_cotangents;
 
 /* Workspace (device bytes) for mode 0 = residual, 1 = JVP, 2 = VJP. */
@@ -227,6 +232,10 @@ int afx_scalar_vjp(const afx_mesh* mesh, const afx_geometry* geom, const afx_sca
 /* ------------------------------------------------------------------------------------------- */
 /* coupled RANS (u, v[, w], p, k, omega) residual with the k-omega SST closure                   */
 
+/* Parametrisation of a solved turbulence scalar (reference: DirectScalars / LogScalars,
+ * aquaflux/turbulence/coupled.py:155-211): state rows hold phi, or w with phi = e^w. */
+enum { AFX_SCALAR_DIRECT = 0, AFX_SCALAR_LOG = 1 };
+
 enum { AFX_BC_NONE = 0, AFX_BC_DIRICHLET = 1, AFX_BC_ZERO_GRADIENT = 2, AFX_BC_NEUMANN = 3, AFX_BC_CONVECTIVE = 4 };
 
 typedef struct {
@@ -243,6 +252,8 @@ typedef struct {
   const int8_t* wall_cell;            /* [C] 1 on their owner cells (the omega fixation set) */
   const double* wall_distance;        /* [C] */
   const double* wall_distance_gradient; /* [d][C] */
+  int32_t k_transform;                /* AFX_SCALAR_DIRECT / AFX_SCALAR_LOG: the solved k unknown */
+  int32_t omega_transform;            /* ... and omega (log: the wall fixation row is log(omega / target)) */
 } afx_rans_config;
 
 typedef struct {
@@ -268,7 +279,8 @@ typedef struct {
 /* Workspace (device bytes) for mode 0 = residual, 1 = JVP, 2 = VJP. */
 int afx_rans_workspace_bytes(const afx_mesh* mesh, int mode, int64_t* bytes);
 
-/* R(u, p, k, omega).  Replaces CoupledRANS.residual with DirectScalars (reference:
+/* R(u, p, k, omega) at the solved state (k / omega rows e^w under AFX_SCALAR_LOG).  Replaces
+ * CoupledRANS.residual with DirectScalars or LogScalars (reference:
  * aquaflux/turbulence/coupled.py:437-473, closure_fields / k_residual / omega_residual in
  * aquaflux/turbulence/transport.py:548-718). */
 int afx_rans_residual(const afx_mesh* mesh, const afx_geometry* geom, const afx_rans_config* cfg,

@@ -617,9 +617,19 @@ def scalar_boundary_values(m, g, s: ScalarSetup, phi, grad, gamma):
     return out
 
 
-def scalar_residual(m, g, s: ScalarSetup, phi, flux, gamma):
+def transient_term(phi, phi_old, phi_older, dt, first_step, volume):
+    """``TransientTerm.residual`` (``discretization/transient.py:32-62``): BDF1 on the first step,
+    BDF2 afterwards."""
+    if first_step:
+        return volume * (phi - phi_old) / dt
+    return volume * (1.5 * phi - 2.0 * phi_old + 0.5 * phi_older) / dt
+
+
+def scalar_residual(m, g, s: ScalarSetup, phi, flux, gamma, phi_old=None, phi_older=None, dt=None,
+                    first_step=False):
     """``ResidualAssembler.residual`` for ``ScalarTransport`` (Advection, Diffusion) order
-    (``transport/scalar.py:185-208``, ``discretization/residual.py:308-412``)."""
+    (``transport/scalar.py:185-208``, ``discretization/residual.py:308-412``); with ``phi_old`` the
+    accumulation term is added after the flux scatter (``residual.py:141-147``)."""
     C, d = m.n_cells, m.dim
     if s.gradient == "none":
         grad = torch.zeros((C, d), dtype=F64)
@@ -641,4 +651,7 @@ def scalar_residual(m, g, s: ScalarSetup, phi, flux, gamma):
         psi = venkatakrishnan(m, g, phi, grad, s.limiter_k) if s.limiter else torch.ones(C, dtype=F64)
         face = face + flux * limited_face_value(m, g, phi, grad, psi, flux, bvals)
     face = face + diffusion_flux(m, g, phi, grad, gamma, bvals)
-    return scatter_conservative(m, face)
+    R = scatter_conservative(m, face)
+    if phi_old is not None:
+        R = R + transient_term(phi, phi_old, phi_older, dt, first_step, g.volume)
+    return R

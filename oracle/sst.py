@@ -254,14 +254,21 @@ def _scalar_balance(m, g, gradient, bc, phi, Q, gamma, boundary_gamma, advection
     return fv.scatter_conservative(m, face)
 
 
-def rans_residual(m, g, s: fv.FlowSetup, t: SSTSetup, state, mu, rho, scalar_advection="limited"):
-    """``CoupledRANS.residual`` (``turbulence/coupled.py:437-473``) with direct scalars."""
+def rans_residual(m, g, s: fv.FlowSetup, t: SSTSetup, state, mu, rho, scalar_advection="limited", k_log=False,
+                  omega_log=False):
+    """``CoupledRANS.residual`` (``turbulence/coupled.py:437-473``); ``k_log`` / ``omega_log``
+    select ``LogScalars`` for that unknown (``coupled.py:178-211``): the physics runs on ``e^w`` and
+    the omega wall fixation becomes ``LogRatioRow`` (``discretization/fixed_value.py:96-124``)."""
     d, C = m.dim, m.n_cells
     model = t.model
     gspec = gradient_spec(s)
     blocks = state.reshape(d + 3, C)
     flow = blocks[: d + 1].reshape(-1)
     k, omega = blocks[d + 1], blocks[d + 2]
+    if k_log:
+        k = torch.exp(k)
+    if omega_log:
+        omega = torch.exp(omega)
     vel, p = blocks[:d].T, blocks[d]
     ub, _ = fv.flow_boundary_fields(m, g, s, vel, p)
     grad_vel = torch.stack([fv._gradient(m, g, s, vel[:, i], ub[:, i]) for i in range(d)], dim=1)
@@ -307,5 +314,6 @@ def rans_residual(m, g, s: fv.FlowSetup, t: SSTSetup, state, mu, rho, scalar_adv
     cross = 2.0 * (1.0 - cl["f1"]) * model.sigma_omega2 * fv.dot(cl["grad_k"], cl["grad_omega"]) / omega
     Rw = Rw - cross * V
     Rw = Rw.clone()
-    Rw[wall] = omega[wall] - omega_wall(t.nu[wall], dw[wall], kw, model)
+    target = omega_wall(t.nu[wall], dw[wall], kw, model)
+    Rw[wall] = torch.log(omega[wall] / target) if omega_log else omega[wall] - target
     return torch.cat([flow_res, Rk, Rw])

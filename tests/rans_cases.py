@@ -10,7 +10,7 @@ import torch
 from aquaflux_amd.boundary import BoundaryConditions, Dirichlet, ZeroGradient
 from aquaflux_amd.discretization import FirstOrderUpwind, LimitedUpwind
 from aquaflux_amd.schemes import VenkatakrishnanLimiter
-from aquaflux_amd.turbulence import CoupledRANS, SSTModel, SSTTurbulence
+from aquaflux_amd.turbulence import CoupledRANS, DirectScalars, LogScalars, SSTModel, SSTTurbulence
 from oracle import fv
 from oracle import sst as S
 
@@ -55,7 +55,8 @@ def _oracle_scalar(mesh, bcs):
     return fv.ScalarSetup(kind=kind, value=value, t_inf=torch.zeros(F, dtype=F64))
 
 
-def build(name, flow_scheme="upwind", scalar_scheme="upwind", gradient="gg", nu=1e-2, rho=1.0, device="cuda"):
+def build(name, flow_scheme="upwind", scalar_scheme="upwind", gradient="gg", nu=1e-2, rho=1.0, device="cuda",
+          k_log=False, omega_log=False):
     base, walls, k_in, w_in = CASES[name]
     mesh, geom, mc = flow_cases.build(base, flow_scheme, gradient, mu=nu * rho, rho=rho, device=device)
     adv = {"upwind": FirstOrderUpwind(), "limited": LimitedUpwind(VenkatakrishnanLimiter(5.0))}[scalar_scheme]
@@ -66,7 +67,8 @@ def build(name, flow_scheme="upwind", scalar_scheme="upwind", gradient="gg", nu=
     nu_t = torch.full((C,), nu, dtype=F64)
     turb = SSTTurbulence.build(SSTModel(), mesh, geom, gs, adv, rho, nu_t, list(walls), BoundaryConditions(kb),
                                BoundaryConditions(wb))
-    rans = CoupledRANS.build(mc, turb)
+    rans = CoupledRANS.build(mc, turb, LogScalars() if k_log else DirectScalars(),
+                             LogScalars() if omega_log else DirectScalars())
     # the oracle's view
     m = fv.OMesh.from_mesh(mesh)
     g = fv.geometry(m)
@@ -92,13 +94,24 @@ def seeded_state(mesh, seed=0, kind="channel"):
     return np.concatenate(vel + [p, k, w])
 
 
-def oracle_residual(setup, state, mu, rho, scalar_scheme, nu=None, u_b=None, p_b=None):
+def oracle_residual(setup, state, mu, rho, scalar_scheme, nu=None, u_b=None, p_b=None, k_log=False,
+                    omega_log=False):
     m, g, s, t = setup
     if nu is not None:
         t = dataclasses.replace(t, nu=nu)
     if u_b is not None or p_b is not None:
         s = dataclasses.replace(s, u_b=s.u_b if u_b is None else u_b, p_b=s.p_b if p_b is None else p_b)
-    return S.rans_residual(m, g, s, t, state, mu, rho, scalar_scheme)
+    return S.rans_residual(m, g, s, t, state, mu, rho, scalar_scheme, k_log=k_log, omega_log=omega_log)
+
+
+def to_solved(state, dim, k_log, omega_log):
+    """Physical seeded state -> solved state (log of the transformed k / omega blocks)."""
+    b = state.reshape(dim + 3, -1).clone()
+    if k_log:
+        b[dim + 1] = torch.log(b[dim + 1])
+    if omega_log:
+        b[dim + 2] = torch.log(b[dim + 2])
+    return b.reshape(-1)
 
 
 def block_errors(a, b, nblocks):

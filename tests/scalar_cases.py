@@ -6,7 +6,7 @@ import numpy as np
 import torch
 
 from aquaflux_amd.boundary import BoundaryConditions, Convective, Dirichlet, Neumann, ZeroGradient
-from aquaflux_amd.discretization import FirstOrderUpwind, LimitedUpwind
+from aquaflux_amd.discretization import FirstOrderUpwind, LimitedUpwind, TransientTerm
 from aquaflux_amd.mesh import perturbed_grid_2d, tet_grid_3d
 from aquaflux_amd.properties import FieldProperty
 from aquaflux_amd.schemes import (
@@ -73,10 +73,18 @@ def seeded(mesh, seed=0):
     return phi, gamma
 
 
-def build_gpu(mesh, bcs, adv, grad, gamma, device, node_coords=None):
+def history(mesh, seed=0):
+    """Two earlier time levels near phi (the transient cases)."""
+    rng = np.random.default_rng(100 + seed)
+    C = mesh.n_cells
+    return rng.uniform(0.0, 1.0, C), rng.uniform(0.0, 1.0, C)
+
+
+def build_gpu(mesh, bcs, adv, grad, gamma, device, node_coords=None, transient=False):
     geom = mesh.geometry(node_coords=node_coords, device=device)
     st = ScalarTransport.build(mesh, geom, FieldProperty(torch.tensor(gamma, dtype=F64, device=device)),
-                               BoundaryConditions(bcs), ADV[adv], gradient_scheme=GRAD[grad])
+                               BoundaryConditions(bcs), ADV[adv], gradient_scheme=GRAD[grad],
+                               transient=TransientTerm() if transient else None)
     return geom, st
 
 

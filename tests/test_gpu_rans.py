@@ -208,3 +208,86 @@ def test_rans_autograd_through_inlet_velocity(cuda):
     Ro = RC.oracle_residual(oset, st, torch.full((C,), 1e-4, dtype=F64), torch.ones(C, dtype=F64), "limited", u_b=ub)
     (go,) = torch.autograd.grad((Ro[:C] ** 2).sum(), u0)
     assert float((g.cpu() - go).abs().max() / go.abs().max()) < TOL_D
+
+
+# --- LogScalars: k and / or omega solved in log form ---------------------------------------------
+LOG_CASES = [
+    ("channel2d_sst", "limited", "limited", "lsq", 1e-5, False, True),
+    ("channel3d_sst", "limited", "limited", "lsq", 1e-4, True, True),
+    ("cavity_sst", "upwind", "upwind", "gg", 1e-2, True, False),
+]
+
+
+def _lids(c):
+    return _ids(c[:5]) + ("-logk" if c[5] else "") + ("-logw" if c[6] else "")
+
+
+def _log_setup(case, cuda, seed):
+    name, fs, ss, gr, nu, kl, wl = case
+    print(f"case {_lids(case)} seed {seed}", flush=True)
+    mesh, rans, oset = RC.build(name, fs, ss, gr, nu=nu, device=cuda, k_log=kl, omega_log=wl)
+    kind = "cavity" if name.startswith("cavity") else "channel"
+    st = RC.to_solved(torch.tensor(RC.seeded_state(mesh, seed, kind)), mesh.dim, kl, wl)
+    return mesh, rans, oset, st
+
+
+@pytest.mark.parametrize("case", LOG_CASES, ids=_lids)
+def test_rans_log_residual_jvp_vjp_match_oracle(case, cuda):
+    mesh, rans, oset, st = _log_setup(case, cuda, seed=5)
+    kl, wl = case[5], case[6]
+    C, nb = mesh.n_cells, mesh.dim + 3
+    mu = torch.full((C,), case[4], dtype=F64)
+    rho = torch.ones(C, dtype=F64)
+
+    def f(z):
+        return RC.oracle_residual(oset, z, mu, rho, case[2], k_log=kl, omega_log=wl)
+
+    ref = f(st)
+    got = rans.residual(st.to(cuda))
+    errs = RC.block_errors(got, ref, nb)
+    print("log residual block errors", errs)
+    assert max(errs) < TOL_R
+    rng = np.random.default_rng(21)
+    v = torch.tensor(rng.normal(size=st.numel()))
+    _, ref_t = torch.func.jvp(f, (st,), (v,))
+    R, dR = rans.residual_jvp(st.to(cuda), v.to(cuda))
+    errs = RC.block_errors(dR, ref_t, nb)
+    print("log jvp block errors", errs)
+    assert max(errs) < TOL_D
+    lam = torch.tensor(rng.normal(size=st.numel()))
+    x = st.clone().requires_grad_(True)
+    (gx,) = torch.autograd.grad(f(x), x, lam)
+    bar = rans.residual_vjp(st.to(cuda), lam.to(cuda))
+    errs = RC.block_errors(bar, gx, nb)
+    print("log vjp block errors", errs)
+    assert max(errs) < TOL_D
+    R1, b1 = rans._raw.residual_vjp(rans._detached(st.to(cuda)), lam.to(cuda))
+    assert torch.equal(b1[0], bar)
+    assert max(RC.block_errors(R1, ref, nb)) < TOL_R
+
+
+@pytest.mark.parametrize("case", LOG_CASES[:1], ids=_lids)
+def test_rans_log_omega_keeps_transport_rows_and_linear_fixation(case, cuda):
+    """Under log(omega) the transport rows equal the direct residual at the same physical state
+    (test_coupled_rans.py:547-570) and the wall fixation rows have unit slope in the solved unknown
+    (fixed_value.py:96-124)."""
+    mesh, rans, oset, st = _log_setup(case, cuda, seed=6)
+    d, C = mesh.dim, mesh.n_cells
+    direct_mesh, direct, _ = RC.build(*case[:4], nu=case[4], device=cuda)
+    phys = rans.physical_fields(st.to(cuda))
+    st_direct = direct.state_from_physical(*phys)
+    Rl = rans.residual(st.to(cuda)).reshape(d + 3, C)
+    Rd = direct.residual(st_direct).reshape(d + 3, C)
+    wall = oset[3].wall_cells
+    interior = torch.ones(C, dtype=torch.bool)
+    interior[wall] = False
+    # (the physical states agree to an ulp of exp(log(.)), hence rounding-level tolerances)
+    assert max(RC.block_errors(Rl[: d + 2], Rd[: d + 2], d + 2)) < TOL_R
+    scale = float(Rd[d + 2][interior].abs().max())
+    assert float((Rl[d + 2][interior] - Rd[d + 2][interior]).abs().max()) < TOL_R * scale
+    # d R_w / d w_omega on a wall cell is exactly 1 (log-ratio row), checked through the JVP
+    e = torch.zeros((d + 3) * C, dtype=F64, device=cuda)
+    cw = int(wall[0])
+    e[(d + 2) * C + cw] = 1.0
+    _, Je = rans.residual_jvp(st.to(cuda), e)
+    assert abs(float(Je[(d + 2) * C + cw]) - 1.0) < 1e-14

@@ -195,3 +195,123 @@ def test_scalar_node_adjoint_finite_difference(cuda):
     fd = (4.0 * cd(5e-5) - cd(1e-4)) / 3.0
     print("fd", fd, "adjoint", an)
     assert abs(fd - an) <= 1e-7 * max(abs(an), 1e-12)
+
+
+# --- transient (BDF1 first step, BDF2 afterwards) ----------------------------------------------
+TRANSIENT_CASES = [
+    ("tet3", "limited", "gg", True),
+    ("tet3", "limited", "gg", False),
+    ("quad_mixed", "upwind", "lsq", False),
+    ("quad_mixed", "limited", "cgg", True),
+    ("tet3", "none", "none", False),
+]
+DT = 0.037
+
+
+def _tids(c):
+    return "-".join(c[:3]) + ("-bdf1" if c[3] else "-bdf2")
+
+
+def _transient_setup(case, cuda, seed):
+    name, adv, grad, first = case
+    print(f"case {_tids(case)} seed {seed}", flush=True)
+    mesh, u, bcs = SC.make(name)
+    phi, gamma = SC.seeded(mesh, seed)
+    old, older = SC.history(mesh, seed)
+    m = fv.OMesh.from_mesh(mesh)
+    g = fv.geometry(m)
+    s = SC.oracle_setup(mesh, bcs, adv, grad)
+    geom, st = SC.build_gpu(mesh, bcs, adv, grad, gamma, cuda, transient=True)
+    return mesh, u, phi, gamma, old, older, first, (m, g, s), geom, st
+
+
+@pytest.mark.parametrize("case", TRANSIENT_CASES, ids=_tids)
+def test_scalar_transient_residual_matches_oracle(case, cuda):
+    mesh, u, phi, gamma, old, older, first, (m, g, s), geom, st = _transient_setup(case, cuda, 0)
+    T = torch.tensor
+    ref = fv.scalar_residual(m, g, s, T(phi), SC.oracle_flux(g, u), T(gamma), T(old), T(older), DT, first)
+    fn = st.residual(SC.gpu_flux(geom, u))
+    got = fn(T(phi, device=cuda), T(old, device=cuda), T(older, device=cuda) if not first else None, DT, first)
+    err = rel(got, ref)
+    print("transient residual err", err)
+    assert err < TOL_R
+    # the steady residual plus the accumulation term alone
+    steady = fv.scalar_residual(m, g, s, T(phi), SC.oracle_flux(g, u), T(gamma))
+    acc = fv.transient_term(T(phi), T(old), T(older), DT, first, g.volume)
+    assert rel(got, steady + acc) < TOL_R
+
+
+@pytest.mark.parametrize("case", TRANSIENT_CASES, ids=_tids)
+def test_scalar_transient_jvp_matches_oracle(case, cuda):
+    """Tangent with respect to phi and both history levels (forward AD through the autograd
+    Function)."""
+    mesh, u, phi, gamma, old, older, first, (m, g, s), geom, st = _transient_setup(case, cuda, 1)
+    rng = np.random.default_rng(11)
+    C = mesh.n_cells
+    tp, to, too = (torch.tensor(rng.normal(size=C)) for _ in range(3))
+    flux_o = SC.oracle_flux(g, u)
+    T = torch.tensor
+
+    def f(ph, po, poo):
+        return fv.scalar_residual(m, g, s, ph, flux_o, T(gamma), po, poo, DT, first)
+
+    _, ref = torch.func.jvp(f, (T(phi), T(old), T(older)), (tp, to, too if not first else torch.zeros(C)))
+    fn = st.residual(SC.gpu_flux(geom, u).detach())
+    import torch.autograd.forward_ad as fwAD
+    with fwAD.dual_level():
+        ph = fwAD.make_dual(T(phi, device=cuda), tp.to(cuda))
+        po = fwAD.make_dual(T(old, device=cuda), to.to(cuda))
+        poo = None if first else fwAD.make_dual(T(older, device=cuda), too.to(cuda))
+        dR = fwAD.unpack_dual(fn(ph, po, poo, DT, first)).tangent
+    err = rel(dR, ref)
+    print("transient jvp err", err)
+    assert err < TOL_D
+
+
+@pytest.mark.parametrize("case", TRANSIENT_CASES, ids=_tids)
+def test_scalar_transient_vjp_matches_oracle(case, cuda):
+    mesh, u, phi, gamma, old, older, first, (m, g, s), geom, st = _transient_setup(case, cuda, 2)
+    rng = np.random.default_rng(12)
+    lam = torch.tensor(rng.normal(size=mesh.n_cells))
+    ph, po, poo = (torch.tensor(x, requires_grad=True) for x in (phi, old, older))
+    R = fv.scalar_residual(m, g, s, ph, SC.oracle_flux(g, u), torch.tensor(gamma), po, poo, DT, first)
+    refs = torch.autograd.grad(R, (ph, po, poo), lam, allow_unused=True)
+    T = torch.tensor
+    xs = [T(phi, device=cuda, requires_grad=True), T(old, device=cuda, requires_grad=True),
+          None if first else T(older, device=cuda, requires_grad=True)]
+    Rg = st.residual(SC.gpu_flux(geom, u).detach())(xs[0], xs[1], xs[2], DT, first)
+    got = torch.autograd.grad(Rg, [x for x in xs if x is not None], lam.to(cuda))
+    for name, gv, ref in zip(("phi", "phi_old", "phi_older"), got, refs):
+        err = rel(gv, ref)
+        print(name, err)
+        assert err < TOL_D, name
+    # the raw entry returns the same cotangents (and None for phi_older on the first step)
+    bars, _ = st.residual_vjp(SC.gpu_flux(geom, u), xs[0].detach(), lam.to(cuda), phi_old=xs[1].detach(),
+                              phi_older=None if first else xs[2].detach(), dt=DT, first_step=first)
+    assert rel(bars[0], refs[0]) < TOL_D and rel(bars[5], refs[1]) < TOL_D
+    assert (bars[6] is None) == first
+
+
+@pytest.mark.parametrize("first", [True, False], ids=["bdf1", "bdf2"])
+def test_scalar_transient_node_adjoint_matches_oracle(first, cuda):
+    """The accumulation term's volume enters the node-coordinate adjoint."""
+    mesh, u, bcs = SC.make("tet3")
+    phi, gamma = SC.seeded(mesh, 3)
+    old, older = SC.history(mesh, 3)
+    rng = np.random.default_rng(5)
+    lam = torch.tensor(rng.normal(size=mesh.n_cells))
+    T = torch.tensor
+    Xo = torch.tensor(mesh.node_coords, dtype=F64, requires_grad=True)
+    m = fv.OMesh.from_mesh(mesh, node_coords=Xo)
+    g = fv.geometry(m)
+    s = SC.oracle_setup(mesh, bcs, "limited", "gg")
+    R = fv.scalar_residual(m, g, s, T(phi), SC.oracle_flux(g, u), T(gamma), T(old), T(older), DT, first)
+    (ref,) = torch.autograd.grad((R * lam).sum(), Xo)
+    X = torch.tensor(mesh.node_coords, dtype=F64, device=cuda, requires_grad=True)
+    geom, st = SC.build_gpu(mesh, bcs, "limited", "gg", gamma, cuda, node_coords=X, transient=True)
+    Rg = st.residual(SC.gpu_flux(geom, u))(T(phi, device=cuda), T(old, device=cuda), T(older, device=cuda), DT,
+                                           first)
+    (got,) = torch.autograd.grad((Rg * lam.to(cuda)).sum(), X)
+    err = rel(got, ref)
+    print("transient node adjoint err", err)
+    assert err < TOL_D

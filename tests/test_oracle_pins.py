@@ -442,3 +442,81 @@ def test_corrected_green_gauss_pins():
     direct = torch.linalg.solve(A, fv.corrected_rhs(ms, gs, phis, bvs).reshape(-1)).reshape(C, 2)
     swept = fv.corrected_green_gauss(ms, gs, phis, bvs, sweeps=60)
     assert float((swept - direct).abs().max() / direct.abs().max()) < 1e-12
+
+
+# --- transient accumulation: tests/unit/test_transient.py -------------------------------------
+
+
+def test_transient_term_known_answers():
+    """test_transient.py:9-43 inputs: V = (2, 4), phi = (1, 2), phi_old = (0.5, 1), phi_older =
+    (0, 0.5), dt = 0.1 give BDF1 (10, 40) and BDF2 (10, 50); BDF1 ignores phi_older; a constant
+    history vanishes."""
+    V = torch.tensor([2.0, 4.0], dtype=F64)
+    phi, old, older = (torch.tensor(x, dtype=F64) for x in ([1.0, 2.0], [0.5, 1.0], [0.0, 0.5]))
+    assert torch.allclose(fv.transient_term(phi, old, older, 0.1, True, V), torch.tensor([10.0, 40.0], dtype=F64))
+    assert torch.allclose(fv.transient_term(phi, old, older, 0.1, False, V), torch.tensor([10.0, 50.0], dtype=F64))
+    assert torch.equal(fv.transient_term(phi, old, older, 0.1, True, V),
+                       fv.transient_term(phi, old, 99.0 * older, 0.1, True, V))
+    c = torch.full((2,), 3.0, dtype=F64)
+    assert torch.all(fv.transient_term(c, c, c, 0.1, False, V) == 0.0)
+
+
+def _order(errors):
+    return min(float(np.log2(errors[i] / errors[i + 1])) for i in range(len(errors) - 1))
+
+
+def test_transient_term_orders_of_accuracy():
+    """test_transient.py:46-101: the stencils estimate d/dt exp(t) at t = 0 to first (BDF1) and
+    second (BDF2) order; a BDF2 branch replaced by BDF1 fails the second bound."""
+    one = torch.ones(1, dtype=F64)
+
+    def err(dt, first):
+        r = fv.transient_term(one, np.exp(-dt) * one, np.exp(-2 * dt) * one, dt, first, one)
+        return abs(float(r[0]) - 1.0)
+
+    dts = [0.1, 0.05, 0.025, 0.0125]
+    assert 0.9 < _order([err(d, True) for d in dts]) < 1.1
+    assert _order([err(d, False) for d in dts]) > 1.9
+
+
+def test_transient_decay_march_is_second_order():
+    """test_transient.py:104-134: dphi/dt = -2 phi over [0, 1] marched BDF1 then BDF2 (each step
+    linear in phi_new, solved from the stencil's own slope and offset) converges at second order."""
+    rate, one = 2.0, torch.ones(1, dtype=F64)
+
+    def march(n):
+        dt = 1.0 / n
+        phi, older = one.clone(), one.clone()
+        for k in range(n):
+            first = k == 0
+            rest = fv.transient_term(0 * one, phi, older, dt, first, one)
+            slope = fv.transient_term(one, phi, older, dt, first, one) - rest
+            phi, older = -rest / (slope + rate), phi
+        return float(phi[0])
+
+    errors = [abs(march(n) - np.exp(-rate)) for n in (10, 20, 40, 80)]
+    assert _order(errors) > 1.9
+
+
+def test_scalar_transient_residual_adds_accumulation():
+    """residual.py:141-147: R_transient = R_steady + V (BDF stencil) / dt on a perturbed quad mesh,
+    and the accumulation cotangent with respect to phi is lambda V c0 / dt."""
+    from tests import scalar_cases as SC
+    mesh, u, bcs = SC.make("quad_mixed")
+    m, g = _geom(mesh)
+    s = SC.oracle_setup(mesh, bcs, "limited", "gg")
+    phi, gam = (torch.tensor(x) for x in SC.seeded(mesh, 0))
+    old, older = (torch.tensor(x) for x in SC.history(mesh, 0))
+    flux = SC.oracle_flux(g, u)
+    steady = fv.scalar_residual(m, g, s, phi, flux, gam)
+    for first, c0 in ((True, 1.0), (False, 1.5)):
+        R = fv.scalar_residual(m, g, s, phi, flux, gam, old, older, 0.25, first)
+        acc = (R - steady) * 0.25 / g.volume
+        want = phi - old if first else 1.5 * phi - 2.0 * old + 0.5 * older
+        assert torch.allclose(acc, want, rtol=0, atol=1e-12)
+        lam = torch.randn(mesh.n_cells, dtype=F64, generator=torch.Generator().manual_seed(1))
+        p = phi.clone().requires_grad_(True)
+        (gs,) = torch.autograd.grad(fv.scalar_residual(m, g, s, p, flux, gam), p, lam)
+        p = phi.clone().requires_grad_(True)
+        (gt,) = torch.autograd.grad(fv.scalar_residual(m, g, s, p, flux, gam, old, older, 0.25, first), p, lam)
+        assert torch.allclose(gt - gs, lam * g.volume * c0 / 0.25, rtol=0, atol=1e-12)

@@ -167,3 +167,93 @@ def test_max_min_tangent_rule():
     x = torch.tensor([0.0, -1.0, 2.0], dtype=F64, requires_grad=True)
     (gx,) = torch.autograd.grad(fv.maximum(x, 0.0).sum(), x)
     assert gx.tolist() == [0.5, 0.0, 1.0]
+
+
+# --- LogScalars / LogRatioRow: tests/unit/test_coupled_rans.py:531-595, test_fixed_value.py ------
+
+
+def _healthy(C, rng):
+    return np.concatenate([0.1 * rng.normal(size=2 * C), 0.1 * rng.normal(size=C), 0.05 + 0.01 * rng.uniform(size=C),
+                           10.0 + rng.uniform(size=C)])
+
+
+def test_log_omega_keeps_transport_rows_and_changes_only_the_fixation():
+    """test_coupled_rans.py:547-570: at the same physical fields every transport row of the
+    log-omega residual equals the direct one; only the wall fixation rows differ, and both kinds
+    vanish where omega equals its wall value."""
+    mesh, m, g, s, t = cavity_rans()
+    C = m.n_cells
+    phys = torch.tensor(_healthy(C, np.random.default_rng(3)))
+    mu, rho = torch.full((C,), 1e-2, dtype=F64), torch.ones(C, dtype=F64)
+    solved = phys.clone()
+    solved[4 * C:] = torch.log(phys[4 * C:])
+    Rd = S.rans_residual(m, g, s, t, phys, mu, rho, "upwind")
+    Rl = S.rans_residual(m, g, s, t, solved, mu, rho, "upwind", omega_log=True)
+    wall = torch.zeros(C, dtype=torch.bool)
+    wall[t.wall_cells] = True
+    keep = torch.ones(5 * C, dtype=torch.bool)
+    keep[4 * C:][wall] = False
+    scale = float(Rd.abs().max())
+    assert float((Rd[keep] - Rl[keep]).abs().max()) < 1e-12 * scale
+    # rows of the wall cells: direct omega - target vs log(omega / target)
+    target = phys[4 * C:][wall] - Rd[4 * C:][wall]
+    assert torch.allclose(Rl[4 * C:][wall], torch.log(phys[4 * C:][wall] / target), rtol=1e-12, atol=0)
+    # set omega to its wall value on the wall cells: both forms vanish there
+    phys2 = phys.clone()
+    phys2[4 * C:][wall] = target
+    sol2 = phys2.clone()
+    sol2[4 * C:] = torch.log(phys2[4 * C:])
+    r_d = S.rans_residual(m, g, s, t, phys2, mu, rho, "upwind")[4 * C:][wall]
+    r_l = S.rans_residual(m, g, s, t, sol2, mu, rho, "upwind", omega_log=True)[4 * C:][wall]
+    assert float(r_d.abs().max()) < 1e-12 * float(target.abs().max()) and float(r_l.abs().max()) < 1e-12
+
+
+def test_log_fixation_row_has_unit_slope_and_log_k_chain_rule():
+    """fixed_value.py:96-124: d log(e^w / t) / dw = 1 on every wall cell; transport rows under
+    log k pick up the chain factor d(e^w)/dw = k (coupled.py:205-207) -- the JVP along a log-k
+    direction equals the direct JVP along k * direction."""
+    mesh, m, g, s, t = cavity_rans()
+    C = m.n_cells
+    rng = np.random.default_rng(4)
+    phys = torch.tensor(_healthy(C, rng))
+    mu, rho = torch.full((C,), 1e-2, dtype=F64), torch.ones(C, dtype=F64)
+    solved = phys.clone()
+    solved[3 * C:] = torch.log(phys[3 * C:])
+    f_log = lambda z: S.rans_residual(m, g, s, t, z, mu, rho, "upwind", k_log=True, omega_log=True)  # noqa: E731
+    f_dir = lambda z: S.rans_residual(m, g, s, t, z, mu, rho, "upwind")  # noqa: E731
+    wc = t.wall_cells
+    e = torch.zeros(5 * C, dtype=F64)
+    e[4 * C + wc] = 1.0
+    _, je = torch.func.jvp(f_log, (solved,), (e,))
+    assert torch.allclose(je[4 * C + wc], torch.ones(wc.numel(), dtype=F64), rtol=0, atol=1e-13)
+    v = torch.zeros(5 * C, dtype=F64)
+    v[3 * C:4 * C] = torch.tensor(rng.normal(size=C))
+    _, jl = torch.func.jvp(f_log, (solved,), (v,))
+    vk = v.clone()
+    vk[3 * C:4 * C] = v[3 * C:4 * C] * phys[3 * C:4 * C]
+    _, jd = torch.func.jvp(f_dir, (phys,), (vk,))
+    keep = torch.ones(5 * C, dtype=torch.bool)
+    keep[4 * C + wc] = False  # the fixation rows differ in form (log ratio vs difference)
+    assert float((jl[keep] - jd[keep]).abs().max()) < 1e-11 * float(jd.abs().max())
+
+
+def test_fixation_rows_known_answers():
+    """test_fixed_value.py / test_coupled_rans.py:573-595 on the shipped row strategies: same root,
+    the log row's slope in w = log(phi) is one, the difference row's is phi."""
+    from aquaflux_amd.discretization import DifferenceRow, LogRatioRow
+    from aquaflux_amd.turbulence import DirectScalars, LogScalars
+
+    phi = torch.tensor([1.0, 5.0, 1.0e4], dtype=F64)
+    target = torch.tensor([1.0, 5.0, 5.0], dtype=F64)
+    d, lr = DifferenceRow().row(phi, target), LogRatioRow().row(phi, target)
+    assert torch.equal(d == 0.0, lr == 0.0) and bool(torch.all(lr[:2] == 0.0))
+    w = torch.log(phi).requires_grad_(True)
+    (slope,) = torch.autograd.grad(LogRatioRow().row(torch.exp(w), target).sum(), w)
+    assert torch.allclose(slope, torch.ones(3, dtype=F64))
+    (slope_d,) = torch.autograd.grad(DifferenceRow().row(torch.exp(w), target).sum(), w)
+    assert torch.allclose(slope_d, phi)
+    assert torch.allclose(LogRatioRow().jacobian_scale(phi, phi), torch.ones(3, dtype=F64))
+    ww = torch.tensor([-3.0, 0.0, 2.5], dtype=F64)
+    assert torch.equal(DirectScalars().to_physical(ww), ww)
+    assert torch.allclose(LogScalars().to_solved(LogScalars().to_physical(ww)), ww)
+    assert isinstance(LogScalars().fixation_row(), LogRatioRow)
