@@ -28,6 +28,7 @@ class AfxGeometry(ctypes.Structure):
         ("face_centroid", c_void_p),
         ("volume", c_void_p),
         ("cell_centroid", c_void_p),
+        ("face_pack", c_void_p),
     ]
 
 
@@ -167,6 +168,8 @@ _SIGNATURES = {
     "afx_mesh_destroy": [c_void_p],
     "afx_mesh_info": [c_void_p, c_void_p],
     "afx_mesh_cell_faces": [c_void_p, c_void_p, c_void_p, c_void_p],
+    "afx_mesh_face_order": [c_void_p, c_void_p, c_void_p, c_void_p],
+    "afx_geometry_pack": [c_void_p, ctypes.POINTER(AfxGeometry), c_void_p, c_void_p],
     "afx_geometry_compute": [c_void_p, c_void_p, ctypes.POINTER(AfxGeometry), c_void_p],
     "afx_flow_workspace_bytes": [c_void_p, ctypes.c_int, c_int64_p],
     "afx_flow_residual": [

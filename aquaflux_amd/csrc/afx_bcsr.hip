@@ -108,13 +108,13 @@ __global__ void __launch_bounds__(256) k_flow_jac(MeshView m, JacArgs a, int32_t
     } else {
       dual ub[D];
 #pragma unroll
-      for (int i = 0; i < D; ++i) ub[i] = dual{a.ub[(int64_t)i * m.F + r.f], 0.0};
-      const dual pb{a.pb[r.f], 0.0};
-      const dual gb = a.gB ? dual{a.gB[r.f], 0.0} : X.mu;
+      for (int i = 0; i < D; ++i) ub[i] = dual{a.ub[(int64_t)i * m.F + r.fu], 0.0};
+      const dual pb{a.pb[r.fu], 0.0};
+      const dual gb = a.gB ? dual{a.gB[r.fu], 0.0} : X.mu;
       for (int v = 0; v < NV; ++v) {
         seed<D>(X, v, 1.0);
         dual out[NV];
-        flow_flux_boundary<D, ADV, dual>(q, a.kind[r.f], X, ub, pb, a.gB ? gb : X.mu, out);
+        flow_flux_boundary<D, ADV, dual>(q, a.kind[r.fu], X, ub, pb, a.gB ? gb : X.mu, out);
         seed<D>(X, v, 0.0);
 #pragma unroll
         for (int rr = 0; rr < NV; ++rr) diag[rr][v] += out[rr].d;
@@ -313,7 +313,8 @@ extern "C" int afx_flow_jacobian(const afx_mesh* m, const afx_geometry* g, const
   if (rc) return rc;
   JacArgs a{cfg->face_kind, in->state, in->u_b, in->p_b, in->gamma_b, in->mu, in->rho, G, aP, psi,
             cfg->pressure_pin >= 0 ? (int32_t)cfg->pressure_pin : -1};
-  MeshView v = afx_view(m, g);
+  MeshView v;
+  if (int rv_ = afx_rview(m, g, s, &v)) return rv_;
   const int32_t nslot = 1 + m->max_faces;
 #define AFX_JAC(DD, AA, TT) \
   AFX_LAUNCH(kKBcsrAssemble, (k_flow_jac<DD, AA, TT>), grid_for(m->C), dim3(kBlock), 0, s, v, a, nslot, (TT*)vals, cols, tslot)

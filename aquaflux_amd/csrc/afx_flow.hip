@@ -162,12 +162,12 @@ __global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Ou
         }
       }
     } else {
-      const int kind = a.kind[r.f];
+      const int kind = a.kind[r.fu];
       if (GRAD == 0) {
         const double A = m.area[r.f];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-          S phi = flow_bval<D, S>(a, kind, v, m.F, r.f, xc);
+          S phi = flow_bval<D, S>(a, kind, v, m.F, r.fu, xc);
 #pragma unroll
           for (int j = 0; j < D; ++j) acc[v][j] += phi * (m.nrm[(int64_t)j * m.F + r.f] * A);
         }
@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Ou
           for (int q2 = 0; q2 < D; ++q2) M[p][q2] += wgt * dx[p] * dx[q2];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-          S dphi = flow_bval<D, S>(a, kind, v, m.F, r.f, xc) - xc[v];
+          S dphi = flow_bval<D, S>(a, kind, v, m.F, r.fu, xc) - xc[v];
 #pragma unroll
           for (int j = 0; j < D; ++j) acc[v][j] += (wgt * dx[j]) * dphi;
         }
@@ -280,8 +280,8 @@ __global__ void __launch_bounds__(256) k_flow_ap(MeshView m, FlowArgs<S> a, Out<
         ap_interior<D, ADV, S>(q, O, X, oc, nc);
       acc += r.own ? oc : nc;
     } else {
-      S gb = a.gB.v ? a.gB[r.f] : X.mu;
-      acc += ap_boundary<D, ADV, S>(q, a.kind[r.f], X, gb);
+      S gb = a.gB.v ? a.gB[r.fu] : X.mu;
+      acc += ap_boundary<D, ADV, S>(q, a.kind[r.fu], X, gb);
     }
   }
   aP.set(c, acc);
@@ -315,9 +315,9 @@ __global__ void __launch_bounds__(256) k_flow_res(MeshView m, FlowArgs<S> a, Out
     } else {
       S ub[D];
 #pragma unroll
-      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.f];
-      S gb = a.gB.v ? a.gB[r.f] : X.mu;
-      flow_flux_boundary<D, ADV, S>(q, a.kind[r.f], X, ub, a.pb[r.f], gb, out);
+      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.fu];
+      S gb = a.gB.v ? a.gB[r.fu] : X.mu;
+      flow_flux_boundary<D, ADV, S>(q, a.kind[r.fu], X, ub, a.pb[r.fu], gb, out);
     }
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = r.own ? acc[v] + out[v] : acc[v] - out[v];
@@ -379,26 +379,20 @@ __global__ void __launch_bounds__(256) k_flow_res_adj(MeshView m, FlowArgs<doubl
         adj_flux_interior<D, ADV>(q, X, O, lf, 0, B);
       else
         adj_flux_interior<D, ADV>(q, O, X, lf, 1, B);
-      if (r.own) {
-#pragma unroll
-        for (int i = 0; i < D; ++i) bar.ub[(int64_t)i * m.F + r.f] = 0.0;
-        bar.pb[r.f] = 0.0;
-        if (bar.gB) bar.gB[r.f] = 0.0;
-      }
     } else {
       double lf[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) lf[v] = v == D ? lc_cont : lc[v];
       double ub[D], ubb[D], pbb, gbb;
 #pragma unroll
-      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.f];
-      const double gb = a.gB.v ? a.gB[r.f] : X.mu;
-      adj_flux_boundary<D, ADV>(q, a.kind[r.f], X, ub, a.pb[r.f], gb, lf, B, ubb, pbb, gbb);
+      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.fu];
+      const double gb = a.gB.v ? a.gB[r.fu] : X.mu;
+      adj_flux_boundary<D, ADV>(q, a.kind[r.fu], X, ub, a.pb[r.fu], gb, lf, B, ubb, pbb, gbb);
 #pragma unroll
-      for (int i = 0; i < D; ++i) bar.ub[(int64_t)i * m.F + r.f] = ubb[i];
-      bar.pb[r.f] = pbb;
+      for (int i = 0; i < D; ++i) bar.ub[(int64_t)i * m.F + r.fu] = ubb[i];
+      bar.pb[r.fu] = pbb;
       if (a.gB.v) {
-        if (bar.gB) bar.gB[r.f] = gbb;
+        if (bar.gB) bar.gB[r.fu] = gbb;
       } else {
         B.mu += gbb;
       }
@@ -458,11 +452,11 @@ __global__ void __launch_bounds__(256) k_flow_ap_adj(MeshView m, FlowArgs<double
       else
         adj_ap_interior<D, ADV>(q, O, X, apb_o, apb_c, 1, B);
     } else {
-      const double gb = a.gB.v ? a.gB[r.f] : X.mu;
+      const double gb = a.gB.v ? a.gB[r.fu] : X.mu;
       double gbb;
-      adj_ap_boundary<D, ADV>(q, a.kind[r.f], X, gb, apb_c, B, gbb);
+      adj_ap_boundary<D, ADV>(q, a.kind[r.fu], X, gb, apb_c, B, gbb);
       if (a.gB.v) {
-        if (bar.gB) bar.gB[r.f] += gbb;
+        if (bar.gB) bar.gB[r.fu] += gbb;
       } else {
         B.mu += gbb;
       }
@@ -629,7 +623,7 @@ __global__ void __launch_bounds__(256) k_flow_grad_adj(MeshView m, FlowArgs<doub
         }
       }
     } else {
-      const int kind = a.kind[r.f];
+      const int kind = a.kind[r.fu];
       double phib[NV];
       if (GRAD == 0) {
         const double A = m.area[r.f];
@@ -658,14 +652,14 @@ __global__ void __launch_bounds__(256) k_flow_grad_adj(MeshView m, FlowArgs<doub
 #pragma unroll
       for (int i = 0; i < D; ++i) {
         if (kind == kFlowWall || kind == kFlowInlet)
-          bar.ub[(int64_t)i * m.F + r.f] += phib[i];
+          bar.ub[(int64_t)i * m.F + r.fu] += phib[i];
         else if (kind == kFlowOutlet)
           xb[i] += phib[i];
       }
       if (kind == kFlowWall || kind == kFlowInlet)
         xb[D] += phib[D];
       else if (kind == kFlowOutlet)
-        bar.pb[r.f] += phib[D];
+        bar.pb[r.fu] += phib[D];
     }
   }
 #pragma unroll
@@ -755,7 +749,8 @@ template <int D, class S, int ADV, int GRAD>
 int run_forward(const afx_mesh* mh, const afx_geometry* g, FlowArgs<S> a, Out<S> R, double* G, double* Gt,
                 double* aP, double* aPt, double* psi, double* psit, int32_t* ext, double* cgg, int sweeps,
                 double relax, hipStream_t s) {
-  MeshView m = afx_view(mh, g);
+  MeshView m;
+  if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
   Out<S> oG{G, Gt}, oP{psi, psit}, oA{aP, aPt};
   constexpr int GK = GRAD == kGradCorrected ? 0 : GRAD;  // the compact pass supplies V^-1 b
   if (ADV == kAdvLimited && GRAD != kGradCorrected)
@@ -797,13 +792,18 @@ int run_forward(const afx_mesh* mh, const afx_geometry* g, FlowArgs<S> a, Out<S>
 template <int D, int ADV, int GRAD>
 int run_vjp(const afx_mesh* mh, const afx_geometry* g, FlowArgs<double> a, const double* lam, FlowBars bar,
             const Work& w, int sweeps, double relax, hipStream_t s) {
-  MeshView m = afx_view(mh, g);
+  MeshView m;
+  if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
   int rc = run_forward<D, double, ADV, GRAD>(mh, g, a, Out<double>{nullptr, nullptr}, w.G, nullptr, w.aP, nullptr,
                                              w.psi, nullptr, w.ext, w.cgg, sweeps, relax, s);
   if (rc) return rc;
   a.G.v = w.G;
   a.aP.v = w.aP;
   a.psi.v = w.psi;
+  // boundary-value cotangents: zero over all faces here, boundary entries written by the kernel
+  if (int rz = afx_fill_zero(bar.ub, (int64_t)D * m.F, s)) return rz;
+  if (int rz = afx_fill_zero(bar.pb, m.F, s)) return rz;
+  if (int rz = afx_fill_zero(bar.gB, m.F, s)) return rz;
   AFX_LAUNCH(kKFlowResAdj, (k_flow_res_adj<D, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, lam, bar);
   AFX_CHECK_LAUNCH();
   AFX_LAUNCH(kKFlowApAdj, (k_flow_ap_adj<D, ADV, GRAD>), grid_for(m.C), dim3(kBlock), 0, s, m, a, w.ext, bar);

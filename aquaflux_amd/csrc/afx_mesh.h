@@ -21,10 +21,25 @@ struct afx_mesh {
   double* normal_raw = nullptr; // [d][F] scratch: winding-order normals
   double* scratch = nullptr;    // lazily grown scratch of entry points without a workspace argument
   int64_t scratch_n = 0;
+  // Internal face order of the residual kernels: faces sorted by owner (stable), so the faces a
+  // run of consecutive cells visits are contiguous.  Null when the caller's order already is.
+  int32_t* fu = nullptr;        // [F] internal -> caller face index
+  int32_t* owner_i = nullptr;   // [F] owner / neighbour in internal order
+  int32_t* nb_i = nullptr;
+  int32_t* cface_i = nullptr;   // [n_cface] cell -> face list over internal face indices
+  double* pack_scratch = nullptr;  // [(2d+1)][F] face geometry gathered for a call without face_pack
 };
 
 // Device scratch of at least n doubles owned by the handle (grown on demand, freed with it);
 // nullptr on allocation failure.  Not for concurrent use from several host threads.
 double* afx_mesh_scratch(afx_mesh* m, int64_t n);
 
+// Zero n doubles on `s` with a fill kernel (capturable in a graph, unlike a memset node).
+int afx_fill_zero(double* p, int64_t n, hipStream_t s);
+
+// Caller-order view (the geometry kernels).
 afx::MeshView afx_view(const afx_mesh* m, const afx_geometry* g);
+
+// Internal-order view of the residual kernels: internal topology and face geometry, from
+// g->face_pack or (when that is null) gathered on `s` into the handle's pack scratch.
+int afx_rview(const afx_mesh* m, const afx_geometry* g, hipStream_t s, afx::MeshView* out);

@@ -52,6 +52,29 @@ extern "C" int afx_mesh_create(int dim, int64_t n_cells, int64_t n_faces, int64_
     if (neighbour[f] >= 0) cface[fill[neighbour[f]]++] = ~(int32_t)f;
   }
   for (int64_t c = 0; c < n_cells; ++c) max_faces = std::max(max_faces, count[c + 1] - count[c]);
+  // internal face order: stable sort by owner (skipped when the caller's order already is)
+  bool sorted = true;
+  for (int64_t f = 1; f < n_faces && sorted; ++f) sorted = owner[f] >= owner[f - 1];
+  std::vector<int32_t> fu, owner_i, nb_i, cface_i;
+  if (!sorted) {
+    fu.resize(n_faces);
+    std::vector<int32_t> ostart(n_cells + 1, 0);
+    for (int64_t f = 0; f < n_faces; ++f) ostart[owner[f] + 1]++;
+    for (int64_t c = 0; c < n_cells; ++c) ostart[c + 1] += ostart[c];
+    for (int64_t f = 0; f < n_faces; ++f) fu[ostart[owner[f]]++] = (int32_t)f;
+    owner_i.resize(n_faces);
+    nb_i.resize(n_faces);
+    for (int64_t i = 0; i < n_faces; ++i) {
+      owner_i[i] = owner[fu[i]];
+      nb_i[i] = neighbour[fu[i]];
+    }
+    cface_i.resize(count[n_cells]);
+    std::vector<int32_t> fill_i(count.begin(), count.end() - 1);
+    for (int64_t i = 0; i < n_faces; ++i) {
+      cface_i[fill_i[owner_i[i]]++] = (int32_t)i;
+      if (nb_i[i] >= 0) cface_i[fill_i[nb_i[i]]++] = ~(int32_t)i;
+    }
+  }
   // node -> incidence CSR (for the node-coordinate adjoint gather)
   int64_t n_inc = face_node_offsets[n_faces];
   std::vector<int32_t> ncount(n_nodes + 1, 0);
@@ -98,6 +121,12 @@ extern "C" int afx_mesh_create(int dim, int64_t n_cells, int64_t n_faces, int64_
   if (e == hipSuccess) e = up((void**)&m->ninc, ninc.data(), n_inc * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&m->approx, (size_t)n_cells * dim * 8);
   if (e == hipSuccess) e = hipMalloc((void**)&m->normal_raw, (size_t)n_faces * dim * 8);
+  if (!sorted) {
+    if (e == hipSuccess) e = up((void**)&m->fu, fu.data(), n_faces * 4);
+    if (e == hipSuccess) e = up((void**)&m->owner_i, owner_i.data(), n_faces * 4);
+    if (e == hipSuccess) e = up((void**)&m->nb_i, nb_i.data(), n_faces * 4);
+    if (e == hipSuccess) e = up((void**)&m->cface_i, cface_i.data(), cface_i.size() * 4);
+  }
   if (e != hipSuccess) {
     afx_set_error(std::string("afx_mesh_create: ") + hipGetErrorString(e));
     afx_mesh_destroy(m);
@@ -109,8 +138,9 @@ extern "C" int afx_mesh_create(int dim, int64_t n_cells, int64_t n_faces, int64_
 
 extern "C" int afx_mesh_destroy(afx_mesh* m) {
   if (!m) return 0;
-  void* ptrs[] = {m->owner, m->nb,  m->cptr,   m->cface,      m->foff,       m->fidx,
-                  m->inc_face, m->nptr, m->ninc, m->approx, m->normal_raw, m->scratch};
+  void* ptrs[] = {m->owner,   m->nb,      m->cptr,       m->cface,   m->foff, m->fidx,   m->inc_face,
+                  m->nptr,    m->ninc,    m->approx,     m->normal_raw, m->scratch, m->fu, m->owner_i,
+                  m->nb_i,    m->cface_i, m->pack_scratch};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete m;
@@ -559,7 +589,91 @@ afx::MeshView afx_view(const afx_mesh* m, const afx_geometry* g) {
   v.fc = g ? g->face_centroid : nullptr;
   v.vol = g ? g->volume : nullptr;
   v.cc = g ? g->cell_centroid : nullptr;
+  v.fu = nullptr;
   return v;
+}
+
+namespace {
+
+// Face geometry in internal order: pack[(0 | 1+j | 1+D+j) F + i] = (area, normal_j, centroid_j)[fu[i]].
+template <int D>
+__global__ void __launch_bounds__(256) k_pack_faces(int32_t F, const int32_t* __restrict__ fu,
+                                                     const double* __restrict__ area, const double* __restrict__ nrm,
+                                                     const double* __restrict__ fc, double* __restrict__ pack) {
+  const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= F) return;
+  const int64_t u = fu[i];
+  pack[i] = area[u];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    pack[(int64_t)(1 + j) * F + i] = nrm[(int64_t)j * F + u];
+    pack[(int64_t)(1 + D + j) * F + i] = fc[(int64_t)j * F + u];
+  }
+}
+
+int pack_faces(const afx_mesh* m, const afx_geometry* g, double* pack, hipStream_t s) {
+  if (m->dim == 2)
+    k_pack_faces<2><<<grid_for(m->F), dim3(kBlock), 0, s>>>(m->F, m->fu, g->area, g->normal, g->face_centroid, pack);
+  else
+    k_pack_faces<3><<<grid_for(m->F), dim3(kBlock), 0, s>>>(m->F, m->fu, g->area, g->normal, g->face_centroid, pack);
+  AFX_CHECK_LAUNCH();
+  return 0;
+}
+
+}  // namespace
+
+namespace {
+__global__ void __launch_bounds__(256) k_fill_zero(double* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+}  // namespace
+
+int afx_fill_zero(double* p, int64_t n, hipStream_t s) {
+  if (!p || n <= 0) return 0;
+  k_fill_zero<<<grid_for(n), dim3(kBlock), 0, s>>>(p, n);
+  AFX_CHECK_LAUNCH();
+  return 0;
+}
+
+int afx_rview(const afx_mesh* m, const afx_geometry* g, hipStream_t s, afx::MeshView* out) {
+  AFX_REQUIRE(m && g && out, "internal view: null argument");
+  afx::MeshView v = afx_view(m, g);
+  if (m->fu) {
+    const double* pack = g->face_pack;
+    if (!pack) {
+      afx_mesh* mm = const_cast<afx_mesh*>(m);
+      if (!mm->pack_scratch) AFX_CHECK_HIP(hipMalloc((void**)&mm->pack_scratch, (size_t)(2 * m->dim + 1) * m->F * 8));
+      int rc = pack_faces(m, g, mm->pack_scratch, s);
+      if (rc) return rc;
+      pack = mm->pack_scratch;
+    }
+    v.owner = m->owner_i;
+    v.nb = m->nb_i;
+    v.cface = m->cface_i;
+    v.fu = m->fu;
+    v.area = pack;
+    v.nrm = pack + m->F;
+    v.fc = pack + (int64_t)(1 + m->dim) * m->F;
+  }
+  *out = v;
+  return 0;
+}
+
+extern "C" int afx_geometry_pack(const afx_mesh* m, const afx_geometry* g, double* face_pack, hipStream_t s) {
+  AFX_REQUIRE(m && g && face_pack, "afx_geometry_pack: null argument");
+  AFX_REQUIRE(g->area && g->normal && g->face_centroid, "afx_geometry_pack: face geometry pointer missing");
+  if (!m->fu) return 0;  // the caller's face order is the internal one: nothing to pack
+  return pack_faces(m, g, face_pack, s);
+}
+
+extern "C" int afx_mesh_face_order(const afx_mesh* m, int32_t* permuted, int32_t* face_of_internal,
+                                   hipStream_t s) {
+  AFX_REQUIRE(m && permuted, "afx_mesh_face_order: null argument");
+  *permuted = m->fu ? 1 : 0;
+  if (m->fu && face_of_internal)
+    AFX_CHECK_HIP(hipMemcpyAsync(face_of_internal, m->fu, (size_t)m->F * 4, hipMemcpyDeviceToDevice, s));
+  return 0;
 }
 
 extern "C" int afx_geometry_compute(const afx_mesh* m, const double* node_coords, afx_geometry* g,
@@ -594,6 +708,7 @@ extern "C" int afx_geometry_compute(const afx_mesh* m, const double* node_coords
     AFX_LAUNCH(kKGeomVolume, k_cell_volume<3>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx, vol, cc);
     AFX_CHECK_LAUNCH();
   }
+  if (g->face_pack && m->fu) return pack_faces(m, g, const_cast<double*>(g->face_pack), s);
   return 0;
 }
 

@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
         const double A = m.area[r.f];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-          S phi = rans_bval_lead<D, S>(a, v, m.F, r.f, xc);
+          S phi = rans_bval_lead<D, S>(a, v, m.F, r.fu, xc);
 #pragma unroll
           for (int j = 0; j < D; ++j) acc[v][j] += phi * (m.nrm[(int64_t)j * m.F + r.f] * A);
         }
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
           for (int q2 = 0; q2 < D; ++q2) M[p][q2] += wgt * dx[p] * dx[q2];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-          S dphi = rans_bval_lead<D, S>(a, v, m.F, r.f, xc) - xc[v];
+          S dphi = rans_bval_lead<D, S>(a, v, m.F, r.fu, xc) - xc[v];
 #pragma unroll
           for (int j = 0; j < D; ++j) acc[v][j] += (wgt * dx[j]) * dphi;
         }
@@ -283,11 +283,13 @@ AFX_DEV S wall_shear_rate(const RansArgs<S>& a, const MeshView& m, int32_t c, do
   for (int v = 0; v <= D; ++v) xc[v] = a.x[(int64_t)v * m.C + c];
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
     const int32_t e = m.cface[k];
-    if (e < 0 || !a.wallf[e]) continue;
+    if (e < 0) continue;
+    const int32_t eu = user_face(m, e);
+    if (!a.wallf[eu]) continue;
     S s2 = sconst<S>(0.0);
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      S sl = xc[j] - rans_bval_lead<D, S>(a, j, m.F, e, xc);
+      S sl = xc[j] - rans_bval_lead<D, S>(a, j, m.F, eu, xc);
       s2 = j == 0 ? sl * sl : s2 + sl * sl;
     }
     const double A = m.area[e];
@@ -389,8 +391,8 @@ __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<
         ap_interior<D, ADV, S>(q, O, X, oc, nc);
       acc += r.own ? oc : nc;
     } else {
-      const int kind = a.kind[r.f];
-      acc += ap_boundary<D, ADV, S>(q, kind, X, rans_gamma_b<S>(a, kind, r.f, c, X.mu));
+      const int kind = a.kind[r.fu];
+      acc += ap_boundary<D, ADV, S>(q, kind, X, rans_gamma_b<S>(a, kind, r.fu, c, X.mu));
     }
   }
   aP.set(c, acc);
@@ -425,9 +427,9 @@ __global__ void __launch_bounds__(256) k_rans_fres(MeshView m, RansArgs<S> a, Ou
     } else {
       S ub[D];
 #pragma unroll
-      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.f];
-      const int kind = a.kind[r.f];
-      flow_flux_boundary<D, ADVF, S>(q, kind, X, ub, a.pb[r.f], rans_gamma_b<S>(a, kind, r.f, c, X.mu), out);
+      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.fu];
+      const int kind = a.kind[r.fu];
+      flow_flux_boundary<D, ADVF, S>(q, kind, X, ub, a.pb[r.fu], rans_gamma_b<S>(a, kind, r.fu, c, X.mu), out);
     }
     if (r.own) mdot.set(r.f, out[D]);
 #pragma unroll
@@ -460,11 +462,11 @@ AFX_DEV void rans_scalar_face(const RansArgs<S>& a, const MeshView& m, const Fac
     }
   } else {
     const S zero = sconst<S>(0.0);
-    const S bk = scalar_bval<D, S>(a.kbc[r.f], q, Kc.phi, Kc.G, a.kval[r.f], zero, zero);
+    const S bk = scalar_bval<D, S>(a.kbc[r.fu], q, Kc.phi, Kc.G, a.kval[r.fu], zero, zero);
     S gk = Kc.gam;
-    if (a.wallf[r.f]) gk = (1.0 - wall_function_weight<S>(sc, a.nu[r.P], dP, Kc.phi)) * Kc.gam;
+    if (a.wallf[r.fu]) gk = (1.0 - wall_function_weight<S>(sc, a.nu[r.P], dP, Kc.phi)) * Kc.gam;
     fk = scalar_flux_boundary<D, ADVS, S>(q, Qf, Kc.phi, Kc.G, Kc.psi, bk, gk);
-    const S bw = scalar_bval<D, S>(a.wbc[r.f], q, Wc.phi, Wc.G, a.wval[r.f], zero, zero);
+    const S bw = scalar_bval<D, S>(a.wbc[r.fu], q, Wc.phi, Wc.G, a.wval[r.fu], zero, zero);
     fw = scalar_flux_boundary<D, ADVS, S>(q, Qf, Wc.phi, Wc.G, Wc.psi, bw, Wc.gam);
   }
 }
@@ -641,16 +643,16 @@ __global__ void __launch_bounds__(256) k_rans_sres_adj(MeshView m, RansArgs<doub
       }
     } else {
       // k face
-      const double bk = scalar_bval<D, double>(a.kbc[r.f], q, Kc.phi, Kc.G, a.kval[r.f], 0.0, 0.0);
+      const double bk = scalar_bval<D, double>(a.kbc[r.fu], q, Kc.phi, Kc.G, a.kval[r.fu], 0.0, 0.0);
       double wgt = 0.0, gk = Kc.gam;
-      if (a.wallf[r.f]) {
+      if (a.wallf[r.fu]) {
         wgt = wall_function_weight<double>(sc, a.nu[c], dP, Kc.phi);
         gk = (1.0 - wgt) * Kc.gam;
       }
       double bkb, gkb, vb;
       Qb = adj_scalar_flux_boundary<D, ADVS>(q, Qf, Kc.phi, Kc.G, Kc.psi, bk, gk, lck, B.k, bkb, gkb);
-      adj_scalar_bval<D>(a.kbc[r.f], q, bkb, B.k, vb);
-      if (a.wallf[r.f]) {
+      adj_scalar_bval<D>(a.kbc[r.fu], q, bkb, B.k, vb);
+      if (a.wallf[r.fu]) {
         B.k.gam += (1.0 - wgt) * gkb;
         const double wb = -Kc.gam * gkb;
         dual wk = wall_function_weight<dual>(sc, dual{a.nu[c], 0.0}, dP, dual{Kc.phi, 1.0});
@@ -661,10 +663,10 @@ __global__ void __launch_bounds__(256) k_rans_sres_adj(MeshView m, RansArgs<doub
         B.k.gam += gkb;
       }
       // omega face
-      const double bw = scalar_bval<D, double>(a.wbc[r.f], q, Wc.phi, Wc.G, a.wval[r.f], 0.0, 0.0);
+      const double bw = scalar_bval<D, double>(a.wbc[r.fu], q, Wc.phi, Wc.G, a.wval[r.fu], 0.0, 0.0);
       double bwb, gwb;
       Qb += adj_scalar_flux_boundary<D, ADVS>(q, Qf, Wc.phi, Wc.G, Wc.psi, bw, Wc.gam, lcw, B.w, bwb, gwb);
-      adj_scalar_bval<D>(a.wbc[r.f], q, bwb, B.w, vb);
+      adj_scalar_bval<D>(a.wbc[r.fu], q, bwb, B.w, vb);
       B.w.gam += gwb;
       if (R) {
         acck += scalar_flux_boundary<D, ADVS, double>(q, Qf, Kc.phi, Kc.G, Kc.psi, bk, gk);
@@ -781,26 +783,21 @@ __global__ void __launch_bounds__(256) k_rans_fres_adj(MeshView m, RansArgs<doub
         adj_flux_interior<D, ADVF>(q, X, O, lf, 0, B.fl);
       else
         adj_flux_interior<D, ADVF>(q, O, X, lf, 1, B.fl);
-      if (r.own) {
-#pragma unroll
-        for (int i = 0; i < D; ++i) bar.ub[(int64_t)i * m.F + r.f] = 0.0;
-        bar.pb[r.f] = 0.0;
-      }
     } else {
-      const int kind = a.kind[r.f];
+      const int kind = a.kind[r.fu];
       double ub[D];
 #pragma unroll
-      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.f];
-      const double gB = rans_gamma_b<double>(a, kind, r.f, c, X.mu);
+      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.fu];
+      const double gB = rans_gamma_b<double>(a, kind, r.fu, c, X.mu);
 #pragma unroll
       for (int v = 0; v < NV; ++v) lf[v] = v == D ? lc_cont : lc[v];
       lf[D] += mdot_bar[r.f];
       double ubb[D], pbb, gbb;
-      adj_flux_boundary<D, ADVF>(q, kind, X, ub, a.pb[r.f], gB, lf, B.fl, ubb, pbb, gbb);
-      adj_gamma_b<D>(a, kind, r.f, c, gbb, B);
+      adj_flux_boundary<D, ADVF>(q, kind, X, ub, a.pb[r.fu], gB, lf, B.fl, ubb, pbb, gbb);
+      adj_gamma_b<D>(a, kind, r.fu, c, gbb, B);
 #pragma unroll
-      for (int i = 0; i < D; ++i) bar.ub[(int64_t)i * m.F + r.f] = ubb[i];
-      bar.pb[r.f] = pbb;
+      for (int i = 0; i < D; ++i) bar.ub[(int64_t)i * m.F + r.fu] = ubb[i];
+      bar.pb[r.fu] = pbb;
     }
   }
   if (c == a.pin) B.fl.p += lc[D];
@@ -848,11 +845,11 @@ __global__ void __launch_bounds__(256) k_rans_ap_adj(MeshView m, RansArgs<double
       else
         adj_ap_interior<D, ADV>(q, O, X, apb_o, apb_c, 1, B.fl);
     } else {
-      const int kind = a.kind[r.f];
-      const double gB = rans_gamma_b<double>(a, kind, r.f, c, X.mu);
+      const int kind = a.kind[r.fu];
+      const double gB = rans_gamma_b<double>(a, kind, r.fu, c, X.mu);
       double gbb;
       adj_ap_boundary<D, ADV>(q, kind, X, gB, apb_c, B.fl, gbb);
-      adj_gamma_b<D>(a, kind, r.f, c, gbb, B);
+      adj_gamma_b<D>(a, kind, r.fu, c, gbb, B);
     }
   }
 #pragma unroll
@@ -947,27 +944,29 @@ __global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<d
     double den = 0.0;
     for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
       const int32_t e = m.cface[k];
-      if (e >= 0 && a.wallf[e]) den += m.area[e];
+      if (e >= 0 && a.wallf[user_face(m, e)]) den += m.area[e];
     }
     for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
       const int32_t e = m.cface[k];
-      if (e < 0 || !a.wallf[e]) continue;
+      if (e < 0) continue;
+      const int32_t eu = user_face(m, e);
+      if (!a.wallf[eu]) continue;
       double sl[D], s2 = 0.0;
 #pragma unroll
       for (int j = 0; j < D; ++j) {
-        sl[j] = xcl[j] - rans_bval_lead<D, double>(a, j, m.F, e, xcl);
+        sl[j] = xcl[j] - rans_bval_lead<D, double>(a, j, m.F, eu, xcl);
         s2 += sl[j] * sl[j];
       }
       if (s2 <= 0.0) continue;
       const double rate_b = tau_b * m.area[e] / den;
       const double mag = sqrt(s2);
-      const int kind = a.kind[e];
+      const int kind = a.kind[eu];
 #pragma unroll
       for (int j = 0; j < D; ++j) {
         const double sb = rate_b / d * sl[j] / mag;
         xb[j] += sb;  // d slip / d u_P
         if (kind == kFlowWall || kind == kFlowInlet)
-          bar.ub[(int64_t)j * m.F + e] -= sb;
+          bar.ub[(int64_t)j * m.F + eu] -= sb;
         else if (kind == kFlowOutlet)
           xb[j] -= sb;
       }
@@ -1126,7 +1125,7 @@ __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<doub
         if (ext[(int64_t)(NL + l) * m.C + o] == c) xb[v] += bar.ext[(int64_t)(NL + l) * m.C + o];
       }
     } else {
-      const int kind = a.kind[r.f];
+      const int kind = a.kind[r.fu];
       double phib[NA];
       if (GRAD == 0) {
         const double A = m.area[r.f];
@@ -1154,16 +1153,16 @@ __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<doub
 #pragma unroll
       for (int i = 0; i < D; ++i) {
         if (kind == kFlowWall || kind == kFlowInlet)
-          bar.ub[(int64_t)i * m.F + r.f] += phib[i];
+          bar.ub[(int64_t)i * m.F + r.fu] += phib[i];
         else if (kind == kFlowOutlet)
           xb[i] += phib[i];
       }
       if (kind == kFlowWall || kind == kFlowInlet)
         xb[D] += phib[D];
       else if (kind == kFlowOutlet)
-        bar.pb[r.f] += phib[D];
-      if (a.kbc[r.f] == kBcZeroGradient) xb[D + 1] += phib[D + 1];
-      if (a.wbc[r.f] == kBcZeroGradient) xb[D + 2] += phib[D + 2];
+        bar.pb[r.fu] += phib[D];
+      if (a.kbc[r.fu] == kBcZeroGradient) xb[D + 1] += phib[D + 1];
+      if (a.wbc[r.fu] == kBcZeroGradient) xb[D + 2] += phib[D + 2];
     }
   }
 #pragma unroll
@@ -1357,7 +1356,8 @@ void bind_closure(RansArgs<S>& a, double* cl, double* clt, int64_t C) {
 template <int D, class S, int ADVF, int ADVS, int GRAD>
 int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S> R, const RWork& w, double* Gt,
                  double* psit, double* aPt, double* clt, double* mdott, int32_t* ext, bool mdot_only, hipStream_t s) {
-  MeshView m = afx_view(mh, g);
+  MeshView m;
+  if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
   const int64_t C = m.C;
   Out<S> oG{w.G, Gt}, oP{w.psi, psit};
   constexpr bool CG = GRAD == kGradCorrected;
@@ -1523,7 +1523,8 @@ int rans_vjp_impl(const afx_mesh* m, const afx_geometry* g, const afx_rans_confi
                                                                   nullptr, nullptr, nullptr, nullptr, nullptr, w.ext,
                                                                   true, s);
                      if (r2) return r2;
-                     MeshView v = afx_view(m, g);
+                     MeshView v;
+                     if (int rv_ = afx_rview(m, g, s, &v)) return rv_;
                      RansArgs<double> b = a;
                      b.G.v = w.G;
                      b.psi.v = w.psi;
@@ -1532,6 +1533,9 @@ int rans_vjp_impl(const afx_mesh* m, const afx_geometry* g, const afx_rans_confi
                      AFX_LAUNCH(kKSstResAdj, (k_rans_sres_adj<D, AS>), grid_for(m->C), dim3(kBlock), 0, s, v, b,
                                 w.mdot, cotangent, bar, w.mdotb, residual);
                      AFX_CHECK_LAUNCH();
+                     // boundary-value cotangents: zero over all faces, boundary entries written below
+                     if (int rz = afx_fill_zero(bar.ub, (int64_t)D * m->F, s)) return rz;
+                     if (int rz = afx_fill_zero(bar.pb, m->F, s)) return rz;
                      AFX_LAUNCH(kKRansFlowResAdj, (k_rans_fres_adj<D, AF>), grid_for(m->C), dim3(kBlock), 0, s, v, b,
                                 cotangent, w.mdotb, bar);
                      AFX_CHECK_LAUNCH();

@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(256) k_sc_grad(MeshView m, ScArgs<S> a, Out<S>
     } else if (GRAD != kGradNone) {
       FaceGeo<D> q;
       load_face_geo<D>(m, r.f, r.P, r.N, q);
-      const S pb = scalar_bval<D, S>(a.kind[r.f], q, xc, zero, a.val[r.f], a.gam[c], a.tinf[r.f]);
+      const S pb = scalar_bval<D, S>(a.kind[r.fu], q, xc, zero, a.val[r.fu], a.gam[c], a.tinf[r.fu]);
       if (GRAD == 0) {
 #pragma unroll
         for (int j = 0; j < D; ++j) acc[j] += pb * (q.n[j] * q.A);
@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(256) k_sc_res(MeshView m, ScArgs<S> a, Out<S> 
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
-    const S Qf = a.flux[r.f];
+    const S Qf = a.flux[r.fu];
     S fl;
     if (o >= 0) {
       const S xo = a.phi[o], go = a.gam[o];
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(256) k_sc_res(MeshView m, ScArgs<S> a, Out<S> 
       else
         fl = scalar_flux_interior<D, ADV, S>(q, Qf, xo, xc, Go, Gc, po, pc, go, gc);
     } else {
-      const S bv = scalar_bval<D, S>(a.kind[r.f], q, xc, Gc, a.val[r.f], gc, a.tinf[r.f]);
+      const S bv = scalar_bval<D, S>(a.kind[r.fu], q, xc, Gc, a.val[r.fu], gc, a.tinf[r.fu]);
       fl = scalar_flux_boundary<D, ADV, S>(q, Qf, xc, Gc, pc, bv, gc);
     }
     acc = r.own ? acc + fl : acc - fl;
@@ -521,7 +521,7 @@ __global__ void __launch_bounds__(256) k_sc_res_adj(MeshView m, ScArgs<double> a
     load_face_geo<D>(m, r.f, r.P, r.N, q);
     FaceGeoBar<D> qb;
     qb.zero();
-    const double Qf = a.flux[r.f];
+    const double Qf = a.flux[r.fu];
     if (o >= 0) {
       const double xo = a.phi[o], go = a.gam[o];
       const double po = ADV == kAdvLimited ? a.psi[o] : 1.0;
@@ -536,22 +536,22 @@ __global__ void __launch_bounds__(256) k_sc_res_adj(MeshView m, ScArgs<double> a
         Qb = adj_sc_interior<D, ADV, GEO>(q, Qf, xo, xc, Go, Gc, po, pc, go, gc, lf, 1, B, qb);
       if (GEO) adj_face_geo<D>(q, true, qb);
       if (r.own) {
-        bar.flux[r.f] = Qb;
-        bar.val[r.f] = 0.0;
-        bar.tinf[r.f] = 0.0;
-        write_face_geo_bar<D, GEO>(bar, m.F, r.f, qb, false);
+        bar.flux[r.fu] = Qb;
+        bar.val[r.fu] = 0.0;
+        bar.tinf[r.fu] = 0.0;
+        write_face_geo_bar<D, GEO>(bar, m.F, r.fu, qb, false);
       }
 #pragma unroll
       for (int j = 0; j < D; ++j) ccb[j] += r.own ? qb.xP[j] : qb.xN[j];
     } else {
       double vb = 0.0, tb = 0.0;
-      const double Qb = adj_sc_boundary<D, ADV, GEO>(a.kind[r.f], q, Qf, xc, Gc, pc, gc, a.val[r.f], a.tinf[r.f],
+      const double Qb = adj_sc_boundary<D, ADV, GEO>(a.kind[r.fu], q, Qf, xc, Gc, pc, gc, a.val[r.fu], a.tinf[r.fu],
                                                      lc, B, vb, tb, qb);
       if (GEO) adj_face_geo<D>(q, false, qb);
-      bar.flux[r.f] = Qb;
-      bar.val[r.f] = vb;
-      bar.tinf[r.f] = tb;
-      write_face_geo_bar<D, GEO>(bar, m.F, r.f, qb, false);
+      bar.flux[r.fu] = Qb;
+      bar.val[r.fu] = vb;
+      bar.tinf[r.fu] = tb;
+      write_face_geo_bar<D, GEO>(bar, m.F, r.fu, qb, false);
 #pragma unroll
       for (int j = 0; j < D; ++j) ccb[j] += qb.xP[j];
     }
@@ -796,7 +796,7 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
         for (int j = 0; j < D; ++j) s += (q.n[j] * q.A) * Bc[j];
         phib = s;
         if (GEO) {
-          const double pbv = scalar_bval<D, double>(a.kind[r.f], q, xc, zero, a.val[r.f], a.gam[c], a.tinf[r.f]);
+          const double pbv = scalar_bval<D, double>(a.kind[r.fu], q, xc, zero, a.val[r.fu], a.gam[c], a.tinf[r.fu]);
 #pragma unroll
           for (int j = 0; j < D; ++j) qb.n[j] += pbv * q.A * Bc[j];
           double dn = 0.0;
@@ -815,11 +815,11 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
       ScalarBar<D> Bl;
       Bl.zero();
       double vb = 0.0, tb = 0.0;
-      adj_sc_bval<D, GEO>(a.kind[r.f], q, xc, zero, a.val[r.f], a.gam[c], a.tinf[r.f], phib, Bl, vb, tb, qb);
+      adj_sc_bval<D, GEO>(a.kind[r.fu], q, xc, zero, a.val[r.fu], a.gam[c], a.tinf[r.fu], phib, Bl, vb, tb, qb);
       xb += Bl.phi;
       gamb += Bl.gam;
-      bar.val[r.f] += vb;
-      bar.tinf[r.f] += tb;
+      bar.val[r.fu] += vb;
+      bar.tinf[r.fu] += tb;
       if (GEO) {
         adj_face_geo<D>(q, false, qb);
         have_geo = true;
@@ -836,7 +836,7 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
 #pragma unroll
           for (int j = 0; j < D; ++j) qb.xf[j] += bar.limx[(int64_t)j * m.C + r.N];
         }
-        write_face_geo_bar<D, GEO>(bar, m.F, r.f, qb, true);
+        write_face_geo_bar<D, GEO>(bar, m.F, r.fu, qb, true);
       }
       if (have_geo) {
 #pragma unroll
@@ -905,7 +905,7 @@ __global__ void __launch_bounds__(256) k_scalar_grad(MeshView m, const int8_t* _
         for (int j = 0; j < D; ++j) acc[j] += (wgt * dx[j]) * dphi;
       }
     } else {
-      const double pb = lead_bval(kind[r.f], bval[r.f], xc);
+      const double pb = lead_bval(kind[r.fu], bval[r.fu], xc);
       if (GRAD == 0) {
         const double A = m.area[r.f];
 #pragma unroll
@@ -1040,7 +1040,8 @@ int sc_dispatch(int D, int adv, int grad, Fn&& fn) {
 template <int D, class S, int ADV, int GRAD>
 int sc_forward(const afx_mesh* mh, const afx_geometry* g, ScArgs<S> a, Out<S> R, const ScWork& w, double* Gt,
                double* psit, int32_t* ext, int sweeps, double relax, hipStream_t s) {
-  MeshView m = afx_view(mh, g);
+  MeshView m;
+  if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
   constexpr bool LIM = ADV == kAdvLimited;
   constexpr int GK = GRAD == kGradCorrected ? 0 : GRAD;
   AFX_LAUNCH(kKScalarGrad, (k_sc_grad<D, S, GK, LIM && GRAD != kGradCorrected>), grid_for(m.C), dim3(kBlock), 0, s,
@@ -1095,7 +1096,8 @@ extern "C" int afx_scalar_gradient(const afx_mesh* mh, const afx_geometry* g, in
   AFX_REQUIRE(mh && g && kind && value && phi && grad, "afx_scalar_gradient: null argument");
   AFX_REQUIRE(gradient == 0 || gradient == 1 || gradient == 3, "afx_scalar_gradient: unknown gradient scheme");
   AFX_REQUIRE(gradient != 3 || relaxation > 0.0, "afx_scalar_gradient: relaxation must be positive");
-  MeshView m = afx_view(mh, g);
+  MeshView m;
+  if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
   if (m.C == 0) return 0;
   const int gk = gradient == 3 ? 0 : gradient;
 #define AFX_SG(DD, GG)                                                                                  \
@@ -1176,7 +1178,8 @@ extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const af
     int r2 = sc_forward<D, double, ADV, GR>(m, g, a, Out<double>{nullptr, nullptr}, w, nullptr, nullptr, w.ext,
                                             cfg->gradient_sweeps, cfg->gradient_relaxation, s);
     if (r2) return r2;
-    MeshView v = afx_view(m, g);
+    MeshView v;
+    if (int rv_ = afx_rview(m, g, s, &v)) return rv_;
     ScArgs<double> b = a;
     b.G.v = w.G;
     b.psi.v = w.psi;

@@ -50,6 +50,10 @@ class DeviceMesh:
         _lib.check(L.afx_mesh_info(handle, ctypes.addressof(info)), "afx_mesh_info")
         self.n_cell_face = int(info[4])
         self.max_faces = int(info[5])
+        permuted = ctypes.c_int32()
+        _lib.check(L.afx_mesh_face_order(handle, ctypes.byref(permuted), None, None), "afx_mesh_face_order")
+        # the residual kernels visit faces sorted by owner; geometry then carries a packed copy
+        self.face_permuted = bool(permuted.value)
         self._work = {}
 
     def __del__(self):
@@ -68,6 +72,17 @@ class DeviceMesh:
             buf = torch.empty(int(nbytes), dtype=torch.uint8, device=self.device)
             self._work[key] = buf
         return buf
+
+    def face_order(self):
+        """Caller face index of each internal face (int32 device tensor), or None when the mesh's
+        own face order is the internal one."""
+        if not self.face_permuted:
+            return None
+        fo = torch.empty(self.n_faces, dtype=torch.int32, device=self.device)
+        flag = ctypes.c_int32()
+        _lib.check(_lib.lib().afx_mesh_face_order(self.handle, ctypes.byref(flag), fo.data_ptr(),
+                                                  _lib.stream_of(self.device)), "afx_mesh_face_order")
+        return fo
 
     def cell_faces(self):
         """(cell_ptr, cell_faces) int32 device tensors of the cell -> face list."""

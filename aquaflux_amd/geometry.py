@@ -46,6 +46,26 @@ class MeshGeometry:
     face: FaceGeometry
     cell: CellGeometry
     node_coords_soa: torch.Tensor | None = None  # (d, N), the coordinates this was derived from
+    device_mesh: object = dataclasses.field(default=None, repr=False)
+    _pack: tuple | None = dataclasses.field(default=None, repr=False)
+
+    def face_pack(self):
+        """Face area, normal and centroid in the residual kernels' internal face order (None when
+        the mesh needs no permutation).  Rebuilt when a face tensor was modified in place."""
+        dm = self.device_mesh
+        if dm is None or not dm.face_permuted:
+            return None
+        f = self.face
+        key = (f.area.data_ptr(), f.normal_soa.data_ptr(), f.centroid_soa.data_ptr(), f.area._version,
+               f.normal_soa._version, f.centroid_soa._version)
+        if self._pack is None or self._pack[1] != key:
+            pack = torch.empty((2 * dm.dim + 1) * dm.n_faces, dtype=torch.float64, device=f.area.device)
+            gs = _lib.AfxGeometry(f.area.data_ptr(), f.normal_soa.data_ptr(), f.centroid_soa.data_ptr(), None, None,
+                                  None)
+            _lib.check(_lib.lib().afx_geometry_pack(dm.handle, ctypes.byref(gs), pack.data_ptr(),
+                                                    _lib.stream_of(f.area.device)), "afx_geometry_pack")
+            self._pack = (pack, key)
+        return self._pack[0]
 
     def c_struct(self) -> _lib.AfxGeometry:
         return _lib.AfxGeometry(
@@ -54,6 +74,7 @@ class MeshGeometry:
             self.face.centroid_soa.data_ptr(),
             self.cell.volume.data_ptr(),
             self.cell.centroid_soa.data_ptr(),
+            _lib.ptr(self.face_pack()),
         )
 
 
@@ -120,4 +141,4 @@ def compute_geometry(mesh, node_coords=None, device=None) -> MeshGeometry:
     area, nrm, fc, vol, cc = _GeometryFn.apply(dm, X)
     face = FaceGeometry(area=area, centroid_soa=fc, normal_soa=nrm)
     cell = CellGeometry(volume=vol, centroid_soa=cc)
-    return MeshGeometry(face=face, cell=cell, node_coords_soa=X)
+    return MeshGeometry(face=face, cell=cell, node_coords_soa=X, device_mesh=dm)

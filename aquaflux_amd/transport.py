@@ -79,6 +79,7 @@ class _ScalarRaw:
 
     def __init__(self, mesh, geometry, cfg, kind):
         self.mesh = mesh
+        self.geometry = geometry
         self.dm = mesh.device_mesh(geometry.face.area.device)
         self.cfg = cfg
         self.kind = kind
@@ -89,9 +90,13 @@ class _ScalarRaw:
             _lib.check(L.afx_scalar_workspace_bytes(self.dm.handle, mode, ctypes.byref(n)), "afx_scalar_workspace_bytes")
             self.work_bytes.append(int(n.value))
 
-    @staticmethod
-    def _gs(geo):
-        return _lib.AfxGeometry(*[t.data_ptr() for t in geo])
+    def _gs(self, geo):
+        """The C geometry of `geo`; the internal-order face pack comes from the bound geometry when
+        `geo` holds its face arrays (otherwise the library gathers one per call)."""
+        g = self.geometry
+        same = all(a.data_ptr() == b.data_ptr() for a, b in zip(geo[:3], (g.face.area, g.face.normal_soa,
+                                                                          g.face.centroid_soa)))
+        return _lib.AfxGeometry(*[t.data_ptr() for t in geo], _lib.ptr(g.face_pack() if same else None))
 
     def _cfg(self, step):
         """The configuration for one call: ``step`` is None (steady) or (bdf order, dt)."""

@@ -34,10 +34,11 @@ S8 = 8  # bytes per FP64 value
 STEP_BYTES_PER_FACE = {"sst": 1386.0}
 
 
-def _face_pass(D):
-    """One face pass: owner + neighbour (2 x int32), area, unit normal and centroid (FP64), and the
-    cell -> face list entries that reach the face from both sides (2 x int32)."""
-    return 2 * 4 + S8 * (1 + 2 * D) + 2 * 4
+def _face_pass(D, permuted=True):
+    """One face pass: owner + neighbour (2 x int32), area, unit normal and centroid (FP64), the
+    cell -> face list entries that reach the face from both sides (2 x int32) and, when the mesh's
+    faces are visited in the internal owner-sorted order, the caller index of the face (int32)."""
+    return 2 * 4 + S8 * (1 + 2 * D) + 2 * 4 + (4 if permuted else 0)
 
 
 def _flow_kernel_bytes(D=3, limited=True):
@@ -76,15 +77,30 @@ def _rans_kernel_bytes(D=3, limited=True):
     cell_sst = 5 * S8  # nu_t, F1, strain, cross-diffusion, wall shear rate
     res_in = state + G + psi + S8 + 3 * S8 + cell_sst + S8 + 1 + cgeo  # ... + a_P, mu_eff rho nu, d, flag
     bars10 = 10 * S8
+    NV = D + 1
+    flow_cell = NV * S8 + (D * D + D) * S8 + 2 * S8  # u, p, grad u, grad p, mu_eff, rho
     return {
         # launched twice per step (residual, and the recompute inside the VJP, which also writes
         # the extremum owners): the average of the two
         "rans_grad": dict(c=4 + state + cgeo + G + psi + ext / 2, f=fp, b=bnd),
         "rans_closure": dict(c=4 + closure_in + 7 * S8, f=0, b=0, w=D * S8 + D * S8, wf=1 + D * S8 + S8 + 4),
-        "rans_ap": dict(c=4 + D * S8 + D * D * S8 + 2 * S8 + cgeo + S8, f=fp, b=2, w=2 * S8),
-        "rans_res": dict(c=4 + res_in + state, f=fp, b=bnd, w=2 * S8),
-        "rans_res_adj": dict(c=4 + res_in + state + state + G + S8 + psi + bars10, f=fp + (D + 1) * S8, b=bnd,
-                             w=2 * S8),
+        "rans_ap": dict(c=4 + flow_cell + cgeo + S8, f=fp, b=2, w=2 * S8),
+        # flow rows of R (momentum, continuity) and the face mass flux (written once per face)
+        "rans_flow_res": dict(c=4 + flow_cell + S8 + (D * S8 if limited else 0) + cgeo + NV * S8, f=fp + S8,
+                              b=2 + NV * S8, w=2 * S8),
+        # k / omega rows: k, omega, their gradients and limiters, the closure (nu_t, F1, strain,
+        # cross-diffusion, wall shear rate), nu, d, wall flag, the cotangent rows; writes the k /
+        # omega state, gradient and limiter cotangents, six closure cotangents, the two R rows
+        # (fused residual + VJP) and the mass-flux cotangent; reads the mass flux
+        "rans_scalar_res_adj": dict(c=4 + 2 * S8 + 2 * D * S8 + (2 * S8 if limited else 0) + 7 * S8 + 1 + 2 * S8
+                                    + cgeo + 2 * S8 + 2 * D * S8 + (2 * S8 if limited else 0) + 6 * S8 + 2 * S8,
+                                    f=fp + 2 * S8, b=3 + 2 * S8),
+        # flow rows: the flow cell (+ a_P, limiters), the cotangent rows, the mass-flux cotangent;
+        # writes the flow state / gradient / a_P / limiter / mu_eff / rho / mu / wall-nu_t
+        # cotangents and the (D + 1) boundary-value cotangent arrays over all faces
+        "rans_flow_res_adj": dict(c=4 + flow_cell + S8 + (D * S8 if limited else 0) + NV * S8 + cgeo + NV * S8
+                                  + (D * D + D) * S8 + S8 + (D * S8 if limited else 0) + 4 * S8,
+                                  f=fp + S8 + NV * S8, b=2 + NV * S8, w=2 * S8),
         "rans_ap_adj": dict(c=4 + D * S8 + D * D * S8 + 4 * S8 + cgeo + S8 + 2 * D * S8 + 2 * D * D * S8
                             + 2 * 4 * S8, f=fp, b=2),
         "rans_closure_adj": dict(c=4 + closure_in + 2 * S8 + bars10 + 2 * G + psi + ext + 2 * state + 3 * S8

@@ -58,6 +58,12 @@ int afx_mesh_info(const afx_mesh* mesh, int64_t* out);
 /* Copies the cell -> face list (entries f when the cell owns f, ~f when it is f's neighbour). */
 int afx_mesh_cell_faces(const afx_mesh* mesh, int32_t* cell_ptr, int32_t* cell_faces, hipStream_t stream);
 
+/* The residual kernels visit faces in an internal order, the faces stably sorted by owner, so that
+ * consecutive cells read contiguous face records.  *permuted = 0 when the caller's order already is
+ * that order; otherwise 1, and face_of_internal[i] (nullable, [F] device) receives the caller's
+ * index of internal face i.  No reference counterpart (a layout choice of this library). */
+int afx_mesh_face_order(const afx_mesh* mesh, int32_t* permuted, int32_t* face_of_internal, hipStream_t stream);
+
 /* ------------------------------------------------------------------------------------------- */
 /* geometry                                                                                      */
 
@@ -67,12 +73,21 @@ typedef struct {
   const double* face_centroid; /* [d][F] */
   const double* volume;        /* [C]    */
   const double* cell_centroid; /* [d][C] */
+  const double* face_pack;     /* optional [(2d+1)][F]: area, normal, centroid in the internal face order
+                                  (afx_mesh_face_order), filled by afx_geometry_compute or
+                                  afx_geometry_pack; NULL makes every residual call gather it, and it
+                                  is unused when the mesh reports no permutation */
 } afx_geometry;
 
 /* Face/cell geometry from node coordinates node_coords[d][N] into the (writable) buffers of `out`.
  * Replaces Mesh.geometry / _fused_geometry (reference: aquaflux/mesh/mesh.py:341-387,
  * aquaflux/mesh/face.py:196-349, aquaflux/mesh/cell.py:84-170). */
 int afx_geometry_compute(const afx_mesh* mesh, const double* node_coords, afx_geometry* out, hipStream_t stream);
+
+/* Writes geom's face arrays in the internal face order into face_pack[(2d+1)][F] (a no-op when the
+ * mesh reports no permutation).  For geometry not produced by afx_geometry_compute.  No reference
+ * counterpart. */
+int afx_geometry_pack(const afx_mesh* mesh, const afx_geometry* geom, double* face_pack, hipStream_t stream);
 
 /* Cotangents of the five geometry arrays (written, not accumulated; NULL = not requested). */
 typedef struct {

@@ -157,3 +157,39 @@ def test_residual_is_deterministic(cuda):
     mesh, geom, mc = build("channel3d", "limited", "lsq", device=cuda)
     state = torch.tensor(seeded_state(mesh, 5, "channel"), dtype=F64, device=cuda)
     assert torch.equal(mc.residual(state), mc.residual(state))
+
+
+def test_internal_face_order_and_face_pack(cuda):
+    """The residual kernels' internal face order is the stable owner sort of the caller's faces; a
+    call without the geometry's face pack (the library gathers one) gives the same bits; a mesh
+    whose faces already are owner-sorted reports no permutation."""
+    import ctypes
+
+    from aquaflux_amd import _lib
+    from aquaflux_amd.mesh import tet_grid_3d
+
+    mesh, geom, mc = build("channel3d", "limited", "lsq", device=cuda)
+    dm = mesh.device_mesh(cuda)
+    assert dm.face_permuted
+    fo = dm.face_order().cpu().numpy()
+    assert np.array_equal(fo, np.argsort(mesh.owner, kind="stable"))
+    state = torch.tensor(seeded_state(mesh, 7, "channel"), dtype=F64, device=cuda)
+    raw = mc._raw
+    prim = [state] + [t.detach() for t in mc._inputs()[:4]]
+    R_pack = raw.residual(*prim)
+    gs_pack = raw.gs
+    assert gs_pack.face_pack
+    raw.gs = _lib.AfxGeometry(*[getattr(gs_pack, n) for n in ("area", "normal", "face_centroid", "volume",
+                                                                   "cell_centroid")], None)
+    try:
+        R_gather = raw.residual(*prim)
+    finally:
+        raw.gs = gs_pack
+    assert torch.equal(R_pack, R_gather)
+    tet = tet_grid_3d(3, perturb=0.1, seed=1)
+    tdm = tet.device_mesh(cuda)
+    assert not tdm.face_permuted and tdm.face_order() is None
+    assert tet.geometry(device=cuda).face_pack() is None
+    flag = ctypes.c_int32(7)
+    _lib.check(_lib.lib().afx_mesh_face_order(tdm.handle, ctypes.byref(flag), None, None), "afx_mesh_face_order")
+    assert flag.value == 0
