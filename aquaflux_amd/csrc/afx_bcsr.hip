@@ -86,7 +86,11 @@ __global__ void __launch_bounds__(256) k_flow_jac(MeshView m, JacArgs a, int32_t
     if (o >= 0) {
       FlowCell<D, dual> O;
       load_flow_cell<D>(a, m, o, ADV == kAdvLimited, O);
+      // fully unrolled so that `side` and `v` are constants: a runtime choice of X or O (or a
+      // runtime seed index) would place both cells in per-lane scratch memory
+#pragma unroll
       for (int side = 0; side < 2; ++side) {     // 0: seed own cell, 1: seed the other cell
+#pragma unroll
         for (int v = 0; v < NV; ++v) {
           FlowCell<D, dual>& S = side == 0 ? X : O;
           seed<D>(S, v, 1.0);
@@ -111,6 +115,7 @@ __global__ void __launch_bounds__(256) k_flow_jac(MeshView m, JacArgs a, int32_t
       for (int i = 0; i < D; ++i) ub[i] = dual{a.ub[(int64_t)i * m.F + r.fu], 0.0};
       const dual pb{a.pb[r.fu], 0.0};
       const dual gb = a.gB ? dual{a.gB[r.fu], 0.0} : X.mu;
+#pragma unroll
       for (int v = 0; v < NV; ++v) {
         seed<D>(X, v, 1.0);
         dual out[NV];
@@ -239,12 +244,15 @@ __global__ void __launch_bounds__(256) k_pbj_setup(int32_t C, const T* __restric
       bad = true;
       break;
     }
-    if (piv != col) {
+    // row swap over constant indices (a runtime row index would place `a` in scratch memory)
+#pragma unroll
+    for (int r = col + 1; r < B; ++r) {
+      const bool sw = r == piv;
 #pragma unroll
       for (int k = 0; k < 2 * B; ++k) {
-        double t = a[col][k];
-        a[col][k] = a[piv][k];
-        a[piv][k] = t;
+        const double t = a[col][k];
+        a[col][k] = sw ? a[r][k] : t;
+        a[r][k] = sw ? t : a[r][k];
       }
     }
     const double ip = 1.0 / a[col][col];

@@ -291,6 +291,27 @@ AFX_DEV int32_t xcd_thread_index() {
 }
 inline dim3 grid_for(int64_t n, int block = kBlock) { return dim3((unsigned)((n + block - 1) / block)); }
 
+// Cells in XCD-contiguous order over a bounded set of resident workgroups: workgroup b on XCD
+// x = b % 8 walks that XCD's logical blocks t = b / 8, b / 8 + g_x, ... (g_x = workgroups on XCD
+// x), so the cells in flight form a front advancing through each XCD's contiguous range instead of
+// the whole range at once.  With gridDim.x equal to the block count this is xcd_thread_index().
+template <class Fn>
+AFX_DEV void for_each_cell(int32_t n, Fn&& fn) {
+  const uint32_t bs = blockDim.x, nb = ((uint32_t)n + bs - 1) / bs;
+  const uint32_t G = gridDim.x, b = blockIdx.x;
+  const uint32_t q = nb / kNumXcd, r = nb % kNumXcd, x = b % kNumXcd, w = b / kNumXcd;
+  const uint32_t start = x * q + (x < r ? x : r), size = q + (x < r ? 1u : 0u);
+  const uint32_t gx = G / kNumXcd + (x < G % kNumXcd ? 1u : 0u);
+  for (uint32_t t = w; t < size; t += gx) {
+    const int32_t c = (int32_t)((start + t) * bs + threadIdx.x);
+    if (c < n) fn(c);
+  }
+}
+
+// Launch size for for_each_cell: every block when that is at most `resident_per_cu` workgroups per
+// compute unit, otherwise that many per compute unit (a multiple of the XCD count).
+dim3 cell_grid(int64_t n, int resident_per_cu);
+
 }  // namespace afx
 
 // ---------------------------------------------------------------------------------------------

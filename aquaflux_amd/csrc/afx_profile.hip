@@ -108,3 +108,20 @@ extern "C" int afx_profile_kernel_count(void) { return kKNumKernels; }
 extern "C" const char* afx_profile_kernel_name(int id) {
   return (id >= 0 && id < kKNumKernels) ? kKernelNames[id] : "";
 }
+
+namespace afx {
+dim3 cell_grid(int64_t n, int resident_per_cu) {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+      cus = v;
+    if (cus <= 0) cus = 256;
+  }
+  const int64_t nb = (n + kBlock - 1) / kBlock;
+  if (resident_per_cu <= 0) return dim3((unsigned)nb);
+  int64_t g = (int64_t)cus * resident_per_cu;
+  g = (g + kNumXcd - 1) / kNumXcd * kNumXcd;
+  return dim3((unsigned)(nb < g ? nb : g));
+}
+}  // namespace afx

@@ -371,8 +371,7 @@ AFX_DEV void load_scalar(const RansArgs<S>& a, const MeshView& m, int32_t c, int
 // pass D
 template <int D, class S, int ADV>
 __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<S> aP) {
-  const int32_t c = xcd_thread_index();
-  if (c >= m.C) return;
+  for_each_cell(m.C, [&](int32_t c) {
   FlowCell<D, S> X;
   load_flow<D, S>(a, m, c, false, false, X);
   S acc = sconst<S>(0.0);
@@ -396,6 +395,7 @@ __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<
     }
   }
   aP.set(c, acc);
+  });
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1405,7 +1405,11 @@ int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S
   AFX_LAUNCH(kKSstClosure, (k_rans_closure<D, S>), grid_for(C), dim3(kBlock), 0, s, m, a, co);
   AFX_CHECK_LAUNCH();
   bind_closure<S>(a, w.cl, clt, C);
-  AFX_LAUNCH(kKRansAp, (k_rans_ap<D, S, ADVF>), grid_for(C), dim3(kBlock), 0, s, m, a, Out<S>{w.aP, aPt});
+#ifndef AFX_WG_PER_CU
+#define AFX_WG_PER_CU 0
+#endif
+  AFX_LAUNCH(kKRansAp, (k_rans_ap<D, S, ADVF>), cell_grid(C, AFX_WG_PER_CU), dim3(kBlock), 0, s, m, a,
+             Out<S>{w.aP, aPt});
   AFX_CHECK_LAUNCH();
   if (R.v == nullptr && !mdot_only) return 0;
   a.aP.v = w.aP;

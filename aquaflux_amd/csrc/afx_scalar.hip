@@ -349,8 +349,14 @@ AFX_DEV double adj_sc_interior(const FaceGeo<D>& q, double Qf, double phiP, doub
     const bool mine = (side == 0) == outflow;
     if (mine) B.phi += fb;
     if (ADV == kAdvLimited) {
-      const double* xu = outflow ? q.xP : q.xN;
-      const double* Gu = outflow ? GP : GN;
+      // element-wise selects: a runtime choice between two arrays' addresses would place both
+      // in per-lane scratch memory
+      double xu[D], Gu[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        xu[j] = outflow ? q.xP[j] : q.xN[j];
+        Gu[j] = outflow ? GP[j] : GN[j];
+      }
       const double ps = outflow ? psiP : psiN;
       double t = 0.0;
 #pragma unroll
@@ -364,11 +370,12 @@ AFX_DEV double adj_sc_interior(const FaceGeo<D>& q, double Qf, double phiP, doub
         const double tb = fb * ps;
 #pragma unroll
         for (int j = 0; j < D; ++j) {
-          qb.xf[j] += tb * Gu[j];
-          if (outflow)
-            qb.xP[j] -= tb * Gu[j];
-          else
-            qb.xN[j] -= tb * Gu[j];
+          const double v = tb * Gu[j];
+          qb.xf[j] += v;
+          // both fields updated, one by an exact 0.0: a branch per field would be merged into a
+          // store through a selected address, which keeps qb out of registers
+          qb.xP[j] -= outflow ? v : 0.0;
+          qb.xN[j] -= outflow ? 0.0 : v;
         }
       }
     }
@@ -542,7 +549,7 @@ __global__ void __launch_bounds__(256) k_sc_res_adj(MeshView m, ScArgs<double> a
         write_face_geo_bar<D, GEO>(bar, m.F, r.fu, qb, false);
       }
 #pragma unroll
-      for (int j = 0; j < D; ++j) ccb[j] += r.own ? qb.xP[j] : qb.xN[j];
+      for (int j = 0; j < D; ++j) ccb[j] += (r.own ? qb.xP[j] : 0.0) + (r.own ? 0.0 : qb.xN[j]);
     } else {
       double vb = 0.0, tb = 0.0;
       const double Qb = adj_sc_boundary<D, ADV, GEO>(a.kind[r.fu], q, Qf, xc, Gc, pc, gc, a.val[r.fu], a.tinf[r.fu],
@@ -840,7 +847,7 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
       }
       if (have_geo) {
 #pragma unroll
-        for (int j = 0; j < D; ++j) ccb[j] += r.own ? qb.xP[j] : qb.xN[j];
+        for (int j = 0; j < D; ++j) ccb[j] += (r.own ? qb.xP[j] : 0.0) + (r.own ? 0.0 : qb.xN[j]);
       }
     }
   }
