@@ -273,6 +273,12 @@ AFX_DEV void sym_inverse(const double M[D][D], double I[D][D]) {
   }
 }
 
+#define AFX_PRAGMA_STR(x) #x
+#define AFX_UNROLL(n) _Pragma(AFX_PRAGMA_STR(unroll n))
+#ifndef AFX_FACE_UNROLL
+#define AFX_FACE_UNROLL 1
+#endif
+
 constexpr int kBlock = 256;
 constexpr int kNumXcd = 8;
 

@@ -17,12 +17,14 @@ struct afx_mesh {
   int32_t* inc_face = nullptr;  // [n_inc] face of each incidence
   int32_t* nptr = nullptr;      // [N+1] node -> incidence CSR
   int32_t* ninc = nullptr;      // [n_inc]
+  int32_t* kpos = nullptr;      // [n_inc] position of each incidence in the node-major list (ninc^-1)
   double* approx = nullptr;     // [d][C] scratch: approximate cell centroids
   double* normal_raw = nullptr; // [d][F] scratch: winding-order normals
   double* scratch = nullptr;    // lazily grown scratch of entry points without a workspace argument
   int64_t scratch_n = 0;
-  // Internal face order of the residual kernels: faces sorted by owner (stable), so the faces a
-  // run of consecutive cells visits are contiguous.  Null when the caller's order already is.
+  // Internal face order of the residual kernels (faces by rank among their owner's faces, then
+  // owner), so the faces a run of consecutive cells visits are consecutive records.  Null when the
+  // caller's faces are already sorted by owner.
   int32_t* fu = nullptr;        // [F] internal -> caller face index
   int32_t* owner_i = nullptr;   // [F] owner / neighbour in internal order
   int32_t* nb_i = nullptr;

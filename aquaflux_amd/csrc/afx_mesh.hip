@@ -52,16 +52,41 @@ extern "C" int afx_mesh_create(int dim, int64_t n_cells, int64_t n_faces, int64_
     if (neighbour[f] >= 0) cface[fill[neighbour[f]]++] = ~(int32_t)f;
   }
   for (int64_t c = 0; c < n_cells; ++c) max_faces = std::max(max_faces, count[c + 1] - count[c]);
-  // internal face order: stable sort by owner (skipped when the caller's order already is)
+  // Internal face order.  A caller order that is not sorted by owner (the structured generators
+  // number faces family by family, k fastest) is replaced by faces sorted by (rank among their
+  // owner's faces, owner, caller index): slot-r faces of consecutive cells are then consecutive
+  // records, so each face read of a wave -- the owner's, and one cell / row / plane away the
+  // neighbour's -- is contiguous.  Owner-sorted caller orders (tetrahedral and OpenFOAM-style
+  // meshes) are kept as they are.
   bool sorted = true;
   for (int64_t f = 1; f < n_faces && sorted; ++f) sorted = owner[f] >= owner[f - 1];
-  std::vector<int32_t> fu, owner_i, nb_i, cface_i;
+  std::vector<int32_t> fu;
   if (!sorted) {
-    fu.resize(n_faces);
-    std::vector<int32_t> ostart(n_cells + 1, 0);
+    std::vector<int32_t> rank(n_faces), by_owner(n_faces);
+    std::vector<int64_t> ostart(n_cells + 1, 0);
     for (int64_t f = 0; f < n_faces; ++f) ostart[owner[f] + 1]++;
     for (int64_t c = 0; c < n_cells; ++c) ostart[c + 1] += ostart[c];
-    for (int64_t f = 0; f < n_faces; ++f) fu[ostart[owner[f]]++] = (int32_t)f;
+    int32_t max_rank = 0;
+    for (int64_t f = 0; f < n_faces; ++f) {
+      const int64_t pos = ostart[owner[f]]++;
+      by_owner[pos] = (int32_t)f;
+    }
+    for (int64_t i = 0; i < n_faces; ++i) {  // rank = position within the owner's run
+      const int32_t f = by_owner[i];
+      rank[f] = (i > 0 && owner[by_owner[i - 1]] == owner[f]) ? rank[by_owner[i - 1]] + 1 : 0;
+      max_rank = std::max(max_rank, rank[f]);
+    }
+    std::vector<int64_t> rstart(max_rank + 2, 0);  // stable counting sort of by_owner by rank
+    for (int64_t f = 0; f < n_faces; ++f) rstart[rank[f] + 1]++;
+    for (int32_t r = 0; r <= max_rank; ++r) rstart[r + 1] += rstart[r];
+    fu.resize(n_faces);
+    for (int64_t i = 0; i < n_faces; ++i) {
+      const int32_t f = by_owner[i];
+      fu[rstart[rank[f]]++] = f;
+    }
+  }
+  std::vector<int32_t> owner_i, nb_i, cface_i;
+  if (!sorted) {
     owner_i.resize(n_faces);
     nb_i.resize(n_faces);
     for (int64_t i = 0; i < n_faces; ++i) {
@@ -89,6 +114,8 @@ extern "C" int afx_mesh_create(int dim, int64_t n_cells, int64_t n_faces, int64_
     std::vector<int32_t> nfill(ncount.begin(), ncount.end() - 1);
     for (int64_t i = 0; i < n_inc; ++i) ninc[nfill[face_node_indices[i]]++] = (int32_t)i;
   }
+  std::vector<int32_t> kpos(n_inc);
+  for (int64_t k = 0; k < n_inc; ++k) kpos[ninc[k]] = (int32_t)k;
   std::vector<int32_t> inc_face(n_inc);
   for (int64_t f = 0; f < n_faces; ++f)
     for (int64_t i = face_node_offsets[f]; i < face_node_offsets[f + 1]; ++i) inc_face[i] = (int32_t)f;
@@ -119,6 +146,7 @@ extern "C" int afx_mesh_create(int dim, int64_t n_cells, int64_t n_faces, int64_
   if (e == hipSuccess) e = up((void**)&m->inc_face, inc_face.data(), n_inc * 4);
   if (e == hipSuccess) e = up((void**)&m->nptr, ncount.data(), (n_nodes + 1) * 4);
   if (e == hipSuccess) e = up((void**)&m->ninc, ninc.data(), n_inc * 4);
+  if (e == hipSuccess) e = up((void**)&m->kpos, kpos.data(), n_inc * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&m->approx, (size_t)n_cells * dim * 8);
   if (e == hipSuccess) e = hipMalloc((void**)&m->normal_raw, (size_t)n_faces * dim * 8);
   if (!sorted) {
@@ -140,7 +168,7 @@ extern "C" int afx_mesh_destroy(afx_mesh* m) {
   if (!m) return 0;
   void* ptrs[] = {m->owner,   m->nb,      m->cptr,       m->cface,   m->foff, m->fidx,   m->inc_face,
                   m->nptr,    m->ninc,    m->approx,     m->normal_raw, m->scratch, m->fu, m->owner_i,
-                  m->nb_i,    m->cface_i, m->pack_scratch};
+                  m->nb_i,    m->cface_i, m->pack_scratch, m->kpos};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete m;
@@ -171,7 +199,7 @@ namespace {
 // ---------------------------------------------------------------------------------------------
 // face geometry from nodes (unoriented normal)
 template <int D>
-__global__ void k_face_geometry(int32_t F, int32_t Nn, const int32_t* __restrict__ foff,
+__global__ void __launch_bounds__(256) k_face_geometry(int32_t F, int32_t Nn, const int32_t* __restrict__ foff,
                                 const int32_t* __restrict__ fidx, const double* __restrict__ X,
                                 double* __restrict__ area, double* __restrict__ nrm,
                                 double* __restrict__ fc) {
@@ -249,7 +277,7 @@ __global__ void k_face_geometry(int32_t F, int32_t Nn, const int32_t* __restrict
 }
 
 template <int D>
-__global__ void k_approx_centroid(MeshView m, double* __restrict__ approx) {
+__global__ void __launch_bounds__(256) k_approx_centroid(MeshView m, double* __restrict__ approx) {
   int32_t c = xcd_thread_index();
   if (c >= m.C) return;
   double s[D];
@@ -268,7 +296,7 @@ __global__ void k_approx_centroid(MeshView m, double* __restrict__ approx) {
 }
 
 template <int D>
-__global__ void k_orient(MeshView m, const double* __restrict__ approx, const double* __restrict__ raw,
+__global__ void __launch_bounds__(256) k_orient(MeshView m, const double* __restrict__ approx, const double* __restrict__ raw,
                          double* __restrict__ nrm) {
   int32_t f = xcd_thread_index();
   if (f >= m.F) return;
@@ -283,7 +311,7 @@ __global__ void k_orient(MeshView m, const double* __restrict__ approx, const do
 }
 
 template <int D>
-__global__ void k_cell_volume(MeshView m, const double* __restrict__ approx, double* __restrict__ vol,
+__global__ void __launch_bounds__(256) k_cell_volume(MeshView m, const double* __restrict__ approx, double* __restrict__ vol,
                               double* __restrict__ cc) {
   int32_t c = xcd_thread_index();
   if (c >= m.C) return;
@@ -327,7 +355,7 @@ __global__ void k_cell_volume(MeshView m, const double* __restrict__ approx, dou
 // Cell pass: cotangents of the divergence-theorem sums (csum [d], S = d V) and of the approximate
 // centroid, per cell: cellbar[(j)*C + c] for csum_j, [d*C + c] for S, [(d+1+j)*C + c] for approx_j.
 template <int D>
-__global__ void k_geom_cell_adj(MeshView m, const double* __restrict__ approx, const double* __restrict__ vol_b,
+__global__ void __launch_bounds__(256) k_geom_cell_adj(MeshView m, const double* __restrict__ approx, const double* __restrict__ vol_b,
                                 const double* __restrict__ cc_b, double* __restrict__ cellbar) {
   int32_t c = xcd_thread_index();
   if (c >= m.C) return;
@@ -374,13 +402,15 @@ AFX_DEV void cross3(const double* a, const double* b, double* o) {
 }
 
 // Face pass: total cotangents of (area, oriented normal, centroid) of each face, then the reverse of
-// its node construction into per-incidence node cotangents inc_b[j * n_inc + i].
+// its node construction into per-incidence node cotangents, each written once, at the incidence's
+// node-major position: inc_b[kpos[i] * D + j] (the node pass then reads contiguous ranges).
 template <int D>
-__global__ void k_geom_face_adj(MeshView m, int32_t Nn, const int32_t* __restrict__ foff,
-                                const int32_t* __restrict__ fidx, const double* __restrict__ X,
-                                const double* __restrict__ approx, const double* __restrict__ cellbar,
-                                const double* __restrict__ area_b, const double* __restrict__ nrm_b,
-                                const double* __restrict__ fc_b, double* __restrict__ inc_b, int32_t n_inc) {
+__global__ void __launch_bounds__(256) k_geom_face_adj(MeshView m, int32_t Nn, const int32_t* __restrict__ foff,
+                                const int32_t* __restrict__ fidx, const int32_t* __restrict__ kpos,
+                                const double* __restrict__ X, const double* __restrict__ approx,
+                                const double* __restrict__ cellbar, const double* __restrict__ area_b,
+                                const double* __restrict__ nrm_b, const double* __restrict__ fc_b,
+                                double* __restrict__ inc_b) {
   int32_t f = xcd_thread_index();
   if (f >= m.F) return;
   const int32_t P = m.owner[f], N = m.nb[f];
@@ -430,10 +460,11 @@ __global__ void k_geom_face_adj(MeshView m, int32_t Nn, const int32_t* __restric
     const double vb0 = (rb0 - proj * r0) / vv, vb1 = (rb1 - proj * r1) / vv;
     // area = |d|, raw normal = unit(d1, -d0), centroid = (p0 + p1) / 2
     const double db0 = Ab * d0 / A - vb1, db1 = Ab * d1 / A + vb0;
-    inc_b[b] = -db0 + 0.5 * xfb[0];
-    inc_b[n_inc + b] = -db1 + 0.5 * xfb[1];
-    inc_b[b + 1] = db0 + 0.5 * xfb[0];
-    inc_b[n_inc + b + 1] = db1 + 0.5 * xfb[1];
+    const int64_t k0 = (int64_t)kpos[b] * 2, k1 = (int64_t)kpos[b + 1] * 2;
+    inc_b[k0] = -db0 + 0.5 * xfb[0];
+    inc_b[k0 + 1] = -db1 + 0.5 * xfb[1];
+    inc_b[k1] = db0 + 0.5 * xfb[0];
+    inc_b[k1 + 1] = db1 + 0.5 * xfb[1];
   } else {
     const int32_t k = e - b;
     const double invk = 1.0 / (double)k;
@@ -494,14 +525,10 @@ __global__ void k_geom_face_adj(MeshView m, int32_t Nn, const int32_t* __restric
     double Svb[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) Svb[j] = Ab * nh[j] + (nhb[j] - pr * nh[j]) / Ar;
-    double ctrb[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-      for (int32_t i = b; i < e; ++i) inc_b[(int64_t)j * n_inc + i] = 0.0;
-    for (int32_t i = b; i < e; ++i) {
-      const int32_t a = fidx[i];
-      const int32_t inext = i + 1 < e ? i + 1 : b;
-      const int32_t c2 = fidx[inext];
+    // reverse of edge (i, i+1) of the centre fan: cotangents ub (node i), wb (node i+1) and the
+    // centroid share tcb
+    auto edge = [&](int32_t i, double* ub, double* wb, double* tcb) {
+      const int32_t a = fidx[i], c2 = fidx[i + 1 < e ? i + 1 : b];
       double pb_[3], pc_[3], u[3], w[3], dir[3];
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
@@ -514,7 +541,7 @@ __global__ void k_geom_face_adj(MeshView m, int32_t Nn, const int32_t* __restric
       double sgn = 0.0;
 #pragma unroll
       for (int j = 0; j < 3; ++j) sgn += 0.5 * dir[j] * nh[j];
-      double dirb[3], ub[3], wb[3];
+      double dirb[3];
 #pragma unroll
       for (int j = 0; j < 3; ++j) dirb[j] = 0.5 * (Svb[j] + (0.0));
       // sgn_i = dir_i . nh (with dir_i = u x w / 2): cotangent sgnb_i adds sgnb_i nh / 2 to (u x w)'s
@@ -526,34 +553,46 @@ __global__ void k_geom_face_adj(MeshView m, int32_t Nn, const int32_t* __restric
       // d(u x w) = du x w + u x dw:  u_b = w x dirb, w_b = dirb x u
       cross3(w, dirb, ub);
       cross3(dirb, u, wb);
-      double tcb[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) tcb[j] = sgn * csb[j] / 3.0;
+    };
+    double ctrb[3] = {0.0, 0.0, 0.0};
+    for (int32_t i = b; i < e; ++i) {
+      double ub[3], wb[3], tcb[3];
+      edge(i, ub, wb, tcb);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) ctrb[j] += tcb[j] - ub[j] - wb[j];
+    }
+    // incidence i collects edge i's ub + tcb and edge i-1's wb + tcb, in the order the edges run
+    // (edge i-1 first, except for the first incidence, whose previous edge is the last one)
+    for (int32_t i = b; i < e; ++i) {
+      const int32_t ip = i > b ? i - 1 : e - 1;
+      double ub[3], wb[3], tcb[3], ub_p[3], wb_p[3], tcb_p[3];
+      edge(i, ub, wb, tcb);
+      edge(ip, ub_p, wb_p, tcb_p);
+      const int64_t kk = (int64_t)kpos[i] * 3;
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        tcb[j] = sgn * csb[j] / 3.0;
-        inc_b[(int64_t)j * n_inc + i] += ub[j] + tcb[j];
-        inc_b[(int64_t)j * n_inc + inext] += wb[j] + tcb[j];
-        ctrb[j] += tcb[j] - ub[j] - wb[j];
+        const double mine = ub[j] + tcb[j], prev = wb_p[j] + tcb_p[j];
+        const double t = i == b ? (0.0 + mine) + prev : (0.0 + prev) + mine;
+        inc_b[kk + j] = t + ctrb[j] * invk;
       }
     }
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-      for (int32_t i = b; i < e; ++i) inc_b[(int64_t)j * n_inc + i] += ctrb[j] * invk;
   }
 }
 
 // Node pass: node cotangent = sum over the node's incidences.
 template <int D>
-__global__ void k_geom_node_adj(int32_t Nn, const int32_t* __restrict__ nptr, const int32_t* __restrict__ ninc,
-                                const double* __restrict__ inc_b, int32_t n_inc, double* __restrict__ node_b) {
+__global__ void __launch_bounds__(256) k_geom_node_adj(int32_t Nn, const int32_t* __restrict__ nptr, const double* __restrict__ inc_b,
+                                double* __restrict__ node_b) {
   int32_t p = xcd_thread_index();
   if (p >= Nn) return;
   double s[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) s[j] = 0.0;
   for (int32_t k = nptr[p]; k < nptr[p + 1]; ++k) {
-    const int32_t i = ninc[k];
 #pragma unroll
-    for (int j = 0; j < D; ++j) s[j] += inc_b[(int64_t)j * n_inc + i];
+    for (int j = 0; j < D; ++j) s[j] += inc_b[(int64_t)k * D + j];
   }
 #pragma unroll
   for (int j = 0; j < D; ++j) node_b[(int64_t)j * Nn + p] = s[j];
@@ -741,10 +780,10 @@ extern "C" int afx_geometry_vjp(const afx_mesh* m, const double* node_coords, co
                gb->cell_centroid, cellbar);
     AFX_CHECK_LAUNCH();
     AFX_LAUNCH(kKGeomAdjFace, k_geom_face_adj<2>, grid_for(m->F), dim3(kBlock), 0, s, v, m->N, m->foff, m->fidx,
-               node_coords, approx, cellbar, gb->area, gb->normal, gb->face_centroid, inc_b, m->n_inc);
+               m->kpos, node_coords, approx, cellbar, gb->area, gb->normal, gb->face_centroid, inc_b);
     AFX_CHECK_LAUNCH();
-    AFX_LAUNCH(kKGeomAdjNode, k_geom_node_adj<2>, grid_for(m->N), dim3(kBlock), 0, s, m->N, m->nptr, m->ninc, inc_b,
-               m->n_inc, node_b);
+    AFX_LAUNCH(kKGeomAdjNode, k_geom_node_adj<2>, grid_for(m->N), dim3(kBlock), 0, s, m->N, m->nptr, inc_b,
+               node_b);
     AFX_CHECK_LAUNCH();
   } else {
     AFX_LAUNCH(kKGeomApprox, k_approx_centroid<3>, grid_for(m->C), dim3(kBlock), 0, s, v, approx);
@@ -753,10 +792,10 @@ extern "C" int afx_geometry_vjp(const afx_mesh* m, const double* node_coords, co
                gb->cell_centroid, cellbar);
     AFX_CHECK_LAUNCH();
     AFX_LAUNCH(kKGeomAdjFace, k_geom_face_adj<3>, grid_for(m->F), dim3(kBlock), 0, s, v, m->N, m->foff, m->fidx,
-               node_coords, approx, cellbar, gb->area, gb->normal, gb->face_centroid, inc_b, m->n_inc);
+               m->kpos, node_coords, approx, cellbar, gb->area, gb->normal, gb->face_centroid, inc_b);
     AFX_CHECK_LAUNCH();
-    AFX_LAUNCH(kKGeomAdjNode, k_geom_node_adj<3>, grid_for(m->N), dim3(kBlock), 0, s, m->N, m->nptr, m->ninc, inc_b,
-               m->n_inc, node_b);
+    AFX_LAUNCH(kKGeomAdjNode, k_geom_node_adj<3>, grid_for(m->N), dim3(kBlock), 0, s, m->N, m->nptr, inc_b,
+               node_b);
     AFX_CHECK_LAUNCH();
   }
   return 0;

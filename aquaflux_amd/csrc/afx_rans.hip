@@ -375,6 +375,7 @@ __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<
   FlowCell<D, S> X;
   load_flow<D, S>(a, m, c, false, false, X);
   S acc = sconst<S>(0.0);
+  AFX_UNROLL(AFX_FACE_UNROLL)
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
     FaceRef r = decode_face(m, m.cface[k]);
     const int32_t o = r.own ? r.N : r.P;

@@ -58,10 +58,11 @@ int afx_mesh_info(const afx_mesh* mesh, int64_t* out);
 /* Copies the cell -> face list (entries f when the cell owns f, ~f when it is f's neighbour). */
 int afx_mesh_cell_faces(const afx_mesh* mesh, int32_t* cell_ptr, int32_t* cell_faces, hipStream_t stream);
 
-/* The residual kernels visit faces in an internal order, the faces stably sorted by owner, so that
- * consecutive cells read contiguous face records.  *permuted = 0 when the caller's order already is
- * that order; otherwise 1, and face_of_internal[i] (nullable, [F] device) receives the caller's
- * index of internal face i.  No reference counterpart (a layout choice of this library). */
+/* The residual kernels visit faces in an internal order: when the caller's faces are not sorted by
+ * owner, by (rank among the owner's faces, owner), so that consecutive cells read consecutive face
+ * records.  *permuted = 0 when the caller's order is kept (it is sorted by owner); otherwise 1, and
+ * face_of_internal[i] (nullable, [F] device) receives the caller's index of internal face i.  No
+ * reference counterpart (a layout choice of this library). */
 int afx_mesh_face_order(const afx_mesh* mesh, int32_t* permuted, int32_t* face_of_internal, hipStream_t stream);
 
 /* ------------------------------------------------------------------------------------------- */

@@ -160,7 +160,8 @@ def test_residual_is_deterministic(cuda):
 
 
 def test_internal_face_order_and_face_pack(cuda):
-    """The residual kernels' internal face order is the stable owner sort of the caller's faces; a
+    """The residual kernels' internal face order sorts the caller's faces by (rank among the owner's
+    faces, owner); a
     call without the geometry's face pack (the library gathers one) gives the same bits; a mesh
     whose faces already are owner-sorted reports no permutation."""
     import ctypes
@@ -172,7 +173,12 @@ def test_internal_face_order_and_face_pack(cuda):
     dm = mesh.device_mesh(cuda)
     assert dm.face_permuted
     fo = dm.face_order().cpu().numpy()
-    assert np.array_equal(fo, np.argsort(mesh.owner, kind="stable"))
+    by_owner = np.argsort(mesh.owner, kind="stable")
+    rank = np.empty(mesh.n_faces, dtype=np.int64)
+    run = np.r_[0, np.flatnonzero(np.diff(mesh.owner[by_owner])) + 1]
+    pos = np.arange(mesh.n_faces) - np.repeat(run, np.diff(np.r_[run, mesh.n_faces]))
+    rank[by_owner] = pos
+    assert np.array_equal(fo, by_owner[np.argsort(rank[by_owner], kind="stable")])
     state = torch.tensor(seeded_state(mesh, 7, "channel"), dtype=F64, device=cuda)
     raw = mc._raw
     prim = [state] + [t.detach() for t in mc._inputs()[:4]]
