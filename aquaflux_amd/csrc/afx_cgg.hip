@@ -144,6 +144,7 @@ __global__ void __launch_bounds__(256) k_venkat(MeshView m, LimiterSlots sl, In<
     for (int j = 0; j < D; ++j) g[j] = G[(int64_t)(v * D + j) * m.C + c];
     S lo = sconst<S>(0.0);
     bool first = true;
+    int32_t ks = 0;
     for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
       const int32_t e = m.cface[k];
       const int32_t f = e >= 0 ? e : ~e;
@@ -151,10 +152,14 @@ __global__ void __launch_bounds__(256) k_venkat(MeshView m, LimiterSlots sl, In<
 #pragma unroll
       for (int j = 1; j < D; ++j) dm = dm + g[j] * (m.fc[(int64_t)j * m.F + f] - xcen[j]);
       const S vf = venkat_face<S>(dm, mx - xc, mn - xc, e2);
-      if (first || val(vf) < val(lo)) lo = vf;
+      if (first || val(vf) < val(lo)) {
+        lo = vf;
+        ks = k - m.cptr[c];
+      }
       first = false;
     }
     const int sj = sl.slot[l];
+    if (sl.kmin) sl.kmin[(int64_t)sj * m.C + c] = ks;
     psi.set((int64_t)sj * m.C + c, dmin(dmax(lo, 0.0), 1.0));
     if (ext) {
       ext[(int64_t)sj * m.C + c] = amx;

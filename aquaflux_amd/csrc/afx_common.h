@@ -106,6 +106,84 @@ AFX_HD S sconst(double v) { return S(v); }
 template <>
 AFX_HD dual sconst<dual>(double v) { return dual{v, 0.0}; }
 
+// Multi-tangent dual number: one value and N directional derivatives, so a cell-local function is
+// evaluated once for N seeds (each transcendental once).  Every tangent component follows exactly
+// the operations `dual` applies to its single tangent.
+template <int N>
+struct dualn {
+  double v;
+  double d[N];
+  AFX_HD dualn() {}
+  AFX_HD dualn(double x) : v(x) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) d[i] = 0.0;
+  }
+};
+template <int N>
+AFX_HD dualn<N> seeded(double x, int s) {
+  dualn<N> r;
+  r.v = x;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = i == s ? 1.0 : 0.0;
+  return r;
+}
+template <int N>
+AFX_HD double val(const dualn<N>& x) { return x.v; }
+#define AFX_DN_MAP(expr_v, expr_d)      \
+  dualn<N> r;                            \
+  r.v = (expr_v);                        \
+  _Pragma("unroll") for (int i = 0; i < N; ++i) r.d[i] = (expr_d); \
+  return r
+template <int N> AFX_HD dualn<N> operator+(const dualn<N>& a, const dualn<N>& b) { AFX_DN_MAP(a.v + b.v, a.d[i] + b.d[i]); }
+template <int N> AFX_HD dualn<N> operator-(const dualn<N>& a, const dualn<N>& b) { AFX_DN_MAP(a.v - b.v, a.d[i] - b.d[i]); }
+template <int N> AFX_HD dualn<N> operator*(const dualn<N>& a, const dualn<N>& b) {
+  AFX_DN_MAP(a.v * b.v, a.d[i] * b.v + a.v * b.d[i]);
+}
+template <int N> AFX_HD dualn<N> operator/(const dualn<N>& a, const dualn<N>& b) {
+  const double q = a.v / b.v;
+  AFX_DN_MAP(q, (a.d[i] - q * b.d[i]) / b.v);
+}
+template <int N> AFX_HD dualn<N> operator-(const dualn<N>& a) { AFX_DN_MAP(-a.v, -a.d[i]); }
+template <int N> AFX_HD dualn<N> operator+(const dualn<N>& a, double b) { AFX_DN_MAP(a.v + b, a.d[i]); }
+template <int N> AFX_HD dualn<N> operator+(double a, const dualn<N>& b) { AFX_DN_MAP(a + b.v, b.d[i]); }
+template <int N> AFX_HD dualn<N> operator-(const dualn<N>& a, double b) { AFX_DN_MAP(a.v - b, a.d[i]); }
+template <int N> AFX_HD dualn<N> operator-(double a, const dualn<N>& b) { AFX_DN_MAP(a - b.v, -b.d[i]); }
+template <int N> AFX_HD dualn<N> operator*(const dualn<N>& a, double b) { AFX_DN_MAP(a.v * b, a.d[i] * b); }
+template <int N> AFX_HD dualn<N> operator*(double a, const dualn<N>& b) { AFX_DN_MAP(a * b.v, a * b.d[i]); }
+template <int N> AFX_HD dualn<N> operator/(const dualn<N>& a, double b) { AFX_DN_MAP(a.v / b, a.d[i] / b); }
+template <int N> AFX_HD dualn<N> operator/(double a, const dualn<N>& b) {
+  const double q = a / b.v;
+  AFX_DN_MAP(q, -q * b.d[i] / b.v);
+}
+template <int N> AFX_HD dualn<N> dsqrt(const dualn<N>& x) {
+  const double s = sqrt(x.v);
+  AFX_DN_MAP(s, x.d[i] * 0.5 / s);
+}
+template <int N> AFX_HD dualn<N> dtanh(const dualn<N>& x) {
+  const double t = tanh(x.v);
+  AFX_DN_MAP(t, x.d[i] * (1.0 - t * t));
+}
+template <int N> AFX_HD dualn<N> dlog(const dualn<N>& x) { AFX_DN_MAP(log(x.v), x.d[i] / x.v); }
+template <int N> AFX_HD dualn<N> dexp(const dualn<N>& x) {
+  const double e = exp(x.v);
+  AFX_DN_MAP(e, x.d[i] * e);
+}
+template <int N> AFX_HD dualn<N> dmax(const dualn<N>& a, const dualn<N>& b) {
+  if (a.v > b.v) return a;
+  if (a.v < b.v) return b;
+  AFX_DN_MAP(a.v, 0.5 * (a.d[i] + b.d[i]));
+}
+template <int N> AFX_HD dualn<N> dmin(const dualn<N>& a, const dualn<N>& b) {
+  if (a.v < b.v) return a;
+  if (a.v > b.v) return b;
+  AFX_DN_MAP(a.v, 0.5 * (a.d[i] + b.d[i]));
+}
+template <int N> AFX_HD dualn<N> dmax(const dualn<N>& a, double b) { return dmax(a, dualn<N>(b)); }
+template <int N> AFX_HD dualn<N> dmin(const dualn<N>& a, double b) { return dmin(a, dualn<N>(b)); }
+template <int N> AFX_HD dualn<N> dmax(double a, const dualn<N>& b) { return dmax(dualn<N>(a), b); }
+template <int N> AFX_HD dualn<N> dmin(double a, const dualn<N>& b) { return dmin(dualn<N>(a), b); }
+#undef AFX_DN_MAP
+
 // ---------------------------------------------------------------------------------------------
 // structure-of-arrays views
 template <class S>

@@ -86,7 +86,7 @@ AFX_DEV int lim_slot(int v, bool limf, bool lims) {
 // pass G
 template <int D, class S, int GRAD, bool LIMF, bool LIMS>
 __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Out<S> G, Out<S> psi_out,
-                                                   int32_t* __restrict__ ext) {
+                                                   int32_t* __restrict__ ext, int32_t* __restrict__ kmin_out) {
   constexpr int NV = D + 3, NL = D + 2;
   const int32_t c = xcd_thread_index();
   if (c >= m.C) return;
@@ -223,6 +223,7 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
 #pragma unroll
     for (int j = 0; j < D; ++j) xcen[j] = m.cc[(int64_t)j * m.C + c];
     S lo[NL];
+    int32_t ks[NL];
     bool first = true;
     for (int32_t k = kb; k < ke; ++k) {
       const int32_t e = m.cface[k];
@@ -238,7 +239,10 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
 #pragma unroll
         for (int j = 1; j < D; ++j) dm = dm + grad[v][j] * dxf[j];
         S vf = venkat_face<S>(dm, mx[l] - xc[v], mn[l] - xc[v], e2);
-        if (first || val(vf) < val(lo[l])) lo[l] = vf;
+        if (first || val(vf) < val(lo[l])) {
+          lo[l] = vf;
+          ks[l] = k - kb;
+        }
       }
       first = false;
     }
@@ -251,6 +255,7 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
         ext[(int64_t)l * m.C + c] = amx[l];
         ext[(int64_t)(NL + l) * m.C + c] = amn[l];
       }
+      if (kmin_out) kmin_out[(int64_t)l * m.C + c] = ks[l];
     }
   }
 }
@@ -869,7 +874,8 @@ __global__ void __launch_bounds__(256) k_rans_ap_adj(MeshView m, RansArgs<double
 // is pre-multiplied by the cell's reconstruction operator.
 template <int D, int ADVF, int ADVS, int GRAD>
 __global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<double> a,
-                                                          const int32_t* __restrict__ ext, RansBars bar) {
+                                                          const int32_t* __restrict__ ext,
+                                                          const int32_t* __restrict__ kmin_pos, RansBars bar) {
   constexpr int NA = D + 3, NL = D + 2;
   const SstConst& sc = a.sc;
   const int32_t c = xcd_thread_index();
@@ -894,18 +900,17 @@ __global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<d
   const double strain0 = strain_magnitude<D, double>(Gu0);
   double A0, B0;
   closure_products<D, double>(Gk0, Gw0, dg, wall, A0, B0);
+  // one evaluation carrying the six tangents (seed s on input s of k, omega, nu, strain0, A, B)
+  using D6 = dualn<6>;
+  const SstClosure<D6> o = sst_closure_core<D6>(sc, seeded<6>(k0, 0), seeded<6>(w0, 1), seeded<6>(strain0, 3),
+                                                seeded<6>(A0, 4), seeded<6>(B0, 5), seeded<6>(nu0, 2), d, wall);
   double gin[6];
   const int nseed = wall ? 6 : 5;
-  for (int s = 0; s < 6; ++s) {
-    if (s >= nseed) {
-      gin[s] = 0.0;
-      continue;
-    }
-    auto sd = [&](int t) { return t == s ? 1.0 : 0.0; };
-    SstClosure<dual> o = sst_closure_core<dual>(sc, dual{k0, sd(0)}, dual{w0, sd(1)}, dual{strain0, sd(3)},
-                                                dual{A0, sd(4)}, dual{B0, sd(5)}, dual{nu0, sd(2)}, d, wall);
-    gin[s] = nut_b * o.nut.d + f1_b * o.f1.d + Sr_b * o.strain.d + cd_b * o.cd.d + nutw_b * o.nutw.d;
-  }
+#pragma unroll
+  for (int s = 0; s < 6; ++s)
+    gin[s] = s < nseed ? nut_b * o.nut.d[s] + f1_b * o.f1.d[s] + Sr_b * o.strain.d[s] + cd_b * o.cd.d[s] +
+                             nutw_b * o.nutw.d[s]
+                       : 0.0;
   double xb_k = gin[0], xb_w = gin[1];
   nu_b += gin[2];
   double Gb[NA][D];
@@ -990,24 +995,21 @@ __global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<d
       double Gv[D];
 #pragma unroll
       for (int j = 0; j < D; ++j) Gv[j] = a.G[(int64_t)(v * D + j) * m.C + c];
-      double lo = 0.0;
-      int32_t kmin = -1;
-      for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-        const int32_t e = m.cface[k];
+      // the first face attaining the forward minimum, recorded by the forward limiter pass
+      const int32_t kmin = m.cptr[c] + kmin_pos[(int64_t)l * m.C + c];
+      double lo;
+      {
+        const int32_t e = m.cface[kmin];
         const int32_t f = e >= 0 ? e : ~e;
         double dmf = Gv[0] * (m.fc[f] - xcen[0]);
 #pragma unroll
         for (int j = 1; j < D; ++j) dmf = dmf + Gv[j] * (m.fc[(int64_t)j * m.F + f] - xcen[j]);
-        const double vf = venkat_face<double>(dmf, mxv - xv, mnv - xv, e2);
-        if (kmin < 0 || vf < lo) {
-          lo = vf;
-          kmin = k;
-        }
+        lo = venkat_face<double>(dmf, mxv - xv, mnv - xv, e2);
       }
       const double dclip = (lo > 0.0 && lo < 1.0) ? 1.0 : ((lo == 0.0 || lo == 1.0) ? 0.5 : 0.0);
       const double vbar = pbar * dclip;
       double mxb = 0.0, mnb = 0.0;
-      if (vbar != 0.0 && kmin >= 0) {
+      if (vbar != 0.0) {
         const int32_t e = m.cface[kmin];
         const int32_t f = e >= 0 ? e : ~e;
         double dxf[D];
@@ -1179,7 +1181,11 @@ int64_t rans_work_doubles(int D, int64_t C, int64_t F, int mode) {
   const int64_t cgg = (int64_t)2 * NA * D * C;            // corrected Green-Gauss sweep scratch
   const int64_t phys = (int64_t)NA * C;                   // physical state of a transformed k / omega
   if (mode == 1) return 2 * fwd + cgg + 2 * phys;
-  if (mode == 2) return fwd + (int64_t)(NA * D + 1 + NL + 2 * NL + 7) * C + (int64_t)NL * C + F + cgg + phys;
+  // mode 2 adds the cotangents, the stencil extremum owners (2 NL int32) and the limiter argmin
+  // face positions (NL int32)
+  if (mode == 2)
+    return fwd + (int64_t)(NA * D + 1 + NL + 2 * NL + 7) * C + (int64_t)NL * C + ((int64_t)NL * C + 1) / 2 + F + cgg +
+           phys;
   return fwd + cgg + phys;
 }
 
@@ -1188,6 +1194,7 @@ struct RWork {
   double *Gt, *psit, *aPt, *clt, *mdott;  // tangents
   double *Gb, *aPb, *psib, *extb, *mueffb, *nutb, *f1b, *Srb, *cdb, *taub, *nutwb, *mdotb;
   int32_t* ext;
+  int32_t* kmin;  // [NL][C] limiter argmin face positions (VJP only)
   double* cgg;
   double *xs, *xst;  // physical state (and tangent) when k or omega is solved in log form
 };
@@ -1226,6 +1233,7 @@ RWork rcarve(void* base, int D, int64_t C, int64_t F, int mode) {
     w.taub = take(C);
     w.nutwb = take(C);
     w.ext = (int32_t*)take((int64_t)NL * C);
+    w.kmin = (int32_t*)take(((int64_t)NL * C + 1) / 2);
     w.mdotb = take(F);
   }
   w.cgg = take((int64_t)2 * NA * D * C);
@@ -1365,7 +1373,7 @@ int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S
   constexpr int GK = CG ? 0 : GRAD;
   constexpr bool LF = ADVF == kAdvLimited, LS = ADVS == kAdvLimited;
   AFX_LAUNCH(kKRansGrad, (k_rans_grad<D, S, GK, LF && !CG, LS && !CG>), grid_for(C), dim3(kBlock), 0, s, m, a, oG,
-             oP, ext);
+             oP, ext, ext ? w.kmin : nullptr);
   AFX_CHECK_LAUNCH();
   a.G.v = w.G;
   a.G.t = Gt;
@@ -1380,6 +1388,7 @@ int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S
     }
     LimiterSlots sl{};
     sl.ext_stride = D + 2;
+    sl.kmin = ext ? w.kmin : nullptr;
     if (LF)
       for (int i = 0; i < D; ++i) {
         sl.var[sl.nl] = i;
@@ -1547,7 +1556,7 @@ int rans_vjp_impl(const afx_mesh* m, const afx_geometry* g, const afx_rans_confi
                      AFX_LAUNCH(kKRansApAdj, (k_rans_ap_adj<D, AF>), grid_for(m->C), dim3(kBlock), 0, s, v, b, bar);
                      AFX_CHECK_LAUNCH();
                      AFX_LAUNCH(kKSstClosureAdj, (k_rans_closure_adj<D, AF, AS, GR>), grid_for(m->C), dim3(kBlock), 0,
-                                s, v, b, w.ext, bar);
+                                s, v, b, w.ext, w.kmin, bar);
                      AFX_CHECK_LAUNCH();
                      constexpr int GK = GR == kGradCorrected ? 0 : GR;
                      if (GR == kGradCorrected) {
