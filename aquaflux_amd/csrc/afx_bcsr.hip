@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(256) k_flow_jac(MeshView m, JacArgs a, int32_t
     for (int k = 0; k < NV; ++k) diag[r][k] = 0.0;
   const int32_t kb = m.cptr[c], ke = m.cptr[c + 1];
   for (int32_t k = kb; k < ke; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);

@@ -226,6 +226,7 @@ struct MeshView {
   const int32_t* nb;     // -1 on boundary faces
   const int32_t* cptr;   // [C+1] cell -> face CSR
   const int32_t* cface;  // f (cell owns f) or ~f (cell is f's neighbour)
+  const int32_t* cnbr;   // the other cell of each cface entry (-1 on a boundary face)
   const double* area;    // [F]
   const double* nrm;     // [d][F] owner-outward unit normal
   const double* fc;      // [d][F] face centroid
@@ -292,12 +293,15 @@ struct FaceRef {
 };
 AFX_DEV int32_t user_face(const MeshView& m, int32_t f) { return m.fu ? m.fu[f] : f; }
 
-AFX_DEV FaceRef decode_face(const MeshView& m, int32_t e) {
+// Entry k of cell c's face list; the other cell comes from the list's own neighbour array, so no
+// owner / neighbour lookup stands between the entry and the neighbour's data.
+AFX_DEV FaceRef decode_face(const MeshView& m, int32_t k, int32_t c) {
   FaceRef r;
+  const int32_t e = m.cface[k], o = m.cnbr[k];
   r.own = e >= 0;
   r.f = r.own ? e : ~e;
-  r.P = m.owner[r.f];
-  r.N = m.nb[r.f];
+  r.P = r.own ? c : o;
+  r.N = r.own ? o : c;
   r.fu = m.fu ? m.fu[r.f] : r.f;
   return r;
 }
@@ -310,7 +314,7 @@ AFX_DEV void lsq_matrix(const MeshView& m, int32_t c, double M[D][D]) {
 #pragma unroll
     for (int b = 0; b < D; ++b) M[a][b] = 0.0;
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     int32_t o = r.own ? r.N : r.P;
     double dx[D];
 #pragma unroll

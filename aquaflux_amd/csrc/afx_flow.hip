@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Ou
   }
   const int32_t kb = m.cptr[c], ke = m.cptr[c + 1];
   for (int32_t k = kb; k < ke; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
       S xo[NV];
@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(256) k_flow_ap(MeshView m, FlowArgs<S> a, Out<
   X.V = m.vol[c];
   S acc = sconst<S>(0.0);
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(256) k_flow_res(MeshView m, FlowArgs<S> a, Out
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = sconst<S>(0.0);
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -360,7 +360,7 @@ __global__ void __launch_bounds__(256) k_flow_res_adj(MeshView m, FlowArgs<doubl
   for (int v = 0; v < NV; ++v) lc[v] = lam[(int64_t)v * m.C + c];
   const double lc_cont = (c == a.pin) ? 0.0 : lc[D];
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -435,7 +435,7 @@ __global__ void __launch_bounds__(256) k_flow_ap_adj(MeshView m, FlowArgs<double
   B.zero();
   const double apb_c = bar.aP[c];
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -582,7 +582,7 @@ __global__ void __launch_bounds__(256) k_flow_grad_adj(MeshView m, FlowArgs<doub
 #pragma unroll
     for (int j = 0; j < D; ++j) Bc[v][j] = bar.G[(int64_t)(v * D + j) * m.C + c];
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
       if (GRAD == 0) {

@@ -12,6 +12,7 @@ struct afx_mesh {
   int32_t* nb = nullptr;
   int32_t* cptr = nullptr;      // [C+1]
   int32_t* cface = nullptr;     // [n_cface]  f (owner side) or ~f (neighbour side)
+  int32_t* cnbr = nullptr;      // [n_cface]  the other cell of each entry (-1: boundary face)
   int32_t* foff = nullptr;      // [F+1] face -> node CSR
   int32_t* fidx = nullptr;      // [n_inc]
   int32_t* inc_face = nullptr;  // [n_inc] face of each incidence
@@ -29,6 +30,7 @@ struct afx_mesh {
   int32_t* owner_i = nullptr;   // [F] owner / neighbour in internal order
   int32_t* nb_i = nullptr;
   int32_t* cface_i = nullptr;   // [n_cface] cell -> face list over internal face indices
+  int32_t* cnbr_i = nullptr;    // [n_cface] its other cells
   double* pack_scratch = nullptr;  // [(2d+1)][F] face geometry gathered for a call without face_pack
 };
 

@@ -52,6 +52,11 @@ extern "C" int afx_mesh_create(int dim, int64_t n_cells, int64_t n_faces, int64_
     if (neighbour[f] >= 0) cface[fill[neighbour[f]]++] = ~(int32_t)f;
   }
   for (int64_t c = 0; c < n_cells; ++c) max_faces = std::max(max_faces, count[c + 1] - count[c]);
+  std::vector<int32_t> cnbr(cface.size());
+  for (size_t k = 0; k < cface.size(); ++k) {
+    const int32_t e = cface[k], f = e >= 0 ? e : ~e;
+    cnbr[k] = e >= 0 ? neighbour[f] : owner[f];
+  }
   // Internal face order.  A caller order that is not sorted by owner (the structured generators
   // number faces family by family, k fastest) is replaced by faces sorted by (rank among their
   // owner's faces, owner, caller index): slot-r faces of consecutive cells are then consecutive
@@ -85,7 +90,7 @@ extern "C" int afx_mesh_create(int dim, int64_t n_cells, int64_t n_faces, int64_
       fu[rstart[rank[f]]++] = f;
     }
   }
-  std::vector<int32_t> owner_i, nb_i, cface_i;
+  std::vector<int32_t> owner_i, nb_i, cface_i, cnbr_i;
   if (!sorted) {
     owner_i.resize(n_faces);
     nb_i.resize(n_faces);
@@ -98,6 +103,11 @@ extern "C" int afx_mesh_create(int dim, int64_t n_cells, int64_t n_faces, int64_
     for (int64_t i = 0; i < n_faces; ++i) {
       cface_i[fill_i[owner_i[i]]++] = (int32_t)i;
       if (nb_i[i] >= 0) cface_i[fill_i[nb_i[i]]++] = ~(int32_t)i;
+    }
+    cnbr_i.resize(cface_i.size());
+    for (size_t k = 0; k < cface_i.size(); ++k) {
+      const int32_t e = cface_i[k], f = e >= 0 ? e : ~e;
+      cnbr_i[k] = e >= 0 ? nb_i[f] : owner_i[f];
     }
   }
   // node -> incidence CSR (for the node-coordinate adjoint gather)
@@ -141,6 +151,7 @@ extern "C" int afx_mesh_create(int dim, int64_t n_cells, int64_t n_faces, int64_
   if (e == hipSuccess) e = up((void**)&m->nb, neighbour, n_faces * 4);
   if (e == hipSuccess) e = up((void**)&m->cptr, count.data(), (n_cells + 1) * 4);
   if (e == hipSuccess) e = up((void**)&m->cface, cface.data(), cface.size() * 4);
+  if (e == hipSuccess) e = up((void**)&m->cnbr, cnbr.data(), cnbr.size() * 4);
   if (e == hipSuccess) e = up((void**)&m->foff, foff32.data(), (n_faces + 1) * 4);
   if (e == hipSuccess) e = up((void**)&m->fidx, face_node_indices, n_inc * 4);
   if (e == hipSuccess) e = up((void**)&m->inc_face, inc_face.data(), n_inc * 4);
@@ -154,6 +165,7 @@ extern "C" int afx_mesh_create(int dim, int64_t n_cells, int64_t n_faces, int64_
     if (e == hipSuccess) e = up((void**)&m->owner_i, owner_i.data(), n_faces * 4);
     if (e == hipSuccess) e = up((void**)&m->nb_i, nb_i.data(), n_faces * 4);
     if (e == hipSuccess) e = up((void**)&m->cface_i, cface_i.data(), cface_i.size() * 4);
+    if (e == hipSuccess) e = up((void**)&m->cnbr_i, cnbr_i.data(), cnbr_i.size() * 4);
   }
   if (e != hipSuccess) {
     afx_set_error(std::string("afx_mesh_create: ") + hipGetErrorString(e));
@@ -168,7 +180,7 @@ extern "C" int afx_mesh_destroy(afx_mesh* m) {
   if (!m) return 0;
   void* ptrs[] = {m->owner,   m->nb,      m->cptr,       m->cface,   m->foff, m->fidx,   m->inc_face,
                   m->nptr,    m->ninc,    m->approx,     m->normal_raw, m->scratch, m->fu, m->owner_i,
-                  m->nb_i,    m->cface_i, m->pack_scratch, m->kpos};
+                  m->nb_i,    m->cface_i, m->pack_scratch, m->kpos, m->cnbr, m->cnbr_i};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete m;
@@ -623,6 +635,7 @@ afx::MeshView afx_view(const afx_mesh* m, const afx_geometry* g) {
   v.nb = m->nb;
   v.cptr = m->cptr;
   v.cface = m->cface;
+  v.cnbr = m->cnbr;
   v.area = g ? g->area : nullptr;
   v.nrm = g ? g->normal : nullptr;
   v.fc = g ? g->face_centroid : nullptr;
@@ -690,6 +703,7 @@ int afx_rview(const afx_mesh* m, const afx_geometry* g, hipStream_t s, afx::Mesh
     v.owner = m->owner_i;
     v.nb = m->nb_i;
     v.cface = m->cface_i;
+    v.cnbr = m->cnbr_i;
     v.fu = m->fu;
     v.area = pack;
     v.nrm = pack + m->F;

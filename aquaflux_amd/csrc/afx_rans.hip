@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
   }
   const int32_t kb = m.cptr[c], ke = m.cptr[c + 1];
   for (int32_t k = kb; k < ke; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
       S xo[NV];
@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<
   S acc = sconst<S>(0.0);
   AFX_UNROLL(AFX_FACE_UNROLL)
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -418,7 +418,7 @@ __global__ void __launch_bounds__(256) k_rans_fres(MeshView m, RansArgs<S> a, Ou
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = sconst<S>(0.0);
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -490,7 +490,7 @@ __global__ void __launch_bounds__(256) k_rans_sres(MeshView m, RansArgs<S> a, In
   const double irho = 1.0 / sc.density;
   const double dP = a.dwall[c];
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -610,7 +610,7 @@ __global__ void __launch_bounds__(256) k_rans_sres_adj(MeshView m, RansArgs<doub
   const double lcw = wall ? 0.0 : lam[(int64_t)(D + 2) * m.C + c];  // flux / source part of the omega row
   double acck = 0.0, accw = 0.0;
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -769,7 +769,7 @@ __global__ void __launch_bounds__(256) k_rans_fres_adj(MeshView m, RansArgs<doub
   for (int v = 0; v < NV; ++v) lc[v] = lam[(int64_t)v * m.C + c];
   const double lc_cont = c == a.pin ? 0.0 : lc[D];
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -838,7 +838,7 @@ __global__ void __launch_bounds__(256) k_rans_ap_adj(MeshView m, RansArgs<double
   B.zero();
   const double apb_c = bar.aP[c];
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -1089,7 +1089,7 @@ __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<doub
 #pragma unroll
     for (int j = 0; j < D; ++j) Bc[v][j] = bar.G[(int64_t)(v * D + j) * m.C + c];
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
       if (GRAD == 0) {

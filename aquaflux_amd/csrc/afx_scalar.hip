@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(256) k_sc_grad(MeshView m, ScArgs<S> a, Out<S>
 #pragma unroll
   for (int j = 0; j < D; ++j) zero[j] = sconst<S>(0.0);
   for (int32_t k = kb; k < ke; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
       const S xo = a.phi[o];
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(256) k_sc_res(MeshView m, ScArgs<S> a, Out<S> 
   load_grad_row<D, S>(a.G, m.C, c, Gc);
   S acc = sconst<S>(0.0);
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -522,7 +522,7 @@ __global__ void __launch_bounds__(256) k_sc_res_adj(MeshView m, ScArgs<double> a
 #pragma unroll
   for (int j = 0; j < D; ++j) ccb[j] = 0.0;
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -743,7 +743,7 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
 #pragma unroll
   for (int j = 0; j < D; ++j) zero[j] = 0.0;
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeoBar<D> qb;
     qb.zero();
@@ -884,7 +884,7 @@ __global__ void __launch_bounds__(256) k_scalar_grad(MeshView m, const int8_t* _
 #pragma unroll
     for (int q = 0; q < D; ++q) M[p][q] = 0.0;
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, m.cface[k]);
+    FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
       const double xo = phi[o];
