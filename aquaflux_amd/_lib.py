@@ -169,6 +169,11 @@ _SIGNATURES = {
     "afx_mesh_info": [c_void_p, c_void_p],
     "afx_mesh_cell_faces": [c_void_p, c_void_p, c_void_p, c_void_p],
     "afx_mesh_face_order": [c_void_p, c_void_p, c_void_p, c_void_p],
+    "afx_block_colouring": [ctypes.c_int64, ctypes.c_int64, c_void_p, c_void_p, ctypes.c_int32, c_void_p, c_void_p,
+                            c_void_p, c_void_p, c_void_p],
+    "afx_probe_seed": [ctypes.c_int64, ctypes.c_int32, c_void_p, ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p],
+    "afx_probe_scatter": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p, ctypes.c_int32,
+                          c_void_p, ctypes.c_int32, ctypes.c_int32, c_void_p, ctypes.c_int, c_void_p, c_void_p],
     "afx_geometry_pack": [c_void_p, ctypes.POINTER(AfxGeometry), c_void_p, c_void_p],
     "afx_geometry_compute": [c_void_p, c_void_p, ctypes.POINTER(AfxGeometry), c_void_p],
     "afx_flow_workspace_bytes": [c_void_p, ctypes.c_int, c_int64_p],

@@ -347,11 +347,12 @@ extern "C" int afx_flow_jacobian(const afx_mesh* m, const afx_geometry* g, const
 
 template <class Fn>
 static int by_block(int B, int precision, Fn&& fn) {
-  if (B == 3 && precision == 0) return fn(std::integral_constant<int, 3>{}, (float*)nullptr);
-  if (B == 3 && precision == 1) return fn(std::integral_constant<int, 3>{}, (double*)nullptr);
-  if (B == 4 && precision == 0) return fn(std::integral_constant<int, 4>{}, (float*)nullptr);
-  if (B == 4 && precision == 1) return fn(std::integral_constant<int, 4>{}, (double*)nullptr);
-  afx_set_error("block size must be 3 or 4 and precision 0 or 1");
+#define AFX_BB(BB)                                                                                    \
+  if (B == BB && precision == 0) return fn(std::integral_constant<int, BB>{}, (float*)nullptr);       \
+  if (B == BB && precision == 1) return fn(std::integral_constant<int, BB>{}, (double*)nullptr);
+  AFX_BB(1) AFX_BB(2) AFX_BB(3) AFX_BB(4) AFX_BB(5) AFX_BB(6)
+#undef AFX_BB
+  afx_set_error("block size must be 1 to 6 and precision 0 or 1");
   return 2;
 }
 

@@ -44,6 +44,7 @@ enum KernelId : int {
   kKLimiter,
   kKCggSweep,
   kKCggSweepAdj,
+  kKProbeScatter,
   kKNumKernels
 };
 

@@ -329,6 +329,39 @@ int afx_rans_fields(const afx_mesh* mesh, const afx_geometry* geom, const afx_ra
                     const afx_rans_inputs* in, double* fields, hipStream_t stream);
 
 /* ------------------------------------------------------------------------------------------- */
+/* compressed (coloured) Jacobian assembly                                                       */
+
+/* Cell-block sparsity pattern at stencil `reach` on the interior-face cell graph and a probing
+ * colouring of it, from HOST arrays (owner / neighbour of the n_edges interior faces).  Replaces
+ * block_stencil_colouring and _saturation_colouring (reference:
+ * aquaflux/solve/sparse_jacobian.py:338-534).  HOST outputs: pattern_ptr[n_cells + 1] (always) and,
+ * when pattern_cols is not NULL, pattern_cols[pattern_ptr[n_cells]] (each row: the cell itself,
+ * then the cells within `reach` steps in ascending order) and, when pattern_depth is not NULL, each
+ * entry's graph distance from its row cell; when colour is not NULL, colour[n_cells]
+ * and *n_colours (two cells share a colour only if their graph distance exceeds 2 reach, so no row
+ * couples to both; chosen by saturation degree, ties by conflict degree). */
+int afx_block_colouring(int64_t n_cells, int64_t n_edges, const int32_t* owner, const int32_t* neighbour,
+                        int32_t reach, int64_t* pattern_ptr, int32_t* pattern_cols, int8_t* pattern_depth,
+                        int32_t* colour, int32_t* n_colours);
+
+/* Probe seed of (colour k, field): seed[v * n_cells + c] = 1 where v == field and colour[c] == k,
+ * else 0 (field-major, `block` fields).  Replaces ColumnProbePlan.seed_block (reference:
+ * aquaflux/solve/sparse_jacobian.py:208-247). */
+int afx_probe_seed(int64_t n_cells, int32_t block, const int32_t* colour, int32_t k, int32_t field, double* seed,
+                   hipStream_t stream);
+
+/* Writes column `field` of every block whose column cell has colour k from the probe response
+ * J . seed (field-major) into block-ELL storage vals[((s*B + a)*B + field)*C + c] (float when
+ * precision == 0, double when 1) over cols[s*C + c] (slot 0 the row cell, a later slot equal to
+ * the row cell is padding).  With depth (nullable, int8 [nslot*C], the entries' graph distances)
+ * entries farther than max_depth are left untouched: a column probed with a shorter-reach
+ * colouring than the pattern (ColumnProbePlan, sparse_jacobian.py:88-247) keeps its outer entries
+ * at zero.  Replaces ProbeGather.scatter (reference: aquaflux/solve/sparse_jacobian.py:632-720). */
+int afx_probe_scatter(int64_t n_cells, int32_t block, int32_t nslot, const int32_t* cols, const int8_t* depth,
+                      int32_t max_depth, const int32_t* colour, int32_t k, int32_t field, const double* response,
+                      int precision, void* vals, hipStream_t stream);
+
+/* ------------------------------------------------------------------------------------------- */
 /* Newton-Krylov inner operators: compact block Jacobian, SpMV, point-block Jacobi              */
 
 /* Block-ELL slot count per row (1 + max faces per cell). */
@@ -345,7 +378,8 @@ int afx_flow_jacobian(const afx_mesh* mesh, const afx_geometry* geom, const afx_
                       const afx_flow_inputs* in, int precision, void* vals, int32_t* cols, int32_t* tslot,
                       double* fields, hipStream_t stream);
 
-/* y = A x for a block-ELL matrix with cell-major vectors x[c*B + v] (B = 3 or 4). */
+/* y = A x for a block-ELL matrix with cell-major vectors x[c*B + v] (B = 1 to 6); a slot > 0 whose
+ * column is the row itself is padding. */
 int afx_bell_spmv(int64_t n_rows, int32_t block, int32_t nslot, int precision, const void* vals,
                   const int32_t* cols, const void* x, void* y, hipStream_t stream);
 

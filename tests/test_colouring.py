@@ -1,0 +1,95 @@
+"""Sparsity pattern and probing colouring of the compressed Jacobian (CPU: the library's host
+code against the CPU restatement), after tests/unit/test_sparse_jacobian.py:29-220."""
+
+import numpy as np
+import pytest
+
+from aquaflux_amd.jacobian import block_stencil_colouring
+from oracle import colouring as OC
+
+
+def _path(n):
+    return np.arange(n - 1), np.arange(1, n)
+
+
+def _lattice(nx, ny, nz):
+    index = np.arange(nx * ny * nz).reshape(nx, ny, nz)
+    owner = np.concatenate([index[:-1].ravel(), index[:, :-1].ravel(), index[:, :, :-1].ravel()])
+    nb = np.concatenate([index[1:].ravel(), index[:, 1:].ravel(), index[:, :, 1:].ravel()])
+    return owner, nb, nx * ny * nz
+
+
+def _pairs(c):
+    return {(int(i), int(j)) for i, j in zip(c.pattern_rows, c.pattern_cols)}
+
+
+def test_reach_one_pattern_is_the_adjacency():
+    """test_sparse_jacobian.py:29-37 on a 5-cell chain; each row starts with its own cell."""
+    owner, nb = _path(5)
+    c = block_stencil_colouring(owner, nb, 5, reach=1)
+    expected = {(i, i) for i in range(5)} | {(i, i + 1) for i in range(4)} | {(i + 1, i) for i in range(4)}
+    assert _pairs(c) == expected
+    assert all(int(c.pattern_cols[c.pattern_ptr[i]]) == i for i in range(5))
+
+
+def test_reach_two_adds_next_nearest_neighbours():
+    """test_sparse_jacobian.py:40-48."""
+    owner, nb = _path(5)
+    p1 = _pairs(block_stencil_colouring(owner, nb, 5, reach=1))
+    p2 = _pairs(block_stencil_colouring(owner, nb, 5, reach=2))
+    assert p1 < p2 and (0, 2) in p2 and (0, 2) not in p1
+
+
+@pytest.mark.parametrize("reach", [1, 2, 3])
+def test_pattern_matches_adjacency_powers(reach):
+    owner, nb, n = _lattice(5, 4, 3)
+    c = block_stencil_colouring(owner, nb, n, reach)
+    ref = OC.pattern(owner, nb, n, reach).tocoo()
+    assert _pairs(c) == {(int(i), int(j)) for i, j in zip(ref.row, ref.col)}
+
+
+def test_colouring_is_collision_free_on_a_random_graph():
+    """test_sparse_jacobian.py:51-68: no row couples to two cells of one colour."""
+    rng = np.random.default_rng(0)
+    n = 40
+    owner = np.concatenate([np.arange(n - 1), rng.integers(0, n, 6)])
+    nb = np.concatenate([np.arange(1, n), rng.integers(0, n, 6)])
+    keep = owner != nb
+    c = block_stencil_colouring(owner[keep], nb[keep], n, reach=2)
+    assert OC.collision_free(OC.conflict(owner[keep], nb[keep], n, 2), c.colour)
+    assert c.n_colours == int(c.colour.max()) + 1
+
+
+def test_saturation_beats_degree_ordering_on_a_lattice():
+    """test_sparse_jacobian.py:90-113: 8^3 lattice at reach 3 (the coupled RANS reach)."""
+    owner, nb, n = _lattice(8, 8, 8)
+    c = block_stencil_colouring(owner, nb, n, reach=3)
+    conf = OC.conflict(owner, nb, n, 3)
+    assert OC.collision_free(conf, c.colour)
+    assert c.n_colours < int(OC.degree_colouring(conf, n).max()) + 1
+
+
+@pytest.mark.parametrize("shape, reach", [((6, 5, 1), 1), ((6, 5, 4), 1), ((5, 4, 3), 2), ((7, 3, 1), 3)])
+def test_colour_count_comparable_to_the_full_scan_greedy(shape, reach):
+    """test_sparse_jacobian.py:185-220: valid and within two colours of the full-scan greedy."""
+    owner, nb, n = _lattice(*shape)
+    c = block_stencil_colouring(owner, nb, n, reach)
+    conf = OC.conflict(owner, nb, n, reach)
+    assert OC.collision_free(conf, c.colour)
+    assert c.n_colours <= int(OC.saturation_colouring(conf, n).max()) + 1 + 2
+
+
+def test_bad_arguments_raise():
+    owner, nb = _path(4)
+    with pytest.raises(ValueError):
+        block_stencil_colouring(owner, nb, 4, reach=0)
+    with pytest.raises(ValueError):
+        block_stencil_colouring(owner, nb + 10, 4, reach=1)
+
+
+def test_pattern_depth_is_the_graph_distance():
+    from scipy.sparse.csgraph import shortest_path
+    owner, nb, n = _lattice(4, 3, 3)
+    c = block_stencil_colouring(owner, nb, n, reach=3)
+    dist = shortest_path(OC.adjacency(owner, nb, n), unweighted=True)
+    assert np.array_equal(c.pattern_depth, dist[c.pattern_rows, c.pattern_cols].astype(np.int8))

@@ -1,0 +1,120 @@
+"""Compressed (coloured) Jacobian assembly on the GPU: the probed block Jacobian equals the matrix
+of unit-vector tangents (the HIP JVP, itself pinned to the oracle) when the reach covers the
+stencil, folds far couplings when it does not, and its SpMV / transposed SpMV / point-block Jacobi
+agree with the dense matrix.  After tests/unit/test_sparse_jacobian.py:230-400."""
+
+import numpy as np
+import pytest
+import torch
+
+from aquaflux_amd.jacobian import jacobian_relative_error, materialize_block_jacobian, mesh_colouring, mesh_probe_plan
+from tests import flow_cases as FC
+from tests import rans_cases as RC
+from tests import scalar_cases as SC
+
+pytestmark = pytest.mark.gpu
+F64 = torch.float64
+
+
+def _dense_from_jvp(matvec, n, device):
+    cols = []
+    for i in range(n):
+        e = torch.zeros(n, dtype=F64, device=device)
+        e[i] = 1.0
+        cols.append(matvec(e).cpu())
+    return torch.stack(cols, dim=1)
+
+
+def _rel(a, b):
+    return float((a - b).abs().max() / max(float(b.abs().max()), 1e-300))
+
+
+# Column reaches.  The scalar residual couples cells at most two steps apart (the neighbours'
+# gradients and limiters).  In the flow and RANS residuals the velocity columns reach three: the
+# Rhie-Chow a_P of a neighbour holds its faces' lagged mass-flux estimates, whose face velocities
+# carry a skewness correction with the next ring's velocity gradients (zero on an orthogonal mesh),
+# and in RANS its face viscosities carry the next ring's eddy viscosity, a function of that ring's
+# velocity gradient; the pressure, k and omega columns reach two.
+def _rans(cuda):
+    print("case box3d_sst limited lsq", flush=True)
+    mesh, rans, _ = RC.build("box3d_sst", "limited", "limited", "lsq", nu=1e-3, device=cuda)
+    st = torch.tensor(RC.seeded_state(mesh, 3, "cavity"), device=cuda)
+    return mesh, (lambda v: rans.residual_jvp(st, v)[1]), (3, 3, 3, 2, 2, 2)
+
+
+def _flow(cuda):
+    print("case channel2d limited lsq", flush=True)
+    mesh, geom, mc = FC.build("channel2d", "limited", "lsq", device=cuda)
+    st = torch.tensor(FC.seeded_state(mesh, 2, "channel"), dtype=F64, device=cuda)
+    return mesh, (lambda v: mc.residual_jvp(st, v)[1]), (3, 3, 2)
+
+
+def _scalar(cuda):
+    print("case tet3 limited gg", flush=True)
+    mesh, u, bcs = SC.make("tet3")
+    phi, gamma = SC.seeded(mesh, 4)
+    geom, st = SC.build_gpu(mesh, bcs, "limited", "gg", gamma, cuda)
+    flux = SC.gpu_flux(geom, u).detach()
+    x = torch.tensor(phi, device=cuda)
+    return mesh, (lambda v: st.residual_jvp(flux, x, v)[1]), (2,)
+
+
+@pytest.mark.parametrize("make", [_rans, _flow, _scalar], ids=["rans", "flow", "scalar"])
+def test_probed_jacobian_equals_the_tangent_matrix(make, cuda):
+    mesh, matvec, reach = make(cuda)
+    B = len(reach)
+    n = B * mesh.n_cells
+    dense = _dense_from_jvp(matvec, n, cuda)
+    plan = mesh_probe_plan(mesh, reach)
+    J = materialize_block_jacobian(matvec, plan, device=cuda)
+    err = _rel(J.to_dense(), dense)
+    print("reach", reach, plan.n_probes, "probes, err", err)
+    assert err < 1e-13
+    assert jacobian_relative_error(J, matvec) < 1e-13
+    # every nonzero of the true Jacobian lies in its column's reach
+    C = mesh.n_cells
+    col = plan.pattern
+    depth = torch.full((C, C), 99, dtype=torch.int64)
+    depth[torch.from_numpy(col.pattern_rows), torch.from_numpy(col.pattern_cols.astype(np.int64))] = torch.from_numpy(
+        col.pattern_depth.astype(np.int64))
+    for b in range(B):
+        nz = dense.reshape(B, C, B, C)[:, :, b, :].abs().sum(dim=0) > 0
+        assert not bool((nz & (depth > reach[b])).any()), b
+
+
+def test_short_reach_folds_far_couplings(cuda):
+    """A probe shorter than the residual's reach is not the Jacobian (far couplings land on near
+    entries: ColumnProbePlan, sparse_jacobian.py:88-147) -- the relative-error guard shows it, for
+    reach 2 on the velocity columns of the flow (perturbed mesh) and RANS residuals."""
+    mesh, matvec, reach = _flow(cuda)
+    J1 = materialize_block_jacobian(matvec, mesh_colouring(mesh, reach=2), len(reach), device=cuda)
+    assert jacobian_relative_error(J1, matvec) > 1e-6
+    mesh, matvec, reach = _rans(cuda)
+    J2 = materialize_block_jacobian(matvec, mesh_colouring(mesh, reach=2), len(reach), device=cuda)
+    assert jacobian_relative_error(J2, matvec) > 1e-6
+
+
+def test_spmv_transpose_and_point_block_jacobi(cuda):
+    mesh, matvec, reach = _rans(cuda)
+    B = len(reach)
+    plan = mesh_probe_plan(mesh, reach)
+    J = materialize_block_jacobian(matvec, plan, device=cuda)
+    Jf = materialize_block_jacobian(matvec, plan, device=cuda, precision="fp32")
+    dense = J.to_dense()
+    C = mesh.n_cells
+    g = torch.Generator(device="cpu").manual_seed(1)
+    v = torch.randn(B * C, generator=g, dtype=F64)
+    assert _rel(J.field_matvec(v.to(cuda)).cpu(), dense @ v) < 1e-13
+    assert _rel(Jf.field_matvec(v.to(cuda)).cpu(), dense @ v) < 1e-5
+    # transposed product in the cell-major layout
+    y = J.from_cell_major(J.rmatvec(J.to_cell_major(v.to(cuda)))).cpu()
+    assert _rel(y, dense.T @ v) < 1e-13
+    # point-block Jacobi inverts the diagonal blocks
+    D = torch.zeros_like(dense)
+    for a in range(B):
+        for b in range(B):
+            idx = torch.arange(C)
+            D[a * C + idx, b * C + idx] = dense[a * C + idx, b * C + idx]
+    x = J.from_cell_major(J.pbj(J.to_cell_major((D @ v).to(cuda)))).cpu()
+    assert not J.singular
+    assert _rel(x, v) < 1e-10
