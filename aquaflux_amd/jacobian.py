@@ -221,6 +221,11 @@ class ProbedJacobian:
         )
         return y
 
+    def pbj_transpose(self, x):
+        """``y = D^-T x`` (the adjoint solve's preconditioner)."""
+        inv = self.inv.reshape(self.B, self.B, self.C)  # inv[r, k, c]
+        return torch.einsum("rkc,cr->ck", inv, x.reshape(self.C, self.B)).reshape(-1).contiguous()
+
     def add_diagonal(self, shift):
         """Adds a per-row shift (field-major FP64 vector) to the diagonal blocks
         (``shifted_jacobian``, ``sparse_jacobian.py:893-927``) and refreshes the Jacobi inverses."""

@@ -214,8 +214,12 @@ class DampedNewtonStep:
     it.
 
     ``preconditioner``: ``None`` (plain GMRES on the JVP), ``"pbj"`` (point-block Jacobi of the
-    compact Jacobian) or ``"inner"`` (FGMRES whose preconditioner is GMRES(``inner_restart``) on the
-    compact Jacobian with point-block Jacobi); ``precision`` is the compact Jacobian's arithmetic.
+    block Jacobian) or ``"inner"`` (FGMRES whose preconditioner is GMRES(``inner_restart``) on the
+    block Jacobian with point-block Jacobi); ``precision`` is the block Jacobian's arithmetic.
+    ``jacobian``: ``"compact"`` (the flow residual's reach-1 dual-number assembly with gradients,
+    a_P and limiters frozen) or ``"probed"`` (the exact Jacobian of any residual by coloured probing
+    of its tangent, ``sparse_jacobian.py:722-848``, with ``column_reach`` per state field; the plan
+    is built once per mesh and kept).
     """
 
     preconditioner: str | None = "inner"
@@ -231,6 +235,26 @@ class DampedNewtonStep:
     # pseudo-transient continuation (switched evolution relaxation): None = plain Newton
     ptc_cfl0: float | None = None
     ptc_cfl_max: float = 1e12
+    jacobian: str = "compact"
+    column_reach: tuple | None = None
+    _plans: dict = dataclasses.field(default_factory=dict, repr=False)
+
+    def block_jacobian(self, op, phi):
+        if self.jacobian == "compact":
+            return BlockJacobian(op, phi, self.precision)
+        if self.jacobian != "probed":
+            raise ValueError(f"unknown jacobian {self.jacobian!r}")
+        from .jacobian import materialize_block_jacobian, mesh_probe_plan
+
+        if self.column_reach is None:
+            raise ValueError("the probed Jacobian needs column_reach (the stencil reach of each state field)")
+        key = (id(op.mesh), tuple(self.column_reach))
+        plan = self._plans.get(key)
+        if plan is None:
+            plan = mesh_probe_plan(op.mesh, self.column_reach)
+            self._plans[key] = plan
+        return materialize_block_jacobian(lambda v: op.residual_jvp(phi, v)[1], plan, device=phi.device,
+                                          precision=self.precision)
 
     def pseudo_time_shift(self, op, phi, cfl):
         """Diagonal ``a_P / CFL`` on the momentum rows (zero on continuity): a local pseudo-time step
@@ -247,7 +271,7 @@ class DampedNewtonStep:
         flexible = False
         shift = None if cfl is None or cfl >= self.ptc_cfl_max else self.pseudo_time_shift(op, phi, cfl)
         if self.preconditioner is not None:
-            J = BlockJacobian(op, phi, self.precision)
+            J = self.block_jacobian(op, phi)
             if shift is not None:
                 J.add_diagonal(shift)
             if self.preconditioner == "pbj":

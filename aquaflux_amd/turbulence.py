@@ -427,6 +427,10 @@ class CoupledRANS:
 
     # --- layout (turbulence/coupled.py:56-120) ---------------------------------------------------
     @property
+    def mesh(self):
+        return self.momentum.mesh
+
+    @property
     def dim(self):
         return self.momentum.mesh.dim
 

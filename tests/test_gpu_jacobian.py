@@ -118,3 +118,40 @@ def test_spmv_transpose_and_point_block_jacobi(cuda):
     x = J.from_cell_major(J.pbj(J.to_cell_major((D @ v).to(cuda)))).cpu()
     assert not J.singular
     assert _rel(x, v) < 1e-10
+
+
+def test_newton_with_the_probed_jacobian_converges_on_the_cavity(cuda):
+    """The Re=100 cavity (test_cavity.py:59-70, here 12x10) by Newton whose inner solves run on the
+    FP32 probed Jacobian with point-block Jacobi: same root as with the compact Jacobian."""
+    from aquaflux_amd.solve import DampedNewtonStep, ImplicitNewtonSolver
+
+    print("case cavity 12x10 upwind gg, probed fp32", flush=True)
+    mesh, geom, mc = FC.build("cavity", "upwind", "gg", device=cuda)
+    roots = {}
+    for jac in ("compact", "probed"):
+        step = DampedNewtonStep(preconditioner="inner", precision="fp32", jacobian=jac, column_reach=(3, 3, 2))
+        phi, rep = ImplicitNewtonSolver(forward_step=step).forward(mc, mc.initial_state())
+        assert rep.converged and rep.residual_norms[-1] <= 1e-12 + 1e-10 * rep.residual_norms[0]
+        roots[jac] = (phi, sum(rep.krylov_iterations))
+        print(jac, rep.steps, "steps", roots[jac][1], "Krylov iterations", flush=True)
+    assert float((roots["probed"][0] - roots["compact"][0]).abs().max()) < 1e-9
+    # the exact Jacobian preconditions at least as well as the frozen compact one
+    assert roots["probed"][1] <= roots["compact"][1]
+
+
+def test_probed_jacobian_preconditions_the_rans_newton_step(cuda):
+    """With the exact FP64 Jacobian solved tightly inside, the outer FGMRES on the HIP tangent of the
+    coupled RANS residual converges in one or two iterations."""
+    from aquaflux_amd.solve import DampedNewtonStep
+
+    print("case channel2d_sst limited lsq, probed fp64", flush=True)
+    mesh, rans, _ = RC.build("channel2d_sst", "limited", "limited", "lsq", nu=1e-3, device=cuda)
+    st = torch.tensor(RC.seeded_state(mesh, 5, "channel"), device=cuda)
+    r = rans.residual_raw(st)
+    step = DampedNewtonStep(preconditioner="inner", precision="fp64", jacobian="probed", column_reach=(3, 3, 2, 2, 2),
+                            rtol=1e-8, inner_rtol=1e-12, inner_restart=200, inner_iters=2000)
+    delta, info = step.correction(rans, st, r)
+    print("outer iterations", info.iterations, flush=True)
+    assert info.converged and info.iterations <= 2
+    _, Jd = rans.residual_jvp(st, delta)
+    assert float((Jd + r).norm()) <= 1e-8 * float(r.norm())
