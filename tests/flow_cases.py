@@ -61,6 +61,18 @@ def build(name, scheme="upwind", gradient="gg", mu=0.01, rho=1.0, device="cuda",
     return mesh, geom, mc
 
 
+def oracle_case(name, scheme="upwind", gradient="gg", inlet_velocity=None):
+    """The oracle's view of a case without building the HIP operator (no GPU needed): the mesh and
+    ``(OMesh, OGeometry, FlowSetup)``."""
+    import types
+
+    mesh = make_mesh(name)
+    bcs, pin = flow_bcs(name, mesh.dim, inlet_velocity)
+    view = types.SimpleNamespace(boundary=BoundaryConditions(bcs).resolve(mesh.face_patches), pressure_pin=pin)
+    m = fv.OMesh.from_mesh(mesh)
+    return mesh, (m, fv.geometry(m), oracle_setup(mesh, view, scheme, gradient))
+
+
 def gradient_scheme(gradient):
     """'gg' compact Green-Gauss, 'lsq' least squares, 'cgg' / 'cgg<n>' / 'cgg<n>r<relax>' corrected."""
     if gradient == "gg":

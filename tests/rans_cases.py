@@ -73,10 +73,23 @@ def build(name, flow_scheme="upwind", scalar_scheme="upwind", gradient="gg", nu=
     m = fv.OMesh.from_mesh(mesh)
     g = fv.geometry(m)
     s = flow_cases.oracle_setup(mesh, mc, flow_scheme, gradient)
+    return mesh, rans, (m, g, s, _oracle_sst(mesh, m, g, s, walls, kb, wb, nu, rho))
+
+
+def _oracle_sst(mesh, m, g, s, walls, kb, wb, nu, rho):
     wall_faces = np.concatenate([np.asarray(mesh.face_patches.indices(w), dtype=np.int64) for w in walls])
-    t = S.build_sst_setup(m, g, S.gradient_spec(s), S.SSTModel(), nu_t.clone(), rho, wall_faces,
-                          _oracle_scalar(mesh, kb), _oracle_scalar(mesh, wb))
-    return mesh, rans, (m, g, s, t)
+    return S.build_sst_setup(m, g, S.gradient_spec(s), S.SSTModel(), torch.full((mesh.n_cells,), nu, dtype=F64),
+                             rho, wall_faces, _oracle_scalar(mesh, kb), _oracle_scalar(mesh, wb))
+
+
+def oracle_case(name, flow_scheme="upwind", gradient="gg", nu=1e-2, rho=1.0):
+    """The oracle's view of a case without building the HIP operator (no GPU needed): the mesh and
+    ``(OMesh, OGeometry, FlowSetup, SSTSetup)``."""
+    base, walls, k_in, w_in = CASES[name]
+    mesh, (m, g, s) = flow_cases.oracle_case(base, flow_scheme, gradient)
+    kb = _scalar_bcs(mesh, walls, k_in, omega=False)
+    wb = _scalar_bcs(mesh, walls, w_in, omega=True)
+    return mesh, (m, g, s, _oracle_sst(mesh, m, g, s, walls, kb, wb, nu, rho))
 
 
 def seeded_state(mesh, seed=0, kind="channel"):
