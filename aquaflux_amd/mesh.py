@@ -123,6 +123,14 @@ class FacePatches(_Groups):
             label[idx] = gid
         return cls(label, tuple(names))
 
+    def is_boundary_patch(self, name: str, neighbour) -> bool:
+        """True when the named patch is non-empty and all its faces are boundary faces
+        (``aquaflux/mesh/groups.py:182-197``); ``neighbour`` is the per-face neighbour array or a
+        :class:`Mesh`."""
+        nb = np.asarray(getattr(neighbour, "neighbour", neighbour))
+        m = self.mask(name)
+        return bool(m.any()) and bool(np.all(nb[m] < 0))
+
 
 @dataclasses.dataclass(eq=False)
 class Mesh:

@@ -33,6 +33,11 @@ def make_mesh(name):
         return perturbed_grid_3d(6, 4, 4, lx=2.0, perturb=0.15, seed=1, named_boundaries=True)
     if name == "box3d":
         return structured_grid_3d(5, 4, 3, named_boundaries=True)
+    if name in ("brick2d", "brick3d"):
+        # general polygons / polyhedra (hexagonal bricks, eight-faced prisms) read from a polyMesh
+        from tests.polymesh_cases import brick_mesh
+
+        return brick_mesh(2 if name == "brick2d" else 3)
     raise KeyError(name)
 
 

@@ -24,6 +24,8 @@ CASES = {
     "channel2d_sst": ("channel2d", ("bottom", "top"), 1e-3, 1.0),
     "channel3d_sst": ("channel3d", ("bottom", "top", "back", "front"), 1e-3, 1.0),
     "box3d_sst": ("box3d", ("top", "bottom", "left", "right", "back", "front"), None, None),
+    "brick2d_sst": ("brick2d", ("bottom", "top"), 1e-3, 1.0),
+    "brick3d_sst": ("brick3d", ("bottom", "top", "back", "front"), 1e-3, 1.0),
 }
 
 

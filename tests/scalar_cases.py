@@ -32,7 +32,17 @@ CASES = {
                    (1.0, 0.3),
                    {"left": Dirichlet(1.0), "right": ZeroGradient(), "bottom": Neumann(0.7),
                     "top": Convective(2.0, 0.25)}),
+    # eight-faced prisms with hexagonal caps, read from an OpenFOAM polyMesh
+    "brick3d": (lambda: _brick3d(), (1.0, 0.3, 0.1),
+                {"left": Dirichlet(1.0), "right": ZeroGradient(), "bottom": Neumann(0.7), "top": Convective(2.0, 0.25),
+                 "back": ZeroGradient(), "front": ZeroGradient()}),
 }
+
+
+def _brick3d():
+    from tests.polymesh_cases import brick_mesh
+
+    return brick_mesh(3)
 
 ADV = {"none": None, "upwind": FirstOrderUpwind(), "limited": LimitedUpwind(VenkatakrishnanLimiter(5.0))}
 GRAD = {"gg": CompactGreenGauss(), "lsq": LeastSquaresGradient(), "none": None,

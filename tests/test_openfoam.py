@@ -1,0 +1,276 @@
+"""OpenFOAM polyMesh import on the CPU: the file envelope and list grammars on text snippets
+(after tests/unit/test_foamfile.py), the assembler on hand-built arrays (test_openfoam_assemble.py),
+the reader on the reference's three polyMesh fixtures (test_openfoam_reader.py, the fixture files
+copied under tests/golden/polymesh/), and a general polyhedral mesh (hexagonal-brick prisms)
+written and read back, whose cell volumes have closed forms."""
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from aquaflux_amd.mesh import structured_grid_2d, structured_grid_3d
+from aquaflux_amd.openfoam import (
+    CellZone,
+    FoamPatch,
+    OpenFOAMReader,
+    PolyMeshData,
+    assemble,
+    collapse_extruded_direction,
+    is_binary,
+    parse_boundary,
+    parse_cell_zones,
+    parse_face_list,
+    parse_foamfile,
+    parse_scalar_list,
+    parse_vector_list,
+    read_openfoam,
+)
+from oracle import fv
+from tests import polymesh_cases as PC
+
+FIXTURES = Path(__file__).parent / "golden" / "polymesh"
+
+_HEADER = (
+    "/*---------------------------------------------------------------------------*\\\n"
+    "| banner                                                                    |\n"
+    "\\*---------------------------------------------------------------------------*/\n"
+    "FoamFile\n{\n    version 2.0;\n    format ascii;\n    class vectorField;\n"
+    '    location "constant/polyMesh";\n    object points;\n}\n'
+    "// * * * * * * * * * * * * * //\n"
+)
+
+
+def _geometry(mesh):
+    return fv.geometry(fv.OMesh.from_mesh(mesh))
+
+
+def _sorted(t):
+    return np.sort(t.numpy())
+
+
+# --- envelope and grammars ------------------------------------------------------------------------
+def test_header_is_read_and_comments_stripped():
+    foam = parse_foamfile(_HEADER + "0\n(\n)\n")
+    assert foam.header["format"] == "ascii" and foam.header["object"] == "points"
+    assert foam.header["location"] == "constant/polyMesh"
+    assert not is_binary(foam)
+    assert "banner" not in foam.body
+
+
+def test_binary_format_is_detected():
+    assert is_binary(parse_foamfile("FoamFile { format binary; class vectorField; }\n0\n(\n)\n"))
+
+
+def test_missing_header_raises():
+    with pytest.raises(ValueError, match="no 'FoamFile"):
+        parse_foamfile("3\n(\n(0 0 0)\n)\n")
+
+
+def test_vector_list():
+    pts = parse_vector_list("3\n(\n(0 0 0)\n(1 0 0)\n(0.5 2 -1)\n)")
+    assert pts.shape == (3, 3)
+    np.testing.assert_array_equal(pts[2], [0.5, 2.0, -1.0])
+    assert parse_vector_list("0\n(\n)").shape == (0, 3)
+    with pytest.raises(ValueError, match=r"declares 3 .* lists 2"):
+        parse_vector_list("3\n(\n(0 0 0)\n(1 0 0)\n)")
+    with pytest.raises(ValueError, match="three coordinates"):
+        parse_vector_list("2\n(\n(0 0 0)\n(1 0)\n)")
+
+
+def test_label_list_and_trailing_text():
+    np.testing.assert_array_equal(parse_scalar_list("3\n(\n0\n0\n1\n)"), [0, 0, 1])
+    np.testing.assert_array_equal(parse_scalar_list("2(4 5) trailing ( )"), [4, 5])
+    with pytest.raises(ValueError, match="declares 3"):
+        parse_scalar_list("3\n(\n0\n1\n)")
+    with pytest.raises(ValueError, match="unbalanced"):
+        parse_scalar_list("2\n(\n0\n1\n")
+    with pytest.raises(ValueError, match="'N \\( ... \\)'"):
+        parse_scalar_list("no list here")
+
+
+def test_ragged_face_list():
+    off, idx = parse_face_list("3\n(\n4(0 1 2 3)\n3(4 5 6)\n6(7 8 9 10 11 12)\n)")
+    np.testing.assert_array_equal(off, [0, 4, 7, 13])
+    np.testing.assert_array_equal(idx, np.arange(13))
+    with pytest.raises(ValueError, match="declares 4 nodes but lists 3"):
+        parse_face_list("1\n(\n4(0 1 2)\n)")
+    with pytest.raises(ValueError, match="declares 2 entries but lists 1"):
+        parse_face_list("2\n(\n3(0 1 2)\n)")
+
+
+def test_boundary_dictionary():
+    patches = parse_boundary(
+        "2\n(\n"
+        "  inlet\n  {\n    type patch;\n    nFaces 1;\n    startFace 10;\n  }\n"
+        "  walls\n  {\n    type wall;\n    inGroups 1(walls);\n    nFaces 8;\n    startFace 11;\n  }\n"
+        ")\n"
+    )
+    assert [p.name for p in patches] == ["inlet", "walls"]
+    assert patches[1] == FoamPatch("walls", "wall", 11, 8)
+    with pytest.raises(ValueError, match="missing"):
+        parse_boundary("1\n(\n  inlet { type patch; nFaces 1; }\n)\n")
+
+
+def test_cell_zones():
+    zones = parse_cell_zones("1\n(\n  fluid\n  {\n    type cellZone;\n    cellLabels List<label> 3 ( 0 1 2 );\n  }\n)\n")
+    assert zones[0].name == "fluid"
+    np.testing.assert_array_equal(zones[0].cell_labels, [0, 1, 2])
+    with pytest.raises(ValueError, match="no cellLabels"):
+        parse_cell_zones("1\n(\n  fluid\n  {\n    type cellZone;\n  }\n)\n")
+
+
+# --- assembly ---------------------------------------------------------------------------------
+def _two_cubes():
+    """Two unit cubes sharing the x = 1 face; nodes x + 3 y + 6 z on the 3 x 2 x 2 lattice."""
+    pts = np.array([[x, y, z] for z in (0, 1) for y in (0, 1) for x in (0, 1, 2)], dtype=np.float64)
+    faces = [[1, 4, 10, 7], [0, 3, 9, 6], [2, 5, 11, 8], [0, 1, 7, 6], [3, 4, 10, 9], [0, 1, 4, 3],
+             [6, 7, 10, 9], [1, 2, 8, 7], [4, 5, 11, 10], [1, 2, 5, 4], [7, 8, 11, 10]]
+    return PolyMeshData(
+        points=pts,
+        face_node_offsets=np.arange(12, dtype=np.int64) * 4,
+        face_node_indices=np.array(faces, dtype=np.int64).reshape(-1),
+        owner=np.array([0, 0, 1, 0, 0, 0, 0, 1, 1, 1, 1], dtype=np.int64),
+        neighbour_internal=np.array([1], dtype=np.int64),
+        patches=(FoamPatch("inlet", "patch", 1, 1), FoamPatch("outlet", "patch", 2, 1), FoamPatch("walls", "wall", 3, 8)),
+        cell_zones=(CellZone("left", np.array([0])), CellZone("right", np.array([1]))),
+    )
+
+
+def test_assembles_two_cubes():
+    mesh = assemble(_two_cubes())
+    assert (mesh.dim, mesh.n_cells, mesh.n_faces) == (3, 2, 11)
+    assert mesh.neighbour[0] == 1 and np.all(mesh.neighbour[1:] == -1)
+    assert mesh.face_patches.size("inlet") == 1 and mesh.face_patches.size("walls") == 8
+    assert mesh.face_patches.is_boundary_patch("walls", mesh)
+    assert not mesh.face_patches.is_boundary_patch("interior", mesh)
+    np.testing.assert_array_equal(mesh.cell_zones.indices("left"), [0])
+    np.testing.assert_array_equal(mesh.cell_zones.indices("right"), [1])
+    np.testing.assert_allclose(_geometry(mesh).volume.numpy(), 1.0, rtol=0, atol=1e-15)
+
+
+def test_assembly_errors():
+    with pytest.raises(ValueError, match="reserved"):
+        assemble(_two_cubes()._replace(patches=(FoamPatch("boundary", "wall", 1, 10),)))
+    with pytest.raises(ValueError, match="longer than the face count"):
+        assemble(_two_cubes()._replace(neighbour_internal=np.arange(20, dtype=np.int64)))
+    empty = PolyMeshData(np.zeros((3, 3)), np.array([0], dtype=np.int64), np.zeros(0, dtype=np.int64),
+                         np.zeros(0, dtype=np.int64), np.zeros(0, dtype=np.int64), (), ())
+    with pytest.raises(ValueError, match="no faces"):
+        assemble(empty)
+
+
+# --- the reader on the reference's fixtures -------------------------------------------------------
+def test_reads_two_cube_fixture():
+    mesh = read_openfoam(FIXTURES / "polymesh_3d_two_cubes")
+    assert (mesh.dim, mesh.n_cells, mesh.n_faces) == (3, 2, 11)
+    assert int(mesh.interior.sum()) == 1
+    assert mesh.face_patches.size("inlet") == 1 and mesh.face_patches.size("walls") == 8
+    np.testing.assert_array_equal(mesh.cell_zones.indices("left"), [0])
+    g = _geometry(mesh)
+    np.testing.assert_allclose(g.volume.numpy(), 1.0, rtol=0, atol=1e-15)
+    ref = _geometry(structured_grid_3d(2, 1, 1, lx=2.0, ly=1.0, lz=1.0))
+    np.testing.assert_allclose(_sorted(g.volume), _sorted(ref.volume), rtol=1e-14)
+    np.testing.assert_allclose(_sorted(g.area), _sorted(ref.area), rtol=1e-14)
+
+
+@pytest.mark.parametrize("case,caps", [("polymesh_2d_slab", ("back", "front")),
+                                       ("polymesh_2d_slab_frontandback", ("frontAndBack",))])
+def test_reads_and_collapses_2d_slabs(case, caps):
+    mesh = read_openfoam(FIXTURES / case)
+    assert (mesh.dim, mesh.n_cells) == (2, 2)
+    names = set(mesh.face_patches.names)
+    assert not names & set(caps) and {"left", "right", "bottom", "top"} <= names
+    g = _geometry(mesh)
+    ref = _geometry(structured_grid_2d(2, 1, lx=2.0, ly=1.0))
+    np.testing.assert_allclose(_sorted(g.volume), _sorted(ref.volume), rtol=1e-14)
+    np.testing.assert_allclose(_sorted(g.area), _sorted(ref.area), rtol=1e-14)
+
+
+def test_reader_resolves_a_case_directory(tmp_path):
+    poly = tmp_path / "case" / "constant" / "polyMesh"
+    poly.mkdir(parents=True)
+    for name in ("points", "faces", "owner", "neighbour", "boundary", "cellZones"):
+        (poly / name).write_text((FIXTURES / "polymesh_3d_two_cubes" / name).read_text())
+    assert OpenFOAMReader(tmp_path / "case").read().n_cells == 2
+    assert OpenFOAMReader(poly).directory == str(poly)
+
+
+def test_binary_and_missing_files_are_rejected(tmp_path):
+    src = FIXTURES / "polymesh_3d_two_cubes"
+    for name in ("faces", "owner", "neighbour", "boundary", "cellZones"):
+        (tmp_path / name).write_text((src / name).read_text())
+    body = parse_foamfile((src / "points").read_text()).body
+    (tmp_path / "points").write_text("FoamFile { format binary; class vectorField; object points; }\n" + body)
+    with pytest.raises(NotImplementedError, match="binary"):
+        read_openfoam(tmp_path)
+    (tmp_path / "points").write_text((src / "points").read_text())
+    (tmp_path / "faces").unlink()
+    with pytest.raises(FileNotFoundError, match="faces"):
+        read_openfoam(tmp_path)
+    with pytest.raises(FileNotFoundError, match="no polyMesh"):
+        read_openfoam(tmp_path / "does_not_exist")
+
+
+# --- general polyhedra ------------------------------------------------------------------------------
+@pytest.mark.parametrize("perturb", [0.0, 0.15])
+def test_brick_polyhedra_round_trip(tmp_path, perturb):
+    nx, ny, nz, lx, ly, lz = 6, 4, 3, 2.0, 1.0, 0.5
+    data = PC.brick_polymesh(nx, ny, nz, lx, ly, lz, perturb=perturb, seed=4)
+    PC.write_polymesh(tmp_path, data)
+    mesh = read_openfoam(tmp_path)
+    assert mesh.dim == 3 and mesh.n_cells == data["n_cells"] == nz * (ny * nx + ny // 2)
+    np.testing.assert_array_equal(mesh.node_coords, data["points"])
+    np.testing.assert_array_equal(mesh.owner, data["owner"])
+    counts = np.diff(mesh.face_node_offsets)
+    assert counts.max() == 6 and counts.min() == 4  # hexagonal caps, quad sides
+    faces_per_cell = np.bincount(mesh.owner, minlength=mesh.n_cells) + np.bincount(
+        mesh.neighbour[mesh.interior], minlength=mesh.n_cells)
+    assert faces_per_cell.max() == 8
+    g = _geometry(mesh)
+    assert abs(float(g.volume.sum()) - lx * ly * lz) < 1e-13
+    if perturb == 0.0:
+        full = (lx / nx) * (ly / ny) * (lz / nz)
+        vols = np.unique(np.round(g.volume.numpy() / full, 12))
+        np.testing.assert_array_equal(vols, [0.5, 1.0])  # half bricks close the shifted rows
+    # every face normal leaves its owner (the written orientation is kept by the reader)
+    assert all(a in mesh.face_patches.names for a in PC.PATCHES)
+
+
+def test_brick_one_layer_with_empty_caps_collapses_to_polygons(tmp_path):
+    nx, ny, lx, ly = 6, 4, 2.0, 1.0
+    PC.write_polymesh(tmp_path, PC.brick_polymesh(nx, ny, 1, lx, ly, 0.1, perturb=0.15, seed=2, empty_caps=True))
+    mesh = read_openfoam(tmp_path)
+    assert mesh.dim == 2 and mesh.n_cells == ny * nx + ny // 2
+    assert "back" not in mesh.face_patches.names and "front" not in mesh.face_patches.names
+    faces_per_cell = np.bincount(mesh.owner, minlength=mesh.n_cells) + np.bincount(
+        mesh.neighbour[mesh.interior], minlength=mesh.n_cells)
+    assert faces_per_cell.max() == 6
+    assert abs(float(_geometry(mesh).volume.sum()) - lx * ly) < 1e-13
+
+
+def test_collapse_errors():
+    mesh = assemble(_two_cubes())
+    with pytest.raises(ValueError, match="at least one"):
+        collapse_extruded_direction(mesh, [])
+    with pytest.raises(ValueError, match="exactly two parallel planes"):
+        collapse_extruded_direction(mesh, ["inlet"])
+    with pytest.raises(ValueError, match="different axes"):
+        collapse_extruded_direction(mesh, ["walls"])
+    with pytest.raises(ValueError, match="3D mesh"):
+        collapse_extruded_direction(structured_grid_2d(2, 2, named_boundaries=True), ["left"])
+
+
+def test_brick_mesh_geometry_is_owner_outward():
+    mesh = PC.brick_mesh(3)
+    g = _geometry(mesh)
+    # divergence theorem per cell: sum of outward n A over a closed cell vanishes
+    own = torch.tensor(mesh.owner.astype(np.int64))
+    nb = torch.tensor(mesh.neighbour.astype(np.int64))
+    s = torch.zeros(mesh.n_cells, 3, dtype=torch.float64)
+    vec = g.normal * g.area[:, None]
+    s.index_add_(0, own, vec)
+    inter = nb >= 0
+    s.index_add_(0, nb[inter], -vec[inter])
+    assert float(s.abs().max()) < 1e-14
