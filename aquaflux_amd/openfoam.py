@@ -42,6 +42,9 @@ __all__ = [
     "collapse_extruded_direction",
     "OpenFOAMReader",
     "read_openfoam",
+    "parse_scalar_field",
+    "read_volume_scalar_field",
+    "read_surface_scalar_field",
 ]
 
 # patch names the mesh assigns by itself from the boundary mask
@@ -351,6 +354,111 @@ def collapse_extruded_direction(mesh: Mesh, removed_patch_names) -> Mesh:
     return Mesh.from_csr(coords2, np.arange(kept.size + 1, dtype=np.int64) * 2, edges.reshape(-1),
                          mesh.owner[kept], mesh.neighbour[kept], n_cells=mesh.n_cells, cell_zones=zones,
                          face_patches=patches)
+
+
+# --- scalar fields on an imported mesh (io/openfoam/fields.py:1-229) ------------------------------
+_BOUNDARY_FIELD = re.compile(r"\bboundaryField\s*\{")
+_INTERNAL_FIELD = re.compile(r"\binternalField\b")
+_UNIFORM = re.compile(r"\buniform\s+([-+0-9.eE]+)")
+
+
+def _closing_brace(text: str, at: int) -> int:
+    """Index of the ``}`` closing the ``{`` at ``at`` (vectorised nesting depth)."""
+    raw = np.frombuffer(text[at:].encode("latin-1", "replace"), dtype=np.uint8)
+    depth = np.cumsum((raw == ord("{")).astype(np.int32) - (raw == ord("}")).astype(np.int32))
+    closed = np.flatnonzero(depth == 0)
+    if closed.size == 0:
+        raise ValueError("unbalanced braces")
+    return at + int(closed[0])
+
+
+def _entry_values(text: str, count: int, what: str) -> np.ndarray:
+    """``count`` values of a ``uniform X`` or ``nonuniform List<scalar> N ( ... )`` entry (a file
+    mixes both: walls are usually ``uniform 0``)."""
+    if "nonuniform" in text:
+        declared, inner = list_envelope(text)
+        vals = _numbers(inner, np.float64)
+        if declared != vals.size:
+            raise ValueError(f"{what} declares {declared} values but lists {vals.size}")
+        if declared != count:
+            raise ValueError(f"{what} has {declared} values but the patch has {count} faces")
+        return vals
+    m = _UNIFORM.search(text)
+    if m is None:
+        raise ValueError(f"{what} is neither a 'uniform' nor a 'nonuniform' entry")
+    return np.full(count, float(m.group(1)), dtype=np.float64)
+
+
+def parse_scalar_field(body: str, n_internal: int, patch_sizes: dict) -> np.ndarray:
+    """``[internal, patch_0, patch_1, ...]`` of a scalar-field body, patches in ``patch_sizes``
+    order.  ``ValueError`` when the internal block or a listed patch is missing or of the wrong
+    length."""
+    mi = _INTERNAL_FIELD.search(body)
+    if mi is None:
+        raise ValueError("field has no internalField entry")
+    mb = _BOUNDARY_FIELD.search(body)
+    parts = [_entry_values(body[mi.end(): mb.start() if mb else len(body)], n_internal, "internalField")]
+    if patch_sizes and mb is None:
+        raise ValueError("field has no boundaryField entry")
+    blocks = {}
+    if mb is not None:
+        lo = mb.end() - 1
+        inner = body[lo + 1: _closing_brace(body, lo)]
+        pos = 0
+        while True:
+            brace = inner.find("{", pos)
+            if brace < 0:
+                break
+            words = inner[pos:brace].split()
+            end = _closing_brace(inner, brace)
+            blocks[words[-1] if words else ""] = inner[brace + 1: end]
+            pos = end + 1
+    for name, size in patch_sizes.items():
+        if name not in blocks:
+            raise ValueError(f"field has no boundaryField entry for patch '{name}'")
+        parts.append(_entry_values(blocks[name], size, f"patch '{name}'"))
+    return np.concatenate(parts)
+
+
+def read_volume_scalar_field(path, mesh: Mesh) -> np.ndarray:
+    """The ``internalField`` of a ``volScalarField`` on the mesh's cells (``(n_cells,)``).  Cell
+    labels are OpenFOAM's own (the assembler keeps them), so no ordering check is needed."""
+    return parse_scalar_field(read_foam_body(path), mesh.n_cells, {})
+
+
+def read_surface_scalar_field(path, mesh: Mesh) -> np.ndarray:
+    """A ``surfaceScalarField`` (the face flux ``phi``) on the mesh's faces, ``(n_faces,)``,
+    owner-outward.  Value ``i`` lands on face ``i`` only when the mesh keeps OpenFOAM's face order
+    (interior faces first, every patch a contiguous block); that order is checked, and a mesh
+    without it (a generated grid, or a collapsed 2D case, which is renumbered) is refused."""
+    interior = mesh.interior
+    n_int = int(interior.sum())
+    if not interior[:n_int].all() or interior[n_int:].any():
+        raise ValueError(
+            "mesh face ordering is not OpenFOAM's (interior faces are not the leading block), so field values "
+            "cannot be placed by index; a 2D case collapsed from an empty-capped polyMesh is renumbered and "
+            "cannot be used here"
+        )
+    sizes = {}
+    for name in mesh.face_patches.names:
+        if name == "interior":
+            continue
+        idx = mesh.face_patches.indices(name)
+        if idx.size == 0:
+            continue
+        if name == "boundary":
+            raise ValueError(f"{idx.size} boundary faces are not in a named patch, so the field has no values "
+                             "for them; the polyMesh must tile its boundary with named patches")
+        if idx[0] < n_int:
+            raise ValueError(f"patch '{name}' includes an interior face; ordering is not OpenFOAM's")
+        if idx[-1] - idx[0] + 1 != idx.size:
+            raise ValueError(f"patch '{name}' is not a contiguous block of faces")
+        sizes[name] = (int(idx[0]), int(idx.size))
+    ordered = {name: n for name, (_, n) in sorted(sizes.items(), key=lambda kv: kv[1][0])}
+    values = parse_scalar_field(read_foam_body(path), n_int, ordered)
+    if values.shape[0] != mesh.n_faces:
+        raise ValueError(f"field has {values.shape[0]} values but the mesh has {mesh.n_faces} faces")
+    return values
 
 
 # --- the reader (io/openfoam/reader.py:34-146) ---------------------------------------------------

@@ -274,3 +274,65 @@ def test_brick_mesh_geometry_is_owner_outward():
     inter = nb >= 0
     s.index_add_(0, nb[inter], -vec[inter])
     assert float(s.abs().max()) < 1e-14
+
+
+# --- scalar fields (after tests/unit/test_openfoam_fields.py) ---------------------------------------
+def _field_text(internal, patches):
+    blocks = "\n".join(f"    {n}\n    {{\n        type calculated;\n        value {e};\n    }}" for n, e in patches.items())
+    return ("FoamFile\n{\n    format ascii;\n    class surfaceScalarField;\n    object phi;\n}\n"
+            f"dimensions [0 3 -1 0 0 0 0];\n\ninternalField   {internal};\n\nboundaryField\n{{\n{blocks}\n}}\n")
+
+
+def _nonuniform(values):
+    return f"nonuniform List<scalar> {len(values)} ({' '.join(repr(float(v)) for v in values)})"
+
+
+def test_field_internal_block_and_mixed_patch_spellings():
+    from aquaflux_amd.openfoam import parse_foamfile, parse_scalar_field
+
+    body = parse_foamfile(_field_text(_nonuniform([1.0, 2.0, 3.0]), {})).body
+    np.testing.assert_array_equal(parse_scalar_field(body, 3, {}), [1.0, 2.0, 3.0])
+    body = parse_foamfile(_field_text(_nonuniform([1.0, 2.0]), {"inlet": _nonuniform([-5.0, -6.0]),
+                                                                 "wall": "uniform 0"})).body
+    np.testing.assert_array_equal(parse_scalar_field(body, 2, {"inlet": 2, "wall": 3}), [1, 2, -5, -6, 0, 0, 0])
+    with pytest.raises(ValueError, match="patch 'inlet'"):
+        parse_scalar_field(parse_foamfile(_field_text(_nonuniform([1.0]), {"inlet": _nonuniform([1, 2])})).body,
+                           1, {"inlet": 3})
+    with pytest.raises(ValueError, match="no boundaryField entry for patch 'outlet'"):
+        parse_scalar_field(parse_foamfile(_field_text(_nonuniform([1.0]), {"inlet": "uniform 0"})).body,
+                           1, {"outlet": 2})
+
+
+def test_surface_field_values_land_on_their_faces(tmp_path):
+    """Each written value is its own face index, so any misplacement shows."""
+    from aquaflux_amd.openfoam import read_surface_scalar_field
+
+    for src in (FIXTURES / "polymesh_3d_two_cubes", None):
+        if src is None:
+            src = tmp_path / "brick"
+            PC.write_polymesh(src, PC.brick_polymesh(4, 3, 2, perturb=0.1, seed=1))
+        mesh = read_openfoam(src)
+        n_int = int(mesh.interior.sum())
+        patches = {n: _nonuniform(mesh.face_patches.indices(n)) for n in mesh.face_patches.names
+                   if mesh.face_patches.size(n)}
+        (tmp_path / "phi").write_text(_field_text(_nonuniform(range(n_int)), patches))
+        np.testing.assert_array_equal(read_surface_scalar_field(tmp_path / "phi", mesh), np.arange(mesh.n_faces))
+
+
+def test_surface_field_refuses_a_mesh_out_of_openfoam_order(tmp_path):
+    from aquaflux_amd.openfoam import read_surface_scalar_field
+
+    (tmp_path / "phi").write_text(_field_text("uniform 0", {}))
+    with pytest.raises(ValueError, match=r"not OpenFOAM's|not a contiguous block"):
+        read_surface_scalar_field(tmp_path / "phi", structured_grid_2d(3, 3, named_boundaries=True))
+
+
+def test_volume_field_reads_cells_and_checks_length(tmp_path):
+    from aquaflux_amd.openfoam import read_volume_scalar_field
+
+    mesh = read_openfoam(FIXTURES / "polymesh_3d_two_cubes")
+    (tmp_path / "nut").write_text(_field_text(_nonuniform(range(mesh.n_cells)), {}))
+    np.testing.assert_array_equal(read_volume_scalar_field(tmp_path / "nut", mesh), np.arange(mesh.n_cells))
+    (tmp_path / "nut").write_text(_field_text(_nonuniform([1.0] * (mesh.n_cells + 1)), {}))
+    with pytest.raises(ValueError, match="internalField"):
+        read_volume_scalar_field(tmp_path / "nut", mesh)
