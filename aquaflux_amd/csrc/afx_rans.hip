@@ -752,9 +752,12 @@ __global__ void __launch_bounds__(256) k_rans_sres_adj(MeshView m, RansArgs<doub
 }
 
 // reverse pass R', flow part: momentum / continuity flux cotangents plus the scalar part's
-// mass-flux cotangent.
+// mass-flux cotangent.  Two resident waves per SIMD (a 256-register cap): the unconstrained 3D
+// build needs ~300 registers and runs one wave per SIMD; the cap spills 24 values and the gather
+// latency hidden by the second wave more than pays for them (270 -> 239 us on the 128x64x64 SST
+// channel).
 template <int D, int ADVF>
-__global__ void __launch_bounds__(256) k_rans_fres_adj(MeshView m, RansArgs<double> a, const double* __restrict__ lam,
+__global__ void __launch_bounds__(256, 2) k_rans_fres_adj(MeshView m, RansArgs<double> a, const double* __restrict__ lam,
                                                        const double* __restrict__ mdot_bar, RansBars bar) {
   constexpr int NV = D + 1;
   const int32_t c = xcd_thread_index();
@@ -827,9 +830,10 @@ __global__ void __launch_bounds__(256) k_rans_fres_adj(MeshView m, RansArgs<doub
   bar.nutw[c] = B.nutw;
 }
 
-// reverse pass D'
+// reverse pass D'.  Three waves per SIMD (168 registers; the 3D build needs 174 unconstrained,
+// the cap spills two): 152 -> 145 us on the 128x64x64 SST channel.
 template <int D, int ADV>
-__global__ void __launch_bounds__(256) k_rans_ap_adj(MeshView m, RansArgs<double> a, RansBars bar) {
+__global__ void __launch_bounds__(256, 3) k_rans_ap_adj(MeshView m, RansArgs<double> a, RansBars bar) {
   const int32_t c = xcd_thread_index();
   if (c >= m.C) return;
   FlowCell<D, double> X;

@@ -505,9 +505,10 @@ AFX_DEV void write_face_geo_bar(const ScBars& bar, int32_t F, int32_t f, const F
 }
 
 // ---------------------------------------------------------------------------------------------
-// reverse pass R'
+// reverse pass R'.  Three waves per SIMD: the 3D limited build with geometry cotangents needs
+// 174 registers unconstrained and fits 168 without spills (1.59 -> 1.55 ms on 6 x 96^3 tets).
 template <int D, int ADV, bool GEO>
-__global__ void __launch_bounds__(256) k_sc_res_adj(MeshView m, ScArgs<double> a, const double* __restrict__ lam,
+__global__ void __launch_bounds__(256, 3) k_sc_res_adj(MeshView m, ScArgs<double> a, const double* __restrict__ lam,
                                                     ScBars bar) {
   const int32_t c = xcd_thread_index();
   if (c >= m.C) return;

@@ -29,6 +29,9 @@ CASES = [
     ("channel2d", "upwind", "cgg9r0.8"),
     ("channel3d", "limited", "cgg6"),
     ("cavity", "none", "cgg1"),
+    ("brick2d", "limited", "lsq"),
+    ("brick3d", "limited", "lsq"),
+    ("brick3d", "upwind", "cgg"),
 ]
 
 

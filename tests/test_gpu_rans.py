@@ -26,6 +26,8 @@ CASES = [
     ("box3d_sst", "limited", "limited", "gg", 1e-3),
     ("channel2d_sst", "limited", "limited", "cgg", 1e-5),
     ("channel3d_sst", "upwind", "upwind", "cgg6r0.9", 1e-4),
+    ("brick2d_sst", "limited", "limited", "lsq", 1e-5),
+    ("brick3d_sst", "limited", "limited", "lsq", 1e-4),
 ]
 
 

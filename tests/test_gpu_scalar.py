@@ -30,6 +30,8 @@ CASES = [
     ("quad_mixed", "limited", "none"),
     ("tet3", "limited", "cgg"),
     ("quad_mixed", "limited", "cgg"),
+    ("brick3d", "limited", "gg"),
+    ("brick3d", "limited", "lsq"),
 ]
 GEO_CASES = [c for c in CASES if c[2] not in ("lsq", "cgg")]
 
@@ -204,6 +206,7 @@ TRANSIENT_CASES = [
     ("quad_mixed", "upwind", "lsq", False),
     ("quad_mixed", "limited", "cgg", True),
     ("tet3", "none", "none", False),
+    ("brick3d", "limited", "gg", False),
 ]
 DT = 0.037
 
