@@ -293,3 +293,53 @@ def test_rans_log_omega_keeps_transport_rows_and_linear_fixation(case, cuda):
     e[(d + 2) * C + cw] = 1.0
     _, Je = rans.residual_jvp(st.to(cuda), e)
     assert abs(float(Je[(d + 2) * C + cw]) - 1.0) < 1e-14
+
+
+def test_rans_outlet_objective_vjp_matches_central_differences(cuda):
+    """The 3D SST channel's objective J = mean over outlet faces of u^2 (outlet face velocity = owner
+    value): with lambda = dJ/dx plus a seeded N(0, 0.1) part (on this short channel the outlet seed
+    alone does not reach the inlet rows), the HIP VJP of <lambda, R(x; u_in, nu)> with respect to the
+    state, the inlet velocity vector and the molecular viscosity matches central differences of the
+    HIP residual to 1e-6 relative."""
+    name, fs, ss, gr, nu = ("channel3d_sst", "limited", "limited", "lsq", 1e-4)
+    mesh, rans, oset, st = _setup((name, fs, ss, gr, nu), cuda, seed=8)
+    C, d = mesh.n_cells, mesh.dim
+    out = torch.from_numpy(np.asarray(mesh.face_patches.indices("right"), dtype=np.int64))
+    own = torch.from_numpy(mesh.owner.astype(np.int64))[out]
+    x = st.to(cuda)
+    lam = torch.zeros_like(x)
+    lam[:C].index_add_(0, own.to(cuda), 2.0 * x[:C][own.to(cuda)] / out.numel())  # dJ/du at the owners
+    lam += 0.1 * torch.tensor(np.random.default_rng(10).normal(size=x.numel()), device=cuda)
+    left = torch.from_numpy(np.asarray(mesh.face_patches.indices("left"), dtype=np.int64))
+
+    def phi(x_, uin, nu_):
+        from aquaflux_amd.boundary import BoundaryConditions
+        from aquaflux_amd.flow import VelocityInlet
+
+        _, r_, _ = RC.build(name, fs, ss, gr, nu=nu_, device=cuda)
+        conds = dict(r_.momentum.boundary.conditions)
+        conds["left"] = VelocityInlet(tuple(float(u) for u in uin))
+        mom = type(r_.momentum)(**r_.momentum.__dict__)
+        mom.boundary = BoundaryConditions(conds).resolve(mesh.face_patches)
+        return float(lam @ type(r_).build(mom, r_.turbulence).residual_raw(x_))
+
+    bars = rans._raw.vjp(rans._detached(x), lam)
+    uin0 = np.array([1.0, 0.0, 0.0])
+    g_nu = float(bars[1].sum() + bars[3].sum())           # mu = rho nu with rho = 1, and the SST nu
+    g_uin = bars[4][:, left.to(cuda)].sum(dim=1).cpu().numpy()
+    rng = np.random.default_rng(9)
+    v = torch.tensor(rng.normal(size=x.numel()), device=cuda) * x.abs().clamp_min(1e-3)
+    checks = []
+    h = 1e-6
+    checks.append(("state", float(bars[0] @ v), (phi(x + h * v, uin0, nu) - phi(x - h * v, uin0, nu)) / (2 * h)))
+    hn = 1e-4 * nu  # nu = 1e-4: a smaller step is dominated by rounding of <lambda, R>
+    checks.append(("nu", g_nu, (phi(x, uin0, nu + hn) - phi(x, uin0, nu - hn)) / (2 * hn)))
+    for i in range(d):
+        e = np.zeros(d)
+        e[i] = 1e-6
+        checks.append((f"u_in[{i}]", float(g_uin[i]), (phi(x, uin0 + e, nu) - phi(x, uin0 - e, nu)) / 2e-6))
+    scale = max(abs(c[2]) for c in checks)
+    for label, ad, fd in checks:
+        print(label, ad, fd, flush=True)
+        assert abs(fd) > 1e-8 * scale, label  # every derivative is exercised
+        assert abs(ad - fd) <= 1e-6 * max(abs(fd), 1e-3 * scale), label
