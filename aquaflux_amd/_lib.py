@@ -169,6 +169,7 @@ _SIGNATURES = {
     "afx_mesh_info": [c_void_p, c_void_p],
     "afx_mesh_cell_faces": [c_void_p, c_void_p, c_void_p, c_void_p],
     "afx_mesh_face_order": [c_void_p, c_void_p, c_void_p, c_void_p],
+    "afx_mesh_set_block_order": [c_void_p, c_void_p, ctypes.c_int64],
     "afx_block_colouring": [ctypes.c_int64, ctypes.c_int64, c_void_p, c_void_p, ctypes.c_int32, c_void_p, c_void_p,
                             c_void_p, c_void_p, c_void_p],
     "afx_probe_seed": [ctypes.c_int64, ctypes.c_int32, c_void_p, ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p],

@@ -13,7 +13,7 @@ template <int D>
 __global__ void __launch_bounds__(256) k_cgg_sweep(MeshView m, int nv, const double* __restrict__ x0,
                                                    const double* __restrict__ x, double* __restrict__ xo,
                                                    double relax) {
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   const double V = m.vol[c];
   const double iv = 1.0 / V;
@@ -54,7 +54,7 @@ template <int D>
 __global__ void __launch_bounds__(256) k_cgg_sweep_adj(MeshView m, int nv, const double* __restrict__ y,
                                                        double* __restrict__ yo, double* __restrict__ bbar,
                                                        double relax, int first) {
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   const double V = m.vol[c];
   for (int v = 0; v < nv; ++v) {
@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(256) k_cgg_sweep_adj(MeshView m, int nv, const
 template <int D>
 __global__ void __launch_bounds__(256) k_cgg_final_adj(MeshView m, int nv, const double* __restrict__ y,
                                                        double* __restrict__ bbar, double relax, int first) {
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   const double V = m.vol[c];
   for (int v = 0; v < nv; ++v)
@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(256) k_zero(int64_t n, double* __restrict__ x)
 template <int D, class S>
 __global__ void __launch_bounds__(256) k_venkat(MeshView m, LimiterSlots sl, In<S> x, In<S> G, double lim_k,
                                                 Out<S> psi, int32_t* __restrict__ ext) {
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   const double e2 = m.vol[c] * (lim_k * lim_k * lim_k);
   double xcen[D];

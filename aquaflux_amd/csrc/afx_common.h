@@ -233,6 +233,8 @@ struct MeshView {
   const double* vol;     // [C]
   const double* cc;      // [d][C] cell centroid
   const int32_t* fu;     // [F] kernel face index -> caller face index (nullptr: the same)
+  const int32_t* bsched; // [nbs] block order of the cell kernels (nullptr: memory order)
+  int32_t nbs;           // number of 256-cell blocks bsched orders
 };
 
 // Per-face geometry seen from a face's owner (P) and neighbour (N).
@@ -371,10 +373,22 @@ constexpr int kNumXcd = 8;
 // 128 x 64 x 64 hex RANS residual + adjoint this is 1.13x faster with owner-sorted faces; with
 // face-family-major faces (consecutive cells' faces 4096 apart) it was 2x slower, because every
 // XCD then streams its own scattered face lines instead of sharing them through the fabric.
-AFX_DEV int32_t xcd_thread_index() {
+AFX_DEV uint32_t xcd_logical_block() {
   const uint32_t n = gridDim.x, b = blockIdx.x;
   const uint32_t q = n / kNumXcd, r = n % kNumXcd, x = b % kNumXcd, t = b / kNumXcd;
-  const uint32_t lb = x * q + (x < r ? x : r) + t;
+  return x * q + (x < r ? x : r) + t;
+}
+
+AFX_DEV int32_t xcd_thread_index() { return (int32_t)(xcd_logical_block() * blockDim.x + threadIdx.x); }
+
+// Cell index of a cell kernel launched one thread per cell: as xcd_thread_index(), then, when the
+// mesh carries a block order for this grid, the logical block is mapped through it.  The order is
+// a permutation of the blocks (each XCD keeps its contiguous share and walks it tile by tile along
+// the share's thinnest axis), so every cell is still visited exactly once and results do not change;
+// only which neighbours are resident in an XCD's L2 at the same time does.
+AFX_DEV int32_t xcd_thread_index(const MeshView& m) {
+  uint32_t lb = xcd_logical_block();
+  if (m.bsched != nullptr && gridDim.x == (uint32_t)m.nbs) lb = (uint32_t)m.bsched[lb];
   return (int32_t)(lb * blockDim.x + threadIdx.x);
 }
 inline dim3 grid_for(int64_t n, int block = kBlock) { return dim3((unsigned)((n + block - 1) / block)); }
@@ -384,14 +398,17 @@ inline dim3 grid_for(int64_t n, int block = kBlock) { return dim3((unsigned)((n 
 // x), so the cells in flight form a front advancing through each XCD's contiguous range instead of
 // the whole range at once.  With gridDim.x equal to the block count this is xcd_thread_index().
 template <class Fn>
-AFX_DEV void for_each_cell(int32_t n, Fn&& fn) {
+AFX_DEV void for_each_cell(const MeshView& m, Fn&& fn) {
+  const int32_t n = m.C;
   const uint32_t bs = blockDim.x, nb = ((uint32_t)n + bs - 1) / bs;
   const uint32_t G = gridDim.x, b = blockIdx.x;
   const uint32_t q = nb / kNumXcd, r = nb % kNumXcd, x = b % kNumXcd, w = b / kNumXcd;
   const uint32_t start = x * q + (x < r ? x : r), size = q + (x < r ? 1u : 0u);
   const uint32_t gx = G / kNumXcd + (x < G % kNumXcd ? 1u : 0u);
+  const bool ordered = m.bsched != nullptr && nb == (uint32_t)m.nbs;
   for (uint32_t t = w; t < size; t += gx) {
-    const int32_t c = (int32_t)((start + t) * bs + threadIdx.x);
+    const uint32_t lb = ordered ? (uint32_t)m.bsched[start + t] : start + t;
+    const int32_t c = (int32_t)(lb * bs + threadIdx.x);
     if (c < n) fn(c);
   }
 }

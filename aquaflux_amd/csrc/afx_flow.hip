@@ -81,7 +81,7 @@ template <int D, class S, int GRAD, bool LIM>
 __global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Out<S> G, Out<S> psi_out,
                                                    int32_t* __restrict__ ext) {
   constexpr int NV = D + 1;
-  int32_t c = xcd_thread_index();
+  int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   S xc[NV];
 #pragma unroll
@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Ou
 // pass D: momentum diagonal a_P
 template <int D, class S, int ADV>
 __global__ void __launch_bounds__(256) k_flow_ap(MeshView m, FlowArgs<S> a, Out<S> aP) {
-  int32_t c = xcd_thread_index();
+  int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   FlowCell<D, S> X;
   load_state<D, S>(a, m.C, c, X);
@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(256) k_flow_ap(MeshView m, FlowArgs<S> a, Out<
 template <int D, class S, int ADV>
 __global__ void __launch_bounds__(256) k_flow_res(MeshView m, FlowArgs<S> a, Out<S> R) {
   constexpr int NV = D + 1;
-  int32_t c = xcd_thread_index();
+  int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   FlowCell<D, S> X;
   load_cell_full<D, S>(a, m, c, true, ADV == kAdvLimited, X);
@@ -349,7 +349,7 @@ template <int D, int ADV>
 __global__ void __launch_bounds__(256) k_flow_res_adj(MeshView m, FlowArgs<double> a, const double* __restrict__ lam,
                                                       FlowBars bar) {
   constexpr int NV = D + 1;
-  int32_t c = xcd_thread_index();
+  int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   FlowCell<D, double> X;
   load_cell_full<D, double>(a, m, c, true, ADV == kAdvLimited, X);
@@ -423,7 +423,7 @@ template <int D, int ADV, int GRAD>
 __global__ void __launch_bounds__(256) k_flow_ap_adj(MeshView m, FlowArgs<double> a, const int32_t* __restrict__ ext,
                                                      FlowBars bar) {
   constexpr int NV = D + 1;
-  int32_t c = xcd_thread_index();
+  int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   FlowCell<D, double> X;
   load_state<D, double>(a, m.C, c, X);
@@ -571,7 +571,7 @@ template <int D, int GRAD, bool LIM>
 __global__ void __launch_bounds__(256) k_flow_grad_adj(MeshView m, FlowArgs<double> a, const int32_t* __restrict__ ext,
                                                        FlowBars bar) {
   constexpr int NV = D + 1;
-  int32_t c = xcd_thread_index();
+  int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   double xb[NV];
 #pragma unroll

@@ -32,6 +32,8 @@ struct afx_mesh {
   int32_t* cface_i = nullptr;   // [n_cface] cell -> face list over internal face indices
   int32_t* cnbr_i = nullptr;    // [n_cface] its other cells
   double* pack_scratch = nullptr;  // [(2d+1)][F] face geometry gathered for a call without face_pack
+  int32_t* bsched = nullptr;    // [nbs] block order of the cell kernels (afx_mesh_set_block_order)
+  int32_t nbs = 0;
 };
 
 // Device scratch of at least n doubles owned by the handle (grown on demand, freed with it);

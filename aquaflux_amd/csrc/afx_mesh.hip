@@ -180,7 +180,7 @@ extern "C" int afx_mesh_destroy(afx_mesh* m) {
   if (!m) return 0;
   void* ptrs[] = {m->owner,   m->nb,      m->cptr,       m->cface,   m->foff, m->fidx,   m->inc_face,
                   m->nptr,    m->ninc,    m->approx,     m->normal_raw, m->scratch, m->fu, m->owner_i,
-                  m->nb_i,    m->cface_i, m->pack_scratch, m->kpos, m->cnbr, m->cnbr_i};
+                  m->nb_i,    m->cface_i, m->pack_scratch, m->kpos, m->cnbr, m->cnbr_i, m->bsched};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete m;
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(256) k_face_geometry(int32_t F, int32_t Nn, co
 
 template <int D>
 __global__ void __launch_bounds__(256) k_approx_centroid(MeshView m, double* __restrict__ approx) {
-  int32_t c = xcd_thread_index();
+  int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   double s[D];
 #pragma unroll
@@ -325,7 +325,7 @@ __global__ void __launch_bounds__(256) k_orient(MeshView m, const double* __rest
 template <int D>
 __global__ void __launch_bounds__(256) k_cell_volume(MeshView m, const double* __restrict__ approx, double* __restrict__ vol,
                               double* __restrict__ cc) {
-  int32_t c = xcd_thread_index();
+  int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   double ap[D], v = 0.0, cs[D];
 #pragma unroll
@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(256) k_cell_volume(MeshView m, const double* _
 template <int D>
 __global__ void __launch_bounds__(256) k_geom_cell_adj(MeshView m, const double* __restrict__ approx, const double* __restrict__ vol_b,
                                 const double* __restrict__ cc_b, double* __restrict__ cellbar) {
-  int32_t c = xcd_thread_index();
+  int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   const double V = m.vol[c];
   double ap[D], csb[D], apb[D];
@@ -642,6 +642,8 @@ afx::MeshView afx_view(const afx_mesh* m, const afx_geometry* g) {
   v.vol = g ? g->volume : nullptr;
   v.cc = g ? g->cell_centroid : nullptr;
   v.fu = nullptr;
+  v.bsched = m->bsched;
+  v.nbs = m->nbs;
   return v;
 }
 
@@ -726,6 +728,30 @@ extern "C" int afx_mesh_face_order(const afx_mesh* m, int32_t* permuted, int32_t
   *permuted = m->fu ? 1 : 0;
   if (m->fu && face_of_internal)
     AFX_CHECK_HIP(hipMemcpyAsync(face_of_internal, m->fu, (size_t)m->F * 4, hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+extern "C" int afx_mesh_set_block_order(afx_mesh* m, const int32_t* order, int64_t n_blocks) {
+  AFX_REQUIRE(m, "afx_mesh_set_block_order: null mesh");
+  const int64_t nb = ((int64_t)m->C + kBlock - 1) / kBlock;
+  if (order == nullptr) {
+    if (m->bsched) AFX_CHECK_HIP(hipDeviceSynchronize());
+    if (m->bsched) (void)hipFree(m->bsched);
+    m->bsched = nullptr;
+    m->nbs = 0;
+    return 0;
+  }
+  AFX_REQUIRE(n_blocks == nb, "afx_mesh_set_block_order: n_blocks must be ceil(n_cells / 256)");
+  std::vector<uint8_t> seen((size_t)nb, 0);
+  for (int64_t i = 0; i < nb; ++i) {
+    const int32_t b = order[i];
+    AFX_REQUIRE(b >= 0 && b < nb && !seen[(size_t)b], "afx_mesh_set_block_order: order is not a permutation");
+    seen[(size_t)b] = 1;
+  }
+  AFX_CHECK_HIP(hipSetDevice(m->device));
+  if (m->bsched == nullptr) AFX_CHECK_HIP(hipMalloc(&m->bsched, (size_t)nb * sizeof(int32_t)));
+  AFX_CHECK_HIP(hipMemcpy(m->bsched, order, (size_t)nb * sizeof(int32_t), hipMemcpyHostToDevice));
+  m->nbs = (int32_t)nb;
   return 0;
 }
 

@@ -88,7 +88,7 @@ template <int D, class S, int GRAD, bool LIMF, bool LIMS>
 __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Out<S> G, Out<S> psi_out,
                                                    int32_t* __restrict__ ext, int32_t* __restrict__ kmin_out) {
   constexpr int NV = D + 3, NL = D + 2;
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   S xc[NV];
 #pragma unroll
@@ -306,7 +306,7 @@ AFX_DEV S wall_shear_rate(const RansArgs<S>& a, const MeshView& m, int32_t c, do
 
 template <int D, class S>
 __global__ void __launch_bounds__(256) k_rans_closure(MeshView m, RansArgs<S> a, ClosureOut out) {
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   S k, w, Gu[D][D], Gk[D], Gw[D];
   load_closure_inputs<D, S>(a, m, c, k, w, Gu, Gk, Gw);
@@ -376,7 +376,7 @@ AFX_DEV void load_scalar(const RansArgs<S>& a, const MeshView& m, int32_t c, int
 // pass D
 template <int D, class S, int ADV>
 __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<S> aP) {
-  for_each_cell(m.C, [&](int32_t c) {
+  for_each_cell(m, [&](int32_t c) {
   FlowCell<D, S> X;
   load_flow<D, S>(a, m, c, false, false, X);
   S acc = sconst<S>(0.0);
@@ -410,7 +410,7 @@ __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<
 template <int D, class S, int ADVF>
 __global__ void __launch_bounds__(256) k_rans_fres(MeshView m, RansArgs<S> a, Out<S> R, Out<S> mdot) {
   constexpr int NV = D + 1;
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   FlowCell<D, S> X;
   load_flow<D, S>(a, m, c, true, ADVF == kAdvLimited, X);
@@ -481,7 +481,7 @@ AFX_DEV void rans_scalar_face(const RansArgs<S>& a, const MeshView& m, const Fac
 template <int D, class S, int ADVS>
 __global__ void __launch_bounds__(256) k_rans_sres(MeshView m, RansArgs<S> a, In<S> mdot, Out<S> R) {
   const SstConst& sc = a.sc;
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   ScalarCell<D, S> Kc, Wc;
   load_scalar<D, S>(a, m, c, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Kc);
@@ -595,7 +595,7 @@ __global__ void __launch_bounds__(256) k_rans_sres_adj(MeshView m, RansArgs<doub
                                                        const double* __restrict__ lam, RansBars bar,
                                                        double* __restrict__ mdot_bar, double* __restrict__ R) {
   const SstConst& sc = a.sc;
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   ScalarCell<D, double> Kc, Wc;
   load_scalar<D, double>(a, m, c, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Kc);
@@ -760,7 +760,7 @@ template <int D, int ADVF>
 __global__ void __launch_bounds__(256, 2) k_rans_fres_adj(MeshView m, RansArgs<double> a, const double* __restrict__ lam,
                                                        const double* __restrict__ mdot_bar, RansBars bar) {
   constexpr int NV = D + 1;
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   FlowCell<D, double> X;
   load_flow<D, double>(a, m, c, true, ADVF == kAdvLimited, X);
@@ -834,7 +834,7 @@ __global__ void __launch_bounds__(256, 2) k_rans_fres_adj(MeshView m, RansArgs<d
 // the cap spills two): 152 -> 145 us on the 128x64x64 SST channel.
 template <int D, int ADV>
 __global__ void __launch_bounds__(256, 3) k_rans_ap_adj(MeshView m, RansArgs<double> a, RansBars bar) {
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   FlowCell<D, double> X;
   load_flow<D, double>(a, m, c, false, false, X);
@@ -882,7 +882,7 @@ __global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<d
                                                           const int32_t* __restrict__ kmin_pos, RansBars bar) {
   constexpr int NA = D + 3, NL = D + 2;
   const SstConst& sc = a.sc;
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   const bool wall = a.wallc[c] != 0;
   const double d = a.dwall[c];
@@ -1082,7 +1082,7 @@ template <int D, int GRAD, bool LIMF, bool LIMS>
 __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<double> a, const int32_t* __restrict__ ext,
                                                        RansBars bar) {
   constexpr int NA = D + 3, NL = D + 2;
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   double xb[NA];
 #pragma unroll

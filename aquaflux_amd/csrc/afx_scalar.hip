@@ -60,7 +60,7 @@ AFX_DEV void load_grad_row(const In<S>& G, int32_t C, int32_t c, S (&g)[D]) {
 template <int D, class S, int GRAD, bool LIM>
 __global__ void __launch_bounds__(256) k_sc_grad(MeshView m, ScArgs<S> a, Out<S> G, Out<S> psi_out,
                                                  int32_t* __restrict__ ext) {
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   const S xc = a.phi[c];
   S acc[D];
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(256) k_sc_grad(MeshView m, ScArgs<S> a, Out<S>
 // pass R
 template <int D, class S, int ADV>
 __global__ void __launch_bounds__(256) k_sc_res(MeshView m, ScArgs<S> a, Out<S> R) {
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   const S xc = a.phi[c], gc = a.gam[c];
   const S pc = ADV == kAdvLimited ? a.psi[c] : sconst<S>(1.0);
@@ -510,7 +510,7 @@ AFX_DEV void write_face_geo_bar(const ScBars& bar, int32_t F, int32_t f, const F
 template <int D, int ADV, bool GEO>
 __global__ void __launch_bounds__(256, 3) k_sc_res_adj(MeshView m, ScArgs<double> a, const double* __restrict__ lam,
                                                     ScBars bar) {
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   const double xc = a.phi[c], gc = a.gam[c];
   const double pc = ADV == kAdvLimited ? a.psi[c] : 1.0;
@@ -607,7 +607,7 @@ AFX_DEV dual venkat_face_de2(double dm0, double hmax, double hmin, double e2) {
 template <int D, int GRAD, bool LIM, bool GEO>
 __global__ void __launch_bounds__(256) k_sc_lim_adj(MeshView m, ScArgs<double> a, const int32_t* __restrict__ ext,
                                                     ScBars bar) {
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   double Gb[D], Gc[D];
 #pragma unroll
@@ -731,7 +731,7 @@ __global__ void __launch_bounds__(256) k_sc_lim_adj(MeshView m, ScArgs<double> a
 template <int D, int GRAD, bool LIM, bool GEO>
 __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> a, const int32_t* __restrict__ ext,
                                                      ScBars bar) {
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   double Bc[D];
 #pragma unroll
@@ -873,7 +873,7 @@ template <int D, int GRAD>
 __global__ void __launch_bounds__(256) k_scalar_grad(MeshView m, const int8_t* __restrict__ kind,
                                                      const double* __restrict__ bval,
                                                      const double* __restrict__ phi, double* __restrict__ G) {
-  const int32_t c = xcd_thread_index();
+  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   const double xc = phi[c];
   double acc[D];

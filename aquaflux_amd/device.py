@@ -3,6 +3,7 @@
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 import torch
@@ -55,6 +56,9 @@ class DeviceMesh:
         # the residual kernels visit faces sorted by owner; geometry then carries a packed copy
         self.face_permuted = bool(permuted.value)
         self._work = {}
+        self.block_order = None  # the cell kernels' block order once set (host int32 array)
+        # set from the first geometry's cell centroids unless AFX_BLOCK_ORDER=0 or set explicitly
+        self.auto_block_order = os.environ.get("AFX_BLOCK_ORDER", "1") != "0"
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -84,6 +88,22 @@ class DeviceMesh:
                                                   _lib.stream_of(self.device)), "afx_mesh_face_order")
         return fo
 
+    def set_block_order(self, order) -> None:
+        """Hands the cell kernels a block order (a permutation of the ceil(n_cells / 256) blocks, see
+        :func:`slab_tile_block_order`); ``None`` restores the memory order.  Results do not depend on it."""
+        L = _lib.lib()
+        self.auto_block_order = False
+        with torch.cuda.device(self.device):
+            torch.cuda.synchronize(self.device)
+            if order is None:
+                _lib.check(L.afx_mesh_set_block_order(self.handle, None, 0), "afx_mesh_set_block_order")
+                self.block_order = None
+                return
+            order = np.ascontiguousarray(order, dtype=np.int32)
+            _lib.check(L.afx_mesh_set_block_order(self.handle, order.ctypes.data, order.size),
+                       "afx_mesh_set_block_order")
+        self.block_order = order
+
     def cell_faces(self):
         """(cell_ptr, cell_faces) int32 device tensors of the cell -> face list."""
         cptr = torch.empty(self.n_cells + 1, dtype=torch.int32, device=self.device)
@@ -93,3 +113,49 @@ class DeviceMesh:
             "afx_mesh_cell_faces",
         )
         return cptr, cf
+
+
+CELL_BLOCK = 256  # threads per workgroup of the cell kernels
+N_XCD = 8         # accelerator complex dies of an MI355X
+
+
+def slab_tile_block_order(cell_centroids, per_tile: int = 64, n_xcd: int = N_XCD) -> np.ndarray:
+    """Block order of the cell kernels from cell centroids ``(C, d)`` (host array).
+
+    Workgroup ``b`` of a launch runs on XCD ``b % 8``, and the kernels give each XCD a contiguous
+    share of the 256-cell blocks.  Within its share, an XCD by default walks the blocks in memory
+    order; on a mesh numbered slab by slab this keeps a slab's worth of cells between a cell and its
+    neighbour across the slowest axis, more than an XCD's 4 MB L2 holds.  This order keeps each
+    share but walks it in tiles of about ``per_tile`` blocks cut along the share's longest axis,
+    and inside a tile along its thinnest axis first, so that the neighbours on all sides are read
+    while they are still resident.  Measured on MI355X: 6 x 96^3 tetrahedra, residual + adjoint to
+    node coordinates 10.99 -> 10.12 ms; 128 x 64 x 64 SST residual + VJP 1.458 -> 1.428 ms."""
+    cc = np.asarray(cell_centroids, dtype=np.float64)
+    C = cc.shape[0]
+    if cc.ndim != 2 or C == 0:
+        raise ValueError("cell_centroids must be a non-empty (n_cells, d) array")
+    if cc.shape[1] < 3:
+        cc = np.column_stack([cc, np.zeros((C, 3 - cc.shape[1]))])
+    nb = (C + CELL_BLOCK - 1) // CELL_BLOCK
+    pad = nb * CELL_BLOCK - C
+    if pad:
+        cc = np.concatenate([cc, np.repeat(cc[-1:], pad, axis=0)])
+    centre = cc.reshape(nb, CELL_BLOCK, 3).mean(axis=1)
+    q, r = divmod(nb, n_xcd)
+    order = np.empty(nb, dtype=np.int32)
+    start = 0
+    for x in range(n_xcd):
+        n = q + (1 if x < r else 0)
+        if n == 0:
+            continue
+        c = centre[start:start + n]
+        lo, hi = c.min(axis=0), c.max(axis=0)
+        ext = hi - lo
+        thin = int(np.argmin(np.where(ext > 0, ext, np.inf)))
+        long_ = int(np.argmax(np.where(np.arange(3) == thin, -1.0, ext)))
+        ntiles = max(1, int(round(n / per_tile)))
+        width = ext[long_] / ntiles if ext[long_] > 0 else 1.0
+        tile = np.minimum(((c[:, long_] - lo[long_]) / width).astype(np.int64), ntiles - 1)
+        order[start:start + n] = start + np.lexsort((np.arange(n), c[:, thin], tile))
+        start += n
+    return order

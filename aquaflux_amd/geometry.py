@@ -139,6 +139,12 @@ def compute_geometry(mesh, node_coords=None, device=None) -> MeshGeometry:
     dev = dm.device
     X = _nodes_soa(mesh, node_coords, dev)
     area, nrm, fc, vol, cc = _GeometryFn.apply(dm, X)
+    if dm.block_order is None and dm.auto_block_order:
+        # first geometry of this handle: order the cell kernels' blocks by these centroids
+        from .device import slab_tile_block_order
+
+        dm.set_block_order(slab_tile_block_order(cc.detach().T.cpu().numpy()))
+        dm.auto_block_order = False
     face = FaceGeometry(area=area, centroid_soa=fc, normal_soa=nrm)
     cell = CellGeometry(volume=vol, centroid_soa=cc)
     return MeshGeometry(face=face, cell=cell, node_coords_soa=X, device_mesh=dm)

@@ -58,6 +58,14 @@ int afx_mesh_info(const afx_mesh* mesh, int64_t* out);
 /* Copies the cell -> face list (entries f when the cell owns f, ~f when it is f's neighbour). */
 int afx_mesh_cell_faces(const afx_mesh* mesh, int32_t* cell_ptr, int32_t* cell_faces, hipStream_t stream);
 
+/* Block order of the cell kernels (no reference counterpart: a scheduling choice of this build).
+ * order[n_blocks] (host memory, n_blocks = ceil(n_cells / 256)) is a permutation of the 256-cell
+ * blocks: logical workgroup i of a one-thread-per-cell launch processes block order[i].  Results
+ * are unchanged (every cell is still computed by exactly one thread, with the same arithmetic);
+ * what changes is which cells an XCD holds in its L2 at the same time.  order == NULL restores the
+ * memory order.  Synchronous; not to be called while kernels on this mesh are in flight. */
+int afx_mesh_set_block_order(afx_mesh* mesh, const int32_t* order, int64_t n_blocks);
+
 /* The residual kernels visit faces in an internal order: when the caller's faces are not sorted by
  * owner, by (rank among the owner's faces, owner), so that consecutive cells read consecutive face
  * records.  *permuted = 0 when the caller's order is kept (it is sorted by owner); otherwise 1, and
