@@ -413,6 +413,130 @@ AFX_DEV void cross3(const double* a, const double* b, double* o) {
   o[2] = a[0] * b[1] - a[1] * b[0];
 }
 
+// The 3D centre-fan reverse of the face pass below for a face of k <= 4 nodes (triangles, quads):
+// the same operations in the same order as the general loop, with the k node positions loaded once
+// and kept in registers (the general loop re-reads each node up to 23 times); per-edge products are
+// re-formed from them where needed rather than stored.  Indices into the fixed arrays are
+// compile-time constants after unrolling; the run-time k only selects values.
+AFX_DEV void face_adj_small(int32_t k, const int32_t* __restrict__ fidx, const int32_t* __restrict__ kpos,
+                            int32_t Nn, const double* __restrict__ X, const double* n, const double* nb,
+                            const double* xf, const double* xfb, double Ab, double* __restrict__ inc_b) {
+  double P[4][3];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int32_t a = i < k ? fidx[i] : fidx[0];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) P[i][j] = X[(int64_t)j * Nn + a];
+  }
+  const double invk = 1.0 / (double)k;
+  double ctr[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) ctr[j] = i < k ? ctr[j] + P[i][j] : ctr[j];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) ctr[j] = ctr[j] * invk;
+  // edge i runs from node i to node (i + 1) mod k; its end point, fan vectors and product
+  auto ends = [&](int i, double* u, double* w, double* q) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      q[j] = i + 1 < k ? P[(i + 1) & 3][j] : P[0][j];
+      u[j] = P[i & 3][j] - ctr[j];
+      w[j] = q[j] - ctr[j];
+    }
+  };
+  double Sv[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double u[3], w[3], q[3], dir[3];
+    ends(i, u, w, q);
+    cross3(u, w, dir);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Sv[j] = i < k ? Sv[j] + 0.5 * dir[j] : Sv[j];
+  }
+  const double Ar = sqrt(Sv[0] * Sv[0] + Sv[1] * Sv[1] + Sv[2] * Sv[2]);
+  double nh[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) nh[j] = Sv[j] / Ar;
+  const double s = (nh[0] * n[0] + nh[1] * n[1] + nh[2] * n[2]) < 0.0 ? -1.0 : 1.0;
+  double nhb[3], csb[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    nhb[j] = s * nb[j];
+    csb[j] = xfb[j] / Ar;
+  }
+  Ab -= (xfb[0] * xf[0] + xfb[1] * xf[1] + xfb[2] * xf[2]) / Ar;
+  double Sg[4], Sn[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double u[3], w[3], q[3], dir[3];
+    ends(i, u, w, q);
+    cross3(u, w, dir);
+    double sgnb = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) sgnb += csb[j] * ((ctr[j] + P[i][j] + q[j]) / 3.0);
+    Sg[i] = sgnb;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) nhb[j] = i < k ? nhb[j] + sgnb * 0.5 * dir[j] : nhb[j];
+    double sgn = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) sgn += 0.5 * dir[j] * nh[j];
+    Sn[i] = sgn;
+  }
+  const double pr = nhb[0] * nh[0] + nhb[1] * nh[1] + nhb[2] * nh[2];
+  double Svb[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) Svb[j] = Ab * nh[j] + (nhb[j] - pr * nh[j]) / Ar;
+  // edge i's cotangents of its first node (ub) and second node (wb), and its centroid share (tcb)
+  auto edge = [&](int i, double* ub, double* wb, double* tcb) {
+    double u[3], w[3], q[3], dirb[3];
+    ends(i, u, w, q);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dirb[j] = 0.5 * (Svb[j] + (0.0));
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dirb[j] += 0.5 * Sg[i] * nh[j];
+    cross3(w, dirb, ub);
+    cross3(dirb, u, wb);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) tcb[j] = Sn[i] * csb[j] / 3.0;
+  };
+  double ctrb[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double ub[3], wb[3], tcb[3];
+    edge(i, ub, wb, tcb);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) ctrb[j] = i < k ? ctrb[j] + (tcb[j] - ub[j] - wb[j]) : ctrb[j];
+  }
+  // incidence i collects edge i's ub + tcb and edge i-1's wb + tcb (for the first incidence the
+  // last edge's, added after its own share); edge i-1's share is carried from the previous step
+  double first[3], prev[3], last[3];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double ub[3], wb[3], tcb[3];
+    edge(i, ub, wb, tcb);
+    if (i < k) {
+      const int64_t kk = (int64_t)kpos[i] * 3;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double mine = ub[j] + tcb[j];
+        if (i == 0)
+          first[j] = mine;
+        else
+          inc_b[kk + j] = (0.0 + prev[j]) + mine + ctrb[j] * invk;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      prev[j] = wb[j] + tcb[j];
+      last[j] = i < k ? prev[j] : last[j];
+    }
+  }
+  const int64_t k0 = (int64_t)kpos[0] * 3;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) inc_b[k0 + j] = ((0.0 + first[j]) + last[j]) + ctrb[j] * invk;
+}
+
 // Face pass: total cotangents of (area, oriented normal, centroid) of each face, then the reverse of
 // its node construction into per-incidence node cotangents, each written once, at the incidence's
 // node-major position: inc_b[kpos[i] * D + j] (the node pass then reads contiguous ranges).
@@ -477,6 +601,8 @@ __global__ void __launch_bounds__(256) k_geom_face_adj(MeshView m, int32_t Nn, c
     inc_b[k0 + 1] = -db1 + 0.5 * xfb[1];
     inc_b[k1] = db0 + 0.5 * xfb[0];
     inc_b[k1 + 1] = db1 + 0.5 * xfb[1];
+  } else if (e - b <= 4) {
+    face_adj_small(e - b, fidx + b, kpos + b, Nn, X, n, nb, xf, xfb, Ab, inc_b);
   } else {
     const int32_t k = e - b;
     const double invk = 1.0 / (double)k;
