@@ -37,8 +37,20 @@ class KrylovInfo:
     converged: bool = False
 
 
-def gmres(matvec, b, *, x0=None, rtol=1e-10, atol=0.0, restart=30, max_iter=1000, precond=None, flexible=False):
-    """Solve ``A x = b``; returns ``(x, KrylovInfo)``.  ``precond`` is applied on the right."""
+# a new Arnoldi vector is orthogonalised a second time only when the first classical Gram-Schmidt
+# pass removed more than 1 - 1/sqrt(2) of its norm (the Daniel-Gragg-Kaufman-Stewart criterion,
+# "twice is enough"): each pass reads the whole Krylov basis, so this halves the basis traffic when
+# the basis is well conditioned, which is the usual case
+DGKS_ETA = 1.0 / math.sqrt(2.0)
+
+
+def gmres(matvec, b, *, x0=None, rtol=1e-10, atol=0.0, restart=30, max_iter=1000, precond=None, flexible=False,
+          reorthogonalize: str = "dgks"):
+    """Solve ``A x = b``; returns ``(x, KrylovInfo)``.  ``precond`` is applied on the right.
+    ``reorthogonalize``: ``"dgks"`` (a second Gram-Schmidt pass when the first lost more than
+    1 - 1/sqrt(2) of the vector's norm) or ``"always"`` (two passes every step)."""
+    if reorthogonalize not in ("dgks", "always"):
+        raise ValueError(f"unknown reorthogonalize {reorthogonalize!r}")
     dtype, dev = b.dtype, b.device
     n = b.numel()
     bnorm = float(torch.linalg.vector_norm(b))
@@ -73,13 +85,18 @@ def gmres(matvec, b, *, x0=None, rtol=1e-10, atol=0.0, restart=30, max_iter=1000
                 Z[j] = z
             w = matvec(z)
             Vj = V[: j + 1]
+            w0 = torch.linalg.vector_norm(w)
             h = Vj @ w
             w = w - Vj.T @ h
-            h2 = Vj @ w
-            w = w - Vj.T @ h2
-            h = h + h2
             hn = torch.linalg.vector_norm(w)
-            hv = torch.cat([h, hn.reshape(1)]).double().cpu().numpy()
+            hv = torch.cat([h, hn.reshape(1), w0.reshape(1)]).double().cpu().numpy()
+            if reorthogonalize == "always" or not hv[j + 1] > DGKS_ETA * hv[j + 2]:
+                h2 = Vj @ w
+                w = w - Vj.T @ h2
+                h = h + h2
+                hn = torch.linalg.vector_norm(w)
+                hv = torch.cat([h, hn.reshape(1)]).double().cpu().numpy()
+            hv = hv[: j + 2]
             H[: j + 2, j] = hv
             if hv[j + 1] > 0:
                 V[j + 1] = w / hn
