@@ -178,10 +178,13 @@ __global__ void __launch_bounds__(256) k_bell_spmv(int32_t C, int32_t nslot, con
     T xv[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) xv[k] = x[(int64_t)col * B + k];
+    // the block values are read exactly once per product: non-temporal loads keep them from
+    // evicting the gathered x rows from L2 (FP32 K5 product 194 -> 171 us)
 #pragma unroll
     for (int r = 0; r < B; ++r)
 #pragma unroll
-      for (int k = 0; k < B; ++k) acc[r] += vals[((int64_t)s * B * B + r * B + k) * C + c] * xv[k];
+      for (int k = 0; k < B; ++k)
+        acc[r] += __builtin_nontemporal_load(&vals[((int64_t)s * B * B + r * B + k) * C + c]) * xv[k];
   }
 #pragma unroll
   for (int r = 0; r < B; ++r) y[(int64_t)c * B + r] = acc[r];
