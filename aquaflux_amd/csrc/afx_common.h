@@ -203,19 +203,22 @@ struct In<dual> {
 
 template <class S>
 struct Out;
+// Per-cell outputs are written once and read back only by a later kernel, after the L2 has cycled
+// through other data: non-temporal stores leave the L2 to the neighbour gathers of the running
+// kernel (about 1% on the SST channel step).
 template <>
 struct Out<double> {
   double* v = nullptr;
   double* t = nullptr;
-  AFX_DEV void set(int64_t i, double x) const { v[i] = x; }
+  AFX_DEV void set(int64_t i, double x) const { __builtin_nontemporal_store(x, &v[i]); }
 };
 template <>
 struct Out<dual> {
   double* v = nullptr;
   double* t = nullptr;
   AFX_DEV void set(int64_t i, dual x) const {
-    v[i] = x.v;
-    t[i] = x.d;
+    __builtin_nontemporal_store(x.v, &v[i]);
+    __builtin_nontemporal_store(x.d, &t[i]);
   }
 };
 
