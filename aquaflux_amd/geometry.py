@@ -110,6 +110,9 @@ class _GeometryFn(torch.autograd.Function):
         )
         ctx.dm = dm
         ctx.save_for_backward(X, area, nrm, fc, vol, cc)
+        # an output nobody differentiates arrives as None (the adjoint takes a null cotangent)
+        # rather than as a materialised zero array
+        ctx.set_materialize_grads(False)
         return area, nrm, fc, vol, cc
 
     @staticmethod

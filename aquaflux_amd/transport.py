@@ -40,10 +40,9 @@ def face_mass_flux(geometry, velocity) -> torch.Tensor:
     geometry."""
     n = geometry.face.normal_soa
     u = torch.as_tensor(velocity, dtype=F64, device=n.device)
-    if u.dim() == 1:
-        un = sum(u[j] * n[j] for j in range(n.shape[0]))
-    else:
-        un = sum(u[:, j] * n[j] for j in range(n.shape[0]))
+    # one product over the (d, F) normals, whose backward is one (d, F) tensor (a sum of per-component
+    # slices would build and add d full-size gradient tensors)
+    un = u @ n if u.dim() == 1 else (u.T * n).sum(dim=0)
     return un * geometry.face.area
 
 
