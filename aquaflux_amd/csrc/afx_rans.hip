@@ -82,11 +82,18 @@ AFX_DEV int lim_slot(int v, bool limf, bool lims) {
   return lims ? v - 1 : -1;
 }
 
+// The SST closure of cell c from its k, omega and gradients (pass C, defined below).
+template <int D, class S>
+AFX_DEV void closure_cell(const MeshView& m, const RansArgs<S>& a, int32_t c, S k, S w, const S (&Gu)[D][D],
+                          const S (&Gk)[D], const S (&Gw)[D], const ClosureOut& out);
+
 // ---------------------------------------------------------------------------------------------
-// pass G
-template <int D, class S, int GRAD, bool LIMF, bool LIMS>
+// pass G.  With CLO the cell-local closure (pass C) follows in the same thread from the gradient
+// still in registers, saving the closure pass's re-read of k, omega and the gradients.
+template <int D, class S, int GRAD, bool LIMF, bool LIMS, bool CLO = false>
 __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Out<S> G, Out<S> psi_out,
-                                                   int32_t* __restrict__ ext, int32_t* __restrict__ kmin_out) {
+                                                   int32_t* __restrict__ ext, int32_t* __restrict__ kmin_out,
+                                                   ClosureOut co = ClosureOut{}) {
   constexpr int NV = D + 3, NL = D + 2;
   const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
@@ -258,6 +265,19 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
       if (kmin_out) kmin_out[(int64_t)l * m.C + c] = ks[l];
     }
   }
+  if constexpr (CLO) {
+    S Gu[D][D], Gk[D], Gw[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) Gu[i][j] = grad[i][j];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      Gk[j] = grad[D + 1][j];
+      Gw[j] = grad[D + 2][j];
+    }
+    closure_cell<D, S>(m, a, c, xc[D + 1], xc[D + 2], Gu, Gk, Gw, co);
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -305,11 +325,8 @@ AFX_DEV S wall_shear_rate(const RansArgs<S>& a, const MeshView& m, int32_t c, do
 }
 
 template <int D, class S>
-__global__ void __launch_bounds__(256) k_rans_closure(MeshView m, RansArgs<S> a, ClosureOut out) {
-  const int32_t c = xcd_thread_index(m);
-  if (c >= m.C) return;
-  S k, w, Gu[D][D], Gk[D], Gw[D];
-  load_closure_inputs<D, S>(a, m, c, k, w, Gu, Gk, Gw);
+AFX_DEV void closure_cell(const MeshView& m, const RansArgs<S>& a, int32_t c, S k, S w, const S (&Gu)[D][D],
+                          const S (&Gk)[D], const S (&Gw)[D], const ClosureOut& out) {
   const bool wall = a.wallc[c] != 0;
   const double d = a.dwall[c];
   double dg[D];
@@ -326,6 +343,15 @@ __global__ void __launch_bounds__(256) k_rans_closure(MeshView m, RansArgs<S> a,
   Out<S>{out.nutw, out.nutw_t}.set(c, o.nutw);
   Out<S>{out.tau, out.tau_t}.set(c, tau);
   Out<S>{out.mueff, out.mueff_t}.set(c, mueff);
+}
+
+template <int D, class S>
+__global__ void __launch_bounds__(256) k_rans_closure(MeshView m, RansArgs<S> a, ClosureOut out) {
+  const int32_t c = xcd_thread_index(m);
+  if (c >= m.C) return;
+  S k, w, Gu[D][D], Gk[D], Gw[D];
+  load_closure_inputs<D, S>(a, m, c, k, w, Gu, Gk, Gw);
+  closure_cell<D, S>(m, a, c, k, w, Gu, Gk, Gw, out);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1376,8 +1402,20 @@ int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S
   constexpr bool CG = GRAD == kGradCorrected;
   constexpr int GK = CG ? 0 : GRAD;
   constexpr bool LF = ADVF == kAdvLimited, LS = ADVS == kAdvLimited;
-  AFX_LAUNCH(kKRansGrad, (k_rans_grad<D, S, GK, LF && !CG, LS && !CG>), grid_for(C), dim3(kBlock), 0, s, m, a, oG,
-             oP, ext, ext ? w.kmin : nullptr);
+  ClosureOut co{};
+  {
+    double** vv[7] = {&co.nut, &co.f1, &co.Sr, &co.cd, &co.nutw, &co.tau, &co.mueff};
+    double** tt[7] = {&co.nut_t, &co.f1_t, &co.Sr_t, &co.cd_t, &co.nutw_t, &co.tau_t, &co.mueff_t};
+    for (int i = 0; i < 7; ++i) {
+      *vv[i] = w.cl + i * C;
+      *tt[i] = clt ? clt + i * C : nullptr;
+    }
+  }
+  // the closure rides on the gradient pass unless corrected-GG sweeps still change the gradient
+  // (the dual-number build keeps them apart: fused, it would drop to one wave per SIMD)
+  constexpr bool FUSE = !CG && std::is_same<S, double>::value;
+  AFX_LAUNCH(kKRansGrad, (k_rans_grad<D, S, GK, LF && !CG, LS && !CG, FUSE>), grid_for(C), dim3(kBlock), 0, s, m,
+             a, oG, oP, ext, ext ? w.kmin : nullptr, co);
   AFX_CHECK_LAUNCH();
   a.G.v = w.G;
   a.G.t = Gt;
@@ -1409,15 +1447,10 @@ int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S
   }
   a.psi.v = w.psi;
   a.psi.t = psit;
-  ClosureOut co{};
-  double** vv[7] = {&co.nut, &co.f1, &co.Sr, &co.cd, &co.nutw, &co.tau, &co.mueff};
-  double** tt[7] = {&co.nut_t, &co.f1_t, &co.Sr_t, &co.cd_t, &co.nutw_t, &co.tau_t, &co.mueff_t};
-  for (int i = 0; i < 7; ++i) {
-    *vv[i] = w.cl + i * C;
-    *tt[i] = clt ? clt + i * C : nullptr;
+  if (!FUSE) {
+    AFX_LAUNCH(kKSstClosure, (k_rans_closure<D, S>), grid_for(C), dim3(kBlock), 0, s, m, a, co);
+    AFX_CHECK_LAUNCH();
   }
-  AFX_LAUNCH(kKSstClosure, (k_rans_closure<D, S>), grid_for(C), dim3(kBlock), 0, s, m, a, co);
-  AFX_CHECK_LAUNCH();
   bind_closure<S>(a, w.cl, clt, C);
 #ifndef AFX_WG_PER_CU
 #define AFX_WG_PER_CU 0
