@@ -14,6 +14,7 @@ Jacobi of the Newton-Krylov inner solves.
 from __future__ import annotations
 
 import ctypes
+import copy
 import dataclasses
 
 import numpy as np
@@ -250,6 +251,17 @@ class ProbedJacobian:
                 for b in range(B):
                     J[a * C + r, b * C + j] = vals[s, a, b][keep]
         return J
+
+
+def shifted_jacobian(jacobian: ProbedJacobian, shift_diagonal) -> ProbedJacobian:
+    """A copy of ``jacobian`` with the pseudo-transient shift ``shift_diagonal`` (a field-major FP64
+    vector, one entry per row) added to its diagonal (``solve/sparse_jacobian.py:893-927``); the
+    point-block Jacobi inverses of the copy are refreshed.  The input is left unchanged."""
+    out = copy.copy(jacobian)
+    out.vals = jacobian.vals.clone()
+    out.inv = jacobian.inv.clone()
+    out.add_diagonal(torch.as_tensor(shift_diagonal, dtype=torch.float64, device=jacobian.device))
+    return out
 
 
 def materialize_block_jacobian(matvec, plan, n_fields: int | None = None, *, device=None,

@@ -155,3 +155,22 @@ def test_probed_jacobian_preconditions_the_rans_newton_step(cuda):
     assert info.converged and info.iterations <= 2
     _, Jd = rans.residual_jvp(st, delta)
     assert float((Jd + r).norm()) <= 1e-8 * float(r.norm())
+
+
+def test_shifted_jacobian_adds_the_shift_and_leaves_the_input(cuda):
+    """shifted_jacobian (sparse_jacobian.py:893-927): J + diag(shift) as a new matrix, J unchanged."""
+    from aquaflux_amd.jacobian import shifted_jacobian
+
+    mesh, matvec, reach = _scalar(cuda)
+    plan = mesh_probe_plan(mesh, reach)
+    J = materialize_block_jacobian(matvec, plan, device=cuda, precision="fp64")
+    n = mesh.n_cells
+    v = torch.randn(n, dtype=F64, device=cuda)
+    before = J.field_matvec(v)
+    shift = torch.linspace(0.5, 2.0, n, dtype=F64, device=cuda)
+    Js = shifted_jacobian(J, shift)
+    assert torch.allclose(Js.field_matvec(v), before + shift * v, rtol=1e-13, atol=1e-13)
+    assert torch.equal(J.field_matvec(v), before)
+    # the shifted copy's point-block Jacobi inverts its own (shifted) 1x1 diagonal blocks
+    d = Js.to_dense().diagonal().to(cuda)
+    assert torch.allclose(Js.from_cell_major(Js.pbj(Js.to_cell_major(v))), v / d, rtol=1e-13, atol=0.0)
