@@ -275,7 +275,10 @@ class DampedNewtonStep:
 
     def pseudo_time_shift(self, op, phi, cfl):
         """Diagonal ``a_P / CFL`` on the momentum rows (zero on continuity): a local pseudo-time step
-        scaled by the momentum diagonal, as the reference's pseudo-transient shift basis is."""
+        scaled by the momentum diagonal, as the reference's pseudo-transient shift basis is.  An
+        operator with its own basis (``pseudo_time_diagonal``, e.g. the coupled RANS) supplies it."""
+        if hasattr(op, "pseudo_time_diagonal"):
+            return op.pseudo_time_diagonal(phi) / cfl
         _, aP, _ = op.fields(phi)
         d = op.mesh.dim
         shift = torch.zeros_like(phi).reshape(d + 1, -1)

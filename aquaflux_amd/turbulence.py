@@ -499,6 +499,24 @@ class CoupledRANS:
     def eddy_viscosity(self, state):
         return self.closure_fields(state)["nu_t"]
 
+    def pseudo_time_diagonal(self, state):
+        """The pseudo-transient shift basis at ``state`` (``turbulence/coupled.py:607-640``): the
+        momentum diagonal a_P on every velocity row, zero on continuity, and on the k and omega rows
+        the same transport diagonal per unit density, a_P / rho, times the coordinate factor
+        d(phi)/d(w) of their transform (phi for LogScalars, 1 for DirectScalars)."""
+        f = self.closure_fields(state)
+        d, C = self.dim, self.n_cells
+        aP = f["a_p"][0]
+        _, rho = self.momentum.material()
+        t = aP / rho.detach()
+        blocks = state.reshape(d + 3, C)
+        diag = torch.zeros_like(blocks)
+        diag[:d] = aP[None, :]
+        phys = self.physical_fields(state)
+        for row, transform, val in ((d + 1, self.k_transform, phys[1]), (d + 2, self.omega_transform, phys[2])):
+            diag[row] = t * (val if isinstance(transform, LogScalars) else 1.0)
+        return diag.reshape(-1)
+
 
 class MomentumScaled:
     """Copy of a momentum assembler with its molecular viscosity scaled (``flow/momentum.py``)."""

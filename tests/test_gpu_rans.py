@@ -343,3 +343,19 @@ def test_rans_outlet_objective_vjp_matches_central_differences(cuda):
         print(label, ad, fd, flush=True)
         assert abs(fd) > 1e-8 * scale, label  # every derivative is exercised
         assert abs(ad - fd) <= 1e-6 * max(abs(fd), 1e-3 * scale), label
+
+
+def test_rans_pseudo_time_diagonal(cuda):
+    """The pseudo-transient shift basis of the coupled RANS (turbulence/coupled.py:607-640): a_P on
+    the velocity rows, zero on continuity, a_P / rho times d(phi)/d(w) on the k and omega rows."""
+    mesh, rans, _ = RC.build("channel2d_sst", "limited", "limited", "lsq", nu=1e-3, device=cuda, k_log=False,
+                             omega_log=True)
+    st = RC.to_solved(torch.tensor(RC.seeded_state(mesh, 3, "channel")), mesh.dim, False, True).to(cuda)
+    d, C = mesh.dim, mesh.n_cells
+    diag = rans.pseudo_time_diagonal(st).reshape(d + 3, C)
+    aP = rans.closure_fields(st)["a_p"][0]
+    assert torch.equal(diag[0], aP) and torch.equal(diag[1], aP)
+    assert torch.equal(diag[d], torch.zeros_like(aP))
+    assert torch.allclose(diag[d + 1], aP, rtol=1e-15, atol=0.0)  # rho = 1, direct k
+    omega = torch.exp(st.reshape(d + 3, C)[d + 2])
+    assert torch.allclose(diag[d + 2], aP * omega, rtol=1e-14, atol=0.0)
