@@ -740,6 +740,10 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
   double xb = 0.0, gamb = 0.0, ccb[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) ccb[j] = 0.0;
+  double Gc[D];  // the forward gradient (least-squares geometry cotangents)
+#pragma unroll
+  for (int j = 0; j < D; ++j) Gc[j] = 0.0;
+  if (GRAD == 1 && GEO) load_grad_row<D, double>(a.G, m.C, c, Gc);
   double zero[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) zero[j] = 0.0;
@@ -782,13 +786,37 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
 #pragma unroll
         for (int j = 0; j < D; ++j) dx[j] = m.cc[(int64_t)j * m.C + o] - m.cc[(int64_t)j * m.C + c];
         const double wgt = 1.0 / vdot<D>(dx, dx);
-        double own_dot = 0.0, oth_dot = 0.0;
+        double own_dot = 0.0, oth_dot = 0.0, Bo[D];
 #pragma unroll
         for (int j = 0; j < D; ++j) {
+          Bo[j] = bar.G[(int64_t)j * m.C + o];
           own_dot += dx[j] * Bc[j];
-          oth_dot += dx[j] * bar.G[(int64_t)j * m.C + o];
+          oth_dot += dx[j] * Bo[j];
         }
         xb += -wgt * own_dot - wgt * oth_dot;
+        if (GEO) {
+          // G = M^-1 b with M = sum w d d^T, b = sum w d dphi, w = 1 / d.d; with y = M^-1 Gbar
+          // (bar.G here) and the fit residual r = dphi - G.d of the face, the cotangent of its d is
+          // w (r y - (y.d) G) - 2 w^2 (y.d) r d.  c's own fit runs along d = x_o - x_c, o's along -d;
+          // both ends are cell centroids, of which this thread owns x_c.
+          double Go[D];
+          load_grad_row<D, double>(a.G, m.C, o, Go);
+          const double po = a.phi[o];
+          double gd_c = 0.0, gd_o = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+            gd_c += Gc[j] * dx[j];
+            gd_o -= Go[j] * dx[j];
+          }
+          const double rc = (po - xc) - gd_c, ro = (xc - po) - gd_o;
+          const double yd_c = own_dot, yd_o = -oth_dot;
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+            const double db_c = wgt * (rc * Bc[j] - yd_c * Gc[j]) - 2.0 * wgt * wgt * yd_c * rc * dx[j];
+            const double db_o = wgt * (ro * Bo[j] - yd_o * Go[j]) + 2.0 * wgt * wgt * yd_o * ro * dx[j];
+            ccb[j] += db_o - db_c;
+          }
+        }
       }
       if (LIM) {
         if (ext[o] == c) xb += bar.ext[o];
@@ -819,6 +847,15 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
         for (int j = 0; j < D; ++j) s += q.dp[j] * Bc[j];
         phib = wgt * s;
         xb -= wgt * s;
+        if (GEO) {  // the fit's direction d = x_f - x_c (folded into x_f, x_c below)
+          const double pbv = scalar_bval<D, double>(a.kind[r.fu], q, xc, zero, a.val[r.fu], a.gam[c], a.tinf[r.fu]);
+          double gd = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) gd += Gc[j] * q.dp[j];
+          const double rr = (pbv - xc) - gd;
+#pragma unroll
+          for (int j = 0; j < D; ++j) qb.dp[j] += wgt * (rr * Bc[j] - s * Gc[j]) - 2.0 * wgt * wgt * s * rr * q.dp[j];
+        }
       }
       ScalarBar<D> Bl;
       Bl.zero();
@@ -1171,9 +1208,9 @@ extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const af
   const bool geo = gb.area || gb.normal || gb.face_centroid || gb.volume || gb.cell_centroid;
   AFX_REQUIRE(!geo || (gb.area && gb.normal && gb.face_centroid && gb.volume && gb.cell_centroid),
               "afx_scalar_vjp: geometry cotangents need all five arrays");
-  AFX_REQUIRE(!geo || cfg->gradient == 0 || cfg->gradient == 2,
-              "afx_scalar_vjp: geometry cotangents are available for compact Green-Gauss or no gradient "
-              "reconstruction");
+  AFX_REQUIRE(!geo || cfg->gradient == 0 || cfg->gradient == 1 || cfg->gradient == 2,
+              "afx_scalar_vjp: geometry cotangents are available for compact Green-Gauss, least-squares or no "
+              "gradient reconstruction");
   ScWork w = sc_carve(work, m->dim, m->C, 2);
   ScArgs<double> a = sc_args<double>(cfg, in, nullptr);
   AFX_REQUIRE(cfg->transient == 0 || out->phi_old, "afx_scalar_vjp: the transient term needs a phi_old cotangent");
@@ -1209,7 +1246,7 @@ extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const af
       AFX_CHECK_LAUNCH();
       return 0;
     };
-    if constexpr (GR == 1 || GR == kGradCorrected) {
+    if constexpr (GR == kGradCorrected) {
       return run(std::false_type{});
     } else {
       return geo ? run(std::true_type{}) : run(std::false_type{});

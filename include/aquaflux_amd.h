@@ -227,7 +227,8 @@ typedef struct {
   double* flux;  /* [F] */
   double* value; /* [F] */
   double* t_inf; /* [F] */
-  afx_geometry_cotangents geometry; /* all five set, or all NULL (compact Green-Gauss / no gradient only) */
+  afx_geometry_cotangents geometry; /* all five set, or all NULL (compact Green-Gauss, least squares or no
+                                      * gradient; refused for corrected Green-Gauss) */
   double* phi_old;   /* [C] (transient only) */
   double* phi_older; /* [C] (BDF2 only) */
 } afx_scalar_cotangents;

@@ -92,9 +92,9 @@ def test_scalar_matches_golden(name, cuda):
     _same_mesh(mesh, z)
     transient = first is not None
     T = lambda a, **kw: torch.tensor(a, dtype=F64, device=cuda, **kw)  # noqa: E731
-    # node-coordinate cotangents are built for compact Green-Gauss and gradient-free schemes; the
-    # least-squares and corrected schemes refuse them (checked below) and are compared on the rest
-    node_adjoint = grad in ("gg", "none")
+    # node-coordinate cotangents are built for compact Green-Gauss, least-squares and gradient-free
+    # schemes; the corrected scheme refuses them (checked below) and is compared on the rest
+    node_adjoint = grad in ("gg", "lsq", "none")
     X = T(z["node_coords"], requires_grad=node_adjoint)
     geom, st = SC.build_gpu(mesh, bcs, adv, grad, z["gamma"], cuda, node_coords=X, transient=transient)
     hist = (T(z["phi_old"]), None if first else T(z["phi_older"]), float(z["dt"]), bool(first)) if transient else ()

@@ -33,7 +33,11 @@ CASES = [
     ("brick3d", "limited", "gg"),
     ("brick3d", "limited", "lsq"),
 ]
-GEO_CASES = [c for c in CASES if c[2] not in ("lsq", "cgg")]
+# Node-coordinate adjoints: every scheme but the corrected one.  tet3 with limited least squares is
+# left out: at that case's seed two limiter candidates of some cells are exactly equal, where the
+# oracle (like JAX's segment_min) splits the derivative between the tied faces and the kernels take
+# the first (a documented parity-unpinned point); brick3d and quad_mixed cover the scheme.
+GEO_CASES = [c for c in CASES if c[2] != "cgg" and c != ("tet3", "limited", "lsq")]
 
 
 def _ids(c):
