@@ -231,8 +231,12 @@ class DampedNewtonStep:
     it.
 
     ``preconditioner``: ``None`` (plain GMRES on the JVP), ``"pbj"`` (point-block Jacobi of the
-    block Jacobian) or ``"inner"`` (FGMRES whose preconditioner is GMRES(``inner_restart``) on the
-    block Jacobian with point-block Jacobi); ``precision`` is the block Jacobian's arithmetic.
+    block Jacobian), ``"inner"`` (FGMRES whose preconditioner is GMRES(``inner_restart``) on the
+    block Jacobian with point-block Jacobi) or ``"block"`` (FGMRES with the SIMPLE-family
+    ``block_preconditioner``, a :class:`~aquaflux_amd.block_preconditioner.BlockPreconditioner` of
+    the flow assembler, built with its defaults when None, its saddle blocks read off the block
+    Jacobian and its Schur built at the pseudo-time-shifted ``a_P``); ``precision`` is the block
+    Jacobian's arithmetic.
     ``jacobian``: ``"compact"`` (the flow residual's reach-1 dual-number assembly with gradients,
     a_P and limiters frozen) or ``"probed"`` (the exact Jacobian of any residual by coloured probing
     of its tangent, ``sparse_jacobian.py:722-848``, with ``column_reach`` per state field; the plan
@@ -256,6 +260,7 @@ class DampedNewtonStep:
     ptc_cfl_max: float = 1e12
     jacobian: str = "compact"
     column_reach: tuple | None = None
+    block_preconditioner: object = None
     _plans: dict = dataclasses.field(default_factory=dict, repr=False)
 
     def block_jacobian(self, op, phi):
@@ -306,6 +311,18 @@ class DampedNewtonStep:
                     z, _ = gmres(J.matvec, vc, rtol=self.inner_rtol, restart=self.inner_restart,
                                  max_iter=self.inner_iters, precond=J.pbj)
                     return J.from_cell_major(z)
+            elif self.preconditioner == "block":
+                flexible = True
+                bp = self.block_preconditioner
+                if bp is None:
+                    from .block_preconditioner import BlockPreconditioner
+
+                    bp = BlockPreconditioner.build(op)
+                    self.block_preconditioner = bp
+                elif bp.assembler is not op:
+                    raise ValueError("block_preconditioner was built for another assembler")
+                a_p = bp.frozen_momentum_diagonal(phi)
+                precond = bp.apply_at(phi, J, a_p if shift is None else a_p * (1.0 + 1.0 / cfl))
             else:
                 raise ValueError(f"unknown preconditioner {self.preconditioner!r}")
 

@@ -393,9 +393,11 @@ def tet_cpu_baseline(n=32, reps=5):
             f"({C} cells, {F} faces), median {tm:.2f} s each"}
 
 
-def newton_compare(device, nx=200, ny=100, nz=100, mu=1e-3, max_steps=80, cfl0=10.0):
-    """Newton to ||R|| <= 1e-12 + 1e-10 ||R0|| on the Re = 1000 hex box (u, v, w, p), from the seeded
-    state, with the inner Krylov solve on the compact block Jacobian in FP32 and in FP64."""
+def newton_compare(device, nx=200, ny=100, nz=100, mu=1e-3, max_steps=80, cfl0=10.0, preconditioner="inner"):
+    """Newton to ||R|| <= 1e-12 + 1e-10 ||R0|| on the hex box (u, v, w, p) at Re = 1/mu, from the
+    seeded state, with the compact block Jacobian in FP32 and in FP64 under ``preconditioner``:
+    "inner" (GMRES with point-block Jacobi) or "block-<composition>" (the SIMPLE-family block
+    preconditioner with that composition)."""
     import torch
 
     from aquaflux_amd.boundary import BoundaryConditions
@@ -404,8 +406,11 @@ def newton_compare(device, nx=200, ny=100, nz=100, mu=1e-3, max_steps=80, cfl0=1
     from aquaflux_amd.mesh import structured_grid_3d
     from aquaflux_amd.properties import Constant, PropertyModel
     from aquaflux_amd.schemes import LeastSquaresGradient
+    from aquaflux_amd.block_preconditioner import COMPOSITIONS, BlockPreconditioner
     from aquaflux_amd.solve import DampedNewtonStep, ImplicitNewtonSolver
 
+    if preconditioner != "inner" and preconditioner.removeprefix("block-") not in COMPOSITIONS:
+        raise ValueError(f"unknown Newton preconditioner {preconditioner!r}")
     mesh = structured_grid_3d(nx, ny, nz, named_boundaries=True)
     geom = mesh.geometry(device=device)
     bcs = BoundaryConditions({"left": VelocityInlet((1.0, 0.0, 0.0)), "right": PressureOutlet(0.0),
@@ -418,14 +423,21 @@ def newton_compare(device, nx=200, ny=100, nz=100, mu=1e-3, max_steps=80, cfl0=1
                                       0.05 * rng.normal(size=C), rng.normal(0.0, 0.1, C)]), dtype=torch.float64,
                       device=device)
     t_setup = time.perf_counter()
+
+    def make_step(prec, **kw):
+        if preconditioner == "inner":
+            return DampedNewtonStep(preconditioner="inner", precision=prec, **kw)
+        bp = BlockPreconditioner.build(mc, composition=preconditioner.removeprefix("block-"))
+        return DampedNewtonStep(preconditioner="block", precision=prec, block_preconditioner=bp, **kw)
+
     for prec in ("fp32", "fp64"):  # warm up every kernel and allocation outside the timed solves
-        DampedNewtonStep(preconditioner="inner", precision=prec, max_iter=2, inner_iters=2).correction(
-            mc, x0, mc.residual_raw(x0))
+        make_step(prec, max_iter=2, inner_iters=2).correction(mc, x0, mc.residual_raw(x0))
     torch.cuda.synchronize()
     print(f"newton: setup + warmup {time.perf_counter() - t_setup:.2f} s", file=sys.stderr, flush=True)
     out = {"workload": f"hex box {nx}x{ny}x{nz} ({C} cells, {4 * C} unknowns), Re={1.0 / mu:g}, first-order upwind, "
                        "least-squares gradients, seeded state; pseudo-transient continuation (SER, CFL0="
-                       f"{cfl0:g}) into Newton; stop at ||R|| <= 1e-12 + 1e-10 ||R0||"}
+                       f"{cfl0:g}) into Newton; stop at ||R|| <= 1e-12 + 1e-10 ||R0||",
+           "preconditioner": preconditioner}
     roots = {}
     for prec in ("fp32", "fp64"):
         def progress(step, rel, kits, alpha, secs, prec=prec):
@@ -433,8 +445,7 @@ def newton_compare(device, nx=200, ny=100, nz=100, mu=1e-3, max_steps=80, cfl0=1
                   f"{secs:.2f} s", file=sys.stderr, flush=True)
 
         solver = ImplicitNewtonSolver(max_steps=max_steps, progress=progress,
-                                      forward_step=DampedNewtonStep(preconditioner="inner", precision=prec,
-                                                                    ptc_cfl0=cfl0))
+                                      forward_step=make_step(prec, ptc_cfl0=cfl0))
         torch.cuda.synchronize()
         phi, rep = solver.forward(mc, x0)
         roots[prec] = phi
@@ -483,6 +494,9 @@ def main(argv=None):
     ap.add_argument("--newton-only", action="store_true", help="only the Newton comparison")
     ap.add_argument("--newton-size", type=int, nargs=3, default=(200, 100, 100), metavar=("NX", "NY", "NZ"))
     ap.add_argument("--newton-cfl0", type=float, default=10.0)
+    ap.add_argument("--newton-mu", type=float, default=1e-3, help="viscosity of the Newton box (Re = 1/mu)")
+    ap.add_argument("--newton-preconditioner", default="inner",
+                    choices=["inner", "block-triangular", "block-simple", "block-simpler"])
     ap.add_argument(
         "--calibrate-copy",
         type=int,
@@ -508,7 +522,8 @@ def main(argv=None):
 
     dev = torch.device("cuda", 0)
     if args.newton_only:
-        print(json.dumps(newton_compare(dev, *args.newton_size, cfl0=args.newton_cfl0)), flush=True)
+        print(json.dumps(newton_compare(dev, *args.newton_size, mu=args.newton_mu, cfl0=args.newton_cfl0,
+                                        preconditioner=args.newton_preconditioner)), flush=True)
         return 0
     wl = args.workload
     if wl == "tet":
@@ -618,7 +633,8 @@ def main(argv=None):
                                                   "roofline")}
         torch.cuda.empty_cache()
     if not args.no_newton:
-        out["newton"] = newton_compare(dev, *args.newton_size, cfl0=args.newton_cfl0)
+        out["newton"] = newton_compare(dev, *args.newton_size, mu=args.newton_mu, cfl0=args.newton_cfl0,
+                                       preconditioner=args.newton_preconditioner)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, *args.cpu_size, reps=args.cpu_reps)
     print(json.dumps(out), flush=True)

@@ -449,6 +449,44 @@ def normal_distance(m, g):
     return torch.where(m.interior, dot(d, g.normal), dot(g.face_centroid - x_p, g.normal))
 
 
+def schur_face_coefficient(m, g, a_p, rho):
+    """Per-face SIMPLE Schur coefficient ``rho_f (V/a_P)_f A / (d.n)``, blended ``(1 - g) P + g N``,
+    zero on boundary faces (``flow/preconditioner.py:43-81``)."""
+    fac = interpolation_factor(m, g)
+    d = g.volume / a_p
+    rho_f = blend(m, rho, rho, fac)
+    d_f = blend(m, d, d, fac)
+    c = rho_f * d_f * g.area / normal_distance(m, g)
+    return torch.where(m.interior, c, torch.zeros_like(c))
+
+
+def pressure_schur_matrix(m, g, a_p, rho, outlet_faces=(), pin=None):
+    """Dense pressure Schur: the graph Laplacian of the interior face coefficients
+    (``solve/frozen_operator.py:214-271`` without a flux) plus the outlet stiffness
+    ``rho_P (V/a_P)_P A / (d.n)`` on each pressure-fixing boundary face's owner
+    (``flow/boundary.py:437-441``, ``flow/block_preconditioner.py:191-209``), with a pinned cell
+    decoupled (``solve/frozen_operator.py:274-299``)."""
+    n = m.n_cells
+    c = schur_face_coefficient(m, g, a_p, rho)
+    S = torch.zeros((n, n), dtype=F64)
+    for f in torch.nonzero(m.interior).flatten().tolist():
+        P, N = int(m.owner[f]), int(m.neighbour[f])
+        S[P, P] += c[f]
+        S[N, N] += c[f]
+        S[P, N] -= c[f]
+        S[N, P] -= c[f]
+    nd = normal_distance(m, g)
+    d = g.volume / a_p
+    for f in outlet_faces:
+        P = int(m.owner[f])
+        S[P, P] += rho[P] * d[P] * g.area[f] / nd[f]
+    if pin is not None:
+        S[pin, :] = 0.0
+        S[:, pin] = 0.0
+        S[pin, pin] = 1.0
+    return S
+
+
 def advective_momentum_flux(m, g, vel, rho, fac, grad_vel):
     """``flow/rhie_chow.py:220-271``: ``(rho u)_f . n`` reconstructed to the integration point."""
     mom = scale(vel, rho)
