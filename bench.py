@@ -393,7 +393,7 @@ def tet_cpu_baseline(n=32, reps=5):
             f"({C} cells, {F} faces), median {tm:.2f} s each"}
 
 
-def newton_compare(device, nx=200, ny=100, nz=100, mu=1e-3, max_steps=80, cfl0=10.0, preconditioner="inner"):
+def newton_compare(device, nx=200, ny=100, nz=100, mu=1e-3, max_steps=80, cfl0=10.0, preconditioner="block-simpler"):
     """Newton to ||R|| <= 1e-12 + 1e-10 ||R0|| on the hex box (u, v, w, p) at Re = 1/mu, from the
     seeded state, with the compact block Jacobian in FP32 and in FP64 under ``preconditioner``:
     "inner" (GMRES with point-block Jacobi) or "block-<composition>" (the SIMPLE-family block
@@ -495,7 +495,7 @@ def main(argv=None):
     ap.add_argument("--newton-size", type=int, nargs=3, default=(200, 100, 100), metavar=("NX", "NY", "NZ"))
     ap.add_argument("--newton-cfl0", type=float, default=10.0)
     ap.add_argument("--newton-mu", type=float, default=1e-3, help="viscosity of the Newton box (Re = 1/mu)")
-    ap.add_argument("--newton-preconditioner", default="inner",
+    ap.add_argument("--newton-preconditioner", default="block-simpler",
                     choices=["inner", "block-triangular", "block-simple", "block-simpler"])
     ap.add_argument(
         "--calibrate-copy",
