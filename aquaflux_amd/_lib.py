@@ -199,6 +199,12 @@ _SIGNATURES = {
     "afx_bell_spmv": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "afx_pbj_setup": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "afx_pbj_apply": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    "afx_ell_jacobi": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                       ctypes.c_double, c_void_p],
+    "afx_ell_residual": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                         c_void_p],
+    "afx_aggregate_restrict": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    "afx_aggregate_prolong": [ctypes.c_int64, ctypes.c_int, c_void_p, c_void_p, ctypes.c_double, c_void_p, c_void_p],
     "afx_profile_enable": [ctypes.c_int64],
     "afx_profile_collect": [c_void_p, c_void_p],
     "afx_profile_kernel_count": [],

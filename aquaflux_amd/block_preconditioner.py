@@ -114,6 +114,8 @@ class AggregationLevel:
     cols: torch.Tensor  # (nslot, n) int32
     agg: torch.Tensor | None  # (n,) int64; None on the coarsest level
     gmap: torch.Tensor | None  # (nslot * n,) int64 into the next level's (nslot', n') values
+    agg32: torch.Tensor | None = None  # (n,) int32, the kernels' copy of agg
+    members: torch.Tensor | None = None  # (max_members, n') int32 rows of each aggregate, -1 padded
 
 
 class AggregationHierarchy:
@@ -186,7 +188,13 @@ class AggregationHierarchy:
             cols_c = np.repeat(np.arange(nc, dtype=np.int64)[None, :], ns_c, axis=0)
             cols_c[slot, rc] = cc
             gmap = slot[inv] * nc + rc[inv]
-            levels.append(AggregationLevel(n, nslot, T(cols.astype(np.int32)), T(agg.astype(np.int64)), T(gmap)))
+            order = np.argsort(agg, kind="stable")
+            size = np.bincount(agg, minlength=nc)
+            rank = np.arange(n) - np.repeat(np.concatenate([[0], np.cumsum(size)[:-1]]), size)
+            members = np.full((int(size.max()), nc), -1, dtype=np.int32)
+            members[rank, agg[order]] = order
+            levels.append(AggregationLevel(n, nslot, T(cols.astype(np.int32)), T(agg.astype(np.int64)), T(gmap),
+                                           T(agg.astype(np.int32)), T(members)))
             sums = np.zeros((nc, dim))
             np.add.at(sums, agg, x)
             x = sums / np.bincount(agg, minlength=nc)[:, None]
@@ -234,6 +242,43 @@ class AggregationHierarchy:
             return x
 
         return level(0, b)
+
+    def native_cycle(self, ops, coarse_inverse, b, sweeps: int = 2, omega: float = 2.0 / 3.0,
+                     correction: float = 1.8):
+        """:meth:`cycle` on the library's kernels: a fused Jacobi sweep per smoothing step, a fused
+        residual, a member-table restriction (fixed summation order) and a fused prolongation."""
+        L = _lib.lib()
+        st = _lib.stream_of(b.device)
+        prec = 0 if b.dtype == torch.float32 else 1
+
+        def jacobi(k, rhs, x):
+            lv = self.levels[k]
+            y = torch.empty_like(rhs)
+            _lib.check(L.afx_ell_jacobi(lv.n, lv.nslot, prec, ops[k].data_ptr(), lv.cols.data_ptr(), rhs.data_ptr(),
+                                        None if x is None else x.data_ptr(), y.data_ptr(), omega, st), "afx_ell_jacobi")
+            return y
+
+        def level(k, rhs):
+            if k == len(self.levels) - 1:
+                return coarse_inverse @ rhs
+            lv, nx = self.levels[k], self.levels[k + 1]
+            x = jacobi(k, rhs, None)
+            for _ in range(sweeps - 1):
+                x = jacobi(k, rhs, x)
+            r = torch.empty_like(rhs)
+            _lib.check(L.afx_ell_residual(lv.n, lv.nslot, prec, ops[k].data_ptr(), lv.cols.data_ptr(), rhs.data_ptr(),
+                                          x.data_ptr(), r.data_ptr(), st), "afx_ell_residual")
+            rc = torch.empty(nx.n, dtype=rhs.dtype, device=rhs.device)
+            _lib.check(L.afx_aggregate_restrict(nx.n, lv.members.shape[0], prec, lv.members.data_ptr(), r.data_ptr(),
+                                                rc.data_ptr(), st), "afx_aggregate_restrict")
+            xc = level(k + 1, rc).contiguous()
+            _lib.check(L.afx_aggregate_prolong(lv.n, prec, lv.agg32.data_ptr(), xc.data_ptr(), correction,
+                                               x.data_ptr(), st), "afx_aggregate_prolong")
+            for _ in range(sweeps):
+                x = jacobi(k, rhs, x)
+            return x
+
+        return level(0, b.contiguous())
 
 
 # --- the pressure Schur (preconditioner.py:43-81, block_preconditioner.py:123-210) -------------
@@ -439,12 +484,8 @@ class BlockPreconditioner:
             H = self.hierarchy
             ops, coarse_inverse = H.operators(S)
 
-            def level_spmv(k, vals, x):
-                lv = H.levels[k]
-                return spmv(1, lv.nslot, vals, lv.cols, x, lv.n)
-
             def schur_pre(r):
-                return H.cycle(level_spmv, ops, coarse_inverse, r, sweeps=self.schur_sweeps)
+                return H.native_cycle(ops, coarse_inverse, r, sweeps=self.schur_sweeps)
 
         def schur_solve(rp):
             return conjugate_gradient(lambda x: spmv(1, self.graph.nslot, S, S_cols, x), rp.contiguous(), schur_pre,

@@ -45,6 +45,7 @@ enum KernelId : int {
   kKCggSweep,
   kKCggSweepAdj,
   kKProbeScatter,
+  kKAmg,
   kKNumKernels
 };
 

@@ -18,6 +18,7 @@ const char* const kKernelNames[kKNumKernels] = {
     "rans_grad",     "rans_ap",       "rans_ap_adj",     "rans_grad_adj",   "scalar_lim_adj",
     "geom_adj_face", "geom_adj_node",   "rans_scalar_res", "rans_flow_res_adj",
     "limiter",       "cgg_sweep",     "cgg_sweep_adj",   "probe_scatter",
+    "amg",
 };
 
 namespace {

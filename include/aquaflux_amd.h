@@ -404,6 +404,23 @@ int afx_pbj_setup(int64_t n_rows, int32_t block, int precision, const void* vals
 int afx_pbj_apply(int64_t n_rows, int32_t block, int precision, const void* inv, const void* x, void* y,
                   hipStream_t stream);
 
+/* ---- aggregation multigrid on a scalar ELL operator (the block preconditioner's pressure Schur) --
+ * Replaces the host smoothed-aggregation V-cycle of the reference's Schur solve
+ * (aquaflux/flow/block_preconditioner.py:266-343, aquaflux/solve/multigrid.py:1596-1700).
+ * ELL as afx_bell_spmv with block 1 (slot 0 the diagonal). */
+/* y = x + omega (b - A x) / diag(A); x may be NULL (a sweep from zero); x and y must differ. */
+int afx_ell_jacobi(int64_t n, int32_t nslot, int precision, const void* vals, const int32_t* cols, const void* b,
+                   const void* x, void* y, double omega, hipStream_t stream);
+/* r = b - A x (x and r must differ). */
+int afx_ell_residual(int64_t n, int32_t nslot, int precision, const void* vals, const int32_t* cols, const void* b,
+                     const void* x, void* r, hipStream_t stream);
+/* rc[I] = sum of r over the members of aggregate I: members[k * n_coarse + I], padded with -1. */
+int afx_aggregate_restrict(int64_t n_coarse, int32_t max_members, int precision, const int32_t* members,
+                           const void* r, void* rc, hipStream_t stream);
+/* x[i] += alpha xc[agg[i]]. */
+int afx_aggregate_prolong(int64_t n, int precision, const int32_t* agg, const void* xc, double alpha, void* x,
+                          hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,8 @@ def make_mesh(name):
         return perturbed_grid_2d(16, 8, lx=4.0, ly=1.0, perturb=0.2, seed=0, named_boundaries=True)
     if name == "channel3d":
         return perturbed_grid_3d(6, 4, 4, lx=2.0, perturb=0.15, seed=1, named_boundaries=True)
+    if name == "channel3d_mg":  # large enough for a three-level Schur multigrid
+        return perturbed_grid_3d(32, 16, 16, lx=2.0, perturb=0.15, seed=1, named_boundaries=True)
     if name == "box3d":
         return structured_grid_3d(5, 4, 3, named_boundaries=True)
     if name in ("brick2d", "brick3d"):

@@ -271,3 +271,14 @@ def test_v_cycle_is_symmetric():
     a = float(torch.dot(u, H.cycle(spmv, ops, cinv, v)))
     b = float(torch.dot(H.cycle(spmv, ops, cinv, u), v))
     assert abs(a - b) < 1e-12 * max(abs(a), 1.0)
+
+
+def test_member_tables_list_each_aggregate():
+    _, _, _, H = _grid_schur(16, 12, pin=37)
+    for lv in H.levels[:-1]:
+        agg = lv.agg.numpy()
+        mem = lv.members.numpy()
+        assert np.array_equal(lv.agg32.numpy(), agg)
+        for a in range(mem.shape[1]):
+            got = sorted(int(m) for m in mem[:, a] if m >= 0)
+            assert got == sorted(np.flatnonzero(agg == a).tolist())
