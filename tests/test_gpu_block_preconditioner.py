@@ -100,3 +100,28 @@ def test_native_v_cycle_matches_the_torch_cycle(case, dtype, cuda):
     print("levels", len(H.levels), "relative difference", err, flush=True)
     assert len(H.levels) >= 3
     assert err <= (1e-12 if dtype == torch.float64 else 1e-5)
+
+
+def test_block_preconditioner_over_the_probed_jacobian(cuda):
+    """The saddle blocks read off the probed exact Jacobian (reach 3 / 3 / 2) precondition Newton to
+    the same root as the inner solve on the compact Jacobian."""
+    print("newton channel2d simpler over the probed Jacobian", flush=True)
+    mesh, geom, mc = FC.build("channel2d", "upwind", "gg", device=cuda)
+    x0 = torch.tensor(FC.seeded_state(mesh, seed=0, kind="channel"), dtype=F64, device=cuda)
+    ref, _ = ImplicitNewtonSolver(forward_step=DampedNewtonStep(preconditioner="inner", precision="fp64")).forward(mc, x0)
+    bp = BlockPreconditioner.build(mc, composition="simpler")
+    step = DampedNewtonStep(preconditioner="block", precision="fp32", jacobian="probed", column_reach=(3, 3, 2),
+                            block_preconditioner=bp)
+    phi, rep = ImplicitNewtonSolver(forward_step=step).forward(mc, x0)
+    print("steps", rep.steps, "krylov", rep.krylov_iterations, flush=True)
+    assert rep.converged
+    assert float((phi - ref).abs().max()) / float(ref.abs().max()) <= 1e-8
+
+
+def test_block_preconditioner_refuses_a_foreign_assembler(cuda):
+    _, _, mc = FC.build("cavity", "upwind", "gg", device=cuda)
+    _, _, other = FC.build("cavity", "upwind", "gg", device=cuda)
+    step = DampedNewtonStep(preconditioner="block", block_preconditioner=BlockPreconditioner.build(mc))
+    x0 = other.initial_state()
+    with pytest.raises(ValueError, match="another assembler"):
+        step.correction(other, x0, other.residual_raw(x0))
