@@ -205,6 +205,8 @@ _SIGNATURES = {
                          c_void_p],
     "afx_aggregate_restrict": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "afx_aggregate_prolong": [ctypes.c_int64, ctypes.c_int, c_void_p, c_void_p, ctypes.c_double, c_void_p, c_void_p],
+    "afx_dot": [ctypes.c_int64, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "afx_dense_matvec": [ctypes.c_int64, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "afx_profile_enable": [ctypes.c_int64],
     "afx_profile_collect": [c_void_p, c_void_p],
     "afx_profile_kernel_count": [],

@@ -76,29 +76,57 @@ def compose(composition, velocity_solve, schur_solve, divergence, gradient, pres
     return {"triangular": triangular, "simple": simple, "simpler": simpler}[composition]
 
 
-def conjugate_gradient(matvec, b, preconditioner, iterations: int):
+def conjugate_gradient(matvec, b, preconditioner, iterations: int, dot=torch.dot):
     """``iterations`` preconditioned CG steps from zero, with no host synchronisation (the step
     sizes stay on the device; a zero curvature or residual leaves the iterate unchanged).
-    ``preconditioner`` is a symmetric positive map or, as a tensor, a Jacobi inverse diagonal."""
+    ``preconditioner`` is a symmetric positive map or, as a tensor, a Jacobi inverse diagonal;
+    ``dot`` returns a 0-d device tensor."""
     precond = preconditioner if callable(preconditioner) else (lambda v: v * preconditioner)
     x = torch.zeros_like(b)
     r = b.clone()
     z = precond(r)
     p = z.clone()
-    rz = torch.dot(r, z)
+    rz = dot(r, z)
     zero = torch.zeros((), dtype=b.dtype, device=b.device)
     for _ in range(int(iterations)):
         Ap = matvec(p)
-        pAp = torch.dot(p, Ap)
+        pAp = dot(p, Ap)
         alpha = torch.where(pAp > 0, rz / torch.where(pAp > 0, pAp, torch.ones_like(pAp)), zero)
         x.add_(alpha * p)
         r.sub_(alpha * Ap)
         z = precond(r)
-        rz_new = torch.dot(r, z)
+        rz_new = dot(r, z)
         beta = torch.where(rz > 0, rz_new / torch.where(rz > 0, rz, torch.ones_like(rz)), zero)
         p = z + beta * p
         rz = rz_new
     return x
+
+
+def captured(fn, n, dtype, device):
+    """``fn`` (a map of one length-``n`` vector with no host synchronisation) recorded as a HIP graph
+    on a static input buffer: the returned callable copies its argument in, replays the graph and
+    returns a copy of the static output.  One warm-up call on a side stream first, so that lazy
+    initialisation stays out of the recording.  ``fn`` must make no call that records a memory
+    clear (library reductions do), since such nodes are unreliable on replay."""
+    static_in = torch.zeros(n, dtype=dtype, device=device)
+    side = torch.cuda.Stream(device)
+    side.wait_stream(torch.cuda.current_stream(device))
+    with torch.cuda.stream(side):
+        fn(static_in)
+    torch.cuda.current_stream(device).wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        static_out = fn(static_in)
+
+    def run(x):
+        static_in.copy_(x)
+        graph.replay()
+        return static_out.clone()
+
+    # the recording reads the operators ``fn`` closes over (the Schur, the level operators, the
+    # coarse inverse) at their addresses: keep ``fn`` alive as long as the graph
+    run.graph, run.recorded = graph, fn
+    return run
 
 
 # --- aggregation multigrid for the Schur -------------------------------------------------------
@@ -215,7 +243,7 @@ class AggregationHierarchy:
         last = self.levels[-1]
         A = torch.zeros(last.n, last.n, dtype=torch.float64, device=vals.device)
         A.index_put_((self.coarse_rows, self.coarse_cols.long()), out[-1].reshape(-1).double(), accumulate=True)
-        return out, torch.linalg.pinv(A, hermitian=True).to(vals.dtype)
+        return out, torch.linalg.pinv(A, hermitian=True).to(vals.dtype).contiguous()
 
     def cycle(self, spmv, ops, coarse_inverse, b, sweeps: int = 2, omega: float = 2.0 / 3.0,
               correction: float = 1.8):
@@ -260,7 +288,10 @@ class AggregationHierarchy:
 
         def level(k, rhs):
             if k == len(self.levels) - 1:
-                return coarse_inverse @ rhs
+                y = torch.empty_like(rhs)
+                _lib.check(L.afx_dense_matvec(rhs.numel(), prec, coarse_inverse.data_ptr(), rhs.data_ptr(),
+                                              y.data_ptr(), st), "afx_dense_matvec")
+                return y
             lv, nx = self.levels[k], self.levels[k + 1]
             x = jacobi(k, rhs, None)
             for _ in range(sweeps - 1):
@@ -390,10 +421,15 @@ class BlockPreconditioner:
     ``"triangular"``, ``"simple"`` or ``"simpler"``.  ``schur_solver``: ``"amg"`` (conjugate
     gradient preconditioned by one aggregation-multigrid V-cycle with ``schur_sweeps`` Jacobi
     sweeps each way) or ``"jacobi"``; ``schur_iters`` conjugate-gradient steps per Schur solve.
+    ``capture``: record the Schur solve (its conjugate gradient and V-cycles, launch-bound on the
+    coarse levels) once per Newton step as a HIP graph and replay it (:func:`captured`); off by
+    default, as replaying the ~120-node graph measured slower than launching its kernels on the
+    2M-cell box (FP32 Newton 5.75 s against 4.94 s).
     """
 
     def __init__(self, assembler, graph, velocity, schur_scaling, composition, mass_scale, velocity_iters,
-                 velocity_rtol, schur_iters, schur_solver="jacobi", schur_sweeps=2, hierarchy=None):
+                 velocity_rtol, schur_iters, schur_solver="jacobi", schur_sweeps=2, hierarchy=None,
+                 capture=False):
         self.assembler = assembler
         self.graph = graph
         self.velocity = velocity
@@ -406,12 +442,13 @@ class BlockPreconditioner:
         self.schur_solver = schur_solver
         self.schur_sweeps = int(schur_sweeps)
         self.hierarchy = hierarchy
+        self.capture = bool(capture)
 
     @classmethod
     def build(cls, assembler, *, velocity: str = "gmres", schur_scaling: str = "simple",
               composition: str = "triangular", mass_scale: float | None = None, velocity_iters: int = 10,
               velocity_rtol: float = 1e-2, schur_solver: str = "amg", schur_iters: int | None = None,
-              schur_sweeps: int = 2) -> "BlockPreconditioner":
+              schur_sweeps: int = 2, capture: bool = False) -> "BlockPreconditioner":
         if velocity not in VELOCITY_SOLVES:
             raise ValueError(f"unknown velocity block {velocity!r}; use one of {VELOCITY_SOLVES} (the reference's "
                              "multigrid velocity hierarchies are replaced by GPU Krylov inner solves)")
@@ -432,7 +469,7 @@ class BlockPreconditioner:
         if schur_iters is None:  # two multigrid-preconditioned steps measured fastest on 2M cells
             schur_iters = 2 if schur_solver == "amg" else 30
         return cls(assembler, graph, velocity, schur_scaling, composition, mass_scale, velocity_iters,
-                   velocity_rtol, schur_iters, schur_solver, schur_sweeps, hierarchy)
+                   velocity_rtol, schur_iters, schur_solver, schur_sweeps, hierarchy, capture)
 
     def frozen_momentum_diagonal(self, state):
         """``a_P`` per cell at ``state`` (``block_preconditioner.py:1379-1389``)."""
@@ -474,9 +511,10 @@ class BlockPreconditioner:
         inv_diag = (1.0 / schur_ap).to(dt)
 
         def spmv(block, nslot, vals, cols, x, n_rows=C):
+            # the stream is read per launch, so that a graph capture records these launches too
             y = torch.empty_like(x)
             _lib.check(L.afx_bell_spmv(n_rows, block, nslot, prec, vals.data_ptr(), cols.data_ptr(), x.data_ptr(),
-                                       y.data_ptr(), st), "afx_bell_spmv")
+                                       y.data_ptr(), _lib.stream_of(x.device)), "afx_bell_spmv")
             return y
 
         schur_pre = S_inv_diag
@@ -487,9 +525,20 @@ class BlockPreconditioner:
             def schur_pre(r):
                 return H.native_cycle(ops, coarse_inverse, r, sweeps=self.schur_sweeps)
 
+        def lib_dot(a, b):
+            partial = torch.empty(256, dtype=torch.float64, device=a.device)
+            out = torch.empty((), dtype=a.dtype, device=a.device)
+            _lib.check(L.afx_dot(a.numel(), prec, a.data_ptr(), b.data_ptr(), partial.data_ptr(), out.data_ptr(),
+                                 _lib.stream_of(a.device)), "afx_dot")
+            return out
+
         def schur_solve(rp):
+            # recorded: the library's dot (no library reduction that records a memory clear)
             return conjugate_gradient(lambda x: spmv(1, self.graph.nslot, S, S_cols, x), rp.contiguous(), schur_pre,
-                                      self.schur_iters)
+                                      self.schur_iters, dot=lib_dot if self.capture else torch.dot)
+
+        if self.capture:
+            schur_solve = captured(schur_solve, C, dt, dev)
 
         full = jacobian.vals.view(ns, Bk, Bk, C)
         Fv = full[:, :d, :d, :].contiguous()

@@ -76,6 +76,52 @@ __global__ void __launch_bounds__(256) k_aggregate_prolong(int32_t n, const int3
   x[i] += alpha * xc[agg[i]];
 }
 
+// Dot product in two passes with no atomics and no cleared memory: kDotBlocks partial sums
+// (grid-stride, FP64 accumulation, LDS tree per block), then one block sums the partials.
+constexpr int kDotBlocks = 256;
+
+AFX_DEV double block_sum(double v) {
+  __shared__ double lds[256];
+  lds[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+    __syncthreads();
+  }
+  return lds[0];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_dot_partial(int64_t n, const T* __restrict__ a, const T* __restrict__ b,
+                                                     double* __restrict__ partial) {
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc += (double)a[i] * (double)b[i];
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_dot_final(const double* __restrict__ partial, T* __restrict__ out) {
+  const double s = block_sum(partial[threadIdx.x]);
+  if (threadIdx.x == 0) out[0] = (T)s;
+}
+
+// y = A x for a small dense row-major matrix (the coarsest level's pseudo-inverse): one 64-lane
+// wavefront per row, lanes striding the row (coalesced), a shuffle tree for the sum.
+template <class T>
+__global__ void __launch_bounds__(256) k_dense_matvec(int32_t n, const T* __restrict__ A, const T* __restrict__ x,
+                                                      T* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const int32_t i = (int32_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= n) return;  // i is uniform across the wavefront
+  const T* row = A + (int64_t)i * n;
+  double acc = 0.0;
+  for (int32_t j = lane; j < n; j += 64) acc += (double)row[j] * (double)x[j];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) y[i] = (T)acc;
+}
+
 template <class Fn>
 int by_precision(int precision, Fn&& fn) {
   if (precision == 0) return fn((float*)nullptr);
@@ -132,6 +178,30 @@ extern "C" int afx_aggregate_prolong(int64_t n, int precision, const int32_t* ag
     using T = std::remove_pointer_t<decltype(tp)>;
     AFX_LAUNCH(kKAmg, k_aggregate_prolong<T>, grid_for(n), dim3(kBlock), 0, s, (int32_t)n, agg, (const T*)xc,
                (T)alpha, (T*)x);
+    AFX_CHECK_LAUNCH();
+    return 0;
+  });
+}
+
+extern "C" int afx_dot(int64_t n, int precision, const void* a, const void* b, double* partial, void* out,
+                       hipStream_t s) {
+  AFX_REQUIRE(a && b && partial && out && n > 0, "afx_dot: bad argument");
+  return by_precision(precision, [&](auto* tp) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    AFX_LAUNCH(kKAmg, k_dot_partial<T>, dim3(kDotBlocks), dim3(kBlock), 0, s, n, (const T*)a, (const T*)b, partial);
+    AFX_CHECK_LAUNCH();
+    AFX_LAUNCH(kKAmg, k_dot_final<T>, dim3(1), dim3(kBlock), 0, s, (const double*)partial, (T*)out);
+    AFX_CHECK_LAUNCH();
+    return 0;
+  });
+}
+
+extern "C" int afx_dense_matvec(int64_t n, int precision, const void* A, const void* x, void* y, hipStream_t s) {
+  AFX_REQUIRE(A && x && y && n > 0 && n < INT32_MAX && x != y, "afx_dense_matvec: bad argument");
+  return by_precision(precision, [&](auto* tp) {
+    using T = std::remove_pointer_t<decltype(tp)>;
+    AFX_LAUNCH(kKAmg, k_dense_matvec<T>, grid_for(n * 64), dim3(kBlock), 0, s, (int32_t)n, (const T*)A, (const T*)x,
+               (T*)y);
     AFX_CHECK_LAUNCH();
     return 0;
   });

@@ -420,6 +420,11 @@ int afx_aggregate_restrict(int64_t n_coarse, int32_t max_members, int precision,
 /* x[i] += alpha xc[agg[i]]. */
 int afx_aggregate_prolong(int64_t n, int precision, const int32_t* agg, const void* xc, double alpha, void* x,
                           hipStream_t stream);
+/* out[0] = a . b (FP64 accumulation): 256 per-block partial sums into partial[256], then their sum;
+ * no atomics and no cleared memory, so it may be recorded in a HIP graph. */
+int afx_dot(int64_t n, int precision, const void* a, const void* b, double* partial, void* out, hipStream_t stream);
+/* y = A x for a dense row-major n x n matrix (the coarsest level's pseudo-inverse); x and y differ. */
+int afx_dense_matvec(int64_t n, int precision, const void* A, const void* x, void* y, hipStream_t stream);
 
 #ifdef __cplusplus
 }
