@@ -150,3 +150,18 @@ def test_captured_schur_solve_replays_match_eager(precision, cuda):
         err = float((got - want).abs().max() / want.abs().max())
         print("replay", k, "relative difference", err, flush=True)
         assert err <= tol
+
+
+def test_jacobi_schur_solve_reaches_the_root(cuda):
+    """The Schur solved by Jacobi-preconditioned CG (no multigrid) still preconditions Newton to the
+    inner-solve root."""
+    print("newton channel2d simple, jacobi schur", flush=True)
+    mesh, geom, mc = FC.build("channel2d", "upwind", "gg", device=cuda)
+    x0 = torch.tensor(FC.seeded_state(mesh, seed=0, kind="channel"), dtype=F64, device=cuda)
+    ref, _ = ImplicitNewtonSolver(forward_step=DampedNewtonStep(preconditioner="inner", precision="fp64")).forward(mc, x0)
+    bp = BlockPreconditioner.build(mc, composition="simple", schur_solver="jacobi")
+    assert bp.hierarchy is None and bp.schur_iters == 30
+    phi, rep = ImplicitNewtonSolver(forward_step=DampedNewtonStep(preconditioner="block", precision="fp64",
+                                                                  block_preconditioner=bp)).forward(mc, x0)
+    assert rep.converged
+    assert float((phi - ref).abs().max()) / float(ref.abs().max()) <= 1e-8
