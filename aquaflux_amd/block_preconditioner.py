@@ -82,21 +82,22 @@ def conjugate_gradient(matvec, b, preconditioner, iterations: int, dot=torch.dot
     ``preconditioner`` is a symmetric positive map or, as a tensor, a Jacobi inverse diagonal;
     ``dot`` returns a 0-d device tensor."""
     precond = preconditioner if callable(preconditioner) else (lambda v: v * preconditioner)
-    x = torch.zeros_like(b)
+    # no zero-filled allocations (torch fills them with a memory clear, which a graph recording must
+    # not contain): the zero iterate is b * 0 and the guards take Python scalars
+    x = b * 0.0
     r = b.clone()
     z = precond(r)
     p = z.clone()
     rz = dot(r, z)
-    zero = torch.zeros((), dtype=b.dtype, device=b.device)
     for _ in range(int(iterations)):
         Ap = matvec(p)
         pAp = dot(p, Ap)
-        alpha = torch.where(pAp > 0, rz / torch.where(pAp > 0, pAp, torch.ones_like(pAp)), zero)
+        alpha = torch.where(pAp > 0, rz / torch.where(pAp > 0, pAp, 1.0), 0.0)
         x.add_(alpha * p)
         r.sub_(alpha * Ap)
         z = precond(r)
         rz_new = dot(r, z)
-        beta = torch.where(rz > 0, rz_new / torch.where(rz > 0, rz, torch.ones_like(rz)), zero)
+        beta = torch.where(rz > 0, rz_new / torch.where(rz > 0, rz, 1.0), 0.0)
         p = z + beta * p
         rz = rz_new
     return x
