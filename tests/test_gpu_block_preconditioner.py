@@ -165,3 +165,19 @@ def test_jacobi_schur_solve_reaches_the_root(cuda):
                                                                   block_preconditioner=bp)).forward(mc, x0)
     assert rep.converged
     assert float((phi - ref).abs().max()) / float(ref.abs().max()) <= 1e-8
+
+
+def test_re1000_cavity_from_rest(cuda):
+    """The 64 x 64 lid-driven cavity at Re = 1000 from rest (pseudo-transient continuation from CFL
+    1): block-SIMPLER reaches the root the inner point-block solve reaches."""
+    print("newton cavity64 Re=1000 from rest, block-simpler", flush=True)
+    mesh, geom, mc = FC.build("cavity64", "upwind", "gg", mu=1e-3, device=cuda)
+    x0 = mc.initial_state()
+    ref, _ = ImplicitNewtonSolver(max_steps=60, forward_step=DampedNewtonStep(
+        preconditioner="inner", precision="fp32", ptc_cfl0=1.0, restart=60)).forward(mc, x0)
+    bp = BlockPreconditioner.build(mc, composition="simpler")
+    phi, rep = ImplicitNewtonSolver(max_steps=60, forward_step=DampedNewtonStep(
+        preconditioner="block", precision="fp32", ptc_cfl0=1.0, restart=60, block_preconditioner=bp)).forward(mc, x0)
+    print("steps", rep.steps, "krylov", sum(rep.krylov_iterations), flush=True)
+    assert rep.converged
+    assert float((phi - ref).abs().max()) / float(ref.abs().max()) <= 1e-8
