@@ -187,6 +187,7 @@ class AggregationHierarchy:
             h[j] = 2.0 * (float(np.median(along)) if along.size else iso)
         T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
         levels = []
+        tries = 0
         while n > coarsest and len(levels) < max_levels:
             key = np.floor((x - lo) / h + 0.25).astype(np.int64)  # off the bin edges of a regular grid
             span = key.max(axis=0) + 2
@@ -197,9 +198,14 @@ class AggregationHierarchy:
                 lin[pin] = -1  # the pinned cell stays a singleton (it holds an identity row)
             _, agg = np.unique(lin, return_inverse=True)
             nc = int(agg.max()) + 1
-            if nc > 0.85 * n:  # bins too fine for this level's spacing: widen and retry
-                h *= 2.0
+            target = n / 2.0**dim
+            if nc > 2.0 * target and tries < 4:
+                # bins too fine for this level (cells clustered closer than their edge spacing, as
+                # the tetrahedra of one hexahedron are): widen towards 2^d cells a bin and retry
+                h = h * (nc / target) ** (1.0 / dim)
+                tries += 1
                 continue
+            tries = 0
             nslot = cols.shape[0]
             I = np.broadcast_to(agg[None, :], cols.shape)
             J = agg[cols]

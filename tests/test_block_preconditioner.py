@@ -282,3 +282,35 @@ def test_member_tables_list_each_aggregate():
         for a in range(mem.shape[1]):
             got = sorted(int(m) for m in mem[:, a] if m >= 0)
             assert got == sorted(np.flatnonzero(agg == a).tolist())
+
+
+def test_multigrid_on_jittered_tetrahedra():
+    """Geometric aggregation on an unstructured mesh (10 x 8 x 8 Kuhn tetrahedra, nodes jittered):
+    the hierarchy coarsens by at least 4 per level, its coarse operators stay Galerkin products, and
+    10 multigrid-preconditioned CG steps beat 10 Jacobi-preconditioned ones by 100x in residual."""
+    from aquaflux_amd.mesh import tet_grid_3d
+
+    mesh = tet_grid_3d(10, 8, 8, perturb=0.15, seed=3)
+    op, m, g, outlet = _fake_op(mesh, outlet=True)
+    graph = BP.SchurGraph.of(op)
+    C = mesh.n_cells
+    aP = 1.0 + torch.rand(C, generator=torch.Generator().manual_seed(2), dtype=F64)
+    vals = graph.laplacian(graph.volume / aP, torch.ones(C, dtype=F64))
+    H = BP.AggregationHierarchy.build(graph.cols.numpy(), g.cell_centroid.numpy(), None, coarsest=200)
+    sizes = [lv.n for lv in H.levels]
+    assert len(sizes) >= 3 and all(a >= 4 * b for a, b in zip(sizes, sizes[1:])), sizes
+    ops, cinv = H.operators(vals)
+    A = _dense(graph, vals)
+    lv, nx_ = H.levels[0], H.levels[1]
+    P = torch.zeros(lv.n, nx_.n, dtype=F64)
+    P[torch.arange(lv.n), lv.agg] = 1.0
+    Ac = torch.zeros(nx_.n, nx_.n, dtype=F64)
+    Ac.index_put_((torch.arange(nx_.n).repeat(nx_.nslot), nx_.cols.long().reshape(-1)), ops[1].reshape(-1),
+                  accumulate=True)
+    assert torch.allclose(Ac, P.T @ A @ P, atol=1e-12)
+    b = torch.randn(C, generator=torch.Generator().manual_seed(5), dtype=F64)
+    spmv = _torch_spmv(H)
+    x = BP.conjugate_gradient(lambda v: A @ v, b, lambda r: H.cycle(spmv, ops, cinv, r), 10)
+    xj = BP.conjugate_gradient(lambda v: A @ v, b, 1.0 / torch.diagonal(A), 10)
+    res = lambda y: float(torch.linalg.vector_norm(A @ y - b) / torch.linalg.vector_norm(b))  # noqa: E731
+    assert res(x) < 1e-2 * res(xj), (res(x), res(xj))
