@@ -45,10 +45,13 @@ DGKS_ETA = 1.0 / math.sqrt(2.0)
 
 
 def gmres(matvec, b, *, x0=None, rtol=1e-10, atol=0.0, restart=30, max_iter=1000, precond=None, flexible=False,
-          reorthogonalize: str = "dgks"):
+          reorthogonalize: str = "dgks", basis_dtype=None):
     """Solve ``A x = b``; returns ``(x, KrylovInfo)``.  ``precond`` is applied on the right.
     ``reorthogonalize``: ``"dgks"`` (a second Gram-Schmidt pass when the first lost more than
-    1 - 1/sqrt(2) of the vector's norm) or ``"always"`` (two passes every step)."""
+    1 - 1/sqrt(2) of the vector's norm) or ``"always"`` (two passes every step).  ``basis_dtype``
+    stores the Krylov (and flexible) basis and runs Gram-Schmidt in another precision (FP32 halves
+    the orthogonalisation traffic of a solve that needs only a few digits); residuals, products
+    and the update stay in ``b``'s precision."""
     if reorthogonalize not in ("dgks", "always"):
         raise ValueError(f"unknown reorthogonalize {reorthogonalize!r}")
     dtype, dev = b.dtype, b.device
@@ -61,8 +64,9 @@ def gmres(matvec, b, *, x0=None, rtol=1e-10, atol=0.0, restart=30, max_iter=1000
         info.residual_norm, info.converged = 0.0, True
         return x, info
     m = int(restart)
-    V = torch.empty((m + 1, n), dtype=dtype, device=dev)
-    Z = torch.empty((m, n), dtype=dtype, device=dev) if (flexible and precond is not None) else None
+    bdt = dtype if basis_dtype is None else basis_dtype
+    V = torch.empty((m + 1, n), dtype=bdt, device=dev)
+    Z = torch.empty((m, n), dtype=bdt, device=dev) if (flexible and precond is not None) else None
     while True:
         r = b - matvec(x) if (x0 is not None or info.restarts > 0) else b.clone()
         beta = float(torch.linalg.vector_norm(r))
@@ -72,7 +76,7 @@ def gmres(matvec, b, *, x0=None, rtol=1e-10, atol=0.0, restart=30, max_iter=1000
             return x, info
         if info.iterations >= max_iter:
             return x, info
-        V[0] = r / beta
+        V[0] = (r / beta).to(bdt)
         H = np.zeros((m + 1, m))
         cs = np.zeros(m)
         sn = np.zeros(m)
@@ -83,7 +87,8 @@ def gmres(matvec, b, *, x0=None, rtol=1e-10, atol=0.0, restart=30, max_iter=1000
             z = V[j] if precond is None else precond(V[j])
             if Z is not None:
                 Z[j] = z
-            w = matvec(z)
+                z = Z[j]  # the stored (possibly rounded) vector, so that A Z = V H holds as stored
+            w = matvec(z.to(dtype)).to(bdt)
             Vj = V[: j + 1]
             w0 = torch.linalg.vector_norm(w)
             h = Vj @ w
@@ -115,12 +120,12 @@ def gmres(matvec, b, *, x0=None, rtol=1e-10, atol=0.0, restart=30, max_iter=1000
             if abs(g[j + 1]) <= tol or info.iterations >= max_iter or hv[j + 1] == 0.0:
                 break
         y = np.linalg.solve(np.triu(H[:k, :k]) + np.diag(np.where(np.diag(H[:k, :k]) == 0, 1.0, 0.0)), g[:k])
-        yt = torch.tensor(y, dtype=dtype, device=dev)
+        yt = torch.tensor(y, dtype=bdt, device=dev)
         if Z is not None:
-            x = x + Z[:k].T @ yt
+            x = x + (Z[:k].T @ yt).to(dtype)
         else:
             upd = V[:k].T @ yt
-            x = x + (upd if precond is None else precond(upd))
+            x = x + (upd if precond is None else precond(upd)).to(dtype)
         info.restarts += 1
         info.residual_norm = abs(g[k])
 
@@ -261,6 +266,7 @@ class DampedNewtonStep:
     jacobian: str = "compact"
     column_reach: tuple | None = None
     block_preconditioner: object = None
+    outer_basis: str = "fp64"  # precision of the outer Krylov basis ("fp64" or "fp32")
     _plans: dict = dataclasses.field(default_factory=dict, repr=False)
 
     def block_jacobian(self, op, phi):
@@ -330,8 +336,11 @@ class DampedNewtonStep:
             jv = op.residual_jvp(phi, v)[1]
             return jv if shift is None else jv + shift * v
 
+        if self.outer_basis not in ("fp64", "fp32"):
+            raise ValueError(f"outer_basis must be 'fp64' or 'fp32', got {self.outer_basis!r}")
         delta, info = gmres(matvec, -r, rtol=self.rtol, atol=self.atol, restart=self.restart, max_iter=self.max_iter,
-                            precond=precond, flexible=flexible)
+                            precond=precond, flexible=flexible,
+                            basis_dtype=torch.float32 if self.outer_basis == "fp32" else None)
         return delta, info
 
 

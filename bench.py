@@ -394,12 +394,13 @@ def tet_cpu_baseline(n=32, reps=5):
 
 
 def newton_compare(device, nx=200, ny=100, nz=100, mu=1e-3, max_steps=80, cfl0=10.0, preconditioner="block-simpler",
-                   block_options=None, restart=60):
+                   block_options=None, restart=60, outer_basis="fp32"):
     """Newton to ||R|| <= 1e-12 + 1e-10 ||R0|| on the hex box (u, v, w, p) at Re = 1/mu, from the
     seeded state, with the compact block Jacobian in FP32 and in FP64 under ``preconditioner``:
     "inner" (GMRES with point-block Jacobi) or "block-<composition>" (the SIMPLE-family block
     preconditioner with that composition, further ``BlockPreconditioner.build`` arguments in
-    ``block_options``); the outer FGMRES restarts every ``restart`` iterations."""
+    ``block_options``); the outer FGMRES restarts every ``restart`` iterations and keeps its basis
+    in ``outer_basis`` precision."""
     import torch
 
     from aquaflux_amd.boundary import BoundaryConditions
@@ -430,9 +431,11 @@ def newton_compare(device, nx=200, ny=100, nz=100, mu=1e-3, max_steps=80, cfl0=1
         # a 60-vector outer FGMRES basis: the block preconditioner's steps take up to ~100 outer
         # iterations, and restarting at 30 cost 7% on the 2M-cell box
         if preconditioner == "inner":
-            return DampedNewtonStep(preconditioner="inner", precision=prec, restart=restart, **kw)
+            return DampedNewtonStep(preconditioner="inner", precision=prec, restart=restart, outer_basis=outer_basis,
+                                    **kw)
         bp = BlockPreconditioner.build(mc, composition=preconditioner.removeprefix("block-"), **(block_options or {}))
-        return DampedNewtonStep(preconditioner="block", precision=prec, block_preconditioner=bp, restart=restart, **kw)
+        return DampedNewtonStep(preconditioner="block", precision=prec, block_preconditioner=bp, restart=restart,
+                                outer_basis=outer_basis, **kw)
 
     for prec in ("fp32", "fp64"):  # warm up every kernel and allocation outside the timed solves
         make_step(prec, max_iter=2, inner_iters=2).correction(mc, x0, mc.residual_raw(x0))
@@ -441,7 +444,8 @@ def newton_compare(device, nx=200, ny=100, nz=100, mu=1e-3, max_steps=80, cfl0=1
     out = {"workload": f"hex box {nx}x{ny}x{nz} ({C} cells, {4 * C} unknowns), Re={1.0 / mu:g}, first-order upwind, "
                        "least-squares gradients, seeded state; pseudo-transient continuation (SER, CFL0="
                        f"{cfl0:g}) into Newton; stop at ||R|| <= 1e-12 + 1e-10 ||R0||",
-           "preconditioner": preconditioner, "block_options": block_options or {}, "outer_restart": restart}
+           "preconditioner": preconditioner, "block_options": block_options or {}, "outer_restart": restart,
+           "outer_basis": outer_basis}
     roots = {}
     for prec in ("fp32", "fp64"):
         def progress(step, rel, kits, alpha, secs, prec=prec):
@@ -504,6 +508,8 @@ def main(argv=None):
     ap.add_argument("--newton-block-options", type=json.loads, default=None, metavar="JSON",
                     help='further BlockPreconditioner.build arguments, e.g. \'{"schur_iters": 4}\'')
     ap.add_argument("--newton-restart", type=int, default=60, help="outer FGMRES restart of the Newton solves")
+    ap.add_argument("--newton-outer-basis", default="fp32", choices=["fp64", "fp32"],
+                    help="precision of the outer FGMRES basis (residuals, JVPs and updates stay FP64)")
     ap.add_argument(
         "--calibrate-copy",
         type=int,
@@ -531,7 +537,8 @@ def main(argv=None):
     if args.newton_only:
         print(json.dumps(newton_compare(dev, *args.newton_size, mu=args.newton_mu, cfl0=args.newton_cfl0,
                                         preconditioner=args.newton_preconditioner,
-                                        block_options=args.newton_block_options, restart=args.newton_restart)),
+                                        block_options=args.newton_block_options, restart=args.newton_restart,
+                                        outer_basis=args.newton_outer_basis)),
               flush=True)
         return 0
     wl = args.workload
@@ -644,7 +651,8 @@ def main(argv=None):
     if not args.no_newton:
         out["newton"] = newton_compare(dev, *args.newton_size, mu=args.newton_mu, cfl0=args.newton_cfl0,
                                        preconditioner=args.newton_preconditioner,
-                                       block_options=args.newton_block_options, restart=args.newton_restart)
+                                       block_options=args.newton_block_options, restart=args.newton_restart,
+                                       outer_basis=args.newton_outer_basis)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, *args.cpu_size, reps=args.cpu_reps)
     print(json.dumps(out), flush=True)

@@ -314,3 +314,22 @@ def test_multigrid_on_jittered_tetrahedra():
     xj = BP.conjugate_gradient(lambda v: A @ v, b, 1.0 / torch.diagonal(A), 10)
     res = lambda y: float(torch.linalg.vector_norm(A @ y - b) / torch.linalg.vector_norm(b))  # noqa: E731
     assert res(x) < 1e-2 * res(xj), (res(x), res(xj))
+
+
+def test_gmres_with_an_fp32_basis():
+    """An FP32 Krylov basis (products, residuals and the update in FP64) still reaches a few-digit
+    tolerance, in about the iterations of the FP64 basis, flexible or not."""
+    from aquaflux_amd.solve import gmres
+
+    g = torch.Generator().manual_seed(11)
+    n = 300
+    A = torch.eye(n, dtype=F64) * 4.0 + torch.randn(n, n, generator=g, dtype=F64) / n**0.5
+    b = torch.randn(n, generator=g, dtype=F64)
+    dinv = 1.0 / torch.diagonal(A)
+    for flexible in (False, True):
+        x64, i64 = gmres(lambda v: A @ v, b, rtol=1e-5, restart=20, max_iter=200, precond=lambda v: v * dinv,
+                         flexible=flexible)
+        x32, i32 = gmres(lambda v: A @ v, b, rtol=1e-5, restart=20, max_iter=200, precond=lambda v: v * dinv,
+                         flexible=flexible, basis_dtype=torch.float32)
+        assert x32.dtype == F64 and i32.converged and abs(i32.iterations - i64.iterations) <= 2
+        assert float(torch.linalg.vector_norm(A @ x32 - b)) <= 1.01e-5 * float(torch.linalg.vector_norm(b))
