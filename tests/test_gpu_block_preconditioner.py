@@ -181,3 +181,24 @@ def test_re1000_cavity_from_rest(cuda):
     print("steps", rep.steps, "krylov", sum(rep.krylov_iterations), flush=True)
     assert rep.converged
     assert float((phi - ref).abs().max()) / float(ref.abs().max()) <= 1e-8
+
+
+@pytest.mark.parametrize("preconditioner", ["inner", "block"])
+def test_fp32_outer_basis_reaches_the_same_root(preconditioner, cuda):
+    """Newton whose outer FGMRES keeps its basis in FP32 (residuals, JVPs and updates in FP64)
+    reaches the root of the FP64-basis solve, in the same number of Newton steps."""
+    print("newton channel3d_mg outer basis fp32,", preconditioner, flush=True)
+    mesh, geom, mc = FC.build("channel3d_mg", "upwind", "gg", device=cuda)
+    x0 = torch.tensor(FC.seeded_state(mesh, seed=0, kind="channel"), dtype=F64, device=cuda)
+    roots = {}
+    for basis in ("fp64", "fp32"):
+        kw = {"block_preconditioner": BlockPreconditioner.build(mc, composition="simpler")} if preconditioner == "block" else {}
+        step = DampedNewtonStep(preconditioner=preconditioner, precision="fp32", outer_basis=basis, **kw)
+        phi, rep = ImplicitNewtonSolver(forward_step=step).forward(mc, x0)
+        assert rep.converged
+        roots[basis] = (phi, rep.steps)
+        print(basis, rep.steps, rep.krylov_iterations, flush=True)
+    assert roots["fp32"][1] == roots["fp64"][1]
+    assert float((roots["fp32"][0] - roots["fp64"][0]).abs().max()) / float(roots["fp64"][0].abs().max()) <= 1e-9
+    with pytest.raises(ValueError, match="outer_basis"):
+        DampedNewtonStep(outer_basis="bf16").correction(mc, x0, mc.residual_raw(x0))
