@@ -497,6 +497,66 @@ def advective_momentum_flux(m, g, vel, rho, fac, grad_vel):
     return dot(mf, g.normal)
 
 
+def momentum_diagonal_coefficient(kind, viscous, convective):
+    """``FlowBoundary.momentum_diagonal_coefficient`` per patch (``flow/boundary.py:216-238, 302-308,
+    349-355, 443-446``): a wall adds the viscous ``mu A / (d.n)`` only, a pressure outlet the
+    convective ``max(mdot, 0)`` only, a velocity inlet both; other faces nothing."""
+    out = torch.where(kind == FLOW_WALL, viscous, torch.zeros_like(viscous))
+    out = torch.where(kind == FLOW_INLET, viscous + convective, out)
+    return torch.where(kind == FLOW_OUTLET, convective, out)
+
+
+def momentum_diagonal(m, g, setup: FlowSetup, mu_eff, boundary_mu, mdot_est):
+    """``momentum_matrix_diagonal`` (``flow/momentum.py:596-635``) with ``momentum_diagonal``
+    (``flow/rhie_chow.py:81-168``): per cell, the interior conductances plus ``max(+-mdot_est, 0)``
+    and each boundary face's per-patch term."""
+    dist = normal_distance(m, g)
+    visc_b = boundary_mu * g.area / dist
+    conv_b = maximum(mdot_est, 0.0) if mdot_est is not None else torch.zeros(m.n_faces, dtype=F64)
+    bcoef = momentum_diagonal_coefficient(setup.kind, visc_b, conv_b)
+    visc = conductance(m, g, mu_eff)
+    oc, nc = visc, visc
+    if mdot_est is not None:
+        oc = oc + maximum(mdot_est, 0.0)
+        nc = nc + maximum(-mdot_est, 0.0)
+    oc = torch.where(m.interior, oc, bcoef)
+    return scatter(m, oc, nc)
+
+
+def interior_mass_flux(m, g, vel, grad_vel, p, grad_p, d_coeff, rho, fac, dist):
+    """``interior_mass_flux`` (``flow/rhie_chow.py:274-353``): ``[(rho u)_f . n - rho_f d_hat ((p_N -
+    p_P) - grad p_f . d) / (d.n)] A`` with ``d_hat = sum_i n_i^2 (V/a_P)_{f,i}``."""
+    own = m.owner
+    mom_n = advective_momentum_flux(m, g, vel, rho, fac, grad_vel)
+    rho_f = blend(m, rho, rho, fac)
+    gp_f = blend(m, grad_p, grad_p, fac)
+    dc_f = blend(m, d_coeff, d_coeff, fac)
+    d_hat = dot(g.normal * g.normal, dc_f)
+    dvec = g.cell_centroid[m.safe_nb] - g.cell_centroid[own]
+    dp = p[m.safe_nb] - p[own]
+    damping = (dp - dot(gp_f, dvec)) / dist
+    return (mom_n - rho_f * d_hat * damping) * g.area
+
+
+def inlet_mass_flux(rho_owner, u_b, normal, area):
+    """``VelocityInlet.mass_flux`` (``flow/boundary.py:381-397``): ``rho (u_in . n) A``."""
+    return rho_owner * dot(u_b, normal) * area
+
+
+def outlet_mass_flux(vel_owner, p_owner, grad_p_owner, d_coeff_owner, normal, area, dist, rho_owner, p_b):
+    """``PressureOutlet.mass_flux`` (``flow/boundary.py:418-434``): ``rho (u_P . n - d_hat ((p_b -
+    p_P) / (d.n) - grad p_P . n)) A``."""
+    compact = (p_b - p_owner) / dist
+    interp = dot(grad_p_owner, normal)
+    dh_own = dot(normal * normal, d_coeff_owner)
+    return rho_owner * (dot(vel_owner, normal) - dh_own * (compact - interp)) * area
+
+
+def pressure_force_flux(p_face, normal, area, component):
+    """``PressureForce.face_flux`` (``flow/momentum.py:112-155``): ``p_f n_i A``."""
+    return p_face * normal[:, component] * area
+
+
 def flow_fields(m, g, setup: FlowSetup, state, mu, rho, nu_t=None, wall_nu_t=None, shear_mask=None, frozen=None):
     """``MomentumContinuity.flow_fields`` (``flow/momentum.py:822-857``) -- returns a dict.
 
@@ -526,40 +586,18 @@ def flow_fields(m, g, setup: FlowSetup, state, mu, rho, nu_t=None, wall_nu_t=Non
     mdot_est = None
     if setup.advection != "none":
         mdot_est = advective_momentum_flux(m, g, vel, rho, fac, grad_vel) * g.area
-    k = setup.kind
-    visc_b = bvisc * g.area / dist
-    conv_b = maximum(mdot_est, 0.0) if mdot_est is not None else torch.zeros(m.n_faces, dtype=F64)
-    bcoef = torch.where(k == FLOW_WALL, visc_b, torch.zeros_like(visc_b))
-    bcoef = torch.where(k == FLOW_INLET, visc_b + conv_b, bcoef)
-    bcoef = torch.where(k == FLOW_OUTLET, conv_b, bcoef)
-    visc = conductance(m, g, mu_eff)
-    oc, nc = visc, visc
-    if mdot_est is not None:
-        oc = oc + maximum(mdot_est, 0.0)
-        nc = nc + maximum(-mdot_est, 0.0)
-    oc = torch.where(m.interior, oc, bcoef)
-    a_p = scatter(m, oc, nc)
+    a_p = momentum_diagonal(m, g, setup, mu_eff, bvisc, mdot_est)
     if frozen is not None:
         a_p = frozen["a_p"].detach()
     d_coeff = (g.volume / a_p)[:, None].expand(C, d)
     # _mass_flux (momentum.py:531-554) + interior_mass_flux (rhie_chow.py:274-353)
-    mom_n = advective_momentum_flux(m, g, vel, rho, fac, grad_vel)
-    rho_f = blend(m, rho, rho, fac)
-    gp_f = blend(m, grad_p, grad_p, fac)
-    dc_f = blend(m, d_coeff, d_coeff, fac)
-    d_hat = dot(g.normal * g.normal, dc_f)
-    dvec = g.cell_centroid[m.safe_nb] - g.cell_centroid[own]
-    dp = p[m.safe_nb] - p[own]
-    damping = (dp - dot(gp_f, dvec)) / dist
-    mdot_int = (mom_n - rho_f * d_hat * damping) * g.area
+    mdot_int = interior_mass_flux(m, g, vel, grad_vel, p, grad_p, d_coeff, rho, fac, dist)
     mdot = torch.where(m.interior, mdot_int, torch.zeros_like(mdot_int))
     # boundary mass-flux closures (flow/boundary.py:278-446)
+    k = setup.kind
     n = g.normal
-    mdot_in = rho[own] * dot(ub, n) * g.area
-    compact = (setup.p_b - p[own]) / dist
-    interp = dot(grad_p[own], n)
-    dh_own = dot(n * n, d_coeff[own])
-    mdot_out = rho[own] * (dot(vel[own], n) - dh_own * (compact - interp)) * g.area
+    mdot_in = inlet_mass_flux(rho[own], ub, n, g.area)
+    mdot_out = outlet_mass_flux(vel[own], p[own], grad_p[own], d_coeff[own], n, g.area, dist, rho[own], setup.p_b)
     mdot = torch.where(k == FLOW_WALL, torch.zeros_like(mdot), mdot)
     mdot = torch.where(k == FLOW_INLET, mdot_in, mdot)
     mdot = torch.where(k == FLOW_OUTLET, mdot_out, mdot)
@@ -584,7 +622,7 @@ def flow_residual_from_fields(m, g, setup: FlowSetup, f, psi_vel=None):
         bv = f["ub"][:, i]
         flux = torch.zeros(m.n_faces, dtype=F64)
         flux = flux + diffusion_flux(m, g, phi, grad, f["mu_eff"], bv, f["boundary_gamma"])
-        flux = flux + p_face * g.normal[:, i] * g.area
+        flux = flux + pressure_force_flux(p_face, g.normal, g.area, i)
         if setup.advection == "upwind":
             flux = flux + mdot * upwind_face_value(m, phi, mdot, bv)
         elif setup.advection == "limited":
@@ -633,25 +671,45 @@ class ScalarSetup:
     gradient_relaxation: float = 1.0
 
 
+def tangential_correction(grad_owner, d, normal):
+    """``boundary/conditions.py`` ``_tangential_correction``: ``grad_P . (d - (d.n) n)``."""
+    return dot(grad_owner, d - scale(normal, dot(d, normal)))
+
+
+def zero_gradient_face_value(phi_owner, grad_owner, d, normal):
+    """``ZeroGradient.face_value`` (``boundary/conditions.py:145-153``): ``phi_P + corr``."""
+    return phi_owner + tangential_correction(grad_owner, d, normal)
+
+
+def neumann_face_value(phi_owner, grad_owner, d, normal, gamma_owner, q):
+    """``Neumann.face_value`` (``boundary/conditions.py:156-173``): ``phi_P + corr - (q / Gamma_P)(d.n)``."""
+    return phi_owner + tangential_correction(grad_owner, d, normal) - q / gamma_owner * dot(d, normal)
+
+
+def convective_face_value(phi_owner, grad_owner, d, normal, gamma_owner, h, t_inf):
+    """``Convective.face_value`` (``boundary/conditions.py:176-201``):
+    ``(phi_P + corr + beta T_inf) / (1 + beta)`` with ``beta = (h / Gamma_P)(d.n)``."""
+    beta = h / gamma_owner * dot(d, normal)
+    return (phi_owner + tangential_correction(grad_owner, d, normal) + beta * t_inf) / (1.0 + beta)
+
+
 def scalar_boundary_values(m, g, s: ScalarSetup, phi, grad, gamma):
-    """``ResidualAssembler.boundary_values`` + ``boundary/conditions.py`` closures."""
+    """``ResidualAssembler.boundary_values`` (``discretization/residual.py:298-356``): each boundary
+    face's closure (``boundary/conditions.py:111-201``) evaluated from its owner."""
     own = m.owner
     dvec = g.face_centroid - g.cell_centroid[own]
     n = g.normal
-    tang = dvec - scale(n, dot(dvec, n))
-    corr = dot(grad[own], tang)
-    dn = dot(dvec, n)
     k = s.kind
+    phi_o, grad_o = phi[own], grad[own]
     out = torch.zeros(m.n_faces, dtype=F64)
     out = torch.where(k == BC_DIRICHLET, s.value, out)
-    out = torch.where(k == BC_ZERO_GRADIENT, phi[own] + corr, out)
+    out = torch.where(k == BC_ZERO_GRADIENT, zero_gradient_face_value(phi_o, grad_o, dvec, n), out)
     # the Neumann / Convective branches divide by the owner diffusivity; faces of other kinds read a
     # unit stand-in there so that no unselected 0/0 reaches the reverse pass of torch.where
     robin = (k == BC_NEUMANN) | (k == BC_CONVECTIVE)
     gam = torch.where(robin, gamma[own], torch.ones_like(gamma[own]))
-    out = torch.where(k == BC_NEUMANN, phi[own] + corr - s.value / gam * dn, out)
-    beta = s.value / gam * dn
-    out = torch.where(k == BC_CONVECTIVE, (phi[own] + corr + beta * s.t_inf) / (1.0 + beta), out)
+    out = torch.where(k == BC_NEUMANN, neumann_face_value(phi_o, grad_o, dvec, n, gam, s.value), out)
+    out = torch.where(k == BC_CONVECTIVE, convective_face_value(phi_o, grad_o, dvec, n, gam, s.value, s.t_inf), out)
     return out
 
 

@@ -150,6 +150,74 @@ def omega_wall_gradient(nu, d, k, dgrad, kgrad, model):
     return fv.scale(dgrad, dod) + fv.scale(kgrad, dok)
 
 
+# --- volume sources (turbulence/sources.py) -----------------------------------------------------------
+# Each returns the source per unit volume; the balance multiplies by the cell volume.
+
+
+def production_and_limit(nu_t, strain, omega, k, model):
+    """``sources.py:165-202``: ``(nu_t S^2, 10 beta* max(k, 0) omega)``."""
+    production = nu_t * strain**2
+    limit = PRODUCTION_LIMIT_RATIO * model.beta_star * fv.maximum(k, 0.0) * omega
+    return production, limit
+
+
+def near_wall_production(resolved, k, cells, nu, d, shear_rate, model):
+    """``NearWallKClosure.production`` (``sources.py:109-139``): the wall cells' resolved production
+    blended toward ``k_wall_production`` by ``wall_function_weight`` at the live ``k``."""
+    kw = k[cells]
+    w = wall_function_weight(nu, d, kw, model)
+    wall_value = k_wall_production(nu, d, kw, shear_rate, model)
+    out = resolved.clone()
+    out[cells] = (1.0 - w) * resolved[cells] + w * wall_value
+    return out
+
+
+def near_wall_dissipation_rate(omega, k, cells, nu, d, model):
+    """``NearWallKClosure.dissipation_rate`` (``sources.py:141-162``): ``omega_wall(k)`` on the wall
+    cells, ``omega`` elsewhere."""
+    out = omega.clone()
+    out[cells] = omega_wall(nu, d, k[cells], model)
+    return out
+
+
+def k_production(nu_t, strain, omega, k, model, near_wall=None):
+    """``KProduction.source / V`` (``sources.py:205-262``): ``min(nu_t S^2, 10 beta* max(k,0) omega)``,
+    then the near-wall blend; ``near_wall`` = (cells, nu, d, shear_rate) or None."""
+    production, limit = production_and_limit(nu_t, strain, omega, k, model)
+    limited = fv.minimum(production, limit)
+    if near_wall is not None:
+        limited = near_wall_production(limited, k, *near_wall, model)
+    return limited
+
+
+def k_destruction(k, omega, model, near_wall=None):
+    """``KDestruction.source / V`` (``sources.py:265-293``): ``-beta* k omega`` with the wall cells'
+    ``omega`` read live as ``omega_wall(k)``; ``near_wall`` = (cells, nu, d) or None."""
+    if near_wall is not None:
+        omega = near_wall_dissipation_rate(omega, k, *near_wall, model)
+    return -model.beta_star * k * omega
+
+
+def omega_production(strain, nu_t, k, omega, f1, model):
+    """``OmegaProduction.source / V`` (``sources.py:296-355``): ``alpha(F1) min(S^2, 10 beta* max(k,0)
+    omega / max(nu_t, 1e-30))``."""
+    alpha = model.blend(f1, model.alpha_1, model.alpha_2)
+    cap = (PRODUCTION_LIMIT_RATIO * model.beta_star * fv.maximum(k, 0.0) * omega
+           / fv.maximum(nu_t, EDDY_VISCOSITY_FLOOR))
+    return alpha * fv.minimum(strain**2, cap)
+
+
+def omega_destruction(omega, f1, model):
+    """``OmegaDestruction.source / V`` (``sources.py:358-376``): ``-beta(F1) omega^2``."""
+    beta = model.blend(f1, model.beta_1, model.beta_2)
+    return -beta * omega**2
+
+
+def omega_cross_diffusion(omega, grad_k, grad_omega, f1, model):
+    """``OmegaCrossDiffusion.source / V`` (``sources.py:379-412``): ``2 (1 - F1) sigma_w2 (grad k . grad omega) / omega``."""
+    return 2.0 * (1.0 - f1) * model.sigma_omega2 * fv.dot(grad_k, grad_omega) / omega
+
+
 # --- SST turbulence (turbulence/transport.py) --------------------------------------------------------
 
 
@@ -291,28 +359,16 @@ def rans_residual(m, g, s: fv.FlowSetup, t: SSTSetup, state, mu, rho, scalar_adv
     wo = own[t.wall_faces]
     face_gamma[t.wall_faces] = (1.0 - wall_function_weight(t.nu[wo], dw[wo], k[wo], model)) * face_gamma[t.wall_faces]
     Rk = _scalar_balance(m, g, gspec, t.k_bc, k, Q, gk, face_gamma, scalar_advection, s.limiter_k)
-    prod = cl["nu_t"] * cl["strain"] ** 2
-    limit = PRODUCTION_LIMIT_RATIO * model.beta_star * fv.maximum(k, 0.0) * omega
-    limited = fv.minimum(prod, limit).clone()
     kw = k[wall]
-    lw = wall_function_weight(t.nu[wall], dw[wall], kw, model)
-    wv = k_wall_production(t.nu[wall], dw[wall], kw, cl["tau"], model)
-    limited[wall] = (1.0 - lw) * limited[wall] + lw * wv
-    Rk = Rk - limited * V
-    om = omega.clone()
-    om[wall] = omega_wall(t.nu[wall], dw[wall], kw, model)
-    Rk = Rk - (-model.beta_star * k * om * V)
+    near = (wall, t.nu[wall], dw[wall])
+    Rk = Rk - k_production(cl["nu_t"], cl["strain"], omega, k, model, near_wall=near + (cl["tau"],)) * V
+    Rk = Rk - k_destruction(k, omega, model, near_wall=near) * V
     # omega residual (transport.py:655-718)
     gw = t.nu + model.blend(cl["f1"], model.sigma_omega1, model.sigma_omega2) * cl["nu_t"]
     Rw = _scalar_balance(m, g, gspec, t.omega_bc, omega, Q, gw, None, scalar_advection, s.limiter_k)
-    alpha = model.blend(cl["f1"], model.alpha_1, model.alpha_2)
-    cap = (PRODUCTION_LIMIT_RATIO * model.beta_star * fv.maximum(k, 0.0) * omega
-           / fv.maximum(cl["nu_t"], EDDY_VISCOSITY_FLOOR))
-    Rw = Rw - alpha * fv.minimum(cl["strain"] ** 2, cap) * V
-    beta = model.blend(cl["f1"], model.beta_1, model.beta_2)
-    Rw = Rw - (-beta * omega**2 * V)
-    cross = 2.0 * (1.0 - cl["f1"]) * model.sigma_omega2 * fv.dot(cl["grad_k"], cl["grad_omega"]) / omega
-    Rw = Rw - cross * V
+    Rw = Rw - omega_production(cl["strain"], cl["nu_t"], k, omega, cl["f1"], model) * V
+    Rw = Rw - omega_destruction(omega, cl["f1"], model) * V
+    Rw = Rw - omega_cross_diffusion(omega, cl["grad_k"], cl["grad_omega"], cl["f1"], model) * V
     Rw = Rw.clone()
     target = omega_wall(t.nu[wall], dw[wall], kw, model)
     Rw[wall] = torch.log(omega[wall] / target) if omega_log else omega[wall] - target
