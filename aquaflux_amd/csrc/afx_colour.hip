@@ -105,7 +105,10 @@ struct BallSearch {
 // Saturation-degree greedy colouring of the distance-(2 reach) conflict graph.  A bucket per rank
 // (saturation * (max degree + 1) + degree) holds the uncoloured vertices at that rank as a doubly
 // linked list; the highest occupied bucket is popped each round (last-in first), so a round costs
-// one ball search around the coloured vertex.
+// one ball search around the coloured vertex.  Vertices enter their buckets in index order and
+// neighbours are re-bucketed in ascending index order, as the reference's _saturation_colouring
+// does (aquaflux/solve/sparse_jacobian.py:458-534), so the colouring is the reference's, vertex for
+// vertex.
 void saturation_colouring(const CellGraph& g, int reach, int32_t* colour, int32_t* n_colours) {
   const int64_t n = g.n;
   const int r2 = 2 * reach;
@@ -163,6 +166,9 @@ void saturation_colouring(const CellGraph& g, int reach, int32_t* colour, int32_
     const int cw = chosen / 64;
     const uint64_t bit = 1ull << (chosen % 64);
     bs.ball(g, (int32_t)v, r2, ball);
+    // neighbours move to their new buckets in ascending index order (the conflict row's order in
+    // the reference), which fixes the last-in-first-out tie order and so the colouring itself
+    std::sort(ball.begin() + 1, ball.end());
     for (size_t i = 1; i < ball.size(); ++i) {
       const int64_t u = ball[i];
       if (colour[u] >= 0) continue;

@@ -93,3 +93,81 @@ def test_pattern_depth_is_the_graph_distance():
     c = block_stencil_colouring(owner, nb, n, reach=3)
     dist = shortest_path(OC.adjacency(owner, nb, n), unweighted=True)
     assert np.array_equal(c.pattern_depth, dist[c.pattern_rows, c.pattern_cols].astype(np.int8))
+
+
+def test_bucket_queue_tie_order_hand_worked():
+    """The reference's bucket queue on the 4-cell chain at reach 1, worked by hand from
+    sparse_jacobian.py:458-534.  Conflict rows (distance <= 2, diagonal included): degrees 3, 4, 4, 3.
+    Pushing 0..3 at their buckets' heads leaves 2 on top of bucket 4, so 2 takes colour 0; 0, 1, 3
+    each gain one colour and are re-pushed in that order (bucket of 1 above the shared bucket [3, 0]);
+    1 takes colour 1; 0 and 3 gain again (re-pushed 0 then 3, so 3 heads the bucket) and both take
+    colour 2.  A smallest-index tie rule would colour 1 first instead."""
+    owner, nb = _path(4)
+    conf = OC.conflict(owner, nb, 4, 1)
+    assert OC.bucket_saturation_colouring(conf, 4).tolist() == [2, 1, 0, 2]
+    assert OC.saturation_colouring(conf, 4).tolist() != [2, 1, 0, 2]
+
+
+@pytest.mark.parametrize("graph, reach", [("complete4", 1), ("cycle6", 1), ("star5", 1), ("lattice", 2)])
+def test_bucket_queue_is_valid_and_matches_known_colour_counts(graph, reach):
+    """Collision-free, and on graphs whose conflict graphs have known chromatic numbers the bucket queue
+    and the full-scan greedy use the same count (test_sparse_jacobian.py:185-220)."""
+    if graph == "complete4":
+        owner, nb = np.array([0, 0, 0, 1, 1, 2]), np.array([1, 2, 3, 2, 3, 3])
+        n, want = 4, 4
+    elif graph == "cycle6":  # distance <= 2 on a 6-cycle: every vertex conflicts with 4 others, chi = 3
+        owner, nb = np.arange(6), (np.arange(6) + 1) % 6
+        n, want = 6, 3
+    elif graph == "star5":  # reach 1 on a star: all leaves within 2 of each other, chi = 5
+        owner, nb = np.zeros(4, dtype=int), np.arange(1, 5)
+        n, want = 5, 5
+    else:
+        owner, nb, n = _lattice(4, 4, 1)
+        want = None
+    conf = OC.conflict(owner, nb, n, reach)
+    col = OC.bucket_saturation_colouring(conf, n)
+    assert OC.collision_free(conf, col)
+    if want is not None:
+        assert int(col.max()) + 1 == want == int(OC.saturation_colouring(conf, n).max()) + 1
+
+
+def _tet_edges():
+    from aquaflux_amd.mesh import tet_grid_3d
+
+    m = tet_grid_3d(3, 3, 2, perturb=0.1, seed=1)
+    inner = m.neighbour >= 0
+    return m.owner[inner], m.neighbour[inner], m.n_cells
+
+
+def _brick_edges():
+    from tests.polymesh_cases import brick_mesh
+
+    m = brick_mesh(3, 4, 3, 2)
+    inner = m.neighbour >= 0
+    return m.owner[inner], m.neighbour[inner], m.n_cells
+
+
+@pytest.mark.parametrize("case, reach", [("lattice2d", 1), ("lattice2d", 2), ("lattice2d", 3), ("lattice3d", 1),
+                                         ("lattice3d", 2), ("lattice3d", 3), ("random", 2), ("tet", 1), ("tet", 2),
+                                         ("brick", 1), ("brick", 2)])
+def test_library_colouring_is_the_reference_colouring(case, reach):
+    """``afx_block_colouring`` equals the restated bucket queue vertex for vertex (bit-exact)."""
+    if case == "lattice2d":
+        owner, nb, n = _lattice(9, 7, 1)
+    elif case == "lattice3d":
+        owner, nb, n = _lattice(6, 5, 4)
+    elif case == "random":
+        rng = np.random.default_rng(3)
+        n = 60
+        owner = np.concatenate([np.arange(n - 1), rng.integers(0, n, 15)])
+        nb = np.concatenate([np.arange(1, n), rng.integers(0, n, 15)])
+        keep = owner != nb
+        owner, nb = owner[keep], nb[keep]
+    elif case == "tet":
+        owner, nb, n = _tet_edges()
+    else:
+        owner, nb, n = _brick_edges()
+    c = block_stencil_colouring(owner, nb, n, reach)
+    want = OC.bucket_saturation_colouring(OC.conflict(owner, nb, n, reach), n)
+    assert np.array_equal(c.colour.astype(np.int64), want)
+    assert c.n_colours == int(want.max()) + 1

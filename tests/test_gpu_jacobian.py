@@ -174,3 +174,42 @@ def test_shifted_jacobian_adds_the_shift_and_leaves_the_input(cuda):
     # the shifted copy's point-block Jacobi inverts its own (shifted) 1x1 diagonal blocks
     d = Js.to_dense().diagonal().to(cuda)
     assert torch.allclose(Js.from_cell_major(Js.pbj(Js.to_cell_major(v))), v / d, rtol=1e-13, atol=0.0)
+
+
+def _oracle_jacobian(kind):
+    """Dense Jacobian of the oracle residual by forward-mode AD (torch.func.jacfwd) at the same state
+    and parameters as the matching case above."""
+    from oracle import fv
+
+    if kind == "rans":
+        mesh, oset = RC.oracle_case("box3d_sst", "limited", "lsq", nu=1e-3)
+        st = torch.tensor(RC.seeded_state(mesh, 3, "cavity"))
+        C = mesh.n_cells
+        mu, rho = torch.full((C,), 1e-3, dtype=F64), torch.ones(C, dtype=F64)
+        return torch.func.jacfwd(lambda z: RC.oracle_residual(oset, z, mu, rho, "limited"))(st)
+    if kind == "flow":
+        mesh, (m, g, s) = FC.oracle_case("channel2d", "limited", "lsq")
+        st = torch.tensor(FC.seeded_state(mesh, 2, "channel"), dtype=F64)
+        C = mesh.n_cells
+        mu, rho = torch.full((C,), 0.01, dtype=F64), torch.ones(C, dtype=F64)
+        return torch.func.jacfwd(lambda z: fv.flow_residual(m, g, s, z, mu, rho))(st)
+    mesh, u, bcs = SC.make("tet3")
+    phi, gamma = SC.seeded(mesh, 4)
+    m = fv.OMesh.from_mesh(mesh)
+    g = fv.geometry(m)
+    s = SC.oracle_setup(mesh, bcs, "limited", "gg")
+    flux = SC.oracle_flux(g, u)
+    return torch.func.jacfwd(lambda z: fv.scalar_residual(m, g, s, z, flux, torch.tensor(gamma)))(torch.tensor(phi))
+
+
+@pytest.mark.parametrize("make", [_rans, _flow, _scalar], ids=["rans", "flow", "scalar"])
+def test_probed_jacobian_equals_the_oracle_jacobian(make, cuda):
+    """materialize_block_jacobian on the GPU against torch.func.jacfwd of the CPU oracle (the stand-in
+    for jax.jacfwd of the reference residual), FP64, on the same small cases."""
+    mesh, matvec, reach = make(cuda)
+    plan = mesh_probe_plan(mesh, reach)
+    J = materialize_block_jacobian(matvec, plan, device=cuda).to_dense()
+    ref = _oracle_jacobian(make.__name__.lstrip("_"))
+    err = _rel(J, ref)
+    print("probed vs oracle jacfwd err", err, flush=True)
+    assert err < 1e-11
