@@ -207,6 +207,8 @@ _SIGNATURES = {
     "afx_aggregate_prolong": [ctypes.c_int64, ctypes.c_int, c_void_p, c_void_p, ctypes.c_double, c_void_p, c_void_p],
     "afx_dot": [ctypes.c_int64, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "afx_dense_matvec": [ctypes.c_int64, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    "afx_basis_combine": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int, c_void_p, ctypes.c_int64, c_void_p, c_void_p,
+                          c_void_p],
     "afx_profile_enable": [ctypes.c_int64],
     "afx_profile_collect": [c_void_p, c_void_p],
     "afx_profile_kernel_count": [],

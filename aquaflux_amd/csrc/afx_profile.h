@@ -46,6 +46,7 @@ enum KernelId : int {
   kKCggSweepAdj,
   kKProbeScatter,
   kKAmg,
+  kKKrylov,
   kKNumKernels
 };
 

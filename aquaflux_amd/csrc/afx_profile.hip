@@ -19,6 +19,7 @@ const char* const kKernelNames[kKNumKernels] = {
     "geom_adj_face", "geom_adj_node",   "rans_scalar_res", "rans_flow_res_adj",
     "limiter",       "cgg_sweep",     "cgg_sweep_adj",   "probe_scatter",
     "amg",
+    "krylov",
 };
 
 namespace {

@@ -22,6 +22,7 @@ import ctypes
 import dataclasses
 import math
 import time
+import weakref
 
 import numpy as np
 import torch
@@ -50,8 +51,10 @@ def gmres(matvec, b, *, x0=None, rtol=1e-10, atol=0.0, restart=30, max_iter=1000
     ``reorthogonalize``: ``"dgks"`` (a second Gram-Schmidt pass when the first lost more than
     1 - 1/sqrt(2) of the vector's norm) or ``"always"`` (two passes every step).  ``basis_dtype``
     stores the Krylov (and flexible) basis and runs Gram-Schmidt in another precision (FP32 halves
-    the orthogonalisation traffic of a solve that needs only a few digits); residuals, products
-    and the update stay in ``b``'s precision."""
+    the orthogonalisation traffic of a solve that needs only a few digits).  Residuals and products
+    stay in ``b``'s precision, and so does the update: ``x + Z y`` widens each stored basis entry
+    and sums in FP64 (``afx_basis_combine``), so an FP32 basis rounds the search directions but
+    not the correction built from them."""
     if reorthogonalize not in ("dgks", "always"):
         raise ValueError(f"unknown reorthogonalize {reorthogonalize!r}")
     dtype, dev = b.dtype, b.device
@@ -120,14 +123,28 @@ def gmres(matvec, b, *, x0=None, rtol=1e-10, atol=0.0, restart=30, max_iter=1000
             if abs(g[j + 1]) <= tol or info.iterations >= max_iter or hv[j + 1] == 0.0:
                 break
         y = np.linalg.solve(np.triu(H[:k, :k]) + np.diag(np.where(np.diag(H[:k, :k]) == 0, 1.0, 0.0)), g[:k])
-        yt = torch.tensor(y, dtype=bdt, device=dev)
         if Z is not None:
-            x = x + (Z[:k].T @ yt).to(dtype)
+            x = _combine(x, Z, k, y)
+        elif precond is None:
+            x = _combine(x, V, k, y)
         else:
-            upd = V[:k].T @ yt
-            x = x + (upd if precond is None else precond(upd)).to(dtype)
+            upd = _combine(torch.zeros_like(x), V, k, y)
+            x = x + precond(upd.to(bdt) if bdt != dtype else upd).to(dtype)
         info.restarts += 1
         info.residual_norm = abs(g[k])
+
+
+def _combine(x, basis, k, y):
+    """``x + basis[:k]^T y`` accumulated in ``x``'s precision from a basis of either precision
+    (``afx_basis_combine`` for FP64 ``x``)."""
+    if x.dtype != torch.float64:
+        return x + (basis[:k].T @ torch.tensor(y, dtype=basis.dtype, device=basis.device)).to(x.dtype)
+    yd = torch.tensor(np.asarray(y, dtype=np.float64), device=x.device)
+    x = x.contiguous()
+    prec = 0 if basis.dtype == torch.float32 else 1
+    _lib.check(_lib.lib().afx_basis_combine(x.numel(), int(k), prec, basis.data_ptr(), basis.stride(0), yd.data_ptr(),
+                                           x.data_ptr(), _lib.stream_of(x.device)), "afx_basis_combine")
+    return x
 
 
 class BlockJacobian:
@@ -267,24 +284,33 @@ class DampedNewtonStep:
     column_reach: tuple | None = None
     block_preconditioner: object = None
     outer_basis: str = "fp64"  # precision of the outer Krylov basis ("fp64" or "fp32")
-    _plans: dict = dataclasses.field(default_factory=dict, repr=False)
+    # probe plans per mesh (held weakly: a plan dies with its mesh) and column reach
+    _plans: weakref.WeakKeyDictionary = dataclasses.field(default_factory=weakref.WeakKeyDictionary, repr=False)
 
-    def block_jacobian(self, op, phi):
+    def block_jacobian(self, op, phi, precision: str | None = None):
+        """The block Jacobian of ``op`` at ``phi`` in ``precision`` (default: the step's own)."""
+        precision = self.precision if precision is None else precision
         if self.jacobian == "compact":
-            return BlockJacobian(op, phi, self.precision)
+            from .flow import MomentumContinuity
+
+            if not isinstance(op, MomentumContinuity):
+                raise ValueError(f"the compact Jacobian is the coupled-flow residual's own; a {type(op).__name__} "
+                                 "needs jacobian='probed' with column_reach")
+            return BlockJacobian(op, phi, precision)
         if self.jacobian != "probed":
             raise ValueError(f"unknown jacobian {self.jacobian!r}")
         from .jacobian import materialize_block_jacobian, mesh_probe_plan
 
         if self.column_reach is None:
             raise ValueError("the probed Jacobian needs column_reach (the stencil reach of each state field)")
-        key = (id(op.mesh), tuple(self.column_reach))
-        plan = self._plans.get(key)
+        per_mesh = self._plans.setdefault(op.mesh, {})
+        key = tuple(self.column_reach)
+        plan = per_mesh.get(key)
         if plan is None:
             plan = mesh_probe_plan(op.mesh, self.column_reach)
-            self._plans[key] = plan
+            per_mesh[key] = plan
         return materialize_block_jacobian(lambda v: op.residual_jvp(phi, v)[1], plan, device=phi.device,
-                                          precision=self.precision)
+                                          precision=precision)
 
     def pseudo_time_shift(self, op, phi, cfl):
         """Diagonal ``a_P / CFL`` on the momentum rows (zero on continuity): a local pseudo-time step
@@ -437,9 +463,10 @@ class ImplicitNewtonSolver:
 
     def adjoint(self, op, phi, cotangent):
         """``lambda`` solving ``J(phi)^T lambda = cotangent``: FGMRES on the HIP adjoint, preconditioned
-        by GMRES on the transposed compact Jacobian with transposed point-block Jacobi (the forward
-        preconditioner, transposed, as ``solve/implicit.py:709-730`` does)."""
-        J = BlockJacobian(op, phi, "fp64")
+        by GMRES on the transposed block Jacobian of the forward step (compact or probed, here in
+        FP64) with transposed point-block Jacobi (the forward preconditioner, transposed, as
+        ``solve/implicit.py:709-730`` does)."""
+        J = self.forward_step.block_jacobian(op, phi, precision="fp64")
         step = self.forward_step
 
         def precond(v):

@@ -394,13 +394,14 @@ def tet_cpu_baseline(n=32, reps=5):
 
 
 def newton_compare(device, nx=200, ny=100, nz=100, mu=1e-3, max_steps=80, cfl0=10.0, preconditioner="block-simpler",
-                   block_options=None, restart=60, outer_basis="fp32"):
+                   block_options=None, restart=60, fp32_outer_basis="fp32", inner_restart=50):
     """Newton to ||R|| <= 1e-12 + 1e-10 ||R0|| on the hex box (u, v, w, p) at Re = 1/mu, from the
-    seeded state, with the compact block Jacobian in FP32 and in FP64 under ``preconditioner``:
-    "inner" (GMRES with point-block Jacobi) or "block-<composition>" (the SIMPLE-family block
-    preconditioner with that composition, further ``BlockPreconditioner.build`` arguments in
-    ``block_options``); the outer FGMRES restarts every ``restart`` iterations and keeps its basis
-    in ``outer_basis`` precision."""
+    seeded state, twice: FP32 inner solves on the FP32 compact block Jacobian, and the all-FP64 path
+    (FP64 Jacobian, FP64 inner solves, FP64 outer Krylov basis).  ``preconditioner``: "inner"
+    (GMRES(``inner_restart``) with point-block Jacobi) or "block-<composition>" (the SIMPLE-family
+    block preconditioner with that composition, further ``BlockPreconditioner.build`` arguments in
+    ``block_options``).  The outer FGMRES restarts every ``restart`` iterations; the FP32 run keeps
+    its outer basis in ``fp32_outer_basis`` precision (its residuals, JVPs and update stay FP64)."""
     import torch
 
     from aquaflux_amd.boundary import BoundaryConditions
@@ -430,12 +431,14 @@ def newton_compare(device, nx=200, ny=100, nz=100, mu=1e-3, max_steps=80, cfl0=1
     def make_step(prec, **kw):
         # a 60-vector outer FGMRES basis: the block preconditioner's steps take up to ~100 outer
         # iterations, and restarting at 30 cost 7% on the 2M-cell box
+        basis = fp32_outer_basis if prec == "fp32" else "fp64"
         if preconditioner == "inner":
-            return DampedNewtonStep(preconditioner="inner", precision=prec, restart=restart, outer_basis=outer_basis,
-                                    **kw)
+            kw.setdefault("inner_iters", inner_restart)
+            return DampedNewtonStep(preconditioner="inner", precision=prec, restart=restart, outer_basis=basis,
+                                    inner_restart=inner_restart, **kw)
         bp = BlockPreconditioner.build(mc, composition=preconditioner.removeprefix("block-"), **(block_options or {}))
         return DampedNewtonStep(preconditioner="block", precision=prec, block_preconditioner=bp, restart=restart,
-                                outer_basis=outer_basis, **kw)
+                                outer_basis=basis, **kw)
 
     for prec in ("fp32", "fp64"):  # warm up every kernel and allocation outside the timed solves
         make_step(prec, max_iter=2, inner_iters=2).correction(mc, x0, mc.residual_raw(x0))
@@ -444,8 +447,10 @@ def newton_compare(device, nx=200, ny=100, nz=100, mu=1e-3, max_steps=80, cfl0=1
     out = {"workload": f"hex box {nx}x{ny}x{nz} ({C} cells, {4 * C} unknowns), Re={1.0 / mu:g}, first-order upwind, "
                        "least-squares gradients, seeded state; pseudo-transient continuation (SER, CFL0="
                        f"{cfl0:g}) into Newton; stop at ||R|| <= 1e-12 + 1e-10 ||R0||",
-           "preconditioner": preconditioner, "block_options": block_options or {}, "outer_restart": restart,
-           "outer_basis": outer_basis}
+           "preconditioner": preconditioner if preconditioner != "inner" else
+           f"inner GMRES({inner_restart}) on the block Jacobian with point-block Jacobi",
+           "block_options": block_options or {}, "outer_restart": restart,
+           "outer_basis": {"fp32": fp32_outer_basis, "fp64": "fp64"}}
     roots = {}
     for prec in ("fp32", "fp64"):
         def progress(step, rel, kits, alpha, secs, prec=prec):
@@ -503,13 +508,17 @@ def main(argv=None):
     ap.add_argument("--newton-size", type=int, nargs=3, default=(200, 100, 100), metavar=("NX", "NY", "NZ"))
     ap.add_argument("--newton-cfl0", type=float, default=10.0)
     ap.add_argument("--newton-mu", type=float, default=1e-3, help="viscosity of the Newton box (Re = 1/mu)")
-    ap.add_argument("--newton-preconditioner", default="block-simpler",
-                    choices=["inner", "block-triangular", "block-simple", "block-simpler"])
+    ap.add_argument("--newton-preconditioner", default=None,
+                    choices=["inner", "block-triangular", "block-simple", "block-simpler"],
+                    help="one Newton comparison with this preconditioner (default: the inner GMRES(30) "
+                    "point-block-Jacobi comparison, then the block-SIMPLER one)")
     ap.add_argument("--newton-block-options", type=json.loads, default=None, metavar="JSON",
                     help='further BlockPreconditioner.build arguments, e.g. \'{"schur_iters": 4}\'')
     ap.add_argument("--newton-restart", type=int, default=60, help="outer FGMRES restart of the Newton solves")
-    ap.add_argument("--newton-outer-basis", default="fp32", choices=["fp64", "fp32"],
-                    help="precision of the outer FGMRES basis (residuals, JVPs and updates stay FP64)")
+    ap.add_argument("--newton-outer-basis", default=None, choices=["fp64", "fp32"],
+                    help="precision of the FP32 run's outer FGMRES basis (residuals, JVPs and updates stay FP64; "
+                    "default fp64 for the inner GMRES comparison, fp32 for the block ones); the FP64 run is all-FP64")
+    ap.add_argument("--newton-inner-restart", type=int, default=30, help="inner GMRES basis of the 'inner' comparison")
     ap.add_argument(
         "--calibrate-copy",
         type=int,
@@ -534,12 +543,21 @@ def main(argv=None):
     from aquaflux_amd import _lib
 
     dev = torch.device("cuda", 0)
+    def newton_lines():
+        precs = [args.newton_preconditioner] if args.newton_preconditioner else ["inner", "block-simpler"]
+        out = {}
+        for prec in precs:
+            basis = args.newton_outer_basis or ("fp64" if prec == "inner" else "fp32")
+            key = "newton" if prec == precs[0] else "newton_block"
+            out[key] = newton_compare(dev, *args.newton_size, mu=args.newton_mu, cfl0=args.newton_cfl0,
+                                      preconditioner=prec, block_options=args.newton_block_options,
+                                      restart=args.newton_restart, fp32_outer_basis=basis,
+                                      inner_restart=args.newton_inner_restart)
+            torch.cuda.empty_cache()
+        return out
+
     if args.newton_only:
-        print(json.dumps(newton_compare(dev, *args.newton_size, mu=args.newton_mu, cfl0=args.newton_cfl0,
-                                        preconditioner=args.newton_preconditioner,
-                                        block_options=args.newton_block_options, restart=args.newton_restart,
-                                        outer_basis=args.newton_outer_basis)),
-              flush=True)
+        print(json.dumps(newton_lines()), flush=True)
         return 0
     wl = args.workload
     if wl == "tet":
@@ -649,10 +667,7 @@ def main(argv=None):
                                                   "roofline")}
         torch.cuda.empty_cache()
     if not args.no_newton:
-        out["newton"] = newton_compare(dev, *args.newton_size, mu=args.newton_mu, cfl0=args.newton_cfl0,
-                                       preconditioner=args.newton_preconditioner,
-                                       block_options=args.newton_block_options, restart=args.newton_restart,
-                                       outer_basis=args.newton_outer_basis)
+        out.update(newton_lines())
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, *args.cpu_size, reps=args.cpu_reps)
     print(json.dumps(out), flush=True)

@@ -426,6 +426,13 @@ int afx_dot(int64_t n, int precision, const void* a, const void* b, double* part
 /* y = A x for a dense row-major n x n matrix (the coarsest level's pseudo-inverse); x and y differ. */
 int afx_dense_matvec(int64_t n, int precision, const void* A, const void* x, void* y, hipStream_t stream);
 
+/* ---- Krylov basis ------------------------------------------------------------------------------ */
+/* x[i] += sum_{j<k} y[j] basis[j * ld + i] with basis FP32 (precision 0) or FP64 (1), y and x FP64,
+ * the k terms summed in FP64 in index order.  Replaces the update x + Z y of the restarted GMRES in
+ * solve_linear (reference: aquaflux/solve/linear.py:199-298), where the basis is FP64. */
+int afx_basis_combine(int64_t n, int32_t k, int precision, const void* basis, int64_t ld, const double* y, double* x,
+                      hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

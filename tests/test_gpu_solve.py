@@ -156,3 +156,97 @@ def test_ift_adjoint_matches_finite_difference(cuda):
     fd = float(mean_speed(0.01 + h) - mean_speed(0.01 - h)) / (2 * h)
     assert abs(fd) > 1e-2
     assert abs(float(grad) - fd) <= 3e-2 * abs(fd)
+
+
+def _hex_box(cuda, nx, ny, nz, mu):
+    """The 3D hex box of the mixed-precision Newton workload: inlet u = (1, 0, 0) left, pressure outlet
+    right, no-slip walls elsewhere, first-order upwind, least-squares gradients, Re = 1/mu, and the
+    seeded state u = 1 + 0.05 N, v, w = 0.05 N, p = N(0, 0.1) (seed 2)."""
+    from aquaflux_amd.boundary import BoundaryConditions
+    from aquaflux_amd.discretization import FirstOrderUpwind
+    from aquaflux_amd.flow import MomentumContinuity, NoSlipWall, PressureOutlet, VelocityInlet
+    from aquaflux_amd.mesh import structured_grid_3d
+    from aquaflux_amd.properties import Constant, PropertyModel
+    from aquaflux_amd.schemes import LeastSquaresGradient
+
+    mesh = structured_grid_3d(nx, ny, nz, named_boundaries=True)
+    geom = mesh.geometry(device=cuda)
+    bcs = BoundaryConditions({"left": VelocityInlet((1.0, 0.0, 0.0)), "right": PressureOutlet(0.0),
+                              "bottom": NoSlipWall(), "top": NoSlipWall(), "back": NoSlipWall(), "front": NoSlipWall()})
+    mc = MomentumContinuity.build(mesh, geom, PropertyModel({"viscosity": Constant(mu), "density": Constant(1.0)}),
+                                  LeastSquaresGradient(), bcs, advection_scheme=FirstOrderUpwind())
+    rng = np.random.default_rng(2)
+    C = mesh.n_cells
+    x0 = torch.tensor(np.concatenate([1.0 + 0.05 * rng.normal(size=C), 0.05 * rng.normal(size=C),
+                                      0.05 * rng.normal(size=C), rng.normal(0.0, 0.1, C)]), dtype=F64, device=cuda)
+    return mc, x0
+
+
+@pytest.mark.parametrize("preconditioner", ["inner", "block"])
+def test_mixed_precision_newton_on_the_re1000_hex_box(cuda, preconditioner):
+    """The mixed-precision Newton workload on a reduced box (40x20x20, Re = 1000, the same boundary
+    conditions and seeded state): FP32 inner solves on the FP32 block Jacobian (GMRES(30) with
+    point-block Jacobi, or the block-SIMPLER preconditioner) against the all-FP64 path (FP64 Jacobian,
+    FP64 inner solves, FP64 outer basis).  Both reach ||R|| <= 1e-12 + 1e-10 ||R0|| and their roots
+    agree to 1e-9."""
+    from aquaflux_amd.block_preconditioner import BlockPreconditioner
+
+    print(f"case hex box 40x20x20 Re=1000, {preconditioner}", flush=True)
+    mc, x0 = _hex_box(cuda, 40, 20, 20, 1e-3)
+    roots = {}
+    for prec in ("fp32", "fp64"):
+        kw = dict(precision=prec, ptc_cfl0=10.0, restart=60, outer_basis=prec)
+        if preconditioner == "inner":
+            step = DampedNewtonStep(preconditioner="inner", inner_restart=30, inner_iters=30, **kw)
+        else:
+            step = DampedNewtonStep(preconditioner="block", block_preconditioner=BlockPreconditioner.build(
+                mc, composition="simpler"), **kw)
+        phi, rep = ImplicitNewtonSolver(max_steps=60, forward_step=step).forward(mc, x0)
+        print(prec, rep.steps, "steps", f"{rep.seconds:.2f} s", rep.residual_norms[-1] / rep.residual_norms[0],
+              flush=True)
+        assert rep.converged and rep.residual_norms[-1] <= 1e-12 + 1e-10 * rep.residual_norms[0]
+        roots[prec] = phi
+    d = float((roots["fp32"] - roots["fp64"]).abs().max() / roots["fp64"].abs().max())
+    print("root difference", d, flush=True)
+    assert d <= 1e-9
+
+
+def test_fp32_basis_update_keeps_fp64_accuracy(cuda):
+    """The GMRES update x + Z y from an FP32 basis widens each entry and sums in FP64
+    (afx_basis_combine): equal to the FP64 product of the widened basis to 1e-15; and GMRES with an
+    FP32 basis still reaches a 1e-13 residual through its FP64 restarts."""
+    from aquaflux_amd.solve import _combine
+
+    g = torch.Generator(device="cpu").manual_seed(3)
+    Z = torch.randn(12, 10001, generator=g, dtype=torch.float32).to(cuda)
+    y = np.random.default_rng(4).normal(size=9)
+    x0 = torch.randn(10001, generator=g, dtype=F64).to(cuda)
+    got = _combine(x0.clone(), Z, 9, y)
+    want = x0 + Z[:9].double().T @ torch.tensor(y, dtype=F64, device=cuda)
+    assert float((got - want).abs().max()) <= 1e-15 * float(want.abs().max())
+    n = 4096
+    d = torch.linspace(1.0, 3.0, n, dtype=F64, device=cuda)
+    b = torch.sin(torch.arange(n, dtype=F64, device=cuda))
+    x, info = gmres(lambda v: d * v, b, rtol=1e-13, restart=20, max_iter=400, basis_dtype=torch.float32)
+    assert info.converged
+    assert float((d * x - b).norm() / b.norm()) <= 1e-13
+
+
+def test_rans_adjoint_solve_uses_the_probed_jacobian(cuda):
+    """The IFT adjoint solve J^T lambda = g for a coupled RANS operator runs on the forward step's
+    probed Jacobian (FP64) and returns lambda with ||J^T lambda - g|| <= 1e-9 ||g|| (checked with the
+    HIP adjoint); a compact-Jacobian step refuses the RANS operator with a ValueError."""
+    from tests import rans_cases as RC
+
+    print("case channel2d_sst limited lsq, adjoint solve", flush=True)
+    mesh, rans, _ = RC.build("channel2d_sst", "limited", "limited", "lsq", nu=1e-3, device=cuda)
+    st = torch.tensor(RC.seeded_state(mesh, 5, "channel"), device=cuda)
+    g = torch.tensor(np.random.default_rng(7).normal(size=st.numel()), device=cuda)
+    step = DampedNewtonStep(preconditioner="inner", precision="fp32", jacobian="probed", column_reach=(3, 3, 2, 2, 2))
+    solver = ImplicitNewtonSolver(forward_step=step, adjoint_rtol=1e-11)
+    lam = solver.adjoint(rans, st, g)
+    res = rans.residual_vjp(st, lam) - g
+    print("adjoint residual", float(res.norm() / g.norm()), flush=True)
+    assert float(res.norm()) <= 1e-9 * float(g.norm())
+    with pytest.raises(ValueError):
+        ImplicitNewtonSolver(forward_step=DampedNewtonStep()).adjoint(rans, st, g)
