@@ -259,3 +259,23 @@ int venkat_pass(const MeshView& m, int dim, const LimiterSlots& sl, In<dual> x, 
 }
 
 }  // namespace afx
+
+// Stand-alone Venkatakrishnan limiter of one scalar (and, with tangents, its JVP).
+extern "C" int afx_venkat_limiter(const afx_mesh* mh, const afx_geometry* g, double limiter_k, const double* phi,
+                                  const double* grad, const double* phi_t, const double* grad_t, double* psi,
+                                  double* psi_t, hipStream_t s) {
+  AFX_REQUIRE(mh && g && phi && grad && psi, "afx_venkat_limiter: null argument");
+  AFX_REQUIRE((phi_t == nullptr && grad_t == nullptr) || psi_t != nullptr,
+              "afx_venkat_limiter: tangents need a psi_t buffer");
+  afx::MeshView m;
+  if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
+  if (m.C == 0) return 0;
+  afx::LimiterSlots sl{};
+  sl.nl = 1;
+  sl.ext_stride = 1;
+  if (psi_t == nullptr)
+    return afx::venkat_pass(m, mh->dim, sl, afx::In<double>{phi, nullptr}, afx::In<double>{grad, nullptr}, limiter_k,
+                            afx::Out<double>{psi, nullptr}, nullptr, s);
+  return afx::venkat_pass(m, mh->dim, sl, afx::In<afx::dual>{phi, phi_t}, afx::In<afx::dual>{grad, grad_t}, limiter_k,
+                          afx::Out<afx::dual>{psi, psi_t}, nullptr, s);
+}

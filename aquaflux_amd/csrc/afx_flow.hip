@@ -329,6 +329,35 @@ __global__ void __launch_bounds__(256) k_flow_res(MeshView m, FlowArgs<S> a, Out
   R.set((int64_t)D * m.C + c, cont);
 }
 
+// Rhie-Chow face mass flux of every face, written by the face's owner in the caller's face order
+// (the mdot that continuity sums and the advection fluxes carry).
+template <int D, class S, int ADV>
+__global__ void __launch_bounds__(256) k_flow_mdot(MeshView m, FlowArgs<S> a, Out<S> mdot) {
+  int32_t c = xcd_thread_index(m);
+  if (c >= m.C) return;
+  FlowCell<D, S> X;
+  load_cell_full<D, S>(a, m, c, true, ADV == kAdvLimited, X);
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, k, c);
+    if (!r.own) continue;
+    FaceGeo<D> q;
+    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    S out[D + 1];
+    if (r.N >= 0) {
+      FlowCell<D, S> O;
+      load_cell_full<D, S>(a, m, r.N, true, ADV == kAdvLimited, O);
+      flow_flux_interior<D, ADV, S>(q, X, O, out);
+    } else {
+      S ub[D];
+#pragma unroll
+      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.fu];
+      S gb = a.gB.v ? a.gB[r.fu] : X.mu;
+      flow_flux_boundary<D, ADV, S>(q, a.kind[r.fu], X, ub, a.pb[r.fu], gb, out);
+    }
+    mdot.set(r.fu, out[D]);
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // reverse pass R': cotangent of the residual -> cell-local cotangents of state, gradients, a_P,
 // mu, rho, psi (written), and of the boundary-face data of faces this cell owns.
@@ -748,7 +777,7 @@ FlowArgs<S> make_args(const afx_flow_config* cfg, const afx_flow_inputs* in, con
 template <int D, class S, int ADV, int GRAD>
 int run_forward(const afx_mesh* mh, const afx_geometry* g, FlowArgs<S> a, Out<S> R, double* G, double* Gt,
                 double* aP, double* aPt, double* psi, double* psit, int32_t* ext, double* cgg, int sweeps,
-                double relax, hipStream_t s) {
+                double relax, hipStream_t s, Out<S> mdot = Out<S>{}) {
   MeshView m;
   if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
   Out<S> oG{G, Gt}, oP{psi, psit}, oA{aP, aPt};
@@ -779,11 +808,16 @@ int run_forward(const afx_mesh* mh, const afx_geometry* g, FlowArgs<S> a, Out<S>
   }
   AFX_LAUNCH(kKFlowAp, (k_flow_ap<D, S, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oA);
   AFX_CHECK_LAUNCH();
-  if (R.v == nullptr) return 0;
+  if (R.v == nullptr && mdot.v == nullptr) return 0;
   a.aP.v = aP;
   a.aP.t = aPt;
   a.psi.v = psi;
   a.psi.t = psit;
+  if (mdot.v != nullptr) {
+    AFX_LAUNCH(kKFlowRes, (k_flow_mdot<D, S, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, mdot);
+    AFX_CHECK_LAUNCH();
+  }
+  if (R.v == nullptr) return 0;
   AFX_LAUNCH(kKFlowRes, (k_flow_res<D, S, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, R);
   AFX_CHECK_LAUNCH();
   return 0;
@@ -919,5 +953,30 @@ extern "C" int afx_flow_fields(const afx_mesh* m, const afx_geometry* g, const a
     return run_forward<decltype(Dc)::value, double, decltype(Ac)::value, decltype(Gc)::value>(
         m, g, a, Out<double>{nullptr, nullptr}, gradients, nullptr, a_p, nullptr, psi, nullptr, nullptr, cgg,
         cfg->gradient_sweeps, cfg->gradient_relaxation, s);
+  });
+}
+
+extern "C" int afx_flow_mass_flux(const afx_mesh* m, const afx_geometry* g, const afx_flow_config* cfg,
+                                  const afx_flow_inputs* in, const afx_flow_inputs* tangent, double* mdot,
+                                  double* mdot_tangent, void* work, int64_t work_size, hipStream_t s) {
+  const int mode = tangent ? 1 : 0;
+  int rc = check_common(m, g, cfg, in, work, work_size, mode);
+  if (rc) return rc;
+  AFX_REQUIRE(mdot, "afx_flow_mass_flux: null mdot");
+  AFX_REQUIRE(!tangent || mdot_tangent, "afx_flow_mass_flux: a tangent needs an mdot_tangent buffer");
+  Work w = carve(work, m->dim, m->C, mode);
+  if (!tangent) {
+    FlowArgs<double> a = make_args<double>(cfg, in, nullptr);
+    return dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
+      return run_forward<decltype(Dc)::value, double, decltype(Ac)::value, decltype(Gc)::value>(
+          m, g, a, Out<double>{nullptr, nullptr}, w.G, nullptr, w.aP, nullptr, w.psi, nullptr, nullptr, w.cgg,
+          cfg->gradient_sweeps, cfg->gradient_relaxation, s, Out<double>{mdot, nullptr});
+    });
+  }
+  FlowArgs<dual> a = make_args<dual>(cfg, in, tangent);
+  return dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
+    return run_forward<decltype(Dc)::value, dual, decltype(Ac)::value, decltype(Gc)::value>(
+        m, g, a, Out<dual>{nullptr, nullptr}, w.G, w.Gt, w.aP, w.aPt, w.psi, w.psit, nullptr, w.cgg,
+        cfg->gradient_sweeps, cfg->gradient_relaxation, s, Out<dual>{mdot, mdot_tangent});
   });
 }

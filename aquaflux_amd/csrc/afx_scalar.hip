@@ -987,6 +987,90 @@ __global__ void __launch_bounds__(256) k_scalar_grad(MeshView m, const int8_t* _
   }
 }
 
+// Reverse of k_scalar_grad.  Green-Gauss (GRAD 0): `bbar` is the cotangent of the face-value sum
+// b_c = sum_f +-phi_f n_f A_f (compact: gradient cotangent / V, applied here when COMPACT; corrected:
+// the sweeps' reverse already produced it); an interior face passes (1 - g) resp. g of
+// n A . (bbar_P - bbar_N) to its owner resp. neighbour.  Least squares (GRAD 1): `bbar` holds
+// y = M^-T Gbar per cell (k_lsq_solve_t) and a face adds w dx . y of each side with the sign of its
+// difference.  A Dirichlet boundary face's value receives its share in value_bar (caller order);
+// zero-gradient-type closures read the owner value, whose difference to the owner has zero
+// derivative in least squares and which feeds the owner in Green-Gauss.
+template <int D, int GRAD, bool COMPACT>
+__global__ void __launch_bounds__(256) k_scalar_grad_vjp(MeshView m, const int8_t* __restrict__ kind,
+                                                         const double* __restrict__ bbar,
+                                                         double* __restrict__ phi_bar, double* __restrict__ value_bar) {
+  const int32_t c = xcd_thread_index(m);
+  if (c >= m.C) return;
+  double yc[D];
+  const double ivc = COMPACT ? 1.0 / m.vol[c] : 1.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) yc[j] = bbar[(int64_t)j * m.C + c] * ivc;
+  double acc = 0.0;
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    FaceRef r = decode_face(m, k, c);
+    const int32_t o = r.own ? r.N : r.P;
+    if (o >= 0) {
+      const double ivo = COMPACT ? 1.0 / m.vol[o] : 1.0;
+      double yo[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) yo[j] = bbar[(int64_t)j * m.C + o] * ivo;
+      if (GRAD == 0) {
+        FaceGeo<D> q;
+        load_face_geo<D>(m, r.f, r.P, r.N, q);
+        double sz = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) sz += (q.n[j] * q.A) * (r.own ? yc[j] - yo[j] : yo[j] - yc[j]);
+        acc += (r.own ? 1.0 - q.g : q.g) * sz;
+      } else {
+        double dx[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dx[j] = m.cc[(int64_t)j * m.C + o] - m.cc[(int64_t)j * m.C + c];
+        const double wgt = 1.0 / vdot<D>(dx, dx);
+        double s2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) s2 += dx[j] * (yc[j] + yo[j]);
+        acc -= wgt * s2;
+      }
+    } else {
+      const int8_t kd = kind[r.fu];
+      double t = 0.0;
+      if (GRAD == 0) {
+        const double A = m.area[r.f];
+#pragma unroll
+        for (int j = 0; j < D; ++j) t += (m.nrm[(int64_t)j * m.F + r.f] * A) * yc[j];
+        if (kd != kBcDirichlet && kd != kBcNone) acc += t;
+      } else {
+        double dx[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dx[j] = m.fc[(int64_t)j * m.F + r.f] - m.cc[(int64_t)j * m.C + c];
+        const double wgt = 1.0 / vdot<D>(dx, dx);
+        t = wgt * vdot<D>(dx, yc);
+        if (kd == kBcDirichlet || kd == kBcNone) acc -= t;  // the face value is fixed (or 0), phi_P enters as -phi_P
+      }
+      if (value_bar && kd == kBcDirichlet) value_bar[r.fu] = t;
+    }
+  }
+  phi_bar[c] = acc;
+}
+
+// y_c = M_c^-T gbar_c, with M the least-squares normal matrix and its inverse as k_scalar_grad forms it.
+template <int D>
+__global__ void __launch_bounds__(256) k_lsq_solve_t(MeshView m, const double* __restrict__ gbar,
+                                                     double* __restrict__ y) {
+  const int32_t c = xcd_thread_index(m);
+  if (c >= m.C) return;
+  double M[D][D], I[D][D];
+  lsq_matrix<D>(m, c, M);
+  sym_inverse<D>(M, I);
+#pragma unroll
+  for (int p = 0; p < D; ++p) {
+    double s = I[0][p] * gbar[c];
+#pragma unroll
+    for (int q = 1; q < D; ++q) s = s + I[q][p] * gbar[(int64_t)q * m.C + c];
+    y[(int64_t)p * m.C + c] = s;
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // host side
 
@@ -1158,6 +1242,46 @@ extern "C" int afx_scalar_gradient(const afx_mesh* mh, const afx_geometry* g, in
   AFX_SG(2, 0) AFX_SG(2, 1) AFX_SG(3, 0) AFX_SG(3, 1)
 #undef AFX_SG
   afx_set_error("afx_scalar_gradient: unsupported dimension");
+  return 2;
+}
+
+extern "C" int afx_scalar_gradient_vjp(const afx_mesh* mh, const afx_geometry* g, int gradient, int sweeps,
+                                       double relaxation, const int8_t* kind, const double* grad_bar,
+                                       double* phi_bar, double* value_bar, hipStream_t s) {
+  AFX_REQUIRE(mh && g && kind && grad_bar && phi_bar, "afx_scalar_gradient_vjp: null argument");
+  AFX_REQUIRE(gradient == 0 || gradient == 1 || gradient == 3, "afx_scalar_gradient_vjp: unknown gradient scheme");
+  AFX_REQUIRE(gradient != 3 || relaxation > 0.0, "afx_scalar_gradient_vjp: relaxation must be positive");
+  MeshView m;
+  if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
+  if (m.C == 0) return 0;
+  if (value_bar)
+    if (int rz = afx_fill_zero(value_bar, m.F, s)) return rz;
+  const int D = mh->dim;
+  const int64_t n = (int64_t)D * m.C;
+  double* scratch = afx_mesh_scratch(const_cast<afx_mesh*>(mh), (gradient == 3 ? 3 : 1) * n);
+  AFX_REQUIRE(scratch, "afx_scalar_gradient_vjp: could not allocate the scratch");
+#define AFX_SGV(DD)                                                                                          \
+  if (D == DD) {                                                                                             \
+    if (gradient == 1) {                                                                                     \
+      AFX_LAUNCH(kKScalarGradAdj, k_lsq_solve_t<DD>, grid_for(m.C), dim3(kBlock), 0, s, m, grad_bar, scratch);\
+      AFX_CHECK_LAUNCH();                                                                                    \
+      AFX_LAUNCH(kKScalarGradAdj, (k_scalar_grad_vjp<DD, 1, false>), grid_for(m.C), dim3(kBlock), 0, s, m,    \
+                 kind, scratch, phi_bar, value_bar);                                                         \
+    } else if (gradient == 0) {                                                                              \
+      AFX_LAUNCH(kKScalarGradAdj, (k_scalar_grad_vjp<DD, 0, true>), grid_for(m.C), dim3(kBlock), 0, s, m,     \
+                 kind, grad_bar, phi_bar, value_bar);                                                        \
+    } else {                                                                                                 \
+      AFX_CHECK_HIP(hipMemcpyAsync(scratch, grad_bar, n * 8, hipMemcpyDeviceToDevice, s));                   \
+      if (int rc = cgg_sweeps_adjoint(m, DD, 1, scratch, scratch + n, sweeps, relaxation, s)) return rc;     \
+      AFX_LAUNCH(kKScalarGradAdj, (k_scalar_grad_vjp<DD, 0, false>), grid_for(m.C), dim3(kBlock), 0, s, m,    \
+                 kind, scratch, phi_bar, value_bar);                                                         \
+    }                                                                                                        \
+    AFX_CHECK_LAUNCH();                                                                                      \
+    return 0;                                                                                                \
+  }
+  AFX_SGV(2) AFX_SGV(3)
+#undef AFX_SGV
+  afx_set_error("afx_scalar_gradient_vjp: unsupported dimension");
   return 2;
 }
 

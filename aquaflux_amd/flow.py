@@ -13,6 +13,7 @@ the per-cell viscosity / density, and the boundary velocity / pressure.
 from __future__ import annotations
 
 import ctypes
+from typing import NamedTuple
 
 import numpy as np
 import torch
@@ -32,6 +33,11 @@ class FlowBoundary:
 
     def prescribed_velocity(self, dim):
         return None
+
+    def face_velocity(self, dim, centroids):
+        """Prescribed velocity on faces with these centroids ``(n, d)``: ``(n, d)``, or None."""
+        v = self.prescribed_velocity(dim)
+        return None if v is None else v.to(centroids.device)[None, :].expand(centroids.shape[0], dim)
 
     def prescribed_pressure(self):
         return None
@@ -69,7 +75,9 @@ class MovingWall(FlowBoundary):
 
 class VelocityInlet(FlowBoundary):
     """Prescribed velocity, zero-gradient pressure, ``mdot = rho (u_in . n) A``
-    (``flow/boundary.py:349-384``).  ``velocity`` may be a tensor requiring grad."""
+    (``flow/boundary.py:360-397``).  ``velocity`` is a constant vector (a tensor may require grad)
+    or a profile: a callable mapping face centroids ``(n, d)`` to velocities ``(n, d)``, evaluated
+    at the patch's face centroids (``flow/boundary.py:372-376``)."""
 
     kind = FLOW_INLET
 
@@ -77,7 +85,23 @@ class VelocityInlet(FlowBoundary):
         self.velocity = velocity
 
     def prescribed_velocity(self, dim):
+        if _is_profile(self.velocity):
+            raise ValueError("this VelocityInlet carries a profile; its velocity depends on the face centroid")
         return _as_vector(self.velocity, dim)
+
+    def face_velocity(self, dim, centroids):
+        if not _is_profile(self.velocity):
+            return super().face_velocity(dim, centroids)
+        v = self.velocity(centroids)
+        v = v if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v), dtype=torch.float64)
+        v = v.to(device=centroids.device, dtype=torch.float64)
+        if v.shape != (centroids.shape[0], dim):
+            raise ValueError(f"velocity profile must return shape ({centroids.shape[0]}, {dim}), got {tuple(v.shape)}")
+        return v
+
+    def face_velocity_function(self, face_centroid, velocity_owner, normal):
+        """``FlowBoundary.velocity_face`` (``flow/boundary.py:372-376``) of this inlet."""
+        return self.face_velocity(face_centroid.shape[1], face_centroid)
 
 
 class PressureOutlet(FlowBoundary):
@@ -93,9 +117,13 @@ class PressureOutlet(FlowBoundary):
         return self.pressure
 
 
+def _is_profile(v):
+    return callable(v) and not isinstance(v, torch.Tensor)
+
+
 def _as_vector(v, dim):
-    if callable(v) and not isinstance(v, torch.Tensor):
-        raise ValueError("velocity profiles (callables) are not supported; pass a constant vector")
+    if _is_profile(v):
+        raise ValueError("only a VelocityInlet takes a velocity profile; pass a constant vector")
     t = v if isinstance(v, torch.Tensor) else torch.tensor(list(v), dtype=torch.float64)
     t = t.to(torch.float64)
     if t.shape != (dim,):
@@ -103,16 +131,21 @@ def _as_vector(v, dim):
     return t
 
 
-def _boundary_data(boundary, d, F, device):
+def _boundary_data(boundary, d, F, device, face_centroid=None):
     """Prescribed face velocity ``(d, F)`` and pressure ``(F,)`` folded patch by patch
-    (``boundary/collection.py:97-137``); tensor-valued closures stay differentiable."""
+    (``boundary/collection.py:97-137``); tensor-valued closures stay differentiable.  Profiles are
+    evaluated at ``face_centroid`` ``(F, d)``."""
     ub = torch.zeros((F, d), dtype=torch.float64, device=device)
     pb = torch.zeros(F, dtype=torch.float64, device=device)
     for _, bc, faces in boundary.items():
         idx = torch.from_numpy(faces).to(device)
-        v = bc.prescribed_velocity(d)
+        if face_centroid is not None:
+            v = bc.face_velocity(d, face_centroid[idx])
+        else:
+            v0 = bc.prescribed_velocity(d)
+            v = None if v0 is None else v0.to(device)[None, :].expand(idx.numel(), d)
         if v is not None:
-            ub = ub.index_put((idx,), v.to(device)[None, :].expand(idx.numel(), d))
+            ub = ub.index_put((idx,), v.to(device))
         p = bc.prescribed_pressure()
         if p is not None:
             pv = p if isinstance(p, torch.Tensor) else torch.tensor(float(p), dtype=torch.float64)
@@ -191,6 +224,26 @@ class _Raw:
         )
         return bars + ([gbb] if gb is not None else [])
 
+    def mass_flux(self, primals, tangents=None):
+        """Face mass flux ``(F,)`` (caller face order) and, with tangents, its tangent."""
+        state = primals[0]
+        F = self.mesh.n_faces
+        mdot = torch.empty(F, dtype=torch.float64, device=state.device)
+        dm = torch.empty_like(mdot) if tangents is not None else None
+        mode = 0 if tangents is None else 1
+        w = self.dm.workspace(f"flow{mode}", self.work_bytes[mode])
+        ins = self.inputs(*primals)
+        tns = self.inputs(*tangents) if tangents is not None else None
+        _lib.check(
+            _lib.lib().afx_flow_mass_flux(
+                self.dm.handle, ctypes.byref(self.gs), ctypes.byref(self.cfg), ctypes.byref(ins),
+                ctypes.byref(tns) if tns is not None else None, mdot.data_ptr(), _lib.ptr(dm), w.data_ptr(),
+                w.numel(), _lib.stream_of(self.dm.device),
+            ),
+            "afx_flow_mass_flux",
+        )
+        return mdot, dm
+
     def fields(self, state, mu, rho, ub, pb, gb=None):
         d, C = self.mesh.dim, self.mesh.n_cells
         dev = state.device
@@ -235,6 +288,47 @@ class _FlowResidualFn(torch.autograd.Function):
             tans.append(dgb.contiguous() if dgb is not None else None)
         _, dR = ctx.raw.jvp(prim, tans)
         return dR
+
+
+class _MassFluxFn(torch.autograd.Function):
+    """mdot(state, mu, rho, u_b, p_b[, gamma_b]): HIP forward and dual-number tangent."""
+
+    @staticmethod
+    def forward(ctx, raw, state, mu, rho, ub, pb, gb):
+        ctx.raw = raw
+        ctx.has_gb = gb is not None
+        prim = (state, mu, rho, ub, pb) + ((gb,) if gb is not None else ())
+        ctx.save_for_forward(*prim)
+        return raw.mass_flux(prim)[0]
+
+    @staticmethod
+    def backward(ctx, g):
+        raise NotImplementedError("mass_flux has a forward-mode derivative only; differentiate the residual, whose "
+                                  "adjoint carries mdot's, for reverse mode")
+
+    @staticmethod
+    def jvp(ctx, _raw_t, dstate, dmu, drho, dub, dpb, dgb):
+        prim = ctx.saved_tensors
+        tans = [t.contiguous() if t is not None else None for t in (dstate, dmu, drho, dub, dpb)]
+        if ctx.has_gb:
+            tans.append(dgb.contiguous() if dgb is not None else None)
+        return ctx.raw.mass_flux(prim, tans)[1]
+
+
+class FlowFields(NamedTuple):
+    """``FlowFields`` (``flow/momentum.py:92-109``) with the velocity fields flattened in: velocity
+    ``(C, d)``, boundary velocity ``(F, d)``, velocity gradient ``(C, d, d)`` (``[c, i, j] = du_i /
+    dx_j``), pressure ``(C,)``, boundary pressure ``(F,)`` (the owner value on walls and inlets),
+    pressure gradient ``(C, d)``, mass flux ``(F,)``; plus the momentum diagonal ``a_P`` ``(C,)``."""
+
+    velocity: torch.Tensor
+    boundary_velocity: torch.Tensor
+    grad_velocity: torch.Tensor
+    pressure: torch.Tensor
+    boundary_pressure: torch.Tensor
+    grad_pressure: torch.Tensor
+    mdot: torch.Tensor
+    a_p: torch.Tensor
 
 
 class MomentumContinuity:
@@ -353,7 +447,8 @@ class MomentumContinuity:
 
     def boundary_data(self):
         """Per-face prescribed velocity ``(d, F)`` and outlet pressure ``(F,)`` (differentiable)."""
-        return _boundary_data(self.boundary, self.dim, self.mesh.n_faces, self.device)
+        return _boundary_data(self.boundary, self.dim, self.mesh.n_faces, self.device,
+                              self.geometry.face.centroid.detach())
 
     def _inputs(self):
         mu, rho = self.material()
@@ -397,6 +492,32 @@ class MomentumContinuity:
         if gb is not None:
             prim.append(gb.detach())
         return self._raw.vjp(prim, cotangent.contiguous())[0]
+
+    def mass_flux(self, state):
+        """The Rhie-Chow face mass flux ``mdot`` ``(F,)`` that continuity sums and the advection
+        fluxes carry (``flow/momentum.py:884-899``); differentiable in forward mode."""
+        _lib.require_cuda(state, "state")
+        mu, rho, ub, pb, gb = self._inputs()
+        return _MassFluxFn.apply(self._raw, state.contiguous(), mu, rho, ub, pb, gb)
+
+    def flow_fields(self, state):
+        """The shared flow quantities at ``state`` (``flow/momentum.py:822-857``): boundary fields,
+        gradients, ``a_P`` and the mass flux, as a :class:`FlowFields` (values, no autograd)."""
+        _lib.require_cuda(state, "state")
+        d, C = self.dim, self.mesh.n_cells
+        with torch.no_grad():
+            G, aP, _ = self.fields(state)
+            vel, p = self.unpack(state)
+            ub, pb = self._inputs()[2:4]
+            own = self.mesh.device_mesh(self.device).owner
+            kinds = self._face_kinds()
+            bvel = torch.where((kinds == FLOW_OUTLET)[:, None], vel[own], ub.T)
+            bp = torch.where(kinds == FLOW_OUTLET, pb, torch.where(kinds == FLOW_NONE, torch.zeros_like(pb), p[own]))
+            Gr = G.reshape(d + 1, d, C)
+            return FlowFields(vel, bvel, Gr[:d].permute(2, 0, 1), p, bp, Gr[d].T, self.mass_flux(state).detach(), aP)
+
+    def _face_kinds(self):
+        return self._raw.kind.to(torch.int64)
 
     def fields(self, state):
         """Gradients ``((d+1) d, C)``, ``a_P`` ``(C,)`` and limiter ``psi`` for a state."""

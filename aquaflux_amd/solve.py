@@ -139,6 +139,8 @@ def _combine(x, basis, k, y):
     (``afx_basis_combine`` for FP64 ``x``)."""
     if x.dtype != torch.float64:
         return x + (basis[:k].T @ torch.tensor(y, dtype=basis.dtype, device=basis.device)).to(x.dtype)
+    if not x.is_cuda:  # host vectors (the Krylov algorithm run on the CPU): the same sum in torch FP64
+        return x + basis[:k].to(torch.float64).T @ torch.tensor(np.asarray(y, dtype=np.float64))
     yd = torch.tensor(np.asarray(y, dtype=np.float64), device=x.device)
     x = x.contiguous()
     prec = 0 if basis.dtype == torch.float32 else 1
@@ -303,6 +305,8 @@ class DampedNewtonStep:
 
         if self.column_reach is None:
             raise ValueError("the probed Jacobian needs column_reach (the stencil reach of each state field)")
+        if getattr(op, "mesh", None) is None:
+            raise ValueError("the probed Jacobian needs the operator's mesh (pass mesh= with a residual function)")
         per_mesh = self._plans.setdefault(op.mesh, {})
         key = tuple(self.column_reach)
         plan = per_mesh.get(key)
@@ -399,9 +403,76 @@ class NewtonReport:
     converged: bool = False
 
 
+@dataclasses.dataclass(frozen=True)
+class GMRES:
+    """Settings of a restarted GMRES solve, in the role of the reference's ``lineax.GMRES(rtol, atol,
+    restart, max_steps)`` linear-solver objects (``solve/linear.py:40-136``): passed as
+    ``ImplicitNewtonSolver(solver=..., adjoint_solver=...)``."""
+
+    rtol: float = 1e-10
+    atol: float = 0.0
+    restart: int = 50
+    max_steps: int = 2000
+
+
+class ResidualFunction:
+    """A residual callable ``residual_fn(phi, theta)`` as the operator the Newton drivers take: the
+    residual with no autograd, its tangent in ``phi`` by forward-mode AD and its adjoint in ``phi`` by
+    reverse-mode AD (through the HIP operators' own tangent and adjoint), and the residual with
+    ``theta`` live for the implicit-function gradient.  ``mesh`` (optional) enables block-Jacobian
+    preconditioners by coloured probing (``DampedNewtonStep(jacobian="probed", column_reach=...)``)."""
+
+    def __init__(self, residual_fn, theta, mesh=None):
+        self.fn = residual_fn
+        self.theta = theta
+        self.mesh = mesh
+        self._theta_detached = _tree_map(lambda t: t.detach() if isinstance(t, torch.Tensor) else t, theta)
+
+    def residual_raw(self, phi):
+        with torch.no_grad():
+            return self.fn(phi, self._theta_detached)
+
+    def residual(self, phi):
+        return self.fn(phi, self.theta)
+
+    def residual_jvp(self, phi, v):
+        import torch.autograd.forward_ad as fwAD
+
+        with torch.no_grad(), fwAD.dual_level():
+            out = self.fn(fwAD.make_dual(phi.detach(), v), self._theta_detached)
+            prim, tan = fwAD.unpack_dual(out)
+            return prim, (tan if tan is not None else torch.zeros_like(prim))
+
+    def residual_vjp(self, phi, w):
+        with torch.enable_grad():
+            x = phi.detach().requires_grad_(True)
+            (g,) = torch.autograd.grad(self.fn(x, self._theta_detached), (x,), w, allow_unused=True)
+        return g if g is not None else torch.zeros_like(phi)
+
+
+def _tree_map(f, tree):
+    if isinstance(tree, dict):
+        return {k: _tree_map(f, v) for k, v in tree.items()}
+    if isinstance(tree, (tuple, list)):
+        return type(tree)(_tree_map(f, v) for v in tree)
+    return f(tree)
+
+
+def _tree_leaves(tree):
+    if isinstance(tree, dict):
+        return [x for v in tree.values() for x in _tree_leaves(v)]
+    if isinstance(tree, (tuple, list)):
+        return [x for v in tree for x in _tree_leaves(v)]
+    return [tree]
+
+
 @dataclasses.dataclass
 class ImplicitNewtonSolver:
-    """Newton to ``||R|| <= atol + rtol ||R0||`` with an IFT adjoint (``solve/implicit.py:767-864``)."""
+    """Newton to ``||R|| <= atol + rtol ||R0||`` with an IFT adjoint (``solve/implicit.py:767-864``).
+
+    ``solver`` / ``adjoint_solver`` (:class:`GMRES` settings, or None) override the forward step's
+    Krylov tolerances and the adjoint solve's (``adjoint_rtol`` / ``adjoint_restart`` /
+    ``adjoint_max_iter``), as the reference's ``solver`` / ``adjoint_solver`` fields do."""
 
     rtol: float = 1e-10
     atol: float = 1e-12
@@ -411,6 +482,17 @@ class ImplicitNewtonSolver:
     adjoint_restart: int = 50
     adjoint_max_iter: int = 2000
     progress: object = None  # optional callable(step, ||R||, krylov_iterations, alpha, seconds)
+    solver: GMRES | None = None
+    adjoint_solver: GMRES | None = None
+
+    def __post_init__(self):
+        if self.solver is not None:
+            self.forward_step = dataclasses.replace(self.forward_step, rtol=self.solver.rtol, atol=self.solver.atol,
+                                                    restart=self.solver.restart, max_iter=self.solver.max_steps)
+        if self.adjoint_solver is not None:
+            self.adjoint_rtol = self.adjoint_solver.rtol
+            self.adjoint_restart = self.adjoint_solver.restart
+            self.adjoint_max_iter = self.adjoint_solver.max_steps
 
     def forward(self, op, phi0):
         """Runs Newton from ``phi0`` (no autograd); returns ``(phi, NewtonReport)``."""
@@ -452,11 +534,24 @@ class ImplicitNewtonSolver:
         rep.converged = True
         return phi, rep
 
-    def solve(self, op, phi0, params=()):
-        """Root ``phi*`` of ``op``; differentiable w.r.t. ``params`` (tensors ``op`` depends on)."""
+    def solve(self, residual_fn, phi0, theta=(), *, mesh=None):
+        """Root ``phi*`` of the residual, differentiable w.r.t. the tensors in ``theta`` by the
+        implicit-function theorem (``solve/implicit.py:813-864``).
+
+        ``residual_fn`` is either the reference's ``residual_fn(phi, theta)`` callable (any
+        composition of this package's residual operators and torch operations; ``theta`` a tensor or
+        a tuple / list / dict of them; ``mesh`` enables probed block-Jacobian preconditioners), or an
+        operator with ``residual_raw`` / ``residual_jvp`` / ``residual_vjp`` / ``residual`` (a
+        ``MomentumContinuity``, ``CoupledRANS``, ...), in which case ``theta`` lists the tensors that
+        operator depends on."""
+        if hasattr(residual_fn, "residual_raw"):
+            op, params = residual_fn, tuple(_tree_leaves(theta))
+        else:
+            op = ResidualFunction(residual_fn, theta, mesh)
+            params = tuple(_tree_leaves(theta))
+        params = tuple(p for p in params if isinstance(p, torch.Tensor))
         phi, rep = self.forward(op, phi0)
         self.last_report = rep
-        params = tuple(params)
         if not any(isinstance(p, torch.Tensor) and p.requires_grad for p in params):
             return phi
         return _ImplicitRoot.apply(self, op, phi, *params)

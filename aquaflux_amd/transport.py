@@ -19,7 +19,7 @@ import torch
 import torch.autograd.forward_ad as fwAD
 
 from . import _lib
-from .boundary import BC_DIRICHLET, BoundaryConditions, Convective, Dirichlet, Neumann, ZeroGradient
+from .boundary import BC_DIRICHLET, BoundaryConditions, Convective, Dirichlet, DirichletField, Neumann, ZeroGradient
 from .discretization import AdvectionScheme, LimitedUpwind, TransientTerm
 from .properties import FieldProperty
 
@@ -46,9 +46,9 @@ def face_mass_flux(geometry, velocity) -> torch.Tensor:
     return un * geometry.face.area
 
 
-def _boundary_arrays(bound: BoundaryConditions, mesh, device):
+def _boundary_arrays(bound: BoundaryConditions, mesh, device, face_centroid=None):
     """Per-face closure kind, value (Dirichlet value / Neumann q / convective h) and t_inf; tensor
-    values stay differentiable."""
+    values stay differentiable.  Position-dependent closures are evaluated at ``face_centroid``."""
     F = mesh.n_faces
     kind = np.zeros(F, dtype=np.int8)
     val = torch.zeros(F, dtype=F64, device=device)
@@ -56,18 +56,16 @@ def _boundary_arrays(bound: BoundaryConditions, mesh, device):
     for name, bc, faces in bound.items():
         if np.any(mesh.neighbour[faces] >= 0):
             raise ValueError(f"patch '{name}' names interior faces")
-        if not isinstance(bc, (Dirichlet, ZeroGradient, Neumann, Convective)):
+        if not isinstance(bc, (Dirichlet, DirichletField, ZeroGradient, Neumann, Convective)):
             raise ValueError(f"patch '{name}': {type(bc).__name__} is not a scalar boundary condition")
         kind[faces] = bc.kind
         idx = torch.from_numpy(np.asarray(faces, dtype=np.int64)).to(device)
-        v, t = bc.values()
-        for arr_name, x in (("val", v), ("tinf", t)):
-            xt = x if isinstance(x, torch.Tensor) else torch.tensor(float(x), dtype=F64)
-            xt = xt.to(device=device, dtype=F64).reshape(()).expand(idx.numel())
-            if arr_name == "val":
-                val = val.index_put((idx,), xt)
-            else:
-                tinf = tinf.index_put((idx,), xt)
+        if face_centroid is not None:
+            v, t = bc.face_values(face_centroid[idx])
+        else:
+            v, t = bc.face_values(torch.zeros((idx.numel(), mesh.dim), dtype=F64, device=device))
+        val = val.index_put((idx,), v)
+        tinf = tinf.index_put((idx,), t)
     if np.any((mesh.neighbour < 0) & (kind == 0)):
         raise ValueError("every boundary face needs a scalar closure")
     return torch.from_numpy(kind).to(device), val, tinf
@@ -218,7 +216,7 @@ class ScalarTransport:
             raise ValueError(f"unsupported transient term {type(transient).__name__}")
         dev = geometry.face.area.device
         bound = boundary.resolve(mesh.face_patches)
-        kind, val, tinf = _boundary_arrays(bound, mesh, dev)
+        kind, val, tinf = _boundary_arrays(bound, mesh, dev, geometry.face.centroid.detach())
         cfg = _lib.AfxScalarConfig()
         cfg.advection = 0 if advection_scheme is None else int(advection_scheme.code)
         cfg.gradient = 2 if gradient_scheme is None else int(gradient_scheme.code)

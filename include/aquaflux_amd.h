@@ -186,6 +186,14 @@ int afx_flow_vjp(const afx_mesh* mesh, const afx_geometry* geom, const afx_flow_
 int afx_flow_fields(const afx_mesh* mesh, const afx_geometry* geom, const afx_flow_config* cfg,
                     const afx_flow_inputs* in, double* gradients, double* a_p, double* psi, hipStream_t stream);
 
+/* Rhie-Chow face mass flux mdot[F] (caller face order; boundary faces by their patch closure) and,
+ * when `tangent` is not NULL, its tangent mdot_tangent[F].  Workspace as afx_flow_workspace_bytes
+ * mode 0 (mode 1 with a tangent).  Replaces MomentumContinuity.mass_flux / FlowFields.mdot
+ * (reference: aquaflux/flow/momentum.py:531-554, :884-899; aquaflux/flow/rhie_chow.py:274-353). */
+int afx_flow_mass_flux(const afx_mesh* mesh, const afx_geometry* geom, const afx_flow_config* cfg,
+                       const afx_flow_inputs* in, const afx_flow_inputs* tangent, double* mdot, double* mdot_tangent,
+                       void* work, int64_t work_bytes, hipStream_t stream);
+
 /* ------------------------------------------------------------------------------------------- */
 /* scalar fields                                                                                 */
 
@@ -197,6 +205,22 @@ int afx_flow_fields(const afx_mesh* mesh, const afx_geometry* geom, const afx_fl
 int afx_scalar_gradient(const afx_mesh* mesh, const afx_geometry* geom, int gradient, int sweeps, double relaxation,
                         const int8_t* kind, const double* value, const double* phi, double* grad,
                         hipStream_t stream);
+
+/* Reverse of afx_scalar_gradient (same scheme, closures and geometry): phi_bar[C] and, when not
+ * NULL, value_bar[F] (zero except on Dirichlet faces, caller order) from grad_bar[j][C].  The
+ * gradient is linear in (phi, value), so this is its transpose.  Replaces jax.vjp through
+ * GradientScheme.gradients (reference: aquaflux/schemes/gradient.py:98-153, :1122-1337). */
+int afx_scalar_gradient_vjp(const afx_mesh* mesh, const afx_geometry* geom, int gradient, int sweeps,
+                            double relaxation, const int8_t* kind, const double* grad_bar, double* phi_bar,
+                            double* value_bar, hipStream_t stream);
+
+/* Venkatakrishnan limiter psi[C] of phi[C] with its gradient grad[j][C] (eps^2 = V K^3, stencil
+ * extrema over the face neighbours, minimum over the cell's faces, clipped to [0, 1]); with phi_t /
+ * grad_t (either may be NULL = zero) also its tangent psi_t.  Replaces
+ * VenkatakrishnanLimiter.limit (reference: aquaflux/schemes/limiter.py:78-117). */
+int afx_venkat_limiter(const afx_mesh* mesh, const afx_geometry* geom, double limiter_k, const double* phi,
+                       const double* grad, const double* phi_t, const double* grad_t, double* psi, double* psi_t,
+                       hipStream_t stream);
 
 enum { AFX_GRAD_NONE = 2 };
 

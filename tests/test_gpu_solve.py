@@ -146,7 +146,7 @@ def test_ift_adjoint_matches_finite_difference(cuda):
                                 "right": NoSlipWall()}),
             advection_scheme=FirstOrderUpwind(), pressure_pin=0,
         )
-        phi = solver.solve(mc, mc.initial_state(), params=(mu,) if isinstance(mu, torch.Tensor) else ())
+        phi = solver.solve(mc, mc.initial_state(), (mu,) if isinstance(mu, torch.Tensor) else ())
         vel, _ = mc.unpack(phi)
         return vel[:, 0].abs().mean()
 
