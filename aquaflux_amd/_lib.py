@@ -221,6 +221,9 @@ _SIGNATURES = {
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxFlowConfig), ctypes.POINTER(AfxFlowInputs),
         c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p,
     ],
+    "afx_scalar_residual_noalloc": [c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxScalarConfig),
+                                    ctypes.POINTER(AfxScalarInputs), c_void_p, c_void_p],
+    "afx_scalar_noalloc_max_bytes": [c_int64_p],
     "afx_scalar_gradient_vjp": [c_void_p, ctypes.POINTER(AfxGeometry), ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "afx_venkat_limiter": [c_void_p, ctypes.POINTER(AfxGeometry), ctypes.c_double, c_void_p, c_void_p, c_void_p,

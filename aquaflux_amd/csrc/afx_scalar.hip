@@ -15,7 +15,10 @@
 // fluxes), L' (limiter; gradient cotangent premultiplied by the reconstruction operator), G'.
 // Face-array cotangents are written by the face's owner thread only; each cell keeps its own
 // centroid / volume cotangent, so every output element has exactly one writer per pass.
+#include <mutex>
+#include <string>
 #include <type_traits>
+#include <vector>
 
 #include "afx_common.h"
 #include "afx_mesh.h"
@@ -1304,6 +1307,64 @@ extern "C" int afx_scalar_residual(const afx_mesh* m, const afx_geometry* g, con
         m, g, a, Out<double>{residual, nullptr}, w, nullptr, nullptr, nullptr, cfg->gradient_sweeps,
         cfg->gradient_relaxation, s);
   });
+}
+
+// ---------------------------------------------------------------------------------------------
+// Allocation-free residual: the workspace lives in a file-scope device array, so a caller that may
+// not allocate (graph capture, a tight serving loop) passes only inputs and outputs.  Every call on
+// one GPU shares that GPU's copy; calls that overlap on one GPU race.  Nothing here needs zeroed
+// memory: every workspace array is written by the gradient pass before any kernel reads it.
+namespace {
+
+constexpr int64_t kNoAllocScratchBytes = 64ll << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;  // per device: the address of `scratch`, taken once
+
+// Device address of `scratch` on the current device (looked up on the first call on that device).
+int noalloc_scratch(void** out) {
+  int dev = 0;
+  AFX_CHECK_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    AFX_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    AFX_REQUIRE(ptr != nullptr, "afx_scalar_residual_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int afx_scalar_noalloc_max_bytes(int64_t* bytes) {
+  AFX_REQUIRE(bytes, "afx_scalar_noalloc_max_bytes: null argument");
+  *bytes = kNoAllocScratchBytes;
+  return 0;
+}
+
+extern "C" int afx_scalar_residual_noalloc(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cfg,
+                                           const afx_scalar_inputs* in, double* residual, hipStream_t s) {
+  AFX_REQUIRE(m && g, "afx_scalar_residual_noalloc: null argument");
+  int dev = 0;
+  AFX_CHECK_HIP(hipGetDevice(&dev));
+  AFX_REQUIRE(dev == m->device, "afx_scalar_residual_noalloc: the mesh handle's device is not the current device");
+  // with a permuted face order the residual kernels read the packed face geometry; gathering it
+  // would need the handle's own (allocated) pack buffer
+  AFX_REQUIRE(m->fu == nullptr || g->face_pack != nullptr,
+              "afx_scalar_residual_noalloc: this mesh needs geometry.face_pack (afx_geometry_pack)");
+  const int64_t need = sc_work_doubles(m->dim, m->C, 0) * 8;
+  if (need > kNoAllocScratchBytes) {
+    afx_set_error("afx_scalar_residual_noalloc: the mesh needs " + std::to_string(need) +
+                  " scratch bytes, more than the " + std::to_string(kNoAllocScratchBytes) + " available");
+    return 2;
+  }
+  void* work = nullptr;
+  if (int rc = noalloc_scratch(&work)) return rc;
+  return afx_scalar_residual(m, g, cfg, in, residual, work, kNoAllocScratchBytes, s);
 }
 
 extern "C" int afx_scalar_jvp(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cfg,

@@ -267,6 +267,17 @@ int afx_scalar_residual(const afx_mesh* mesh, const afx_geometry* geom, const af
                         const afx_scalar_inputs* in, double* residual, void* work, int64_t work_bytes,
                         hipStream_t stream);
 
+/* afx_scalar_residual without a workspace argument, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  The workspace is a file-scope device array of afx_scalar_noalloc_max_bytes() bytes
+ * (64 MiB: up to 762,600 cells in 3D, 1,048,576 in 2D), held on each GPU the library runs on until
+ * the process ends; a larger mesh is refused (status 2).  A mesh whose faces the library reorders
+ * needs geom->face_pack.  Calls on one GPU share the array: calls that overlap on one GPU race. */
+int afx_scalar_residual_noalloc(const afx_mesh* mesh, const afx_geometry* geom, const afx_scalar_config* cfg,
+                                const afx_scalar_inputs* in, double* residual, hipStream_t stream);
+/* Size in bytes of the workspace array of afx_scalar_residual_noalloc. */
+int afx_scalar_noalloc_max_bytes(int64_t* bytes);
+
 /* R and J . tangent (tangent pointers may be NULL = zero).  Replaces jax.jvp of the scalar residual. */
 int afx_scalar_jvp(const afx_mesh* mesh, const afx_geometry* geom, const afx_scalar_config* cfg,
                    const afx_scalar_inputs* in, const afx_scalar_inputs* tangent, double* residual,
