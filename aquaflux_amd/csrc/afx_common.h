@@ -288,6 +288,35 @@ AFX_DEV void load_face_geo(const MeshView& m, int32_t f, int32_t P, int32_t N, F
   for (int j = 0; j < D; ++j) q.skew[j] = q.xf[j] - (q.xP[j] + q.dvec[j] * q.g);
 }
 
+// As load_face_geo, with the owner / neighbour centroids given (e.g. from LDS).
+template <int D>
+AFX_DEV void load_face_geo_pts(const MeshView& m, int32_t f, bool interior, const double* xP, const double* xN,
+                               FaceGeo<D>& q) {
+  q.A = m.area[f];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    q.n[j] = m.nrm[(int64_t)j * m.F + f];
+    q.xf[j] = m.fc[(int64_t)j * m.F + f];
+    q.xP[j] = xP[j];
+    q.xN[j] = interior ? xN[j] : q.xP[j];
+    q.dvec[j] = q.xN[j] - q.xP[j];
+    q.dp[j] = q.xf[j] - q.xP[j];
+    q.dn[j] = q.xf[j] - q.xN[j];
+  }
+  q.dpn = vdot<D>(q.dp, q.n);
+  q.dnn = vdot<D>(q.dn, q.n);
+  if (interior) {
+    double dd = vdot<D>(q.dvec, q.dvec);
+    q.g = vdot<D>(q.dp, q.dvec) / dd;
+    q.dist = vdot<D>(q.dvec, q.n);
+  } else {
+    q.g = 0.0;
+    q.dist = q.dpn;
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) q.skew[j] = q.xf[j] - (q.xP[j] + q.dvec[j] * q.g);
+}
+
 // Decodes one entry of the cell -> face list.
 // `f` indexes the view's face arrays (geometry, per-face workspace); `fu` indexes the caller's
 // per-face arrays (boundary kinds and values, face fluxes and their cotangents).
@@ -395,6 +424,21 @@ AFX_DEV int32_t xcd_thread_index(const MeshView& m) {
   return (int32_t)(lb * blockDim.x + threadIdx.x);
 }
 inline dim3 grid_for(int64_t n, int block = kBlock) { return dim3((unsigned)((n + block - 1) / block)); }
+
+// LDS staging of a workgroup's own cells.  A cell kernel launched one 256-thread workgroup per
+// block of 256 consecutive cells (xcd_thread_index(m) == block_base(m) + threadIdx.x) stages the
+// data its cells' neighbours read into LDS; a face whose other cell lies in the same block reads it
+// there (tile_slot >= 0), the others from global memory.  Values are copied, not recomputed, so
+// results are bitwise those of the unstaged gather.
+AFX_DEV int32_t block_base(const MeshView& m) {
+  uint32_t lb = xcd_logical_block();
+  if (m.bsched != nullptr && gridDim.x == (uint32_t)m.nbs) lb = (uint32_t)m.bsched[lb];
+  return (int32_t)(lb * kBlock);
+}
+AFX_DEV int tile_slot(int32_t o, int32_t base) {
+  const uint32_t d = (uint32_t)(o - base);
+  return d < (uint32_t)kBlock ? (int)d : -1;
+}
 
 // Cells in XCD-contiguous order over a bounded set of resident workgroups: workgroup b on XCD
 // x = b % 8 walks that XCD's logical blocks t = b / 8, b / 8 + g_x, ... (g_x = workgroups on XCD

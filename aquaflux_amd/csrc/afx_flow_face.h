@@ -31,6 +31,70 @@ struct FlowCell {
   double V;
 };
 
+// LDS copy of the flow data of a workgroup's 256 cells (and their centroids).  FULL adds the
+// pressure, pressure gradient, a_P, limiters and volume to the velocity, velocity gradient,
+// viscosity and density that the momentum-diagonal pass reads.
+template <int D, class S, bool FULL>
+struct FlowTile {
+  static constexpr int NF = FULL ? 3 * D + D * D + 4 : D + D * D + 2;
+  S f[NF][256];
+  double cc[D][256];
+  double V[FULL ? 256 : 1];
+  AFX_DEV void put(int t, const FlowCell<D, S>& X, const double* xc) {
+    int i = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) f[i++][t] = X.u[a];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int b = 0; b < D; ++b) f[i++][t] = X.Gu[a][b];
+    f[i++][t] = X.mu;
+    f[i++][t] = X.rho;
+    if (FULL) {
+      f[i++][t] = X.p;
+#pragma unroll
+      for (int a = 0; a < D; ++a) f[i++][t] = X.Gp[a];
+      f[i++][t] = X.aP;
+#pragma unroll
+      for (int a = 0; a < D; ++a) f[i++][t] = X.psi[a];
+      V[t] = X.V;
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) cc[a][t] = xc[a];
+  }
+  AFX_DEV void get(int t, FlowCell<D, S>& O, double* xo) const {
+    int i = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) O.u[a] = f[i++][t];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int b = 0; b < D; ++b) O.Gu[a][b] = f[i++][t];
+    O.mu = f[i++][t];
+    O.rho = f[i++][t];
+    if (FULL) {
+      O.p = f[i++][t];
+#pragma unroll
+      for (int a = 0; a < D; ++a) O.Gp[a] = f[i++][t];
+      O.aP = f[i++][t];
+#pragma unroll
+      for (int a = 0; a < D; ++a) O.psi[a] = f[i++][t];
+      O.V = V[t];
+    } else {
+      O.p = sconst<S>(0.0);
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        O.Gp[a] = sconst<S>(0.0);
+        O.psi[a] = sconst<S>(1.0);
+      }
+      O.aP = sconst<S>(1.0);
+      O.V = 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) xo[a] = cc[a][t];
+  }
+};
+
 // Cotangent accumulators of one cell's flow data.
 template <int D>
 struct FlowCellBar {

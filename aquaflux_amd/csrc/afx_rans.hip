@@ -95,11 +95,22 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
                                                    int32_t* __restrict__ ext, int32_t* __restrict__ kmin_out,
                                                    ClosureOut co = ClosureOut{}) {
   constexpr int NV = D + 3, NL = D + 2;
-  const int32_t c = xcd_thread_index(m);
-  if (c >= m.C) return;
+  __shared__ S tx[NV][kBlock];
+  __shared__ double tcc[D][kBlock];
+  const int32_t base = block_base(m);
+  const int t = threadIdx.x;
+  const int32_t c = base + t;
+  const bool valid = c < m.C;
   S xc[NV];
+  double ccc[D];
+  if (valid) {
 #pragma unroll
-  for (int v = 0; v < NV; ++v) xc[v] = a.x[(int64_t)v * m.C + c];
+    for (int v = 0; v < NV; ++v) tx[v][t] = xc[v] = a.x[(int64_t)v * m.C + c];
+#pragma unroll
+    for (int j = 0; j < D; ++j) tcc[j][t] = ccc[j] = m.cc[(int64_t)j * m.C + c];
+  }
+  __syncthreads();
+  if (!valid) return;
   S acc[NV][D];
 #pragma unroll
   for (int v = 0; v < NV; ++v)
@@ -128,8 +139,19 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
       S xo[NV];
+      double cco[D];
+      const int ls = tile_slot(o, base);
+      if (ls >= 0) {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) xo[v] = a.x[(int64_t)v * m.C + o];
+        for (int v = 0; v < NV; ++v) xo[v] = tx[v][ls];
+#pragma unroll
+        for (int j = 0; j < D; ++j) cco[j] = tcc[j][ls];
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) xo[v] = a.x[(int64_t)v * m.C + o];
+#pragma unroll
+        for (int j = 0; j < D; ++j) cco[j] = m.cc[(int64_t)j * m.C + o];
+      }
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const int l = lim_slot<D>(v, LIMF, LIMS);
@@ -146,7 +168,7 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
       }
       if (GRAD == 0) {
         FaceGeo<D> q;
-        load_face_geo<D>(m, r.f, r.P, r.N, q);
+        load_face_geo_pts<D>(m, r.f, true, r.own ? ccc : cco, r.own ? cco : ccc, q);
         const double w = 1.0 - q.g;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -160,7 +182,7 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
       } else {
         double dx[D];
 #pragma unroll
-        for (int j = 0; j < D; ++j) dx[j] = m.cc[(int64_t)j * m.C + o] - m.cc[(int64_t)j * m.C + c];
+        for (int j = 0; j < D; ++j) dx[j] = cco[j] - ccc[j];
         const double wgt = 1.0 / vdot<D>(dx, dx);
 #pragma unroll
         for (int p = 0; p < D; ++p)
@@ -398,23 +420,77 @@ AFX_DEV void load_scalar(const RansArgs<S>& a, const MeshView& m, int32_t c, int
   Z.gam = a.nu[c] + sst_blend(a.f1[c], s1, s2) * a.nut[c];
 }
 
+// LDS copy of a workgroup's k and omega cells (value, gradient, limiter, diffusivity), centroids and,
+// with LAM, the k row's cotangent and the omega row's flux cotangent (zero on wall cells).
+template <int D, class S, bool LAM>
+struct ScalarTile {
+  S f[2 * (D + 3)][256];
+  double cc[D][256];
+  double lam[LAM ? 2 : 1][LAM ? 256 : 1];
+  AFX_DEV static void put1(S (*f)[256], int t, const ScalarCell<D, S>& Z) {
+    f[0][t] = Z.phi;
+#pragma unroll
+    for (int j = 0; j < D; ++j) f[1 + j][t] = Z.G[j];
+    f[D + 1][t] = Z.psi;
+    f[D + 2][t] = Z.gam;
+  }
+  AFX_DEV static void get1(const S (*f)[256], int t, ScalarCell<D, S>& Z) {
+    Z.phi = f[0][t];
+#pragma unroll
+    for (int j = 0; j < D; ++j) Z.G[j] = f[1 + j][t];
+    Z.psi = f[D + 1][t];
+    Z.gam = f[D + 2][t];
+  }
+  AFX_DEV void put(int t, const ScalarCell<D, S>& K, const ScalarCell<D, S>& W, const double* xc) {
+    put1(f, t, K);
+    put1(f + D + 3, t, W);
+#pragma unroll
+    for (int j = 0; j < D; ++j) cc[j][t] = xc[j];
+  }
+  AFX_DEV void get(int t, ScalarCell<D, S>& K, ScalarCell<D, S>& W, double* xo) const {
+    get1(f, t, K);
+    get1(f + D + 3, t, W);
+#pragma unroll
+    for (int j = 0; j < D; ++j) xo[j] = cc[j][t];
+  }
+};
+
 // ---------------------------------------------------------------------------------------------
 // pass D
 template <int D, class S, int ADV>
 __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<S> aP) {
-  for_each_cell(m, [&](int32_t c) {
+  __shared__ FlowTile<D, S, false> tile;
+  const int32_t base = block_base(m);
+  const int t = threadIdx.x;
+  const int32_t c = base + t;
+  const bool valid = c < m.C;
   FlowCell<D, S> X;
-  load_flow<D, S>(a, m, c, false, false, X);
+  double xc[D];
+  if (valid) {
+    load_flow<D, S>(a, m, c, false, false, X);
+#pragma unroll
+    for (int j = 0; j < D; ++j) xc[j] = m.cc[(int64_t)j * m.C + c];
+    tile.put(t, X, xc);
+  }
+  __syncthreads();
+  if (!valid) return;
   S acc = sconst<S>(0.0);
-  AFX_UNROLL(AFX_FACE_UNROLL)
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
     FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
-    load_face_geo<D>(m, r.f, r.P, r.N, q);
     if (o >= 0) {
       FlowCell<D, S> O;
-      load_flow<D, S>(a, m, o, false, false, O);
+      double xo[D];
+      const int lo = tile_slot(o, base);
+      if (lo >= 0) {
+        tile.get(lo, O, xo);
+      } else {
+        load_flow<D, S>(a, m, o, false, false, O);
+#pragma unroll
+        for (int j = 0; j < D; ++j) xo[j] = m.cc[(int64_t)j * m.C + o];
+      }
+      load_face_geo_pts<D>(m, r.f, true, r.own ? xc : xo, r.own ? xo : xc, q);
       S oc, nc;
       if (r.own)
         ap_interior<D, ADV, S>(q, X, O, oc, nc);
@@ -422,12 +498,12 @@ __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<
         ap_interior<D, ADV, S>(q, O, X, oc, nc);
       acc += r.own ? oc : nc;
     } else {
+      load_face_geo_pts<D>(m, r.f, false, xc, xc, q);
       const int kind = a.kind[r.fu];
       acc += ap_boundary<D, ADV, S>(q, kind, X, rans_gamma_b<S>(a, kind, r.fu, c, X.mu));
     }
   }
   aP.set(c, acc);
-  });
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -436,10 +512,24 @@ __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<
 template <int D, class S, int ADVF>
 __global__ void __launch_bounds__(256) k_rans_fres(MeshView m, RansArgs<S> a, Out<S> R, Out<S> mdot) {
   constexpr int NV = D + 1;
-  const int32_t c = xcd_thread_index(m);
-  if (c >= m.C) return;
+  // the FP64 pass stages its block's cells (53 KB of LDS in 3D); the dual-number (tangent) pass,
+  // whose tile would be twice that, gathers from global memory
+  constexpr bool kStage = std::is_same<S, double>::value;
+  __shared__ FlowTile<D, double, true> tile;
+  const int32_t base = block_base(m);
+  const int t = threadIdx.x;
+  const int32_t c = base + t;
+  const bool valid = c < m.C;
   FlowCell<D, S> X;
-  load_flow<D, S>(a, m, c, true, ADVF == kAdvLimited, X);
+  double xc[D];
+  if (valid) {
+    load_flow<D, S>(a, m, c, true, ADVF == kAdvLimited, X);
+#pragma unroll
+    for (int j = 0; j < D; ++j) xc[j] = m.cc[(int64_t)j * m.C + c];
+    if constexpr (kStage) tile.put(t, X, xc);
+  }
+  if constexpr (kStage) __syncthreads();
+  if (!valid) return;
   S acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = sconst<S>(0.0);
@@ -447,16 +537,26 @@ __global__ void __launch_bounds__(256) k_rans_fres(MeshView m, RansArgs<S> a, Ou
     FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
-    load_face_geo<D>(m, r.f, r.P, r.N, q);
     S out[NV];
     if (o >= 0) {
       FlowCell<D, S> O;
-      load_flow<D, S>(a, m, o, true, ADVF == kAdvLimited, O);
+      double xo[D];
+      int lo = -1;
+      if constexpr (kStage) lo = tile_slot(o, base);
+      if (lo >= 0) {
+        if constexpr (kStage) tile.get(lo, O, xo);
+      } else {
+        load_flow<D, S>(a, m, o, true, ADVF == kAdvLimited, O);
+#pragma unroll
+        for (int j = 0; j < D; ++j) xo[j] = m.cc[(int64_t)j * m.C + o];
+      }
+      load_face_geo_pts<D>(m, r.f, true, r.own ? xc : xo, r.own ? xo : xc, q);
       if (r.own)
         flow_flux_interior<D, ADVF, S>(q, X, O, out);
       else
         flow_flux_interior<D, ADVF, S>(q, O, X, out);
     } else {
+      load_face_geo_pts<D>(m, r.f, false, xc, xc, q);
       S ub[D];
 #pragma unroll
       for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.fu];
@@ -479,12 +579,9 @@ __global__ void __launch_bounds__(256) k_rans_fres(MeshView m, RansArgs<S> a, Ou
 template <int D, class S, int ADVS>
 AFX_DEV void rans_scalar_face(const RansArgs<S>& a, const MeshView& m, const FaceRef& r, int32_t o,
                               const FaceGeo<D>& q, S Qf, const ScalarCell<D, S>& Kc, const ScalarCell<D, S>& Wc,
-                              double dP, S& fk, S& fw) {
+                              const ScalarCell<D, S>& Ko, const ScalarCell<D, S>& Wo, double dP, S& fk, S& fw) {
   const SstConst& sc = a.sc;
   if (o >= 0) {
-    ScalarCell<D, S> Ko, Wo;
-    load_scalar<D, S>(a, m, o, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Ko);
-    load_scalar<D, S>(a, m, o, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wo);
     if (r.own) {
       fk = scalar_flux_interior<D, ADVS, S>(q, Qf, Kc.phi, Ko.phi, Kc.G, Ko.G, Kc.psi, Ko.psi, Kc.gam, Ko.gam);
       fw = scalar_flux_interior<D, ADVS, S>(q, Qf, Wc.phi, Wo.phi, Wc.G, Wo.G, Wc.psi, Wo.psi, Wc.gam, Wo.gam);
@@ -507,11 +604,22 @@ AFX_DEV void rans_scalar_face(const RansArgs<S>& a, const MeshView& m, const Fac
 template <int D, class S, int ADVS>
 __global__ void __launch_bounds__(256) k_rans_sres(MeshView m, RansArgs<S> a, In<S> mdot, Out<S> R) {
   const SstConst& sc = a.sc;
-  const int32_t c = xcd_thread_index(m);
-  if (c >= m.C) return;
+  __shared__ ScalarTile<D, S, false> tile;
+  const int32_t base = block_base(m);
+  const int t = threadIdx.x;
+  const int32_t c = base + t;
+  const bool valid = c < m.C;
   ScalarCell<D, S> Kc, Wc;
-  load_scalar<D, S>(a, m, c, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Kc);
-  load_scalar<D, S>(a, m, c, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wc);
+  double xc[D];
+  if (valid) {
+    load_scalar<D, S>(a, m, c, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Kc);
+    load_scalar<D, S>(a, m, c, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wc);
+#pragma unroll
+    for (int j = 0; j < D; ++j) xc[j] = m.cc[(int64_t)j * m.C + c];
+    tile.put(t, Kc, Wc, xc);
+  }
+  __syncthreads();
+  if (!valid) return;
   S acck = sconst<S>(0.0), accw = sconst<S>(0.0);
   const double irho = 1.0 / sc.density;
   const double dP = a.dwall[c];
@@ -519,10 +627,25 @@ __global__ void __launch_bounds__(256) k_rans_sres(MeshView m, RansArgs<S> a, In
     FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
-    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    ScalarCell<D, S> Ko, Wo;
+    if (o >= 0) {
+      double xo[D];
+      const int lo = tile_slot(o, base);
+      if (lo >= 0) {
+        tile.get(lo, Ko, Wo, xo);
+      } else {
+        load_scalar<D, S>(a, m, o, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Ko);
+        load_scalar<D, S>(a, m, o, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wo);
+#pragma unroll
+        for (int j = 0; j < D; ++j) xo[j] = m.cc[(int64_t)j * m.C + o];
+      }
+      load_face_geo_pts<D>(m, r.f, true, r.own ? xc : xo, r.own ? xo : xc, q);
+    } else {
+      load_face_geo_pts<D>(m, r.f, false, xc, xc, q);
+    }
     const S Qf = mdot[r.f] * irho;
     S fk, fw;
-    rans_scalar_face<D, S, ADVS>(a, m, r, o, q, Qf, Kc, Wc, dP, fk, fw);
+    rans_scalar_face<D, S, ADVS>(a, m, r, o, q, Qf, Kc, Wc, Ko, Wo, dP, fk, fw);
     acck = r.own ? acck + fk : acck - fk;
     accw = r.own ? accw + fw : accw - fw;
   }
@@ -533,8 +656,8 @@ __global__ void __launch_bounds__(256) k_rans_sres(MeshView m, RansArgs<S> a, In
   R.set((int64_t)(D + 1) * m.C + c, (acck - pk) - dk);
   S rw = ((accw - pw) - dw) - cw;
   if (wall) {
-    const S t = omega_wall_k<S>(sc, a.nu[c], dP, Kc.phi);
-    rw = a.log_w ? dlog(Wc.phi / t) : Wc.phi - t;
+    const S t2 = omega_wall_k<S>(sc, a.nu[c], dP, Kc.phi);
+    rw = a.log_w ? dlog(Wc.phi / t2) : Wc.phi - t2;
   }
   R.set((int64_t)(D + 2) * m.C + c, rw);
 }
@@ -621,33 +744,58 @@ __global__ void __launch_bounds__(256) k_rans_sres_adj(MeshView m, RansArgs<doub
                                                        const double* __restrict__ lam, RansBars bar,
                                                        double* __restrict__ mdot_bar, double* __restrict__ R) {
   const SstConst& sc = a.sc;
-  const int32_t c = xcd_thread_index(m);
-  if (c >= m.C) return;
+  __shared__ ScalarTile<D, double, true> tile;
+  const int32_t base = block_base(m);
+  const int t = threadIdx.x;
+  const int32_t c = base + t;
+  const bool valid = c < m.C;
   ScalarCell<D, double> Kc, Wc;
-  load_scalar<D, double>(a, m, c, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Kc);
-  load_scalar<D, double>(a, m, c, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wc);
+  double xc[D];
+  bool wall = false;
+  double lck = 0.0, lcw = 0.0;
+  if (valid) {
+    load_scalar<D, double>(a, m, c, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Kc);
+    load_scalar<D, double>(a, m, c, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wc);
+#pragma unroll
+    for (int j = 0; j < D; ++j) xc[j] = m.cc[(int64_t)j * m.C + c];
+    tile.put(t, Kc, Wc, xc);
+    wall = a.wallc[c] != 0;
+    lck = lam[(int64_t)(D + 1) * m.C + c];
+    lcw = wall ? 0.0 : lam[(int64_t)(D + 2) * m.C + c];  // flux / source part of the omega row
+    tile.lam[0][t] = lck;
+    tile.lam[1][t] = lcw;
+  }
+  __syncthreads();
+  if (!valid) return;
   ScalarBarExt<D> B;
   B.zero();
-  const bool wall = a.wallc[c] != 0;
   const double irho = 1.0 / sc.density;
   const double dP = a.dwall[c];
   const double V = m.vol[c];
-  const double lck = lam[(int64_t)(D + 1) * m.C + c];
-  const double lcw = wall ? 0.0 : lam[(int64_t)(D + 2) * m.C + c];  // flux / source part of the omega row
   double acck = 0.0, accw = 0.0;
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
     FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
-    load_face_geo<D>(m, r.f, r.P, r.N, q);
     const double Qf = mdot[r.f] * irho;
     double Qb;
     if (o >= 0) {
       ScalarCell<D, double> Ko, Wo;
-      load_scalar<D, double>(a, m, o, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Ko);
-      load_scalar<D, double>(a, m, o, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wo);
-      const double lk_o = lam[(int64_t)(D + 1) * m.C + o];
-      const double lw_o = a.wallc[o] ? 0.0 : lam[(int64_t)(D + 2) * m.C + o];
+      double xo[D], lk_o, lw_o;
+      const int ls = tile_slot(o, base);
+      if (ls >= 0) {
+        tile.get(ls, Ko, Wo, xo);
+        lk_o = tile.lam[0][ls];
+        lw_o = tile.lam[1][ls];
+      } else {
+        load_scalar<D, double>(a, m, o, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Ko);
+        load_scalar<D, double>(a, m, o, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wo);
+#pragma unroll
+        for (int j = 0; j < D; ++j) xo[j] = m.cc[(int64_t)j * m.C + o];
+        lk_o = lam[(int64_t)(D + 1) * m.C + o];
+        lw_o = a.wallc[o] ? 0.0 : lam[(int64_t)(D + 2) * m.C + o];
+      }
+      load_face_geo_pts<D>(m, r.f, true, r.own ? xc : xo, r.own ? xo : xc, q);
       const double lkf = r.own ? lck - lk_o : lk_o - lck;
       const double lwf = r.own ? lcw - lw_o : lw_o - lcw;
       if (r.own) {
@@ -674,6 +822,7 @@ __global__ void __launch_bounds__(256) k_rans_sres_adj(MeshView m, RansArgs<doub
         }
       }
     } else {
+      load_face_geo_pts<D>(m, r.f, false, xc, xc, q);
       // k face
       const double bk = scalar_bval<D, double>(a.kbc[r.fu], q, Kc.phi, Kc.G, a.kval[r.fu], 0.0, 0.0);
       double wgt = 0.0, gk = Kc.gam;
@@ -786,29 +935,49 @@ template <int D, int ADVF>
 __global__ void __launch_bounds__(256, 2) k_rans_fres_adj(MeshView m, RansArgs<double> a, const double* __restrict__ lam,
                                                        const double* __restrict__ mdot_bar, RansBars bar) {
   constexpr int NV = D + 1;
-  const int32_t c = xcd_thread_index(m);
-  if (c >= m.C) return;
+  __shared__ FlowTile<D, double, true> tile;
+  __shared__ double tlam[NV][kBlock];
+  const int32_t base = block_base(m);
+  const int t = threadIdx.x;
+  const int32_t c = base + t;
+  const bool valid = c < m.C;
   FlowCell<D, double> X;
-  load_flow<D, double>(a, m, c, true, ADVF == kAdvLimited, X);
+  double xc[D];
+  double lc[NV];
+  if (valid) {
+    load_flow<D, double>(a, m, c, true, ADVF == kAdvLimited, X);
+#pragma unroll
+    for (int j = 0; j < D; ++j) xc[j] = m.cc[(int64_t)j * m.C + c];
+    tile.put(t, X, xc);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) tlam[v][t] = lc[v] = lam[(int64_t)v * m.C + c];
+  }
+  __syncthreads();
+  if (!valid) return;
   FlowBarExt<D> B;
   B.fl.zero();
   B.mum = B.nutw = 0.0;
-  double lc[NV];
-#pragma unroll
-  for (int v = 0; v < NV; ++v) lc[v] = lam[(int64_t)v * m.C + c];
   const double lc_cont = c == a.pin ? 0.0 : lc[D];
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
     FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
-    load_face_geo<D>(m, r.f, r.P, r.N, q);
     double lf[NV];
     if (o >= 0) {
       FlowCell<D, double> O;
-      load_flow<D, double>(a, m, o, true, ADVF == kAdvLimited, O);
+      double xo[D];
+      const int ls = tile_slot(o, base);
+      if (ls >= 0) {
+        tile.get(ls, O, xo);
+      } else {
+        load_flow<D, double>(a, m, o, true, ADVF == kAdvLimited, O);
+#pragma unroll
+        for (int j = 0; j < D; ++j) xo[j] = m.cc[(int64_t)j * m.C + o];
+      }
+      load_face_geo_pts<D>(m, r.f, true, r.own ? xc : xo, r.own ? xo : xc, q);
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
-        double lo = lam[(int64_t)v * m.C + o];
+        double lo = ls >= 0 ? tlam[v][ls] : lam[(int64_t)v * m.C + o];
         if (v == D && o == a.pin) lo = 0.0;
         const double lcv = v == D ? lc_cont : lc[v];
         lf[v] = r.own ? lcv - lo : lo - lcv;
@@ -819,6 +988,7 @@ __global__ void __launch_bounds__(256, 2) k_rans_fres_adj(MeshView m, RansArgs<d
       else
         adj_flux_interior<D, ADVF>(q, O, X, lf, 1, B.fl);
     } else {
+      load_face_geo_pts<D>(m, r.f, false, xc, xc, q);
       const int kind = a.kind[r.fu];
       double ub[D];
 #pragma unroll
@@ -860,27 +1030,51 @@ __global__ void __launch_bounds__(256, 2) k_rans_fres_adj(MeshView m, RansArgs<d
 // the cap spills two): 152 -> 145 us on the 128x64x64 SST channel.
 template <int D, int ADV>
 __global__ void __launch_bounds__(256, 3) k_rans_ap_adj(MeshView m, RansArgs<double> a, RansBars bar) {
-  const int32_t c = xcd_thread_index(m);
-  if (c >= m.C) return;
+  __shared__ FlowTile<D, double, false> tile;
+  __shared__ double tapb[kBlock];
+  const int32_t base = block_base(m);
+  const int t = threadIdx.x;
+  const int32_t c = base + t;
+  const bool valid = c < m.C;
   FlowCell<D, double> X;
-  load_flow<D, double>(a, m, c, false, false, X);
+  double xc[D];
+  double apb_c = 0.0;
+  if (valid) {
+    load_flow<D, double>(a, m, c, false, false, X);
+#pragma unroll
+    for (int j = 0; j < D; ++j) xc[j] = m.cc[(int64_t)j * m.C + c];
+    tile.put(t, X, xc);
+    tapb[t] = apb_c = bar.aP[c];
+  }
+  __syncthreads();
+  if (!valid) return;
   RansCellBar<D> B;
   B.zero();
-  const double apb_c = bar.aP[c];
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
     FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
-    load_face_geo<D>(m, r.f, r.P, r.N, q);
     if (o >= 0) {
       FlowCell<D, double> O;
-      load_flow<D, double>(a, m, o, false, false, O);
-      const double apb_o = bar.aP[o];
+      double xo[D];
+      double apb_o;
+      const int ls = tile_slot(o, base);
+      if (ls >= 0) {
+        tile.get(ls, O, xo);
+        apb_o = tapb[ls];
+      } else {
+        load_flow<D, double>(a, m, o, false, false, O);
+#pragma unroll
+        for (int j = 0; j < D; ++j) xo[j] = m.cc[(int64_t)j * m.C + o];
+        apb_o = bar.aP[o];
+      }
+      load_face_geo_pts<D>(m, r.f, true, r.own ? xc : xo, r.own ? xo : xc, q);
       if (r.own)
         adj_ap_interior<D, ADV>(q, X, O, apb_c, apb_o, 0, B.fl);
       else
         adj_ap_interior<D, ADV>(q, O, X, apb_o, apb_c, 1, B.fl);
     } else {
+      load_face_geo_pts<D>(m, r.f, false, xc, xc, q);
       const int kind = a.kind[r.fu];
       const double gB = rans_gamma_b<double>(a, kind, r.fu, c, X.mu);
       double gbb;
@@ -1108,36 +1302,74 @@ template <int D, int GRAD, bool LIMF, bool LIMS>
 __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<double> a, const int32_t* __restrict__ ext,
                                                        RansBars bar) {
   constexpr int NA = D + 3, NL = D + 2;
-  const int32_t c = xcd_thread_index(m);
-  if (c >= m.C) return;
+  constexpr int NE = (LIMF ? D : 0) + (LIMS ? 2 : 0);  // limited variables
+  __shared__ double tB[NA * D][kBlock];
+  __shared__ double tcc[D][kBlock];
+  __shared__ int32_t tex[2 * (NE > 0 ? NE : 1)][kBlock];
+  const int32_t base = block_base(m);
+  const int t = threadIdx.x;
+  const int32_t c = base + t;
+  const bool valid = c < m.C;
+  double Bc[NA][D];
+  double ccc[D];
+  if (valid) {
+#pragma unroll
+    for (int v = 0; v < NA; ++v)
+#pragma unroll
+      for (int j = 0; j < D; ++j) tB[v * D + j][t] = Bc[v][j] = bar.G[(int64_t)(v * D + j) * m.C + c];
+#pragma unroll
+    for (int j = 0; j < D; ++j) tcc[j][t] = ccc[j] = m.cc[(int64_t)j * m.C + c];
+    int e = 0;
+#pragma unroll
+    for (int v = 0; v < NA; ++v) {
+      const int l = lim_slot<D>(v, LIMF, LIMS);
+      if (l < 0) continue;
+      tex[2 * e][t] = ext[(int64_t)l * m.C + c];
+      tex[2 * e + 1][t] = ext[(int64_t)(NL + l) * m.C + c];
+      ++e;
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
   double xb[NA];
 #pragma unroll
   for (int v = 0; v < NA; ++v) xb[v] = 0.0;
-  double Bc[NA][D];
-#pragma unroll
-  for (int v = 0; v < NA; ++v)
-#pragma unroll
-    for (int j = 0; j < D; ++j) Bc[v][j] = bar.G[(int64_t)(v * D + j) * m.C + c];
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
     FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
+      const int ls = tile_slot(o, base);
+      double Bo[NA][D], cco[D];
+      if (ls >= 0) {
+#pragma unroll
+        for (int v = 0; v < NA; ++v)
+#pragma unroll
+          for (int j = 0; j < D; ++j) Bo[v][j] = tB[v * D + j][ls];
+#pragma unroll
+        for (int j = 0; j < D; ++j) cco[j] = tcc[j][ls];
+      } else {
+#pragma unroll
+        for (int v = 0; v < NA; ++v)
+#pragma unroll
+          for (int j = 0; j < D; ++j) Bo[v][j] = bar.G[(int64_t)(v * D + j) * m.C + o];
+#pragma unroll
+        for (int j = 0; j < D; ++j) cco[j] = m.cc[(int64_t)j * m.C + o];
+      }
       if (GRAD == 0) {
         FaceGeo<D> q;
-        load_face_geo<D>(m, r.f, r.P, r.N, q);
+        load_face_geo_pts<D>(m, r.f, true, r.own ? ccc : cco, r.own ? cco : ccc, q);
         const double ws = r.own ? 1.0 - q.g : q.g;
 #pragma unroll
         for (int v = 0; v < NA; ++v) {
           double pb_ = 0.0;
 #pragma unroll
-          for (int j = 0; j < D; ++j)
-            pb_ += (q.n[j] * q.A) * (bar.G[(int64_t)(v * D + j) * m.C + r.P] - bar.G[(int64_t)(v * D + j) * m.C + r.N]);
+          for (int j = 0; j < D; ++j) pb_ += (q.n[j] * q.A) * (r.own ? Bc[v][j] - Bo[v][j] : Bo[v][j] - Bc[v][j]);
           xb[v] += ws * pb_;
         }
       } else {
         double dx[D];
 #pragma unroll
-        for (int j = 0; j < D; ++j) dx[j] = m.cc[(int64_t)j * m.C + o] - m.cc[(int64_t)j * m.C + c];
+        for (int j = 0; j < D; ++j) dx[j] = cco[j] - ccc[j];
         const double wgt = 1.0 / vdot<D>(dx, dx);
 #pragma unroll
         for (int v = 0; v < NA; ++v) {
@@ -1145,17 +1377,21 @@ __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<doub
 #pragma unroll
           for (int j = 0; j < D; ++j) {
             own_dot += dx[j] * Bc[v][j];
-            oth_dot += dx[j] * bar.G[(int64_t)(v * D + j) * m.C + o];
+            oth_dot += dx[j] * Bo[v][j];
           }
           xb[v] += -wgt * own_dot - wgt * oth_dot;
         }
       }
+      int e = 0;
 #pragma unroll
       for (int v = 0; v < NA; ++v) {
         const int l = lim_slot<D>(v, LIMF, LIMS);
         if (l < 0) continue;
-        if (ext[(int64_t)l * m.C + o] == c) xb[v] += bar.ext[(int64_t)l * m.C + o];
-        if (ext[(int64_t)(NL + l) * m.C + o] == c) xb[v] += bar.ext[(int64_t)(NL + l) * m.C + o];
+        const int32_t amx = ls >= 0 ? tex[2 * e][ls] : ext[(int64_t)l * m.C + o];
+        const int32_t amn = ls >= 0 ? tex[2 * e + 1][ls] : ext[(int64_t)(NL + l) * m.C + o];
+        ++e;
+        if (amx == c) xb[v] += bar.ext[(int64_t)l * m.C + o];
+        if (amn == c) xb[v] += bar.ext[(int64_t)(NL + l) * m.C + o];
       }
     } else {
       const int kind = a.kind[r.fu];
@@ -1452,11 +1688,7 @@ int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S
     AFX_CHECK_LAUNCH();
   }
   bind_closure<S>(a, w.cl, clt, C);
-#ifndef AFX_WG_PER_CU
-#define AFX_WG_PER_CU 0
-#endif
-  AFX_LAUNCH(kKRansAp, (k_rans_ap<D, S, ADVF>), cell_grid(C, AFX_WG_PER_CU), dim3(kBlock), 0, s, m, a,
-             Out<S>{w.aP, aPt});
+  AFX_LAUNCH(kKRansAp, (k_rans_ap<D, S, ADVF>), grid_for(C), dim3(kBlock), 0, s, m, a, Out<S>{w.aP, aPt});
   AFX_CHECK_LAUNCH();
   if (R.v == nullptr && !mdot_only) return 0;
   a.aP.v = w.aP;
