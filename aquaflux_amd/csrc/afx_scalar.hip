@@ -186,34 +186,85 @@ __global__ void __launch_bounds__(256) k_sc_grad(MeshView m, ScArgs<S> a, Out<S>
   }
 }
 
+// LDS copy of a workgroup's cells as the face passes read their neighbours: value, diffusivity,
+// limiter, gradient, centroid and (adjoint) the residual cotangent.
+template <int D, class S>
+struct ScTile {
+  S f[D + 3][kBlock];
+  double cc[D][kBlock];
+  double lam[kBlock];
+  AFX_DEV void put(int t, S x, S g, S p, const S* G, const double* xc) {
+    f[0][t] = x;
+    f[1][t] = g;
+    f[2][t] = p;
+#pragma unroll
+    for (int j = 0; j < D; ++j) f[3 + j][t] = G[j];
+#pragma unroll
+    for (int j = 0; j < D; ++j) cc[j][t] = xc[j];
+  }
+  AFX_DEV void get(int t, S& x, S& g, S& p, S* G, double* xo) const {
+    x = f[0][t];
+    g = f[1][t];
+    p = f[2][t];
+#pragma unroll
+    for (int j = 0; j < D; ++j) G[j] = f[3 + j][t];
+#pragma unroll
+    for (int j = 0; j < D; ++j) xo[j] = cc[j][t];
+  }
+};
+
 // ---------------------------------------------------------------------------------------------
 // pass R
 template <int D, class S, int ADV>
 __global__ void __launch_bounds__(256) k_sc_res(MeshView m, ScArgs<S> a, Out<S> R) {
-  const int32_t c = xcd_thread_index(m);
-  if (c >= m.C) return;
-  const S xc = a.phi[c], gc = a.gam[c];
-  const S pc = ADV == kAdvLimited ? a.psi[c] : sconst<S>(1.0);
+  __shared__ ScTile<D, S> tile;
+  const int32_t base = block_base(m);
+  const int t = threadIdx.x;
+  const int32_t c = base + t;
+  const bool valid = c < m.C;
+  S xc, gc, pc;
   S Gc[D];
-  load_grad_row<D, S>(a.G, m.C, c, Gc);
+  double ccc[D];
+  if (valid) {
+    xc = a.phi[c];
+    gc = a.gam[c];
+    pc = ADV == kAdvLimited ? a.psi[c] : sconst<S>(1.0);
+    load_grad_row<D, S>(a.G, m.C, c, Gc);
+#pragma unroll
+    for (int j = 0; j < D; ++j) ccc[j] = m.cc[(int64_t)j * m.C + c];
+    tile.put(t, xc, gc, pc, Gc, ccc);
+  }
+  __syncthreads();
+  if (!valid) return;
   S acc = sconst<S>(0.0);
   for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
     FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
-    load_face_geo<D>(m, r.f, r.P, r.N, q);
     const S Qf = a.flux[r.fu];
     S fl;
     if (o >= 0) {
-      const S xo = a.phi[o], go = a.gam[o];
-      const S po = ADV == kAdvLimited ? a.psi[o] : sconst<S>(1.0);
+      S xo, go, po;
       S Go[D];
-      load_grad_row<D, S>(a.G, m.C, o, Go);
+      double cco[D];
+      const int ls = tile_slot(o, base);
+      if (ls >= 0) {
+        tile.get(ls, xo, go, po, Go, cco);
+      } else {
+        xo = a.phi[o];
+        go = a.gam[o];
+        po = ADV == kAdvLimited ? a.psi[o] : sconst<S>(1.0);
+        load_grad_row<D, S>(a.G, m.C, o, Go);
+#pragma unroll
+        for (int j = 0; j < D; ++j) cco[j] = m.cc[(int64_t)j * m.C + o];
+      }
+      load_face_geo_pts<D>(m, r.f, true, r.own ? ccc : cco, r.own ? cco : ccc, q);
       if (r.own)
         fl = scalar_flux_interior<D, ADV, S>(q, Qf, xc, xo, Gc, Go, pc, po, gc, go);
       else
         fl = scalar_flux_interior<D, ADV, S>(q, Qf, xo, xc, Go, Gc, po, pc, go, gc);
     } else {
+      load_face_geo_pts<D>(m, r.f, false, ccc, ccc, q);
       const S bv = scalar_bval<D, S>(a.kind[r.fu], q, xc, Gc, a.val[r.fu], gc, a.tinf[r.fu]);
       fl = scalar_flux_boundary<D, ADV, S>(q, Qf, xc, Gc, pc, bv, gc);
     }
@@ -513,13 +564,25 @@ AFX_DEV void write_face_geo_bar(const ScBars& bar, int32_t F, int32_t f, const F
 template <int D, int ADV, bool GEO>
 __global__ void __launch_bounds__(256, 3) k_sc_res_adj(MeshView m, ScArgs<double> a, const double* __restrict__ lam,
                                                     ScBars bar) {
-  const int32_t c = xcd_thread_index(m);
-  if (c >= m.C) return;
-  const double xc = a.phi[c], gc = a.gam[c];
-  const double pc = ADV == kAdvLimited ? a.psi[c] : 1.0;
-  double Gc[D];
-  load_grad_row<D, double>(a.G, m.C, c, Gc);
-  const double lc = lam[c];
+  __shared__ ScTile<D, double> tile;
+  const int32_t base = block_base(m);
+  const int t = threadIdx.x;
+  const int32_t c = base + t;
+  const bool valid = c < m.C;
+  double xc = 0.0, gc = 0.0, pc = 0.0, lc = 0.0;
+  double Gc[D], ccc[D];
+  if (valid) {
+    xc = a.phi[c];
+    gc = a.gam[c];
+    pc = ADV == kAdvLimited ? a.psi[c] : 1.0;
+    load_grad_row<D, double>(a.G, m.C, c, Gc);
+#pragma unroll
+    for (int j = 0; j < D; ++j) ccc[j] = m.cc[(int64_t)j * m.C + c];
+    tile.put(t, xc, gc, pc, Gc, ccc);
+    tile.lam[t] = lc = lam[c];
+  }
+  __syncthreads();
+  if (!valid) return;
   ScalarBar<D> B;
   B.zero();
   double ccb[D];
@@ -529,16 +592,26 @@ __global__ void __launch_bounds__(256, 3) k_sc_res_adj(MeshView m, ScArgs<double
     FaceRef r = decode_face(m, k, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
-    load_face_geo<D>(m, r.f, r.P, r.N, q);
     FaceGeoBar<D> qb;
     qb.zero();
     const double Qf = a.flux[r.fu];
     if (o >= 0) {
-      const double xo = a.phi[o], go = a.gam[o];
-      const double po = ADV == kAdvLimited ? a.psi[o] : 1.0;
-      double Go[D];
-      load_grad_row<D, double>(a.G, m.C, o, Go);
-      const double lo = lam[o];
+      double xo, go, po, lo;
+      double Go[D], cco[D];
+      const int ls = tile_slot(o, base);
+      if (ls >= 0) {
+        tile.get(ls, xo, go, po, Go, cco);
+        lo = tile.lam[ls];
+      } else {
+        xo = a.phi[o];
+        go = a.gam[o];
+        po = ADV == kAdvLimited ? a.psi[o] : 1.0;
+        load_grad_row<D, double>(a.G, m.C, o, Go);
+#pragma unroll
+        for (int j = 0; j < D; ++j) cco[j] = m.cc[(int64_t)j * m.C + o];
+        lo = lam[o];
+      }
+      load_face_geo_pts<D>(m, r.f, true, r.own ? ccc : cco, r.own ? cco : ccc, q);
       const double lf = r.own ? lc - lo : lo - lc;
       double Qb;
       if (r.own)
@@ -555,6 +628,7 @@ __global__ void __launch_bounds__(256, 3) k_sc_res_adj(MeshView m, ScArgs<double
 #pragma unroll
       for (int j = 0; j < D; ++j) ccb[j] += (r.own ? qb.xP[j] : 0.0) + (r.own ? 0.0 : qb.xN[j]);
     } else {
+      load_face_geo_pts<D>(m, r.f, false, ccc, ccc, q);
       double vb = 0.0, tb = 0.0;
       const double Qb = adj_sc_boundary<D, ADV, GEO>(a.kind[r.fu], q, Qf, xc, Gc, pc, gc, a.val[r.fu], a.tinf[r.fu],
                                                      lc, B, vb, tb, qb);
