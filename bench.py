@@ -209,15 +209,37 @@ def build_workload(device, workload="sst", nx=128, ny=64, nz=64, nu=1e-5):
     return mesh, CoupledRANS.build(mc, turb), x, lam_t, state, lam
 
 
-def cpu_baseline(workload, nx, ny, nz, reps=2, nu=1e-5):
+def _cpu_threads():
+    """Threads for the CPU baseline: OMP_NUM_THREADS when set (the host's share of cores given to
+    this process), else every core the process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+
+def _time_cpu(fn, warmups, reps):
+    """Median wall time of ``reps`` calls of ``fn`` after ``warmups`` untimed ones."""
+    for _ in range(warmups):
+        fn()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), times
+
+
+def cpu_baseline(workload, nx, ny, nz, reps=10, warmups=3, nu=1e-5):
     """The CPU oracle (torch float64, reference pass structure) timed on a bounded sample: the same
-    channel workload on an nx x ny x nz mesh, residual + VJP (torch autograd) ``reps`` times."""
+    channel workload on an nx x ny x nz mesh, residual + VJP (torch autograd), median of ``reps``
+    runs after ``warmups`` untimed ones, input construction outside the timed calls."""
     import torch
 
     from oracle import fv
     from oracle import sst as S
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads = _cpu_threads()
     torch.set_num_threads(threads)
     mesh = _channel_mesh(nx, ny, nz)
     m = fv.OMesh.from_mesh(mesh)
@@ -262,20 +284,19 @@ def cpu_baseline(workload, nx, ny, nz, reps=2, nu=1e-5):
         def fn(z):
             return fv.flow_residual(m, g, s, z, mu, rho)
 
-    times = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        R = fn(x)
-        torch.autograd.grad(R, (x,), lam_t)
-        times.append(time.perf_counter() - t0)
-    tm = float(np.median(times))
+    def run():
+        torch.autograd.grad(fn(x), (x,), lam_t)
+
+    tm, times = _time_cpu(run, warmups, reps)
     return {
         "value": F / tm / 1e9,
         "unit": "Gfaces/s",
         "cores": threads,
+        "host_cpus": os.cpu_count(),
         "kind": "port",
-        "sample": f"{reps} residual+VJP evaluations by the torch-CPU float64 oracle of the same channel "
-        f"workload at {nx}x{ny}x{nz} ({C} cells, {F} faces), median {tm:.2f} s each",
+        "sample": f"residual+VJP by the torch-CPU float64 oracle of the same channel workload at {nx}x{ny}x{nz} "
+        f"({C} cells, {F} faces): median of {reps} runs after {warmups} warm-ups, {tm:.2f} s each "
+        f"(min {min(times):.2f}, max {max(times):.2f}), {threads} threads",
     }
 
 
@@ -355,15 +376,16 @@ def tet_workload(device, n=96, steps=10, warmup=2):
     }
 
 
-def tet_cpu_baseline(n=32, reps=5):
+def tet_cpu_baseline(n=32, reps=10, warmups=3):
     """The CPU oracle on a bounded sample of the tet workload: geometry, flux, residual and torch
-    autograd to node coordinates and phi on the 6 x n^3 mesh."""
+    autograd to node coordinates and phi on the 6 x n^3 mesh, median of ``reps`` runs after
+    ``warmups`` untimed ones."""
     import torch
 
     from aquaflux_amd.mesh import tet_grid_3d
     from oracle import fv
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads = _cpu_threads()
     torch.set_num_threads(threads)
     mesh = tet_grid_3d(n, perturb=0.15, seed=3)
     F, C = mesh.n_faces, mesh.n_cells
@@ -378,19 +400,20 @@ def tet_cpu_baseline(n=32, reps=5):
     s = fv.ScalarSetup(kind=kind, value=value, t_inf=torch.zeros(F, dtype=torch.float64), advection="limited",
                        gradient="green_gauss")
     gam = torch.full((C,), 0.01, dtype=torch.float64)
-    times = []
-    for _ in range(reps):
-        X = torch.tensor(mesh.node_coords, dtype=torch.float64, requires_grad=True)
-        t0 = time.perf_counter()
+    X = torch.tensor(mesh.node_coords, dtype=torch.float64, requires_grad=True)
+
+    def run():
         m = fv.OMesh.from_mesh(mesh, node_coords=X)
         g = fv.geometry(m)
         R = fv.scalar_residual(m, g, s, phi, g.normal[:, 0] * g.area, gam)
         torch.autograd.grad((R * lam).sum(), (X, phi))
-        times.append(time.perf_counter() - t0)
-    tm = float(np.median(times))
-    return {"value": F / tm / 1e9, "unit": "Gfaces/s", "cores": threads, "kind": "port",
-            "sample": f"{reps} evaluations by the torch-CPU float64 oracle of the same tet workload at 6 x {n}^3 "
-            f"({C} cells, {F} faces), median {tm:.2f} s each"}
+
+    tm, times = _time_cpu(run, warmups, reps)
+    return {"value": F / tm / 1e9, "unit": "Gfaces/s", "cores": threads, "host_cpus": os.cpu_count(),
+            "kind": "port",
+            "sample": f"geometry + flux + residual + adjoint by the torch-CPU float64 oracle of the same tet "
+            f"workload at 6 x {n}^3 ({C} cells, {F} faces): median of {reps} runs after {warmups} warm-ups, "
+            f"{tm:.2f} s each (min {min(times):.2f}, max {max(times):.2f}), {threads} threads"}
 
 
 def newton_compare(device, nx=200, ny=100, nz=100, mu=1e-3, max_steps=80, cfl0=10.0, preconditioner="block-simpler",
@@ -501,7 +524,7 @@ def main(argv=None):
     ap.add_argument("--tet-n", type=int, default=96)
     ap.add_argument("--size", type=int, nargs=3, default=(128, 64, 64), metavar=("NX", "NY", "NZ"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-reps", type=int, default=2)
+    ap.add_argument("--cpu-reps", type=int, default=10, help="timed CPU-baseline runs (after 3 warm-ups)")
     ap.add_argument("--cpu-size", type=int, nargs=3, default=(128, 64, 16), metavar=("NX", "NY", "NZ"))
     ap.add_argument("--no-newton", action="store_true", help="skip the Newton time-to-1e-10 comparison")
     ap.add_argument("--newton-only", action="store_true", help="only the Newton comparison")
@@ -564,7 +587,7 @@ def main(argv=None):
         _, tres = tet_workload(dev, args.tet_n, steps=args.steps, warmup=args.warmup)
         tres.update({"n_gpus": 1, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
                      "vs_baseline": None, "data": "synthetic (jittered mesh seed 3, phi ~ U(0,1), cotangent N(0,1))",
-                     "cpu_baseline": None if args.no_cpu_baseline else tet_cpu_baseline()})
+                     "cpu_baseline": None if args.no_cpu_baseline else tet_cpu_baseline(reps=args.cpu_reps)})
         print(json.dumps(tres), flush=True)
         return 0
     nx, ny, nz = args.size
@@ -665,6 +688,8 @@ def main(argv=None):
         _, tres = tet_workload(dev, args.tet_n, steps=10, warmup=2)
         out["tet_scalar"] = {k: tres[k] for k in ("value", "unit", "ms_per_step", "config", "kernel_avg_ms",
                                                   "roofline")}
+        if not args.no_cpu_baseline:
+            out["tet_scalar"]["cpu_baseline"] = tet_cpu_baseline(reps=args.cpu_reps)
         torch.cuda.empty_cache()
     if not args.no_newton:
         out.update(newton_lines())
