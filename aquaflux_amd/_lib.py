@@ -209,6 +209,13 @@ _SIGNATURES = {
     "afx_dense_matvec": [ctypes.c_int64, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "afx_basis_combine": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int, c_void_p, ctypes.c_int64, c_void_p, c_void_p,
                           c_void_p],
+    "afx_gmres_sizes": [ctypes.c_int64, ctypes.c_int32, c_void_p, c_void_p],
+    "afx_gmres_start": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int, ctypes.c_int, c_void_p, ctypes.c_double,
+                        ctypes.c_double, ctypes.c_int, c_void_p, ctypes.c_int64, c_void_p, c_void_p, c_void_p],
+    "afx_gmres_arnoldi": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p,
+                          ctypes.c_int64, ctypes.c_int, c_void_p, c_void_p, c_void_p],
+    "afx_gmres_update": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int, c_void_p, ctypes.c_int64, c_void_p, ctypes.c_int,
+                         c_void_p, c_void_p],
     "afx_profile_enable": [ctypes.c_int64],
     "afx_profile_collect": [c_void_p, c_void_p],
     "afx_profile_kernel_count": [],

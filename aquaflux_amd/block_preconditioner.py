@@ -567,7 +567,7 @@ class BlockPreconditioner:
             def velocity_solve(ru):
                 x, _ = gmres(lambda v: spmv(d, ns, Fv, jacobian.cols, v), ru.contiguous().reshape(-1),
                              rtol=self.velocity_rtol, restart=self.velocity_iters, max_iter=self.velocity_iters,
-                             precond=f_pbj)
+                             precond=f_pbj, check_every=None)
                 return x.view(C, d)
 
         def column(x):

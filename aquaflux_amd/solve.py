@@ -44,19 +44,137 @@ class KrylovInfo:
 # the basis is well conditioned, which is the usual case
 DGKS_ETA = 1.0 / math.sqrt(2.0)
 
+# status doubles of the device GMRES state (afx_gmres_*)
+_ST_K, _ST_DONE, _ST_BREAK, _ST_RES, _ST_HEAD = 0, 1, 2, 6, 8
+
 
 def gmres(matvec, b, *, x0=None, rtol=1e-10, atol=0.0, restart=30, max_iter=1000, precond=None, flexible=False,
-          reorthogonalize: str = "dgks", basis_dtype=None):
+          reorthogonalize: str = "dgks", basis_dtype=None, check_every=1):
     """Solve ``A x = b``; returns ``(x, KrylovInfo)``.  ``precond`` is applied on the right.
     ``reorthogonalize``: ``"dgks"`` (a second Gram-Schmidt pass when the first lost more than
     1 - 1/sqrt(2) of the vector's norm) or ``"always"`` (two passes every step).  ``basis_dtype``
     stores the Krylov (and flexible) basis and runs Gram-Schmidt in another precision (FP32 halves
     the orthogonalisation traffic of a solve that needs only a few digits).  Residuals and products
-    stay in ``b``'s precision, and so does the update: ``x + Z y`` widens each stored basis entry
-    and sums in FP64 (``afx_basis_combine``), so an FP32 basis rounds the search directions but
-    not the correction built from them."""
+    stay in ``b``'s precision; dot products, norms and the update ``x + Z y`` widen each stored basis
+    entry and sum in FP64.
+
+    For CUDA tensors the Arnoldi recurrence runs on the device (``afx_gmres_*``): Gram-Schmidt, the
+    Hessenberg column, the Givens rotations, the stopping test and the update are kernels, and the
+    host synchronises only to read the cycle's status.  ``check_every``: an int reads it after every
+    that many steps; ``None`` predicts from the residual estimates' recent rate how many steps
+    remain and reads it then (at most every 8 steps).  Steps enqueued after the device has stopped
+    the cycle do nothing, so the result does not depend on when the host looks; only the products
+    and preconditioner calls of those steps are wasted.  Host (CPU) tensors run the same algorithm
+    in torch, for tests of host-side logic."""
     if reorthogonalize not in ("dgks", "always"):
         raise ValueError(f"unknown reorthogonalize {reorthogonalize!r}")
+    if not b.is_cuda:
+        return _gmres_host(matvec, b, x0=x0, rtol=rtol, atol=atol, restart=restart, max_iter=max_iter,
+                           precond=precond, flexible=flexible, reorthogonalize=reorthogonalize,
+                           basis_dtype=basis_dtype)
+    if b.dtype not in (torch.float32, torch.float64):
+        raise ValueError(f"gmres: b must be float32 or float64, got {b.dtype}")
+    dtype, dev, shape = b.dtype, b.device, b.shape
+    b = b.contiguous().reshape(-1)
+    n = b.numel()
+    m = int(restart)
+    bdt = dtype if basis_dtype is None else basis_dtype
+    if bdt not in (torch.float32, torch.float64):
+        raise ValueError(f"gmres: basis_dtype must be float32 or float64, got {bdt}")
+    L = _lib.lib()
+    st_len, wk_len = ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(L.afx_gmres_sizes(n, m, ctypes.byref(st_len), ctypes.byref(wk_len)), "afx_gmres_sizes")
+    state = torch.zeros(st_len.value, dtype=torch.float64, device=dev)
+    work = torch.empty(wk_len.value, dtype=torch.float64, device=dev)
+    V = torch.empty((m + 1, n), dtype=bdt, device=dev)
+    Z = torch.empty((m, n), dtype=bdt, device=dev) if (flexible and precond is not None) else None
+    prec = 0 if bdt == torch.float32 else 1
+    vprec = 0 if dtype == torch.float32 else 1
+    reo = 1 if reorthogonalize == "always" else 0
+    stream = _lib.stream_of(dev)
+    x = torch.zeros_like(b) if x0 is None else x0.contiguous().reshape(-1).to(dtype).clone()
+    info = KrylovInfo()
+    if x0 is not None:  # the tolerance is relative to ||b||, not to the first residual
+        rtol_dev, atol_dev = 0.0, atol + rtol * float(torch.linalg.vector_norm(b))
+    else:
+        rtol_dev, atol_dev = rtol, atol
+    while True:
+        r = b if (x0 is None and info.restarts == 0) else b - matvec(x)
+        r = r.contiguous()
+        _lib.check(L.afx_gmres_start(n, m, prec, vprec, r.data_ptr(), rtol_dev, atol_dev, int(info.restarts == 0),
+                                     V.data_ptr(), V.stride(0), state.data_ptr(), work.data_ptr(), stream),
+                   "afx_gmres_start")
+        limit = min(m, max_iter - info.iterations)
+        if info.restarts > 0 or limit <= 0:
+            st = state[:_ST_HEAD].cpu().numpy()
+            info.residual_norm = float(st[_ST_RES])
+            if st[_ST_DONE]:
+                info.converged = True
+                return x.view(shape), info
+            if limit <= 0:
+                return x.view(shape), info
+        launched = 0
+        next_check = 1 if check_every is None else max(1, int(check_every))
+        st = None
+        while launched < limit:
+            j = launched
+            z = V[j] if precond is None else precond(V[j])
+            if Z is not None:
+                Z[j] = z
+                z = Z[j]  # the stored (possibly rounded) vector, so that A Z = V H holds as stored
+            w = matvec(z if z.dtype == dtype else z.to(dtype)).contiguous()
+            if w.dtype != dtype or w.numel() != n:
+                raise ValueError("gmres: matvec must return a vector of b's size and dtype")
+            _lib.check(L.afx_gmres_arnoldi(n, m, j, prec, vprec, w.data_ptr(), V.data_ptr(), V.stride(0), reo,
+                                           state.data_ptr(), work.data_ptr(), stream), "afx_gmres_arnoldi")
+            launched += 1
+            if launched >= next_check or launched == limit:
+                st = state[: _ST_HEAD + launched].cpu().numpy()
+                if st[_ST_DONE]:
+                    break
+                next_check = launched + _next_check_gap(st[_ST_HEAD:_ST_HEAD + launched], st[4], check_every)
+        if st is None or not st[_ST_DONE]:
+            st = state[: _ST_HEAD + launched].cpu().numpy()
+        k = int(st[_ST_K])
+        if Z is not None:
+            _lib.check(L.afx_gmres_update(n, m, prec, Z.data_ptr(), Z.stride(0), state.data_ptr(), vprec, x.data_ptr(),
+                                          stream), "afx_gmres_update")
+        elif precond is None:
+            _lib.check(L.afx_gmres_update(n, m, prec, V.data_ptr(), V.stride(0), state.data_ptr(), vprec, x.data_ptr(),
+                                          stream), "afx_gmres_update")
+        else:
+            upd = torch.zeros(n, dtype=bdt, device=dev)
+            _lib.check(L.afx_gmres_update(n, m, prec, V.data_ptr(), V.stride(0), state.data_ptr(), prec,
+                                          upd.data_ptr(), stream), "afx_gmres_update")
+            x = x + precond(upd).to(dtype)
+        info.iterations += k
+        info.restarts += 1
+        info.residual_norm = float(st[_ST_RES])
+        if k == 0:  # stopped at the start (||r|| <= tol): converged; or no step possible
+            info.converged = bool(st[_ST_DONE])
+            return x.view(shape), info
+
+
+def _next_check_gap(resid, tol, check_every):
+    """Steps until the host next reads the GMRES status: ``check_every`` when given, else the steps
+    the residual estimates' geometric rate over the last (up to) four steps needs to reach ``tol``,
+    between 1 and 8."""
+    if check_every is not None:
+        return max(1, int(check_every))
+    cap = 8
+    q = len(resid)
+    if q < 2 or not (resid[-1] > 0.0):
+        return 1 if q < 2 else cap
+    p = max(0, q - 5)
+    rate = (resid[-1] / resid[p]) ** (1.0 / (q - 1 - p)) if resid[p] > 0.0 else 1.0
+    if not (0.0 < rate < 1.0) or not (tol > 0.0):
+        return cap
+    need = math.log(tol / resid[-1]) / math.log(rate)
+    return int(min(cap, max(1.0, math.floor(need))))
+
+
+def _gmres_host(matvec, b, *, x0, rtol, atol, restart, max_iter, precond, flexible, reorthogonalize, basis_dtype):
+    """The same restarted GMRES on host tensors, in torch (tests of host-side logic)."""
     dtype, dev = b.dtype, b.device
     n = b.numel()
     bnorm = float(torch.linalg.vector_norm(b))
@@ -90,24 +208,23 @@ def gmres(matvec, b, *, x0=None, rtol=1e-10, atol=0.0, restart=30, max_iter=1000
             z = V[j] if precond is None else precond(V[j])
             if Z is not None:
                 Z[j] = z
-                z = Z[j]  # the stored (possibly rounded) vector, so that A Z = V H holds as stored
-            w = matvec(z.to(dtype)).to(bdt)
-            Vj = V[: j + 1]
-            w0 = torch.linalg.vector_norm(w)
+                z = Z[j]
+            w = matvec(z.to(dtype)).to(torch.float64)
+            Vj = V[: j + 1].to(torch.float64)
+            w = w.to(bdt).to(torch.float64)
+            w0 = float(torch.linalg.vector_norm(w))
             h = Vj @ w
-            w = w - Vj.T @ h
-            hn = torch.linalg.vector_norm(w)
-            hv = torch.cat([h, hn.reshape(1), w0.reshape(1)]).double().cpu().numpy()
-            if reorthogonalize == "always" or not hv[j + 1] > DGKS_ETA * hv[j + 2]:
+            w = (w - Vj.T @ h).to(bdt).to(torch.float64)
+            hn = float(torch.linalg.vector_norm(w))
+            if reorthogonalize == "always" or not hn > DGKS_ETA * w0:
                 h2 = Vj @ w
-                w = w - Vj.T @ h2
+                w = (w - Vj.T @ h2).to(bdt).to(torch.float64)
                 h = h + h2
-                hn = torch.linalg.vector_norm(w)
-                hv = torch.cat([h, hn.reshape(1)]).double().cpu().numpy()
-            hv = hv[: j + 2]
-            H[: j + 2, j] = hv
-            if hv[j + 1] > 0:
-                V[j + 1] = w / hn
+                hn = float(torch.linalg.vector_norm(w))
+            H[: j + 1, j] = h.numpy()
+            H[j + 1, j] = hn
+            if hn > 0:
+                V[j + 1] = (w / hn).to(bdt)
             for i in range(j):  # apply previous rotations
                 t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
                 H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
@@ -120,33 +237,19 @@ def gmres(matvec, b, *, x0=None, rtol=1e-10, atol=0.0, restart=30, max_iter=1000
             g[j] = cs[j] * g[j]
             k = j + 1
             info.iterations += 1
-            if abs(g[j + 1]) <= tol or info.iterations >= max_iter or hv[j + 1] == 0.0:
+            if abs(g[j + 1]) <= tol or info.iterations >= max_iter or hn == 0.0:
                 break
         y = np.linalg.solve(np.triu(H[:k, :k]) + np.diag(np.where(np.diag(H[:k, :k]) == 0, 1.0, 0.0)), g[:k])
+        yt = torch.tensor(y, dtype=torch.float64)
         if Z is not None:
-            x = _combine(x, Z, k, y)
+            x = x + (Z[:k].to(torch.float64).T @ yt).to(dtype)
         elif precond is None:
-            x = _combine(x, V, k, y)
+            x = x + (V[:k].to(torch.float64).T @ yt).to(dtype)
         else:
-            upd = _combine(torch.zeros_like(x), V, k, y)
-            x = x + precond(upd.to(bdt) if bdt != dtype else upd).to(dtype)
+            upd = (V[:k].to(torch.float64).T @ yt).to(bdt)
+            x = x + precond(upd).to(dtype)
         info.restarts += 1
         info.residual_norm = abs(g[k])
-
-
-def _combine(x, basis, k, y):
-    """``x + basis[:k]^T y`` accumulated in ``x``'s precision from a basis of either precision
-    (``afx_basis_combine`` for FP64 ``x``)."""
-    if x.dtype != torch.float64:
-        return x + (basis[:k].T @ torch.tensor(y, dtype=basis.dtype, device=basis.device)).to(x.dtype)
-    if not x.is_cuda:  # host vectors (the Krylov algorithm run on the CPU): the same sum in torch FP64
-        return x + basis[:k].to(torch.float64).T @ torch.tensor(np.asarray(y, dtype=np.float64))
-    yd = torch.tensor(np.asarray(y, dtype=np.float64), device=x.device)
-    x = x.contiguous()
-    prec = 0 if basis.dtype == torch.float32 else 1
-    _lib.check(_lib.lib().afx_basis_combine(x.numel(), int(k), prec, basis.data_ptr(), basis.stride(0), yd.data_ptr(),
-                                           x.data_ptr(), _lib.stream_of(x.device)), "afx_basis_combine")
-    return x
 
 
 class BlockJacobian:
@@ -345,7 +448,7 @@ class DampedNewtonStep:
                 def precond(v):
                     vc = J.to_cell_major(v)
                     z, _ = gmres(J.matvec, vc, rtol=self.inner_rtol, restart=self.inner_restart,
-                                 max_iter=self.inner_iters, precond=J.pbj)
+                                 max_iter=self.inner_iters, precond=J.pbj, check_every=None)
                     return J.from_cell_major(z)
             elif self.preconditioner == "block":
                 flexible = True
@@ -567,7 +670,7 @@ class ImplicitNewtonSolver:
         def precond(v):
             vc = J.to_cell_major(v)
             z, _ = gmres(J.rmatvec, vc, rtol=step.inner_rtol, restart=step.inner_restart,
-                         max_iter=step.inner_iters, precond=J.pbj_transpose)
+                         max_iter=step.inner_iters, precond=J.pbj_transpose, check_every=None)
             return J.from_cell_major(z)
 
         lam, info = gmres(lambda w: op.residual_vjp(phi, w), cotangent, rtol=self.adjoint_rtol,

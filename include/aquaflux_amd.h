@@ -468,6 +468,34 @@ int afx_dense_matvec(int64_t n, int precision, const void* A, const void* x, voi
 int afx_basis_combine(int64_t n, int32_t k, int precision, const void* basis, int64_t ld, const double* y, double* x,
                       hipStream_t stream);
 
+/* Restarted GMRES with the Arnoldi recurrence on the device (replaces the Arnoldi / Givens / update
+ * loop of lineax's GMRES in solve_linear, reference: aquaflux/solve/linear.py:199-298).  The host
+ * enqueues, per cycle: afx_gmres_start, then for j = 0, 1, ... the caller's product
+ * w = A M^-1 basis[j] followed by afx_gmres_arnoldi(j), then afx_gmres_update; it reads the status
+ * doubles of `state` whenever it chooses to synchronise.
+ *   state (FP64, afx_gmres_sizes' state_doubles): [0] k = steps accepted in this cycle, [1] done
+ *   (the residual estimate reached tol, or breakdown), [2] breakdown, [4] tol, [5] beta = ||r||,
+ *   [6] the current residual estimate |g_k|, [8 + j] the estimate after step j.  Once done is set,
+ *   further afx_gmres_arnoldi calls of the cycle do nothing.
+ *   work (FP64, work_doubles): partial sums.  basis: m + 1 vectors of `ld` entries, FP32
+ *   (precision 0) or FP64 (1); every dot product, norm and update is accumulated in FP64 in a fixed
+ *   order, so results are bitwise reproducible. */
+int afx_gmres_sizes(int64_t n, int32_t m, int64_t* state_doubles, int64_t* work_doubles);
+/* beta = ||r||, basis[0] = r / beta, g = beta e_1; when `first`, tol = atol + rtol beta (pass rtol = 0
+ * and atol = the tolerance when the initial guess is not zero).  done is set when beta <= tol. */
+int afx_gmres_start(int64_t n, int32_t m, int precision, int r_precision, const void* r, double rtol, double atol,
+                    int first, void* basis, int64_t ld, double* state, double* work, hipStream_t stream);
+/* Step j: orthogonalises w (FP32 or FP64) against basis[0..j] by classical Gram-Schmidt, a second
+ * pass when the first removed more than 1 - 1/sqrt(2) of its norm (reorthogonalize = 0) or always
+ * (1), stores basis[j + 1] = w / h_{j+1,j}, applies the Givens rotations to the new Hessenberg
+ * column and updates the residual estimate. */
+int afx_gmres_arnoldi(int64_t n, int32_t m, int32_t j, int precision, int w_precision, const void* w, void* basis,
+                      int64_t ld, int reorthogonalize, double* state, double* work, hipStream_t stream);
+/* y = R^-1 g over the k accepted steps (k read from state), then x += basis[:k]^T y (x FP32 or FP64,
+ * accumulated in FP64). */
+int afx_gmres_update(int64_t n, int32_t m, int precision, const void* basis, int64_t ld, double* state,
+                     int x_precision, void* x, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

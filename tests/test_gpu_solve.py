@@ -215,13 +215,16 @@ def test_fp32_basis_update_keeps_fp64_accuracy(cuda):
     """The GMRES update x + Z y from an FP32 basis widens each entry and sums in FP64
     (afx_basis_combine): equal to the FP64 product of the widened basis to 1e-15; and GMRES with an
     FP32 basis still reaches a 1e-13 residual through its FP64 restarts."""
-    from aquaflux_amd.solve import _combine
+    from aquaflux_amd import _lib
 
     g = torch.Generator(device="cpu").manual_seed(3)
     Z = torch.randn(12, 10001, generator=g, dtype=torch.float32).to(cuda)
     y = np.random.default_rng(4).normal(size=9)
     x0 = torch.randn(10001, generator=g, dtype=F64).to(cuda)
-    got = _combine(x0.clone(), Z, 9, y)
+    got = x0.clone()
+    yd = torch.tensor(y, dtype=F64, device=cuda)
+    _lib.check(_lib.lib().afx_basis_combine(got.numel(), 9, 0, Z.data_ptr(), Z.stride(0), yd.data_ptr(), got.data_ptr(),
+                                            _lib.stream_of(cuda)), "afx_basis_combine")
     want = x0 + Z[:9].double().T @ torch.tensor(y, dtype=F64, device=cuda)
     assert float((got - want).abs().max()) <= 1e-15 * float(want.abs().max())
     n = 4096
