@@ -86,8 +86,9 @@ def gmres(matvec, b, *, x0=None, rtol=1e-10, atol=0.0, restart=30, max_iter=1000
     _lib.check(L.afx_gmres_sizes(n, m, ctypes.byref(st_len), ctypes.byref(wk_len)), "afx_gmres_sizes")
     state = torch.zeros(st_len.value, dtype=torch.float64, device=dev)
     work = torch.empty(wk_len.value, dtype=torch.float64, device=dev)
-    V = torch.empty((m + 1, n), dtype=bdt, device=dev)
-    Z = torch.empty((m, n), dtype=bdt, device=dev) if (flexible and precond is not None) else None
+    ld = -(-n // 64) * 64  # rows 256-byte aligned, so that the sweeps load 16 bytes at a time
+    V = torch.empty((m + 1, ld), dtype=bdt, device=dev)[:, :n]
+    Z = torch.empty((m, ld), dtype=bdt, device=dev)[:, :n] if (flexible and precond is not None) else None
     prec = 0 if bdt == torch.float32 else 1
     vprec = 0 if dtype == torch.float32 else 1
     reo = 1 if reorthogonalize == "always" else 0

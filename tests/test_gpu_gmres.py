@@ -60,11 +60,14 @@ def test_gmres_is_bitwise_reproducible_and_independent_of_checks(cuda):
         assert i.residual_norm == i0.residual_norm
 
 
-def test_gmres_basis_is_orthonormal(cuda):
-    """Twelve Arnoldi steps through the C-ABI: V^T V = I to 1e-14 (FP64 basis) and 1e-6 (FP32), and
-    the residual estimate matches the true residual of the updated iterate."""
+@pytest.mark.parametrize("n,ld,m", [(5000, 5000, 12), (4097, 4160, 12), (3001, 3001, 12), (1500, 1536, 70)])
+def test_gmres_basis_is_orthonormal(cuda, n, ld, m):
+    """m Arnoldi steps through the C-ABI: V^T V = I to 1e-14 (FP64 basis) and 1e-6 (FP32), and the
+    residual estimate matches the true residual of the updated iterate.  Sizes cover 16-byte loads
+    with a ragged tail (4097), rows that are not 16-byte aligned (ld 3001: one entry per load) and a
+    basis longer than the 64 vectors the fused sweep holds in registers (m = 70)."""
     L = _lib.lib()
-    n, m = 5000, 12
+    print(f"case arnoldi n={n} ld={ld} m={m}", flush=True)
     A, b = _system(n, 3, cuda)
     st_len, wk_len = ctypes.c_int64(), ctypes.c_int64()
     _lib.check(L.afx_gmres_sizes(n, m, ctypes.byref(st_len), ctypes.byref(wk_len)), "afx_gmres_sizes")
@@ -72,18 +75,20 @@ def test_gmres_basis_is_orthonormal(cuda):
     for dt, prec, tol in ((F64, 1, 1e-14), (torch.float32, 0, 1e-6)):
         state = torch.zeros(st_len.value, dtype=F64, device=cuda)
         work = torch.empty(wk_len.value, dtype=F64, device=cuda)
-        V = torch.empty((m + 1, n), dtype=dt, device=cuda)
-        _lib.check(L.afx_gmres_start(n, m, prec, 1, b.data_ptr(), 0.0, 0.0, 1, V.data_ptr(), n, state.data_ptr(),
+        Vs = torch.zeros((m + 1, ld), dtype=dt, device=cuda)
+        V = Vs[:, :n]
+        _lib.check(L.afx_gmres_start(n, m, prec, 1, b.data_ptr(), 0.0, 0.0, 1, V.data_ptr(), ld, state.data_ptr(),
                                      work.data_ptr(), s), "afx_gmres_start")
         for j in range(m):
             w = (A @ V[j].double()).contiguous()
-            _lib.check(L.afx_gmres_arnoldi(n, m, j, prec, 1, w.data_ptr(), V.data_ptr(), n, 0, state.data_ptr(),
+            _lib.check(L.afx_gmres_arnoldi(n, m, j, prec, 1, w.data_ptr(), V.data_ptr(), ld, 0, state.data_ptr(),
                                            work.data_ptr(), s), "afx_gmres_arnoldi")
+        assert float(Vs[:, n:].abs().max()) == 0.0 if ld > n else True  # nothing written past the rows
         G = V.double() @ V.double().T
         assert float((G - torch.eye(m + 1, dtype=F64, device=cuda)).abs().max()) <= tol
         assert int(state[0]) == m and state[1].item() == 0.0
         x = torch.zeros(n, dtype=F64, device=cuda)
-        _lib.check(L.afx_gmres_update(n, m, prec, V.data_ptr(), n, state.data_ptr(), 1, x.data_ptr(), s),
+        _lib.check(L.afx_gmres_update(n, m, prec, V.data_ptr(), ld, state.data_ptr(), 1, x.data_ptr(), s),
                    "afx_gmres_update")
         true = float(torch.linalg.vector_norm(b - A @ x))
         est = float(state[6])
