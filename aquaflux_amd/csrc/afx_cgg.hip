@@ -21,8 +21,8 @@ __global__ void __launch_bounds__(256) k_cgg_sweep(MeshView m, int nv, const dou
     double corr[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) corr[j] = 0.0;
-    for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-      FaceRef r = decode_face(m, k, c);
+    for (FaceWalk walk(m, c); walk.more();) {
+      FaceRef r = walk.next(m, c);
       if (r.N < 0) continue;
       FaceGeo<D> q;
       load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -61,8 +61,8 @@ __global__ void __launch_bounds__(256) k_cgg_sweep_adj(MeshView m, int nv, const
     double ct[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) ct[j] = 0.0;
-    for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-      FaceRef r = decode_face(m, k, c);
+    for (FaceWalk walk(m, c); walk.more();) {
+      FaceRef r = walk.next(m, c);
       if (r.N < 0) continue;
       FaceGeo<D> q;
       load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -125,8 +125,8 @@ __global__ void __launch_bounds__(256) k_venkat(MeshView m, LimiterSlots sl, In<
     const S xc = x[(int64_t)v * m.C + c];
     S mx = xc, mn = xc;
     int32_t amx = c, amn = c;
-    for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-      FaceRef r = decode_face(m, k, c);
+    for (FaceWalk walk(m, c); walk.more();) {
+      FaceRef r = walk.next(m, c);
       const int32_t o = r.own ? r.N : r.P;
       if (o < 0) continue;
       const S xo = x[(int64_t)v * m.C + o];

@@ -340,6 +340,33 @@ AFX_DEV FaceRef decode_face(const MeshView& m, int32_t k, int32_t c) {
   return r;
 }
 
+// Walks cell c's face list with the next entry's (face, other cell) pair loaded one step ahead, so
+// that its latency overlaps the current face's work instead of heading the next face's chain of
+// dependent loads.  The prefetch is unconditional (the last entry is re-read at the end), so it
+// never sits behind a branch.
+struct FaceWalk {
+  int32_t k, kb, ke, e, o;
+  AFX_DEV FaceWalk(const MeshView& m, int32_t c) : k(m.cptr[c]), kb(k), ke(m.cptr[c + 1]) { fetch(m); }
+  AFX_DEV void fetch(const MeshView& m) {
+    const int32_t kk = k < ke ? k : ke - 1;
+    e = m.cface[kk];
+    o = m.cnbr[kk];
+  }
+  AFX_DEV bool more() const { return k < ke; }
+  // the current entry (its position in the cell's list is `k - kb` before the call); advances
+  AFX_DEV FaceRef next(const MeshView& m, int32_t c) {
+    FaceRef r;
+    r.own = e >= 0;
+    r.f = r.own ? e : ~e;
+    r.P = r.own ? c : o;
+    r.N = r.own ? o : c;
+    r.fu = m.fu ? m.fu[r.f] : r.f;
+    ++k;
+    fetch(m);
+    return r;
+  }
+};
+
 // Least-squares normal matrix of cell c (geometry only): M = sum_f w dx dx^T, w = 1/|dx|^2.
 template <int D>
 AFX_DEV void lsq_matrix(const MeshView& m, int32_t c, double M[D][D]) {
@@ -347,13 +374,15 @@ AFX_DEV void lsq_matrix(const MeshView& m, int32_t c, double M[D][D]) {
   for (int a = 0; a < D; ++a)
 #pragma unroll
     for (int b = 0; b < D; ++b) M[a][b] = 0.0;
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  double ccc[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) ccc[j] = m.cc[(int64_t)j * m.C + c];
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     int32_t o = r.own ? r.N : r.P;
     double dx[D];
 #pragma unroll
-    for (int j = 0; j < D; ++j)
-      dx[j] = (o >= 0 ? m.cc[(int64_t)j * m.C + o] : m.fc[(int64_t)j * m.F + r.f]) - m.cc[(int64_t)j * m.C + c];
+    for (int j = 0; j < D; ++j) dx[j] = (o >= 0 ? m.cc[(int64_t)j * m.C + o] : m.fc[(int64_t)j * m.F + r.f]) - ccc[j];
     double w = 1.0 / vdot<D>(dx, dx);
 #pragma unroll
     for (int a = 0; a < D; ++a)

@@ -110,8 +110,8 @@ __global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Ou
     }
   }
   const int32_t kb = m.cptr[c], ke = m.cptr[c + 1];
-  for (int32_t k = kb; k < ke; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
       S xo[NV];
@@ -261,8 +261,8 @@ __global__ void __launch_bounds__(256) k_flow_ap(MeshView m, FlowArgs<S> a, Out<
   X.rho = a.rho[c];
   X.V = m.vol[c];
   S acc = sconst<S>(0.0);
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -299,8 +299,8 @@ __global__ void __launch_bounds__(256) k_flow_res(MeshView m, FlowArgs<S> a, Out
   S acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = sconst<S>(0.0);
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -337,8 +337,8 @@ __global__ void __launch_bounds__(256) k_flow_mdot(MeshView m, FlowArgs<S> a, Ou
   if (c >= m.C) return;
   FlowCell<D, S> X;
   load_cell_full<D, S>(a, m, c, true, ADV == kAdvLimited, X);
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     if (!r.own) continue;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -388,8 +388,8 @@ __global__ void __launch_bounds__(256) k_flow_res_adj(MeshView m, FlowArgs<doubl
 #pragma unroll
   for (int v = 0; v < NV; ++v) lc[v] = lam[(int64_t)v * m.C + c];
   const double lc_cont = (c == a.pin) ? 0.0 : lc[D];
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -463,8 +463,8 @@ __global__ void __launch_bounds__(256) k_flow_ap_adj(MeshView m, FlowArgs<double
   FlowCellBar<D> B;
   B.zero();
   const double apb_c = bar.aP[c];
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
@@ -610,8 +610,8 @@ __global__ void __launch_bounds__(256) k_flow_grad_adj(MeshView m, FlowArgs<doub
   for (int v = 0; v < NV; ++v)
 #pragma unroll
     for (int j = 0; j < D; ++j) Bc[v][j] = bar.G[(int64_t)(v * D + j) * m.C + c];
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
       if (GRAD == 0) {

@@ -134,8 +134,8 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
     }
   }
   const int32_t kb = m.cptr[c], ke = m.cptr[c + 1];
-  for (int32_t k = kb; k < ke; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
       S xo[NV];
@@ -475,8 +475,8 @@ __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<
   __syncthreads();
   if (!valid) return;
   S acc = sconst<S>(0.0);
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     if (o >= 0) {
@@ -533,8 +533,8 @@ __global__ void __launch_bounds__(256) k_rans_fres(MeshView m, RansArgs<S> a, Ou
   S acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = sconst<S>(0.0);
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     S out[NV];
@@ -623,8 +623,8 @@ __global__ void __launch_bounds__(256) k_rans_sres(MeshView m, RansArgs<S> a, In
   S acck = sconst<S>(0.0), accw = sconst<S>(0.0);
   const double irho = 1.0 / sc.density;
   const double dP = a.dwall[c];
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     ScalarCell<D, S> Ko, Wo;
@@ -773,8 +773,8 @@ __global__ void __launch_bounds__(256) k_rans_sres_adj(MeshView m, RansArgs<doub
   const double dP = a.dwall[c];
   const double V = m.vol[c];
   double acck = 0.0, accw = 0.0;
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     const double Qf = mdot[r.f] * irho;
@@ -958,8 +958,8 @@ __global__ void __launch_bounds__(256, 2) k_rans_fres_adj(MeshView m, RansArgs<d
   B.fl.zero();
   B.mum = B.nutw = 0.0;
   const double lc_cont = c == a.pin ? 0.0 : lc[D];
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     double lf[NV];
@@ -1050,8 +1050,8 @@ __global__ void __launch_bounds__(256, 3) k_rans_ap_adj(MeshView m, RansArgs<dou
   if (!valid) return;
   RansCellBar<D> B;
   B.zero();
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     if (o >= 0) {
@@ -1202,15 +1202,19 @@ __global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<d
       }
     }
   }
-  // limiters (u_i if the momentum advection is limited, k and omega if theirs is)
-  {
+  // limiters (u_i if the momentum advection is limited, k and omega if theirs is).  The extremum
+  // cotangents are stored after all of them, so no store sits between one variable's chain of
+  // gathers and the next one's.
+  constexpr bool LFV = ADVF == kAdvLimited, LSV = ADVS == kAdvLimited;
+  if constexpr (LFV || LSV) {
     const double e2 = m.vol[c] * (a.lim_k * a.lim_k * a.lim_k);
     double xcen[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) xcen[j] = m.cc[(int64_t)j * m.C + c];
+    double mxbs[NL], mnbs[NL];
 #pragma unroll
     for (int v = 0; v < NA; ++v) {
-      const int l = lim_slot<D>(v, ADVF == kAdvLimited, ADVS == kAdvLimited);
+      const int l = lim_slot<D>(v, LFV, LSV);
       if (l < 0) continue;
       const double pbar = bar.psi[(int64_t)l * m.C + c];
       const int32_t amx = ext[(int64_t)l * m.C + c], amn = ext[(int64_t)(NL + l) * m.C + c];
@@ -1220,28 +1224,19 @@ __global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<d
 #pragma unroll
       for (int j = 0; j < D; ++j) Gv[j] = a.G[(int64_t)(v * D + j) * m.C + c];
       // the first face attaining the forward minimum, recorded by the forward limiter pass
-      const int32_t kmin = m.cptr[c] + kmin_pos[(int64_t)l * m.C + c];
-      double lo;
-      {
-        const int32_t e = m.cface[kmin];
-        const int32_t f = e >= 0 ? e : ~e;
-        double dmf = Gv[0] * (m.fc[f] - xcen[0]);
+      const int32_t e = m.cface[m.cptr[c] + kmin_pos[(int64_t)l * m.C + c]];
+      const int32_t f = e >= 0 ? e : ~e;
+      double dxf[D];
 #pragma unroll
-        for (int j = 1; j < D; ++j) dmf = dmf + Gv[j] * (m.fc[(int64_t)j * m.F + f] - xcen[j]);
-        lo = venkat_face<double>(dmf, mxv - xv, mnv - xv, e2);
-      }
+      for (int j = 0; j < D; ++j) dxf[j] = m.fc[(int64_t)j * m.F + f] - xcen[j];
+      double dmf = Gv[0] * dxf[0];
+#pragma unroll
+      for (int j = 1; j < D; ++j) dmf = dmf + Gv[j] * dxf[j];
+      const double lo = venkat_face<double>(dmf, mxv - xv, mnv - xv, e2);
       const double dclip = (lo > 0.0 && lo < 1.0) ? 1.0 : ((lo == 0.0 || lo == 1.0) ? 0.5 : 0.0);
       const double vbar = pbar * dclip;
       double mxb = 0.0, mnb = 0.0;
       if (vbar != 0.0) {
-        const int32_t e = m.cface[kmin];
-        const int32_t f = e >= 0 ? e : ~e;
-        double dxf[D];
-#pragma unroll
-        for (int j = 0; j < D; ++j) dxf[j] = m.fc[(int64_t)j * m.F + f] - xcen[j];
-        double dmf = Gv[0] * dxf[0];
-#pragma unroll
-        for (int j = 1; j < D; ++j) dmf = dmf + Gv[j] * dxf[j];
         dual r_dm = venkat_face<dual>(dual{dmf, 1.0}, dual{mxv - xv, 0.0}, dual{mnv - xv, 0.0}, e2);
         dual r_hx = venkat_face<dual>(dual{dmf, 0.0}, dual{mxv - xv, 1.0}, dual{mnv - xv, 0.0}, e2);
         dual r_hn = venkat_face<dual>(dual{dmf, 0.0}, dual{mxv - xv, 0.0}, dual{mnv - xv, 1.0}, e2);
@@ -1259,8 +1254,15 @@ __global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<d
           mnb = 0.0;
         }
       }
-      bar.ext[(int64_t)l * m.C + c] = mxb;
-      bar.ext[(int64_t)(NL + l) * m.C + c] = mnb;
+      mxbs[l] = mxb;
+      mnbs[l] = mnb;
+    }
+#pragma unroll
+    for (int v = 0; v < NA; ++v) {
+      const int l = lim_slot<D>(v, LFV, LSV);
+      if (l < 0) continue;
+      bar.ext[(int64_t)l * m.C + c] = mxbs[l];
+      bar.ext[(int64_t)(NL + l) * m.C + c] = mnbs[l];
     }
   }
 #pragma unroll
@@ -1306,6 +1308,7 @@ __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<doub
   __shared__ double tB[NA * D][kBlock];
   __shared__ double tcc[D][kBlock];
   __shared__ int32_t tex[2 * (NE > 0 ? NE : 1)][kBlock];
+  __shared__ double tbe[2 * (NE > 0 ? NE : 1)][kBlock];  // the limiter extremum cotangents
   const int32_t base = block_base(m);
   const int t = threadIdx.x;
   const int32_t c = base + t;
@@ -1326,6 +1329,8 @@ __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<doub
       if (l < 0) continue;
       tex[2 * e][t] = ext[(int64_t)l * m.C + c];
       tex[2 * e + 1][t] = ext[(int64_t)(NL + l) * m.C + c];
+      tbe[2 * e][t] = bar.ext[(int64_t)l * m.C + c];
+      tbe[2 * e + 1][t] = bar.ext[(int64_t)(NL + l) * m.C + c];
       ++e;
     }
   }
@@ -1334,12 +1339,16 @@ __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<doub
   double xb[NA];
 #pragma unroll
   for (int v = 0; v < NA; ++v) xb[v] = 0.0;
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
       const int ls = tile_slot(o, base);
+      // the neighbour's gradient cotangent, centroid and limiter extrema (which cells attain them and
+      // their cotangents), all fetched together: no load waits on a comparison
       double Bo[NA][D], cco[D];
+      int32_t oam[2 * (NE > 0 ? NE : 1)];
+      double obe[2 * (NE > 0 ? NE : 1)];
       if (ls >= 0) {
 #pragma unroll
         for (int v = 0; v < NA; ++v)
@@ -1347,6 +1356,11 @@ __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<doub
           for (int j = 0; j < D; ++j) Bo[v][j] = tB[v * D + j][ls];
 #pragma unroll
         for (int j = 0; j < D; ++j) cco[j] = tcc[j][ls];
+#pragma unroll
+        for (int e = 0; e < 2 * NE; ++e) {
+          oam[e] = tex[e][ls];
+          obe[e] = tbe[e][ls];
+        }
       } else {
 #pragma unroll
         for (int v = 0; v < NA; ++v)
@@ -1354,6 +1368,17 @@ __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<doub
           for (int j = 0; j < D; ++j) Bo[v][j] = bar.G[(int64_t)(v * D + j) * m.C + o];
 #pragma unroll
         for (int j = 0; j < D; ++j) cco[j] = m.cc[(int64_t)j * m.C + o];
+        int e = 0;
+#pragma unroll
+        for (int v = 0; v < NA; ++v) {
+          const int l = lim_slot<D>(v, LIMF, LIMS);
+          if (l < 0) continue;
+          oam[2 * e] = ext[(int64_t)l * m.C + o];
+          oam[2 * e + 1] = ext[(int64_t)(NL + l) * m.C + o];
+          obe[2 * e] = bar.ext[(int64_t)l * m.C + o];
+          obe[2 * e + 1] = bar.ext[(int64_t)(NL + l) * m.C + o];
+          ++e;
+        }
       }
       if (GRAD == 0) {
         FaceGeo<D> q;
@@ -1387,11 +1412,9 @@ __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<doub
       for (int v = 0; v < NA; ++v) {
         const int l = lim_slot<D>(v, LIMF, LIMS);
         if (l < 0) continue;
-        const int32_t amx = ls >= 0 ? tex[2 * e][ls] : ext[(int64_t)l * m.C + o];
-        const int32_t amn = ls >= 0 ? tex[2 * e + 1][ls] : ext[(int64_t)(NL + l) * m.C + o];
+        if (oam[2 * e] == c) xb[v] += obe[2 * e];
+        if (oam[2 * e + 1] == c) xb[v] += obe[2 * e + 1];
         ++e;
-        if (amx == c) xb[v] += bar.ext[(int64_t)l * m.C + o];
-        if (amn == c) xb[v] += bar.ext[(int64_t)(NL + l) * m.C + o];
       }
     } else {
       const int kind = a.kind[r.fu];

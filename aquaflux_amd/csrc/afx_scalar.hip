@@ -80,8 +80,8 @@ __global__ void __launch_bounds__(256) k_sc_grad(MeshView m, ScArgs<S> a, Out<S>
   S zero[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) zero[j] = sconst<S>(0.0);
-  for (int32_t k = kb; k < ke; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
       const S xo = a.phi[o];
@@ -237,8 +237,8 @@ __global__ void __launch_bounds__(256) k_sc_res(MeshView m, ScArgs<S> a, Out<S> 
   __syncthreads();
   if (!valid) return;
   S acc = sconst<S>(0.0);
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     const S Qf = a.flux[r.fu];
@@ -588,8 +588,8 @@ __global__ void __launch_bounds__(256, 3) k_sc_res_adj(MeshView m, ScArgs<double
   double ccb[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) ccb[j] = 0.0;
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeo<D> q;
     FaceGeoBar<D> qb;
@@ -824,8 +824,8 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
   double zero[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) zero[j] = 0.0;
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     FaceGeoBar<D> qb;
     qb.zero();
@@ -998,8 +998,8 @@ __global__ void __launch_bounds__(256) k_scalar_grad(MeshView m, const int8_t* _
   for (int p = 0; p < D; ++p)
 #pragma unroll
     for (int q = 0; q < D; ++q) M[p][q] = 0.0;
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
       const double xo = phi[o];
@@ -1083,8 +1083,8 @@ __global__ void __launch_bounds__(256) k_scalar_grad_vjp(MeshView m, const int8_
 #pragma unroll
   for (int j = 0; j < D; ++j) yc[j] = bbar[(int64_t)j * m.C + c] * ivc;
   double acc = 0.0;
-  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-    FaceRef r = decode_face(m, k, c);
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
       const double ivo = COMPACT ? 1.0 / m.vol[o] : 1.0;
