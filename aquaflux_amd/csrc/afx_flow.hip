@@ -644,11 +644,20 @@ __global__ void __launch_bounds__(256) k_flow_grad_adj(MeshView m, FlowArgs<doub
           xb[v] += -wgt * own_dot - wgt * oth_dot;
         }
       }
-      if (LIM) {
+      if (LIM) {  // all extremum cells and cotangents loaded before any comparison
+        int32_t amx[D], amn[D];
+        double bmx[D], bmn[D];
 #pragma unroll
         for (int i = 0; i < D; ++i) {
-          if (ext[(int64_t)i * m.C + o] == c) xb[i] += bar.ext[(int64_t)i * m.C + o];
-          if (ext[(int64_t)(D + i) * m.C + o] == c) xb[i] += bar.ext[(int64_t)(D + i) * m.C + o];
+          amx[i] = ext[(int64_t)i * m.C + o];
+          amn[i] = ext[(int64_t)(D + i) * m.C + o];
+          bmx[i] = bar.ext[(int64_t)i * m.C + o];
+          bmn[i] = bar.ext[(int64_t)(D + i) * m.C + o];
+        }
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+          if (amx[i] == c) xb[i] += bmx[i];
+          if (amn[i] == c) xb[i] += bmn[i];
         }
       }
     } else {

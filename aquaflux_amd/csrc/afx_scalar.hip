@@ -831,21 +831,33 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
     qb.zero();
     bool have_geo = false;
     if (o >= 0) {
+      // the other cell's limiter extrema (which cells attain them, and their cotangents), fetched
+      // with the rest of its data: no load waits on a comparison
+      int32_t oamx = 0, oamn = 0;
+      double obmx = 0.0, obmn = 0.0;
+      if (LIM) {
+        oamx = ext[o];
+        oamn = ext[(int64_t)m.C + o];
+        obmx = bar.ext[o];
+        obmn = bar.ext[(int64_t)m.C + o];
+      }
       if (GRAD == 0) {
         FaceGeo<D> q;
         load_face_geo<D>(m, r.f, r.P, r.N, q);
-        double BP[D], BN[D];
+        double Bo[D], BP[D], BN[D];
 #pragma unroll
         for (int j = 0; j < D; ++j) {
-          BP[j] = bar.G[(int64_t)j * m.C + r.P];
-          BN[j] = bar.G[(int64_t)j * m.C + r.N];
+          Bo[j] = bar.G[(int64_t)j * m.C + o];
+          BP[j] = r.own ? Bc[j] : Bo[j];
+          BN[j] = r.own ? Bo[j] : Bc[j];
         }
         double pb_ = 0.0;
 #pragma unroll
         for (int j = 0; j < D; ++j) pb_ += (q.n[j] * q.A) * (BP[j] - BN[j]);
         xb += (r.own ? 1.0 - q.g : q.g) * pb_;
         if (GEO) {
-          const double phP = a.phi[r.P], phN = a.phi[r.N];
+          const double po = a.phi[o];
+          const double phP = r.own ? xc : po, phN = r.own ? po : xc;
           const double phf = (1.0 - q.g) * phP + q.g * phN;
           double dn = 0.0;
 #pragma unroll
@@ -896,8 +908,8 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
         }
       }
       if (LIM) {
-        if (ext[o] == c) xb += bar.ext[o];
-        if (ext[(int64_t)m.C + o] == c) xb += bar.ext[(int64_t)m.C + o];
+        if (oamx == c) xb += obmx;
+        if (oamn == c) xb += obmn;
       }
     } else if (GRAD != kGradNone) {
       FaceGeo<D> q;
