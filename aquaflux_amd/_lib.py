@@ -209,6 +209,9 @@ _SIGNATURES = {
     "afx_dense_matvec": [ctypes.c_int64, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "afx_basis_combine": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int, c_void_p, ctypes.c_int64, c_void_p, c_void_p,
                           c_void_p],
+    "afx_face_flux": [ctypes.c_int64, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "afx_face_flux_vjp": [ctypes.c_int64, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                          c_void_p, c_void_p, c_void_p],
     "afx_gmres_sizes": [ctypes.c_int64, ctypes.c_int32, c_void_p, c_void_p],
     "afx_gmres_start": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int, ctypes.c_int, c_void_p, ctypes.c_double,
                         ctypes.c_double, ctypes.c_int, c_void_p, ctypes.c_int64, c_void_p, c_void_p, c_void_p],

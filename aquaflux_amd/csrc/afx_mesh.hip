@@ -15,6 +15,7 @@
 
 #include <vector>
 #include <algorithm>
+#include <type_traits>
 
 static thread_local std::string g_err;
 void afx_set_error(const std::string& msg) { g_err = msg; }
@@ -965,4 +966,87 @@ extern "C" int afx_geometry_vjp(const afx_mesh* m, const double* node_coords, co
     AFX_CHECK_LAUNCH();
   }
   return 0;
+}
+
+// ---- face mass flux (u . n) A ------------------------------------------------------------------
+// The verification flux of a prescribed velocity (reference: tests/support/fields.py:16-43), one
+// pass over the faces instead of a matrix product over the (d, F) normals.
+
+namespace {
+
+template <int D, bool PF>
+__global__ void __launch_bounds__(256) k_face_flux(int64_t F, const double* __restrict__ u, const double* __restrict__ n,
+                                                   const double* __restrict__ area, double* __restrict__ mdot) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  double un = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) un += (PF ? u[(int64_t)j * F + f] : u[j]) * n[(int64_t)j * F + f];
+  mdot[f] = un * area[f];
+}
+
+template <int D, bool PF>
+__global__ void __launch_bounds__(256) k_face_flux_vjp(int64_t F, const double* __restrict__ u,
+                                                       const double* __restrict__ n, const double* __restrict__ area,
+                                                       const double* __restrict__ mb, double* __restrict__ nb,
+                                                       double* __restrict__ ab, double* __restrict__ ub) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  const double g = mb[f], A = area[f];
+  double uf[D], nf[D], un = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    uf[j] = PF ? u[(int64_t)j * F + f] : u[j];
+    nf[j] = n[(int64_t)j * F + f];
+    un += uf[j] * nf[j];
+  }
+  if (nb) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) nb[(int64_t)j * F + f] = uf[j] * A * g;
+  }
+  if (ab) ab[f] = un * g;
+  if (PF && ub) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) ub[(int64_t)j * F + f] = nf[j] * A * g;
+  }
+}
+
+template <class F2>
+int by_dim_pf(int dim, int per_face, F2&& f) {
+  if (dim == 2) return per_face ? f(std::integral_constant<int, 2>{}, std::true_type{})
+                                : f(std::integral_constant<int, 2>{}, std::false_type{});
+  return per_face ? f(std::integral_constant<int, 3>{}, std::true_type{})
+                  : f(std::integral_constant<int, 3>{}, std::false_type{});
+}
+
+}  // namespace
+
+extern "C" int afx_face_flux(int64_t F, int dim, int per_face, const double* u, const double* normal,
+                             const double* area, double* mdot, hipStream_t s) {
+  AFX_REQUIRE(F > 0 && (dim == 2 || dim == 3) && (per_face == 0 || per_face == 1) && u && normal && area && mdot,
+              "afx_face_flux: bad argument");
+  return by_dim_pf(dim, per_face, [&](auto Dc, auto Pc) {
+    constexpr int D = decltype(Dc)::value;
+    constexpr bool PF = decltype(Pc)::value;
+    AFX_LAUNCH(kKGeomFace, (k_face_flux<D, PF>), grid_for(F), dim3(kBlock), 0, s, F, u, normal, area, mdot);
+    AFX_CHECK_LAUNCH();
+    return 0;
+  });
+}
+
+extern "C" int afx_face_flux_vjp(int64_t F, int dim, int per_face, const double* u, const double* normal,
+                                 const double* area, const double* mdot_bar, double* normal_bar, double* area_bar,
+                                 double* u_bar, hipStream_t s) {
+  AFX_REQUIRE(F > 0 && (dim == 2 || dim == 3) && (per_face == 0 || per_face == 1) && u && normal && area &&
+                  mdot_bar,
+              "afx_face_flux_vjp: bad argument");
+  AFX_REQUIRE(per_face == 1 || u_bar == nullptr, "afx_face_flux_vjp: u_bar needs a per-face velocity");
+  return by_dim_pf(dim, per_face, [&](auto Dc, auto Pc) {
+    constexpr int D = decltype(Dc)::value;
+    constexpr bool PF = decltype(Pc)::value;
+    AFX_LAUNCH(kKGeomFace, (k_face_flux_vjp<D, PF>), grid_for(F), dim3(kBlock), 0, s, F, u, normal, area, mdot_bar,
+               normal_bar, area_bar, u_bar);
+    AFX_CHECK_LAUNCH();
+    return 0;
+  });
 }

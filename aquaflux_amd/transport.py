@@ -34,16 +34,50 @@ def effective_diffusivity(molecular, eddy_viscosity=None, turbulent_number: floa
     return FieldProperty(molecular + eddy_viscosity / turbulent_number)
 
 
+class _FaceFlux(torch.autograd.Function):
+    """(u . n) A over the faces with the hand-written transpose (afx_face_flux / _vjp)."""
+
+    @staticmethod
+    def forward(ctx, n, area, u):
+        d, F = n.shape
+        per_face = int(u.dim() == 2)
+        n, area, u = n.contiguous(), area.contiguous(), u.contiguous()
+        out = torch.empty(F, dtype=F64, device=n.device)
+        _lib.check(_lib.lib().afx_face_flux(F, d, per_face, u.data_ptr(), n.data_ptr(), area.data_ptr(),
+                                            out.data_ptr(), _lib.stream_of(n.device)), "afx_face_flux")
+        ctx.save_for_backward(n, area, u)
+        return out
+
+    @staticmethod
+    def backward(ctx, gm):
+        n, area, u = ctx.saved_tensors
+        d, F = n.shape
+        per_face = int(u.dim() == 2)
+        gm = gm.contiguous()
+        nb = torch.empty_like(n) if ctx.needs_input_grad[0] else None
+        ab = torch.empty_like(area) if ctx.needs_input_grad[1] else None
+        ub = torch.empty_like(u) if (ctx.needs_input_grad[2] and per_face) else None
+        _lib.check(_lib.lib().afx_face_flux_vjp(F, d, per_face, u.data_ptr(), n.data_ptr(), area.data_ptr(),
+                                                gm.data_ptr(), None if nb is None else nb.data_ptr(),
+                                                None if ab is None else ab.data_ptr(),
+                                                None if ub is None else ub.data_ptr(), _lib.stream_of(n.device)),
+                   "afx_face_flux_vjp")
+        if ctx.needs_input_grad[2] and not per_face:  # a uniform velocity: its cotangent sums over faces
+            ub = (n * area * gm).sum(dim=1)
+        return nb, ab, ub
+
+
 def face_mass_flux(geometry, velocity) -> torch.Tensor:
     """Owner-outward ``(u . n) A`` of a prescribed uniform or per-face velocity (the verification
     flux builder of the reference's ``tests/support/fields.py:16-43``); differentiable through the
-    geometry."""
+    geometry (and the velocity) by its hand-written transpose.  A per-face velocity is ``(F, d)``."""
     n = geometry.face.normal_soa
     u = torch.as_tensor(velocity, dtype=F64, device=n.device)
-    # one product over the (d, F) normals, whose backward is one (d, F) tensor (a sum of per-component
-    # slices would build and add d full-size gradient tensors)
-    un = u @ n if u.dim() == 1 else (u.T * n).sum(dim=0)
-    return un * geometry.face.area
+    if not n.is_cuda:
+        raise RuntimeError("face_mass_flux needs the geometry on the GPU")
+    if u.dim() == 2:
+        u = u.T
+    return _FaceFlux.apply(n, geometry.face.area, u)
 
 
 def _boundary_arrays(bound: BoundaryConditions, mesh, device, face_centroid=None):

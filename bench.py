@@ -31,7 +31,18 @@ S8 = 8  # bytes per FP64 value
 
 # Whole-step algorithmic bytes per face of residual + VJP (pass-structured compulsory traffic of the
 # recompute design: 3 forward face passes, 3 reverse passes) for the two 3D hex workloads.
-STEP_BYTES_PER_FACE = {"sst": 1386.0}
+STEP_BYTES_PER_FACE = {"sst": 1386.0, "tet": 890.0}
+
+
+def _traffic(name, kernel):
+    """HBM bytes per launch of ``kernel`` from profiles/<name>_traffic.json (rocprofv3 FETCH_SIZE /
+    WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        tk = json.load(fh).get("kernels", {}).get(kernel)
+    return tk["hbm_bytes_per_launch"] if tk else None
 
 
 def _face_pass(D, permuted=True):
@@ -359,6 +370,16 @@ def tet_workload(device, n=96, steps=10, warmup=2):
     avg_ms = kern[dom][0] / kern[dom][1]
     nbytes = algorithmic_bytes(dom, counts, "tet")
     achieved = nbytes / (avg_ms * 1e-3) / 1e9
+    ms_step = elapsed / steps * 1e3
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "avg_ms": round(avg_ms, 4),
+                "algorithmic_bytes": int(nbytes), "step_bytes_per_face": STEP_BYTES_PER_FACE["tet"],
+                "step_frac": round(STEP_BYTES_PER_FACE["tet"] * F / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    if n == 96:
+        tb = _traffic("scalar_tet6x96", dom)
+        if tb is not None:
+            roofline["traffic"] = tb
+            roofline["traffic_source"] = "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated (profiles/)"
     return mesh, {
         "metric": "Gfaces/s for residual+VJP",
         "value": round(F * steps / elapsed / 1e9, 4),
@@ -370,9 +391,7 @@ def tet_workload(device, n=96, steps=10, warmup=2):
                    "advection-diffusion Pe=100, Venkatakrishnan-limited upwind, Green-Gauss; step = geometry + "
                    "flux + residual + adjoint to node coordinates and phi", "cells": C, "faces": F},
         "kernel_avg_ms": {k: round(v[0] / v[1], 4) for k, v in sorted(kern.items())},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "avg_ms": round(avg_ms, 4), "algorithmic_bytes": int(nbytes)},
+        "roofline": roofline,
     }
 
 
@@ -646,13 +665,10 @@ def main(argv=None):
         roofline["step_bytes_per_face"] = STEP_BYTES_PER_FACE[wl]
         roofline["step_frac"] = round(step_gbs / HBM_PEAK_GBS, 4)
     tag = "rans" if wl == "sst" else "flow"
-    traffic_file = os.path.join(ROOT, "profiles", f"{tag}_hex{nx}x{ny}x{nz}_traffic.json")
-    if os.path.exists(traffic_file):
-        with open(traffic_file) as fh:
-            tk = json.load(fh).get("kernels", {}).get(dom)
-        if tk:
-            roofline["traffic"] = tk["hbm_bytes_per_launch"]
-            roofline["traffic_source"] = "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated (profiles/)"
+    tb = _traffic(f"{tag}_hex{nx}x{ny}x{nz}", dom)
+    if tb is not None:
+        roofline["traffic"] = tb
+        roofline["traffic_source"] = "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated (profiles/)"
     kernels = {k: round(v[0] / v[1], 4) for k, v in sorted(kern.items())}
     desc = {
         "sst": "coupled u,v,w,p,k,omega with the k-omega SST closure (wall functions, omega wall fixation)",

@@ -461,6 +461,18 @@ int afx_dot(int64_t n, int precision, const void* a, const void* b, double* part
 /* y = A x for a dense row-major n x n matrix (the coarsest level's pseudo-inverse); x and y differ. */
 int afx_dense_matvec(int64_t n, int precision, const void* A, const void* x, void* y, hipStream_t stream);
 
+/* Face mass flux mdot[f] = (u_f . n_f) A_f from the owner-outward unit normals normal[d][F] and areas
+ * area[F]; u is one velocity (per_face = 0, u[d]) or one per face (per_face = 1, u[d][F]).  Replaces
+ * the verification flux builder of the reference's tests/support/fields.py:16-43 (the K4 flux). */
+int afx_face_flux(int64_t n_faces, int dim, int per_face, const double* u, const double* normal, const double* area,
+                  double* mdot, hipStream_t stream);
+/* Its transpose for a cotangent mdot_bar[F]: normal_bar[d][F] = u A mdot_bar, area_bar[F] = (u . n)
+ * mdot_bar (either may be NULL, each written, not accumulated), and with a per-face u, u_bar[d][F] =
+ * n A mdot_bar (NULL to skip).  Replaces jax.vjp through that flux. */
+int afx_face_flux_vjp(int64_t n_faces, int dim, int per_face, const double* u, const double* normal,
+                      const double* area, const double* mdot_bar, double* normal_bar, double* area_bar, double* u_bar,
+                      hipStream_t stream);
+
 /* ---- Krylov basis ------------------------------------------------------------------------------ */
 /* x[i] += sum_{j<k} y[j] basis[j * ld + i] with basis FP32 (precision 0) or FP64 (1), y and x FP64,
  * the k terms summed in FP64 in index order.  Replaces the update x + Z y of the restarted GMRES in
