@@ -1028,7 +1028,7 @@ extern "C" int afx_face_flux(int64_t F, int dim, int per_face, const double* u, 
   return by_dim_pf(dim, per_face, [&](auto Dc, auto Pc) {
     constexpr int D = decltype(Dc)::value;
     constexpr bool PF = decltype(Pc)::value;
-    AFX_LAUNCH(kKGeomFace, (k_face_flux<D, PF>), grid_for(F), dim3(kBlock), 0, s, F, u, normal, area, mdot);
+    AFX_LAUNCH(kKFaceFlux, (k_face_flux<D, PF>), grid_for(F), dim3(kBlock), 0, s, F, u, normal, area, mdot);
     AFX_CHECK_LAUNCH();
     return 0;
   });
@@ -1044,7 +1044,7 @@ extern "C" int afx_face_flux_vjp(int64_t F, int dim, int per_face, const double*
   return by_dim_pf(dim, per_face, [&](auto Dc, auto Pc) {
     constexpr int D = decltype(Dc)::value;
     constexpr bool PF = decltype(Pc)::value;
-    AFX_LAUNCH(kKGeomFace, (k_face_flux_vjp<D, PF>), grid_for(F), dim3(kBlock), 0, s, F, u, normal, area, mdot_bar,
+    AFX_LAUNCH(kKFaceFlux, (k_face_flux_vjp<D, PF>), grid_for(F), dim3(kBlock), 0, s, F, u, normal, area, mdot_bar,
                normal_bar, area_bar, u_bar);
     AFX_CHECK_LAUNCH();
     return 0;

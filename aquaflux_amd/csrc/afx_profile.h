@@ -47,6 +47,7 @@ enum KernelId : int {
   kKProbeScatter,
   kKAmg,
   kKKrylov,
+  kKFaceFlux,
   kKNumKernels
 };
 

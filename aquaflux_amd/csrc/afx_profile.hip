@@ -20,6 +20,7 @@ const char* const kKernelNames[kKNumKernels] = {
     "limiter",       "cgg_sweep",     "cgg_sweep_adj",   "probe_scatter",
     "amg",
     "krylov",
+    "face_flux",
 };
 
 namespace {
