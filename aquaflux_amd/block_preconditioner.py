@@ -145,6 +145,9 @@ class AggregationLevel:
     gmap: torch.Tensor | None  # (nslot * n,) int64 into the next level's (nslot', n') values
     agg32: torch.Tensor | None = None  # (n,) int32, the kernels' copy of agg
     members: torch.Tensor | None = None  # (max_members, n') int32 rows of each aggregate, -1 padded
+    # (nslot' * n', k) int64: the entries of this level that sum into each next-level entry, in
+    # ascending order, padded with nslot * n (a zero appended to the values)
+    gsrc: torch.Tensor | None = None
 
 
 class AggregationHierarchy:
@@ -228,8 +231,14 @@ class AggregationHierarchy:
             rank = np.arange(n) - np.repeat(np.concatenate([[0], np.cumsum(size)[:-1]]), size)
             members = np.full((int(size.max()), nc), -1, dtype=np.int32)
             members[rank, agg[order]] = order
+            # Galerkin sums as gathers: each next-level entry lists its entries of this level
+            src_order = np.argsort(gmap, kind="stable")
+            src_count = np.bincount(gmap, minlength=ns_c * nc)
+            src_rank = np.arange(gmap.size) - np.repeat(np.concatenate([[0], np.cumsum(src_count)[:-1]]), src_count)
+            gsrc = np.full((ns_c * nc, max(1, int(src_count.max()))), gmap.size, dtype=np.int64)
+            gsrc[gmap[src_order], src_rank] = src_order
             levels.append(AggregationLevel(n, nslot, T(cols.astype(np.int32)), T(agg.astype(np.int64)), T(gmap),
-                                           T(agg.astype(np.int32)), T(members)))
+                                           T(agg.astype(np.int32)), T(members), T(gsrc)))
             sums = np.zeros((nc, dim))
             np.add.at(sums, agg, x)
             x = sums / np.bincount(agg, minlength=nc)[:, None]
@@ -244,11 +253,15 @@ class AggregationHierarchy:
         level's dense pseudo-inverse."""
         out = [vals.contiguous()]
         for lv, nx in zip(self.levels[:-1], self.levels[1:]):
-            vc = torch.zeros(nx.nslot * nx.n, dtype=vals.dtype, device=vals.device)
-            vc.index_add_(0, lv.gmap, out[-1].reshape(-1))
+            # a gather of each coarse entry's fine entries and a sum along that short axis: the same
+            # summation order on every call (no atomics), so the cycle is bitwise reproducible
+            flat = torch.cat([out[-1].reshape(-1), out[-1].new_zeros(1)])
+            vc = flat[lv.gsrc].sum(dim=1)
             out.append(vc.view(nx.nslot, nx.n))
         last = self.levels[-1]
         A = torch.zeros(last.n, last.n, dtype=torch.float64, device=vals.device)
+        # duplicate positions here are the ELL's padding slots (column = row, value exactly 0), so the
+        # accumulation order cannot change the sum
         A.index_put_((self.coarse_rows, self.coarse_cols.long()), out[-1].reshape(-1).double(), accumulate=True)
         return out, torch.linalg.pinv(A, hermitian=True).to(vals.dtype).contiguous()
 

@@ -100,6 +100,11 @@ def test_native_v_cycle_matches_the_torch_cycle(case, dtype, cuda):
     print("levels", len(H.levels), "relative difference", err, flush=True)
     assert len(H.levels) >= 3
     assert err <= (1e-12 if dtype == torch.float64 else 1e-5)
+    # the hierarchy set-up and the native cycle are bitwise reproducible (Galerkin sums by gather,
+    # restriction by member table: no atomics)
+    ops2, cinv2 = H.operators(S)
+    assert all(torch.equal(a, c) for a, c in zip(ops, ops2)) and torch.equal(cinv, cinv2)
+    assert torch.equal(H.native_cycle(ops2, cinv2, b), got)
 
 
 def test_block_preconditioner_over_the_probed_jacobian(cuda):
