@@ -209,6 +209,8 @@ _SIGNATURES = {
     "afx_dense_matvec": [ctypes.c_int64, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "afx_basis_combine": [ctypes.c_int64, ctypes.c_int32, ctypes.c_int, c_void_p, ctypes.c_int64, c_void_p, c_void_p,
                           c_void_p],
+    "afx_geometry_jvp_workspace_bytes": [c_void_p, c_void_p],
+    "afx_geometry_jvp": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p],
     "afx_face_flux": [ctypes.c_int64, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "afx_face_flux_vjp": [ctypes.c_int64, ctypes.c_int, ctypes.c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                           c_void_p, c_void_p, c_void_p],
@@ -255,6 +257,11 @@ _SIGNATURES = {
     "afx_scalar_jvp": [
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxScalarConfig), ctypes.POINTER(AfxScalarInputs),
         ctypes.POINTER(AfxScalarInputs), c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p,
+    ],
+    "afx_scalar_jvp_geometry": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxScalarConfig),
+        ctypes.POINTER(AfxScalarInputs), ctypes.POINTER(AfxScalarInputs), c_void_p, c_void_p, c_void_p, ctypes.c_int64,
+        c_void_p,
     ],
     "afx_scalar_vjp": [
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxScalarConfig), ctypes.POINTER(AfxScalarInputs),

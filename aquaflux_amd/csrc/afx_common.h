@@ -240,16 +240,17 @@ struct MeshView {
   int32_t nbs;           // number of 256-cell blocks bsched orders
 };
 
-// Per-face geometry seen from a face's owner (P) and neighbour (N).
-template <int D>
+// Per-face geometry seen from a face's owner (P) and neighbour (N); G is double, or `dual` when a
+// geometry tangent is carried (the node-coordinate JVP).
+template <int D, class G = double>
 struct FaceGeo {
-  double A, n[D], xf[D], xP[D], xN[D];
-  double g;      // interpolation factor (0 on boundary faces)
-  double dvec[D];  // xN - xP (0 on boundary faces)
-  double dist;   // normal distance: d.n (interior) or (xf - xP).n (boundary)
-  double skew[D];  // xf - (xP + g d)
-  double dp[D], dn[D];  // xf - xP, xf - xN
-  double dpn, dnn;
+  G A, n[D], xf[D], xP[D], xN[D];
+  G g;      // interpolation factor (0 on boundary faces)
+  G dvec[D];  // xN - xP (0 on boundary faces)
+  G dist;   // normal distance: d.n (interior) or (xf - xP).n (boundary)
+  G skew[D];  // xf - (xP + g d)
+  G dp[D], dn[D];  // xf - xP, xf - xN
+  G dpn, dnn;
 };
 
 template <int D>
@@ -258,6 +259,38 @@ AFX_DEV double vdot(const double* a, const double* b) {
 #pragma unroll
   for (int j = 1; j < D; ++j) s = s + a[j] * b[j];
   return s;
+}
+
+template <int D, class G>
+AFX_DEV G vdotg(const G* a, const G* b) {
+  G s = a[0] * b[0];
+#pragma unroll
+  for (int j = 1; j < D; ++j) s = s + a[j] * b[j];
+  return s;
+}
+
+// The derived quantities of a FaceGeo from its A, n, xf, xP and xN (xN = xP on a boundary face), in
+// load_face_geo's order of operations.
+template <int D, class G>
+AFX_DEV void finish_face_geo(bool interior, FaceGeo<D, G>& q) {
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    q.dvec[j] = q.xN[j] - q.xP[j];
+    q.dp[j] = q.xf[j] - q.xP[j];
+    q.dn[j] = q.xf[j] - q.xN[j];
+  }
+  q.dpn = vdotg<D, G>(q.dp, q.n);
+  q.dnn = vdotg<D, G>(q.dn, q.n);
+  if (interior) {
+    G dd = vdotg<D, G>(q.dvec, q.dvec);
+    q.g = vdotg<D, G>(q.dp, q.dvec) / dd;
+    q.dist = vdotg<D, G>(q.dvec, q.n);
+  } else {
+    q.g = sconst<G>(0.0);
+    q.dist = q.dpn;
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) q.skew[j] = q.xf[j] - (q.xP[j] + q.dvec[j] * q.g);
 }
 
 // Loads the geometry of face f; `P`/`N` are its owner and neighbour (N < 0 on a boundary face).
@@ -391,21 +424,22 @@ AFX_DEV void lsq_matrix(const MeshView& m, int32_t c, double M[D][D]) {
   }
 }
 
-// Inverse of a symmetric positive-definite D x D matrix (adjugate / determinant).
-template <int D>
-AFX_DEV void sym_inverse(const double M[D][D], double I[D][D]) {
+// Inverse of a symmetric positive-definite D x D matrix (adjugate / determinant); T is double, or
+// `dual` for its tangent.
+template <int D, class T = double>
+AFX_DEV void sym_inverse(const T M[D][D], T I[D][D]) {
   if (D == 2) {
-    double det = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+    T det = M[0][0] * M[1][1] - M[0][1] * M[1][0];
     I[0][0] = M[1][1] / det;
     I[1][1] = M[0][0] / det;
     I[0][1] = -M[0][1] / det;
     I[1][0] = -M[1][0] / det;
   } else {
-    double c00 = M[1][1] * M[2][2] - M[1][2] * M[2][1];
-    double c01 = M[1][2] * M[2][0] - M[1][0] * M[2][2];
-    double c02 = M[1][0] * M[2][1] - M[1][1] * M[2][0];
-    double det = M[0][0] * c00 + M[0][1] * c01 + M[0][2] * c02;
-    double inv = 1.0 / det;
+    T c00 = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+    T c01 = M[1][2] * M[2][0] - M[1][0] * M[2][2];
+    T c02 = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+    T det = M[0][0] * c00 + M[0][1] * c01 + M[0][2] * c02;
+    T inv = 1.0 / det;
     I[0][0] = c00 * inv;
     I[1][0] = c01 * inv;
     I[2][0] = c02 * inv;

@@ -313,8 +313,8 @@ AFX_DEV S ap_boundary(const FaceGeo<D>& q, int kind, const FlowCell<D, S>& P, S 
 }
 
 // Venkatakrishnan face value seen from a cell (schemes/limiter.py:88-113).
-template <class S>
-AFX_DEV S venkat_face(S dm0, S head_max, S head_min, double e2) {
+template <class S, class E = double>
+AFX_DEV S venkat_face(S dm0, S head_max, S head_min, E e2) {
   double sgn = val(dm0) >= 0.0 ? 1.0 : -1.0;
   S dm = sgn * (dabs(dm0) + 1e-12);
   S head = val(dm) > 0.0 ? head_max : head_min;

@@ -1050,3 +1050,214 @@ extern "C" int afx_face_flux_vjp(int64_t F, int dim, int per_face, const double*
     return 0;
   });
 }
+
+// ---- geometry tangent: the four geometry passes on dual numbers --------------------------------
+// Node coordinates with a tangent (X, Xt) -> the geometry and its tangent, each pass the forward
+// pass's operations in the same order on `dual` values (so the values are the forward's bits and
+// the tangents are exact derivatives; branch decisions read the values).
+
+namespace {
+
+template <int D>
+__global__ void __launch_bounds__(256) k_face_geometry_t(int32_t F, int32_t Nn, const int32_t* __restrict__ foff,
+                                                         const int32_t* __restrict__ fidx, In<dual> X, Out<dual> area,
+                                                         Out<dual> raw, Out<dual> fc) {
+  const int32_t f = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (f >= F) return;
+  const int32_t b = foff[f], e = foff[f + 1];
+  if (D == 2) {
+    const int32_t a0 = fidx[b], a1 = fidx[b + 1];
+    const dual p0[2] = {X[a0], X[Nn + a0]}, p1[2] = {X[a1], X[Nn + a1]};
+    const dual d0 = p1[0] - p0[0], d1 = p1[1] - p0[1];
+    const dual sq = d0 * d0 + d1 * d1;
+    area.set(f, sq.v > 0.0 ? dsqrt(sq) : dual{0.0, 0.0});
+    fc.set(f, 0.5 * (p0[0] + p1[0]));
+    fc.set(F + f, 0.5 * (p0[1] + p1[1]));
+    const dual v0 = d1, v1 = -d0;
+    const dual s2 = v0 * v0 + v1 * v1;
+    const dual inv = s2.v > 0.0 ? dsqrt(s2) : dual{1.0, 0.0};
+    raw.set(f, s2.v > 0.0 ? v0 / inv : dual{0.0, 0.0});
+    raw.set(F + f, s2.v > 0.0 ? v1 / inv : dual{0.0, 0.0});
+  } else {
+    const int32_t k = e - b;
+    dual ctr[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    for (int32_t i = b; i < e; ++i) {
+      const int32_t a = fidx[i];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) ctr[j] += X[(int64_t)j * Nn + a];
+    }
+    const double invk = 1.0 / (double)k;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) ctr[j] = ctr[j] * invk;
+    dual S[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    for (int32_t i = b; i < e; ++i) {
+      const int32_t a = fidx[i], c = fidx[i + 1 < e ? i + 1 : b];
+      dual u[3], w[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        u[j] = X[(int64_t)j * Nn + a] - ctr[j];
+        w[j] = X[(int64_t)j * Nn + c] - ctr[j];
+      }
+      S[0] += 0.5 * (u[1] * w[2] - u[2] * w[1]);
+      S[1] += 0.5 * (u[2] * w[0] - u[0] * w[2]);
+      S[2] += 0.5 * (u[0] * w[1] - u[1] * w[0]);
+    }
+    const dual sq = S[0] * S[0] + S[1] * S[1] + S[2] * S[2];
+    const dual A = sq.v > 0.0 ? dsqrt(sq) : dual{0.0, 0.0};
+    dual nn[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) nn[j] = sq.v > 0.0 ? S[j] / A : dual{0.0, 0.0};
+    dual cs[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    for (int32_t i = b; i < e; ++i) {
+      const int32_t a = fidx[i], c = fidx[i + 1 < e ? i + 1 : b];
+      dual pb[3], pc[3], u[3], w[3], dir[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        pb[j] = X[(int64_t)j * Nn + a];
+        pc[j] = X[(int64_t)j * Nn + c];
+        u[j] = pb[j] - ctr[j];
+        w[j] = pc[j] - ctr[j];
+      }
+      dir[0] = 0.5 * (u[1] * w[2] - u[2] * w[1]);
+      dir[1] = 0.5 * (u[2] * w[0] - u[0] * w[2]);
+      dir[2] = 0.5 * (u[0] * w[1] - u[1] * w[0]);
+      const dual sgn = dir[0] * nn[0] + dir[1] * nn[1] + dir[2] * nn[2];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) cs[j] += sgn * ((ctr[j] + pb[j] + pc[j]) / 3.0);
+    }
+    const dual safe = A.v > 0.0 ? A : dual{1.0, 0.0};
+    area.set(f, A);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      raw.set((int64_t)j * F + f, nn[j]);
+      fc.set((int64_t)j * F + f, cs[j] / safe);
+    }
+  }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_approx_centroid_t(MeshView m, In<dual> fc, Out<dual> approx) {
+  const int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= m.C) return;
+  dual s[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) s[j] = dual{0.0, 0.0};
+  const int32_t b = m.cptr[c], e = m.cptr[c + 1];
+  for (int32_t k = b; k < e; ++k) {
+    const int32_t ent = m.cface[k];
+    const int32_t f = ent >= 0 ? ent : ~ent;
+#pragma unroll
+    for (int j = 0; j < D; ++j) s[j] += fc[(int64_t)j * m.F + f];
+  }
+  const double inv = 1.0 / (double)(e - b);
+#pragma unroll
+  for (int j = 0; j < D; ++j) approx.set((int64_t)j * m.C + c, s[j] * inv);
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_orient_t(MeshView m, In<dual> fc, In<dual> approx, In<dual> raw,
+                                                  Out<dual> nrm) {
+  const int32_t f = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (f >= m.F) return;
+  const int32_t P = m.owner[f];
+  double proj = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) proj += (fc[(int64_t)j * m.F + f].v - approx[(int64_t)j * m.C + P].v) * raw[(int64_t)j * m.F + f].v;
+  const double sgn = proj < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) nrm.set((int64_t)j * m.F + f, raw[(int64_t)j * m.F + f] * sgn);
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_cell_volume_t(MeshView m, In<dual> approx, In<dual> fc, In<dual> nrm,
+                                                       In<dual> area, Out<dual> vol, Out<dual> cc) {
+  const int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= m.C) return;
+  dual ap[D], v = {0.0, 0.0}, cs[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    ap[j] = approx[(int64_t)j * m.C + c];
+    cs[j] = dual{0.0, 0.0};
+  }
+  for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+    const int32_t ent = m.cface[k];
+    const int32_t f = ent >= 0 ? ent : ~ent;
+    dual loc[D], nl[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      loc[j] = fc[(int64_t)j * m.F + f] - ap[j];
+      nl[j] = nrm[(int64_t)j * m.F + f];
+    }
+    dual dot = loc[0] * nl[0];
+#pragma unroll
+    for (int j = 1; j < D; ++j) dot = dot + loc[j] * nl[j];
+    const dual flux = dot * area[f];
+    if (ent >= 0) {
+      v += flux;
+#pragma unroll
+      for (int j = 0; j < D; ++j) cs[j] += loc[j] * flux;
+    } else {
+      v += -flux;
+#pragma unroll
+      for (int j = 0; j < D; ++j) cs[j] += -(loc[j] * flux);
+    }
+  }
+  const dual V = v / (double)D;
+  vol.set(c, V);
+  const dual inv = 1.0 / ((double)(D + 1) * V);
+#pragma unroll
+  for (int j = 0; j < D; ++j) cc.set((int64_t)j * m.C + c, cs[j] * inv + ap[j]);
+}
+
+template <int D>
+int geometry_jvp(const afx_mesh* m, const double* X, const double* Xt, afx_geometry* g, afx_geometry* t, double* work,
+                 hipStream_t s) {
+  const int64_t C = m->C, F = m->F;
+  double* approx = work;
+  double* approx_t = approx + D * C;
+  double* raw = approx_t + D * C;
+  double* raw_t = raw + D * F;
+  afx::MeshView v = afx_view(m, nullptr);
+  const double* fcv = g->face_centroid;
+  const double* fct = t->face_centroid;
+  AFX_LAUNCH(kKGeomFace, k_face_geometry_t<D>, grid_for(F), dim3(kBlock), 0, s, m->F, m->N, m->foff, m->fidx,
+             In<dual>{X, Xt}, Out<dual>{const_cast<double*>(g->area), const_cast<double*>(t->area)},
+             Out<dual>{raw, raw_t}, Out<dual>{const_cast<double*>(fcv), const_cast<double*>(fct)});
+  AFX_CHECK_LAUNCH();
+  AFX_LAUNCH(kKGeomApprox, k_approx_centroid_t<D>, grid_for(C), dim3(kBlock), 0, s, v, In<dual>{fcv, fct},
+             Out<dual>{approx, approx_t});
+  AFX_CHECK_LAUNCH();
+  AFX_LAUNCH(kKGeomOrient, k_orient_t<D>, grid_for(F), dim3(kBlock), 0, s, v, In<dual>{fcv, fct},
+             In<dual>{approx, approx_t}, In<dual>{raw, raw_t},
+             Out<dual>{const_cast<double*>(g->normal), const_cast<double*>(t->normal)});
+  AFX_CHECK_LAUNCH();
+  AFX_LAUNCH(kKGeomVolume, k_cell_volume_t<D>, grid_for(C), dim3(kBlock), 0, s, v, In<dual>{approx, approx_t},
+             In<dual>{fcv, fct}, In<dual>{g->normal, t->normal}, In<dual>{g->area, t->area},
+             Out<dual>{const_cast<double*>(g->volume), const_cast<double*>(t->volume)},
+             Out<dual>{const_cast<double*>(g->cell_centroid), const_cast<double*>(t->cell_centroid)});
+  AFX_CHECK_LAUNCH();
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int afx_geometry_jvp_workspace_bytes(const afx_mesh* m, int64_t* bytes) {
+  AFX_REQUIRE(m && bytes, "afx_geometry_jvp_workspace_bytes: null argument");
+  *bytes = (int64_t)2 * m->dim * ((int64_t)m->C + m->F) * 8 + 256;
+  return 0;
+}
+
+extern "C" int afx_geometry_jvp(const afx_mesh* m, const double* node_coords, const double* node_tangent,
+                                afx_geometry* out, afx_geometry* tangent, void* work, int64_t work_bytes,
+                                hipStream_t s) {
+  AFX_REQUIRE(m && node_coords && node_tangent && out && tangent && work, "afx_geometry_jvp: null argument");
+  AFX_REQUIRE(out->area && out->normal && out->face_centroid && out->volume && out->cell_centroid,
+              "afx_geometry_jvp: geometry output pointer missing");
+  AFX_REQUIRE(tangent->area && tangent->normal && tangent->face_centroid && tangent->volume && tangent->cell_centroid,
+              "afx_geometry_jvp: tangent output pointer missing");
+  int64_t need = 0;
+  afx_geometry_jvp_workspace_bytes(m, &need);
+  AFX_REQUIRE(work_bytes >= need - 256, "afx_geometry_jvp: workspace too small");
+  if (m->dim == 2) return geometry_jvp<2>(m, node_coords, node_tangent, out, tangent, (double*)work, s);
+  return geometry_jvp<3>(m, node_coords, node_tangent, out, tangent, (double*)work, s);
+}

@@ -46,8 +46,8 @@ struct ScArgs {
 
 // Accumulation V (phi - phi_old) / dt (BDF1) or V (1.5 phi - 2 phi_old + 0.5 phi_older) / dt
 // (transient.py:35-62).
-template <class S>
-AFX_DEV S accumulation(const ScArgs<S>& a, int32_t c, S phi, double V) {
+template <class S, class VT = double>
+AFX_DEV S accumulation(const ScArgs<S>& a, int32_t c, S phi, VT V) {
   if (a.bdf == 1) return V * (phi - a.phi_old[c]) / a.dt;
   return V * ((1.5 * phi - 2.0 * a.phi_old[c]) + 0.5 * a.phi_older[c]) / a.dt;
 }
@@ -271,6 +271,185 @@ __global__ void __launch_bounds__(256) k_sc_res(MeshView m, ScArgs<S> a, Out<S> 
     acc = r.own ? acc + fl : acc - fl;
   }
   if (a.bdf) acc = acc + accumulation<S>(a, c, xc, m.vol[c]);
+  R.set(c, acc);
+}
+
+// ---------------------------------------------------------------------------------------------
+// passes G and R with a geometry tangent (the node-coordinate JVP): the same face physics on `dual`
+// geometry.  Used only when a geometry tangent is requested; no LDS tiles.
+
+// Geometry tangents in the view's face order: face area / normal / centroid [F] / [d][F] and cell
+// volume / centroid [C] / [d][C].
+struct GeoTan {
+  const double *area, *nrm, *fc, *vol, *cc;
+};
+
+template <int D>
+AFX_DEV void cell_centroid_g(const MeshView& m, const GeoTan& t, int32_t c, dual (&x)[D]) {
+#pragma unroll
+  for (int j = 0; j < D; ++j) x[j] = dual{m.cc[(int64_t)j * m.C + c], t.cc[(int64_t)j * m.C + c]};
+}
+
+template <int D>
+AFX_DEV void load_face_geo_g(const MeshView& m, const GeoTan& t, int32_t f, bool interior, const dual* xP,
+                             const dual* xN, FaceGeo<D, dual>& q) {
+  q.A = dual{m.area[f], t.area[f]};
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    q.n[j] = dual{m.nrm[(int64_t)j * m.F + f], t.nrm[(int64_t)j * m.F + f]};
+    q.xf[j] = dual{m.fc[(int64_t)j * m.F + f], t.fc[(int64_t)j * m.F + f]};
+    q.xP[j] = xP[j];
+    q.xN[j] = interior ? xN[j] : xP[j];
+  }
+  finish_face_geo<D, dual>(interior, q);
+}
+
+template <int D, int GRAD, bool LIM>
+__global__ void __launch_bounds__(256) k_sc_grad_g(MeshView m, GeoTan gt, ScArgs<dual> a, Out<dual> G,
+                                                   Out<dual> psi_out) {
+  const int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= m.C) return;
+  const dual xc = a.phi[c];
+  dual ccc[D];
+  cell_centroid_g<D>(m, gt, c, ccc);
+  dual acc[D], M[D][D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    acc[j] = dual{0.0, 0.0};
+#pragma unroll
+    for (int q2 = 0; q2 < D; ++q2) M[j][q2] = dual{0.0, 0.0};
+  }
+  dual mx = xc, mn = xc, zero[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) zero[j] = dual{0.0, 0.0};
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
+    const int32_t o = r.own ? r.N : r.P;
+    if (o >= 0) {
+      const dual xo = a.phi[o];
+      if (LIM) {
+        if (xo.v > mx.v) mx = xo;
+        if (xo.v < mn.v) mn = xo;
+      }
+      dual cco[D];
+      cell_centroid_g<D>(m, gt, o, cco);
+      if (GRAD == 0) {
+        FaceGeo<D, dual> q;
+        load_face_geo_g<D>(m, gt, r.f, true, r.own ? ccc : cco, r.own ? cco : ccc, q);
+        const dual w = 1.0 - q.g;
+        const dual pf = w * (r.own ? xc : xo) + q.g * (r.own ? xo : xc);
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          const dual t = pf * (q.n[j] * q.A);
+          acc[j] = r.own ? acc[j] + t : acc[j] - t;
+        }
+      } else if (GRAD == 1) {
+        dual dx[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) dx[j] = cco[j] - ccc[j];
+        const dual wgt = 1.0 / vdotg<D, dual>(dx, dx);
+#pragma unroll
+        for (int p = 0; p < D; ++p)
+#pragma unroll
+          for (int q2 = 0; q2 < D; ++q2) M[p][q2] += wgt * dx[p] * dx[q2];
+        const dual dphi = xo - xc;
+#pragma unroll
+        for (int j = 0; j < D; ++j) acc[j] += (wgt * dx[j]) * dphi;
+      }
+    } else if (GRAD != kGradNone) {
+      FaceGeo<D, dual> q;
+      load_face_geo_g<D>(m, gt, r.f, false, ccc, ccc, q);
+      const dual pb = scalar_bval<D, dual, dual>(a.kind[r.fu], q, xc, zero, a.val[r.fu], a.gam[c], a.tinf[r.fu]);
+      if (GRAD == 0) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) acc[j] += pb * (q.n[j] * q.A);
+      } else {
+        const dual wgt = 1.0 / vdotg<D, dual>(q.dp, q.dp);
+#pragma unroll
+        for (int p = 0; p < D; ++p)
+#pragma unroll
+          for (int q2 = 0; q2 < D; ++q2) M[p][q2] += wgt * q.dp[p] * q.dp[q2];
+        const dual dphi = pb - xc;
+#pragma unroll
+        for (int j = 0; j < D; ++j) acc[j] += (wgt * q.dp[j]) * dphi;
+      }
+    }
+  }
+  const dual V = dual{m.vol[c], gt.vol[c]};
+  dual grad[D];
+  if (GRAD == 0) {
+    const dual iv = 1.0 / V;
+#pragma unroll
+    for (int j = 0; j < D; ++j) grad[j] = acc[j] * iv;
+  } else if (GRAD == 1) {
+    dual I[D][D];
+    sym_inverse<D, dual>(M, I);
+#pragma unroll
+    for (int p = 0; p < D; ++p) {
+      dual sum = I[p][0] * acc[0];
+#pragma unroll
+      for (int q2 = 1; q2 < D; ++q2) sum = sum + I[p][q2] * acc[q2];
+      grad[p] = sum;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < D; ++j) grad[j] = dual{0.0, 0.0};
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) G.set((int64_t)j * m.C + c, grad[j]);
+  if (LIM) {
+    const dual e2 = V * (a.lim_k * a.lim_k * a.lim_k);
+    dual lo = dual{0.0, 0.0};
+    bool first = true;
+    for (FaceWalk walk(m, c); walk.more();) {
+      FaceRef r = walk.next(m, c);
+      dual dm = grad[0] * (dual{m.fc[r.f], gt.fc[r.f]} - ccc[0]);
+#pragma unroll
+      for (int j = 1; j < D; ++j)
+        dm = dm + grad[j] * (dual{m.fc[(int64_t)j * m.F + r.f], gt.fc[(int64_t)j * m.F + r.f]} - ccc[j]);
+      const dual vf = venkat_face<dual, dual>(dm, mx - xc, mn - xc, e2);
+      if (first || vf.v < lo.v) lo = vf;
+      first = false;
+    }
+    psi_out.set(c, dmin(dmax(lo, 0.0), 1.0));
+  }
+}
+
+template <int D, int ADV>
+__global__ void __launch_bounds__(256) k_sc_res_g(MeshView m, GeoTan gt, ScArgs<dual> a, Out<dual> R) {
+  const int32_t c = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= m.C) return;
+  const dual xc = a.phi[c], gc = a.gam[c];
+  const dual pc = ADV == kAdvLimited ? a.psi[c] : dual{1.0, 0.0};
+  dual Gc[D], ccc[D];
+  load_grad_row<D, dual>(a.G, m.C, c, Gc);
+  cell_centroid_g<D>(m, gt, c, ccc);
+  dual acc = dual{0.0, 0.0};
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
+    const int32_t o = r.own ? r.N : r.P;
+    FaceGeo<D, dual> q;
+    const dual Qf = a.flux[r.fu];
+    dual fl;
+    if (o >= 0) {
+      const dual xo = a.phi[o], go = a.gam[o];
+      const dual po = ADV == kAdvLimited ? a.psi[o] : dual{1.0, 0.0};
+      dual Go[D], cco[D];
+      load_grad_row<D, dual>(a.G, m.C, o, Go);
+      cell_centroid_g<D>(m, gt, o, cco);
+      load_face_geo_g<D>(m, gt, r.f, true, r.own ? ccc : cco, r.own ? cco : ccc, q);
+      if (r.own)
+        fl = scalar_flux_interior<D, ADV, dual, dual>(q, Qf, xc, xo, Gc, Go, pc, po, gc, go);
+      else
+        fl = scalar_flux_interior<D, ADV, dual, dual>(q, Qf, xo, xc, Go, Gc, po, pc, go, gc);
+    } else {
+      load_face_geo_g<D>(m, gt, r.f, false, ccc, ccc, q);
+      const dual bv = scalar_bval<D, dual, dual>(a.kind[r.fu], q, xc, Gc, a.val[r.fu], gc, a.tinf[r.fu]);
+      fl = scalar_flux_boundary<D, ADV, dual, dual>(q, Qf, xc, Gc, pc, bv, gc);
+    }
+    acc = r.own ? acc + fl : acc - fl;
+  }
+  if (a.bdf) acc = acc + accumulation<dual, dual>(a, c, xc, dual{m.vol[c], gt.vol[c]});
   R.set(c, acc);
 }
 
@@ -1465,6 +1644,59 @@ extern "C" int afx_scalar_jvp(const afx_mesh* m, const afx_geometry* g, const af
     return sc_forward<decltype(Dc)::value, dual, decltype(Ac)::value, decltype(Gc)::value>(
         m, g, a, Out<dual>{residual, residual_tangent}, w, w.Gt, w.psit, nullptr, cfg->gradient_sweeps,
         cfg->gradient_relaxation, s);
+  });
+}
+
+extern "C" int afx_scalar_jvp_geometry(const afx_mesh* m, const afx_geometry* g, const afx_geometry* geom_tangent,
+                                       const afx_scalar_config* cfg, const afx_scalar_inputs* in,
+                                       const afx_scalar_inputs* tangent, double* residual, double* residual_tangent,
+                                       void* work, int64_t work_size, hipStream_t s) {
+  int rc = sc_check(m, g, cfg, in, work, work_size, 1);
+  if (rc) return rc;
+  AFX_REQUIRE(geom_tangent && tangent && residual && residual_tangent, "afx_scalar_jvp_geometry: null argument");
+  AFX_REQUIRE(cfg->gradient != 3,
+              "afx_scalar_jvp_geometry: a geometry tangent is available for compact Green-Gauss, least-squares or no "
+              "gradient reconstruction");
+  AFX_REQUIRE(geom_tangent->area && geom_tangent->normal && geom_tangent->face_centroid && geom_tangent->volume &&
+                  geom_tangent->cell_centroid,
+              "afx_scalar_jvp_geometry: the geometry tangent needs all five arrays");
+  AFX_REQUIRE(m->fu == nullptr || geom_tangent->face_pack,
+              "afx_scalar_jvp_geometry: this mesh's faces are reordered: geom_tangent->face_pack must hold the face "
+              "tangents in the internal order (afx_geometry_pack of the tangent arrays)");
+  ScWork w = sc_carve(work, m->dim, m->C, 1);
+  ScArgs<dual> a = sc_args<dual>(cfg, in, tangent);
+  MeshView v;
+  if (int rv_ = afx_rview(m, g, s, &v)) return rv_;
+  GeoTan gt;
+  if (m->fu) {
+    gt.area = geom_tangent->face_pack;
+    gt.nrm = geom_tangent->face_pack + m->F;
+    gt.fc = geom_tangent->face_pack + (int64_t)(1 + m->dim) * m->F;
+  } else {
+    gt.area = geom_tangent->area;
+    gt.nrm = geom_tangent->normal;
+    gt.fc = geom_tangent->face_centroid;
+  }
+  gt.vol = geom_tangent->volume;
+  gt.cc = geom_tangent->cell_centroid;
+  return sc_dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
+    constexpr int D = decltype(Dc)::value, ADV = decltype(Ac)::value, GR = decltype(Gc)::value;
+    if constexpr (GR == kGradCorrected) {
+      return 2;  // refused above
+    } else {
+      constexpr bool LIM = ADV == kAdvLimited;
+      AFX_LAUNCH(kKScalarGrad, (k_sc_grad_g<D, GR, LIM>), grid_for(v.C), dim3(kBlock), 0, s, v, gt, a,
+                 Out<dual>{w.G, w.Gt}, Out<dual>{w.psi, w.psit});
+      AFX_CHECK_LAUNCH();
+      a.G.v = w.G;
+      a.G.t = w.Gt;
+      a.psi.v = w.psi;
+      a.psi.t = w.psit;
+      AFX_LAUNCH(kKScalarRes, (k_sc_res_g<D, ADV>), grid_for(v.C), dim3(kBlock), 0, s, v, gt, a,
+                 Out<dual>{residual, residual_tangent});
+      AFX_CHECK_LAUNCH();
+      return 0;
+    }
   });
 }
 

@@ -16,8 +16,8 @@ enum ScalarBC : int8_t { kBcNone = 0, kBcDirichlet = 1, kBcZeroGradient = 2, kBc
 
 // Weak boundary value phi_ip of a closure (``grad`` = owner gradient; zero for leading order).
 // Neumann / Convective read ``gam`` (owner diffusivity), ``val`` (q or h) and ``tinf``.
-template <int D, class S>
-AFX_DEV S scalar_bval(int kind, const FaceGeo<D>& q, S phiP, const S* G, S val, S gam, S tinf) {
+template <int D, class S, class GT = double>
+AFX_DEV S scalar_bval(int kind, const FaceGeo<D, GT>& q, S phiP, const S* G, S val, S gam, S tinf) {
   if (kind == kBcDirichlet) return val;
   if (kind == kBcNone) return sconst<S>(0.0);
   S corr = G[0] * (q.dp[0] - q.n[0] * q.dpn);
@@ -30,11 +30,11 @@ AFX_DEV S scalar_bval(int kind, const FaceGeo<D>& q, S phiP, const S* G, S val, 
 }
 
 // Upwind / limited face value (interior face).
-template <int D, int ADV, class S>
-AFX_DEV S scalar_face_value(const FaceGeo<D>& q, bool outflow, S phiP, S phiN, const S* GP, const S* GN, S psiP,
+template <int D, int ADV, class S, class GT = double>
+AFX_DEV S scalar_face_value(const FaceGeo<D, GT>& q, bool outflow, S phiP, S phiN, const S* GP, const S* GN, S psiP,
                             S psiN) {
   if (ADV == kAdvLimited) {
-    const double* xu = outflow ? q.xP : q.xN;
+    const GT* xu = outflow ? q.xP : q.xN;
     const S* Gu = outflow ? GP : GN;
     S t = Gu[0] * (q.xf[0] - xu[0]);
 #pragma unroll
@@ -45,13 +45,13 @@ AFX_DEV S scalar_face_value(const FaceGeo<D>& q, bool outflow, S phiP, S phiN, c
 }
 
 // Interior-face flux: Q phi_f + (-Gamma_P ((phiN - phiP) + cN - cP) / denom) A.
-template <int D, int ADV, class S>
-AFX_DEV S scalar_flux_interior(const FaceGeo<D>& q, S Qf, S phiP, S phiN, const S* GP, const S* GN, S psiP, S psiN,
+template <int D, int ADV, class S, class GT = double>
+AFX_DEV S scalar_flux_interior(const FaceGeo<D, GT>& q, S Qf, S phiP, S phiN, const S* GP, const S* GN, S psiP, S psiN,
                                S gamP, S gamN) {
   S flux = sconst<S>(0.0);
   if (ADV != kAdvNone) {
     const bool outflow = val(Qf) >= 0.0;
-    flux = Qf * scalar_face_value<D, ADV, S>(q, outflow, phiP, phiN, GP, GN, psiP, psiN);
+    flux = Qf * scalar_face_value<D, ADV, S, GT>(q, outflow, phiP, phiN, GP, GN, psiP, psiN);
   }
   S cp = GP[0] * (q.dp[0] - q.n[0] * q.dpn);
   S cn = GN[0] * (q.dn[0] - q.n[0] * q.dnn);
@@ -66,8 +66,8 @@ AFX_DEV S scalar_flux_interior(const FaceGeo<D>& q, S Qf, S phiP, S phiN, const 
 }
 
 // Boundary-face flux with weak value ``bval`` and boundary diffusion coefficient ``gB``.
-template <int D, int ADV, class S>
-AFX_DEV S scalar_flux_boundary(const FaceGeo<D>& q, S Qf, S phiP, const S* GP, S psiP, S bval, S gB) {
+template <int D, int ADV, class S, class GT = double>
+AFX_DEV S scalar_flux_boundary(const FaceGeo<D, GT>& q, S Qf, S phiP, const S* GP, S psiP, S bval, S gB) {
   S flux = sconst<S>(0.0);
   if (ADV != kAdvNone) {
     const bool outflow = val(Qf) >= 0.0;

@@ -97,7 +97,8 @@ def _empty_geometry(d, F, C, dev):
 
 class _GeometryFn(torch.autograd.Function):
     """Node coordinates (dim, N) -> (area, normal, face centroid, volume, cell centroid), with the
-    hand-written HIP adjoint (afx_geometry_vjp) as the backward."""
+    hand-written HIP adjoint (afx_geometry_vjp) as the backward and the dual-number HIP tangent
+    (afx_geometry_jvp) as the forward derivative."""
 
     @staticmethod
     def forward(ctx, dm, X):
@@ -110,6 +111,7 @@ class _GeometryFn(torch.autograd.Function):
         )
         ctx.dm = dm
         ctx.save_for_backward(X, area, nrm, fc, vol, cc)
+        ctx.save_for_forward(X)
         # an output nobody differentiates arrives as None (the adjoint takes a null cotangent)
         # rather than as a materialised zero array
         ctx.set_materialize_grads(False)
@@ -133,6 +135,26 @@ class _GeometryFn(torch.autograd.Function):
             "afx_geometry_vjp",
         )
         return None, out
+
+    @staticmethod
+    def jvp(ctx, _dm_t, dX):
+        (X,) = ctx.saved_tensors
+        dm = ctx.dm
+        d, F, C = dm.dim, dm.n_faces, dm.n_cells
+        vals = _empty_geometry(d, F, C, X.device)  # recomputed values, discarded
+        tans = _empty_geometry(d, F, C, X.device)
+        gs = _lib.AfxGeometry(*[t.data_ptr() for t in vals], None)
+        gt = _lib.AfxGeometry(*[t.data_ptr() for t in tans], None)
+        nbytes = ctypes.c_int64()
+        L = _lib.lib()
+        _lib.check(L.afx_geometry_jvp_workspace_bytes(dm.handle, ctypes.byref(nbytes)), "afx_geometry_jvp_workspace_bytes")
+        work = dm.workspace("geom_jvp", int(nbytes.value))
+        _lib.check(
+            L.afx_geometry_jvp(dm.handle, X.data_ptr(), dX.contiguous().data_ptr(), ctypes.byref(gs), ctypes.byref(gt),
+                               work.data_ptr(), work.numel(), _lib.stream_of(X.device)),
+            "afx_geometry_jvp",
+        )
+        return tans
 
 
 def compute_geometry(mesh, node_coords=None, device=None) -> MeshGeometry:

@@ -7,7 +7,9 @@ advection_scheme, *, gradient_scheme, sources, transient)``, ``with_diffusivity`
 diffusivity, the face flux, the boundary values and -- when the geometry came from node
 coordinates that require grad -- the five geometry arrays, which the geometry's own HIP adjoint
 carries on to the node coordinates.  The forward derivative (torch forward AD) is the dual-number
-HIP tangent with respect to phi, the diffusivity, the face flux and the boundary values.
+HIP tangent with respect to phi, the diffusivity, the face flux, the boundary values and -- for
+Green-Gauss, least-squares or no gradient reconstruction -- the geometry arrays, whose own tangent
+comes from the dual-number geometry kernels when the node coordinates carry one.
 """
 
 from __future__ import annotations
@@ -35,7 +37,8 @@ def effective_diffusivity(molecular, eddy_viscosity=None, turbulent_number: floa
 
 
 class _FaceFlux(torch.autograd.Function):
-    """(u . n) A over the faces with the hand-written transpose (afx_face_flux / _vjp)."""
+    """(u . n) A over the faces with the hand-written transpose (afx_face_flux / _vjp); the forward
+    derivative (u . dn) A + (u . n) dA + (du . n) A is three passes of the same kernel."""
 
     @staticmethod
     def forward(ctx, n, area, u):
@@ -46,7 +49,28 @@ class _FaceFlux(torch.autograd.Function):
         _lib.check(_lib.lib().afx_face_flux(F, d, per_face, u.data_ptr(), n.data_ptr(), area.data_ptr(),
                                             out.data_ptr(), _lib.stream_of(n.device)), "afx_face_flux")
         ctx.save_for_backward(n, area, u)
+        ctx.save_for_forward(n, area, u)
         return out
+
+    @staticmethod
+    def _flux(u, n, area):
+        d, F = n.shape
+        out = torch.empty(F, dtype=F64, device=n.device)
+        _lib.check(_lib.lib().afx_face_flux(F, d, int(u.dim() == 2), u.data_ptr(), n.data_ptr(), area.data_ptr(),
+                                            out.data_ptr(), _lib.stream_of(n.device)), "afx_face_flux")
+        return out
+
+    @staticmethod
+    def jvp(ctx, dn, darea, du):
+        n, area, u = ctx.saved_tensors
+        dm = torch.zeros(n.shape[1], dtype=F64, device=n.device)
+        if dn is not None:
+            dm += _FaceFlux._flux(u, dn.contiguous(), area)
+        if darea is not None:
+            dm += _FaceFlux._flux(u, n, darea.contiguous())
+        if du is not None:
+            dm += _FaceFlux._flux(du.contiguous(), n, area)
+        return dm
 
     @staticmethod
     def backward(ctx, gm):
@@ -166,6 +190,31 @@ class _ScalarRaw:
         )
         return R, dR
 
+    def jvp_geometry(self, prim, tans, geo, dgeo, step=None):
+        """(R, dR) along tangents of the inputs and of the five geometry arrays together."""
+        R = torch.empty_like(prim[0])
+        dR = torch.empty_like(prim[0])
+        w = self.dm.workspace("scalar1", self.work_bytes[1])
+        gs = self._gs(geo)
+        dgeo = [t.contiguous() for t in dgeo]
+        pack = None
+        if self.dm.face_permuted:  # the face tangents in the kernels' internal face order
+            pack = torch.empty((2 * self.dm.dim + 1) * self.dm.n_faces, dtype=F64, device=self.dm.device)
+            fs = _lib.AfxGeometry(*[t.data_ptr() for t in dgeo[:3]], None, None, None)
+            _lib.check(_lib.lib().afx_geometry_pack(self.dm.handle, ctypes.byref(fs), pack.data_ptr(),
+                                                    _lib.stream_of(self.dm.device)), "afx_geometry_pack")
+        gt = _lib.AfxGeometry(*[t.data_ptr() for t in dgeo], _lib.ptr(pack))
+        ins = _lib.AfxScalarInputs(*[_lib.ptr(t) for t in prim])
+        tns = _lib.AfxScalarInputs(*[_lib.ptr(t) for t in tans])
+        _lib.check(
+            _lib.lib().afx_scalar_jvp_geometry(self.dm.handle, ctypes.byref(gs), ctypes.byref(gt),
+                                               ctypes.byref(self._cfg(step)), ctypes.byref(ins), ctypes.byref(tns),
+                                               R.data_ptr(), dR.data_ptr(), w.data_ptr(), w.numel(),
+                                               _lib.stream_of(self.dm.device)),
+            "afx_scalar_jvp_geometry",
+        )
+        return R, dR
+
     def vjp(self, prim, cot, geo, with_geometry, step=None):
         bars = [None if t is None else torch.empty_like(t) for t in prim]
         gbars = [torch.empty_like(t) for t in geo] if with_geometry else [None] * 5
@@ -186,11 +235,12 @@ class _ScalarRaw:
 
 class _ScalarResidualFn(torch.autograd.Function):
     """R(phi, gamma, flux, value, t_inf, phi_old, phi_older; area, normal, face centroid, volume,
-    cell centroid) at the time step ``step`` (None steady, else (bdf order, dt))."""
+    cell centroid) at the time step ``step`` (None steady, else (bdf order, dt)); ``geo_tan`` says
+    whether the geometry arrays carry forward-mode tangents."""
 
     @staticmethod
-    def forward(ctx, raw, step, phi, gam, flux, val, tinf, phi_old, phi_older, area, nrm, fc, vol, cc):
-        ctx.raw, ctx.step = raw, step
+    def forward(ctx, raw, step, geo_tan, phi, gam, flux, val, tinf, phi_old, phi_older, area, nrm, fc, vol, cc):
+        ctx.raw, ctx.step, ctx.geo_tan = raw, step, geo_tan
         prim = (phi, gam, flux, val, tinf, phi_old, phi_older)
         geo = (area, nrm, fc, vol, cc)
         ctx.has_old = (phi_old is not None, phi_older is not None)
@@ -211,17 +261,19 @@ class _ScalarResidualFn(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gR):
         prim, geo = _ScalarResidualFn._unpack(ctx)
-        with_geo = any(ctx.needs_input_grad[9:14])
+        with_geo = any(ctx.needs_input_grad[10:15])
         bars, gbars = ctx.raw.vjp(prim, gR.contiguous(), geo, with_geo, ctx.step)
-        return (None, None, *bars, *gbars)
+        return (None, None, None, *bars, *gbars)
 
     @staticmethod
-    def jvp(ctx, _raw_t, _step_t, dphi, dgam, dflux, dval, dtinf, dold, dolder, *_dgeo):
-        # geometry tangents arrive as zeros (torch fills them for plain tensors); a geometry that
-        # carries a real tangent is refused before apply
+    def jvp(ctx, _raw_t, _step_t, _geo_tan_t, dphi, dgam, dflux, dval, dtinf, dold, dolder, *dgeo):
         prim, geo = _ScalarResidualFn._unpack(ctx)
         tans = [None if t is None else t.contiguous() for t in (dphi, dgam, dflux, dval, dtinf, dold, dolder)]
-        _, dR = ctx.raw.jvp(prim, tans, geo, ctx.step)
+        if ctx.geo_tan:
+            dgeo = [torch.zeros_like(g) if t is None else t for g, t in zip(geo, dgeo)]
+            _, dR = ctx.raw.jvp_geometry(prim, tans, geo, dgeo, ctx.step)
+        else:  # the geometry is constant: torch fills its tangents with zeros, which are not read
+            _, dR = ctx.raw.jvp(prim, tans, geo, ctx.step)
         return dR
 
 
@@ -314,10 +366,13 @@ class ScalarTransport:
             _lib.require_cuda(phi, "phi")
             step, old, older = self._time_level(phi, phi_old, phi_older, dt, first_step)
             geo = self._geometry()
-            if any(fwAD.unpack_dual(t).tangent is not None for t in geo):
-                raise NotImplementedError("forward-mode derivatives with respect to the geometry are not provided")
-            return _ScalarResidualFn.apply(self._raw, step, phi.contiguous(), self._gamma(), flux.contiguous(),
-                                           self._val, self._tinf, old, older, *geo)
+            geo_tan = any(fwAD.unpack_dual(t).tangent is not None for t in geo)
+            if geo_tan and self._raw.cfg.gradient == 3:
+                raise NotImplementedError("forward-mode derivatives with respect to the geometry are provided for "
+                                          "Green-Gauss, least-squares or no gradient reconstruction, not the "
+                                          "corrected Green-Gauss scheme")
+            return _ScalarResidualFn.apply(self._raw, step, geo_tan, phi.contiguous(), self._gamma(),
+                                           flux.contiguous(), self._val, self._tinf, old, older, *geo)
 
         return fn
 

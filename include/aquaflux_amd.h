@@ -93,6 +93,16 @@ typedef struct {
  * aquaflux/mesh/face.py:196-349, aquaflux/mesh/cell.py:84-170). */
 int afx_geometry_compute(const afx_mesh* mesh, const double* node_coords, afx_geometry* out, hipStream_t stream);
 
+/* Forward-mode derivative of afx_geometry_compute: from node coordinates node_coords[d][N] and a
+ * tangent node_tangent[d][N], writes the geometry into `out` (afx_geometry_compute's values up to
+ * rounding; face_pack untouched) and its tangent into `tangent` (area, normal, face centroid, volume,
+ * cell centroid; face_pack unused).  Workspace from afx_geometry_jvp_workspace_bytes.  Replaces
+ * jax.jvp through Mesh.geometry (reference: aquaflux/mesh/mesh.py:341-387,
+ * aquaflux/mesh/face.py:196-349, aquaflux/mesh/cell.py:84-170). */
+int afx_geometry_jvp_workspace_bytes(const afx_mesh* mesh, int64_t* bytes);
+int afx_geometry_jvp(const afx_mesh* mesh, const double* node_coords, const double* node_tangent, afx_geometry* out,
+                     afx_geometry* tangent, void* work, int64_t work_bytes, hipStream_t stream);
+
 /* Writes geom's face arrays in the internal face order into face_pack[(2d+1)][F] (a no-op when the
  * mesh reports no permutation).  For geometry not produced by afx_geometry_compute.  No reference
  * counterpart. */
@@ -282,6 +292,19 @@ int afx_scalar_noalloc_max_bytes(int64_t* bytes);
 int afx_scalar_jvp(const afx_mesh* mesh, const afx_geometry* geom, const afx_scalar_config* cfg,
                    const afx_scalar_inputs* in, const afx_scalar_inputs* tangent, double* residual,
                    double* residual_tangent, void* work, int64_t work_bytes, hipStream_t stream);
+
+/* afx_scalar_jvp with a tangent of the geometry as well: geom_tangent holds the tangents of area,
+ * normal, face centroid, volume and cell centroid (e.g. from afx_geometry_jvp of a node-coordinate
+ * tangent), and face_pack their face arrays in the internal order when the mesh reorders faces
+ * (afx_geometry_pack of the tangent arrays).  Compact Green-Gauss, least-squares or no gradient
+ * reconstruction.  With `tangent` also carrying the face-flux tangent, residual_tangent is the
+ * derivative of the residual along the node-coordinate direction.  Replaces jax.jvp of
+ * ScalarTransport.residual with Mesh.node_coords as a differentiable leaf (reference:
+ * aquaflux/mesh/mesh.py:63-66, aquaflux/transport/scalar.py:185-223). */
+int afx_scalar_jvp_geometry(const afx_mesh* mesh, const afx_geometry* geom, const afx_geometry* geom_tangent,
+                            const afx_scalar_config* cfg, const afx_scalar_inputs* in,
+                            const afx_scalar_inputs* tangent, double* residual, double* residual_tangent, void* work,
+                            int64_t work_bytes, hipStream_t stream);
 
 /* Cotangents of phi, gamma, the face flux, the boundary values and (optionally) the geometry.
  * Replaces jax.vjp of the scalar residual, including through MeshGeometry. */

@@ -1,5 +1,5 @@
-"""GPU parity of the scalar-transport residual, tangent and adjoint (including the adjoint with
-respect to mesh node coordinates through the geometry) against the CPU oracle.
+"""GPU parity of the scalar-transport residual, tangent and adjoint (including the tangent and the
+adjoint with respect to mesh node coordinates through the geometry) against the CPU oracle.
 
 Tolerances: residual 1e-12 and derivatives 1e-11 relative to the largest entry of each compared
 array (register sums on the GPU and index_add scatters in the oracle associate differently).
@@ -10,6 +10,7 @@ import dataclasses
 import numpy as np
 import pytest
 import torch
+import torch.autograd.forward_ad as fwAD
 
 from oracle import fv
 from tests import scalar_cases as SC
@@ -201,6 +202,109 @@ def test_scalar_node_adjoint_finite_difference(cuda):
     fd = (4.0 * cd(5e-5) - cd(1e-4)) / 3.0
     print("fd", fd, "adjoint", an)
     assert abs(fd - an) <= 1e-7 * max(abs(an), 1e-12)
+
+
+def _node_tangent_pair(mesh, u, bcs, adv, grad, phi, gamma, dX, cuda, hist=None):
+    """(R, dR/dX . dX) of the oracle (torch forward AD) and of the HIP path (dual-number geometry
+    and residual kernels under torch forward AD)."""
+    T = torch.tensor
+    s = SC.oracle_setup(mesh, bcs, adv, grad)
+    extra_o = () if hist is None else (T(hist[0]), T(hist[1]), DT, hist[2])
+
+    def f(Xo):
+        m = fv.OMesh.from_mesh(mesh, node_coords=Xo)
+        g = fv.geometry(m)
+        return fv.scalar_residual(m, g, s, T(phi), SC.oracle_flux(g, u), T(gamma), *extra_o)
+
+    ref = torch.func.jvp(f, (T(mesh.node_coords, dtype=F64),), (T(dX),))
+    extra = () if hist is None else (T(hist[0], device=cuda), None if hist[2] else T(hist[1], device=cuda), DT,
+                                     hist[2])
+    with fwAD.dual_level():
+        Xd = fwAD.make_dual(T(mesh.node_coords, dtype=F64, device=cuda), T(dX, device=cuda))
+        geom, st = SC.build_gpu(mesh, bcs, adv, grad, gamma, cuda, node_coords=Xd, transient=hist is not None)
+        out = fwAD.unpack_dual(st.residual(SC.gpu_flux(geom, u))(T(phi, device=cuda), *extra))
+        got = (out.primal.clone(), out.tangent.clone())
+    return ref, got
+
+
+@pytest.mark.parametrize("case", GEO_CASES, ids=_ids)
+def test_scalar_node_coordinate_tangent_matches_oracle(case, cuda):
+    """dR/dX . Xdot of R(phi; geometry(X), flux(geometry(X))) through the dual-number geometry and
+    residual kernels, against torch forward-mode AD of the oracle; then <J Xdot, lambda> against
+    <Xdot, J^T lambda> from the hand-written adjoints (two independent derivations)."""
+    name, adv, grad = case
+    mesh, u, bcs = SC.make(name)
+    phi, gamma = SC.seeded(mesh, 3)
+    rng = np.random.default_rng(11)
+    dX = rng.normal(size=mesh.node_coords.shape)
+    (R_ref, dR_ref), (R, dR) = _node_tangent_pair(mesh, u, bcs, adv, grad, phi, gamma, dX, cuda)
+    assert rel(R, R_ref) < TOL_R
+    err = rel(dR, dR_ref)
+    print("node tangent err", err, float(dR_ref.abs().max()))
+    assert err < TOL_D
+    lam = rng.normal(size=mesh.n_cells)
+    X = torch.tensor(mesh.node_coords, dtype=F64, device=cuda, requires_grad=True)
+    geom, st = SC.build_gpu(mesh, bcs, adv, grad, gamma, cuda, node_coords=X)
+    Rg = st.residual(SC.gpu_flux(geom, u))(torch.tensor(phi, device=cuda))
+    (gx,) = torch.autograd.grad((Rg * torch.tensor(lam, device=cuda)).sum(), X)
+    a = float((dR.cpu() * torch.tensor(lam)).sum())
+    b = float((gx.cpu() * torch.tensor(dX)).sum())
+    scale = float(dR.abs().sum()) * float(np.abs(lam).max())
+    print("duality", a, b)
+    assert abs(a - b) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("first", [True, False], ids=["bdf1", "bdf2"])
+def test_scalar_transient_node_coordinate_tangent_matches_oracle(first, cuda):
+    """The node-coordinate tangent with the accumulation term V (phi - history) / dt, whose volume
+    moves with the nodes."""
+    mesh, u, bcs = SC.make("tet3")
+    phi, gamma = SC.seeded(mesh, 4)
+    old, older = SC.history(mesh, 4)
+    dX = np.random.default_rng(12).normal(size=mesh.node_coords.shape)
+    (R_ref, dR_ref), (R, dR) = _node_tangent_pair(mesh, u, bcs, "limited", "gg", phi, gamma, dX, cuda,
+                                                  hist=(old, older, first))
+    assert rel(R, R_ref) < TOL_R
+    err = rel(dR, dR_ref)
+    print("transient node tangent err", err)
+    assert err < TOL_D
+
+
+def test_geometry_tangent_matches_oracle(cuda):
+    """Tangents of all five geometry arrays along a node-coordinate direction (tet, quad and hex
+    meshes) against torch forward-mode AD of the oracle's geometry."""
+    for name in ("tet3", "quad_mixed", "brick3d"):
+        mesh, _, _ = SC.make(name)
+        dX = np.random.default_rng(13).normal(size=mesh.node_coords.shape)
+
+        def f(Xo):
+            g = fv.geometry(fv.OMesh.from_mesh(mesh, node_coords=Xo))
+            return g.area, g.normal, g.face_centroid, g.volume, g.cell_centroid
+
+        vals, tans = torch.func.jvp(f, (torch.tensor(mesh.node_coords, dtype=F64),), (torch.tensor(dX),))
+        with fwAD.dual_level():
+            Xd = fwAD.make_dual(torch.tensor(mesh.node_coords, dtype=F64, device=cuda), torch.tensor(dX, device=cuda))
+            geom = mesh.geometry(node_coords=Xd, device=cuda)
+            arrs = (geom.face.area, geom.face.normal, geom.face.centroid, geom.cell.volume, geom.cell.centroid)
+            got = [fwAD.unpack_dual(a) for a in arrs]
+            got = [(p.primal.clone(), p.tangent.clone()) for p in got]
+        for label, (pv, pt), v, t in zip(("area", "normal", "face centroid", "volume", "cell centroid"), got, vals,
+                                         tans):
+            ev, et = rel(pv, v), rel(pt, t)
+            print(name, label, "value err", ev, "tangent err", et)
+            assert ev < TOL_R and et < TOL_D, (name, label)
+
+
+def test_node_coordinate_tangent_refused_for_corrected_green_gauss(cuda):
+    mesh, u, bcs = SC.make("quad_mixed")
+    phi, gamma = SC.seeded(mesh, 0)
+    with fwAD.dual_level():
+        X = torch.tensor(mesh.node_coords, dtype=F64, device=cuda)
+        Xd = fwAD.make_dual(X, torch.ones_like(X))
+        geom, st = SC.build_gpu(mesh, bcs, "limited", "cgg", gamma, cuda, node_coords=Xd)
+        fn = st.residual(SC.gpu_flux(geom, u))
+        with pytest.raises(NotImplementedError, match="corrected Green-Gauss"):
+            fn(torch.tensor(phi, device=cuda))
 
 
 # --- transient (BDF1 first step, BDF2 afterwards) ----------------------------------------------
