@@ -350,6 +350,75 @@ AFX_DEV void load_face_geo_pts(const MeshView& m, int32_t f, bool interior, cons
   for (int j = 0; j < D; ++j) q.skew[j] = q.xf[j] - (q.xP[j] + q.dvec[j] * q.g);
 }
 
+// Geometry tangents in the view's face order: face area / normal / centroid [F] / [d][F] and cell
+// volume / centroid [C] / [d][C].
+struct GeoTan {
+  const double *area, *nrm, *fc, *vol, *cc;
+};
+
+template <int D>
+AFX_DEV void cell_centroid_g(const MeshView& m, const GeoTan& t, int32_t c, dual (&x)[D]) {
+#pragma unroll
+  for (int j = 0; j < D; ++j) x[j] = dual{m.cc[(int64_t)j * m.C + c], t.cc[(int64_t)j * m.C + c]};
+}
+
+template <int D>
+AFX_DEV void load_face_geo_g(const MeshView& m, const GeoTan& t, int32_t f, bool interior, const dual* xP,
+                             const dual* xN, FaceGeo<D, dual>& q) {
+  q.A = dual{m.area[f], t.area[f]};
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    q.n[j] = dual{m.nrm[(int64_t)j * m.F + f], t.nrm[(int64_t)j * m.F + f]};
+    q.xf[j] = dual{m.fc[(int64_t)j * m.F + f], t.fc[(int64_t)j * m.F + f]};
+    q.xP[j] = xP[j];
+    q.xN[j] = interior ? xN[j] : xP[j];
+  }
+  finish_face_geo<D, dual>(interior, q);
+}
+
+// Reads the view's geometry as FP64 values (GEO = false) or as (value, tangent) pairs with the
+// tangents of `t` (GEO = true): one kernel body serves the residual and its node-coordinate JVP.
+template <int D, bool GEO>
+struct GeoRead;
+
+template <int D>
+struct GeoRead<D, false> {
+  using T = double;
+  AFX_DEV static double vol(const MeshView& m, const GeoTan&, int32_t c) { return m.vol[c]; }
+  AFX_DEV static double cc(const MeshView& m, const GeoTan&, int j, int32_t c) { return m.cc[(int64_t)j * m.C + c]; }
+  AFX_DEV static double fc(const MeshView& m, const GeoTan&, int j, int32_t f) { return m.fc[(int64_t)j * m.F + f]; }
+  AFX_DEV static double nrm(const MeshView& m, const GeoTan&, int j, int32_t f) { return m.nrm[(int64_t)j * m.F + f]; }
+  AFX_DEV static double area(const MeshView& m, const GeoTan&, int32_t f) { return m.area[f]; }
+  AFX_DEV static void face(const MeshView& m, const GeoTan&, int32_t f, int32_t P, int32_t N, FaceGeo<D>& q) {
+    load_face_geo<D>(m, f, P, N, q);
+  }
+};
+
+template <int D>
+struct GeoRead<D, true> {
+  using T = dual;
+  AFX_DEV static dual vol(const MeshView& m, const GeoTan& t, int32_t c) { return dual{m.vol[c], t.vol[c]}; }
+  AFX_DEV static dual cc(const MeshView& m, const GeoTan& t, int j, int32_t c) {
+    const int64_t i = (int64_t)j * m.C + c;
+    return dual{m.cc[i], t.cc[i]};
+  }
+  AFX_DEV static dual fc(const MeshView& m, const GeoTan& t, int j, int32_t f) {
+    const int64_t i = (int64_t)j * m.F + f;
+    return dual{m.fc[i], t.fc[i]};
+  }
+  AFX_DEV static dual nrm(const MeshView& m, const GeoTan& t, int j, int32_t f) {
+    const int64_t i = (int64_t)j * m.F + f;
+    return dual{m.nrm[i], t.nrm[i]};
+  }
+  AFX_DEV static dual area(const MeshView& m, const GeoTan& t, int32_t f) { return dual{m.area[f], t.area[f]}; }
+  AFX_DEV static void face(const MeshView& m, const GeoTan& t, int32_t f, int32_t P, int32_t N, FaceGeo<D, dual>& q) {
+    dual xP[D], xN[D];
+    cell_centroid_g<D>(m, t, P, xP);
+    if (N >= 0) cell_centroid_g<D>(m, t, N, xN);
+    load_face_geo_g<D>(m, t, f, N >= 0, xP, N >= 0 ? xN : xP, q);
+  }
+};
+
 // Decodes one entry of the cell -> face list.
 // `f` indexes the view's face arrays (geometry, per-face workspace); `fu` indexes the caller's
 // per-face arrays (boundary kinds and values, face fluxes and their cotangents).

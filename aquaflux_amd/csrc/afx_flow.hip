@@ -29,17 +29,18 @@ struct FlowArgs {
   double pin_value;
   double bf[3];
   double lim_k;
+  GeoTan gt;  // geometry tangents (read only by the GEO instantiations of the forward passes)
 };
 
-template <int D, class S>
-AFX_DEV void load_state(const FlowArgs<S>& a, int32_t C, int32_t c, FlowCell<D, S>& X) {
+template <int D, class S, class GT>
+AFX_DEV void load_state(const FlowArgs<S>& a, int32_t C, int32_t c, FlowCell<D, S, GT>& X) {
 #pragma unroll
   for (int i = 0; i < D; ++i) X.u[i] = a.x[(int64_t)i * C + c];
   X.p = a.x[(int64_t)D * C + c];
 }
 
-template <int D, class S>
-AFX_DEV void load_grad(const FlowArgs<S>& a, int32_t C, int32_t c, FlowCell<D, S>& X) {
+template <int D, class S, class GT>
+AFX_DEV void load_grad(const FlowArgs<S>& a, int32_t C, int32_t c, FlowCell<D, S, GT>& X) {
   constexpr int NV = D + 1;
 #pragma unroll
   for (int i = 0; i < D; ++i)
@@ -49,14 +50,14 @@ AFX_DEV void load_grad(const FlowArgs<S>& a, int32_t C, int32_t c, FlowCell<D, S
   for (int j = 0; j < D; ++j) X.Gp[j] = a.G[(int64_t)((NV - 1) * D + j) * C + c];
 }
 
-template <int D, class S>
+template <int D, class S, bool GEO = false, class GT = typename GeoRead<D, GEO>::T>
 AFX_DEV void load_cell_full(const FlowArgs<S>& a, const MeshView& m, int32_t c, bool need_ap, bool need_psi,
-                            FlowCell<D, S>& X) {
+                            FlowCell<D, S, GT>& X) {
   load_state<D, S>(a, m.C, c, X);
   load_grad<D, S>(a, m.C, c, X);
   X.mu = a.mu[c];
   X.rho = a.rho[c];
-  X.V = m.vol[c];
+  X.V = GeoRead<D, GEO>::vol(m, a.gt, c);
   X.aP = need_ap ? a.aP[c] : sconst<S>(1.0);
 #pragma unroll
   for (int i = 0; i < D; ++i) X.psi[i] = need_psi ? a.psi[(int64_t)i * m.C + c] : sconst<S>(1.0);
@@ -77,10 +78,13 @@ AFX_DEV S flow_bval(const FlowArgs<S>& a, int kind, int v, int32_t F, int32_t f,
 
 // ---------------------------------------------------------------------------------------------
 // pass G: gradients (+ limiter)
-template <int D, class S, int GRAD, bool LIM>
+// With GEO the geometry is read as (value, tangent) pairs (the node-coordinate JVP; S = dual).
+template <int D, class S, int GRAD, bool LIM, bool GEO = false>
 __global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Out<S> G, Out<S> psi_out,
                                                    int32_t* __restrict__ ext) {
   constexpr int NV = D + 1;
+  using GR = GeoRead<D, GEO>;
+  using GT = typename GR::T;
   int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   S xc[NV];
@@ -91,12 +95,12 @@ __global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Ou
   for (int v = 0; v < NV; ++v)
 #pragma unroll
     for (int j = 0; j < D; ++j) acc[v][j] = sconst<S>(0.0);
-  double M[D][D];
+  GT M[D][D];
   if (GRAD == 1) {
 #pragma unroll
     for (int p = 0; p < D; ++p)
 #pragma unroll
-      for (int q = 0; q < D; ++q) M[p][q] = 0.0;
+      for (int q = 0; q < D; ++q) M[p][q] = sconst<GT>(0.0);
   }
   S mx[D], mn[D];
   int32_t amx[D], amn[D];
@@ -131,9 +135,9 @@ __global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Ou
         }
       }
       if (GRAD == 0) {
-        FaceGeo<D> q;
-        load_face_geo<D>(m, r.f, r.P, r.N, q);
-        const double w = 1.0 - q.g;
+        FaceGeo<D, GT> q;
+        GR::face(m, a.gt, r.f, r.P, r.N, q);
+        const GT w = 1.0 - q.g;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           S phiP = r.own ? xc[v] : xo[v];
@@ -146,10 +150,10 @@ __global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Ou
           }
         }
       } else {
-        double dx[D];
+        GT dx[D];
 #pragma unroll
-        for (int j = 0; j < D; ++j) dx[j] = m.cc[(int64_t)j * m.C + o] - m.cc[(int64_t)j * m.C + c];
-        const double wgt = 1.0 / vdot<D>(dx, dx);
+        for (int j = 0; j < D; ++j) dx[j] = GR::cc(m, a.gt, j, o) - GR::cc(m, a.gt, j, c);
+        const GT wgt = 1.0 / vdotg<D, GT>(dx, dx);
 #pragma unroll
         for (int p = 0; p < D; ++p)
 #pragma unroll
@@ -164,18 +168,18 @@ __global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Ou
     } else {
       const int kind = a.kind[r.fu];
       if (GRAD == 0) {
-        const double A = m.area[r.f];
+        const GT A = GR::area(m, a.gt, r.f);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           S phi = flow_bval<D, S>(a, kind, v, m.F, r.fu, xc);
 #pragma unroll
-          for (int j = 0; j < D; ++j) acc[v][j] += phi * (m.nrm[(int64_t)j * m.F + r.f] * A);
+          for (int j = 0; j < D; ++j) acc[v][j] += phi * (GR::nrm(m, a.gt, j, r.f) * A);
         }
       } else {
-        double dx[D];
+        GT dx[D];
 #pragma unroll
-        for (int j = 0; j < D; ++j) dx[j] = m.fc[(int64_t)j * m.F + r.f] - m.cc[(int64_t)j * m.C + c];
-        const double wgt = 1.0 / vdot<D>(dx, dx);
+        for (int j = 0; j < D; ++j) dx[j] = GR::fc(m, a.gt, j, r.f) - GR::cc(m, a.gt, j, c);
+        const GT wgt = 1.0 / vdotg<D, GT>(dx, dx);
 #pragma unroll
         for (int p = 0; p < D; ++p)
 #pragma unroll
@@ -191,14 +195,14 @@ __global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Ou
   }
   S grad[NV][D];
   if (GRAD == 0) {
-    const double iv = 1.0 / m.vol[c];
+    const GT iv = 1.0 / GR::vol(m, a.gt, c);
 #pragma unroll
     for (int v = 0; v < NV; ++v)
 #pragma unroll
       for (int j = 0; j < D; ++j) grad[v][j] = acc[v][j] * iv;
   } else {
-    double I[D][D];
-    sym_inverse<D>(M, I);
+    GT I[D][D];
+    sym_inverse<D, GT>(M, I);
 #pragma unroll
     for (int v = 0; v < NV; ++v)
 #pragma unroll
@@ -214,24 +218,24 @@ __global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Ou
 #pragma unroll
     for (int j = 0; j < D; ++j) G.set((int64_t)(v * D + j) * m.C + c, grad[v][j]);
   if (LIM) {
-    const double e2 = m.vol[c] * (a.lim_k * a.lim_k * a.lim_k);
-    double xcen[D];
+    const GT e2 = GR::vol(m, a.gt, c) * (a.lim_k * a.lim_k * a.lim_k);
+    GT xcen[D];
 #pragma unroll
-    for (int j = 0; j < D; ++j) xcen[j] = m.cc[(int64_t)j * m.C + c];
+    for (int j = 0; j < D; ++j) xcen[j] = GR::cc(m, a.gt, j, c);
     S lo[D];
     bool first = true;
     for (int32_t k = kb; k < ke; ++k) {
       int32_t e = m.cface[k];
       int32_t f = e >= 0 ? e : ~e;
-      double dxf[D];
+      GT dxf[D];
 #pragma unroll
-      for (int j = 0; j < D; ++j) dxf[j] = m.fc[(int64_t)j * m.F + f] - xcen[j];
+      for (int j = 0; j < D; ++j) dxf[j] = GR::fc(m, a.gt, j, f) - xcen[j];
 #pragma unroll
       for (int i = 0; i < D; ++i) {
         S dm = grad[i][0] * dxf[0];
 #pragma unroll
         for (int j = 1; j < D; ++j) dm = dm + grad[i][j] * dxf[j];
-        S vf = venkat_face<S>(dm, mx[i] - xc[i], mn[i] - xc[i], e2);
+        S vf = venkat_face<S, GT>(dm, mx[i] - xc[i], mn[i] - xc[i], e2);
         if (first || val(vf) < val(lo[i])) lo[i] = vf;
       }
       first = false;
@@ -250,29 +254,31 @@ __global__ void __launch_bounds__(256) k_flow_grad(MeshView m, FlowArgs<S> a, Ou
 
 // ---------------------------------------------------------------------------------------------
 // pass D: momentum diagonal a_P
-template <int D, class S, int ADV>
+template <int D, class S, int ADV, bool GEO = false>
 __global__ void __launch_bounds__(256) k_flow_ap(MeshView m, FlowArgs<S> a, Out<S> aP) {
+  using GR = GeoRead<D, GEO>;
+  using GT = typename GR::T;
   int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
-  FlowCell<D, S> X;
+  FlowCell<D, S, GT> X;
   load_state<D, S>(a, m.C, c, X);
   load_grad<D, S>(a, m.C, c, X);
   X.mu = a.mu[c];
   X.rho = a.rho[c];
-  X.V = m.vol[c];
+  X.V = GR::vol(m, a.gt, c);
   S acc = sconst<S>(0.0);
   for (FaceWalk walk(m, c); walk.more();) {
     FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
-    FaceGeo<D> q;
-    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    FaceGeo<D, GT> q;
+    GR::face(m, a.gt, r.f, r.P, r.N, q);
     if (o >= 0) {
-      FlowCell<D, S> O;
+      FlowCell<D, S, GT> O;
       load_state<D, S>(a, m.C, o, O);
       load_grad<D, S>(a, m.C, o, O);
       O.mu = a.mu[o];
       O.rho = a.rho[o];
-      O.V = m.vol[o];
+      O.V = GR::vol(m, a.gt, o);
       S oc, nc;
       if (r.own)
         ap_interior<D, ADV, S>(q, X, O, oc, nc);
@@ -289,25 +295,27 @@ __global__ void __launch_bounds__(256) k_flow_ap(MeshView m, FlowArgs<S> a, Out<
 
 // ---------------------------------------------------------------------------------------------
 // pass R: residual
-template <int D, class S, int ADV>
+template <int D, class S, int ADV, bool GEO = false>
 __global__ void __launch_bounds__(256) k_flow_res(MeshView m, FlowArgs<S> a, Out<S> R) {
   constexpr int NV = D + 1;
+  using GR = GeoRead<D, GEO>;
+  using GT = typename GR::T;
   int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
-  FlowCell<D, S> X;
-  load_cell_full<D, S>(a, m, c, true, ADV == kAdvLimited, X);
+  FlowCell<D, S, GT> X;
+  load_cell_full<D, S, GEO>(a, m, c, true, ADV == kAdvLimited, X);
   S acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = sconst<S>(0.0);
   for (FaceWalk walk(m, c); walk.more();) {
     FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
-    FaceGeo<D> q;
-    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    FaceGeo<D, GT> q;
+    GR::face(m, a.gt, r.f, r.P, r.N, q);
     S out[NV];
     if (o >= 0) {
-      FlowCell<D, S> O;
-      load_cell_full<D, S>(a, m, o, true, ADV == kAdvLimited, O);
+      FlowCell<D, S, GT> O;
+      load_cell_full<D, S, GEO>(a, m, o, true, ADV == kAdvLimited, O);
       if (r.own)
         flow_flux_interior<D, ADV, S>(q, X, O, out);
       else
@@ -783,22 +791,27 @@ FlowArgs<S> make_args(const afx_flow_config* cfg, const afx_flow_inputs* in, con
   return a;
 }
 
-template <int D, class S, int ADV, int GRAD>
+// GEO: the geometry carries tangents in a.gt (S = dual; not with corrected Green-Gauss).
+template <int D, class S, int ADV, int GRAD, bool GEO = false>
 int run_forward(const afx_mesh* mh, const afx_geometry* g, FlowArgs<S> a, Out<S> R, double* G, double* Gt,
                 double* aP, double* aPt, double* psi, double* psit, int32_t* ext, double* cgg, int sweeps,
                 double relax, hipStream_t s, Out<S> mdot = Out<S>{}) {
+  static_assert(!GEO || (GRAD != kGradCorrected && std::is_same<S, dual>::value),
+                "geometry tangents: dual state, compact Green-Gauss or least squares");
   MeshView m;
   if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
   Out<S> oG{G, Gt}, oP{psi, psit}, oA{aP, aPt};
   constexpr int GK = GRAD == kGradCorrected ? 0 : GRAD;  // the compact pass supplies V^-1 b
   if (ADV == kAdvLimited && GRAD != kGradCorrected)
-    AFX_LAUNCH(kKFlowGrad, (k_flow_grad<D, S, GK, true>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oG, oP, ext);
+    AFX_LAUNCH(kKFlowGrad, (k_flow_grad<D, S, GK, true, GEO>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oG, oP,
+               ext);
   else
-    AFX_LAUNCH(kKFlowGrad, (k_flow_grad<D, S, GK, false>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oG, oP, ext);
+    AFX_LAUNCH(kKFlowGrad, (k_flow_grad<D, S, GK, false, GEO>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oG, oP,
+               ext);
   AFX_CHECK_LAUNCH();
   a.G.v = G;
   a.G.t = Gt;
-  if (GRAD == kGradCorrected) {
+  if constexpr (GRAD == kGradCorrected) {
     AFX_REQUIRE(cgg, "flow: corrected Green-Gauss needs sweep scratch");
     int rc = cgg_sweeps(m, D, D + 1, G, cgg, sweeps, relax, s);
     if (rc) return rc;
@@ -815,19 +828,21 @@ int run_forward(const afx_mesh* mh, const afx_geometry* g, FlowArgs<S> a, Out<S>
       if (rc) return rc;
     }
   }
-  AFX_LAUNCH(kKFlowAp, (k_flow_ap<D, S, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oA);
+  AFX_LAUNCH(kKFlowAp, (k_flow_ap<D, S, ADV, GEO>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oA);
   AFX_CHECK_LAUNCH();
   if (R.v == nullptr && mdot.v == nullptr) return 0;
   a.aP.v = aP;
   a.aP.t = aPt;
   a.psi.v = psi;
   a.psi.t = psit;
-  if (mdot.v != nullptr) {
-    AFX_LAUNCH(kKFlowRes, (k_flow_mdot<D, S, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, mdot);
-    AFX_CHECK_LAUNCH();
+  if constexpr (!GEO) {
+    if (mdot.v != nullptr) {
+      AFX_LAUNCH(kKFlowRes, (k_flow_mdot<D, S, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, mdot);
+      AFX_CHECK_LAUNCH();
+    }
   }
   if (R.v == nullptr) return 0;
-  AFX_LAUNCH(kKFlowRes, (k_flow_res<D, S, ADV>), grid_for(m.C), dim3(kBlock), 0, s, m, a, R);
+  AFX_LAUNCH(kKFlowRes, (k_flow_res<D, S, ADV, GEO>), grid_for(m.C), dim3(kBlock), 0, s, m, a, R);
   AFX_CHECK_LAUNCH();
   return 0;
 }
@@ -926,6 +941,42 @@ extern "C" int afx_flow_jvp(const afx_mesh* m, const afx_geometry* g, const afx_
     return run_forward<decltype(Dc)::value, dual, decltype(Ac)::value, decltype(Gc)::value>(
         m, g, a, Out<dual>{residual, residual_tangent}, w.G, w.Gt, w.aP, w.aPt, w.psi, w.psit, nullptr, w.cgg,
         cfg->gradient_sweeps, cfg->gradient_relaxation, s);
+  });
+}
+
+extern "C" int afx_flow_jvp_geometry(const afx_mesh* m, const afx_geometry* g, const afx_geometry* geom_tangent,
+                                     const afx_flow_config* cfg, const afx_flow_inputs* in,
+                                     const afx_flow_inputs* tangent, double* residual, double* residual_tangent,
+                                     void* work, int64_t work_size, hipStream_t s) {
+  int rc = check_common(m, g, cfg, in, work, work_size, 1);
+  if (rc) return rc;
+  AFX_REQUIRE(geom_tangent && tangent && residual && residual_tangent, "afx_flow_jvp_geometry: null argument");
+  AFX_REQUIRE(cfg->gradient != kGradCorrected,
+              "afx_flow_jvp_geometry: a geometry tangent is available for compact Green-Gauss or least-squares "
+              "gradients");
+  AFX_REQUIRE(geom_tangent->area && geom_tangent->normal && geom_tangent->face_centroid && geom_tangent->volume &&
+                  geom_tangent->cell_centroid,
+              "afx_flow_jvp_geometry: the geometry tangent needs all five arrays");
+  AFX_REQUIRE(m->fu == nullptr || geom_tangent->face_pack,
+              "afx_flow_jvp_geometry: this mesh's faces are reordered: geom_tangent->face_pack must hold the face "
+              "tangents in the internal order (afx_geometry_pack of the tangent arrays)");
+  Work w = carve(work, m->dim, m->C, 1);
+  FlowArgs<dual> a = make_args<dual>(cfg, in, tangent);
+  const double* pack = geom_tangent->face_pack;
+  a.gt.area = m->fu ? pack : geom_tangent->area;
+  a.gt.nrm = m->fu ? pack + m->F : geom_tangent->normal;
+  a.gt.fc = m->fu ? pack + (int64_t)(1 + m->dim) * m->F : geom_tangent->face_centroid;
+  a.gt.vol = geom_tangent->volume;
+  a.gt.cc = geom_tangent->cell_centroid;
+  return dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
+    constexpr int D = decltype(Dc)::value, ADV = decltype(Ac)::value, GR = decltype(Gc)::value;
+    if constexpr (GR == kGradCorrected) {
+      return 2;  // refused above
+    } else {
+      return run_forward<D, dual, ADV, GR, true>(m, g, a, Out<dual>{residual, residual_tangent}, w.G, w.Gt, w.aP,
+                                                 w.aPt, w.psi, w.psit, nullptr, w.cgg, cfg->gradient_sweeps,
+                                                 cfg->gradient_relaxation, s);
+    }
   });
 }
 

@@ -23,12 +23,13 @@ namespace afx {
 enum FlowKind : int8_t { kFlowNone = 0, kFlowWall = 1, kFlowInlet = 2, kFlowOutlet = 3 };
 enum AdvKind : int { kAdvNone = 0, kAdvUpwind = 1, kAdvLimited = 2 };
 
-// Flow data of one cell as seen by a face.
-template <int D, class S>
+// Flow data of one cell as seen by a face; the volume has the geometry's number type GT (dual
+// when the geometry carries a node-coordinate tangent).
+template <int D, class S, class GT = double>
 struct FlowCell {
   S u[D], p, Gu[D][D], Gp[D], aP, mu, rho;
   S psi[D];
-  double V;
+  GT V;
 };
 
 // LDS copy of the flow data of a workgroup's 256 cells (and their centroids).  FULL adds the
@@ -115,10 +116,12 @@ struct FlowCellBar {
 // ---------------------------------------------------------------------------------------------
 // forward pieces
 
-// (rho u)_f . n at the integration point (interior face, gradient-reconstructed).
-template <int D, class S>
-AFX_DEV S mom_normal_interior(const FaceGeo<D>& q, const FlowCell<D, S>& P, const FlowCell<D, S>& N) {
-  const double w = 1.0 - q.g, g = q.g;
+// (rho u)_f . n at the integration point (interior face, gradient-reconstructed).  The forward
+// pieces take the geometry's number type GT as a template parameter (double, or dual for the
+// node-coordinate JVP, where S is dual too).
+template <int D, class S, class GT = double>
+AFX_DEV S mom_normal_interior(const FaceGeo<D, GT>& q, const FlowCell<D, S, GT>& P, const FlowCell<D, S, GT>& N) {
+  const GT w = 1.0 - q.g, g = q.g;
   S mn;
 #pragma unroll
   for (int i = 0; i < D; ++i) {
@@ -133,8 +136,8 @@ AFX_DEV S mom_normal_interior(const FaceGeo<D>& q, const FlowCell<D, S>& P, cons
 }
 
 // Boundary face: owner value extrapolated to the face (g = 0, skew = xf - xP).
-template <int D, class S>
-AFX_DEV S mom_normal_boundary(const FaceGeo<D>& q, const FlowCell<D, S>& P) {
+template <int D, class S, class GT = double>
+AFX_DEV S mom_normal_boundary(const FaceGeo<D, GT>& q, const FlowCell<D, S, GT>& P) {
   S mn;
 #pragma unroll
   for (int i = 0; i < D; ++i) {
@@ -157,11 +160,11 @@ AFX_DEV double nsq(const FaceGeo<D>& q) {
 }
 
 // Interior face flux: out[i] = owner-outward momentum flux of u_i, out[D] = mass flux.
-template <int D, int ADV, class S>
-AFX_DEV void flow_flux_interior(const FaceGeo<D>& q, const FlowCell<D, S>& P, const FlowCell<D, S>& N,
+template <int D, int ADV, class S, class GT = double>
+AFX_DEV void flow_flux_interior(const FaceGeo<D, GT>& q, const FlowCell<D, S, GT>& P, const FlowCell<D, S, GT>& N,
                                 S out[D + 1]) {
-  const double w = 1.0 - q.g, g = q.g;
-  S mn = mom_normal_interior<D, S>(q, P, N);
+  const GT w = 1.0 - q.g, g = q.g;
+  S mn = mom_normal_interior<D, S, GT>(q, P, N);
   S rho_f = w * P.rho + g * N.rho;
   S gpf[D];
 #pragma unroll
@@ -219,8 +222,8 @@ AFX_DEV void flow_flux_interior(const FaceGeo<D>& q, const FlowCell<D, S>& P, co
 
 // Boundary face flux.  ub / pb are the closure's prescribed velocity / pressure on this face,
 // gB the face's momentum diffusion coefficient (owner mu_eff, or a wall-function override).
-template <int D, int ADV, class S>
-AFX_DEV void flow_flux_boundary(const FaceGeo<D>& q, int kind, const FlowCell<D, S>& P, const S ub[D],
+template <int D, int ADV, class S, class GT = double>
+AFX_DEV void flow_flux_boundary(const FaceGeo<D, GT>& q, int kind, const FlowCell<D, S, GT>& P, const S ub[D],
                                 S pb, S gB, S out[D + 1]) {
   S ubf[D];
   S pf;
@@ -286,12 +289,13 @@ AFX_DEV void flow_flux_boundary(const FaceGeo<D>& q, int kind, const FlowCell<D,
 }
 
 // Momentum-diagonal contributions of an interior face to its owner (oc) and neighbour (nc).
-template <int D, int ADV, class S>
-AFX_DEV void ap_interior(const FaceGeo<D>& q, const FlowCell<D, S>& P, const FlowCell<D, S>& N, S& oc, S& nc) {
+template <int D, int ADV, class S, class GT = double>
+AFX_DEV void ap_interior(const FaceGeo<D, GT>& q, const FlowCell<D, S, GT>& P, const FlowCell<D, S, GT>& N, S& oc,
+                         S& nc) {
   S denom = q.dpn - (P.mu / N.mu) * q.dnn;
   S visc = P.mu * q.A / denom;
   if (ADV != kAdvNone) {
-    S mest = mom_normal_interior<D, S>(q, P, N) * q.A;
+    S mest = mom_normal_interior<D, S, GT>(q, P, N) * q.A;
     oc = visc + dmax(mest, 0.0);
     nc = visc + dmax(-mest, 0.0);
   } else {
@@ -301,12 +305,12 @@ AFX_DEV void ap_interior(const FaceGeo<D>& q, const FlowCell<D, S>& P, const Flo
 }
 
 // Owner momentum-diagonal contribution of a boundary face (per-patch coefficient).
-template <int D, int ADV, class S>
-AFX_DEV S ap_boundary(const FaceGeo<D>& q, int kind, const FlowCell<D, S>& P, S gB) {
+template <int D, int ADV, class S, class GT = double>
+AFX_DEV S ap_boundary(const FaceGeo<D, GT>& q, int kind, const FlowCell<D, S, GT>& P, S gB) {
   if (kind == kFlowNone) return sconst<S>(0.0);
   S visc = gB * q.A / q.dist;
   S conv = sconst<S>(0.0);
-  if (ADV != kAdvNone) conv = dmax(mom_normal_boundary<D, S>(q, P) * q.A, 0.0);
+  if (ADV != kAdvNone) conv = dmax(mom_normal_boundary<D, S, GT>(q, P) * q.A, 0.0);
   if (kind == kFlowWall) return visc;
   if (kind == kFlowInlet) return visc + conv;
   return conv;  // outlet

@@ -278,32 +278,6 @@ __global__ void __launch_bounds__(256) k_sc_res(MeshView m, ScArgs<S> a, Out<S> 
 // passes G and R with a geometry tangent (the node-coordinate JVP): the same face physics on `dual`
 // geometry.  Used only when a geometry tangent is requested; no LDS tiles.
 
-// Geometry tangents in the view's face order: face area / normal / centroid [F] / [d][F] and cell
-// volume / centroid [C] / [d][C].
-struct GeoTan {
-  const double *area, *nrm, *fc, *vol, *cc;
-};
-
-template <int D>
-AFX_DEV void cell_centroid_g(const MeshView& m, const GeoTan& t, int32_t c, dual (&x)[D]) {
-#pragma unroll
-  for (int j = 0; j < D; ++j) x[j] = dual{m.cc[(int64_t)j * m.C + c], t.cc[(int64_t)j * m.C + c]};
-}
-
-template <int D>
-AFX_DEV void load_face_geo_g(const MeshView& m, const GeoTan& t, int32_t f, bool interior, const dual* xP,
-                             const dual* xN, FaceGeo<D, dual>& q) {
-  q.A = dual{m.area[f], t.area[f]};
-#pragma unroll
-  for (int j = 0; j < D; ++j) {
-    q.n[j] = dual{m.nrm[(int64_t)j * m.F + f], t.nrm[(int64_t)j * m.F + f]};
-    q.xf[j] = dual{m.fc[(int64_t)j * m.F + f], t.fc[(int64_t)j * m.F + f]};
-    q.xP[j] = xP[j];
-    q.xN[j] = interior ? xN[j] : xP[j];
-  }
-  finish_face_geo<D, dual>(interior, q);
-}
-
 template <int D, int GRAD, bool LIM>
 __global__ void __launch_bounds__(256) k_sc_grad_g(MeshView m, GeoTan gt, ScArgs<dual> a, Out<dual> G,
                                                    Out<dual> psi_out) {

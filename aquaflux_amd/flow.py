@@ -17,6 +17,7 @@ from typing import NamedTuple
 
 import numpy as np
 import torch
+import torch.autograd.forward_ad as fwAD
 
 from . import _lib
 from .boundary import BoundaryConditions
@@ -224,6 +225,25 @@ class _Raw:
         )
         return bars + ([gbb] if gb is not None else [])
 
+    def jvp_geometry(self, primals, tangents, dgeo):
+        """(R, dR) along tangents of the inputs and of the five geometry arrays together."""
+        state = primals[0]
+        R = torch.empty_like(state)
+        dR = torch.empty_like(state)
+        w = self.dm.workspace("flow1", self.work_bytes[1])
+        ins = self.inputs(*primals)
+        tns = self.inputs(*tangents)
+        gt = _geometry_tangent(self.dm, dgeo)
+        _lib.check(
+            _lib.lib().afx_flow_jvp_geometry(
+                self.dm.handle, ctypes.byref(self.gs), ctypes.byref(gt[0]), ctypes.byref(self.cfg), ctypes.byref(ins),
+                ctypes.byref(tns), R.data_ptr(), dR.data_ptr(), w.data_ptr(), w.numel(),
+                _lib.stream_of(self.dm.device),
+            ),
+            "afx_flow_jvp_geometry",
+        )
+        return R, dR
+
     def mass_flux(self, primals, tangents=None):
         """Face mass flux ``(F,)`` (caller face order) and, with tangents, its tangent."""
         state = primals[0]
@@ -262,32 +282,58 @@ class _Raw:
 
 
 class _FlowResidualFn(torch.autograd.Function):
-    """R(state, mu, rho, u_b, p_b[, gamma_b]) with HIP forward, tangent and adjoint."""
+    """R(state, mu, rho, u_b, p_b[, gamma_b]; area, normal, face centroid, volume, cell centroid)
+    with HIP forward, tangent and adjoint.  The geometry arrays are inputs so that derivatives
+    through them are either computed (forward mode, ``geo_tan``) or refused, never dropped."""
 
     @staticmethod
-    def forward(ctx, raw, state, mu, rho, ub, pb, gb):
+    def forward(ctx, raw, geo_tan, state, mu, rho, ub, pb, gb, area, nrm, fc, vol, cc):
         ctx.raw = raw
         ctx.has_gb = gb is not None
+        ctx.geo_tan = geo_tan
         prim = (state, mu, rho, ub, pb) + ((gb,) if gb is not None else ())
         ctx.save_for_backward(*prim)
-        ctx.save_for_forward(*prim)
+        ctx.save_for_forward(*prim, area, nrm, fc, vol, cc)
         return raw.residual(state, mu, rho, ub, pb, gb)
 
     @staticmethod
     def backward(ctx, gR):
+        if any(ctx.needs_input_grad[8:13]):
+            raise NotImplementedError("reverse-mode derivatives of the flow residual with respect to the geometry "
+                                      "(node coordinates) are not provided; detach the geometry, or use forward "
+                                      "mode")
         prim = ctx.saved_tensors
         bars = ctx.raw.vjp(prim, gR.contiguous())
         gbb = bars[5] if ctx.has_gb else None
-        return (None, bars[0], bars[1], bars[2], bars[3], bars[4], gbb)
+        return (None, None, bars[0], bars[1], bars[2], bars[3], bars[4], gbb) + (None,) * 5
 
     @staticmethod
-    def jvp(ctx, _raw_t, dstate, dmu, drho, dub, dpb, dgb):
-        prim = ctx.saved_tensors
+    def jvp(ctx, _raw_t, _geo_tan_t, dstate, dmu, drho, dub, dpb, dgb, *dgeo):
+        saved = ctx.saved_tensors
+        prim, geo = saved[:-5], saved[-5:]
         tans = [t.contiguous() if t is not None else None for t in (dstate, dmu, drho, dub, dpb)]
         if ctx.has_gb:
             tans.append(dgb.contiguous() if dgb is not None else None)
-        _, dR = ctx.raw.jvp(prim, tans)
+        if ctx.geo_tan:
+            dgeo = [torch.zeros_like(g) if t is None else t for g, t in zip(geo, dgeo)]
+            _, dR = ctx.raw.jvp_geometry(prim, tans, dgeo)
+        else:  # constant geometry: torch fills its tangents with zeros, which are not read
+            _, dR = ctx.raw.jvp(prim, tans)
         return dR
+
+
+def _geometry_tangent(dm, dgeo):
+    """The C struct of geometry tangents (area, normal, face centroid, volume, cell centroid), with
+    the face tangents packed in the kernels' internal face order when the mesh reorders faces.
+    Returns (struct, tensors kept alive)."""
+    dgeo = [t.contiguous() for t in dgeo]
+    pack = None
+    if dm.face_permuted:
+        pack = torch.empty((2 * dm.dim + 1) * dm.n_faces, dtype=torch.float64, device=dm.device)
+        fs = _lib.AfxGeometry(*[t.data_ptr() for t in dgeo[:3]], None, None, None)
+        _lib.check(_lib.lib().afx_geometry_pack(dm.handle, ctypes.byref(fs), pack.data_ptr(),
+                                                _lib.stream_of(dm.device)), "afx_geometry_pack")
+    return _lib.AfxGeometry(*[t.data_ptr() for t in dgeo], _lib.ptr(pack)), (dgeo, pack)
 
 
 class _MassFluxFn(torch.autograd.Function):
@@ -462,11 +508,23 @@ class MomentumContinuity:
         return mu_eff.contiguous(), rho.contiguous(), ub, pb, gb
 
     # --- residual --------------------------------------------------------------------------------
+    def _geometry(self):
+        g = self.geometry
+        return (g.face.area, g.face.normal_soa, g.face.centroid_soa, g.cell.volume, g.cell.centroid_soa)
+
     def residual(self, state):
-        """Coupled residual, same shape as ``state`` (``flow/momentum.py:880-882``)."""
+        """Coupled residual, same shape as ``state`` (``flow/momentum.py:880-882``).  Forward-mode
+        derivatives also follow geometry tangents (node coordinates that carry one) for compact
+        Green-Gauss or least-squares gradients."""
         _lib.require_cuda(state, "state")
         mu, rho, ub, pb, gb = self._inputs()
-        return _FlowResidualFn.apply(self._raw, state.contiguous(), mu, rho, ub, pb, gb)
+        geo = self._geometry()
+        geo_tan = any(fwAD.unpack_dual(t).tangent is not None for t in geo)
+        if geo_tan and self._raw.cfg.gradient == 3:
+            raise NotImplementedError("forward-mode derivatives with respect to the geometry are provided for "
+                                      "compact Green-Gauss or least-squares gradients, not the corrected "
+                                      "Green-Gauss scheme")
+        return _FlowResidualFn.apply(self._raw, geo_tan, state.contiguous(), mu, rho, ub, pb, gb, *geo)
 
     __call__ = residual
 

@@ -21,6 +21,7 @@ import math
 
 import numpy as np
 import torch
+import torch.autograd.forward_ad as fwAD
 
 from . import _lib
 from .boundary import BC_DIRICHLET, BC_ZERO_GRADIENT, BoundaryConditions, Dirichlet, ZeroGradient
@@ -376,10 +377,12 @@ class _RansRaw:
 
 
 class _RansResidualFn(torch.autograd.Function):
-    """R(state, mu, rho, nu, u_b, p_b) with HIP forward, tangent and adjoint."""
+    """R(state, mu, rho, nu, u_b, p_b; geometry arrays) with HIP forward, tangent and adjoint.  The
+    geometry arrays are inputs only so that a derivative through them is refused (the SST residual
+    takes the geometry as fixed), never silently dropped."""
 
     @staticmethod
-    def forward(ctx, raw, state, mu, rho, nu, ub, pb):
+    def forward(ctx, raw, state, mu, rho, nu, ub, pb, *geo):
         ctx.raw = raw
         prim = (state, mu, rho, nu, ub, pb)
         ctx.save_for_backward(*prim)
@@ -388,12 +391,15 @@ class _RansResidualFn(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gR):
+        if any(ctx.needs_input_grad[7:]):
+            raise NotImplementedError("derivatives of the RANS residual with respect to the geometry (node "
+                                      "coordinates) are not provided; detach the geometry")
         bars = ctx.raw.vjp(ctx.saved_tensors, gR.contiguous())
-        return (None, *bars)
+        return (None, *bars) + (None,) * (len(ctx.needs_input_grad) - 7)
 
     @staticmethod
     def jvp(ctx, _raw_t, *tans):
-        tans = [t.contiguous() if t is not None else None for t in tans]
+        tans = [t.contiguous() if t is not None else None for t in tans[:6]]
         _, dR = ctx.raw.jvp(ctx.saved_tensors, tans)
         return dR
 
@@ -475,7 +481,12 @@ class CoupledRANS:
         """``R(u, p, k, omega)``, same shape as ``state`` (``turbulence/coupled.py:437-473``)."""
         _lib.require_cuda(state, "state")
         mu, rho, nu, ub, pb = self._inputs()
-        return _RansResidualFn.apply(self._raw, state.contiguous(), mu, rho, nu, ub, pb)
+        g = self.momentum.geometry
+        geo = (g.face.area, g.face.normal_soa, g.face.centroid_soa, g.cell.volume, g.cell.centroid_soa)
+        if any(fwAD.unpack_dual(t).tangent is not None for t in geo):
+            raise NotImplementedError("derivatives of the RANS residual with respect to the geometry (node "
+                                      "coordinates) are not provided; build it on a geometry without tangents")
+        return _RansResidualFn.apply(self._raw, state.contiguous(), mu, rho, nu, ub, pb, *geo)
 
     __call__ = residual
 

@@ -184,6 +184,18 @@ int afx_flow_jvp(const afx_mesh* mesh, const afx_geometry* geom, const afx_flow_
                  const afx_flow_inputs* in, const afx_flow_inputs* tangent, double* residual,
                  double* residual_tangent, void* work, int64_t work_bytes, hipStream_t stream);
 
+/* R and dR along tangents of the inputs and of the geometry together: geom_tangent holds the
+ * tangents of face area, normal, face centroid, volume and cell centroid (e.g. from
+ * afx_geometry_jvp of a node-coordinate tangent), and face_pack their face arrays in the internal
+ * order when the mesh reorders faces (afx_geometry_pack of the tangent arrays).  Compact
+ * Green-Gauss or least-squares gradients.  Replaces jax.jvp of MomentumContinuity.residual with
+ * Mesh.node_coords as a differentiable leaf (reference: aquaflux/mesh/mesh.py:63-66,
+ * aquaflux/flow/momentum.py:822-882). */
+int afx_flow_jvp_geometry(const afx_mesh* mesh, const afx_geometry* geom, const afx_geometry* geom_tangent,
+                          const afx_flow_config* cfg, const afx_flow_inputs* in, const afx_flow_inputs* tangent,
+                          double* residual, double* residual_tangent, void* work, int64_t work_bytes,
+                          hipStream_t stream);
+
 /* Cotangents J^T . cotangent with respect to state, mu, rho, u_b, p_b (gamma_b).  Replaces jax.vjp
  * of the residual (reference: aquaflux/solve/implicit.py:733-761). */
 int afx_flow_vjp(const afx_mesh* mesh, const afx_geometry* geom, const afx_flow_config* cfg,

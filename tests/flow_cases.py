@@ -57,9 +57,10 @@ def flow_bcs(name, dim, inlet_velocity=None):
     return bcs, None
 
 
-def build(name, scheme="upwind", gradient="gg", mu=0.01, rho=1.0, device="cuda", inlet_velocity=None):
+def build(name, scheme="upwind", gradient="gg", mu=0.01, rho=1.0, device="cuda", inlet_velocity=None,
+          node_coords=None):
     mesh = make_mesh(name)
-    geom = mesh.geometry(device=device)
+    geom = mesh.geometry(node_coords=node_coords, device=device)
     bcs, pin = flow_bcs(name, mesh.dim, inlet_velocity)
     adv = {"none": None, "upwind": FirstOrderUpwind(), "limited": LimitedUpwind(VenkatakrishnanLimiter(5.0))}[scheme]
     gs = gradient_scheme(gradient)

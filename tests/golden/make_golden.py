@@ -34,9 +34,11 @@ F64 = torch.float64
 HERE = os.path.dirname(os.path.abspath(__file__))
 DT = 0.037
 
-# name -> (kind, spec)
+# name -> (kind, spec); a flow spec may end with the state seed (default 20)
 FLOW = {
     "flow_cavity_upwind_gg": ("cavity", "upwind", "gg", "cavity"),
+    # the 64 x 64 lid-driven cavity at full size, state seed 0 (U(-1,1) u, v then N(0, 0.1) p)
+    "flow_cavity64_upwind_gg": ("cavity64", "upwind", "gg", "cavity", 0),
     "flow_channel2d_limited_lsq": ("channel2d", "limited", "lsq", "channel"),
     "flow_channel3d_limited_cgg": ("channel3d", "limited", "cgg", "channel"),
 }
@@ -66,12 +68,17 @@ def _np(t):
     return t.detach().cpu().numpy().astype(np.float64)
 
 
+# fixtures above this size are full-size cases, listed here with their cap
+LARGE = {"flow_cavity64_upwind_gg": 1 << 20}
+
+
 def flow_case(spec):
-    name, scheme, gradient, kind = spec
+    name, scheme, gradient, kind = spec[:4]
+    seed = spec[4] if len(spec) > 4 else 20
     mesh, (m, g, s) = FC.oracle_case(name, scheme, gradient)
     C, F, d = mesh.n_cells, mesh.n_faces, mesh.dim
     rng = np.random.default_rng(20)
-    x = torch.tensor(FC.seeded_state(mesh, seed=20, kind=kind))
+    x = torch.tensor(FC.seeded_state(mesh, seed=seed, kind=kind))
     v = torch.tensor(rng.normal(size=x.numel()))
     w = torch.tensor(rng.normal(size=x.numel()))
     mu = torch.full((C,), 0.01, dtype=F64)

@@ -58,9 +58,10 @@ def _oracle_scalar(mesh, bcs):
 
 
 def build(name, flow_scheme="upwind", scalar_scheme="upwind", gradient="gg", nu=1e-2, rho=1.0, device="cuda",
-          k_log=False, omega_log=False):
+          k_log=False, omega_log=False, node_coords=None):
     base, walls, k_in, w_in = CASES[name]
-    mesh, geom, mc = flow_cases.build(base, flow_scheme, gradient, mu=nu * rho, rho=rho, device=device)
+    mesh, geom, mc = flow_cases.build(base, flow_scheme, gradient, mu=nu * rho, rho=rho, device=device,
+                                      node_coords=node_coords)
     adv = {"upwind": FirstOrderUpwind(), "limited": LimitedUpwind(VenkatakrishnanLimiter(5.0))}[scalar_scheme]
     gs = flow_cases.gradient_scheme(gradient)
     kb = _scalar_bcs(mesh, walls, k_in, omega=False)

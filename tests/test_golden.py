@@ -25,7 +25,7 @@ def _blocks(a, nb):
 def test_fixture_exists_and_is_small(name):
     import os
 
-    assert os.path.getsize(MG.path(name)) < 64 * 1024
+    assert os.path.getsize(MG.path(name)) < MG.LARGE.get(name, 64 * 1024)
 
 
 @pytest.mark.parametrize("name", NAMES)

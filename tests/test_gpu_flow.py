@@ -202,3 +202,95 @@ def test_internal_face_order_and_face_pack(cuda):
     flag = ctypes.c_int32(7)
     _lib.check(_lib.lib().afx_mesh_face_order(tdm.handle, ctypes.byref(flag), None, None), "afx_mesh_face_order")
     assert flag.value == 0
+
+
+# --- node coordinates -----------------------------------------------------------------------------
+GEO_CASES = [
+    ("cavity", "upwind", "gg"),
+    ("cavity", "limited", "lsq"),
+    ("channel2d", "upwind", "gg"),
+    ("channel2d", "limited", "lsq"),
+    ("channel3d", "limited", "lsq"),
+    ("channel3d", "upwind", "gg"),
+    ("box3d", "limited", "gg"),
+    ("brick2d", "limited", "lsq"),
+    ("brick3d", "limited", "lsq"),
+]
+
+
+def _kind(name):
+    return "cavity" if name in ("cavity", "box3d") else "channel"
+
+
+@pytest.mark.parametrize("name,scheme,gradient", GEO_CASES)
+def test_node_coordinate_tangent_matches_oracle(cuda, name, scheme, gradient):
+    """dR/dX . Xdot with the geometry recomputed from the node coordinates: the dual-number
+    geometry kernels feeding the flow passes on dual geometry, against torch forward-mode AD of the
+    oracle (1e-11 relative to the largest entry)."""
+    import torch.autograd.forward_ad as fwAD
+
+    from tests.flow_cases import make_mesh
+
+    print("case", name, scheme, gradient, flush=True)
+    mesh = make_mesh(name)
+    state = seeded_state(mesh, seed=4, kind=_kind(name))
+    dX = np.random.default_rng(5).normal(size=mesh.node_coords.shape) * 0.01
+    T = torch.tensor
+    with fwAD.dual_level():
+        Xd = fwAD.make_dual(T(mesh.node_coords, dtype=F64, device=cuda), T(dX, device=cuda))
+        mesh, geom, mc = build(name, scheme, gradient, device=cuda, node_coords=Xd)
+        out = fwAD.unpack_dual(mc.residual(T(state, dtype=F64, device=cuda)))
+        R, dR = out.primal.clone(), out.tangent.clone()
+    s = oracle_setup(mesh, mc, scheme, gradient)
+    C = mesh.n_cells
+    mu, rho = torch.full((C,), 0.01, dtype=F64), torch.ones(C, dtype=F64)
+
+    def f(X):
+        m = fv.OMesh.from_mesh(mesh, node_coords=X)
+        return fv.flow_residual(m, fv.geometry(m), s, T(state, dtype=F64), mu, rho)
+
+    R_ref, dR_ref = torch.func.jvp(f, (T(mesh.node_coords, dtype=F64),), (T(dX),))
+    assert rel_err(R, R_ref) < TOL
+    err = rel_err(dR, dR_ref)
+    print("node tangent err", err, float(dR_ref.abs().max()))
+    assert err < 1e-11
+
+
+def test_node_coordinate_tangent_refused_for_corrected_green_gauss(cuda):
+    import torch.autograd.forward_ad as fwAD
+
+    from tests.flow_cases import make_mesh
+
+    mesh = make_mesh("channel2d")
+    with fwAD.dual_level():
+        X = torch.tensor(mesh.node_coords, dtype=F64, device=cuda)
+        _, _, mc = build("channel2d", "limited", "cgg", device=cuda, node_coords=fwAD.make_dual(X, torch.ones_like(X)))
+        with pytest.raises(NotImplementedError, match="corrected Green-Gauss"):
+            mc.residual(torch.tensor(seeded_state(mesh, 0, "channel"), dtype=F64, device=cuda))
+
+
+def test_geometry_derivatives_are_refused_not_dropped(cuda):
+    """Reverse mode through the flow geometry, and any derivative through the RANS geometry, raise
+    instead of returning a result that leaves the geometry's part out."""
+    import torch.autograd.forward_ad as fwAD
+
+    from tests import rans_cases as RC
+    from tests.flow_cases import make_mesh
+
+    mesh = make_mesh("channel2d")
+    X = torch.tensor(mesh.node_coords, dtype=F64, device=cuda, requires_grad=True)
+    _, _, mc = build("channel2d", "limited", "lsq", device=cuda, node_coords=X)
+    R = mc.residual(torch.tensor(seeded_state(mesh, 0, "channel"), dtype=F64, device=cuda))
+    with pytest.raises(NotImplementedError, match="geometry"):
+        torch.autograd.grad(R.sum(), X)
+    X2 = torch.tensor(mesh.node_coords, dtype=F64, device=cuda, requires_grad=True)
+    rmesh, rans, _ = RC.build("channel2d_sst", "limited", "limited", "lsq", device=cuda, node_coords=X2)
+    state = torch.tensor(RC.seeded_state(rmesh, 0, "channel"), dtype=F64, device=cuda)
+    with pytest.raises(NotImplementedError, match="geometry"):
+        torch.autograd.grad(rans.residual(state).sum(), X2)
+    with fwAD.dual_level():
+        X3 = torch.tensor(mesh.node_coords, dtype=F64, device=cuda)
+        _, rans3, _ = RC.build("channel2d_sst", "limited", "limited", "lsq", device=cuda,
+                               node_coords=fwAD.make_dual(X3, torch.ones_like(X3)))
+        with pytest.raises(NotImplementedError, match="geometry"):
+            rans3.residual(state)

@@ -42,7 +42,7 @@ def _same_mesh(mesh, z):
 def test_flow_matches_golden(name, cuda):
     print("golden", name, flush=True)
     z = _load(name)
-    base, scheme, gradient, _ = MG.FLOW[name]
+    base, scheme, gradient = MG.FLOW[name][:3]
     mesh, geom, mc = FC.build(base, scheme, gradient, device=cuda)
     _same_mesh(mesh, z)
     nb = int(z["n_fields"])
