@@ -258,6 +258,11 @@ _SIGNATURES = {
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxScalarConfig), ctypes.POINTER(AfxScalarInputs),
         ctypes.POINTER(AfxScalarInputs), c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_void_p,
     ],
+    "afx_flow_vjp_geometry": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxFlowConfig), ctypes.POINTER(AfxFlowInputs),
+        c_void_p, ctypes.POINTER(AfxFlowCotangents), ctypes.POINTER(AfxGeometryCotangents), c_void_p, ctypes.c_int64,
+        c_void_p,
+    ],
     "afx_flow_jvp_geometry": [
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxFlowConfig),
         ctypes.POINTER(AfxFlowInputs), ctypes.POINTER(AfxFlowInputs), c_void_p, c_void_p, c_void_p, ctypes.c_int64,

@@ -713,6 +713,299 @@ __global__ void __launch_bounds__(256) k_flow_grad_adj(MeshView m, FlowArgs<doub
 }
 
 // ---------------------------------------------------------------------------------------------
+// geometry cotangents (node-coordinate VJP).  Run after the state adjoint: they read its a_P and
+// limiter cotangents and its gradient cotangents pre-multiplied by the reconstruction operator
+// (y = V^-1 Gbar for Green-Gauss, M^-1 Gbar for least squares).  One thread per cell again; a
+// face's cotangent is evaluated by both of its cells, the owner writes the face arrays and each
+// cell keeps its own centroid and volume parts, so no two threads write one location.
+
+struct FlowGeoBars {
+  double *area, *nrm, *fc;  // [F], [D][F], [D][F] in the caller's face order (written)
+  double *vol, *cc;         // [C], [D][C] (written)
+  double* limx;             // [D][D][C] scratch: limiter face-centroid cotangent at its argmin face
+  int32_t* limf;            // [D][C]    scratch: that face (view index, -1 none)
+};
+
+// Limiter part: psi_i = clip(min_f venkat(G_i . (xf - xc), heads, V K^3), 0, 1) at its first
+// argmin face.  Writes the cell's volume / centroid partials and the argmin face's centroid part.
+template <int D>
+__global__ void __launch_bounds__(256) k_flow_geo_lim(MeshView m, FlowArgs<double> a, const int32_t* __restrict__ ext,
+                                                      const double* __restrict__ psib, FlowGeoBars gb) {
+  int32_t c = xcd_thread_index(m);
+  if (c >= m.C) return;
+  const double V = m.vol[c], k3 = a.lim_k * a.lim_k * a.lim_k;
+  const double e2 = V * k3;
+  double xcen[D], ccb[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    xcen[j] = m.cc[(int64_t)j * m.C + c];
+    ccb[j] = 0.0;
+  }
+  double Vb = 0.0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    double Gi[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) Gi[j] = a.G[(int64_t)(i * D + j) * m.C + c];
+    const double xi = a.x[(int64_t)i * m.C + c];
+    const double mxv = a.x[(int64_t)i * m.C + ext[(int64_t)i * m.C + c]];
+    const double mnv = a.x[(int64_t)i * m.C + ext[(int64_t)(D + i) * m.C + c]];
+    double lo = 0.0;
+    int32_t fmin = -1;
+    for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+      const int32_t e = m.cface[k];
+      const int32_t f = e >= 0 ? e : ~e;
+      double dmf = Gi[0] * (m.fc[f] - xcen[0]);
+#pragma unroll
+      for (int j = 1; j < D; ++j) dmf = dmf + Gi[j] * (m.fc[(int64_t)j * m.F + f] - xcen[j]);
+      const double vf = venkat_face<double>(dmf, mxv - xi, mnv - xi, e2);
+      if (fmin < 0 || vf < lo) {
+        lo = vf;
+        fmin = f;
+      }
+    }
+    const double dclip = (lo > 0.0 && lo < 1.0) ? 1.0 : ((lo == 0.0 || lo == 1.0) ? 0.5 : 0.0);
+    const double vbar = psib[(int64_t)i * m.C + c] * dclip;
+    double xb[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) xb[j] = 0.0;
+    if (vbar != 0.0 && fmin >= 0) {
+      double dxf[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) dxf[j] = m.fc[(int64_t)j * m.F + fmin] - xcen[j];
+      double dmf = Gi[0] * dxf[0];
+#pragma unroll
+      for (int j = 1; j < D; ++j) dmf = dmf + Gi[j] * dxf[j];
+      const dual r_dm = venkat_face<dual>(dual{dmf, 1.0}, dual{mxv - xi, 0.0}, dual{mnv - xi, 0.0}, e2);
+      const dual r_e2 =
+          venkat_face<dual, dual>(dual{dmf, 0.0}, dual{mxv - xi, 0.0}, dual{mnv - xi, 0.0}, dual{e2, 1.0});
+      const double dmb = vbar * r_dm.d;
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        xb[j] = dmb * Gi[j];
+        ccb[j] -= xb[j];
+      }
+      Vb += vbar * r_e2.d * k3;
+    }
+    gb.limf[(int64_t)i * m.C + c] = vbar != 0.0 ? fmin : -1;
+#pragma unroll
+    for (int j = 0; j < D; ++j) gb.limx[(int64_t)(i * D + j) * m.C + c] = xb[j];
+  }
+  gb.vol[c] = Vb;
+#pragma unroll
+  for (int j = 0; j < D; ++j) gb.cc[(int64_t)j * m.C + c] = ccb[j];
+}
+
+// Least-squares fit direction cotangent of one cell's fit (all variables, shared M):
+// dbar = sum_v w (r_v y_v - (y_v . d) G_v) - 2 w^2 (y_v . d) r_v d, r_v = dphi_v - G_v . d.
+template <int D, int NV>
+AFX_DEV void lsq_dir_bar(const double (&d)[D], const double (&dphi)[NV], const double (&G)[NV][D],
+                         const double (&y)[NV][D], double (&db)[D]) {
+  double dd = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) dd += d[j] * d[j];
+  const double w = 1.0 / dd;
+#pragma unroll
+  for (int j = 0; j < D; ++j) db[j] = 0.0;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    double gd = 0.0, yd = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      gd += G[v][j] * d[j];
+      yd += y[v][j] * d[j];
+    }
+    const double r = dphi[v] - gd;
+#pragma unroll
+    for (int j = 0; j < D; ++j) db[j] += w * (r * y[v][j] - yd * G[v][j]) - 2.0 * w * w * yd * r * d[j];
+  }
+}
+
+template <int D, int ADV, int GRAD>
+__global__ void __launch_bounds__(256) k_flow_geo_adj(MeshView m, FlowArgs<double> a, const double* __restrict__ lam,
+                                                      const double* __restrict__ apb, const double* __restrict__ yb,
+                                                      FlowGeoBars gb) {
+  constexpr int NV = D + 1;
+  constexpr bool LIM = ADV == kAdvLimited;
+  int32_t c = xcd_thread_index(m);
+  if (c >= m.C) return;
+  FlowCell<D, double> X;
+  load_cell_full<D, double>(a, m, c, true, LIM, X);
+  double xc[NV], lc[NV], yc[NV][D], Gc[NV][D];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    xc[v] = a.x[(int64_t)v * m.C + c];
+    lc[v] = lam[(int64_t)v * m.C + c];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      yc[v][j] = yb[(int64_t)(v * D + j) * m.C + c];
+      Gc[v][j] = a.G[(int64_t)(v * D + j) * m.C + c];
+    }
+  }
+  if (c == a.pin) lc[D] = 0.0;
+  const double apb_c = apb[c];
+  double Vb = LIM ? gb.vol[c] : 0.0, ccb[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) ccb[j] = LIM ? gb.cc[(int64_t)j * m.C + c] : 0.0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) Vb -= a.bf[i] * lc[i];  // body force - bf V
+  if (GRAD == 0) {  // G = acc / V
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+      for (int j = 0; j < D; ++j) Vb -= yc[v][j] * Gc[v][j];
+  }
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
+    const int32_t o = r.own ? r.N : r.P;
+    FaceGeo<D> q;
+    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    FaceGeoBarF<D> qb;
+    qb.zero();
+    double xfb[D];  // face-centroid parts outside the face functions (fit directions, limiter)
+#pragma unroll
+    for (int j = 0; j < D; ++j) xfb[j] = 0.0;
+    if (o >= 0) {
+      FlowCell<D, double> O;
+      load_cell_full<D, double>(a, m, o, true, LIM, O);
+      double xo[NV], lo[NV], yo[NV][D], Go[NV][D];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        xo[v] = a.x[(int64_t)v * m.C + o];
+        lo[v] = lam[(int64_t)v * m.C + o];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          yo[v][j] = yb[(int64_t)(v * D + j) * m.C + o];
+          Go[v][j] = a.G[(int64_t)(v * D + j) * m.C + o];
+        }
+      }
+      if (o == a.pin) lo[D] = 0.0;
+      const double apb_o = apb[o];
+      const FlowCell<D, double>& P = r.own ? X : O;
+      const FlowCell<D, double>& N = r.own ? O : X;
+      double lf[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) lf[v] = r.own ? lc[v] - lo[v] : lo[v] - lc[v];
+      double VPb = 0.0, VNb = 0.0;
+      geo_adj_flux_interior<D, ADV>(q, P, N, lf, qb, VPb, VNb);
+      geo_adj_ap_interior<D, ADV>(q, P, N, r.own ? apb_c : apb_o, r.own ? apb_o : apb_c, qb);
+      if (GRAD == 0) {  // acc_P += phi_f n A, acc_N -= phi_f n A
+        const double w = 1.0 - q.g;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          const double phP = r.own ? xc[v] : xo[v], phN = r.own ? xo[v] : xc[v];
+          const double phf = w * phP + q.g * phN;
+          double zn = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+            const double z = r.own ? yc[v][j] - yo[v][j] : yo[v][j] - yc[v][j];
+            qb.n[j] += z * phf * q.A;
+            zn += z * q.n[j];
+          }
+          qb.A += zn * phf;
+          qb.g += zn * q.A * (phN - phP);
+        }
+      } else {  // both cells' fits use the direction between the centroids
+        double d[D], dphi[NV], db[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) d[j] = m.cc[(int64_t)j * m.C + o] - m.cc[(int64_t)j * m.C + c];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) dphi[v] = xo[v] - xc[v];
+        lsq_dir_bar<D, NV>(d, dphi, Gc, yc, db);  // own fit: d = x_o - x_c
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          ccb[j] -= db[j];
+          d[j] = -d[j];
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) dphi[v] = -dphi[v];
+        lsq_dir_bar<D, NV>(d, dphi, Go, yo, db);  // the other cell's fit: d = x_c - x_o
+#pragma unroll
+        for (int j = 0; j < D; ++j) ccb[j] += db[j];
+      }
+      fold_face_geo_bar<D>(q, true, qb);
+      Vb += r.own ? VPb : VNb;
+#pragma unroll
+      for (int j = 0; j < D; ++j) ccb[j] += r.own ? qb.xP[j] : qb.xN[j];
+      if (r.own && LIM) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+          const bool hc = gb.limf[(int64_t)i * m.C + c] == r.f, ho = gb.limf[(int64_t)i * m.C + o] == r.f;
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+            if (hc) xfb[j] += gb.limx[(int64_t)(i * D + j) * m.C + c];
+            if (ho) xfb[j] += gb.limx[(int64_t)(i * D + j) * m.C + o];
+          }
+        }
+      }
+    } else {
+      const int kind = a.kind[r.fu];
+      double lf[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) lf[v] = lc[v];
+      double ub[D];
+#pragma unroll
+      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.fu];
+      const double gB = a.gB.v ? a.gB[r.fu] : X.mu;
+      double VPb = 0.0;
+      geo_adj_flux_boundary<D, ADV>(q, kind, X, ub, a.pb[r.fu], gB, lf, qb, VPb);
+      geo_adj_ap_boundary<D, ADV>(q, kind, X, gB, apb_c, qb);
+      double phb[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) phb[v] = flow_bval<D, double>(a, kind, v, m.F, r.fu, xc);
+      if (GRAD == 0) {  // acc_P += phi_b n A
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          double zn = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+            qb.n[j] += yc[v][j] * phb[v] * q.A;
+            zn += yc[v][j] * q.n[j];
+          }
+          qb.A += zn * phb[v];
+        }
+      } else {  // fit direction d = xf - x_c
+        double d[D], dphi[NV], db[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) d[j] = q.xf[j] - m.cc[(int64_t)j * m.C + c];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) dphi[v] = phb[v] - xc[v];
+        lsq_dir_bar<D, NV>(d, dphi, Gc, yc, db);
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          xfb[j] += db[j];
+          ccb[j] -= db[j];
+        }
+      }
+      fold_face_geo_bar<D>(q, false, qb);
+      Vb += VPb;
+#pragma unroll
+      for (int j = 0; j < D; ++j) ccb[j] += qb.xP[j];
+      if (LIM) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+          if (gb.limf[(int64_t)i * m.C + c] == r.f) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) xfb[j] += gb.limx[(int64_t)(i * D + j) * m.C + c];
+          }
+        }
+      }
+    }
+    if (r.own) {
+      gb.area[r.fu] = qb.A;
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        gb.nrm[(int64_t)j * m.F + r.fu] = qb.n[j];
+        gb.fc[(int64_t)j * m.F + r.fu] = qb.xf[j] + xfb[j];
+      }
+    }
+  }
+  gb.vol[c] = Vb;
+#pragma unroll
+  for (int j = 0; j < D; ++j) gb.cc[(int64_t)j * m.C + c] = ccb[j];
+}
+
+// ---------------------------------------------------------------------------------------------
 // host side
 
 struct Work {
@@ -721,14 +1014,17 @@ struct Work {
   double *Gb, *aPb, *psib, *extb;
   int32_t* ext;
   double* cgg;  // corrected Green-Gauss sweep scratch [2 (D+1) D][C]
+  double* limx;  // mode 3: limiter face-centroid cotangents [D][D][C]
+  int32_t* limf; // mode 3: their faces [D][C]
 };
 
 int64_t work_bytes(int D, int64_t C, int mode) {
   const int NV = D + 1;
   int64_t n = (int64_t)(NV * D + 1 + D) * C;  // G, aP, psi
   if (mode == 1) n *= 2;
-  if (mode == 2) n += (int64_t)(NV * D + 1 + D + 2 * D) * C + (int64_t)D * C;  // bars + ext (2D int32)
+  if (mode >= 2) n += (int64_t)(NV * D + 1 + D + 2 * D) * C + (int64_t)D * C;  // bars + ext (2D int32)
   n += (int64_t)2 * NV * D * C;  // corrected Green-Gauss sweep scratch
+  if (mode == 3) n += (int64_t)(D * D + D) * C;  // geometry cotangents: limiter scratch
   return n * 8 + 256;
 }
 
@@ -750,7 +1046,7 @@ Work carve(void* base, int D, int64_t C, int mode) {
     w.psit = p;
     p += (int64_t)D * C;
   }
-  if (mode == 2) {
+  if (mode >= 2) {
     w.Gb = p;
     p += (int64_t)NV * D * C;
     w.aPb = p;
@@ -763,6 +1059,12 @@ Work carve(void* base, int D, int64_t C, int mode) {
     p += (int64_t)D * C;
   }
   w.cgg = p;
+  if (mode == 3) {
+    p += (int64_t)2 * NV * D * C;
+    w.limx = p;
+    p += (int64_t)D * D * C;
+    w.limf = (int32_t*)p;
+  }
   return w;
 }
 
@@ -879,6 +1181,27 @@ int run_vjp(const afx_mesh* mh, const afx_geometry* g, FlowArgs<double> a, const
   return 0;
 }
 
+// The state adjoint, then the geometry cotangents from its intermediates.
+template <int D, int ADV, int GRAD>
+int run_vjp_geometry(const afx_mesh* mh, const afx_geometry* g, FlowArgs<double> a, const double* lam, FlowBars bar,
+                     FlowGeoBars gb, const Work& w, hipStream_t s) {
+  int rc = run_vjp<D, ADV, GRAD>(mh, g, a, lam, bar, w, 1, 1.0, s);
+  if (rc) return rc;
+  MeshView m;
+  if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
+  a.G.v = w.G;
+  a.aP.v = w.aP;
+  a.psi.v = w.psi;
+  if (ADV == kAdvLimited) {
+    AFX_LAUNCH(kKFlowGradAdj, (k_flow_geo_lim<D>), grid_for(m.C), dim3(kBlock), 0, s, m, a, w.ext, bar.psi, gb);
+    AFX_CHECK_LAUNCH();
+  }
+  AFX_LAUNCH(kKFlowResAdj, (k_flow_geo_adj<D, ADV, GRAD>), grid_for(m.C), dim3(kBlock), 0, s, m, a, lam, bar.aP,
+             bar.G, gb);
+  AFX_CHECK_LAUNCH();
+  return 0;
+}
+
 // Dispatch on (dim, advection, gradient) to a templated body.
 template <class Fn>
 int dispatch(int D, int adv, int grad, Fn&& fn) {
@@ -909,7 +1232,7 @@ int check_common(const afx_mesh* m, const afx_geometry* g, const afx_flow_config
 
 extern "C" int afx_flow_workspace_bytes(const afx_mesh* m, int mode, int64_t* bytes) {
   AFX_REQUIRE(m && bytes, "afx_flow_workspace_bytes: null argument");
-  AFX_REQUIRE(mode >= 0 && mode <= 2, "afx_flow_workspace_bytes: mode must be 0, 1 or 2");
+  AFX_REQUIRE(mode >= 0 && mode <= 3, "afx_flow_workspace_bytes: mode must be 0, 1, 2 or 3");
   *bytes = work_bytes(m->dim, m->C, mode);
   return 0;
 }
@@ -995,6 +1318,37 @@ extern "C" int afx_flow_vjp(const afx_mesh* m, const afx_geometry* g, const afx_
   return dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
     return run_vjp<decltype(Dc)::value, decltype(Ac)::value, decltype(Gc)::value>(
         m, g, a, cotangent, bar, w, cfg->gradient_sweeps, cfg->gradient_relaxation, s);
+  });
+}
+
+extern "C" int afx_flow_vjp_geometry(const afx_mesh* m, const afx_geometry* g, const afx_flow_config* cfg,
+                                     const afx_flow_inputs* in, const double* cotangent, afx_flow_cotangents* out,
+                                     afx_geometry_cotangents* geom_out, void* work, int64_t work_size,
+                                     hipStream_t s) {
+  int rc = check_common(m, g, cfg, in, work, work_size, 3);
+  if (rc) return rc;
+  AFX_REQUIRE(cotangent && out && out->state && out->mu && out->rho && out->u_b && out->p_b && geom_out,
+              "afx_flow_vjp_geometry: null cotangent argument");
+  AFX_REQUIRE(in->gamma_b == nullptr || out->gamma_b != nullptr,
+              "afx_flow_vjp_geometry: gamma_b given without a gamma_b cotangent buffer");
+  AFX_REQUIRE(geom_out->area && geom_out->normal && geom_out->face_centroid && geom_out->volume &&
+                  geom_out->cell_centroid,
+              "afx_flow_vjp_geometry: all five geometry cotangent buffers are needed");
+  AFX_REQUIRE(cfg->gradient != kGradCorrected,
+              "afx_flow_vjp_geometry: geometry cotangents are available for compact Green-Gauss or least-squares "
+              "gradients");
+  Work w = carve(work, m->dim, m->C, 3);
+  FlowArgs<double> a = make_args<double>(cfg, in, nullptr);
+  FlowBars bar{out->state, w.Gb, w.aPb, w.psib, out->mu, out->rho, out->u_b, out->p_b, out->gamma_b, w.extb};
+  FlowGeoBars gb{geom_out->area, geom_out->normal, geom_out->face_centroid, geom_out->volume,
+                 geom_out->cell_centroid, w.limx, w.limf};
+  return dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
+    constexpr int D = decltype(Dc)::value, ADV = decltype(Ac)::value, GR = decltype(Gc)::value;
+    if constexpr (GR == kGradCorrected) {
+      return 2;  // refused above
+    } else {
+      return run_vjp_geometry<D, ADV, GR>(m, g, a, cotangent, bar, gb, w, s);
+    }
   });
 }
 

@@ -636,4 +636,326 @@ AFX_DEV void adj_ap_boundary(const FaceGeo<D>& q, int kind, const FlowCell<D, do
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// geometry cotangents (node-coordinate VJP).  Each routine returns the cotangent of the whole face
+// geometry (the same on both sides of the face) given the cotangents of the face's outputs; cell
+// volumes come back separately.  fold_face_geo_bar then reverses finish_face_geo.
+
+template <int D>
+struct FaceGeoBarF {
+  double A, n[D], xf[D], xP[D], xN[D], g, dvec[D], dist, skew[D], dp[D], dn[D], dpn, dnn;
+  AFX_DEV void zero() {
+    A = g = dist = dpn = dnn = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) n[j] = xf[j] = xP[j] = xN[j] = dvec[j] = skew[j] = dp[j] = dn[j] = 0.0;
+  }
+};
+
+// Folds the cotangents of the derived quantities into A, n, xf, xP, xN (xN aliases xP on a
+// boundary face, where its cotangent is moved onto xP).
+template <int D>
+AFX_DEV void fold_face_geo_bar(const FaceGeo<D>& q, bool interior, FaceGeoBarF<D>& b) {
+#pragma unroll
+  for (int j = 0; j < D; ++j) {  // skew = xf - (xP + dvec g)
+    b.xf[j] += b.skew[j];
+    b.xP[j] -= b.skew[j];
+    b.dvec[j] -= b.skew[j] * q.g;
+    b.g -= b.skew[j] * q.dvec[j];
+  }
+  if (interior) {
+    const double dd = vdot<D>(q.dvec, q.dvec);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      b.dvec[j] += b.dist * q.n[j];  // dist = dvec . n
+      b.n[j] += b.dist * q.dvec[j];
+      b.dp[j] += b.g * q.dvec[j] / dd;  // g = (dp . dvec) / (dvec . dvec)
+      b.dvec[j] += b.g * (q.dp[j] - 2.0 * q.g * q.dvec[j]) / dd;
+    }
+  } else {
+    b.dpn += b.dist;  // dist = dpn; g = 0
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    b.dp[j] += b.dpn * q.n[j];
+    b.n[j] += b.dpn * q.dp[j] + b.dnn * q.dn[j];
+    b.dn[j] += b.dnn * q.n[j];
+    b.xf[j] += b.dp[j] + b.dn[j];
+    b.xP[j] -= b.dp[j] + b.dvec[j];
+    b.xN[j] += b.dvec[j] - b.dn[j];
+    if (!interior) {
+      b.xP[j] += b.xN[j];
+      b.xN[j] = 0.0;
+    }
+  }
+}
+
+// mn = sum_i n_i mf_i, mf_i = w rhoP uP_i + g rhoN uN_i + sum_j (w rhoP GuP_ij + g rhoN GuN_ij) skew_j
+template <int D>
+AFX_DEV void geo_adj_mom_normal_interior(const FaceGeo<D>& q, const FlowCell<D, double>& P,
+                                         const FlowCell<D, double>& N, double mn_bar, FaceGeoBarF<D>& b) {
+  const double w = 1.0 - q.g, g = q.g;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    double mf = w * (P.rho * P.u[i]) + g * (N.rho * N.u[i]);
+    double dmf = N.rho * N.u[i] - P.rho * P.u[i];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      mf += (w * (P.rho * P.Gu[i][j]) + g * (N.rho * N.Gu[i][j])) * q.skew[j];
+      dmf += (N.rho * N.Gu[i][j] - P.rho * P.Gu[i][j]) * q.skew[j];
+    }
+    b.n[i] += mn_bar * mf;
+    const double mfb = mn_bar * q.n[i];
+    b.g += mfb * dmf;
+#pragma unroll
+    for (int j = 0; j < D; ++j) b.skew[j] += mfb * (w * (P.rho * P.Gu[i][j]) + g * (N.rho * N.Gu[i][j]));
+  }
+}
+
+template <int D>
+AFX_DEV void geo_adj_mom_normal_boundary(const FaceGeo<D>& q, const FlowCell<D, double>& P, double mn_bar,
+                                         FaceGeoBarF<D>& b) {
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    double mf = P.rho * P.u[i];
+#pragma unroll
+    for (int j = 0; j < D; ++j) mf += (P.rho * P.Gu[i][j]) * q.skew[j];
+    b.n[i] += mn_bar * mf;
+    const double mfb = mn_bar * q.n[i];
+#pragma unroll
+    for (int j = 0; j < D; ++j) b.skew[j] += mfb * (P.rho * P.Gu[i][j]);
+  }
+}
+
+// Geometry cotangent of flow_flux_interior for output cotangents lam[D+1]; VP_bar / VN_bar receive
+// the cell-volume cotangents (the Rhie-Chow V / a_P).
+template <int D, int ADV>
+AFX_DEV void geo_adj_flux_interior(const FaceGeo<D>& q, const FlowCell<D, double>& P, const FlowCell<D, double>& N,
+                                   const double lam[D + 1], FaceGeoBarF<D>& b, double& VP_bar, double& VN_bar) {
+  const double w = 1.0 - q.g, g = q.g;
+  const double mn = mom_normal_interior<D, double>(q, P, N);
+  const double rho_f = w * P.rho + g * N.rho;
+  double gpf[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) gpf[j] = w * P.Gp[j] + g * N.Gp[j];
+  const double dcP = P.V / P.aP, dcN = N.V / N.aP;
+  const double dcf = w * dcP + g * dcN;
+  const double ns = nsq<D>(q);
+  const double dhat = ns * dcf;
+  double gd = 0.0, gs = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    gd += gpf[j] * q.dvec[j];
+    gs += gpf[j] * q.skew[j];
+  }
+  const double damping = ((N.p - P.p) - gd) / q.dist;
+  const double mdot = (mn - rho_f * dhat * damping) * q.A;
+  const double pf = (w * P.p + g * N.p) + gs;
+  const double ratio = P.mu / N.mu;
+  const double denom = q.dpn - ratio * q.dnn;
+  const bool outflow = mdot >= 0.0;
+  double mdot_bar = lam[D], pf_bar = 0.0, denom_bar = 0.0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    const double li = lam[i];
+    if (ADV == kAdvUpwind) {
+      mdot_bar += li * (outflow ? P.u[i] : N.u[i]);
+    } else if (ADV == kAdvLimited) {
+      const FlowCell<D, double>& U = outflow ? P : N;
+      double t = 0.0;
+#pragma unroll
+      for (int j = 0; j < D; ++j) t += U.Gu[i][j] * (q.xf[j] - (outflow ? q.xP[j] : q.xN[j]));
+      mdot_bar += li * (U.u[i] + U.psi[i] * t);
+      const double tb = li * mdot * U.psi[i];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        b.xf[j] += tb * U.Gu[i][j];
+        if (outflow)
+          b.xP[j] -= tb * U.Gu[i][j];
+        else
+          b.xN[j] -= tb * U.Gu[i][j];
+      }
+    }
+    double cp = 0.0, cn = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      cp += P.Gu[i][j] * (q.dp[j] - q.n[j] * q.dpn);
+      cn += N.Gu[i][j] * (q.dn[j] - q.n[j] * q.dnn);
+    }
+    const double qd = (((N.u[i] - P.u[i]) + cn) - cp) / denom;
+    b.A += li * ((-P.mu) * qd + pf * q.n[i]);
+    b.n[i] += li * pf * q.A;
+    pf_bar += li * q.n[i] * q.A;
+    const double qd_bar = li * (-P.mu) * q.A;
+    denom_bar -= qd_bar * qd / denom;
+    const double cb = qd_bar / denom;  // d qd / d cn = 1 / denom, d qd / d cp = -1 / denom
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      b.dp[j] -= cb * P.Gu[i][j];
+      b.n[j] += cb * P.Gu[i][j] * q.dpn;
+      b.dpn += cb * P.Gu[i][j] * q.n[j];
+      b.dn[j] += cb * N.Gu[i][j];
+      b.n[j] -= cb * N.Gu[i][j] * q.dnn;
+      b.dnn -= cb * N.Gu[i][j] * q.n[j];
+    }
+  }
+  b.dpn += denom_bar;
+  b.dnn -= denom_bar * ratio;
+  // pf = w pP + g pN + gpf . skew
+  double gpf_bar[D];
+  b.g += pf_bar * (N.p - P.p);
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    gpf_bar[j] = pf_bar * q.skew[j];
+    b.skew[j] += pf_bar * gpf[j];
+  }
+  // mdot = (mn - rho_f dhat damping) A
+  b.A += mdot_bar * (mn - rho_f * dhat * damping);
+  const double t = mdot_bar * q.A;
+  const double rho_f_bar = -t * dhat * damping;
+  const double dhat_bar = -t * rho_f * damping;
+  const double damping_bar = -t * rho_f * dhat;
+  b.dist -= damping_bar * damping / q.dist;
+  const double gd_bar = -damping_bar / q.dist;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    gpf_bar[j] += gd_bar * q.dvec[j];
+    b.dvec[j] += gd_bar * gpf[j];
+    b.n[j] += dhat_bar * 2.0 * q.n[j] * dcf;
+  }
+  const double dcf_bar = dhat_bar * ns;
+  b.g += dcf_bar * (dcN - dcP);
+  VP_bar += w * dcf_bar / P.aP;
+  VN_bar += g * dcf_bar / N.aP;
+  double gg = rho_f_bar * (N.rho - P.rho);
+#pragma unroll
+  for (int j = 0; j < D; ++j) gg += gpf_bar[j] * (N.Gp[j] - P.Gp[j]);
+  b.g += gg;
+  geo_adj_mom_normal_interior<D>(q, P, N, t, b);
+}
+
+// Geometry cotangent of flow_flux_boundary (owner side); VP_bar receives the owner volume's.
+template <int D, int ADV>
+AFX_DEV void geo_adj_flux_boundary(const FaceGeo<D>& q, int kind, const FlowCell<D, double>& P, const double ub[D],
+                                   double pb, double gB, const double lam[D + 1], FaceGeoBarF<D>& b,
+                                   double& VP_bar) {
+  double ubf[D], pf = 0.0, mdot = 0.0, un = 0.0, gn = 0.0, dh = 0.0, compact = 0.0;
+  const double dcP = P.V / P.aP;
+  if (kind == kFlowWall || kind == kFlowInlet) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) ubf[i] = ub[i];
+    pf = P.p;
+    if (kind == kFlowInlet) {
+#pragma unroll
+      for (int j = 0; j < D; ++j) un += ub[j] * q.n[j];
+      mdot = P.rho * un * q.A;
+    }
+  } else if (kind == kFlowOutlet) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) ubf[i] = P.u[i];
+    pf = pb;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      un += P.u[j] * q.n[j];
+      gn += P.Gp[j] * q.n[j];
+    }
+    dh = nsq<D>(q) * dcP;
+    compact = (pb - P.p) / q.dist;
+    mdot = P.rho * (un - dh * (compact - gn)) * q.A;
+  } else {
+#pragma unroll
+    for (int i = 0; i < D; ++i) ubf[i] = 0.0;
+  }
+  const bool outflow = mdot >= 0.0;
+  double mdot_bar = lam[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    const double li = lam[i];
+    if (ADV == kAdvUpwind) {
+      mdot_bar += li * (outflow ? P.u[i] : ubf[i]);
+    } else if (ADV == kAdvLimited) {
+      if (outflow) {
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) t += P.Gu[i][j] * (q.xf[j] - q.xP[j]);
+        mdot_bar += li * (P.u[i] + P.psi[i] * t);
+        const double tb = li * mdot * P.psi[i];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          b.xf[j] += tb * P.Gu[i][j];
+          b.xP[j] -= tb * P.Gu[i][j];
+        }
+      } else {
+        mdot_bar += li * ubf[i];
+      }
+    }
+    double cp = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) cp += P.Gu[i][j] * (q.dp[j] - q.n[j] * q.dpn);
+    const double qd = ((ubf[i] - P.u[i]) - cp) / q.dpn;
+    b.A += li * ((-gB) * qd + pf * q.n[i]);
+    b.n[i] += li * pf * q.A;
+    const double qd_bar = li * (-gB) * q.A;
+    b.dpn -= qd_bar * qd / q.dpn;
+    const double cb = -qd_bar / q.dpn;  // cotangent of cp
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      b.dp[j] += cb * P.Gu[i][j];
+      b.n[j] -= cb * P.Gu[i][j] * q.dpn;
+      b.dpn -= cb * P.Gu[i][j] * q.n[j];
+    }
+  }
+  if (kind == kFlowInlet) {
+    b.A += mdot_bar * P.rho * un;
+    const double un_bar = mdot_bar * P.rho * q.A;
+#pragma unroll
+    for (int j = 0; j < D; ++j) b.n[j] += un_bar * ub[j];
+  } else if (kind == kFlowOutlet) {
+    const double inner = un - dh * (compact - gn);
+    b.A += mdot_bar * P.rho * inner;
+    const double inner_bar = mdot_bar * P.rho * q.A;
+    const double dh_bar = -inner_bar * (compact - gn);
+    const double cg_bar = -inner_bar * dh;
+    b.dist -= cg_bar * compact / q.dist;
+#pragma unroll
+    for (int j = 0; j < D; ++j) b.n[j] += inner_bar * P.u[j] - cg_bar * P.Gp[j] + dh_bar * 2.0 * q.n[j] * dcP;
+    VP_bar += dh_bar * nsq<D>(q) / P.aP;
+  }
+}
+
+// Geometry cotangent of ap_interior given the cotangents of the owner's and neighbour's a_P.
+template <int D, int ADV>
+AFX_DEV void geo_adj_ap_interior(const FaceGeo<D>& q, const FlowCell<D, double>& P, const FlowCell<D, double>& N,
+                                 double aPbarP, double aPbarN, FaceGeoBarF<D>& b) {
+  const double ratio = P.mu / N.mu;
+  const double denom = q.dpn - ratio * q.dnn;
+  const double visc_bar = aPbarP + aPbarN;
+  b.A += visc_bar * P.mu / denom;
+  const double denom_bar = -visc_bar * P.mu * q.A / (denom * denom);
+  b.dpn += denom_bar;
+  b.dnn -= denom_bar * ratio;
+  if (ADV != kAdvNone) {
+    const double mn = mom_normal_interior<D, double>(q, P, N);
+    const double mest = mn * q.A;
+    const double mest_bar = aPbarP * dmax0_dx(mest) - aPbarN * dmax0_dx(-mest);
+    b.A += mest_bar * mn;
+    geo_adj_mom_normal_interior<D>(q, P, N, mest_bar * q.A, b);
+  }
+}
+
+template <int D, int ADV>
+AFX_DEV void geo_adj_ap_boundary(const FaceGeo<D>& q, int kind, const FlowCell<D, double>& P, double gB,
+                                 double aPbar, FaceGeoBarF<D>& b) {
+  if (kind == kFlowNone) return;
+  if (kind == kFlowWall || kind == kFlowInlet) {  // gB A / dist
+    b.A += aPbar * gB / q.dist;
+    b.dist -= aPbar * gB * q.A / (q.dist * q.dist);
+  }
+  if (ADV != kAdvNone && (kind == kFlowInlet || kind == kFlowOutlet)) {
+    const double mn = mom_normal_boundary<D, double>(q, P);
+    const double mest_bar = aPbar * dmax0_dx(mn * q.A);
+    b.A += mest_bar * mn;
+    geo_adj_mom_normal_boundary<D>(q, P, mest_bar * q.A, b);
+  }
+}
+
 }  // namespace afx

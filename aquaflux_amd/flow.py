@@ -169,7 +169,7 @@ class _Raw:
         self.kind = kind  # keeps the device buffer alive
         L = _lib.lib()
         self.work_bytes = []
-        for mode in range(3):
+        for mode in range(4):
             n = ctypes.c_int64()
             _lib.check(L.afx_flow_workspace_bytes(self.dm.handle, mode, ctypes.byref(n)), "afx_flow_workspace_bytes")
             self.work_bytes.append(int(n.value))
@@ -244,6 +244,29 @@ class _Raw:
         )
         return R, dR
 
+    def vjp_geometry(self, primals, cot):
+        """The cotangents of afx_flow_vjp plus those of the five geometry arrays (area, normal,
+        face centroid, volume, cell centroid)."""
+        state, mu, rho, ub, pb = primals[:5]
+        gb = primals[5] if len(primals) > 5 else None
+        bars = [torch.empty_like(state), torch.empty_like(mu), torch.empty_like(rho), torch.empty_like(ub), torch.empty_like(pb)]
+        gbb = torch.empty_like(gb) if gb is not None else None
+        g = self.geometry
+        gbars = [torch.empty_like(t) for t in (g.face.area, g.face.normal_soa, g.face.centroid_soa, g.cell.volume,
+                                                g.cell.centroid_soa)]
+        w = self.dm.workspace("flow3", self.work_bytes[3])
+        ins = self.inputs(state, mu, rho, ub, pb, gb)
+        outs = _lib.AfxFlowCotangents(*[b.data_ptr() for b in bars], _lib.ptr(gbb))
+        gouts = _lib.AfxGeometryCotangents(*[b.data_ptr() for b in gbars])
+        _lib.check(
+            _lib.lib().afx_flow_vjp_geometry(
+                self.dm.handle, ctypes.byref(self.gs), ctypes.byref(self.cfg), ctypes.byref(ins), cot.data_ptr(),
+                ctypes.byref(outs), ctypes.byref(gouts), w.data_ptr(), w.numel(), _lib.stream_of(self.dm.device),
+            ),
+            "afx_flow_vjp_geometry",
+        )
+        return bars + ([gbb] if gb is not None else []), gbars
+
     def mass_flux(self, primals, tangents=None):
         """Face mass flux ``(F,)`` (caller face order) and, with tangents, its tangent."""
         state = primals[0]
@@ -283,8 +306,9 @@ class _Raw:
 
 class _FlowResidualFn(torch.autograd.Function):
     """R(state, mu, rho, u_b, p_b[, gamma_b]; area, normal, face centroid, volume, cell centroid)
-    with HIP forward, tangent and adjoint.  The geometry arrays are inputs so that derivatives
-    through them are either computed (forward mode, ``geo_tan``) or refused, never dropped."""
+    with HIP forward, tangent and adjoint, including those with respect to the geometry arrays
+    (forward mode when ``geo_tan``; compact Green-Gauss or least squares), which the geometry's own
+    derivatives carry on to the node coordinates."""
 
     @staticmethod
     def forward(ctx, raw, geo_tan, state, mu, rho, ub, pb, gb, area, nrm, fc, vol, cc):
@@ -298,14 +322,17 @@ class _FlowResidualFn(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gR):
-        if any(ctx.needs_input_grad[8:13]):
-            raise NotImplementedError("reverse-mode derivatives of the flow residual with respect to the geometry "
-                                      "(node coordinates) are not provided; detach the geometry, or use forward "
-                                      "mode")
         prim = ctx.saved_tensors
-        bars = ctx.raw.vjp(prim, gR.contiguous())
+        if any(ctx.needs_input_grad[8:13]):
+            if ctx.raw.cfg.gradient == 3:
+                raise NotImplementedError("reverse-mode derivatives of the flow residual with respect to the "
+                                          "geometry are provided for compact Green-Gauss or least-squares gradients, "
+                                          "not the corrected Green-Gauss scheme")
+            bars, gbars = ctx.raw.vjp_geometry(prim, gR.contiguous())
+        else:
+            bars, gbars = ctx.raw.vjp(prim, gR.contiguous()), [None] * 5
         gbb = bars[5] if ctx.has_gb else None
-        return (None, None, bars[0], bars[1], bars[2], bars[3], bars[4], gbb) + (None,) * 5
+        return (None, None, bars[0], bars[1], bars[2], bars[3], bars[4], gbb, *gbars)
 
     @staticmethod
     def jvp(ctx, _raw_t, _geo_tan_t, dstate, dmu, drho, dub, dpb, dgb, *dgeo):

@@ -169,7 +169,8 @@ typedef struct {
   double* gamma_b; /* [F] (required iff inputs.gamma_b is set) */
 } afx_flow_cotangents;
 
-/* Workspace (device bytes) for mode 0 = residual, 1 = JVP, 2 = VJP. */
+/* Workspace (device bytes) for mode 0 = residual, 1 = JVP, 2 = VJP, 3 = VJP with geometry
+ * cotangents. */
 int afx_flow_workspace_bytes(const afx_mesh* mesh, int mode, int64_t* bytes);
 
 /* R(state).  Replaces MomentumContinuity.residual (reference: aquaflux/flow/momentum.py:880-882,
@@ -201,6 +202,15 @@ int afx_flow_jvp_geometry(const afx_mesh* mesh, const afx_geometry* geom, const 
 int afx_flow_vjp(const afx_mesh* mesh, const afx_geometry* geom, const afx_flow_config* cfg,
                  const afx_flow_inputs* in, const double* cotangent, afx_flow_cotangents* out, void* work,
                  int64_t work_bytes, hipStream_t stream);
+
+/* As afx_flow_vjp, and also the cotangents of the five geometry arrays (written; caller face
+ * order), which afx_geometry_vjp carries on to the node coordinates.  Compact Green-Gauss or
+ * least-squares gradients; workspace of mode 3.  Replaces jax.vjp of MomentumContinuity.residual
+ * with Mesh.node_coords as a differentiable leaf (reference: aquaflux/mesh/mesh.py:63-66,
+ * aquaflux/flow/momentum.py:822-882). */
+int afx_flow_vjp_geometry(const afx_mesh* mesh, const afx_geometry* geom, const afx_flow_config* cfg,
+                          const afx_flow_inputs* in, const double* cotangent, afx_flow_cotangents* out,
+                          afx_geometry_cotangents* geom_out, void* work, int64_t work_bytes, hipStream_t stream);
 
 /* Intermediate fields: gradients [(d+1)d][C], a_P [C], psi [d][C] (limited advection only).
  * Replaces MomentumContinuity.flow_fields' gradient / momentum_matrix_diagonal products

@@ -269,20 +269,66 @@ def test_node_coordinate_tangent_refused_for_corrected_green_gauss(cuda):
             mc.residual(torch.tensor(seeded_state(mesh, 0, "channel"), dtype=F64, device=cuda))
 
 
-def test_geometry_derivatives_are_refused_not_dropped(cuda):
-    """Reverse mode through the flow geometry, and any derivative through the RANS geometry, raise
-    instead of returning a result that leaves the geometry's part out."""
+@pytest.mark.parametrize("name,scheme,gradient", GEO_CASES)
+def test_node_coordinate_adjoint_matches_oracle(cuda, name, scheme, gradient):
+    """d <lambda, R(state; geometry(X))> / dX through the hand-written flow adjoint with geometry
+    cotangents (afx_flow_vjp_geometry) and the geometry adjoint, against torch autograd of the
+    oracle (1e-11 relative to the largest entry), and <J Xdot, lambda> against the forward-mode
+    tangent of the same HIP path."""
+    import torch.autograd.forward_ad as fwAD
+
+    from tests.flow_cases import make_mesh
+
+    print("case", name, scheme, gradient, flush=True)
+    mesh = make_mesh(name)
+    state = seeded_state(mesh, seed=4, kind=_kind(name))
+    rng = np.random.default_rng(6)
+    lam = rng.normal(size=state.size)
+    T = torch.tensor
+    X = T(mesh.node_coords, dtype=F64, device=cuda, requires_grad=True)
+    mesh, geom, mc = build(name, scheme, gradient, device=cuda, node_coords=X)
+    R = mc.residual(T(state, dtype=F64, device=cuda))
+    (got,) = torch.autograd.grad((R * T(lam, device=cuda)).sum(), X)
+    s = oracle_setup(mesh, mc, scheme, gradient)
+    C = mesh.n_cells
+    Xo = T(mesh.node_coords, dtype=F64, requires_grad=True)
+    m = fv.OMesh.from_mesh(mesh, node_coords=Xo)
+    Ro = fv.flow_residual(m, fv.geometry(m), s, T(state, dtype=F64), torch.full((C,), 0.01, dtype=F64),
+                          torch.ones(C, dtype=F64))
+    (ref,) = torch.autograd.grad((Ro * T(lam)).sum(), Xo)
+    err = rel_err(got, ref)
+    print("node adjoint err", err, float(ref.abs().max()))
+    assert err < 1e-11
+    dX = rng.normal(size=mesh.node_coords.shape) * 0.01
+    with fwAD.dual_level():
+        Xd = fwAD.make_dual(T(mesh.node_coords, dtype=F64, device=cuda), T(dX, device=cuda))
+        _, _, mcd = build(name, scheme, gradient, device=cuda, node_coords=Xd)
+        dR = fwAD.unpack_dual(mcd.residual(T(state, dtype=F64, device=cuda))).tangent.clone()
+    a = float((dR.cpu() * T(lam)).sum())
+    b = float((got.cpu() * T(dX)).sum())
+    assert abs(a - b) <= 1e-12 * float(dR.abs().sum()) * float(np.abs(lam).max())
+
+
+def test_node_coordinate_adjoint_refused_for_corrected_green_gauss(cuda):
+    from tests.flow_cases import make_mesh
+
+    mesh = make_mesh("channel2d")
+    X = torch.tensor(mesh.node_coords, dtype=F64, device=cuda, requires_grad=True)
+    _, _, mc = build("channel2d", "limited", "cgg", device=cuda, node_coords=X)
+    R = mc.residual(torch.tensor(seeded_state(mesh, 0, "channel"), dtype=F64, device=cuda))
+    with pytest.raises(NotImplementedError, match="corrected Green-Gauss"):
+        torch.autograd.grad(R.sum(), X)
+
+
+def test_rans_geometry_derivatives_are_refused_not_dropped(cuda):
+    """Any derivative through the RANS geometry raises instead of returning a result that leaves
+    the geometry's part out."""
     import torch.autograd.forward_ad as fwAD
 
     from tests import rans_cases as RC
     from tests.flow_cases import make_mesh
 
     mesh = make_mesh("channel2d")
-    X = torch.tensor(mesh.node_coords, dtype=F64, device=cuda, requires_grad=True)
-    _, _, mc = build("channel2d", "limited", "lsq", device=cuda, node_coords=X)
-    R = mc.residual(torch.tensor(seeded_state(mesh, 0, "channel"), dtype=F64, device=cuda))
-    with pytest.raises(NotImplementedError, match="geometry"):
-        torch.autograd.grad(R.sum(), X)
     X2 = torch.tensor(mesh.node_coords, dtype=F64, device=cuda, requires_grad=True)
     rmesh, rans, _ = RC.build("channel2d_sst", "limited", "limited", "lsq", device=cuda, node_coords=X2)
     state = torch.tensor(RC.seeded_state(rmesh, 0, "channel"), dtype=F64, device=cuda)
