@@ -392,6 +392,10 @@ struct GeoRead<D, false> {
   AFX_DEV static void face(const MeshView& m, const GeoTan&, int32_t f, int32_t P, int32_t N, FaceGeo<D>& q) {
     load_face_geo<D>(m, f, P, N, q);
   }
+  AFX_DEV static void face_pts(const MeshView& m, const GeoTan&, int32_t f, bool interior, const double* xP,
+                               const double* xN, FaceGeo<D>& q) {
+    load_face_geo_pts<D>(m, f, interior, xP, xN, q);
+  }
 };
 
 template <int D>
@@ -416,6 +420,10 @@ struct GeoRead<D, true> {
     cell_centroid_g<D>(m, t, P, xP);
     if (N >= 0) cell_centroid_g<D>(m, t, N, xN);
     load_face_geo_g<D>(m, t, f, N >= 0, xP, N >= 0 ? xN : xP, q);
+  }
+  AFX_DEV static void face_pts(const MeshView& m, const GeoTan& t, int32_t f, bool interior, const dual* xP,
+                               const dual* xN, FaceGeo<D, dual>& q) {
+    load_face_geo_g<D>(m, t, f, interior, xP, xN, q);
   }
 };
 

@@ -37,6 +37,8 @@ struct RansArgs {
   In<S> x, ub, pb, mu, rho, nu, kval, wval;
   const double* dwall;
   const double* dgrad;
+  const double* dwall_t;  // tangents of the wall distance and its gradient (GEO instantiations only)
+  const double* dgrad_t;
   In<S> G, psi, nut, f1, Sr, cd, nutw, tau, mueff, aP;
   int32_t pin;
   double pin_value;
@@ -46,6 +48,7 @@ struct RansArgs {
   int sweeps;    // corrected Green-Gauss sweep count
   double relax;  // ... and relaxation
   int log_w;     // omega solved as log(omega): the wall fixation row is log(omega / omega_wall)
+  GeoTan gt;     // geometry tangents (read only by the GEO instantiations of the forward passes)
 };
 
 // closure arrays (written by pass C)
@@ -82,19 +85,41 @@ AFX_DEV int lim_slot(int v, bool limf, bool lims) {
   return lims ? v - 1 : -1;
 }
 
+// Wall distance and its gradient at cell c: values, or (GEO) (value, tangent) pairs.
+template <bool GEO, class S>
+AFX_DEV typename std::conditional<GEO, dual, double>::type wall_dist(const RansArgs<S>& a, int32_t c) {
+  if constexpr (GEO)
+    return dual{a.dwall[c], a.dwall_t[c]};
+  else
+    return a.dwall[c];
+}
+template <bool GEO, class S>
+AFX_DEV typename std::conditional<GEO, dual, double>::type wall_dist_grad(const RansArgs<S>& a, int32_t C, int j,
+                                                                          int32_t c) {
+  const int64_t i = (int64_t)j * C + c;
+  if constexpr (GEO)
+    return dual{a.dgrad[i], a.dgrad_t[i]};
+  else
+    return a.dgrad[i];
+}
+
 // The SST closure of cell c from its k, omega and gradients (pass C, defined below).
-template <int D, class S>
+template <int D, class S, bool GEO = false>
 AFX_DEV void closure_cell(const MeshView& m, const RansArgs<S>& a, int32_t c, S k, S w, const S (&Gu)[D][D],
                           const S (&Gk)[D], const S (&Gw)[D], const ClosureOut& out);
 
 // ---------------------------------------------------------------------------------------------
 // pass G.  With CLO the cell-local closure (pass C) follows in the same thread from the gradient
-// still in registers, saving the closure pass's re-read of k, omega and the gradients.
-template <int D, class S, int GRAD, bool LIMF, bool LIMS, bool CLO = false>
+// still in registers, saving the closure pass's re-read of k, omega and the gradients.  With GEO
+// the geometry is read as (value, tangent) pairs (the node-coordinate JVP; S = dual), centroids
+// from global memory rather than the tile.
+template <int D, class S, int GRAD, bool LIMF, bool LIMS, bool CLO = false, bool GEO = false>
 __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Out<S> G, Out<S> psi_out,
                                                    int32_t* __restrict__ ext, int32_t* __restrict__ kmin_out,
                                                    ClosureOut co = ClosureOut{}) {
   constexpr int NV = D + 3, NL = D + 2;
+  using GR = GeoRead<D, GEO>;
+  using GT = typename GR::T;
   __shared__ S tx[NV][kBlock];
   __shared__ double tcc[D][kBlock];
   const int32_t base = block_base(m);
@@ -102,12 +127,15 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
   const int32_t c = base + t;
   const bool valid = c < m.C;
   S xc[NV];
-  double ccc[D];
+  GT ccc[D];
   if (valid) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) tx[v][t] = xc[v] = a.x[(int64_t)v * m.C + c];
 #pragma unroll
-    for (int j = 0; j < D; ++j) tcc[j][t] = ccc[j] = m.cc[(int64_t)j * m.C + c];
+    for (int j = 0; j < D; ++j) {
+      ccc[j] = GR::cc(m, a.gt, j, c);
+      tcc[j][t] = val(ccc[j]);
+    }
   }
   __syncthreads();
   if (!valid) return;
@@ -116,11 +144,11 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
   for (int v = 0; v < NV; ++v)
 #pragma unroll
     for (int j = 0; j < D; ++j) acc[v][j] = sconst<S>(0.0);
-  double M[D][D];
+  GT M[D][D];
 #pragma unroll
   for (int p = 0; p < D; ++p)
 #pragma unroll
-    for (int q = 0; q < D; ++q) M[p][q] = 0.0;
+    for (int q = 0; q < D; ++q) M[p][q] = sconst<GT>(0.0);
   S mx[NL], mn[NL];
   int32_t amx[NL], amn[NL];
 #pragma unroll
@@ -139,18 +167,23 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
       S xo[NV];
-      double cco[D];
+      GT cco[D];
       const int ls = tile_slot(o, base);
       if (ls >= 0) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) xo[v] = tx[v][ls];
+        if constexpr (GEO) {
 #pragma unroll
-        for (int j = 0; j < D; ++j) cco[j] = tcc[j][ls];
+          for (int j = 0; j < D; ++j) cco[j] = GR::cc(m, a.gt, j, o);
+        } else {
+#pragma unroll
+          for (int j = 0; j < D; ++j) cco[j] = tcc[j][ls];
+        }
       } else {
 #pragma unroll
         for (int v = 0; v < NV; ++v) xo[v] = a.x[(int64_t)v * m.C + o];
 #pragma unroll
-        for (int j = 0; j < D; ++j) cco[j] = m.cc[(int64_t)j * m.C + o];
+        for (int j = 0; j < D; ++j) cco[j] = GR::cc(m, a.gt, j, o);
       }
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
@@ -167,9 +200,9 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
         }
       }
       if (GRAD == 0) {
-        FaceGeo<D> q;
-        load_face_geo_pts<D>(m, r.f, true, r.own ? ccc : cco, r.own ? cco : ccc, q);
-        const double w = 1.0 - q.g;
+        FaceGeo<D, GT> q;
+        GR::face_pts(m, a.gt, r.f, true, r.own ? ccc : cco, r.own ? cco : ccc, q);
+        const GT w = 1.0 - q.g;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           S phi = w * (r.own ? xc[v] : xo[v]) + q.g * (r.own ? xo[v] : xc[v]);
@@ -180,10 +213,10 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
           }
         }
       } else {
-        double dx[D];
+        GT dx[D];
 #pragma unroll
         for (int j = 0; j < D; ++j) dx[j] = cco[j] - ccc[j];
-        const double wgt = 1.0 / vdot<D>(dx, dx);
+        const GT wgt = 1.0 / vdotg<D, GT>(dx, dx);
 #pragma unroll
         for (int p = 0; p < D; ++p)
 #pragma unroll
@@ -197,18 +230,18 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
       }
     } else {
       if (GRAD == 0) {
-        const double A = m.area[r.f];
+        const GT A = GR::area(m, a.gt, r.f);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           S phi = rans_bval_lead<D, S>(a, v, m.F, r.fu, xc);
 #pragma unroll
-          for (int j = 0; j < D; ++j) acc[v][j] += phi * (m.nrm[(int64_t)j * m.F + r.f] * A);
+          for (int j = 0; j < D; ++j) acc[v][j] += phi * (GR::nrm(m, a.gt, j, r.f) * A);
         }
       } else {
-        double dx[D];
+        GT dx[D];
 #pragma unroll
-        for (int j = 0; j < D; ++j) dx[j] = m.fc[(int64_t)j * m.F + r.f] - m.cc[(int64_t)j * m.C + c];
-        const double wgt = 1.0 / vdot<D>(dx, dx);
+        for (int j = 0; j < D; ++j) dx[j] = GR::fc(m, a.gt, j, r.f) - GR::cc(m, a.gt, j, c);
+        const GT wgt = 1.0 / vdotg<D, GT>(dx, dx);
 #pragma unroll
         for (int p = 0; p < D; ++p)
 #pragma unroll
@@ -224,14 +257,14 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
   }
   S grad[NV][D];
   if (GRAD == 0) {
-    const double iv = 1.0 / m.vol[c];
+    const GT iv = 1.0 / GR::vol(m, a.gt, c);
 #pragma unroll
     for (int v = 0; v < NV; ++v)
 #pragma unroll
       for (int j = 0; j < D; ++j) grad[v][j] = acc[v][j] * iv;
   } else {
-    double I[D][D];
-    sym_inverse<D>(M, I);
+    GT I[D][D];
+    sym_inverse<D, GT>(M, I);
 #pragma unroll
     for (int v = 0; v < NV; ++v)
 #pragma unroll
@@ -247,19 +280,19 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
 #pragma unroll
     for (int j = 0; j < D; ++j) G.set((int64_t)(v * D + j) * m.C + c, grad[v][j]);
   if (LIMF || LIMS) {
-    const double e2 = m.vol[c] * (a.lim_k * a.lim_k * a.lim_k);
-    double xcen[D];
+    const GT e2 = GR::vol(m, a.gt, c) * (a.lim_k * a.lim_k * a.lim_k);
+    GT xcen[D];
 #pragma unroll
-    for (int j = 0; j < D; ++j) xcen[j] = m.cc[(int64_t)j * m.C + c];
+    for (int j = 0; j < D; ++j) xcen[j] = GR::cc(m, a.gt, j, c);
     S lo[NL];
     int32_t ks[NL];
     bool first = true;
     for (int32_t k = kb; k < ke; ++k) {
       const int32_t e = m.cface[k];
       const int32_t f = e >= 0 ? e : ~e;
-      double dxf[D];
+      GT dxf[D];
 #pragma unroll
-      for (int j = 0; j < D; ++j) dxf[j] = m.fc[(int64_t)j * m.F + f] - xcen[j];
+      for (int j = 0; j < D; ++j) dxf[j] = GR::fc(m, a.gt, j, f) - xcen[j];
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const int l = lim_slot<D>(v, LIMF, LIMS);
@@ -267,7 +300,7 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
         S dm = grad[v][0] * dxf[0];
 #pragma unroll
         for (int j = 1; j < D; ++j) dm = dm + grad[v][j] * dxf[j];
-        S vf = venkat_face<S>(dm, mx[l] - xc[v], mn[l] - xc[v], e2);
+        S vf = venkat_face<S, GT>(dm, mx[l] - xc[v], mn[l] - xc[v], e2);
         if (first || val(vf) < val(lo[l])) {
           lo[l] = vf;
           ks[l] = k - kb;
@@ -298,7 +331,7 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
       Gk[j] = grad[D + 1][j];
       Gw[j] = grad[D + 2][j];
     }
-    closure_cell<D, S>(m, a, c, xc[D + 1], xc[D + 2], Gu, Gk, Gw, co);
+    closure_cell<D, S, GEO>(m, a, c, xc[D + 1], xc[D + 2], Gu, Gk, Gw, co);
   }
 }
 
@@ -321,10 +354,11 @@ AFX_DEV void load_closure_inputs(const RansArgs<S>& a, const MeshView& m, int32_
 }
 
 // Wall shear rate of a wall cell: area-weighted |u_P - u_b| / d over its SST wall faces.
-template <int D, class S>
-AFX_DEV S wall_shear_rate(const RansArgs<S>& a, const MeshView& m, int32_t c, double d) {
+template <int D, class S, bool GEO = false>
+AFX_DEV S wall_shear_rate(const RansArgs<S>& a, const MeshView& m, int32_t c, typename GeoRead<D, GEO>::T d) {
+  using GT = typename GeoRead<D, GEO>::T;
   S num = sconst<S>(0.0);
-  double den = 0.0;
+  GT den = sconst<GT>(0.0);
   S xc[D + 1];
 #pragma unroll
   for (int v = 0; v <= D; ++v) xc[v] = a.x[(int64_t)v * m.C + c];
@@ -339,24 +373,25 @@ AFX_DEV S wall_shear_rate(const RansArgs<S>& a, const MeshView& m, int32_t c, do
       S sl = xc[j] - rans_bval_lead<D, S>(a, j, m.F, eu, xc);
       s2 = j == 0 ? sl * sl : s2 + sl * sl;
     }
-    const double A = m.area[e];
+    const GT A = GeoRead<D, GEO>::area(m, a.gt, e);
     num = num + (safe_sqrt(s2) / d) * A;
     den += A;
   }
-  return den > 0.0 ? num / den : sconst<S>(0.0);
+  return val(den) > 0.0 ? num / den : sconst<S>(0.0);
 }
 
-template <int D, class S>
+template <int D, class S, bool GEO>
 AFX_DEV void closure_cell(const MeshView& m, const RansArgs<S>& a, int32_t c, S k, S w, const S (&Gu)[D][D],
                           const S (&Gk)[D], const S (&Gw)[D], const ClosureOut& out) {
   const bool wall = a.wallc[c] != 0;
-  const double d = a.dwall[c];
-  double dg[D];
+  using GT = typename GeoRead<D, GEO>::T;
+  const GT d = wall_dist<GEO>(a, c);
+  GT dg[D];
 #pragma unroll
-  for (int j = 0; j < D; ++j) dg[j] = wall ? a.dgrad[(int64_t)j * m.C + c] : 0.0;
+  for (int j = 0; j < D; ++j) dg[j] = wall ? wall_dist_grad<GEO>(a, m.C, j, c) : sconst<GT>(0.0);
   const S nu = a.nu[c];
-  SstClosure<S> o = sst_closure<D, S>(a.sc, k, w, Gu, Gk, Gw, nu, d, dg, wall);
-  S tau = wall ? wall_shear_rate<D, S>(a, m, c, d) : sconst<S>(0.0);
+  SstClosure<S> o = sst_closure<D, S, GT>(a.sc, k, w, Gu, Gk, Gw, nu, d, dg, wall);
+  S tau = wall ? wall_shear_rate<D, S, GEO>(a, m, c, d) : sconst<S>(0.0);
   S mueff = a.mu[c] + a.rho[c] * o.nut;
   Out<S>{out.nut, out.nut_t}.set(c, o.nut);
   Out<S>{out.f1, out.f1_t}.set(c, o.f1);
@@ -367,20 +402,20 @@ AFX_DEV void closure_cell(const MeshView& m, const RansArgs<S>& a, int32_t c, S 
   Out<S>{out.mueff, out.mueff_t}.set(c, mueff);
 }
 
-template <int D, class S>
+template <int D, class S, bool GEO = false>
 __global__ void __launch_bounds__(256) k_rans_closure(MeshView m, RansArgs<S> a, ClosureOut out) {
   const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   S k, w, Gu[D][D], Gk[D], Gw[D];
   load_closure_inputs<D, S>(a, m, c, k, w, Gu, Gk, Gw);
-  closure_cell<D, S>(m, a, c, k, w, Gu, Gk, Gw, out);
+  closure_cell<D, S, GEO>(m, a, c, k, w, Gu, Gk, Gw, out);
 }
 
 // ---------------------------------------------------------------------------------------------
 // shared loaders for passes D / R
-template <int D, class S>
+template <int D, class S, bool GEO = false, class GT = typename GeoRead<D, GEO>::T>
 AFX_DEV void load_flow(const RansArgs<S>& a, const MeshView& m, int32_t c, bool with_ap, bool with_psi,
-                       FlowCell<D, S>& X) {
+                       FlowCell<D, S, GT>& X) {
 #pragma unroll
   for (int i = 0; i < D; ++i) X.u[i] = a.x[(int64_t)i * m.C + c];
   X.p = a.x[(int64_t)D * m.C + c];
@@ -392,7 +427,7 @@ AFX_DEV void load_flow(const RansArgs<S>& a, const MeshView& m, int32_t c, bool 
   for (int j = 0; j < D; ++j) X.Gp[j] = a.G[(int64_t)(D * D + j) * m.C + c];
   X.mu = a.mueff[c];
   X.rho = a.rho[c];
-  X.V = m.vol[c];
+  X.V = GeoRead<D, GEO>::vol(m, a.gt, c);
   X.aP = with_ap ? a.aP[c] : sconst<S>(1.0);
 #pragma unroll
   for (int i = 0; i < D; ++i) X.psi[i] = with_psi ? a.psi[(int64_t)i * m.C + c] : sconst<S>(1.0);
@@ -457,40 +492,44 @@ struct ScalarTile {
 
 // ---------------------------------------------------------------------------------------------
 // pass D
-template <int D, class S, int ADV>
+// With GEO the geometry is read as (value, tangent) pairs and every cell from global memory.
+template <int D, class S, int ADV, bool GEO = false>
 __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<S> aP) {
+  using GR = GeoRead<D, GEO>;
+  using GT = typename GR::T;
   __shared__ FlowTile<D, S, false> tile;
   const int32_t base = block_base(m);
   const int t = threadIdx.x;
   const int32_t c = base + t;
   const bool valid = c < m.C;
-  FlowCell<D, S> X;
-  double xc[D];
+  FlowCell<D, S, GT> X;
+  GT xc[D];
   if (valid) {
-    load_flow<D, S>(a, m, c, false, false, X);
+    load_flow<D, S, GEO>(a, m, c, false, false, X);
 #pragma unroll
-    for (int j = 0; j < D; ++j) xc[j] = m.cc[(int64_t)j * m.C + c];
-    tile.put(t, X, xc);
+    for (int j = 0; j < D; ++j) xc[j] = GR::cc(m, a.gt, j, c);
+    if constexpr (!GEO) tile.put(t, X, xc);
   }
-  __syncthreads();
+  if constexpr (!GEO) __syncthreads();
   if (!valid) return;
   S acc = sconst<S>(0.0);
   for (FaceWalk walk(m, c); walk.more();) {
     FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
-    FaceGeo<D> q;
+    FaceGeo<D, GT> q;
     if (o >= 0) {
-      FlowCell<D, S> O;
-      double xo[D];
-      const int lo = tile_slot(o, base);
+      FlowCell<D, S, GT> O;
+      GT xo[D];
+      int lo = -1;
+      if constexpr (!GEO) lo = tile_slot(o, base);
       if (lo >= 0) {
-        tile.get(lo, O, xo);
+        if constexpr (!GEO) tile.get(lo, O, xo);
       } else {
-        load_flow<D, S>(a, m, o, false, false, O);
+        load_flow<D, S, GEO>(a, m, o, false, false, O);
 #pragma unroll
-        for (int j = 0; j < D; ++j) xo[j] = m.cc[(int64_t)j * m.C + o];
+        for (int j = 0; j < D; ++j) xo[j] = GR::cc(m, a.gt, j, o);
       }
-      load_face_geo_pts<D>(m, r.f, true, r.own ? xc : xo, r.own ? xo : xc, q);
+      GR::face_pts(m, a.gt, r.f, true, r.own ? xc : xo, r.own ? xo : xc, q);
       S oc, nc;
       if (r.own)
         ap_interior<D, ADV, S>(q, X, O, oc, nc);
@@ -498,7 +537,7 @@ __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<
         ap_interior<D, ADV, S>(q, O, X, oc, nc);
       acc += r.own ? oc : nc;
     } else {
-      load_face_geo_pts<D>(m, r.f, false, xc, xc, q);
+      GR::face_pts(m, a.gt, r.f, false, xc, xc, q);
       const int kind = a.kind[r.fu];
       acc += ap_boundary<D, ADV, S>(q, kind, X, rans_gamma_b<S>(a, kind, r.fu, c, X.mu));
     }
@@ -509,23 +548,25 @@ __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<
 // ---------------------------------------------------------------------------------------------
 // pass R, flow part: momentum and continuity rows, and the Rhie-Chow face mass flux (written by
 // each face's owner) that the k / omega fluxes advect on.
-template <int D, class S, int ADVF>
+template <int D, class S, int ADVF, bool GEO = false>
 __global__ void __launch_bounds__(256) k_rans_fres(MeshView m, RansArgs<S> a, Out<S> R, Out<S> mdot) {
   constexpr int NV = D + 1;
+  using GR = GeoRead<D, GEO>;
+  using GT = typename GR::T;
   // the FP64 pass stages its block's cells (53 KB of LDS in 3D); the dual-number (tangent) pass,
   // whose tile would be twice that, gathers from global memory
-  constexpr bool kStage = std::is_same<S, double>::value;
+  constexpr bool kStage = std::is_same<S, double>::value && !GEO;
   __shared__ FlowTile<D, double, true> tile;
   const int32_t base = block_base(m);
   const int t = threadIdx.x;
   const int32_t c = base + t;
   const bool valid = c < m.C;
-  FlowCell<D, S> X;
-  double xc[D];
+  FlowCell<D, S, GT> X;
+  GT xc[D];
   if (valid) {
-    load_flow<D, S>(a, m, c, true, ADVF == kAdvLimited, X);
+    load_flow<D, S, GEO>(a, m, c, true, ADVF == kAdvLimited, X);
 #pragma unroll
-    for (int j = 0; j < D; ++j) xc[j] = m.cc[(int64_t)j * m.C + c];
+    for (int j = 0; j < D; ++j) xc[j] = GR::cc(m, a.gt, j, c);
     if constexpr (kStage) tile.put(t, X, xc);
   }
   if constexpr (kStage) __syncthreads();
@@ -536,27 +577,27 @@ __global__ void __launch_bounds__(256) k_rans_fres(MeshView m, RansArgs<S> a, Ou
   for (FaceWalk walk(m, c); walk.more();) {
     FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
-    FaceGeo<D> q;
+    FaceGeo<D, GT> q;
     S out[NV];
     if (o >= 0) {
-      FlowCell<D, S> O;
-      double xo[D];
+      FlowCell<D, S, GT> O;
+      GT xo[D];
       int lo = -1;
       if constexpr (kStage) lo = tile_slot(o, base);
       if (lo >= 0) {
         if constexpr (kStage) tile.get(lo, O, xo);
       } else {
-        load_flow<D, S>(a, m, o, true, ADVF == kAdvLimited, O);
+        load_flow<D, S, GEO>(a, m, o, true, ADVF == kAdvLimited, O);
 #pragma unroll
-        for (int j = 0; j < D; ++j) xo[j] = m.cc[(int64_t)j * m.C + o];
+        for (int j = 0; j < D; ++j) xo[j] = GR::cc(m, a.gt, j, o);
       }
-      load_face_geo_pts<D>(m, r.f, true, r.own ? xc : xo, r.own ? xo : xc, q);
+      GR::face_pts(m, a.gt, r.f, true, r.own ? xc : xo, r.own ? xo : xc, q);
       if (r.own)
         flow_flux_interior<D, ADVF, S>(q, X, O, out);
       else
         flow_flux_interior<D, ADVF, S>(q, O, X, out);
     } else {
-      load_face_geo_pts<D>(m, r.f, false, xc, xc, q);
+      GR::face_pts(m, a.gt, r.f, false, xc, xc, q);
       S ub[D];
 #pragma unroll
       for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.fu];
@@ -576,10 +617,10 @@ __global__ void __launch_bounds__(256) k_rans_fres(MeshView m, RansArgs<S> a, Ou
 }
 
 // k and omega face fluxes of one face seen from cell c (all scalar operands in registers).
-template <int D, class S, int ADVS>
+template <int D, class S, int ADVS, class GT = double>
 AFX_DEV void rans_scalar_face(const RansArgs<S>& a, const MeshView& m, const FaceRef& r, int32_t o,
-                              const FaceGeo<D>& q, S Qf, const ScalarCell<D, S>& Kc, const ScalarCell<D, S>& Wc,
-                              const ScalarCell<D, S>& Ko, const ScalarCell<D, S>& Wo, double dP, S& fk, S& fw) {
+                              const FaceGeo<D, GT>& q, S Qf, const ScalarCell<D, S>& Kc, const ScalarCell<D, S>& Wc,
+                              const ScalarCell<D, S>& Ko, const ScalarCell<D, S>& Wo, GT dP, S& fk, S& fw) {
   const SstConst& sc = a.sc;
   if (o >= 0) {
     if (r.own) {
@@ -601,8 +642,10 @@ AFX_DEV void rans_scalar_face(const RansArgs<S>& a, const MeshView& m, const Fac
 }
 
 // pass R, k / omega part: transport on mdot / density, SST sources, omega wall fixation.
-template <int D, class S, int ADVS>
+template <int D, class S, int ADVS, bool GEO = false>
 __global__ void __launch_bounds__(256) k_rans_sres(MeshView m, RansArgs<S> a, In<S> mdot, Out<S> R) {
+  using GR = GeoRead<D, GEO>;
+  using GT = typename GR::T;
   const SstConst& sc = a.sc;
   __shared__ ScalarTile<D, S, false> tile;
   const int32_t base = block_base(m);
@@ -610,38 +653,39 @@ __global__ void __launch_bounds__(256) k_rans_sres(MeshView m, RansArgs<S> a, In
   const int32_t c = base + t;
   const bool valid = c < m.C;
   ScalarCell<D, S> Kc, Wc;
-  double xc[D];
+  GT xc[D];
   if (valid) {
     load_scalar<D, S>(a, m, c, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Kc);
     load_scalar<D, S>(a, m, c, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wc);
 #pragma unroll
-    for (int j = 0; j < D; ++j) xc[j] = m.cc[(int64_t)j * m.C + c];
-    tile.put(t, Kc, Wc, xc);
+    for (int j = 0; j < D; ++j) xc[j] = GR::cc(m, a.gt, j, c);
+    if constexpr (!GEO) tile.put(t, Kc, Wc, xc);
   }
-  __syncthreads();
+  if constexpr (!GEO) __syncthreads();
   if (!valid) return;
   S acck = sconst<S>(0.0), accw = sconst<S>(0.0);
   const double irho = 1.0 / sc.density;
-  const double dP = a.dwall[c];
+  const GT dP = wall_dist<GEO>(a, c);
   for (FaceWalk walk(m, c); walk.more();) {
     FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
-    FaceGeo<D> q;
+    FaceGeo<D, GT> q;
     ScalarCell<D, S> Ko, Wo;
     if (o >= 0) {
-      double xo[D];
-      const int lo = tile_slot(o, base);
+      GT xo[D];
+      int lo = -1;
+      if constexpr (!GEO) lo = tile_slot(o, base);
       if (lo >= 0) {
-        tile.get(lo, Ko, Wo, xo);
+        if constexpr (!GEO) tile.get(lo, Ko, Wo, xo);
       } else {
         load_scalar<D, S>(a, m, o, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Ko);
         load_scalar<D, S>(a, m, o, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wo);
 #pragma unroll
-        for (int j = 0; j < D; ++j) xo[j] = m.cc[(int64_t)j * m.C + o];
+        for (int j = 0; j < D; ++j) xo[j] = GR::cc(m, a.gt, j, o);
       }
-      load_face_geo_pts<D>(m, r.f, true, r.own ? xc : xo, r.own ? xo : xc, q);
+      GR::face_pts(m, a.gt, r.f, true, r.own ? xc : xo, r.own ? xo : xc, q);
     } else {
-      load_face_geo_pts<D>(m, r.f, false, xc, xc, q);
+      GR::face_pts(m, a.gt, r.f, false, xc, xc, q);
     }
     const S Qf = mdot[r.f] * irho;
     S fk, fw;
@@ -651,8 +695,8 @@ __global__ void __launch_bounds__(256) k_rans_sres(MeshView m, RansArgs<S> a, In
   }
   const bool wall = a.wallc[c] != 0;
   S pk, dk, pw, dw, cw;
-  sst_sources<S>(sc, Kc.phi, Wc.phi, a.nut[c], a.Sr[c], a.f1[c], a.cd[c], a.tau[c], a.nu[c], dP, wall, m.vol[c], pk,
-                 dk, pw, dw, cw);
+  sst_sources<S>(sc, Kc.phi, Wc.phi, a.nut[c], a.Sr[c], a.f1[c], a.cd[c], a.tau[c], a.nu[c], dP, wall,
+                 GR::vol(m, a.gt, c), pk, dk, pw, dw, cw);
   R.set((int64_t)(D + 1) * m.C + c, (acck - pk) - dk);
   S rw = ((accw - pw) - dw) - cw;
   if (wall) {
@@ -1651,9 +1695,12 @@ void bind_closure(RansArgs<S>& a, double* cl, double* clt, int64_t C) {
 
 // Forward passes: G, closure, a_P; then, for `R` set, the flow and k / omega residual rows; for
 // `mdot_only`, just the face mass flux (and the flow rows when R is set) for the reverse passes.
-template <int D, class S, int ADVF, int ADVS, int GRAD>
+// GEO: the geometry carries tangents in a.gt (S = dual; not with corrected Green-Gauss).
+template <int D, class S, int ADVF, int ADVS, int GRAD, bool GEO = false>
 int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S> R, const RWork& w, double* Gt,
                  double* psit, double* aPt, double* clt, double* mdott, int32_t* ext, bool mdot_only, hipStream_t s) {
+  static_assert(!GEO || (GRAD != kGradCorrected && std::is_same<S, dual>::value),
+                "geometry tangents: dual state, compact Green-Gauss or least squares");
   MeshView m;
   if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
   const int64_t C = m.C;
@@ -1673,12 +1720,12 @@ int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S
   // the closure rides on the gradient pass unless corrected-GG sweeps still change the gradient
   // (the dual-number build keeps them apart: fused, it would drop to one wave per SIMD)
   constexpr bool FUSE = !CG && std::is_same<S, double>::value;
-  AFX_LAUNCH(kKRansGrad, (k_rans_grad<D, S, GK, LF && !CG, LS && !CG, FUSE>), grid_for(C), dim3(kBlock), 0, s, m,
-             a, oG, oP, ext, ext ? w.kmin : nullptr, co);
+  AFX_LAUNCH(kKRansGrad, (k_rans_grad<D, S, GK, LF && !CG, LS && !CG, FUSE, GEO>), grid_for(C), dim3(kBlock), 0, s,
+             m, a, oG, oP, ext, ext ? w.kmin : nullptr, co);
   AFX_CHECK_LAUNCH();
   a.G.v = w.G;
   a.G.t = Gt;
-  if (CG) {
+  if constexpr (CG) {
     const int sw = a.sweeps;
     const double rl = a.relax;
     int rc = cgg_sweeps(m, D, D + 3, w.G, w.cgg, sw, rl, s);
@@ -1707,22 +1754,23 @@ int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S
   a.psi.v = w.psi;
   a.psi.t = psit;
   if (!FUSE) {
-    AFX_LAUNCH(kKSstClosure, (k_rans_closure<D, S>), grid_for(C), dim3(kBlock), 0, s, m, a, co);
+    AFX_LAUNCH(kKSstClosure, (k_rans_closure<D, S, GEO>), grid_for(C), dim3(kBlock), 0, s, m, a, co);
     AFX_CHECK_LAUNCH();
   }
   bind_closure<S>(a, w.cl, clt, C);
-  AFX_LAUNCH(kKRansAp, (k_rans_ap<D, S, ADVF>), grid_for(C), dim3(kBlock), 0, s, m, a, Out<S>{w.aP, aPt});
+  AFX_LAUNCH(kKRansAp, (k_rans_ap<D, S, ADVF, GEO>), grid_for(C), dim3(kBlock), 0, s, m, a, Out<S>{w.aP, aPt});
   AFX_CHECK_LAUNCH();
   if (R.v == nullptr && !mdot_only) return 0;
   a.aP.v = w.aP;
   a.aP.t = aPt;
-  AFX_LAUNCH(kKSstRes, (k_rans_fres<D, S, ADVF>), grid_for(C), dim3(kBlock), 0, s, m, a, R, Out<S>{w.mdot, mdott});
+  AFX_LAUNCH(kKSstRes, (k_rans_fres<D, S, ADVF, GEO>), grid_for(C), dim3(kBlock), 0, s, m, a, R,
+             Out<S>{w.mdot, mdott});
   AFX_CHECK_LAUNCH();
   if (mdot_only) return 0;
   In<S> md;
   md.v = w.mdot;
   md.t = mdott;
-  AFX_LAUNCH(kKRansScalarRes, (k_rans_sres<D, S, ADVS>), grid_for(C), dim3(kBlock), 0, s, m, a, md, R);
+  AFX_LAUNCH(kKRansScalarRes, (k_rans_sres<D, S, ADVS, GEO>), grid_for(C), dim3(kBlock), 0, s, m, a, md, R);
   AFX_CHECK_LAUNCH();
   return 0;
 }
@@ -1800,6 +1848,50 @@ extern "C" int afx_rans_jvp(const afx_mesh* m, const afx_geometry* g, const afx_
                      return rans_forward<decltype(Dc)::value, dual, decltype(Fc)::value, decltype(Sc)::value,
                                          decltype(Gc)::value>(m, g, a, Out<dual>{residual, residual_tangent}, w,
                                                               w.Gt, w.psit, w.aPt, w.clt, w.mdott, nullptr, false, s);
+                   });
+}
+
+extern "C" int afx_rans_jvp_geometry(const afx_mesh* m, const afx_geometry* g, const afx_geometry* geom_tangent,
+                                     const double* wall_distance_tangent,
+                                     const double* wall_distance_gradient_tangent, const afx_rans_config* cfg,
+                                     const afx_rans_inputs* in, const afx_rans_inputs* tangent, double* residual,
+                                     double* residual_tangent, void* work, int64_t work_size, hipStream_t s) {
+  int rc = rcheck(m, g, cfg, in, work, work_size, 1);
+  if (rc) return rc;
+  AFX_REQUIRE(geom_tangent && wall_distance_tangent && wall_distance_gradient_tangent && tangent && residual &&
+                  residual_tangent,
+              "afx_rans_jvp_geometry: null argument");
+  AFX_REQUIRE(cfg->flow.gradient != kGradCorrected,
+              "afx_rans_jvp_geometry: a geometry tangent is available for compact Green-Gauss or least-squares "
+              "gradients");
+  AFX_REQUIRE(geom_tangent->area && geom_tangent->normal && geom_tangent->face_centroid && geom_tangent->volume &&
+                  geom_tangent->cell_centroid,
+              "afx_rans_jvp_geometry: the geometry tangent needs all five arrays");
+  AFX_REQUIRE(m->fu == nullptr || geom_tangent->face_pack,
+              "afx_rans_jvp_geometry: this mesh's faces are reordered: geom_tangent->face_pack must hold the face "
+              "tangents in the internal order (afx_geometry_pack of the tangent arrays)");
+  RWork w = rcarve(work, m->dim, m->C, m->F, 1);
+  RansArgs<dual> a = rargs<dual>(cfg, in, tangent);
+  const double* pack = geom_tangent->face_pack;
+  a.gt.area = m->fu ? pack : geom_tangent->area;
+  a.gt.nrm = m->fu ? pack + m->F : geom_tangent->normal;
+  a.gt.fc = m->fu ? pack + (int64_t)(1 + m->dim) * m->F : geom_tangent->face_centroid;
+  a.gt.vol = geom_tangent->volume;
+  a.gt.cc = geom_tangent->cell_centroid;
+  a.dwall_t = wall_distance_tangent;
+  a.dgrad_t = wall_distance_gradient_tangent;
+  rc = rans_bind_physical<dual>(m, cfg, in, tangent, w.xs, w.xst, a, s);
+  if (rc) return rc;
+  return rdispatch(m->dim, cfg->flow.advection, cfg->scalar_advection, cfg->flow.gradient,
+                   [&](auto Dc, auto Fc, auto Sc, auto Gc) {
+                     constexpr int GR = decltype(Gc)::value;
+                     if constexpr (GR == kGradCorrected) {
+                       return 2;  // refused above
+                     } else {
+                       return rans_forward<decltype(Dc)::value, dual, decltype(Fc)::value, decltype(Sc)::value, GR,
+                                           true>(m, g, a, Out<dual>{residual, residual_tangent}, w, w.Gt, w.psit,
+                                                 w.aPt, w.clt, w.mdott, nullptr, false, s);
+                     }
                    });
 }
 

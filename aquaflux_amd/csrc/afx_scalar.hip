@@ -1142,89 +1142,95 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
 // ---------------------------------------------------------------------------------------------
 // gradient of a scalar with Dirichlet / zero-gradient leading-order boundary values (used for the
 // SST wall-distance gradient)
-AFX_DEV double lead_bval(int8_t kind, double v, double phiP) {
-  if (kind == kBcDirichlet) return v;
-  if (kind == kBcNone) return 0.0;
+template <class S>
+AFX_DEV S lead_bval(int8_t kind, double v, S phiP) {
+  if (kind == kBcDirichlet) return sconst<S>(v);
+  if (kind == kBcNone) return sconst<S>(0.0);
   return phiP;
 }
 
-template <int D, int GRAD>
-__global__ void __launch_bounds__(256) k_scalar_grad(MeshView m, const int8_t* __restrict__ kind,
+// With GEO the geometry and phi are (value, tangent) pairs (the node-coordinate JVP of the
+// gradient); the boundary values stay constant.
+template <int D, int GRAD, bool GEO = false>
+__global__ void __launch_bounds__(256) k_scalar_grad(MeshView m, GeoTan gt, const int8_t* __restrict__ kind,
                                                      const double* __restrict__ bval,
-                                                     const double* __restrict__ phi, double* __restrict__ G) {
+                                                     In<typename GeoRead<D, GEO>::T> phi,
+                                                     Out<typename GeoRead<D, GEO>::T> G) {
+  using GR = GeoRead<D, GEO>;
+  using T = typename GR::T;
   const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
-  const double xc = phi[c];
-  double acc[D];
+  const T xc = phi[c];
+  T acc[D];
 #pragma unroll
-  for (int j = 0; j < D; ++j) acc[j] = 0.0;
-  double M[D][D];
+  for (int j = 0; j < D; ++j) acc[j] = sconst<T>(0.0);
+  T M[D][D];
 #pragma unroll
   for (int p = 0; p < D; ++p)
 #pragma unroll
-    for (int q = 0; q < D; ++q) M[p][q] = 0.0;
+    for (int q = 0; q < D; ++q) M[p][q] = sconst<T>(0.0);
   for (FaceWalk walk(m, c); walk.more();) {
     FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
-      const double xo = phi[o];
+      const T xo = phi[o];
       if (GRAD == 0) {
-        FaceGeo<D> q;
-        load_face_geo<D>(m, r.f, r.P, r.N, q);
-        const double w = 1.0 - q.g;
-        const double pf = w * (r.own ? xc : xo) + q.g * (r.own ? xo : xc);
+        FaceGeo<D, T> q;
+        GR::face(m, gt, r.f, r.P, r.N, q);
+        const T w = 1.0 - q.g;
+        const T pf = w * (r.own ? xc : xo) + q.g * (r.own ? xo : xc);
 #pragma unroll
         for (int j = 0; j < D; ++j) {
-          const double t = pf * (q.n[j] * q.A);
+          const T t = pf * (q.n[j] * q.A);
           acc[j] = r.own ? acc[j] + t : acc[j] - t;
         }
       } else {
-        double dx[D];
+        T dx[D];
 #pragma unroll
-        for (int j = 0; j < D; ++j) dx[j] = m.cc[(int64_t)j * m.C + o] - m.cc[(int64_t)j * m.C + c];
-        const double wgt = 1.0 / vdot<D>(dx, dx);
+        for (int j = 0; j < D; ++j) dx[j] = GR::cc(m, gt, j, o) - GR::cc(m, gt, j, c);
+        const T wgt = 1.0 / vdotg<D, T>(dx, dx);
 #pragma unroll
         for (int p = 0; p < D; ++p)
 #pragma unroll
           for (int q2 = 0; q2 < D; ++q2) M[p][q2] += wgt * dx[p] * dx[q2];
-        const double dphi = xo - xc;
+        const T dphi = xo - xc;
 #pragma unroll
         for (int j = 0; j < D; ++j) acc[j] += (wgt * dx[j]) * dphi;
       }
     } else {
-      const double pb = lead_bval(kind[r.fu], bval[r.fu], xc);
+      const T pb = lead_bval<T>(kind[r.fu], bval[r.fu], xc);
       if (GRAD == 0) {
-        const double A = m.area[r.f];
+        const T A = GR::area(m, gt, r.f);
 #pragma unroll
-        for (int j = 0; j < D; ++j) acc[j] += pb * (m.nrm[(int64_t)j * m.F + r.f] * A);
+        for (int j = 0; j < D; ++j) acc[j] += pb * (GR::nrm(m, gt, j, r.f) * A);
       } else {
-        double dx[D];
+        T dx[D];
 #pragma unroll
-        for (int j = 0; j < D; ++j) dx[j] = m.fc[(int64_t)j * m.F + r.f] - m.cc[(int64_t)j * m.C + c];
-        const double wgt = 1.0 / vdot<D>(dx, dx);
+        for (int j = 0; j < D; ++j) dx[j] = GR::fc(m, gt, j, r.f) - GR::cc(m, gt, j, c);
+        const T wgt = 1.0 / vdotg<D, T>(dx, dx);
 #pragma unroll
         for (int p = 0; p < D; ++p)
 #pragma unroll
           for (int q2 = 0; q2 < D; ++q2) M[p][q2] += wgt * dx[p] * dx[q2];
-        const double dphi = pb - xc;
+        const T dphi = pb - xc;
 #pragma unroll
         for (int j = 0; j < D; ++j) acc[j] += (wgt * dx[j]) * dphi;
       }
     }
   }
   if (GRAD == 0) {
-    const double iv = 1.0 / m.vol[c];
+    const T iv = 1.0 / GR::vol(m, gt, c);
 #pragma unroll
-    for (int j = 0; j < D; ++j) G[(int64_t)j * m.C + c] = acc[j] * iv;
+    for (int j = 0; j < D; ++j) G.set((int64_t)j * m.C + c, acc[j] * iv);
   } else {
-    double I[D][D];
-    sym_inverse<D>(M, I);
+    T I[D][D];
+    sym_inverse<D, T>(M, I);
 #pragma unroll
     for (int p = 0; p < D; ++p) {
-      double s = I[p][0] * acc[0];
+      T s = I[p][0] * acc[0];
 #pragma unroll
       for (int q2 = 1; q2 < D; ++q2) s = s + I[p][q2] * acc[q2];
-      G[(int64_t)p * m.C + c] = s;
+      G.set((int64_t)p * m.C + c, s);
     }
   }
 }
@@ -1473,8 +1479,8 @@ extern "C" int afx_scalar_gradient(const afx_mesh* mh, const afx_geometry* g, in
   const int gk = gradient == 3 ? 0 : gradient;
 #define AFX_SG(DD, GG)                                                                                  \
   if (mh->dim == DD && gk == GG) {                                                                      \
-    AFX_LAUNCH(kKScalarGrad, (k_scalar_grad<DD, GG>), grid_for(m.C), dim3(kBlock), 0, s, m, kind, value, \
-               phi, grad);                                                                              \
+    AFX_LAUNCH(kKScalarGrad, (k_scalar_grad<DD, GG>), grid_for(m.C), dim3(kBlock), 0, s, m, GeoTan{}, kind, \
+               value, In<double>{phi, nullptr}, Out<double>{grad, nullptr});                             \
     AFX_CHECK_LAUNCH();                                                                                 \
     if (gradient != 3) return 0;                                                                        \
     double* scratch = afx_mesh_scratch(const_cast<afx_mesh*>(mh), (int64_t)2 * DD * m.C);              \
@@ -1484,6 +1490,45 @@ extern "C" int afx_scalar_gradient(const afx_mesh* mh, const afx_geometry* g, in
   AFX_SG(2, 0) AFX_SG(2, 1) AFX_SG(3, 0) AFX_SG(3, 1)
 #undef AFX_SG
   afx_set_error("afx_scalar_gradient: unsupported dimension");
+  return 2;
+}
+
+extern "C" int afx_scalar_gradient_jvp_geometry(const afx_mesh* mh, const afx_geometry* g,
+                                                const afx_geometry* geom_tangent, int gradient, const int8_t* kind,
+                                                const double* value, const double* phi, const double* phi_tangent,
+                                                double* grad, double* grad_tangent, hipStream_t s) {
+  AFX_REQUIRE(mh && g && geom_tangent && kind && value && phi && phi_tangent && grad && grad_tangent,
+              "afx_scalar_gradient_jvp_geometry: null argument");
+  AFX_REQUIRE(gradient == 0 || gradient == 1,
+              "afx_scalar_gradient_jvp_geometry: compact Green-Gauss or least-squares gradients only");
+  AFX_REQUIRE(geom_tangent->area && geom_tangent->normal && geom_tangent->face_centroid && geom_tangent->volume &&
+                  geom_tangent->cell_centroid,
+              "afx_scalar_gradient_jvp_geometry: the geometry tangent needs all five arrays");
+  AFX_REQUIRE(mh->fu == nullptr || geom_tangent->face_pack,
+              "afx_scalar_gradient_jvp_geometry: this mesh's faces are reordered: geom_tangent->face_pack must hold "
+              "the face tangents in the internal order");
+  MeshView m;
+  if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
+  if (m.C == 0) return 0;
+  GeoTan gt;
+  const double* pack = geom_tangent->face_pack;
+  gt.area = mh->fu ? pack : geom_tangent->area;
+  gt.nrm = mh->fu ? pack + mh->F : geom_tangent->normal;
+  gt.fc = mh->fu ? pack + (int64_t)(1 + mh->dim) * mh->F : geom_tangent->face_centroid;
+  gt.vol = geom_tangent->volume;
+  gt.cc = geom_tangent->cell_centroid;
+  const In<dual> ph{phi, phi_tangent};
+  const Out<dual> og{grad, grad_tangent};
+#define AFX_SGJ(DD, GG)                                                                                   \
+  if (mh->dim == DD && gradient == GG) {                                                                  \
+    AFX_LAUNCH(kKScalarGrad, (k_scalar_grad<DD, GG, true>), grid_for(m.C), dim3(kBlock), 0, s, m, gt, kind, \
+               value, ph, og);                                                                            \
+    AFX_CHECK_LAUNCH();                                                                                   \
+    return 0;                                                                                             \
+  }
+  AFX_SGJ(2, 0) AFX_SGJ(2, 1) AFX_SGJ(3, 0) AFX_SGJ(3, 1)
+#undef AFX_SGJ
+  afx_set_error("afx_scalar_gradient_jvp_geometry: unsupported dimension");
   return 2;
 }
 

@@ -40,25 +40,35 @@ AFX_DEV S sst_blend(S f1, double inner, double outer) {
   return f1 * inner + (1.0 - f1) * outer;
 }
 
+// The wall distance d has its own number type DT: dual when the node coordinates carry a tangent.
 template <class S>
-AFX_DEV S wall_y_star(const SstConst& c, S nu, double d, S k) {
+AFX_DEV S lift(double x) {
+  return sconst<S>(x);
+}
+template <class S>
+AFX_DEV S lift(dual x) {
+  return x;
+}
+
+template <class S, class DT = double>
+AFX_DEV S wall_y_star(const SstConst& c, S nu, DT d, S k) {
   return c.bsq * safe_sqrt(kpos(k)) * d / nu;
 }
 
-template <class S>
-AFX_DEV S wall_function_weight(const SstConst& c, S nu, double d, S k) {
+template <class S, class DT = double>
+AFX_DEV S wall_function_weight(const SstConst& c, S nu, DT d, S k) {
   S r = wall_y_star(c, nu, d, k) / c.ylam;
   S r2 = r * r;
   return dtanh(r2 * r2);
 }
 
-template <class S>
-AFX_DEV S log_layer_shear_rate(const SstConst& c, double d, S k) {
+template <class S, class DT = double>
+AFX_DEV S log_layer_shear_rate(const SstConst& c, DT d, S k) {
   return c.bsq * safe_sqrt(kpos(k)) / (c.kappa * d);
 }
 
-template <class S>
-AFX_DEV S nut_wall(const SstConst& c, S nu, double d, S k) {
+template <class S, class DT = double>
+AFX_DEV S nut_wall(const SstConst& c, S nu, DT d, S k) {
   S ys = wall_y_star(c, nu, d, k);
   S arg = c.e_wall * dmax(ys, c.ylam);
   S r = ys * c.kappa / dlog(arg) - 1.0;
@@ -94,9 +104,9 @@ AFX_DEV S omega_wall(const SstConst& c, S nu, S d, S k, S* dodd = nullptr, S* do
   return om;
 }
 
-template <class S>
-AFX_DEV S omega_wall_k(const SstConst& c, S nu, double d, S k) {
-  return omega_wall<S>(c, nu, sconst<S>(d), k);
+template <class S, class DT = double>
+AFX_DEV S omega_wall_k(const SstConst& c, S nu, DT d, S k) {
+  return omega_wall<S>(c, nu, lift<S>(d), k);
 }
 
 // Outputs of the cell-local closure.
@@ -123,8 +133,8 @@ AFX_DEV S strain_magnitude(const S (&Gu)[D][D]) {
 // away from walls kw = A (= grad k . grad omega); at a wall cell the omega gradient is the imposed
 // d omega_wall/dd grad d + d omega_wall/dk grad k, so kw = dodd A + dodk B with A = grad k . grad d,
 // B = |grad k|^2.  (transport.py:369-546, sst.py:125-239)
-template <class S>
-AFX_DEV SstClosure<S> sst_closure_core(const SstConst& c, S k, S w, S strain0, S A, S B, S nu, double d,
+template <class S, class DT = double>
+AFX_DEV SstClosure<S> sst_closure_core(const SstConst& c, S k, S w, S strain0, S A, S B, S nu, DT d,
                                        bool wall) {
   SstClosure<S> o;
   S strain = strain0;
@@ -133,7 +143,7 @@ AFX_DEV SstClosure<S> sst_closure_core(const SstConst& c, S k, S w, S strain0, S
     S wgt = wall_function_weight(c, nu, d, k);
     strain = (1.0 - wgt) * strain0 + wgt * log_layer_shear_rate(c, d, k);
     S dodd, dodk;
-    omega_wall<S>(c, nu, sconst<S>(d), k, &dodd, &dodk);
+    omega_wall<S>(c, nu, lift<S>(d), k, &dodd, &dodk);
     kw = dodd * A + dodk * B;
   }
   // F1 (sst.py:152-195)
@@ -154,8 +164,8 @@ AFX_DEV SstClosure<S> sst_closure_core(const SstConst& c, S k, S w, S strain0, S
 }
 
 // The factored inputs (A, B) of sst_closure_core.
-template <int D, class S>
-AFX_DEV void closure_products(const S (&Gk)[D], const S (&Gw_rec)[D], const double (&dgrad)[D], bool wall, S& A,
+template <int D, class S, class DT = double>
+AFX_DEV void closure_products(const S (&Gk)[D], const S (&Gw_rec)[D], const DT (&dgrad)[D], bool wall, S& A,
                               S& B) {
   if (wall) {
     A = Gk[0] * dgrad[0];
@@ -176,20 +186,20 @@ AFX_DEV void closure_products(const S (&Gk)[D], const S (&Gw_rec)[D], const doub
 // Closure fields of one cell (transport.py:369-546): strain rate (with the wall log-layer blend),
 // F1 from the (wall-imposed) omega gradient, eddy viscosity with the F2 limiter, the cross-diffusion
 // value 2 (1 - F1) sigma_w2 grad k . grad w / w, and the wall-function eddy viscosity of its wall faces.
-template <int D, class S>
+template <int D, class S, class DT = double>
 AFX_DEV SstClosure<S> sst_closure(const SstConst& c, S k, S w, const S (&Gu)[D][D], const S (&Gk)[D],
-                                  const S (&Gw_rec)[D], S nu, double d, const double (&dgrad)[D], bool wall) {
+                                  const S (&Gw_rec)[D], S nu, DT d, const DT (&dgrad)[D], bool wall) {
   S A, B;
-  closure_products<D, S>(Gk, Gw_rec, dgrad, wall, A, B);
-  return sst_closure_core<S>(c, k, w, strain_magnitude<D, S>(Gu), A, B, nu, d, wall);
+  closure_products<D, S, DT>(Gk, Gw_rec, dgrad, wall, A, B);
+  return sst_closure_core<S, DT>(c, k, w, strain_magnitude<D, S>(Gu), A, B, nu, d, wall);
 }
 
 // Volume-integrated sources of one cell: out_k = KProduction + KDestruction, out_w = OmegaProduction
 // + OmegaDestruction + OmegaCrossDiffusion (each production positive; the residual subtracts them
 // in this order).  ``tau`` is the wall shear rate of a wall cell.
-template <class S>
-AFX_DEV void sst_sources(const SstConst& c, S k, S w, S nut, S strain, S f1, S cd, S tau, S nu, double d,
-                         bool wall, double V, S& prod_k, S& dest_k, S& prod_w, S& dest_w, S& cross_w) {
+template <class S, class VT = double, class DT = double>
+AFX_DEV void sst_sources(const SstConst& c, S k, S w, S nut, S strain, S f1, S cd, S tau, S nu, DT d,
+                         bool wall, VT V, S& prod_k, S& dest_k, S& prod_w, S& dest_w, S& cross_w) {
   S limited = dmin(nut * (strain * strain), kProductionLimitRatio * c.beta_star * kpos(k) * w);
   S wdis = w;
   if (wall) {

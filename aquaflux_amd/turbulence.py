@@ -131,20 +131,26 @@ def _scalar_face_data(bc: BoundaryConditions, mesh, what):
     return kind, value
 
 
-def wall_distance(mesh, geometry, wall_faces) -> torch.Tensor:
+def wall_distance(mesh, geometry, wall_faces, with_nearest=False):
     """Distance from each cell centroid to the nearest wall-face centroid (``mesh/distance.py:52-113``).
 
     A set-up product computed once per build on the host with a k-d tree, as the reference does for
-    concrete inputs; the result is ``sqrt(distance**2)`` like the reference's squared-distance path."""
+    concrete inputs; the result is ``sqrt(distance**2)`` like the reference's squared-distance path.
+    With ``with_nearest`` also the nearest wall face of each cell (mesh face index), along which a
+    node-coordinate tangent moves the distance (the reference's traced path differentiates the
+    same nearest-centroid distance, ``mesh/distance.py:115-140``)."""
     from scipy.spatial import cKDTree
 
     if len(wall_faces) == 0:
         raise ValueError("wall_distance: the wall patches contain no faces")
     fc = geometry.face.centroid_soa.detach().cpu().numpy().T[wall_faces]
     cc = geometry.cell.centroid_soa.detach().cpu().numpy().T
-    dist, _ = cKDTree(fc).query(cc)
+    dist, idx = cKDTree(fc).query(cc)
     d2 = torch.tensor(dist, dtype=F64) ** 2
-    return torch.sqrt(d2).to(geometry.face.area.device)
+    d = torch.sqrt(d2).to(geometry.face.area.device)
+    if with_nearest:
+        return d, torch.from_numpy(np.asarray(wall_faces)[idx]).to(geometry.face.area.device)
+    return d
 
 
 def scalar_gradient(mesh, geometry, gradient_scheme, phi, kind, value):
@@ -195,7 +201,7 @@ class SSTTurbulence:
         if np.any(mesh.neighbour[wall_faces] >= 0):
             raise ValueError("wall patches must be boundary faces")
         wall_cells = np.unique(mesh.owner[wall_faces])
-        d = wall_distance(mesh, geometry, wall_faces)
+        d, nearest = wall_distance(mesh, geometry, wall_faces, with_nearest=True)
         # wall-distance gradient: Dirichlet 0 on the walls, zero gradient elsewhere (transport.py:86-110)
         dkind = np.where(mesh.neighbour < 0, BC_ZERO_GRADIENT, 0).astype(np.int8)
         dkind[wall_faces] = BC_DIRICHLET
@@ -237,6 +243,9 @@ class SSTTurbulence:
             _omega_kind=torch.from_numpy(ok).to(dev),
             _omega_value=torch.from_numpy(ov).to(dev),
             _wall_face=torch.from_numpy(wall_face).to(dev),
+            _wall_nearest=nearest,
+            _dkind=dkind_t,
+            _dval=dval,
             _wall_cell=torch.from_numpy(wall_cell).to(dev),
         )
 
@@ -323,6 +332,54 @@ class _RansRaw:
         )
         return R, dR
 
+    def wall_distance_tangent(self, dgeo, gt):
+        """Tangents of the wall distance and of its gradient along the geometry tangents `dgeo`
+        (C struct `gt`): d = |x_c - x_f*| to the nearest wall-face centroid f*, and the gradient
+        scheme's tangent with the geometry and d moving together."""
+        t = self.keep[0]
+        g = t.geometry
+        near = t._wall_nearest
+        d = t.wall_distance
+        cc = fwAD.unpack_dual(g.cell.centroid_soa).primal
+        fc = fwAD.unpack_dual(g.face.centroid_soa).primal
+        diff = cc - fc[:, near]
+        ddiff = dgeo[4] - dgeo[2][:, near]
+        dd = ((diff * ddiff).sum(dim=0) / d).contiguous()
+        dgrad = torch.empty_like(t.wall_distance_gradient)
+        dgrad_t = torch.empty_like(dgrad)
+        _lib.check(
+            _lib.lib().afx_scalar_gradient_jvp_geometry(
+                self.dm.handle, ctypes.byref(self.gs), ctypes.byref(gt), int(self.cfg.flow.gradient),
+                t._dkind.data_ptr(), t._dval.data_ptr(), d.data_ptr(), dd.data_ptr(), dgrad.data_ptr(),
+                dgrad_t.data_ptr(), _lib.stream_of(self.dm.device),
+            ),
+            "afx_scalar_gradient_jvp_geometry",
+        )
+        return dd, dgrad_t
+
+    def jvp_geometry(self, primals, tangents, dgeo):
+        """(R, dR) along tangents of the inputs and of the five geometry arrays together; the wall
+        distance and its gradient move with the geometry."""
+        from .flow import _geometry_tangent
+
+        state = primals[0]
+        R = torch.empty_like(state)
+        dR = torch.empty_like(state)
+        w = self.dm.workspace("rans1", self.work_bytes[1])
+        ins = self.inputs(*primals)
+        tns = _lib.AfxRansInputs(*[_lib.ptr(t) for t in tangents], None, None)
+        gt = _geometry_tangent(self.dm, dgeo)
+        dd, dgrad_t = self.wall_distance_tangent(gt[1][0], gt[0])
+        _lib.check(
+            _lib.lib().afx_rans_jvp_geometry(
+                self.dm.handle, ctypes.byref(self.gs), ctypes.byref(gt[0]), dd.data_ptr(), dgrad_t.data_ptr(),
+                ctypes.byref(self.cfg), ctypes.byref(ins), ctypes.byref(tns), R.data_ptr(), dR.data_ptr(),
+                w.data_ptr(), w.numel(), _lib.stream_of(self.dm.device),
+            ),
+            "afx_rans_jvp_geometry",
+        )
+        return R, dR
+
     def vjp(self, primals, cot):
         bars = [torch.empty_like(p) for p in primals]
         w = self.dm.workspace("rans2", self.work_bytes[2])
@@ -377,30 +434,39 @@ class _RansRaw:
 
 
 class _RansResidualFn(torch.autograd.Function):
-    """R(state, mu, rho, nu, u_b, p_b; geometry arrays) with HIP forward, tangent and adjoint.  The
-    geometry arrays are inputs only so that a derivative through them is refused (the SST residual
-    takes the geometry as fixed), never silently dropped."""
+    """R(state, mu, rho, nu, u_b, p_b; area, normal, face centroid, volume, cell centroid) with HIP
+    forward, tangent and adjoint.  Forward mode follows geometry tangents (``geo_tan``); reverse
+    mode through the geometry is refused, never silently dropped."""
 
     @staticmethod
-    def forward(ctx, raw, state, mu, rho, nu, ub, pb, *geo):
+    def forward(ctx, raw, geo_tan, state, mu, rho, nu, ub, pb, *geo):
         ctx.raw = raw
+        ctx.geo_tan = geo_tan
         prim = (state, mu, rho, nu, ub, pb)
         ctx.save_for_backward(*prim)
-        ctx.save_for_forward(*prim)
+        ctx.save_for_forward(*prim, *geo)
         return raw.residual(*prim)
 
     @staticmethod
     def backward(ctx, gR):
-        if any(ctx.needs_input_grad[7:]):
-            raise NotImplementedError("derivatives of the RANS residual with respect to the geometry (node "
-                                      "coordinates) are not provided; detach the geometry")
+        if any(ctx.needs_input_grad[8:]):
+            raise NotImplementedError("reverse-mode derivatives of the RANS residual with respect to the geometry "
+                                      "(node coordinates) are not provided; detach the geometry, or use forward "
+                                      "mode")
         bars = ctx.raw.vjp(ctx.saved_tensors, gR.contiguous())
-        return (None, *bars) + (None,) * (len(ctx.needs_input_grad) - 7)
+        return (None, None, *bars) + (None,) * (len(ctx.needs_input_grad) - 8)
 
     @staticmethod
-    def jvp(ctx, _raw_t, *tans):
+    def jvp(ctx, _raw_t, _geo_tan_t, *tans):
+        saved = ctx.saved_tensors
+        prim, geo = saved[:6], saved[6:]
+        dgeo = tans[6:]
         tans = [t.contiguous() if t is not None else None for t in tans[:6]]
-        _, dR = ctx.raw.jvp(ctx.saved_tensors, tans)
+        if ctx.geo_tan:
+            dgeo = [torch.zeros_like(g) if t is None else t for g, t in zip(geo, dgeo)]
+            _, dR = ctx.raw.jvp_geometry(prim, tans, dgeo)
+        else:  # constant geometry: torch fills its tangents with zeros, which are not read
+            _, dR = ctx.raw.jvp(prim, tans)
         return dR
 
 
@@ -483,10 +549,12 @@ class CoupledRANS:
         mu, rho, nu, ub, pb = self._inputs()
         g = self.momentum.geometry
         geo = (g.face.area, g.face.normal_soa, g.face.centroid_soa, g.cell.volume, g.cell.centroid_soa)
-        if any(fwAD.unpack_dual(t).tangent is not None for t in geo):
-            raise NotImplementedError("derivatives of the RANS residual with respect to the geometry (node "
-                                      "coordinates) are not provided; build it on a geometry without tangents")
-        return _RansResidualFn.apply(self._raw, state.contiguous(), mu, rho, nu, ub, pb, *geo)
+        geo_tan = any(fwAD.unpack_dual(t).tangent is not None for t in geo)
+        if geo_tan and self._raw.cfg.flow.gradient == 3:
+            raise NotImplementedError("forward-mode derivatives with respect to the geometry are provided for "
+                                      "compact Green-Gauss or least-squares gradients, not the corrected "
+                                      "Green-Gauss scheme")
+        return _RansResidualFn.apply(self._raw, geo_tan, state.contiguous(), mu, rho, nu, ub, pb, *geo)
 
     __call__ = residual
 

@@ -238,6 +238,16 @@ int afx_scalar_gradient(const afx_mesh* mesh, const afx_geometry* geom, int grad
                         const int8_t* kind, const double* value, const double* phi, double* grad,
                         hipStream_t stream);
 
+/* Gradient and its tangent along a tangent of phi and of the geometry together (geom_tangent as
+ * for afx_scalar_jvp_geometry; boundary values constant).  Compact Green-Gauss or least squares.
+ * Replaces jax.jvp of GradientScheme.gradients with Mesh.node_coords as a differentiable leaf (the
+ * SST wall-distance gradient of a differentiated build; reference: aquaflux/schemes/gradient.py:98-153,
+ * aquaflux/turbulence/transport.py:259-313). */
+int afx_scalar_gradient_jvp_geometry(const afx_mesh* mesh, const afx_geometry* geom, const afx_geometry* geom_tangent,
+                                     int gradient, const int8_t* kind, const double* value, const double* phi,
+                                     const double* phi_tangent, double* grad, double* grad_tangent,
+                                     hipStream_t stream);
+
 /* Reverse of afx_scalar_gradient (same scheme, closures and geometry): phi_bar[C] and, when not
  * NULL, value_bar[F] (zero except on Dirichlet faces, caller order) from grad_bar[j][C].  The
  * gradient is linear in (phi, value), so this is its transpose.  Replaces jax.vjp through
@@ -397,6 +407,18 @@ int afx_rans_residual(const afx_mesh* mesh, const afx_geometry* geom, const afx_
 int afx_rans_jvp(const afx_mesh* mesh, const afx_geometry* geom, const afx_rans_config* cfg,
                  const afx_rans_inputs* in, const afx_rans_inputs* tangent, double* residual,
                  double* residual_tangent, void* work, int64_t work_bytes, hipStream_t stream);
+
+/* R and dR along tangents of the inputs and of the geometry together: geom_tangent as for
+ * afx_flow_jvp_geometry, plus the tangents of the configuration's wall distance [C] and wall
+ * distance gradient [d][C] (a differentiated build moves them with the nodes: reference
+ * aquaflux/mesh/distance.py:87-140).  Compact Green-Gauss or least-squares gradients; workspace of
+ * mode 1.  Replaces jax.jvp of CoupledRANS.residual with Mesh.node_coords as a differentiable leaf
+ * (reference: aquaflux/mesh/mesh.py:63-66, aquaflux/turbulence/coupled.py:437-473). */
+int afx_rans_jvp_geometry(const afx_mesh* mesh, const afx_geometry* geom, const afx_geometry* geom_tangent,
+                          const double* wall_distance_tangent, const double* wall_distance_gradient_tangent,
+                          const afx_rans_config* cfg, const afx_rans_inputs* in, const afx_rans_inputs* tangent,
+                          double* residual, double* residual_tangent, void* work, int64_t work_bytes,
+                          hipStream_t stream);
 
 /* J^T . cotangent with respect to state, mu, rho, nu, u_b, p_b.  Replaces jax.vjp of
  * CoupledRANS.residual (reference: aquaflux/solve/implicit.py:733-761). */

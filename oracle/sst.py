@@ -243,6 +243,15 @@ def wall_distance(m, g, wall_faces):
     return torch.sqrt(torch.tensor(dist, dtype=F64) ** 2)
 
 
+def wall_distance_traced(m, g, wall_faces):
+    """``mesh/distance.py:87-140`` (the traced path): sqrt of the smallest squared distance from each
+    cell centroid to a wall-face centroid, as a differentiable search (brute force; small meshes)."""
+    diff = g.cell_centroid[:, None, :] - g.face_centroid[wall_faces][None, :, :]
+    d2 = (diff * diff).sum(dim=-1)
+    idx = torch.argmin(d2.detach(), dim=1, keepdim=True)
+    return torch.sqrt(torch.gather(d2, 1, idx).squeeze(1))
+
+
 def gradient_spec(s: fv.FlowSetup):
     """The flow setup's gradient scheme as scalar_gradient takes it."""
     if s.gradient == "corrected":
@@ -262,11 +271,12 @@ def scalar_gradient(m, g, gradient, phi, bc: fv.ScalarSetup):
     return fv.compact_green_gauss(m, g, phi, lead)
 
 
-def build_sst_setup(m, g, gradient, model, nu, density, wall_faces, k_bc, omega_bc):
-    """``SSTTurbulence.build`` (``transport.py:259-313``)."""
+def build_sst_setup(m, g, gradient, model, nu, density, wall_faces, k_bc, omega_bc, traced=False):
+    """``SSTTurbulence.build`` (``transport.py:259-313``); ``traced`` computes the wall distance
+    the way a differentiated build does (``mesh/distance.py:87-140``)."""
     wall_faces = torch.as_tensor(wall_faces, dtype=torch.long)
     wall_cells = torch.unique(m.owner[wall_faces])
-    d = wall_distance(m, g, wall_faces)
+    d = wall_distance_traced(m, g, wall_faces) if traced else wall_distance(m, g, wall_faces)
     kind = torch.full((m.n_faces,), fv.BC_ZERO_GRADIENT, dtype=torch.long)
     kind[wall_faces] = fv.BC_DIRICHLET
     dbc = fv.ScalarSetup(kind=kind, value=torch.zeros(m.n_faces, dtype=F64), t_inf=torch.zeros(m.n_faces, dtype=F64))

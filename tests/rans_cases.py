@@ -95,6 +95,27 @@ def oracle_case(name, flow_scheme="upwind", gradient="gg", nu=1e-2, rho=1.0):
     return mesh, (m, g, s, _oracle_sst(mesh, m, g, s, walls, kb, wb, nu, rho))
 
 
+def oracle_node_function(name, flow_scheme, scalar_scheme, gradient, state, nu=1e-2, rho=1.0):
+    """X -> R(state; X) of the oracle with everything the node coordinates move rebuilt inside:
+    geometry, and the SST set-up's wall distance (the reference's traced path) and its gradient."""
+    base, walls, k_in, w_in = CASES[name]
+    mesh, (m0, g0, s) = flow_cases.oracle_case(base, flow_scheme, gradient)
+    kb = _oracle_scalar(mesh, _scalar_bcs(mesh, walls, k_in, omega=False))
+    wb = _oracle_scalar(mesh, _scalar_bcs(mesh, walls, w_in, omega=True))
+    wall_faces = np.concatenate([np.asarray(mesh.face_patches.indices(w), dtype=np.int64) for w in walls])
+    C = mesh.n_cells
+    nu_t, mu, rh = torch.full((C,), nu, dtype=F64), torch.full((C,), nu * rho, dtype=F64), torch.full((C,), rho,
+                                                                                                       dtype=F64)
+
+    def f(X):
+        m = fv.OMesh.from_mesh(mesh, node_coords=X)
+        g = fv.geometry(m)
+        t = S.build_sst_setup(m, g, S.gradient_spec(s), S.SSTModel(), nu_t, rho, wall_faces, kb, wb, traced=True)
+        return S.rans_residual(m, g, s, t, state, mu, rh, scalar_scheme)
+
+    return mesh, f
+
+
 def seeded_state(mesh, seed=0, kind="channel"):
     """Turbulent-like seeded state: u = 1 + 0.05 N(0,1), v(,w) = 0.05 N(0,1), p = N(0, 0.1),
     k ~ LogU(1e-5, 1e-2), omega ~ LogU(1, 1e3); the cavity takes U(-0.5, 0.5) velocities."""

@@ -321,8 +321,8 @@ def test_node_coordinate_adjoint_refused_for_corrected_green_gauss(cuda):
 
 
 def test_rans_geometry_derivatives_are_refused_not_dropped(cuda):
-    """Any derivative through the RANS geometry raises instead of returning a result that leaves
-    the geometry's part out."""
+    """Reverse mode through the RANS geometry, and forward mode with corrected Green-Gauss, raise
+    instead of returning a result that leaves the geometry's part out."""
     import torch.autograd.forward_ad as fwAD
 
     from tests import rans_cases as RC
@@ -336,7 +336,7 @@ def test_rans_geometry_derivatives_are_refused_not_dropped(cuda):
         torch.autograd.grad(rans.residual(state).sum(), X2)
     with fwAD.dual_level():
         X3 = torch.tensor(mesh.node_coords, dtype=F64, device=cuda)
-        _, rans3, _ = RC.build("channel2d_sst", "limited", "limited", "lsq", device=cuda,
+        _, rans3, _ = RC.build("channel2d_sst", "limited", "limited", "cgg", device=cuda,
                                node_coords=fwAD.make_dual(X3, torch.ones_like(X3)))
-        with pytest.raises(NotImplementedError, match="geometry"):
+        with pytest.raises(NotImplementedError, match="corrected Green-Gauss"):
             rans3.residual(state)

@@ -257,3 +257,28 @@ def test_fixation_rows_known_answers():
     assert torch.equal(DirectScalars().to_physical(ww), ww)
     assert torch.allclose(LogScalars().to_solved(LogScalars().to_physical(ww)), ww)
     assert isinstance(LogScalars().fixation_row(), LogRatioRow)
+
+
+def test_traced_wall_distance_equals_the_kd_tree_and_its_derivative():
+    """The differentiable nearest-centroid search (mesh/distance.py:87-140) against scipy's k-d tree
+    (the concrete path) on a perturbed 3D channel, and its node-coordinate tangent against central
+    differences (away from ties the nearest face does not change)."""
+    from tests.flow_cases import make_mesh
+
+    mesh = make_mesh("channel3d")
+    walls = np.concatenate([np.asarray(mesh.face_patches.indices(p), dtype=np.int64)
+                            for p in ("bottom", "top", "back", "front")])
+    X0 = torch.tensor(mesh.node_coords, dtype=F64)
+
+    def d_of(X):
+        m = fv.OMesh.from_mesh(mesh, node_coords=X)
+        return S.wall_distance_traced(m, fv.geometry(m), torch.as_tensor(walls))
+
+    m0 = fv.OMesh.from_mesh(mesh)
+    ref = S.wall_distance(m0, fv.geometry(m0), torch.as_tensor(walls))
+    assert float((d_of(X0) - ref).abs().max()) <= 1e-15 * float(ref.abs().max())
+    dX = torch.tensor(np.random.default_rng(2).normal(size=X0.shape)) * 1e-2
+    _, dd = torch.func.jvp(d_of, (X0,), (dX,))
+    h = 1e-6
+    fd = (d_of(X0 + h * dX) - d_of(X0 - h * dX)) / (2 * h)
+    assert float((fd - dd).abs().max()) <= 1e-7 * float(dd.abs().max())
