@@ -427,6 +427,31 @@ struct GeoRead<D, true> {
   }
 };
 
+// Least-squares fit direction cotangent of one cell's fit (all variables, shared M):
+// dbar = sum_v w (r_v y_v - (y_v . d) G_v) - 2 w^2 (y_v . d) r_v d, r_v = dphi_v - G_v . d.
+template <int D, int NV>
+AFX_DEV void lsq_dir_bar(const double (&d)[D], const double (&dphi)[NV], const double (&G)[NV][D],
+                         const double (&y)[NV][D], double (&db)[D]) {
+  double dd = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) dd += d[j] * d[j];
+  const double w = 1.0 / dd;
+#pragma unroll
+  for (int j = 0; j < D; ++j) db[j] = 0.0;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    double gd = 0.0, yd = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      gd += G[v][j] * d[j];
+      yd += y[v][j] * d[j];
+    }
+    const double r = dphi[v] - gd;
+#pragma unroll
+    for (int j = 0; j < D; ++j) db[j] += w * (r * y[v][j] - yd * G[v][j]) - 2.0 * w * w * yd * r * d[j];
+  }
+}
+
 // Decodes one entry of the cell -> face list.
 // `f` indexes the view's face arrays (geometry, per-face workspace); `fu` indexes the caller's
 // per-face arrays (boundary kinds and values, face fluxes and their cotangents).

@@ -19,6 +19,7 @@
 #include "afx_mesh.h"
 #include "afx_flow_face.h"
 #include "afx_scalar_face.h"
+#include "afx_scalar_adj.h"
 #include "afx_sst.h"
 #include "afx_profile.h"
 #include "afx_cgg.h"
@@ -1506,6 +1507,448 @@ __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<doub
 }
 
 // ---------------------------------------------------------------------------------------------
+// geometry cotangents (node-coordinate VJP).  Three passes after the state adjoint, reading its
+// intermediates: the closure-field, a_P, limiter and face mass-flux cotangents, and the gradient
+// cotangents pre-multiplied by the reconstruction operator (y).  Each location has one writer per
+// pass: the cell pass writes (=) the cell arrays, the face and gradient passes add to them, and a
+// face's arrays are written by its owner.  Besides the five geometry arrays the passes produce the
+// cotangents of the wall distance and of its gradient, which the caller carries through the
+// distance and gradient set-up to the geometry.
+struct RansGeoBars {
+  double *area, *nrm, *fc;  // [F], [D][F], [D][F] caller face order
+  double *vol, *cc;         // [C], [D][C]
+  double *dwall, *dgrad;    // [C], [D][C] wall distance and its gradient
+  double* limx;             // [NL][D][C] scratch: limiter face-centroid cotangent at its argmin face
+  int32_t* limf;            // [NL][C]    scratch: that face (view index, -1 none)
+  double* tauA;             // [C]        scratch: tau cotangent / the cell's wall area
+};
+
+// Cell-local terms: source and body-force volumes, Green-Gauss 1/V, the closure's, sources' and
+// wall fixation's wall-distance derivatives, the wall shear rate's, and the limiters.
+template <int D, bool LFV, bool LSV, int GRAD>
+__global__ void __launch_bounds__(256) k_rans_geo_cell(MeshView m, RansArgs<double> a, const double* __restrict__ lam,
+                                                       const int32_t* __restrict__ ext,
+                                                       const int32_t* __restrict__ kmin_pos, RansBars bar,
+                                                       RansGeoBars gb) {
+  constexpr int NA = D + 3, NL = D + 2;
+  const SstConst& sc = a.sc;
+  const int32_t c = xcd_thread_index(m);
+  if (c >= m.C) return;
+  const bool wall = a.wallc[c] != 0;
+  const double V = m.vol[c], d = a.dwall[c];
+  double Vb = 0.0, db = 0.0, dgb[D], ccb[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) dgb[j] = ccb[j] = 0.0;
+  const double lk = lam[(int64_t)(D + 1) * m.C + c];
+  const double lw_row = lam[(int64_t)(D + 2) * m.C + c];
+  const double lw = wall ? 0.0 : lw_row;  // the omega row's transport / source part
+#pragma unroll
+  for (int i = 0; i < D; ++i) Vb -= a.bf[i] * lam[(int64_t)i * m.C + c];
+  if (GRAD == 0) {  // G = acc / V
+#pragma unroll
+    for (int v = 0; v < NA; ++v)
+#pragma unroll
+      for (int j = 0; j < D; ++j) Vb -= bar.G[(int64_t)(v * D + j) * m.C + c] * a.G[(int64_t)(v * D + j) * m.C + c];
+  }
+  const double kc = a.x[(int64_t)(D + 1) * m.C + c], wc = a.x[(int64_t)(D + 2) * m.C + c];
+  const double nu = a.nu[c];
+  {  // sources: linear in V; on wall cells their near-wall blends also read d
+    double pk, dk, pw, dw, cw;
+    sst_sources<double>(sc, kc, wc, a.nut[c], a.Sr[c], a.f1[c], a.cd[c], a.tau[c], nu, d, wall, 1.0, pk, dk, pw, dw,
+                        cw);
+    Vb -= lk * (pk + dk) + lw * ((pw + dw) + cw);
+    if (wall) {
+      dual sk, sdk, sw, sdw, scw;
+      sst_sources<dual>(sc, dual{kc, 0.0}, dual{wc, 0.0}, dual{a.nut[c], 0.0}, dual{a.Sr[c], 0.0},
+                        dual{a.f1[c], 0.0}, dual{a.cd[c], 0.0}, dual{a.tau[c], 0.0}, dual{nu, 0.0}, dual{d, 1.0},
+                        wall, V, sk, sdk, sw, sdw, scw);
+      db -= lk * (sk.d + sdk.d) + lw * ((sw.d + sdw.d) + scw.d);
+      // omega wall fixation: w - omega_wall(nu, d, k), or log(w / omega_wall)
+      const dual t2 = omega_wall_k<dual>(sc, dual{nu, 0.0}, dual{d, 1.0}, dual{kc, 0.0});
+      db -= a.log_w ? lw_row * t2.d / t2.v : lw_row * t2.d;
+    }
+  }
+  {  // closure (nu_t, F1, strain, CD, wall nu_t) and the wall shear rate
+    dual Gu[D][D], Gk[D], Gw[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) Gu[i][j] = dual{a.G[(int64_t)(i * D + j) * m.C + c], 0.0};
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      Gk[j] = dual{a.G[(int64_t)((D + 1) * D + j) * m.C + c], 0.0};
+      Gw[j] = dual{a.G[(int64_t)((D + 2) * D + j) * m.C + c], 0.0};
+    }
+    const double nut_b = bar.nut[c] + a.rho[c] * bar.mueff[c];
+    const double f1_b = bar.f1[c], Sr_b = bar.Sr[c], cd_b = bar.cd[c], nutw_b = bar.nutw[c];
+    // seed 0: d; seeds 1..D: the wall-distance gradient (read on wall cells only)
+    for (int s = 0; s <= (wall ? D : 0); ++s) {
+      dual dd = dual{d, s == 0 ? 1.0 : 0.0}, dg[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) dg[j] = dual{wall ? a.dgrad[(int64_t)j * m.C + c] : 0.0, s == j + 1 ? 1.0 : 0.0};
+      const SstClosure<dual> o =
+          sst_closure<D, dual, dual>(sc, dual{kc, 0.0}, dual{wc, 0.0}, Gu, Gk, Gw, dual{nu, 0.0}, dd, dg, wall);
+      const double g = nut_b * o.nut.d + f1_b * o.f1.d + Sr_b * o.strain.d + cd_b * o.cd.d + nutw_b * o.nutw.d;
+      if (s == 0)
+        db += g;
+      else
+        dgb[s - 1] += g;
+    }
+    if (wall) {  // tau = sum_f (|slip| / d) A_f / sum_f A_f
+      const double tau_b = bar.tau[c];
+      double den = 0.0;
+      for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+        const int32_t e = m.cface[k];
+        if (e >= 0 && a.wallf[user_face(m, e)]) den += m.area[e];
+      }
+      db -= tau_b * a.tau[c] / d;
+      gb.tauA[c] = den > 0.0 ? tau_b / den : 0.0;
+    } else {
+      gb.tauA[c] = 0.0;
+    }
+  }
+  if constexpr (LFV || LSV) {  // limiters at their recorded argmin faces
+    const double k3 = a.lim_k * a.lim_k * a.lim_k, e2 = V * k3;
+    double xcen[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) xcen[j] = m.cc[(int64_t)j * m.C + c];
+#pragma unroll
+    for (int v = 0; v < NA; ++v) {
+      const int l = lim_slot<D>(v, LFV, LSV);
+      if (l < 0) continue;
+      const double pbar = bar.psi[(int64_t)l * m.C + c];
+      const int32_t amx = ext[(int64_t)l * m.C + c], amn = ext[(int64_t)(NL + l) * m.C + c];
+      const double xv = a.x[(int64_t)v * m.C + c];
+      const double mxv = a.x[(int64_t)v * m.C + amx], mnv = a.x[(int64_t)v * m.C + amn];
+      double Gv[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) Gv[j] = a.G[(int64_t)(v * D + j) * m.C + c];
+      const int32_t e = m.cface[m.cptr[c] + kmin_pos[(int64_t)l * m.C + c]];
+      const int32_t f = e >= 0 ? e : ~e;
+      double dxf[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) dxf[j] = m.fc[(int64_t)j * m.F + f] - xcen[j];
+      double dmf = Gv[0] * dxf[0];
+#pragma unroll
+      for (int j = 1; j < D; ++j) dmf = dmf + Gv[j] * dxf[j];
+      const double lo = venkat_face<double>(dmf, mxv - xv, mnv - xv, e2);
+      const double dclip = (lo > 0.0 && lo < 1.0) ? 1.0 : ((lo == 0.0 || lo == 1.0) ? 0.5 : 0.0);
+      const double vbar = pbar * dclip;
+      double xb[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) xb[j] = 0.0;
+      if (vbar != 0.0) {
+        const dual r_dm = venkat_face<dual>(dual{dmf, 1.0}, dual{mxv - xv, 0.0}, dual{mnv - xv, 0.0}, e2);
+        const dual r_e2 =
+            venkat_face<dual, dual>(dual{dmf, 0.0}, dual{mxv - xv, 0.0}, dual{mnv - xv, 0.0}, dual{e2, 1.0});
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          xb[j] = vbar * r_dm.d * Gv[j];
+          ccb[j] -= xb[j];
+        }
+        Vb += vbar * r_e2.d * k3;
+      }
+      gb.limf[(int64_t)l * m.C + c] = vbar != 0.0 ? f : -1;
+#pragma unroll
+      for (int j = 0; j < D; ++j) gb.limx[(int64_t)(l * D + j) * m.C + c] = xb[j];
+    }
+  }
+  gb.vol[c] = Vb;
+  gb.dwall[c] = db;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    gb.cc[(int64_t)j * m.C + c] = ccb[j];
+    gb.dgrad[(int64_t)j * m.C + c] = dgb[j];
+  }
+}
+
+// Face fluxes: momentum / continuity with their face mass flux (whose cotangent also carries the
+// k / omega transport's), the momentum diagonal, the k / omega fluxes, and the wall shear rate's
+// face areas.  Writes the owned faces' arrays (=) and adds the cell's centroid / volume / distance
+// parts.
+template <int D, int ADVF, int ADVS>
+__global__ void __launch_bounds__(256) k_rans_geo_face(MeshView m, RansArgs<double> a, const double* __restrict__ lam,
+                                                       const double* __restrict__ mdot,
+                                                       const double* __restrict__ mdot_bar, RansBars bar,
+                                                       RansGeoBars gb) {
+  constexpr int NF = D + 1;
+  const SstConst& sc = a.sc;
+  const int32_t c = xcd_thread_index(m);
+  if (c >= m.C) return;
+  const bool wall = a.wallc[c] != 0;
+  FlowCell<D, double> X;
+  load_flow<D, double>(a, m, c, true, ADVF == kAdvLimited, X);
+  ScalarCell<D, double> Kc, Wc;
+  load_scalar<D, double>(a, m, c, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Kc);
+  load_scalar<D, double>(a, m, c, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wc);
+  double lc[NF];
+#pragma unroll
+  for (int v = 0; v < NF; ++v) lc[v] = lam[(int64_t)v * m.C + c];
+  if (c == a.pin) lc[D] = 0.0;
+  const double lck = lam[(int64_t)(D + 1) * m.C + c];
+  const double lcw = wall ? 0.0 : lam[(int64_t)(D + 2) * m.C + c];
+  const double apb_c = bar.aP[c], irho = 1.0 / sc.density, dP = a.dwall[c], tauA = gb.tauA[c];
+  double Vb = 0.0, db = 0.0, ccb[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) ccb[j] = 0.0;
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
+    const int32_t o = r.own ? r.N : r.P;
+    FaceGeo<D> q;
+    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    FaceGeoBarF<D> qf;
+    qf.zero();
+    FaceGeoBar<D> qs;
+    qs.zero();
+    ScalarBar<D> Bd;
+    Bd.zero();
+    const double Qf = mdot[r.f] * irho;
+    double xfb[D];  // face-centroid part the folds do not see (none here; kept for symmetry)
+#pragma unroll
+    for (int j = 0; j < D; ++j) xfb[j] = 0.0;
+    if (o >= 0) {
+      FlowCell<D, double> O;
+      load_flow<D, double>(a, m, o, true, ADVF == kAdvLimited, O);
+      double lf[NF];
+#pragma unroll
+      for (int v = 0; v < NF; ++v) {
+        double lo = lam[(int64_t)v * m.C + o];
+        if (v == D && o == a.pin) lo = 0.0;
+        lf[v] = r.own ? lc[v] - lo : lo - lc[v];
+      }
+      lf[D] += mdot_bar[r.f];
+      const FlowCell<D, double>& P = r.own ? X : O;
+      const FlowCell<D, double>& N = r.own ? O : X;
+      double VPb = 0.0, VNb = 0.0;
+      geo_adj_flux_interior<D, ADVF>(q, P, N, lf, qf, VPb, VNb);
+      const double apb_o = bar.aP[o];
+      geo_adj_ap_interior<D, ADVF>(q, P, N, r.own ? apb_c : apb_o, r.own ? apb_o : apb_c, qf);
+      Vb += r.own ? VPb : VNb;
+      ScalarCell<D, double> Ko, Wo;
+      load_scalar<D, double>(a, m, o, D + 1, ADVS == kAdvLimited, sc.sk1, sc.sk2, Ko);
+      load_scalar<D, double>(a, m, o, D + 2, ADVS == kAdvLimited, sc.sw1, sc.sw2, Wo);
+      const double lk_o = lam[(int64_t)(D + 1) * m.C + o];
+      const double lw_o = a.wallc[o] ? 0.0 : lam[(int64_t)(D + 2) * m.C + o];
+      const double lkf = r.own ? lck - lk_o : lk_o - lck, lwf = r.own ? lcw - lw_o : lw_o - lcw;
+      const ScalarCell<D, double>& KP = r.own ? Kc : Ko;
+      const ScalarCell<D, double>& KN = r.own ? Ko : Kc;
+      const ScalarCell<D, double>& WP = r.own ? Wc : Wo;
+      const ScalarCell<D, double>& WN = r.own ? Wo : Wc;
+      adj_sc_interior<D, ADVS, true>(q, Qf, KP.phi, KN.phi, KP.G, KN.G, KP.psi, KN.psi, KP.gam, KN.gam, lkf, 0, Bd,
+                                     qs);
+      adj_sc_interior<D, ADVS, true>(q, Qf, WP.phi, WN.phi, WP.G, WN.G, WP.psi, WN.psi, WP.gam, WN.gam, lwf, 0, Bd,
+                                     qs);
+      fold_face_geo_bar<D>(q, true, qf);
+      adj_face_geo<D>(q, true, qs);
+#pragma unroll
+      for (int j = 0; j < D; ++j) ccb[j] += r.own ? qf.xP[j] + qs.xP[j] : qf.xN[j] + qs.xN[j];
+    } else {
+      const int kind = a.kind[r.fu];
+      double lf[NF], ub[D];
+#pragma unroll
+      for (int v = 0; v < NF; ++v) lf[v] = lc[v];
+      lf[D] += mdot_bar[r.f];
+#pragma unroll
+      for (int i = 0; i < D; ++i) ub[i] = a.ub[(int64_t)i * m.F + r.fu];
+      const double gB = rans_gamma_b<double>(a, kind, r.fu, c, X.mu);
+      double VPb = 0.0;
+      geo_adj_flux_boundary<D, ADVF>(q, kind, X, ub, a.pb[r.fu], gB, lf, qf, VPb);
+      geo_adj_ap_boundary<D, ADVF>(q, kind, X, gB, apb_c, qf);
+      Vb += VPb;
+      // k face (wall-function-reduced diffusivity on the SST wall faces)
+      const bool wf = a.wallf[r.fu] != 0;
+      double gk = Kc.gam, vb, tb;
+      if (wf) gk = (1.0 - wall_function_weight<double>(sc, a.nu[c], dP, Kc.phi)) * Kc.gam;
+      adj_sc_boundary<D, ADVS, true>(a.kbc[r.fu], q, Qf, Kc.phi, Kc.G, Kc.psi, gk, a.kval[r.fu], 0.0, lck, Bd, vb, tb,
+                                     qs);
+      if (wf) {  // d gk / d d = -gamma_k d w / d d
+        const dual wd = wall_function_weight<dual>(sc, dual{a.nu[c], 0.0}, dual{dP, 1.0}, dual{Kc.phi, 0.0});
+        db -= Bd.gam * Kc.gam * wd.d;
+      }
+      adj_sc_boundary<D, ADVS, true>(a.wbc[r.fu], q, Qf, Wc.phi, Wc.G, Wc.psi, Wc.gam, a.wval[r.fu], 0.0, lcw, Bd,
+                                     vb, tb, qs);
+      fold_face_geo_bar<D>(q, false, qf);
+      adj_face_geo<D>(q, false, qs);
+#pragma unroll
+      for (int j = 0; j < D; ++j) ccb[j] += qf.xP[j] + qs.xP[j];
+      if (wall && wf && tauA != 0.0) {  // tau = sum_f r_f A_f / sum_f A_f: d tau / d A_f = (r_f - tau) / sum A
+        double xcl[D + 1], s2 = 0.0;
+#pragma unroll
+        for (int v = 0; v <= D; ++v) xcl[v] = a.x[(int64_t)v * m.C + c];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          const double sl = xcl[j] - rans_bval_lead<D, double>(a, j, m.F, r.fu, xcl);
+          s2 += sl * sl;
+        }
+        qf.A += tauA * (safe_sqrt(s2) / dP - a.tau[c]);
+      }
+    }
+    if (r.own) {
+      gb.area[r.fu] = qf.A + qs.A;
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        gb.nrm[(int64_t)j * m.F + r.fu] = qf.n[j] + qs.n[j];
+        gb.fc[(int64_t)j * m.F + r.fu] = (qf.xf[j] + qs.xf[j]) + xfb[j];
+      }
+    }
+  }
+  gb.vol[c] += Vb;
+  gb.dwall[c] += db;
+#pragma unroll
+  for (int j = 0; j < D; ++j) gb.cc[(int64_t)j * m.C + c] += ccb[j];
+}
+
+// Gradients of all D + 3 fields (Green-Gauss: the face sums and 1/V was in the cell pass; least
+// squares: the fit directions) and the limiters' face-centroid parts.  Adds to the arrays.
+template <int D, int GRAD, bool LFV, bool LSV>
+__global__ void __launch_bounds__(256) k_rans_geo_grad(MeshView m, RansArgs<double> a, RansBars bar, RansGeoBars gb) {
+  constexpr int NA = D + 3, NL = D + 2;
+  const int32_t c = xcd_thread_index(m);
+  if (c >= m.C) return;
+  double ccb[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) ccb[j] = 0.0;
+  double xc[NA];
+#pragma unroll
+  for (int v = 0; v < NA; ++v) xc[v] = a.x[(int64_t)v * m.C + c];
+  for (FaceWalk walk(m, c); walk.more();) {
+    FaceRef r = walk.next(m, c);
+    const int32_t o = r.own ? r.N : r.P;
+    double Ab = 0.0, nb[D], xfb[D], gbar = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) nb[j] = xfb[j] = 0.0;
+    FaceGeo<D> q;
+    load_face_geo<D>(m, r.f, r.P, r.N, q);
+    if (o >= 0) {
+      if (GRAD == 0) {  // acc_P += phi_f n A, acc_N -= phi_f n A
+        const double w = 1.0 - q.g;
+#pragma unroll
+        for (int v = 0; v < NA; ++v) {
+          const double xo = a.x[(int64_t)v * m.C + o];
+          const double phP = r.own ? xc[v] : xo, phN = r.own ? xo : xc[v];
+          const double phf = w * phP + q.g * phN;
+          double zn = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+            const double yc = bar.G[(int64_t)(v * D + j) * m.C + c], yo = bar.G[(int64_t)(v * D + j) * m.C + o];
+            const double z = r.own ? yc - yo : yo - yc;
+            nb[j] += z * phf * q.A;
+            zn += z * q.n[j];
+          }
+          Ab += zn * phf;
+          gbar += zn * q.A * (phN - phP);
+        }
+      } else {
+        double d[D], dphi[NA], db[D], Gc[NA][D], yc[NA][D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) d[j] = m.cc[(int64_t)j * m.C + o] - m.cc[(int64_t)j * m.C + c];
+#pragma unroll
+        for (int v = 0; v < NA; ++v) {
+          dphi[v] = a.x[(int64_t)v * m.C + o] - xc[v];
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+            Gc[v][j] = a.G[(int64_t)(v * D + j) * m.C + c];
+            yc[v][j] = bar.G[(int64_t)(v * D + j) * m.C + c];
+          }
+        }
+        lsq_dir_bar<D, NA>(d, dphi, Gc, yc, db);  // own fit: d = x_o - x_c
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          ccb[j] -= db[j];
+          d[j] = -d[j];
+        }
+#pragma unroll
+        for (int v = 0; v < NA; ++v) {
+          dphi[v] = -dphi[v];
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+            Gc[v][j] = a.G[(int64_t)(v * D + j) * m.C + o];
+            yc[v][j] = bar.G[(int64_t)(v * D + j) * m.C + o];
+          }
+        }
+        lsq_dir_bar<D, NA>(d, dphi, Gc, yc, db);  // the other cell's fit: d = x_c - x_o
+#pragma unroll
+        for (int j = 0; j < D; ++j) ccb[j] += db[j];
+      }
+    } else {
+      double phb[NA];
+#pragma unroll
+      for (int v = 0; v < NA; ++v) phb[v] = rans_bval_lead<D, double>(a, v, m.F, r.fu, xc);
+      if (GRAD == 0) {
+#pragma unroll
+        for (int v = 0; v < NA; ++v) {
+          double zn = 0.0;
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+            const double y = bar.G[(int64_t)(v * D + j) * m.C + c];
+            nb[j] += y * phb[v] * q.A;
+            zn += y * q.n[j];
+          }
+          Ab += zn * phb[v];
+        }
+      } else {
+        double d[D], dphi[NA], db[D], Gc[NA][D], yc[NA][D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) d[j] = q.xf[j] - m.cc[(int64_t)j * m.C + c];
+#pragma unroll
+        for (int v = 0; v < NA; ++v) {
+          dphi[v] = phb[v] - xc[v];
+#pragma unroll
+          for (int j = 0; j < D; ++j) {
+            Gc[v][j] = a.G[(int64_t)(v * D + j) * m.C + c];
+            yc[v][j] = bar.G[(int64_t)(v * D + j) * m.C + c];
+          }
+        }
+        lsq_dir_bar<D, NA>(d, dphi, Gc, yc, db);
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          xfb[j] += db[j];
+          ccb[j] -= db[j];
+        }
+      }
+    }
+    if (GRAD == 0) {  // fold g (interior) onto the centroids
+      FaceGeoBarF<D> qf;
+      qf.zero();
+      qf.g = gbar;
+      fold_face_geo_bar<D>(q, o >= 0, qf);
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        xfb[j] += qf.xf[j];
+        ccb[j] += r.own ? qf.xP[j] : qf.xN[j];
+      }
+    }
+    if (LFV || LSV) {
+      if (r.own || o < 0) {
+#pragma unroll
+        for (int v = 0; v < NA; ++v) {  // the limited fields' slots only (the others are never written)
+          const int l = lim_slot<D>(v, LFV, LSV);
+          if (l < 0) continue;
+          if (gb.limf[(int64_t)l * m.C + c] == r.f) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) xfb[j] += gb.limx[(int64_t)(l * D + j) * m.C + c];
+          }
+          if (o >= 0 && gb.limf[(int64_t)l * m.C + o] == r.f) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) xfb[j] += gb.limx[(int64_t)(l * D + j) * m.C + o];
+          }
+        }
+      }
+    }
+    if (r.own) {
+      gb.area[r.fu] += Ab;
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        gb.nrm[(int64_t)j * m.F + r.fu] += nb[j];
+        gb.fc[(int64_t)j * m.F + r.fu] += xfb[j];
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) gb.cc[(int64_t)j * m.C + c] += ccb[j];
+}
+
+// ---------------------------------------------------------------------------------------------
 // host side
 
 int64_t rans_work_doubles(int D, int64_t C, int64_t F, int mode) {
@@ -1516,9 +1959,9 @@ int64_t rans_work_doubles(int D, int64_t C, int64_t F, int mode) {
   if (mode == 1) return 2 * fwd + cgg + 2 * phys;
   // mode 2 adds the cotangents, the stencil extremum owners (2 NL int32) and the limiter argmin
   // face positions (NL int32)
-  if (mode == 2)
+  if (mode >= 2)
     return fwd + (int64_t)(NA * D + 1 + NL + 2 * NL + 7) * C + (int64_t)NL * C + ((int64_t)NL * C + 1) / 2 + F + cgg +
-           phys;
+           phys + (mode == 3 ? (int64_t)(NL * D + NL + 1) * C : 0);  // mode 3: geometry-cotangent scratch
   return fwd + cgg + phys;
 }
 
@@ -1530,6 +1973,8 @@ struct RWork {
   int32_t* kmin;  // [NL][C] limiter argmin face positions (VJP only)
   double* cgg;
   double *xs, *xst;  // physical state (and tangent) when k or omega is solved in log form
+  double *limx, *tauA;  // mode 3
+  int32_t* limf;
 };
 
 RWork rcarve(void* base, int D, int64_t C, int64_t F, int mode) {
@@ -1553,7 +1998,7 @@ RWork rcarve(void* base, int D, int64_t C, int64_t F, int mode) {
     w.clt = take(7 * C);
     w.mdott = take(F);
   }
-  if (mode == 2) {
+  if (mode >= 2) {
     w.Gb = take((int64_t)NA * D * C);
     w.aPb = take(C);
     w.psib = take((int64_t)NL * C);
@@ -1572,6 +2017,11 @@ RWork rcarve(void* base, int D, int64_t C, int64_t F, int mode) {
   w.cgg = take((int64_t)2 * NA * D * C);
   w.xs = take((int64_t)NA * C);
   if (mode == 1) w.xst = take((int64_t)NA * C);
+  if (mode == 3) {
+    w.limx = take((int64_t)NL * D * C);
+    w.limf = (int32_t*)take((int64_t)NL * C);
+    w.tauA = take(C);
+  }
   return w;
 }
 
@@ -1810,7 +2260,7 @@ int rcheck(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg,
 }  // namespace
 
 extern "C" int afx_rans_workspace_bytes(const afx_mesh* m, int mode, int64_t* bytes) {
-  AFX_REQUIRE(m && bytes && mode >= 0 && mode <= 2, "afx_rans_workspace_bytes: bad argument");
+  AFX_REQUIRE(m && bytes && mode >= 0 && mode <= 3, "afx_rans_workspace_bytes: bad argument");
   *bytes = rans_work_doubles(m->dim, m->C, m->F, mode) * 8 + 256;
   return 0;
 }
@@ -1901,12 +2351,14 @@ namespace {
 // also receives R from the same forward products.
 int rans_vjp_impl(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg, const afx_rans_inputs* in,
                   const double* cotangent, afx_rans_cotangents* out, double* residual, void* work, int64_t work_size,
-                  hipStream_t s) {
-  int rc = rcheck(m, g, cfg, in, work, work_size, 2);
+                  hipStream_t s, const afx_geometry_cotangents* geom_out = nullptr, double* dwall_bar = nullptr,
+                  double* dgrad_bar = nullptr) {
+  const int mode = geom_out ? 3 : 2;
+  int rc = rcheck(m, g, cfg, in, work, work_size, mode);
   if (rc) return rc;
   AFX_REQUIRE(cotangent && out && out->state && out->mu && out->rho && out->nu && out->u_b && out->p_b,
               "afx_rans_vjp: null cotangent argument");
-  RWork w = rcarve(work, m->dim, m->C, m->F, 2);
+  RWork w = rcarve(work, m->dim, m->C, m->F, mode);
   RansArgs<double> a = rargs<double>(cfg, in, nullptr);
   rc = rans_bind_physical<double>(m, cfg, in, nullptr, w.xs, nullptr, a, s);
   if (rc) return rc;
@@ -1950,6 +2402,22 @@ int rans_vjp_impl(const afx_mesh* m, const afx_geometry* g, const afx_rans_confi
                      AFX_LAUNCH(kKRansGradAdj, (k_rans_grad_adj<D, GK, LF, LS>), grid_for(m->C), dim3(kBlock), 0, s,
                                 v, b, w.ext, bar);
                      AFX_CHECK_LAUNCH();
+                     if constexpr (GR != kGradCorrected) {
+                       if (geom_out) {  // the geometry's cotangents from the adjoint's intermediates
+                         RansGeoBars gb{geom_out->area, geom_out->normal, geom_out->face_centroid,
+                                        geom_out->volume, geom_out->cell_centroid, dwall_bar, dgrad_bar,
+                                        w.limx, w.limf, w.tauA};
+                         AFX_LAUNCH(kKSstClosureAdj, (k_rans_geo_cell<D, LF, LS, GR>), grid_for(m->C), dim3(kBlock),
+                                    0, s, v, b, cotangent, w.ext, w.kmin, bar, gb);
+                         AFX_CHECK_LAUNCH();
+                         AFX_LAUNCH(kKRansFlowResAdj, (k_rans_geo_face<D, AF, AS>), grid_for(m->C), dim3(kBlock), 0,
+                                    s, v, b, cotangent, w.mdot, w.mdotb, bar, gb);
+                         AFX_CHECK_LAUNCH();
+                         AFX_LAUNCH(kKRansGradAdj, (k_rans_geo_grad<D, GR, LF, LS>), grid_for(m->C), dim3(kBlock), 0,
+                                    s, v, b, bar, gb);
+                         AFX_CHECK_LAUNCH();
+                       }
+                     }
                      for (int row = D + 1; row <= D + 2; ++row) {
                        const int tr = row == D + 1 ? cfg->k_transform : cfg->omega_transform;
                        if (tr != AFX_SCALAR_LOG) continue;
@@ -1961,6 +2429,23 @@ int rans_vjp_impl(const afx_mesh* m, const afx_geometry* g, const afx_rans_confi
 }
 
 }  // namespace
+
+extern "C" int afx_rans_vjp_geometry(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg,
+                                     const afx_rans_inputs* in, const double* cotangent, afx_rans_cotangents* out,
+                                     const afx_geometry_cotangents* geom_out, double* wall_distance_bar,
+                                     double* wall_distance_gradient_bar, void* work, int64_t work_size,
+                                     hipStream_t s) {
+  AFX_REQUIRE(m && cfg && geom_out && wall_distance_bar && wall_distance_gradient_bar,
+              "afx_rans_vjp_geometry: null argument");
+  AFX_REQUIRE(geom_out->area && geom_out->normal && geom_out->face_centroid && geom_out->volume &&
+                  geom_out->cell_centroid,
+              "afx_rans_vjp_geometry: all five geometry cotangent buffers are needed");
+  AFX_REQUIRE(cfg->flow.gradient != kGradCorrected,
+              "afx_rans_vjp_geometry: geometry cotangents are available for compact Green-Gauss or least-squares "
+              "gradients");
+  return rans_vjp_impl(m, g, cfg, in, cotangent, out, nullptr, work, work_size, s, geom_out, wall_distance_bar,
+                       wall_distance_gradient_bar);
+}
 
 extern "C" int afx_rans_vjp(const afx_mesh* m, const afx_geometry* g, const afx_rans_config* cfg,
                             const afx_rans_inputs* in, const double* cotangent, afx_rans_cotangents* out, void* work,

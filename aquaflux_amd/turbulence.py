@@ -153,6 +153,19 @@ def wall_distance(mesh, geometry, wall_faces, with_nearest=False):
     return d
 
 
+def _nearest_tables(nearest, n_cells, device):
+    """For each wall face that is some cell's nearest: the face (``_near_faces``) and its cells,
+    padded with ``n_cells`` (``_near_table``), so that the distance cotangents reach the face
+    centroids as fixed-order sums rather than atomic scatters."""
+    near = nearest.cpu().numpy()
+    order = np.argsort(near, kind="stable")
+    faces, start, count = np.unique(near[order], return_index=True, return_counts=True)
+    table = np.full((faces.size, int(count.max()) if faces.size else 1), n_cells, dtype=np.int64)
+    for i, (s0, n) in enumerate(zip(start, count)):
+        table[i, :n] = order[s0:s0 + n]
+    return {"_near_faces": torch.from_numpy(faces).to(device), "_near_table": torch.from_numpy(table).to(device)}
+
+
 def scalar_gradient(mesh, geometry, gradient_scheme, phi, kind, value):
     """Cell gradient ``(d, C)`` of ``phi`` with Dirichlet / zero-gradient closures (HIP kernel)."""
     dev = geometry.face.area.device
@@ -244,6 +257,7 @@ class SSTTurbulence:
             _omega_value=torch.from_numpy(ov).to(dev),
             _wall_face=torch.from_numpy(wall_face).to(dev),
             _wall_nearest=nearest,
+            **_nearest_tables(nearest, mesh.n_cells, dev),
             _dkind=dkind_t,
             _dval=dval,
             _wall_cell=torch.from_numpy(wall_cell).to(dev),
@@ -292,7 +306,7 @@ class _RansRaw:
         self.omega_value = t._omega_value
         L = _lib.lib()
         self.work_bytes = []
-        for mode in range(3):
+        for mode in range(4):
             n = ctypes.c_int64()
             _lib.check(L.afx_rans_workspace_bytes(self.dm.handle, mode, ctypes.byref(n)), "afx_rans_workspace_bytes")
             self.work_bytes.append(int(n.value))
@@ -380,6 +394,54 @@ class _RansRaw:
         )
         return R, dR
 
+    def vjp_geometry(self, primals, cot):
+        """The cotangents of afx_rans_vjp plus those of the five geometry arrays, with the wall
+        distance and its gradient moving with the geometry (a differentiated build)."""
+        t = self.keep[0]
+        L = _lib.lib()
+        stream = _lib.stream_of(self.dm.device)
+        bars = [torch.empty_like(p) for p in primals]
+        g = t.geometry
+        geo = [fwAD.unpack_dual(x).primal for x in (g.face.area, g.face.normal_soa, g.face.centroid_soa, g.cell.volume,
+                                                    g.cell.centroid_soa)]
+        gbars = [torch.empty_like(x) for x in geo]
+        d = t.wall_distance
+        db = torch.empty_like(d)
+        dgb = torch.empty_like(t.wall_distance_gradient)
+        w = self.dm.workspace("rans3", self.work_bytes[3])
+        ins = self.inputs(*primals)
+        outs = _lib.AfxRansCotangents(*[b.data_ptr() for b in bars])
+        gouts = _lib.AfxGeometryCotangents(*[b.data_ptr() for b in gbars])
+        _lib.check(
+            L.afx_rans_vjp_geometry(
+                self.dm.handle, ctypes.byref(self.gs), ctypes.byref(self.cfg), ctypes.byref(ins), cot.data_ptr(),
+                ctypes.byref(outs), ctypes.byref(gouts), db.data_ptr(), dgb.data_ptr(), w.data_ptr(), w.numel(), stream,
+            ),
+            "afx_rans_vjp_geometry",
+        )
+        # the wall-distance gradient: d's share and the geometry's
+        gcode = int(self.cfg.flow.gradient)
+        d_extra = torch.empty_like(d)
+        _lib.check(L.afx_scalar_gradient_vjp(self.dm.handle, ctypes.byref(self.gs), gcode, 4, 1.0, t._dkind.data_ptr(),
+                                             dgb.data_ptr(), d_extra.data_ptr(), None, stream), "afx_scalar_gradient_vjp")
+        db = db + d_extra
+        g2 = [torch.empty_like(x) for x in geo]
+        gouts2 = _lib.AfxGeometryCotangents(*[b.data_ptr() for b in g2])
+        _lib.check(L.afx_scalar_gradient_vjp_geometry(self.dm.handle, ctypes.byref(self.gs), gcode, t._dkind.data_ptr(),
+                                                      t._dval.data_ptr(), d.data_ptr(), dgb.data_ptr(),
+                                                      ctypes.byref(gouts2), stream),
+                   "afx_scalar_gradient_vjp_geometry")
+        gbars = [a + b for a, b in zip(gbars, g2)]
+        # d = |x_c - x_f*| to the nearest wall-face centroid f*
+        near, faces, table = t._wall_nearest, t._near_faces, t._near_table
+        unit = (geo[4] - geo[2][:, near]) * (db / d)
+        pad = torch.cat([unit, torch.zeros((unit.shape[0], 1), dtype=unit.dtype, device=unit.device)], dim=1)
+        gbars[4] = gbars[4] + unit
+        fcb = gbars[2].clone()
+        fcb[:, faces] = fcb[:, faces] - pad[:, table].sum(dim=2)  # fixed-order sums over each face's cells
+        gbars[2] = fcb
+        return bars, gbars
+
     def vjp(self, primals, cot):
         bars = [torch.empty_like(p) for p in primals]
         w = self.dm.workspace("rans2", self.work_bytes[2])
@@ -435,8 +497,8 @@ class _RansRaw:
 
 class _RansResidualFn(torch.autograd.Function):
     """R(state, mu, rho, nu, u_b, p_b; area, normal, face centroid, volume, cell centroid) with HIP
-    forward, tangent and adjoint.  Forward mode follows geometry tangents (``geo_tan``); reverse
-    mode through the geometry is refused, never silently dropped."""
+    forward, tangent and adjoint, in both modes also through the geometry arrays (compact
+    Green-Gauss or least squares), with the wall distance and its gradient moving with them."""
 
     @staticmethod
     def forward(ctx, raw, geo_tan, state, mu, rho, nu, ub, pb, *geo):
@@ -450,9 +512,12 @@ class _RansResidualFn(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gR):
         if any(ctx.needs_input_grad[8:]):
-            raise NotImplementedError("reverse-mode derivatives of the RANS residual with respect to the geometry "
-                                      "(node coordinates) are not provided; detach the geometry, or use forward "
-                                      "mode")
+            if ctx.raw.cfg.flow.gradient == 3:
+                raise NotImplementedError("reverse-mode derivatives of the RANS residual with respect to the "
+                                          "geometry are provided for compact Green-Gauss or least-squares gradients, "
+                                          "not the corrected Green-Gauss scheme")
+            bars, gbars = ctx.raw.vjp_geometry(ctx.saved_tensors, gR.contiguous())
+            return (None, None, *bars, *gbars)
         bars = ctx.raw.vjp(ctx.saved_tensors, gR.contiguous())
         return (None, None, *bars) + (None,) * (len(ctx.needs_input_grad) - 8)
 

@@ -248,6 +248,16 @@ int afx_scalar_gradient_jvp_geometry(const afx_mesh* mesh, const afx_geometry* g
                                      const double* phi_tangent, double* grad, double* grad_tangent,
                                      hipStream_t stream);
 
+/* Cotangents of the five geometry arrays (written; caller face order) of the gradient of phi for a
+ * gradient cotangent grad_bar[d][C] (boundary values and phi held fixed; their cotangents come
+ * from afx_scalar_gradient_vjp).  Compact Green-Gauss or least squares.  Replaces jax.vjp of
+ * GradientScheme.gradients with Mesh.node_coords as a differentiable leaf (the SST wall-distance
+ * gradient of a differentiated build; reference: aquaflux/schemes/gradient.py:98-153,
+ * aquaflux/turbulence/transport.py:259-313). */
+int afx_scalar_gradient_vjp_geometry(const afx_mesh* mesh, const afx_geometry* geom, int gradient, const int8_t* kind,
+                                     const double* value, const double* phi, const double* grad_bar,
+                                     afx_geometry_cotangents* out, hipStream_t stream);
+
 /* Reverse of afx_scalar_gradient (same scheme, closures and geometry): phi_bar[C] and, when not
  * NULL, value_bar[F] (zero except on Dirichlet faces, caller order) from grad_bar[j][C].  The
  * gradient is linear in (phi, value), so this is its transpose.  Replaces jax.vjp through
@@ -391,7 +401,8 @@ typedef struct {
   double* p_b;   /* [F] */
 } afx_rans_cotangents;
 
-/* Workspace (device bytes) for mode 0 = residual, 1 = JVP, 2 = VJP. */
+/* Workspace (device bytes) for mode 0 = residual, 1 = JVP, 2 = VJP, 3 = VJP with geometry
+ * cotangents. */
 int afx_rans_workspace_bytes(const afx_mesh* mesh, int mode, int64_t* bytes);
 
 /* R(u, p, k, omega) at the solved state (k / omega rows e^w under AFX_SCALAR_LOG).  Replaces
@@ -425,6 +436,19 @@ int afx_rans_jvp_geometry(const afx_mesh* mesh, const afx_geometry* geom, const 
 int afx_rans_vjp(const afx_mesh* mesh, const afx_geometry* geom, const afx_rans_config* cfg,
                  const afx_rans_inputs* in, const double* cotangent, afx_rans_cotangents* out, void* work,
                  int64_t work_bytes, hipStream_t stream);
+
+/* As afx_rans_vjp, and also the cotangents of the five geometry arrays (written; caller face order)
+ * and of the configuration's wall distance [C] and wall-distance gradient [d][C], which a
+ * differentiated build carries on to the geometry (nearest wall-face centroid distance,
+ * afx_scalar_gradient_vjp / _vjp_geometry) and then afx_geometry_vjp to the nodes.  Compact
+ * Green-Gauss or least-squares gradients; workspace of mode 3.  Replaces jax.vjp of
+ * CoupledRANS.residual with Mesh.node_coords as a differentiable leaf (reference:
+ * aquaflux/mesh/mesh.py:63-66, aquaflux/mesh/distance.py:87-140,
+ * aquaflux/turbulence/coupled.py:437-473). */
+int afx_rans_vjp_geometry(const afx_mesh* mesh, const afx_geometry* geom, const afx_rans_config* cfg,
+                          const afx_rans_inputs* in, const double* cotangent, afx_rans_cotangents* out,
+                          const afx_geometry_cotangents* geom_out, double* wall_distance_bar,
+                          double* wall_distance_gradient_bar, void* work, int64_t work_bytes, hipStream_t stream);
 
 /* R and J^T . cotangent in one call, the forward passes shared (the pair jax.vjp returns: the
  * primal output and its pullback applied to `cotangent`).  Workspace as for mode 2. */

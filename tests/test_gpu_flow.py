@@ -321,7 +321,7 @@ def test_node_coordinate_adjoint_refused_for_corrected_green_gauss(cuda):
 
 
 def test_rans_geometry_derivatives_are_refused_not_dropped(cuda):
-    """Reverse mode through the RANS geometry, and forward mode with corrected Green-Gauss, raise
+    """Geometry derivatives of the RANS residual with corrected Green-Gauss raise (both modes)
     instead of returning a result that leaves the geometry's part out."""
     import torch.autograd.forward_ad as fwAD
 
@@ -330,9 +330,9 @@ def test_rans_geometry_derivatives_are_refused_not_dropped(cuda):
 
     mesh = make_mesh("channel2d")
     X2 = torch.tensor(mesh.node_coords, dtype=F64, device=cuda, requires_grad=True)
-    rmesh, rans, _ = RC.build("channel2d_sst", "limited", "limited", "lsq", device=cuda, node_coords=X2)
+    rmesh, rans, _ = RC.build("channel2d_sst", "limited", "limited", "cgg", device=cuda, node_coords=X2)
     state = torch.tensor(RC.seeded_state(rmesh, 0, "channel"), dtype=F64, device=cuda)
-    with pytest.raises(NotImplementedError, match="geometry"):
+    with pytest.raises(NotImplementedError, match="corrected Green-Gauss"):
         torch.autograd.grad(rans.residual(state).sum(), X2)
     with fwAD.dual_level():
         X3 = torch.tensor(mesh.node_coords, dtype=F64, device=cuda)

@@ -396,3 +396,39 @@ def test_rans_node_coordinate_tangent_matches_oracle(case, cuda):
     errs = RC.block_errors(dR, dR_ref, nb)
     print("node tangent block errors", errs)
     assert max(errs) < TOL_D
+
+
+@pytest.mark.parametrize("case", GEO_CASES, ids=_ids)
+def test_rans_node_coordinate_adjoint_matches_oracle(case, cuda):
+    """d <lambda, R(state; X)> / dX through the hand-written RANS adjoint with geometry, wall-distance
+    and wall-distance-gradient cotangents (afx_rans_vjp_geometry, then the distance and gradient
+    set-up and the geometry adjoints), against torch autograd of the oracle's differentiated build,
+    and <J Xdot, lambda> against the forward-mode tangent of the same HIP path."""
+    import torch.autograd.forward_ad as fwAD
+
+    name, fs, ss, gr, nu = case
+    print("case", _ids(case), flush=True)
+    base = RC.CASES[name][0]
+    mesh0, _ = RC.flow_cases.oracle_case(base, fs, gr)
+    st = torch.tensor(RC.seeded_state(mesh0, 3, "cavity" if name.startswith(("cavity", "box")) else "channel"))
+    mesh, f = RC.oracle_node_function(name, fs, ss, gr, st, nu=nu)
+    rng = np.random.default_rng(8)
+    nb = mesh.dim + 3
+    scale = torch.tensor(np.repeat([1.0] * (nb - 2) + [1e3, 1e-2], mesh.n_cells))  # comparable k / omega rows
+    lam = torch.tensor(rng.normal(size=st.numel())) * scale
+    Xo = torch.tensor(mesh.node_coords, dtype=F64, requires_grad=True)
+    (ref,) = torch.autograd.grad((f(Xo) * lam).sum(), Xo)
+    X = torch.tensor(mesh.node_coords, dtype=F64, device=cuda, requires_grad=True)
+    _, rans, _ = RC.build(name, fs, ss, gr, nu=nu, device=cuda, node_coords=X)
+    (got,) = torch.autograd.grad((rans.residual(st.to(cuda)) * lam.to(cuda)).sum(), X)
+    err = float((got.cpu() - ref).abs().max() / ref.abs().max())
+    print("node adjoint err", err, float(ref.abs().max()))
+    assert err < TOL_D
+    dX = torch.tensor(rng.normal(size=mesh.node_coords.shape)) * 1e-3
+    with fwAD.dual_level():
+        Xd = fwAD.make_dual(torch.tensor(mesh.node_coords, dtype=F64, device=cuda), dX.to(cuda))
+        _, rd, _ = RC.build(name, fs, ss, gr, nu=nu, device=cuda, node_coords=Xd)
+        dR = fwAD.unpack_dual(rd.residual(st.to(cuda))).tangent.clone()
+    a = float((dR.cpu() * lam).sum())
+    b = float((got.cpu() * dX).sum())
+    assert abs(a - b) <= 1e-11 * float((dR.cpu() * lam).abs().sum())
