@@ -236,15 +236,17 @@ _SIGNATURES = {
     "afx_scalar_residual_noalloc": [c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxScalarConfig),
                                     ctypes.POINTER(AfxScalarInputs), c_void_p, c_void_p],
     "afx_scalar_noalloc_max_bytes": [c_int64_p],
-    "afx_scalar_gradient_vjp_geometry": [c_void_p, ctypes.POINTER(AfxGeometry), ctypes.c_int, c_void_p, c_void_p,
-                                         c_void_p, c_void_p, ctypes.POINTER(AfxGeometryCotangents), c_void_p],
+    "afx_scalar_gradient_vjp_geometry": [c_void_p, ctypes.POINTER(AfxGeometry), ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, c_void_p, c_void_p, c_void_p, c_void_p,
+                                         ctypes.POINTER(AfxGeometryCotangents), c_void_p],
     "afx_rans_vjp_geometry": [
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxRansConfig), ctypes.POINTER(AfxRansInputs), c_void_p,
         ctypes.POINTER(AfxRansCotangents), ctypes.POINTER(AfxGeometryCotangents), c_void_p, c_void_p, c_void_p,
         ctypes.c_int64, c_void_p,
     ],
     "afx_scalar_gradient_jvp_geometry": [c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxGeometry), ctypes.c_int,
-                                         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+                                         ctypes.c_int, ctypes.c_double, c_void_p, c_void_p, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_void_p],
     "afx_scalar_gradient_vjp": [c_void_p, ctypes.POINTER(AfxGeometry), ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "afx_venkat_limiter": [c_void_p, ctypes.POINTER(AfxGeometry), ctypes.c_double, c_void_p, c_void_p, c_void_p,

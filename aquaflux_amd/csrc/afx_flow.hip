@@ -724,6 +724,7 @@ struct FlowGeoBars {
   double *vol, *cc;         // [C], [D][C] (written)
   double* limx;             // [D][D][C] scratch: limiter face-centroid cotangent at its argmin face
   int32_t* limf;            // [D][C]    scratch: that face (view index, -1 none)
+  CggGeoBars cgg;           // corrected Green-Gauss: the swept solve's geometry cotangents
 };
 
 // Limiter part: psi_i = clip(min_f venkat(G_i . (xf - xc), heads, V K^3), 0, 1) at its first
@@ -830,6 +831,7 @@ __global__ void __launch_bounds__(256) k_flow_geo_adj(MeshView m, FlowArgs<doubl
 #pragma unroll
       for (int j = 0; j < D; ++j) Vb -= yc[v][j] * Gc[v][j];
   }
+  if (GRAD == kGradCorrected) Vb += gb.cgg.vol[c];  // the sweeps' V^-1
   for (FaceWalk walk(m, c); walk.more();) {
     FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
@@ -864,7 +866,15 @@ __global__ void __launch_bounds__(256) k_flow_geo_adj(MeshView m, FlowArgs<doubl
       double VPb = 0.0, VNb = 0.0;
       geo_adj_flux_interior<D, ADV>(q, P, N, lf, qb, VPb, VNb);
       geo_adj_ap_interior<D, ADV>(q, P, N, r.own ? apb_c : apb_o, r.own ? apb_o : apb_c, qb);
-      if (GRAD == 0) {  // acc_P += phi_f n A, acc_N -= phi_f n A
+      if (GRAD == kGradCorrected) {  // the sweeps' face terms, folded with the rest below
+        double xPb[D], xNb[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) xPb[j] = xNb[j] = 0.0;
+        cgg_face_geo_bar<D>(gb.cgg, m.F, r.f, q, qb.A, qb.n, qb.xf, xPb, xNb);
+#pragma unroll
+        for (int j = 0; j < D; ++j) ccb[j] += r.own ? xPb[j] : xNb[j];
+      }
+      if (GRAD != 1) {  // acc_P += phi_f n A, acc_N -= phi_f n A (compact, or the corrected scheme's b)
         const double w = 1.0 - q.g;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -928,7 +938,7 @@ __global__ void __launch_bounds__(256) k_flow_geo_adj(MeshView m, FlowArgs<doubl
       double phb[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) phb[v] = flow_bval<D, double>(a, kind, v, m.F, r.fu, xc);
-      if (GRAD == 0) {  // acc_P += phi_b n A
+      if (GRAD != 1) {  // acc_P += phi_b n A
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           double zn = 0.0;
@@ -1068,13 +1078,13 @@ FlowArgs<S> make_args(const afx_flow_config* cfg, const afx_flow_inputs* in, con
   return a;
 }
 
-// GEO: the geometry carries tangents in a.gt (S = dual; not with corrected Green-Gauss).
+// GEO: the geometry carries tangents in a.gt (S = dual).  iters (corrected Green-Gauss, nullable):
+// keep the sweeps' iterates for the geometry adjoint.
 template <int D, class S, int ADV, int GRAD, bool GEO = false>
 int run_forward(const afx_mesh* mh, const afx_geometry* g, FlowArgs<S> a, Out<S> R, double* G, double* Gt,
                 double* aP, double* aPt, double* psi, double* psit, int32_t* ext, double* cgg, int sweeps,
-                double relax, hipStream_t s, Out<S> mdot = Out<S>{}) {
-  static_assert(!GEO || (GRAD != kGradCorrected && std::is_same<S, dual>::value),
-                "geometry tangents: dual state, compact Green-Gauss or least squares");
+                double relax, hipStream_t s, Out<S> mdot = Out<S>{}, double* iters = nullptr) {
+  static_assert(!GEO || std::is_same<S, dual>::value, "geometry tangents need a dual state");
   MeshView m;
   if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
   Out<S> oG{G, Gt}, oP{psi, psit}, oA{aP, aPt};
@@ -1089,20 +1099,32 @@ int run_forward(const afx_mesh* mh, const afx_geometry* g, FlowArgs<S> a, Out<S>
   a.G.v = G;
   a.G.t = Gt;
   if constexpr (GRAD == kGradCorrected) {
-    AFX_REQUIRE(cgg, "flow: corrected Green-Gauss needs sweep scratch");
-    int rc = cgg_sweeps(m, D, D + 1, G, cgg, sweeps, relax, s);
-    if (rc) return rc;
-    if (Gt) {
-      rc = cgg_sweeps(m, D, D + 1, Gt, cgg, sweeps, relax, s);
+    LimiterSlots sl{};
+    sl.nl = D;
+    sl.ext_stride = D;
+    for (int i = 0; i < D; ++i) sl.var[i] = sl.slot[i] = i;
+    if constexpr (GEO) {  // the sweeps and the limiter with the geometry's tangents
+      double* scr = afx_mesh_scratch(const_cast<afx_mesh*>(mh), (int64_t)4 * (D + 1) * D * m.C);
+      AFX_REQUIRE(scr, "flow: could not allocate the corrected-gradient sweep scratch");
+      int rc = cgg_sweeps_tangent(m, a.gt, D, D + 1, G, Gt, scr, sweeps, relax, s);
       if (rc) return rc;
-    }
-    if (ADV == kAdvLimited) {
-      LimiterSlots sl{};
-      sl.nl = D;
-      sl.ext_stride = D;
-      for (int i = 0; i < D; ++i) sl.var[i] = sl.slot[i] = i;
-      rc = venkat_pass(m, D, sl, a.x, a.G, a.lim_k, oP, ext, s);
+      if (ADV == kAdvLimited) {
+        rc = venkat_pass_tangent(m, a.gt, D, sl, a.x, a.G, a.lim_k, oP, s);
+        if (rc) return rc;
+      }
+    } else {
+      AFX_REQUIRE(cgg, "flow: corrected Green-Gauss needs sweep scratch");
+      int rc = iters ? cgg_sweeps_store(m, D, D + 1, G, iters, sweeps, relax, s)
+                     : cgg_sweeps(m, D, D + 1, G, cgg, sweeps, relax, s);
       if (rc) return rc;
+      if (Gt) {
+        rc = cgg_sweeps(m, D, D + 1, Gt, cgg, sweeps, relax, s);
+        if (rc) return rc;
+      }
+      if (ADV == kAdvLimited) {
+        rc = venkat_pass(m, D, sl, a.x, a.G, a.lim_k, oP, ext, s);
+        if (rc) return rc;
+      }
     }
   }
   AFX_LAUNCH(kKFlowAp, (k_flow_ap<D, S, ADV, GEO>), grid_for(m.C), dim3(kBlock), 0, s, m, a, oA);
@@ -1124,13 +1146,16 @@ int run_forward(const afx_mesh* mh, const afx_geometry* g, FlowArgs<S> a, Out<S>
   return 0;
 }
 
+// iters / cb (corrected Green-Gauss, geometry adjoint): the sweeps' iterates are kept and their
+// geometry cotangents written to cb.
 template <int D, int ADV, int GRAD>
 int run_vjp(const afx_mesh* mh, const afx_geometry* g, FlowArgs<double> a, const double* lam, FlowBars bar,
-            const Work& w, int sweeps, double relax, hipStream_t s) {
+            const Work& w, int sweeps, double relax, hipStream_t s, double* iters = nullptr,
+            CggGeoBars cb = CggGeoBars{nullptr, nullptr}) {
   MeshView m;
   if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
   int rc = run_forward<D, double, ADV, GRAD>(mh, g, a, Out<double>{nullptr, nullptr}, w.G, nullptr, w.aP, nullptr,
-                                             w.psi, nullptr, w.ext, w.cgg, sweeps, relax, s);
+                                             w.psi, nullptr, w.ext, w.cgg, sweeps, relax, s, Out<double>{}, iters);
   if (rc) return rc;
   a.G.v = w.G;
   a.aP.v = w.aP;
@@ -1145,7 +1170,8 @@ int run_vjp(const afx_mesh* mh, const afx_geometry* g, FlowArgs<double> a, const
   AFX_CHECK_LAUNCH();
   constexpr int GK = GRAD == kGradCorrected ? 0 : GRAD;
   if (GRAD == kGradCorrected) {
-    rc = cgg_sweeps_adjoint(m, D, D + 1, bar.G, w.cgg, sweeps, relax, s);
+    rc = cb.face ? cgg_sweeps_adjoint_geometry(m, D, D + 1, bar.G, w.G, iters, w.cgg, sweeps, relax, cb, s)
+                 : cgg_sweeps_adjoint(m, D, D + 1, bar.G, w.cgg, sweeps, relax, s);
     if (rc) return rc;
   }
   if (ADV == kAdvLimited)
@@ -1159,8 +1185,17 @@ int run_vjp(const afx_mesh* mh, const afx_geometry* g, FlowArgs<double> a, const
 // The state adjoint, then the geometry cotangents from its intermediates.
 template <int D, int ADV, int GRAD>
 int run_vjp_geometry(const afx_mesh* mh, const afx_geometry* g, FlowArgs<double> a, const double* lam, FlowBars bar,
-                     FlowGeoBars gb, const Work& w, hipStream_t s) {
-  int rc = run_vjp<D, ADV, GRAD>(mh, g, a, lam, bar, w, 1, 1.0, s);
+                     FlowGeoBars gb, const Work& w, int sweeps, double relax, hipStream_t s) {
+  double* iters = nullptr;
+  if (GRAD == kGradCorrected) {  // iterates and the sweeps' geometry cotangents in the handle's scratch
+    const int64_t ni = cgg_iterate_doubles(D, D + 1, mh->C, sweeps);
+    double* scr = afx_mesh_scratch(const_cast<afx_mesh*>(mh), ni + cgg_geo_bar_doubles(D, mh->C, mh->F));
+    AFX_REQUIRE(scr, "afx_flow_vjp_geometry: could not allocate the corrected-gradient scratch");
+    iters = ni ? scr : nullptr;
+    gb.cgg.face = scr + ni;
+    gb.cgg.vol = gb.cgg.face + (int64_t)(2 * D + 1) * mh->F;
+  }
+  int rc = run_vjp<D, ADV, GRAD>(mh, g, a, lam, bar, w, sweeps, relax, s, iters, gb.cgg);
   if (rc) return rc;
   MeshView m;
   if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
@@ -1249,9 +1284,6 @@ extern "C" int afx_flow_jvp_geometry(const afx_mesh* m, const afx_geometry* g, c
   int rc = check_common(m, g, cfg, in, work, work_size, 1);
   if (rc) return rc;
   AFX_REQUIRE(geom_tangent && tangent && residual && residual_tangent, "afx_flow_jvp_geometry: null argument");
-  AFX_REQUIRE(cfg->gradient != kGradCorrected,
-              "afx_flow_jvp_geometry: a geometry tangent is available for compact Green-Gauss or least-squares "
-              "gradients");
   AFX_REQUIRE(geom_tangent->area && geom_tangent->normal && geom_tangent->face_centroid && geom_tangent->volume &&
                   geom_tangent->cell_centroid,
               "afx_flow_jvp_geometry: the geometry tangent needs all five arrays");
@@ -1268,13 +1300,9 @@ extern "C" int afx_flow_jvp_geometry(const afx_mesh* m, const afx_geometry* g, c
   a.gt.cc = geom_tangent->cell_centroid;
   return dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
     constexpr int D = decltype(Dc)::value, ADV = decltype(Ac)::value, GR = decltype(Gc)::value;
-    if constexpr (GR == kGradCorrected) {
-      return 2;  // refused above
-    } else {
-      return run_forward<D, dual, ADV, GR, true>(m, g, a, Out<dual>{residual, residual_tangent}, w.G, w.Gt, w.aP,
-                                                 w.aPt, w.psi, w.psit, nullptr, w.cgg, cfg->gradient_sweeps,
-                                                 cfg->gradient_relaxation, s);
-    }
+    return run_forward<D, dual, ADV, GR, true>(m, g, a, Out<dual>{residual, residual_tangent}, w.G, w.Gt, w.aP,
+                                               w.aPt, w.psi, w.psit, nullptr, w.cgg, cfg->gradient_sweeps,
+                                               cfg->gradient_relaxation, s);
   });
 }
 
@@ -1309,9 +1337,6 @@ extern "C" int afx_flow_vjp_geometry(const afx_mesh* m, const afx_geometry* g, c
   AFX_REQUIRE(geom_out->area && geom_out->normal && geom_out->face_centroid && geom_out->volume &&
                   geom_out->cell_centroid,
               "afx_flow_vjp_geometry: all five geometry cotangent buffers are needed");
-  AFX_REQUIRE(cfg->gradient != kGradCorrected,
-              "afx_flow_vjp_geometry: geometry cotangents are available for compact Green-Gauss or least-squares "
-              "gradients");
   Work w = carve(work, m->dim, m->C, 3);
   FlowArgs<double> a = make_args<double>(cfg, in, nullptr);
   FlowBars bar{out->state, w.Gb, w.aPb, w.psib, out->mu, out->rho, out->u_b, out->p_b, out->gamma_b, w.extb};
@@ -1319,11 +1344,8 @@ extern "C" int afx_flow_vjp_geometry(const afx_mesh* m, const afx_geometry* g, c
                  geom_out->cell_centroid, w.limx, w.limf};
   return dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
     constexpr int D = decltype(Dc)::value, ADV = decltype(Ac)::value, GR = decltype(Gc)::value;
-    if constexpr (GR == kGradCorrected) {
-      return 2;  // refused above
-    } else {
-      return run_vjp_geometry<D, ADV, GR>(m, g, a, cotangent, bar, gb, w, s);
-    }
+    return run_vjp_geometry<D, ADV, GR>(m, g, a, cotangent, bar, gb, w, cfg->gradient_sweeps,
+                                        cfg->gradient_relaxation, s);
   });
 }
 

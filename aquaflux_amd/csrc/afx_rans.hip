@@ -1521,6 +1521,7 @@ struct RansGeoBars {
   double* limx;             // [NL][D][C] scratch: limiter face-centroid cotangent at its argmin face
   int32_t* limf;            // [NL][C]    scratch: that face (view index, -1 none)
   double* tauA;             // [C]        scratch: tau cotangent / the cell's wall area
+  CggGeoBars cgg;           // corrected Green-Gauss: the swept solve's geometry cotangents
 };
 
 // Cell-local terms: source and body-force volumes, Green-Gauss 1/V, the closure's, sources' and
@@ -1820,7 +1821,7 @@ __global__ void __launch_bounds__(256) k_rans_geo_grad(MeshView m, RansArgs<doub
     FaceGeo<D> q;
     load_face_geo<D>(m, r.f, r.P, r.N, q);
     if (o >= 0) {
-      if (GRAD == 0) {  // acc_P += phi_f n A, acc_N -= phi_f n A
+      if (GRAD != 1) {  // acc_P += phi_f n A, acc_N -= phi_f n A (compact, or the corrected scheme's b)
         const double w = 1.0 - q.g;
 #pragma unroll
         for (int v = 0; v < NA; ++v) {
@@ -1874,7 +1875,7 @@ __global__ void __launch_bounds__(256) k_rans_geo_grad(MeshView m, RansArgs<doub
       double phb[NA];
 #pragma unroll
       for (int v = 0; v < NA; ++v) phb[v] = rans_bval_lead<D, double>(a, v, m.F, r.fu, xc);
-      if (GRAD == 0) {
+      if (GRAD != 1) {
 #pragma unroll
         for (int v = 0; v < NA; ++v) {
           double zn = 0.0;
@@ -1907,7 +1908,7 @@ __global__ void __launch_bounds__(256) k_rans_geo_grad(MeshView m, RansArgs<doub
         }
       }
     }
-    if (GRAD == 0) {  // fold g (interior) onto the centroids
+    if (GRAD != 1) {  // fold g (interior) onto the centroids
       FaceGeoBarF<D> qf;
       qf.zero();
       qf.g = gbar;
@@ -1917,6 +1918,14 @@ __global__ void __launch_bounds__(256) k_rans_geo_grad(MeshView m, RansArgs<doub
         xfb[j] += qf.xf[j];
         ccb[j] += r.own ? qf.xP[j] : qf.xN[j];
       }
+    }
+    if (GRAD == kGradCorrected && o >= 0) {  // the sweeps' own face terms
+      double xPb[D], xNb[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) xPb[j] = xNb[j] = 0.0;
+      cgg_face_geo_bar<D>(gb.cgg, m.F, r.f, q, Ab, nb, xfb, xPb, xNb);
+#pragma unroll
+      for (int j = 0; j < D; ++j) ccb[j] += r.own ? xPb[j] : xNb[j];
     }
     if (LFV || LSV) {
       if (r.own || o < 0) {
@@ -1946,6 +1955,7 @@ __global__ void __launch_bounds__(256) k_rans_geo_grad(MeshView m, RansArgs<doub
   }
 #pragma unroll
   for (int j = 0; j < D; ++j) gb.cc[(int64_t)j * m.C + c] += ccb[j];
+  if (GRAD == kGradCorrected) gb.vol[c] += gb.cgg.vol[c];  // the sweeps' V^-1
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2145,12 +2155,13 @@ void bind_closure(RansArgs<S>& a, double* cl, double* clt, int64_t C) {
 
 // Forward passes: G, closure, a_P; then, for `R` set, the flow and k / omega residual rows; for
 // `mdot_only`, just the face mass flux (and the flow rows when R is set) for the reverse passes.
-// GEO: the geometry carries tangents in a.gt (S = dual; not with corrected Green-Gauss).
+// GEO: the geometry carries tangents in a.gt (S = dual).  iters (corrected Green-Gauss, nullable):
+// keep the sweeps' iterates for the geometry adjoint.
 template <int D, class S, int ADVF, int ADVS, int GRAD, bool GEO = false>
 int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S> R, const RWork& w, double* Gt,
-                 double* psit, double* aPt, double* clt, double* mdott, int32_t* ext, bool mdot_only, hipStream_t s) {
-  static_assert(!GEO || (GRAD != kGradCorrected && std::is_same<S, dual>::value),
-                "geometry tangents: dual state, compact Green-Gauss or least squares");
+                 double* psit, double* aPt, double* clt, double* mdott, int32_t* ext, bool mdot_only, hipStream_t s,
+                 double* iters = nullptr) {
+  static_assert(!GEO || std::is_same<S, dual>::value, "geometry tangents need a dual state");
   MeshView m;
   if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
   const int64_t C = m.C;
@@ -2178,11 +2189,19 @@ int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S
   if constexpr (CG) {
     const int sw = a.sweeps;
     const double rl = a.relax;
-    int rc = cgg_sweeps(m, D, D + 3, w.G, w.cgg, sw, rl, s);
-    if (rc) return rc;
-    if (Gt) {
-      rc = cgg_sweeps(m, D, D + 3, Gt, w.cgg, sw, rl, s);
+    int rc = 0;
+    if constexpr (GEO) {  // the sweeps with the geometry's tangents
+      double* scr = afx_mesh_scratch(const_cast<afx_mesh*>(mh), (int64_t)4 * (D + 3) * D * C);
+      AFX_REQUIRE(scr, "rans: could not allocate the corrected-gradient sweep scratch");
+      rc = cgg_sweeps_tangent(m, a.gt, D, D + 3, w.G, Gt, scr, sw, rl, s);
       if (rc) return rc;
+    } else {
+      rc = iters ? cgg_sweeps_store(m, D, D + 3, w.G, iters, sw, rl, s) : cgg_sweeps(m, D, D + 3, w.G, w.cgg, sw, rl, s);
+      if (rc) return rc;
+      if (Gt) {
+        rc = cgg_sweeps(m, D, D + 3, Gt, w.cgg, sw, rl, s);
+        if (rc) return rc;
+      }
     }
     LimiterSlots sl{};
     sl.ext_stride = D + 2;
@@ -2198,7 +2217,10 @@ int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S
       sl.var[sl.nl] = D + 2;
       sl.slot[sl.nl++] = D + 1;
     }
-    rc = venkat_pass(m, D, sl, a.x, a.G, a.lim_k, oP, ext, s);
+    if constexpr (GEO)
+      rc = venkat_pass_tangent(m, a.gt, D, sl, a.x, a.G, a.lim_k, oP, s);
+    else
+      rc = venkat_pass(m, D, sl, a.x, a.G, a.lim_k, oP, ext, s);
     if (rc) return rc;
   }
   a.psi.v = w.psi;
@@ -2311,9 +2333,6 @@ extern "C" int afx_rans_jvp_geometry(const afx_mesh* m, const afx_geometry* g, c
   AFX_REQUIRE(geom_tangent && wall_distance_tangent && wall_distance_gradient_tangent && tangent && residual &&
                   residual_tangent,
               "afx_rans_jvp_geometry: null argument");
-  AFX_REQUIRE(cfg->flow.gradient != kGradCorrected,
-              "afx_rans_jvp_geometry: a geometry tangent is available for compact Green-Gauss or least-squares "
-              "gradients");
   AFX_REQUIRE(geom_tangent->area && geom_tangent->normal && geom_tangent->face_centroid && geom_tangent->volume &&
                   geom_tangent->cell_centroid,
               "afx_rans_jvp_geometry: the geometry tangent needs all five arrays");
@@ -2334,14 +2353,10 @@ extern "C" int afx_rans_jvp_geometry(const afx_mesh* m, const afx_geometry* g, c
   if (rc) return rc;
   return rdispatch(m->dim, cfg->flow.advection, cfg->scalar_advection, cfg->flow.gradient,
                    [&](auto Dc, auto Fc, auto Sc, auto Gc) {
-                     constexpr int GR = decltype(Gc)::value;
-                     if constexpr (GR == kGradCorrected) {
-                       return 2;  // refused above
-                     } else {
-                       return rans_forward<decltype(Dc)::value, dual, decltype(Fc)::value, decltype(Sc)::value, GR,
-                                           true>(m, g, a, Out<dual>{residual, residual_tangent}, w, w.Gt, w.psit,
-                                                 w.aPt, w.clt, w.mdott, nullptr, false, s);
-                     }
+                     return rans_forward<decltype(Dc)::value, dual, decltype(Fc)::value, decltype(Sc)::value,
+                                         decltype(Gc)::value, true>(m, g, a, Out<dual>{residual, residual_tangent}, w,
+                                                                    w.Gt, w.psit, w.aPt, w.clt, w.mdott, nullptr,
+                                                                    false, s);
                    });
 }
 
@@ -2369,9 +2384,22 @@ int rans_vjp_impl(const afx_mesh* m, const afx_geometry* g, const afx_rans_confi
                      constexpr int D = decltype(Dc)::value, AF = decltype(Fc)::value, AS = decltype(Sc)::value,
                                    GR = decltype(Gc)::value;
                      constexpr bool LF = AF == kAdvLimited, LS = AS == kAdvLimited;
+                     // corrected Green-Gauss with geometry cotangents: the sweeps' iterates and their
+                     // own geometry cotangents live in the handle's scratch
+                     double* iters = nullptr;
+                     CggGeoBars cb{nullptr, nullptr};
+                     if (GR == kGradCorrected && geom_out) {
+                       const int64_t ni = cgg_iterate_doubles(D, D + 3, m->C, a.sweeps);
+                       double* scr =
+                           afx_mesh_scratch(const_cast<afx_mesh*>(m), ni + cgg_geo_bar_doubles(D, m->C, m->F));
+                       AFX_REQUIRE(scr, "afx_rans_vjp_geometry: could not allocate the corrected-gradient scratch");
+                       iters = ni ? scr : nullptr;
+                       cb.face = scr + ni;
+                       cb.vol = cb.face + (int64_t)(2 * D + 1) * m->F;
+                     }
                      int r2 = rans_forward<D, double, AF, AS, GR>(m, g, a, Out<double>{residual, nullptr}, w,
                                                                   nullptr, nullptr, nullptr, nullptr, nullptr, w.ext,
-                                                                  true, s);
+                                                                  true, s, iters);
                      if (r2) return r2;
                      MeshView v;
                      if (int rv_ = afx_rview(m, g, s, &v)) return rv_;
@@ -2396,27 +2424,27 @@ int rans_vjp_impl(const afx_mesh* m, const afx_geometry* g, const afx_rans_confi
                      AFX_CHECK_LAUNCH();
                      constexpr int GK = GR == kGradCorrected ? 0 : GR;
                      if (GR == kGradCorrected) {
-                       int r3 = cgg_sweeps_adjoint(v, D, D + 3, bar.G, w.cgg, a.sweeps, a.relax, s);
+                       int r3 = cb.face ? cgg_sweeps_adjoint_geometry(v, D, D + 3, bar.G, w.G, iters, w.cgg, a.sweeps,
+                                                                      a.relax, cb, s)
+                                        : cgg_sweeps_adjoint(v, D, D + 3, bar.G, w.cgg, a.sweeps, a.relax, s);
                        if (r3) return r3;
                      }
                      AFX_LAUNCH(kKRansGradAdj, (k_rans_grad_adj<D, GK, LF, LS>), grid_for(m->C), dim3(kBlock), 0, s,
                                 v, b, w.ext, bar);
                      AFX_CHECK_LAUNCH();
-                     if constexpr (GR != kGradCorrected) {
-                       if (geom_out) {  // the geometry's cotangents from the adjoint's intermediates
-                         RansGeoBars gb{geom_out->area, geom_out->normal, geom_out->face_centroid,
-                                        geom_out->volume, geom_out->cell_centroid, dwall_bar, dgrad_bar,
-                                        w.limx, w.limf, w.tauA};
-                         AFX_LAUNCH(kKSstClosureAdj, (k_rans_geo_cell<D, LF, LS, GR>), grid_for(m->C), dim3(kBlock),
-                                    0, s, v, b, cotangent, w.ext, w.kmin, bar, gb);
-                         AFX_CHECK_LAUNCH();
-                         AFX_LAUNCH(kKRansFlowResAdj, (k_rans_geo_face<D, AF, AS>), grid_for(m->C), dim3(kBlock), 0,
-                                    s, v, b, cotangent, w.mdot, w.mdotb, bar, gb);
-                         AFX_CHECK_LAUNCH();
-                         AFX_LAUNCH(kKRansGradAdj, (k_rans_geo_grad<D, GR, LF, LS>), grid_for(m->C), dim3(kBlock), 0,
-                                    s, v, b, bar, gb);
-                         AFX_CHECK_LAUNCH();
-                       }
+                     if (geom_out) {  // the geometry's cotangents from the adjoint's intermediates
+                       RansGeoBars gb{geom_out->area, geom_out->normal, geom_out->face_centroid,
+                                      geom_out->volume, geom_out->cell_centroid, dwall_bar, dgrad_bar,
+                                      w.limx, w.limf, w.tauA, cb};
+                       AFX_LAUNCH(kKSstClosureAdj, (k_rans_geo_cell<D, LF, LS, GR>), grid_for(m->C), dim3(kBlock),
+                                  0, s, v, b, cotangent, w.ext, w.kmin, bar, gb);
+                       AFX_CHECK_LAUNCH();
+                       AFX_LAUNCH(kKRansFlowResAdj, (k_rans_geo_face<D, AF, AS>), grid_for(m->C), dim3(kBlock), 0,
+                                  s, v, b, cotangent, w.mdot, w.mdotb, bar, gb);
+                       AFX_CHECK_LAUNCH();
+                       AFX_LAUNCH(kKRansGradAdj, (k_rans_geo_grad<D, GR, LF, LS>), grid_for(m->C), dim3(kBlock), 0,
+                                  s, v, b, bar, gb);
+                       AFX_CHECK_LAUNCH();
                      }
                      for (int row = D + 1; row <= D + 2; ++row) {
                        const int tr = row == D + 1 ? cfg->k_transform : cfg->omega_transform;
@@ -2440,9 +2468,6 @@ extern "C" int afx_rans_vjp_geometry(const afx_mesh* m, const afx_geometry* g, c
   AFX_REQUIRE(geom_out->area && geom_out->normal && geom_out->face_centroid && geom_out->volume &&
                   geom_out->cell_centroid,
               "afx_rans_vjp_geometry: all five geometry cotangent buffers are needed");
-  AFX_REQUIRE(cfg->flow.gradient != kGradCorrected,
-              "afx_rans_vjp_geometry: geometry cotangents are available for compact Green-Gauss or least-squares "
-              "gradients");
   return rans_vjp_impl(m, g, cfg, in, cotangent, out, nullptr, work, work_size, s, geom_out, wall_distance_bar,
                        wall_distance_gradient_bar);
 }

@@ -446,6 +446,8 @@ struct ScBars {
   int32_t* limf; // [C]    scratch: limiter argmin face (-1 none)
   // geometry cotangents (null when not requested)
   double *area, *nrm, *fc, *vol, *cc;
+  // corrected Green-Gauss: the swept solve's geometry cotangents (null for the other schemes)
+  double *cggf, *cggv;
 };
 
 template <int D, bool GEO>
@@ -778,6 +780,8 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
           qb.A += phf * dn;
           qb.g += pb_ * (phN - phP);
           adj_face_geo<D>(q, true, qb);
+          if (bar.cggf)  // corrected Green-Gauss: the sweeps' own geometry dependence
+            cgg_face_geo_bar<D>(CggGeoBars{bar.cggf, bar.cggv}, m.F, r.f, q, qb.A, qb.n, qb.xf, qb.xP, qb.xN);
           have_geo = true;
         }
       } else if (GRAD == 1) {
@@ -893,6 +897,7 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
   if (GEO) {
 #pragma unroll
     for (int j = 0; j < D; ++j) bar.cc[(int64_t)j * m.C + c] += ccb[j];
+    if (bar.cggv) bar.vol[c] += bar.cggv[c];
   }
 }
 
@@ -1063,17 +1068,21 @@ __global__ void __launch_bounds__(256) k_scalar_grad_vjp(MeshView m, const int8_
 // wall-distance gradient): `yb` holds y = Gbar / V (Green-Gauss, divided here) or M^-1 Gbar (least
 // squares, from k_lsq_solve_t), G the forward gradient.  Writes (=) the owned faces' area, normal
 // and centroid cotangents (caller order) and each cell's volume and centroid cotangents.
+// Corrected Green-Gauss (GRAD 3): `yb` is the cotangent of the Green-Gauss sum b from the swept
+// solve's reverse pass, whose own geometry cotangents come in `cb`.
 template <int D, int GRAD>
 __global__ void __launch_bounds__(256) k_scalar_grad_geo_vjp(MeshView m, const int8_t* __restrict__ kind,
                                                              const double* __restrict__ bval,
                                                              const double* __restrict__ phi,
                                                              const double* __restrict__ G,
-                                                             const double* __restrict__ yb, afx_geometry_cotangents out) {
+                                                             const double* __restrict__ yb, afx_geometry_cotangents out,
+                                                             CggGeoBars cb) {
+  constexpr bool GG = GRAD != 1;  // Green-Gauss face sums (compact, or the corrected scheme's b)
   const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   const double xc = phi[c];
   const double ivc = GRAD == 0 ? 1.0 / m.vol[c] : 1.0;
-  double yc[D], Gc[D], ccb[D], Vb = 0.0;
+  double yc[D], Gc[D], ccb[D], Vb = GRAD == kGradCorrected ? cb.vol[c] : 0.0;
 #pragma unroll
   for (int j = 0; j < D; ++j) {
     yc[j] = yb[(int64_t)j * m.C + c] * ivc;
@@ -1091,8 +1100,8 @@ __global__ void __launch_bounds__(256) k_scalar_grad_geo_vjp(MeshView m, const i
     for (int j = 0; j < D; ++j) nb[j] = xfb[j] = 0.0;
     if (o >= 0) {
       const double xo = phi[o];
-      if (GRAD == 0) {
-        const double ivo = 1.0 / m.vol[o];
+      if (GG) {
+        const double ivo = GRAD == 0 ? 1.0 / m.vol[o] : 1.0;
         const double phP = r.own ? xc : xo, phN = r.own ? xo : xc;
         const double phf = (1.0 - q.g) * phP + q.g * phN;
         double zn = 0.0;
@@ -1112,6 +1121,14 @@ __global__ void __launch_bounds__(256) k_scalar_grad_geo_vjp(MeshView m, const i
         for (int j = 0; j < D; ++j) {
           xfb[j] += qf.xf[j];
           ccb[j] += r.own ? qf.xP[j] : qf.xN[j];
+        }
+        if (GRAD == kGradCorrected) {
+          double xPb[D], xNb[D];
+#pragma unroll
+          for (int j = 0; j < D; ++j) xPb[j] = xNb[j] = 0.0;
+          cgg_face_geo_bar<D>(cb, m.F, r.f, q, Ab, nb, xfb, xPb, xNb);
+#pragma unroll
+          for (int j = 0; j < D; ++j) ccb[j] += r.own ? xPb[j] : xNb[j];
         }
       } else {
         double d[D], dphi[1] = {xo - xc}, db[D], Gv[1][D], yv[1][D];
@@ -1136,7 +1153,7 @@ __global__ void __launch_bounds__(256) k_scalar_grad_geo_vjp(MeshView m, const i
       }
     } else {
       const double pb = lead_bval<double>(kind[r.fu], bval[r.fu], xc);
-      if (GRAD == 0) {
+      if (GG) {
         double zn = 0.0;
 #pragma unroll
         for (int j = 0; j < D; ++j) {
@@ -1288,7 +1305,7 @@ int sc_dispatch(int D, int adv, int grad, Fn&& fn) {
 
 template <int D, class S, int ADV, int GRAD>
 int sc_forward(const afx_mesh* mh, const afx_geometry* g, ScArgs<S> a, Out<S> R, const ScWork& w, double* Gt,
-               double* psit, int32_t* ext, int sweeps, double relax, hipStream_t s) {
+               double* psit, int32_t* ext, int sweeps, double relax, hipStream_t s, double* iters = nullptr) {
   MeshView m;
   if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
   constexpr bool LIM = ADV == kAdvLimited;
@@ -1299,7 +1316,8 @@ int sc_forward(const afx_mesh* mh, const afx_geometry* g, ScArgs<S> a, Out<S> R,
   a.G.v = w.G;
   a.G.t = Gt;
   if (GRAD == kGradCorrected) {
-    int rc = cgg_sweeps(m, D, 1, w.G, w.cgg, sweeps, relax, s);
+    int rc = iters ? cgg_sweeps_store(m, D, 1, w.G, iters, sweeps, relax, s)
+                   : cgg_sweeps(m, D, 1, w.G, w.cgg, sweeps, relax, s);
     if (rc) return rc;
     if (Gt) {
       rc = cgg_sweeps(m, D, 1, Gt, w.cgg, sweeps, relax, s);
@@ -1366,13 +1384,15 @@ extern "C" int afx_scalar_gradient(const afx_mesh* mh, const afx_geometry* g, in
 }
 
 extern "C" int afx_scalar_gradient_jvp_geometry(const afx_mesh* mh, const afx_geometry* g,
-                                                const afx_geometry* geom_tangent, int gradient, const int8_t* kind,
-                                                const double* value, const double* phi, const double* phi_tangent,
-                                                double* grad, double* grad_tangent, hipStream_t s) {
+                                                const afx_geometry* geom_tangent, int gradient, int sweeps,
+                                                double relaxation, const int8_t* kind, const double* value,
+                                                const double* phi, const double* phi_tangent, double* grad,
+                                                double* grad_tangent, hipStream_t s) {
   AFX_REQUIRE(mh && g && geom_tangent && kind && value && phi && phi_tangent && grad && grad_tangent,
               "afx_scalar_gradient_jvp_geometry: null argument");
-  AFX_REQUIRE(gradient == 0 || gradient == 1,
-              "afx_scalar_gradient_jvp_geometry: compact Green-Gauss or least-squares gradients only");
+  AFX_REQUIRE(gradient == 0 || gradient == 1 || gradient == 3,
+              "afx_scalar_gradient_jvp_geometry: unknown gradient scheme");
+  AFX_REQUIRE(gradient != 3 || relaxation > 0.0, "afx_scalar_gradient_jvp_geometry: relaxation must be positive");
   AFX_REQUIRE(geom_tangent->area && geom_tangent->normal && geom_tangent->face_centroid && geom_tangent->volume &&
                   geom_tangent->cell_centroid,
               "afx_scalar_gradient_jvp_geometry: the geometry tangent needs all five arrays");
@@ -1391,12 +1411,16 @@ extern "C" int afx_scalar_gradient_jvp_geometry(const afx_mesh* mh, const afx_ge
   gt.cc = geom_tangent->cell_centroid;
   const In<dual> ph{phi, phi_tangent};
   const Out<dual> og{grad, grad_tangent};
+  const int gk = gradient == 3 ? 0 : gradient;
 #define AFX_SGJ(DD, GG)                                                                                   \
-  if (mh->dim == DD && gradient == GG) {                                                                  \
+  if (mh->dim == DD && gk == GG) {                                                                        \
     AFX_LAUNCH(kKScalarGrad, (k_scalar_grad<DD, GG, true>), grid_for(m.C), dim3(kBlock), 0, s, m, gt, kind, \
                value, ph, og);                                                                            \
     AFX_CHECK_LAUNCH();                                                                                   \
-    return 0;                                                                                             \
+    if (gradient != 3) return 0;                                                                          \
+    double* scr = afx_mesh_scratch(const_cast<afx_mesh*>(mh), (int64_t)4 * DD * m.C);                    \
+    AFX_REQUIRE(scr, "afx_scalar_gradient_jvp_geometry: could not allocate the sweep scratch");          \
+    return cgg_sweeps_tangent(m, gt, DD, 1, grad, grad_tangent, scr, sweeps, relaxation, s);             \
   }
   AFX_SGJ(2, 0) AFX_SGJ(2, 1) AFX_SGJ(3, 0) AFX_SGJ(3, 1)
 #undef AFX_SGJ
@@ -1404,26 +1428,36 @@ extern "C" int afx_scalar_gradient_jvp_geometry(const afx_mesh* mh, const afx_ge
   return 2;
 }
 
-extern "C" int afx_scalar_gradient_vjp_geometry(const afx_mesh* mh, const afx_geometry* g, int gradient,
-                                                const int8_t* kind, const double* value, const double* phi,
-                                                const double* grad_bar, afx_geometry_cotangents* out, hipStream_t s) {
+extern "C" int afx_scalar_gradient_vjp_geometry(const afx_mesh* mh, const afx_geometry* g, int gradient, int sweeps,
+                                                double relaxation, const int8_t* kind, const double* value,
+                                                const double* phi, const double* grad_bar,
+                                                afx_geometry_cotangents* out, hipStream_t s) {
   AFX_REQUIRE(mh && g && kind && value && phi && grad_bar && out, "afx_scalar_gradient_vjp_geometry: null argument");
   AFX_REQUIRE(out->area && out->normal && out->face_centroid && out->volume && out->cell_centroid,
               "afx_scalar_gradient_vjp_geometry: all five geometry cotangent buffers are needed");
-  AFX_REQUIRE(gradient == 0 || gradient == 1,
-              "afx_scalar_gradient_vjp_geometry: compact Green-Gauss or least-squares gradients only");
+  AFX_REQUIRE(gradient == 0 || gradient == 1 || gradient == 3,
+              "afx_scalar_gradient_vjp_geometry: unknown gradient scheme");
+  AFX_REQUIRE(gradient != 3 || relaxation > 0.0, "afx_scalar_gradient_vjp_geometry: relaxation must be positive");
   MeshView m;
   if (int rv_ = afx_rview(mh, g, s, &m)) return rv_;
   if (m.C == 0) return 0;
   const int D = mh->dim;
   const int64_t n = (int64_t)D * m.C;
-  double* scratch = afx_mesh_scratch(const_cast<afx_mesh*>(mh), 2 * n);
+  // G, y; corrected Green-Gauss adds the sweep scratch (2n), the iterates and the sweeps' geometry
+  // cotangents
+  const int64_t ni = gradient == 3 ? cgg_iterate_doubles(D, 1, m.C, sweeps) : 0;
+  const int64_t extra = gradient == 3 ? 2 * n + ni + cgg_geo_bar_doubles(D, m.C, m.F) : 0;
+  double* scratch = afx_mesh_scratch(const_cast<afx_mesh*>(mh), 2 * n + extra);
   AFX_REQUIRE(scratch, "afx_scalar_gradient_vjp_geometry: could not allocate the scratch");
   double* G = scratch;
   double* y = scratch + n;
+  double* sw = scratch + 2 * n;
+  double* iters = ni ? sw + 2 * n : nullptr;
+  CggGeoBars cb{sw + 2 * n + ni, sw + 2 * n + ni + (int64_t)(2 * D + 1) * m.F};
 #define AFX_SGG(DD, GG)                                                                                        \
   if (D == DD && gradient == GG) {                                                                             \
-    AFX_LAUNCH(kKScalarGrad, (k_scalar_grad<DD, GG>), grid_for(m.C), dim3(kBlock), 0, s, m, GeoTan{}, kind,     \
+    constexpr int GK = GG == 3 ? 0 : GG;                                                                       \
+    AFX_LAUNCH(kKScalarGrad, (k_scalar_grad<DD, GK>), grid_for(m.C), dim3(kBlock), 0, s, m, GeoTan{}, kind,     \
                value, In<double>{phi, nullptr}, Out<double>{G, nullptr});                                      \
     AFX_CHECK_LAUNCH();                                                                                        \
     const double* yb = grad_bar;                                                                               \
@@ -1431,13 +1465,18 @@ extern "C" int afx_scalar_gradient_vjp_geometry(const afx_mesh* mh, const afx_ge
       AFX_LAUNCH(kKScalarGradAdj, k_lsq_solve_t<DD>, grid_for(m.C), dim3(kBlock), 0, s, m, grad_bar, y);        \
       AFX_CHECK_LAUNCH();                                                                                      \
       yb = y;                                                                                                  \
+    } else if (GG == 3) {                                                                                      \
+      if (int rc = cgg_sweeps_store(m, DD, 1, G, iters, sweeps, relaxation, s)) return rc;                     \
+      AFX_CHECK_HIP(hipMemcpyAsync(y, grad_bar, n * 8, hipMemcpyDeviceToDevice, s));                           \
+      if (int rc = cgg_sweeps_adjoint_geometry(m, DD, 1, y, G, iters, sw, sweeps, relaxation, cb, s)) return rc; \
+      yb = y;                                                                                                  \
     }                                                                                                          \
     AFX_LAUNCH(kKScalarGradAdj, (k_scalar_grad_geo_vjp<DD, GG>), grid_for(m.C), dim3(kBlock), 0, s, m, kind,    \
-               value, phi, G, yb, *out);                                                                       \
+               value, phi, G, yb, *out, cb);                                                                   \
     AFX_CHECK_LAUNCH();                                                                                        \
     return 0;                                                                                                  \
   }
-  AFX_SGG(2, 0) AFX_SGG(2, 1) AFX_SGG(3, 0) AFX_SGG(3, 1)
+  AFX_SGG(2, 0) AFX_SGG(2, 1) AFX_SGG(2, 3) AFX_SGG(3, 0) AFX_SGG(3, 1) AFX_SGG(3, 3)
 #undef AFX_SGG
   afx_set_error("afx_scalar_gradient_vjp_geometry: unsupported dimension");
   return 2;
@@ -1584,9 +1623,6 @@ extern "C" int afx_scalar_jvp_geometry(const afx_mesh* m, const afx_geometry* g,
   int rc = sc_check(m, g, cfg, in, work, work_size, 1);
   if (rc) return rc;
   AFX_REQUIRE(geom_tangent && tangent && residual && residual_tangent, "afx_scalar_jvp_geometry: null argument");
-  AFX_REQUIRE(cfg->gradient != 3,
-              "afx_scalar_jvp_geometry: a geometry tangent is available for compact Green-Gauss, least-squares or no "
-              "gradient reconstruction");
   AFX_REQUIRE(geom_tangent->area && geom_tangent->normal && geom_tangent->face_centroid && geom_tangent->volume &&
                   geom_tangent->cell_centroid,
               "afx_scalar_jvp_geometry: the geometry tangent needs all five arrays");
@@ -1611,22 +1647,33 @@ extern "C" int afx_scalar_jvp_geometry(const afx_mesh* m, const afx_geometry* g,
   gt.cc = geom_tangent->cell_centroid;
   return sc_dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
     constexpr int D = decltype(Dc)::value, ADV = decltype(Ac)::value, GR = decltype(Gc)::value;
-    if constexpr (GR == kGradCorrected) {
-      return 2;  // refused above
-    } else {
-      constexpr bool LIM = ADV == kAdvLimited;
-      AFX_LAUNCH(kKScalarGrad, (k_sc_grad_g<D, GR, LIM>), grid_for(v.C), dim3(kBlock), 0, s, v, gt, a,
-                 Out<dual>{w.G, w.Gt}, Out<dual>{w.psi, w.psit});
-      AFX_CHECK_LAUNCH();
-      a.G.v = w.G;
-      a.G.t = w.Gt;
-      a.psi.v = w.psi;
-      a.psi.t = w.psit;
-      AFX_LAUNCH(kKScalarRes, (k_sc_res_g<D, ADV>), grid_for(v.C), dim3(kBlock), 0, s, v, gt, a,
-                 Out<dual>{residual, residual_tangent});
-      AFX_CHECK_LAUNCH();
-      return 0;
+    constexpr bool LIM = ADV == kAdvLimited;
+    constexpr bool CG = GR == kGradCorrected;
+    // corrected Green-Gauss: the compact pass supplies V^-1 b, the sweeps and the limiter follow
+    AFX_LAUNCH(kKScalarGrad, (k_sc_grad_g<D, CG ? 0 : GR, LIM && !CG>), grid_for(v.C), dim3(kBlock), 0, s, v, gt, a,
+               Out<dual>{w.G, w.Gt}, Out<dual>{w.psi, w.psit});
+    AFX_CHECK_LAUNCH();
+    a.G.v = w.G;
+    a.G.t = w.Gt;
+    if (CG) {
+      double* scr = afx_mesh_scratch(const_cast<afx_mesh*>(m), (int64_t)4 * D * m->C);
+      AFX_REQUIRE(scr, "afx_scalar_jvp_geometry: could not allocate the sweep scratch");
+      int r2 = cgg_sweeps_tangent(v, gt, D, 1, w.G, w.Gt, scr, cfg->gradient_sweeps, cfg->gradient_relaxation, s);
+      if (r2) return r2;
+      if (LIM) {
+        LimiterSlots sl{};
+        sl.nl = 1;
+        sl.ext_stride = 1;
+        r2 = venkat_pass_tangent(v, gt, D, sl, a.phi, In<dual>{w.G, w.Gt}, a.lim_k, Out<dual>{w.psi, w.psit}, s);
+        if (r2) return r2;
+      }
     }
+    a.psi.v = w.psi;
+    a.psi.t = w.psit;
+    AFX_LAUNCH(kKScalarRes, (k_sc_res_g<D, ADV>), grid_for(v.C), dim3(kBlock), 0, s, v, gt, a,
+               Out<dual>{residual, residual_tangent});
+    AFX_CHECK_LAUNCH();
+    return 0;
   });
 }
 
@@ -1641,9 +1688,6 @@ extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const af
   const bool geo = gb.area || gb.normal || gb.face_centroid || gb.volume || gb.cell_centroid;
   AFX_REQUIRE(!geo || (gb.area && gb.normal && gb.face_centroid && gb.volume && gb.cell_centroid),
               "afx_scalar_vjp: geometry cotangents need all five arrays");
-  AFX_REQUIRE(!geo || cfg->gradient == 0 || cfg->gradient == 1 || cfg->gradient == 2,
-              "afx_scalar_vjp: geometry cotangents are available for compact Green-Gauss, least-squares or no "
-              "gradient reconstruction");
   ScWork w = sc_carve(work, m->dim, m->C, 2);
   ScArgs<double> a = sc_args<double>(cfg, in, nullptr);
   AFX_REQUIRE(cfg->transient == 0 || out->phi_old, "afx_scalar_vjp: the transient term needs a phi_old cotangent");
@@ -1653,8 +1697,22 @@ extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const af
   return sc_dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
     constexpr int D = decltype(Dc)::value, ADV = decltype(Ac)::value, GR = decltype(Gc)::value;
     constexpr bool LIM = ADV == kAdvLimited;
+    // corrected Green-Gauss with geometry cotangents: the forward iterates and the sweeps' own
+    // geometry cotangents live in the mesh handle's scratch
+    double* iters = nullptr;
+    CggGeoBars cb{nullptr, nullptr};
+    if (GR == kGradCorrected && geo) {
+      const int64_t ni = cgg_iterate_doubles(D, 1, m->C, cfg->gradient_sweeps);
+      double* scr = afx_mesh_scratch(const_cast<afx_mesh*>(m), ni + cgg_geo_bar_doubles(D, m->C, m->F));
+      AFX_REQUIRE(scr, "afx_scalar_vjp: could not allocate the corrected-gradient scratch");
+      iters = ni ? scr : nullptr;
+      cb.face = scr + ni;
+      cb.vol = cb.face + (int64_t)(2 * D + 1) * m->F;
+      bar.cggf = cb.face;
+      bar.cggv = cb.vol;
+    }
     int r2 = sc_forward<D, double, ADV, GR>(m, g, a, Out<double>{nullptr, nullptr}, w, nullptr, nullptr, w.ext,
-                                            cfg->gradient_sweeps, cfg->gradient_relaxation, s);
+                                            cfg->gradient_sweeps, cfg->gradient_relaxation, s, iters);
     if (r2) return r2;
     MeshView v;
     if (int rv_ = afx_rview(m, g, s, &v)) return rv_;
@@ -1671,7 +1729,9 @@ extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const af
                  bar);
       AFX_CHECK_LAUNCH();
       if (GR == kGradCorrected) {
-        int r3 = cgg_sweeps_adjoint(v, D, 1, bar.G, w.cgg, cfg->gradient_sweeps, cfg->gradient_relaxation, s);
+        int r3 = GEO ? cgg_sweeps_adjoint_geometry(v, D, 1, bar.G, w.G, iters, w.cgg, cfg->gradient_sweeps,
+                                                   cfg->gradient_relaxation, cb, s)
+                     : cgg_sweeps_adjoint(v, D, 1, bar.G, w.cgg, cfg->gradient_sweeps, cfg->gradient_relaxation, s);
         if (r3) return r3;
       }
       AFX_LAUNCH(kKScalarGradAdj, (k_sc_grad_adj<D, GK, LIM, GEO>), grid_for(m->C), dim3(kBlock), 0, s, v, b,
@@ -1679,10 +1739,6 @@ extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const af
       AFX_CHECK_LAUNCH();
       return 0;
     };
-    if constexpr (GR == kGradCorrected) {
-      return run(std::false_type{});
-    } else {
-      return geo ? run(std::true_type{}) : run(std::false_type{});
-    }
+    return geo ? run(std::true_type{}) : run(std::false_type{});
   });
 }

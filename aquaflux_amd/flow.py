@@ -324,10 +324,6 @@ class _FlowResidualFn(torch.autograd.Function):
     def backward(ctx, gR):
         prim = ctx.saved_tensors
         if any(ctx.needs_input_grad[8:13]):
-            if ctx.raw.cfg.gradient == 3:
-                raise NotImplementedError("reverse-mode derivatives of the flow residual with respect to the "
-                                          "geometry are provided for compact Green-Gauss or least-squares gradients, "
-                                          "not the corrected Green-Gauss scheme")
             bars, gbars = ctx.raw.vjp_geometry(prim, gR.contiguous())
         else:
             bars, gbars = ctx.raw.vjp(prim, gR.contiguous()), [None] * 5
@@ -547,10 +543,6 @@ class MomentumContinuity:
         mu, rho, ub, pb, gb = self._inputs()
         geo = self._geometry()
         geo_tan = any(fwAD.unpack_dual(t).tangent is not None for t in geo)
-        if geo_tan and self._raw.cfg.gradient == 3:
-            raise NotImplementedError("forward-mode derivatives with respect to the geometry are provided for "
-                                      "compact Green-Gauss or least-squares gradients, not the corrected "
-                                      "Green-Gauss scheme")
         return _FlowResidualFn.apply(self._raw, geo_tan, state.contiguous(), mu, rho, ub, pb, gb, *geo)
 
     __call__ = residual

@@ -367,10 +367,6 @@ class ScalarTransport:
             step, old, older = self._time_level(phi, phi_old, phi_older, dt, first_step)
             geo = self._geometry()
             geo_tan = any(fwAD.unpack_dual(t).tangent is not None for t in geo)
-            if geo_tan and self._raw.cfg.gradient == 3:
-                raise NotImplementedError("forward-mode derivatives with respect to the geometry are provided for "
-                                          "Green-Gauss, least-squares or no gradient reconstruction, not the "
-                                          "corrected Green-Gauss scheme")
             return _ScalarResidualFn.apply(self._raw, step, geo_tan, phi.contiguous(), self._gamma(),
                                            flux.contiguous(), self._val, self._tinf, old, older, *geo)
 

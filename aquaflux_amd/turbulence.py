@@ -364,8 +364,9 @@ class _RansRaw:
         _lib.check(
             _lib.lib().afx_scalar_gradient_jvp_geometry(
                 self.dm.handle, ctypes.byref(self.gs), ctypes.byref(gt), int(self.cfg.flow.gradient),
-                t._dkind.data_ptr(), t._dval.data_ptr(), d.data_ptr(), dd.data_ptr(), dgrad.data_ptr(),
-                dgrad_t.data_ptr(), _lib.stream_of(self.dm.device),
+                int(self.cfg.flow.gradient_sweeps), float(self.cfg.flow.gradient_relaxation), t._dkind.data_ptr(),
+                t._dval.data_ptr(), d.data_ptr(), dd.data_ptr(), dgrad.data_ptr(), dgrad_t.data_ptr(),
+                _lib.stream_of(self.dm.device),
             ),
             "afx_scalar_gradient_jvp_geometry",
         )
@@ -421,15 +422,17 @@ class _RansRaw:
         )
         # the wall-distance gradient: d's share and the geometry's
         gcode = int(self.cfg.flow.gradient)
+        sweeps, relax = int(self.cfg.flow.gradient_sweeps), float(self.cfg.flow.gradient_relaxation)
         d_extra = torch.empty_like(d)
-        _lib.check(L.afx_scalar_gradient_vjp(self.dm.handle, ctypes.byref(self.gs), gcode, 4, 1.0, t._dkind.data_ptr(),
-                                             dgb.data_ptr(), d_extra.data_ptr(), None, stream), "afx_scalar_gradient_vjp")
+        _lib.check(L.afx_scalar_gradient_vjp(self.dm.handle, ctypes.byref(self.gs), gcode, sweeps, relax,
+                                             t._dkind.data_ptr(), dgb.data_ptr(), d_extra.data_ptr(), None, stream),
+                   "afx_scalar_gradient_vjp")
         db = db + d_extra
         g2 = [torch.empty_like(x) for x in geo]
         gouts2 = _lib.AfxGeometryCotangents(*[b.data_ptr() for b in g2])
-        _lib.check(L.afx_scalar_gradient_vjp_geometry(self.dm.handle, ctypes.byref(self.gs), gcode, t._dkind.data_ptr(),
-                                                      t._dval.data_ptr(), d.data_ptr(), dgb.data_ptr(),
-                                                      ctypes.byref(gouts2), stream),
+        _lib.check(L.afx_scalar_gradient_vjp_geometry(self.dm.handle, ctypes.byref(self.gs), gcode, sweeps, relax,
+                                                      t._dkind.data_ptr(), t._dval.data_ptr(), d.data_ptr(),
+                                                      dgb.data_ptr(), ctypes.byref(gouts2), stream),
                    "afx_scalar_gradient_vjp_geometry")
         gbars = [a + b for a, b in zip(gbars, g2)]
         # d = |x_c - x_f*| to the nearest wall-face centroid f*
@@ -512,10 +515,6 @@ class _RansResidualFn(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gR):
         if any(ctx.needs_input_grad[8:]):
-            if ctx.raw.cfg.flow.gradient == 3:
-                raise NotImplementedError("reverse-mode derivatives of the RANS residual with respect to the "
-                                          "geometry are provided for compact Green-Gauss or least-squares gradients, "
-                                          "not the corrected Green-Gauss scheme")
             bars, gbars = ctx.raw.vjp_geometry(ctx.saved_tensors, gR.contiguous())
             return (None, None, *bars, *gbars)
         bars = ctx.raw.vjp(ctx.saved_tensors, gR.contiguous())
@@ -615,10 +614,6 @@ class CoupledRANS:
         g = self.momentum.geometry
         geo = (g.face.area, g.face.normal_soa, g.face.centroid_soa, g.cell.volume, g.cell.centroid_soa)
         geo_tan = any(fwAD.unpack_dual(t).tangent is not None for t in geo)
-        if geo_tan and self._raw.cfg.flow.gradient == 3:
-            raise NotImplementedError("forward-mode derivatives with respect to the geometry are provided for "
-                                      "compact Green-Gauss or least-squares gradients, not the corrected "
-                                      "Green-Gauss scheme")
         return _RansResidualFn.apply(self._raw, geo_tan, state.contiguous(), mu, rho, nu, ub, pb, *geo)
 
     __call__ = residual

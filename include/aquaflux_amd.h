@@ -239,24 +239,26 @@ int afx_scalar_gradient(const afx_mesh* mesh, const afx_geometry* geom, int grad
                         hipStream_t stream);
 
 /* Gradient and its tangent along a tangent of phi and of the geometry together (geom_tangent as
- * for afx_scalar_jvp_geometry; boundary values constant).  Compact Green-Gauss or least squares.
- * Replaces jax.jvp of GradientScheme.gradients with Mesh.node_coords as a differentiable leaf (the
- * SST wall-distance gradient of a differentiated build; reference: aquaflux/schemes/gradient.py:98-153,
+ * for afx_scalar_jvp_geometry; boundary values constant).  Compact Green-Gauss, least squares or
+ * corrected Green-Gauss (`sweeps`, `relaxation` as for afx_scalar_gradient).  Replaces jax.jvp of
+ * GradientScheme.gradients with Mesh.node_coords as a differentiable leaf (the SST wall-distance
+ * gradient of a differentiated build; reference: aquaflux/schemes/gradient.py:98-153, :1160-1337,
  * aquaflux/turbulence/transport.py:259-313). */
 int afx_scalar_gradient_jvp_geometry(const afx_mesh* mesh, const afx_geometry* geom, const afx_geometry* geom_tangent,
-                                     int gradient, const int8_t* kind, const double* value, const double* phi,
-                                     const double* phi_tangent, double* grad, double* grad_tangent,
-                                     hipStream_t stream);
+                                     int gradient, int sweeps, double relaxation, const int8_t* kind,
+                                     const double* value, const double* phi, const double* phi_tangent, double* grad,
+                                     double* grad_tangent, hipStream_t stream);
 
 /* Cotangents of the five geometry arrays (written; caller face order) of the gradient of phi for a
  * gradient cotangent grad_bar[d][C] (boundary values and phi held fixed; their cotangents come
- * from afx_scalar_gradient_vjp).  Compact Green-Gauss or least squares.  Replaces jax.vjp of
+ * from afx_scalar_gradient_vjp).  Compact Green-Gauss, least squares or corrected Green-Gauss
+ * (`sweeps`, `relaxation` as for afx_scalar_gradient).  Replaces jax.vjp of
  * GradientScheme.gradients with Mesh.node_coords as a differentiable leaf (the SST wall-distance
- * gradient of a differentiated build; reference: aquaflux/schemes/gradient.py:98-153,
+ * gradient of a differentiated build; reference: aquaflux/schemes/gradient.py:98-153, :1160-1337,
  * aquaflux/turbulence/transport.py:259-313). */
-int afx_scalar_gradient_vjp_geometry(const afx_mesh* mesh, const afx_geometry* geom, int gradient, const int8_t* kind,
-                                     const double* value, const double* phi, const double* grad_bar,
-                                     afx_geometry_cotangents* out, hipStream_t stream);
+int afx_scalar_gradient_vjp_geometry(const afx_mesh* mesh, const afx_geometry* geom, int gradient, int sweeps,
+                                     double relaxation, const int8_t* kind, const double* value, const double* phi,
+                                     const double* grad_bar, afx_geometry_cotangents* out, hipStream_t stream);
 
 /* Reverse of afx_scalar_gradient (same scheme, closures and geometry): phi_bar[C] and, when not
  * NULL, value_bar[F] (zero except on Dirichlet faces, caller order) from grad_bar[j][C].  The

@@ -215,6 +215,11 @@ GEO_CASES = [
     ("box3d", "limited", "gg"),
     ("brick2d", "limited", "lsq"),
     ("brick3d", "limited", "lsq"),
+    ("channel2d", "limited", "cgg"),
+    ("channel2d", "upwind", "cgg9r0.8"),
+    ("channel3d", "limited", "cgg6"),
+    ("cavity", "upwind", "cgg1"),
+    ("brick3d", "upwind", "cgg"),
 ]
 
 
@@ -256,19 +261,6 @@ def test_node_coordinate_tangent_matches_oracle(cuda, name, scheme, gradient):
     assert err < 1e-11
 
 
-def test_node_coordinate_tangent_refused_for_corrected_green_gauss(cuda):
-    import torch.autograd.forward_ad as fwAD
-
-    from tests.flow_cases import make_mesh
-
-    mesh = make_mesh("channel2d")
-    with fwAD.dual_level():
-        X = torch.tensor(mesh.node_coords, dtype=F64, device=cuda)
-        _, _, mc = build("channel2d", "limited", "cgg", device=cuda, node_coords=fwAD.make_dual(X, torch.ones_like(X)))
-        with pytest.raises(NotImplementedError, match="corrected Green-Gauss"):
-            mc.residual(torch.tensor(seeded_state(mesh, 0, "channel"), dtype=F64, device=cuda))
-
-
 @pytest.mark.parametrize("name,scheme,gradient", GEO_CASES)
 def test_node_coordinate_adjoint_matches_oracle(cuda, name, scheme, gradient):
     """d <lambda, R(state; geometry(X))> / dX through the hand-written flow adjoint with geometry
@@ -307,36 +299,3 @@ def test_node_coordinate_adjoint_matches_oracle(cuda, name, scheme, gradient):
     a = float((dR.cpu() * T(lam)).sum())
     b = float((got.cpu() * T(dX)).sum())
     assert abs(a - b) <= 1e-12 * float(dR.abs().sum()) * float(np.abs(lam).max())
-
-
-def test_node_coordinate_adjoint_refused_for_corrected_green_gauss(cuda):
-    from tests.flow_cases import make_mesh
-
-    mesh = make_mesh("channel2d")
-    X = torch.tensor(mesh.node_coords, dtype=F64, device=cuda, requires_grad=True)
-    _, _, mc = build("channel2d", "limited", "cgg", device=cuda, node_coords=X)
-    R = mc.residual(torch.tensor(seeded_state(mesh, 0, "channel"), dtype=F64, device=cuda))
-    with pytest.raises(NotImplementedError, match="corrected Green-Gauss"):
-        torch.autograd.grad(R.sum(), X)
-
-
-def test_rans_geometry_derivatives_are_refused_not_dropped(cuda):
-    """Geometry derivatives of the RANS residual with corrected Green-Gauss raise (both modes)
-    instead of returning a result that leaves the geometry's part out."""
-    import torch.autograd.forward_ad as fwAD
-
-    from tests import rans_cases as RC
-    from tests.flow_cases import make_mesh
-
-    mesh = make_mesh("channel2d")
-    X2 = torch.tensor(mesh.node_coords, dtype=F64, device=cuda, requires_grad=True)
-    rmesh, rans, _ = RC.build("channel2d_sst", "limited", "limited", "cgg", device=cuda, node_coords=X2)
-    state = torch.tensor(RC.seeded_state(rmesh, 0, "channel"), dtype=F64, device=cuda)
-    with pytest.raises(NotImplementedError, match="corrected Green-Gauss"):
-        torch.autograd.grad(rans.residual(state).sum(), X2)
-    with fwAD.dual_level():
-        X3 = torch.tensor(mesh.node_coords, dtype=F64, device=cuda)
-        _, rans3, _ = RC.build("channel2d_sst", "limited", "limited", "cgg", device=cuda,
-                               node_coords=fwAD.make_dual(X3, torch.ones_like(X3)))
-        with pytest.raises(NotImplementedError, match="corrected Green-Gauss"):
-            rans3.residual(state)

@@ -368,6 +368,8 @@ GEO_CASES = [
     ("channel3d_sst", "limited", "upwind", "lsq", 1e-5),
     ("box3d_sst", "limited", "limited", "gg", 1e-3),
     ("brick3d_sst", "limited", "limited", "lsq", 1e-4),
+    ("channel2d_sst", "limited", "limited", "cgg", 1e-5),
+    ("channel3d_sst", "upwind", "upwind", "cgg6r0.9", 1e-4),
 ]
 
 

@@ -34,11 +34,11 @@ CASES = [
     ("brick3d", "limited", "gg"),
     ("brick3d", "limited", "lsq"),
 ]
-# Node-coordinate adjoints: every scheme but the corrected one.  tet3 with limited least squares is
-# left out: at that case's seed two limiter candidates of some cells are exactly equal, where the
-# oracle (like JAX's segment_min) splits the derivative between the tied faces and the kernels take
-# the first (a documented parity-unpinned point); brick3d and quad_mixed cover the scheme.
-GEO_CASES = [c for c in CASES if c[2] != "cgg" and c != ("tet3", "limited", "lsq")]
+# Node-coordinate derivatives: every scheme.  tet3 with limited least squares is left out: at that
+# case's seed two limiter candidates of some cells are exactly equal, where the oracle (like JAX's
+# segment_min) splits the derivative between the tied faces and the kernels take the first (a
+# documented parity-unpinned point); brick3d and quad_mixed cover the scheme.
+GEO_CASES = [c for c in CASES if c != ("tet3", "limited", "lsq")] + [("brick3d", "upwind", "cgg")]
 
 
 def _ids(c):
@@ -293,18 +293,6 @@ def test_geometry_tangent_matches_oracle(cuda):
             ev, et = rel(pv, v), rel(pt, t)
             print(name, label, "value err", ev, "tangent err", et)
             assert ev < TOL_R and et < TOL_D, (name, label)
-
-
-def test_node_coordinate_tangent_refused_for_corrected_green_gauss(cuda):
-    mesh, u, bcs = SC.make("quad_mixed")
-    phi, gamma = SC.seeded(mesh, 0)
-    with fwAD.dual_level():
-        X = torch.tensor(mesh.node_coords, dtype=F64, device=cuda)
-        Xd = fwAD.make_dual(X, torch.ones_like(X))
-        geom, st = SC.build_gpu(mesh, bcs, "limited", "cgg", gamma, cuda, node_coords=Xd)
-        fn = st.residual(SC.gpu_flux(geom, u))
-        with pytest.raises(NotImplementedError, match="corrected Green-Gauss"):
-            fn(torch.tensor(phi, device=cuda))
 
 
 # --- transient (BDF1 first step, BDF2 afterwards) ----------------------------------------------
