@@ -92,10 +92,7 @@ def test_scalar_matches_golden(name, cuda):
     _same_mesh(mesh, z)
     transient = first is not None
     T = lambda a, **kw: torch.tensor(a, dtype=F64, device=cuda, **kw)  # noqa: E731
-    # node-coordinate cotangents are built for compact Green-Gauss, least-squares and gradient-free
-    # schemes; the corrected scheme refuses them (checked below) and is compared on the rest
-    node_adjoint = grad in ("gg", "lsq", "none")
-    X = T(z["node_coords"], requires_grad=node_adjoint)
+    X = T(z["node_coords"], requires_grad=True)
     geom, st = SC.build_gpu(mesh, bcs, adv, grad, z["gamma"], cuda, node_coords=X, transient=transient)
     hist = (T(z["phi_old"]), None if first else T(z["phi_older"]), float(z["dt"]), bool(first)) if transient else ()
 
@@ -110,7 +107,7 @@ def test_scalar_matches_golden(name, cuda):
 
     # cotangents: state and node coordinates through the residual and geometry adjoints
     x = T(z["state"], requires_grad=True)
-    leaves = [x, X] if node_adjoint else [x]
+    leaves = [x, X]
     if transient:
         po = T(z["phi_old"], requires_grad=True)
         poo = None if first else T(z["phi_older"], requires_grad=True)
@@ -120,14 +117,7 @@ def test_scalar_matches_golden(name, cuda):
         Rg = st.residual(SC.gpu_flux(geom, u))(x)
     got = list(torch.autograd.grad(Rg, leaves, T(z["cotangent"])))
     _check("vjp state", got.pop(0), z["vjp_state"], 1, TOL_D)
-    if node_adjoint:
-        _check("vjp node coords", got.pop(0), z["vjp_node_coords"], 1, TOL_D)
-    else:
-        Xg = T(z["node_coords"], requires_grad=True)
-        geom_g, st_g = SC.build_gpu(mesh, bcs, adv, grad, z["gamma"], cuda, node_coords=Xg, transient=transient)
-        Rn = st_g.residual(SC.gpu_flux(geom_g, u))(x.detach(), *hist)
-        with pytest.raises(ValueError, match="geometry cotangents"):
-            torch.autograd.grad(Rn, Xg, T(z["cotangent"]))
+    _check("vjp node coords", got.pop(0), z["vjp_node_coords"], 1, TOL_D)
     if transient:
         _check("vjp phi_old", got.pop(0), z["vjp_phi_old"], 1, TOL_D)
         if not first:
