@@ -251,6 +251,11 @@ _SIGNATURES = {
                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "afx_venkat_limiter": [c_void_p, ctypes.POINTER(AfxGeometry), ctypes.c_double, c_void_p, c_void_p, c_void_p,
                            c_void_p, c_void_p, c_void_p, c_void_p],
+    "afx_rans_residual_vjp_host": [
+        ctypes.c_int, ctypes.c_int64, ctypes.c_int64, c_void_p, c_void_p, ctypes.POINTER(AfxGeometry),
+        ctypes.POINTER(AfxRansConfig), ctypes.POINTER(AfxRansInputs), c_void_p, c_void_p,
+        ctypes.POINTER(AfxRansCotangents), ctypes.c_int,
+    ],
     "afx_scalar_gradient": [
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.c_int, ctypes.c_int, ctypes.c_double, c_void_p, c_void_p,
         c_void_p, c_void_p, c_void_p,

@@ -26,6 +26,7 @@ FLAGS = [
     "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-ffp-contract=fast",
+    "-fopenmp",  # host side only: the OpenMP CPU line (afx_rans_residual_vjp_host)
     f"-I{INCLUDE}",
     f"-I{CSRC}",
 ]
@@ -59,7 +60,7 @@ def build(verbose: bool = False) -> pathlib.Path:
     with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
         objs = list(ex.map(compile_one, sources))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-fopenmp", "-o", str(tmp), *map(str, objs)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")

@@ -17,7 +17,7 @@
 #include <string>
 
 #ifndef AFX_DEV
-#define AFX_DEV __device__ __forceinline__
+#define AFX_DEV __host__ __device__ __forceinline__
 #endif
 #define AFX_HD __host__ __device__ __forceinline__
 
@@ -604,6 +604,26 @@ AFX_DEV int tile_slot(int32_t o, int32_t base) {
   const uint32_t d = (uint32_t)(o - base);
   return d < (uint32_t)kBlock ? (int)d : -1;
 }
+
+// Where the body of a staged cell kernel runs.  DevExec: thread t of the 256-thread workgroup that
+// owns the block at `base`; the workgroup stages its cells in LDS, synchronises, and reads a
+// neighbour inside the block from there.  HostExec: one cell at a time on the host (the OpenMP CPU
+// line), with no workgroup to wait for and every neighbour read from memory (slot() is -1), so the
+// staging writes are never read back.  Either way a cell computes the same values.
+struct DevExec {
+  int32_t base;
+  int t;
+  __device__ explicit DevExec(const MeshView& m) : base(block_base(m)), t((int)threadIdx.x) {}
+  __device__ void sync() const { __syncthreads(); }
+  __device__ int slot(int32_t o) const { return tile_slot(o, base); }
+};
+struct HostExec {
+  int32_t base;
+  int t;
+  explicit HostExec(int32_t c) : base(c - c % kBlock), t(c % kBlock) {}
+  void sync() const {}
+  int slot(int32_t) const { return -1; }
+};
 
 // Cells in XCD-contiguous order over a bounded set of resident workgroups: workgroup b on XCD
 // x = b % 8 walks that XCD's logical blocks t = b / 8, b / 8 + g_x, ... (g_x = workgroups on XCD

@@ -13,7 +13,11 @@
 //   R  momentum / continuity / k / omega fluxes, SST sources, omega wall-cell fixation
 // The adjoint runs G, C, D forward, then R', D', C' (closure and limiter adjoints by seeded duals,
 // face fluxes by hand), G'.
+#include <memory>
 #include <type_traits>
+#include <vector>
+
+#include <omp.h>
 
 #include "afx_common.h"
 #include "afx_mesh.h"
@@ -114,17 +118,22 @@ AFX_DEV void closure_cell(const MeshView& m, const RansArgs<S>& a, int32_t c, S 
 // still in registers, saving the closure pass's re-read of k, omega and the gradients.  With GEO
 // the geometry is read as (value, tangent) pairs (the node-coordinate JVP; S = dual), centroids
 // from global memory rather than the tile.
-template <int D, class S, int GRAD, bool LIMF, bool LIMS, bool CLO = false, bool GEO = false>
-__global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Out<S> G, Out<S> psi_out,
-                                                   int32_t* __restrict__ ext, int32_t* __restrict__ kmin_out,
-                                                   ClosureOut co = ClosureOut{}) {
+// LDS staging of k_rans_grad: the block's states and centroids.
+template <int D, class S>
+struct RansGradSh {
+  S tx[D + 3][kBlock];
+  double tcc[D][kBlock];
+};
+
+template <int D, class S, int GRAD, bool LIMF, bool LIMS, bool CLO = false, bool GEO = false, class Exec, class Sh>
+AFX_DEV void k_rans_grad_body(const Exec& ex, Sh& sh, MeshView m, RansArgs<S> a, Out<S> G, Out<S> psi_out, int32_t* __restrict__ ext, int32_t* __restrict__ kmin_out, ClosureOut co) {
   constexpr int NV = D + 3, NL = D + 2;
   using GR = GeoRead<D, GEO>;
   using GT = typename GR::T;
-  __shared__ S tx[NV][kBlock];
-  __shared__ double tcc[D][kBlock];
-  const int32_t base = block_base(m);
-  const int t = threadIdx.x;
+  auto& tx = sh.tx;
+  auto& tcc = sh.tcc;
+  const int32_t base = ex.base;
+  const int t = ex.t;
   const int32_t c = base + t;
   const bool valid = c < m.C;
   S xc[NV];
@@ -138,7 +147,7 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
       tcc[j][t] = val(ccc[j]);
     }
   }
-  __syncthreads();
+  ex.sync();
   if (!valid) return;
   S acc[NV][D];
 #pragma unroll
@@ -169,7 +178,7 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
     if (o >= 0) {
       S xo[NV];
       GT cco[D];
-      const int ls = tile_slot(o, base);
+      const int ls = ex.slot(o);
       if (ls >= 0) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) xo[v] = tx[v][ls];
@@ -336,6 +345,14 @@ __global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Ou
   }
 }
 
+template <int D, class S, int GRAD, bool LIMF, bool LIMS, bool CLO = false, bool GEO = false>
+__global__ void __launch_bounds__(256) k_rans_grad(MeshView m, RansArgs<S> a, Out<S> G, Out<S> psi_out,
+                                                   int32_t* __restrict__ ext, int32_t* __restrict__ kmin_out,
+                                                   ClosureOut co = ClosureOut{}) {
+  __shared__ RansGradSh<D, S> sh;
+  k_rans_grad_body<D, S, GRAD, LIMF, LIMS, CLO, GEO>(DevExec(m), sh, m, a, G, psi_out, ext, kmin_out, co);
+}
+
 // ---------------------------------------------------------------------------------------------
 // pass C: closure
 template <int D, class S>
@@ -404,12 +421,16 @@ AFX_DEV void closure_cell(const MeshView& m, const RansArgs<S>& a, int32_t c, S 
 }
 
 template <int D, class S, bool GEO = false>
-__global__ void __launch_bounds__(256) k_rans_closure(MeshView m, RansArgs<S> a, ClosureOut out) {
-  const int32_t c = xcd_thread_index(m);
+AFX_DEV void k_rans_closure_body(int32_t c, MeshView m, RansArgs<S> a, ClosureOut out) {
   if (c >= m.C) return;
   S k, w, Gu[D][D], Gk[D], Gw[D];
   load_closure_inputs<D, S>(a, m, c, k, w, Gu, Gk, Gw);
   closure_cell<D, S, GEO>(m, a, c, k, w, Gu, Gk, Gw, out);
+}
+
+template <int D, class S, bool GEO = false>
+__global__ void __launch_bounds__(256) k_rans_closure(MeshView m, RansArgs<S> a, ClosureOut out) {
+  k_rans_closure_body<D, S, GEO>(xcd_thread_index(m), m, a, out);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -494,13 +515,18 @@ struct ScalarTile {
 // ---------------------------------------------------------------------------------------------
 // pass D
 // With GEO the geometry is read as (value, tangent) pairs and every cell from global memory.
-template <int D, class S, int ADV, bool GEO = false>
-__global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<S> aP) {
+template <int D, class S>
+struct RansApSh {
+  FlowTile<D, S, false> tile;
+};
+
+template <int D, class S, int ADV, bool GEO = false, class Exec, class Sh>
+AFX_DEV void k_rans_ap_body(const Exec& ex, Sh& sh, MeshView m, RansArgs<S> a, Out<S> aP) {
   using GR = GeoRead<D, GEO>;
   using GT = typename GR::T;
-  __shared__ FlowTile<D, S, false> tile;
-  const int32_t base = block_base(m);
-  const int t = threadIdx.x;
+  auto& tile = sh.tile;
+  const int32_t base = ex.base;
+  const int t = ex.t;
   const int32_t c = base + t;
   const bool valid = c < m.C;
   FlowCell<D, S, GT> X;
@@ -511,7 +537,7 @@ __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<
     for (int j = 0; j < D; ++j) xc[j] = GR::cc(m, a.gt, j, c);
     if constexpr (!GEO) tile.put(t, X, xc);
   }
-  if constexpr (!GEO) __syncthreads();
+  if constexpr (!GEO) ex.sync();
   if (!valid) return;
   S acc = sconst<S>(0.0);
   for (FaceWalk walk(m, c); walk.more();) {
@@ -522,7 +548,7 @@ __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<
       FlowCell<D, S, GT> O;
       GT xo[D];
       int lo = -1;
-      if constexpr (!GEO) lo = tile_slot(o, base);
+      if constexpr (!GEO) lo = ex.slot(o);
       if (lo >= 0) {
         if constexpr (!GEO) tile.get(lo, O, xo);
       } else {
@@ -546,20 +572,31 @@ __global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<
   aP.set(c, acc);
 }
 
+template <int D, class S, int ADV, bool GEO = false>
+__global__ void __launch_bounds__(256) k_rans_ap(MeshView m, RansArgs<S> a, Out<S> aP) {
+  __shared__ RansApSh<D, S> sh;
+  k_rans_ap_body<D, S, ADV, GEO>(DevExec(m), sh, m, a, aP);
+}
+
 // ---------------------------------------------------------------------------------------------
 // pass R, flow part: momentum and continuity rows, and the Rhie-Chow face mass flux (written by
 // each face's owner) that the k / omega fluxes advect on.
-template <int D, class S, int ADVF, bool GEO = false>
-__global__ void __launch_bounds__(256) k_rans_fres(MeshView m, RansArgs<S> a, Out<S> R, Out<S> mdot) {
+template <int D>
+struct RansFresSh {
+  FlowTile<D, double, true> tile;
+};
+
+template <int D, class S, int ADVF, bool GEO = false, class Exec, class Sh>
+AFX_DEV void k_rans_fres_body(const Exec& ex, Sh& sh, MeshView m, RansArgs<S> a, Out<S> R, Out<S> mdot) {
   constexpr int NV = D + 1;
   using GR = GeoRead<D, GEO>;
   using GT = typename GR::T;
   // the FP64 pass stages its block's cells (53 KB of LDS in 3D); the dual-number (tangent) pass,
   // whose tile would be twice that, gathers from global memory
   constexpr bool kStage = std::is_same<S, double>::value && !GEO;
-  __shared__ FlowTile<D, double, true> tile;
-  const int32_t base = block_base(m);
-  const int t = threadIdx.x;
+  auto& tile = sh.tile;
+  const int32_t base = ex.base;
+  const int t = ex.t;
   const int32_t c = base + t;
   const bool valid = c < m.C;
   FlowCell<D, S, GT> X;
@@ -570,7 +607,7 @@ __global__ void __launch_bounds__(256) k_rans_fres(MeshView m, RansArgs<S> a, Ou
     for (int j = 0; j < D; ++j) xc[j] = GR::cc(m, a.gt, j, c);
     if constexpr (kStage) tile.put(t, X, xc);
   }
-  if constexpr (kStage) __syncthreads();
+  if constexpr (kStage) ex.sync();
   if (!valid) return;
   S acc[NV];
 #pragma unroll
@@ -584,7 +621,7 @@ __global__ void __launch_bounds__(256) k_rans_fres(MeshView m, RansArgs<S> a, Ou
       FlowCell<D, S, GT> O;
       GT xo[D];
       int lo = -1;
-      if constexpr (kStage) lo = tile_slot(o, base);
+      if constexpr (kStage) lo = ex.slot(o);
       if (lo >= 0) {
         if constexpr (kStage) tile.get(lo, O, xo);
       } else {
@@ -617,6 +654,12 @@ __global__ void __launch_bounds__(256) k_rans_fres(MeshView m, RansArgs<S> a, Ou
   R.set((int64_t)D * m.C + c, cont);
 }
 
+template <int D, class S, int ADVF, bool GEO = false>
+__global__ void __launch_bounds__(256) k_rans_fres(MeshView m, RansArgs<S> a, Out<S> R, Out<S> mdot) {
+  __shared__ RansFresSh<D> sh;
+  k_rans_fres_body<D, S, ADVF, GEO>(DevExec(m), sh, m, a, R, mdot);
+}
+
 // k and omega face fluxes of one face seen from cell c (all scalar operands in registers).
 template <int D, class S, int ADVS, class GT = double>
 AFX_DEV void rans_scalar_face(const RansArgs<S>& a, const MeshView& m, const FaceRef& r, int32_t o,
@@ -643,14 +686,19 @@ AFX_DEV void rans_scalar_face(const RansArgs<S>& a, const MeshView& m, const Fac
 }
 
 // pass R, k / omega part: transport on mdot / density, SST sources, omega wall fixation.
-template <int D, class S, int ADVS, bool GEO = false>
-__global__ void __launch_bounds__(256) k_rans_sres(MeshView m, RansArgs<S> a, In<S> mdot, Out<S> R) {
+template <int D, class S>
+struct RansSresSh {
+  ScalarTile<D, S, false> tile;
+};
+
+template <int D, class S, int ADVS, bool GEO = false, class Exec, class Sh>
+AFX_DEV void k_rans_sres_body(const Exec& ex, Sh& sh, MeshView m, RansArgs<S> a, In<S> mdot, Out<S> R) {
   using GR = GeoRead<D, GEO>;
   using GT = typename GR::T;
   const SstConst& sc = a.sc;
-  __shared__ ScalarTile<D, S, false> tile;
-  const int32_t base = block_base(m);
-  const int t = threadIdx.x;
+  auto& tile = sh.tile;
+  const int32_t base = ex.base;
+  const int t = ex.t;
   const int32_t c = base + t;
   const bool valid = c < m.C;
   ScalarCell<D, S> Kc, Wc;
@@ -662,7 +710,7 @@ __global__ void __launch_bounds__(256) k_rans_sres(MeshView m, RansArgs<S> a, In
     for (int j = 0; j < D; ++j) xc[j] = GR::cc(m, a.gt, j, c);
     if constexpr (!GEO) tile.put(t, Kc, Wc, xc);
   }
-  if constexpr (!GEO) __syncthreads();
+  if constexpr (!GEO) ex.sync();
   if (!valid) return;
   S acck = sconst<S>(0.0), accw = sconst<S>(0.0);
   const double irho = 1.0 / sc.density;
@@ -675,7 +723,7 @@ __global__ void __launch_bounds__(256) k_rans_sres(MeshView m, RansArgs<S> a, In
     if (o >= 0) {
       GT xo[D];
       int lo = -1;
-      if constexpr (!GEO) lo = tile_slot(o, base);
+      if constexpr (!GEO) lo = ex.slot(o);
       if (lo >= 0) {
         if constexpr (!GEO) tile.get(lo, Ko, Wo, xo);
       } else {
@@ -705,6 +753,12 @@ __global__ void __launch_bounds__(256) k_rans_sres(MeshView m, RansArgs<S> a, In
     rw = a.log_w ? dlog(Wc.phi / t2) : Wc.phi - t2;
   }
   R.set((int64_t)(D + 2) * m.C + c, rw);
+}
+
+template <int D, class S, int ADVS, bool GEO = false>
+__global__ void __launch_bounds__(256) k_rans_sres(MeshView m, RansArgs<S> a, In<S> mdot, Out<S> R) {
+  __shared__ RansSresSh<D, S> sh;
+  k_rans_sres_body<D, S, ADVS, GEO>(DevExec(m), sh, m, a, mdot, R);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -783,15 +837,17 @@ struct ScalarBarExt {
 // reverse pass R', k / omega part: cotangents of the scalar fluxes, sources and fixation; each
 // face's owner writes the mass-flux cotangent mdot_bar[f] the flow part consumes.  With R set, the
 // k / omega residual rows come out of the same face evaluations.
-template <int D, int ADVS>
-__global__ void __launch_bounds__(256) k_rans_sres_adj(MeshView m, RansArgs<double> a,
-                                                       const double* __restrict__ mdot,
-                                                       const double* __restrict__ lam, RansBars bar,
-                                                       double* __restrict__ mdot_bar, double* __restrict__ R) {
+template <int D>
+struct RansSresAdjSh {
+  ScalarTile<D, double, true> tile;
+};
+
+template <int D, int ADVS, class Exec, class Sh>
+AFX_DEV void k_rans_sres_adj_body(const Exec& ex, Sh& sh, MeshView m, RansArgs<double> a, const double* __restrict__ mdot, const double* __restrict__ lam, RansBars bar, double* __restrict__ mdot_bar, double* __restrict__ R) {
   const SstConst& sc = a.sc;
-  __shared__ ScalarTile<D, double, true> tile;
-  const int32_t base = block_base(m);
-  const int t = threadIdx.x;
+  auto& tile = sh.tile;
+  const int32_t base = ex.base;
+  const int t = ex.t;
   const int32_t c = base + t;
   const bool valid = c < m.C;
   ScalarCell<D, double> Kc, Wc;
@@ -810,7 +866,7 @@ __global__ void __launch_bounds__(256) k_rans_sres_adj(MeshView m, RansArgs<doub
     tile.lam[0][t] = lck;
     tile.lam[1][t] = lcw;
   }
-  __syncthreads();
+  ex.sync();
   if (!valid) return;
   ScalarBarExt<D> B;
   B.zero();
@@ -827,7 +883,7 @@ __global__ void __launch_bounds__(256) k_rans_sres_adj(MeshView m, RansArgs<doub
     if (o >= 0) {
       ScalarCell<D, double> Ko, Wo;
       double xo[D], lk_o, lw_o;
-      const int ls = tile_slot(o, base);
+      const int ls = ex.slot(o);
       if (ls >= 0) {
         tile.get(ls, Ko, Wo, xo);
         lk_o = tile.lam[0][ls];
@@ -971,19 +1027,33 @@ __global__ void __launch_bounds__(256) k_rans_sres_adj(MeshView m, RansArgs<doub
   bar.tau[c] = B.tau;
 }
 
+template <int D, int ADVS>
+__global__ void __launch_bounds__(256) k_rans_sres_adj(MeshView m, RansArgs<double> a,
+                                                       const double* __restrict__ mdot,
+                                                       const double* __restrict__ lam, RansBars bar,
+                                                       double* __restrict__ mdot_bar, double* __restrict__ R) {
+  __shared__ RansSresAdjSh<D> sh;
+  k_rans_sres_adj_body<D, ADVS>(DevExec(m), sh, m, a, mdot, lam, bar, mdot_bar, R);
+}
+
 // reverse pass R', flow part: momentum / continuity flux cotangents plus the scalar part's
 // mass-flux cotangent.  Two resident waves per SIMD (a 256-register cap): the unconstrained 3D
 // build needs ~300 registers and runs one wave per SIMD; the cap spills 24 values and the gather
 // latency hidden by the second wave more than pays for them (270 -> 239 us on the 128x64x64 SST
 // channel).
-template <int D, int ADVF>
-__global__ void __launch_bounds__(256, 2) k_rans_fres_adj(MeshView m, RansArgs<double> a, const double* __restrict__ lam,
-                                                       const double* __restrict__ mdot_bar, RansBars bar) {
+template <int D>
+struct RansFresAdjSh {
+  FlowTile<D, double, true> tile;
+  double tlam[D + 1][kBlock];
+};
+
+template <int D, int ADVF, class Exec, class Sh>
+AFX_DEV void k_rans_fres_adj_body(const Exec& ex, Sh& sh, MeshView m, RansArgs<double> a, const double* __restrict__ lam, const double* __restrict__ mdot_bar, RansBars bar) {
   constexpr int NV = D + 1;
-  __shared__ FlowTile<D, double, true> tile;
-  __shared__ double tlam[NV][kBlock];
-  const int32_t base = block_base(m);
-  const int t = threadIdx.x;
+  auto& tile = sh.tile;
+  auto& tlam = sh.tlam;
+  const int32_t base = ex.base;
+  const int t = ex.t;
   const int32_t c = base + t;
   const bool valid = c < m.C;
   FlowCell<D, double> X;
@@ -997,7 +1067,7 @@ __global__ void __launch_bounds__(256, 2) k_rans_fres_adj(MeshView m, RansArgs<d
 #pragma unroll
     for (int v = 0; v < NV; ++v) tlam[v][t] = lc[v] = lam[(int64_t)v * m.C + c];
   }
-  __syncthreads();
+  ex.sync();
   if (!valid) return;
   FlowBarExt<D> B;
   B.fl.zero();
@@ -1011,7 +1081,7 @@ __global__ void __launch_bounds__(256, 2) k_rans_fres_adj(MeshView m, RansArgs<d
     if (o >= 0) {
       FlowCell<D, double> O;
       double xo[D];
-      const int ls = tile_slot(o, base);
+      const int ls = ex.slot(o);
       if (ls >= 0) {
         tile.get(ls, O, xo);
       } else {
@@ -1071,14 +1141,27 @@ __global__ void __launch_bounds__(256, 2) k_rans_fres_adj(MeshView m, RansArgs<d
   bar.nutw[c] = B.nutw;
 }
 
+template <int D, int ADVF>
+__global__ void __launch_bounds__(256, 2) k_rans_fres_adj(MeshView m, RansArgs<double> a, const double* __restrict__ lam,
+                                                       const double* __restrict__ mdot_bar, RansBars bar) {
+  __shared__ RansFresAdjSh<D> sh;
+  k_rans_fres_adj_body<D, ADVF>(DevExec(m), sh, m, a, lam, mdot_bar, bar);
+}
+
 // reverse pass D'.  Three waves per SIMD (168 registers; the 3D build needs 174 unconstrained,
 // the cap spills two): 152 -> 145 us on the 128x64x64 SST channel.
-template <int D, int ADV>
-__global__ void __launch_bounds__(256, 3) k_rans_ap_adj(MeshView m, RansArgs<double> a, RansBars bar) {
-  __shared__ FlowTile<D, double, false> tile;
-  __shared__ double tapb[kBlock];
-  const int32_t base = block_base(m);
-  const int t = threadIdx.x;
+template <int D>
+struct RansApAdjSh {
+  FlowTile<D, double, false> tile;
+  double tapb[kBlock];
+};
+
+template <int D, int ADV, class Exec, class Sh>
+AFX_DEV void k_rans_ap_adj_body(const Exec& ex, Sh& sh, MeshView m, RansArgs<double> a, RansBars bar) {
+  auto& tile = sh.tile;
+  auto& tapb = sh.tapb;
+  const int32_t base = ex.base;
+  const int t = ex.t;
   const int32_t c = base + t;
   const bool valid = c < m.C;
   FlowCell<D, double> X;
@@ -1091,7 +1174,7 @@ __global__ void __launch_bounds__(256, 3) k_rans_ap_adj(MeshView m, RansArgs<dou
     tile.put(t, X, xc);
     tapb[t] = apb_c = bar.aP[c];
   }
-  __syncthreads();
+  ex.sync();
   if (!valid) return;
   RansCellBar<D> B;
   B.zero();
@@ -1103,7 +1186,7 @@ __global__ void __launch_bounds__(256, 3) k_rans_ap_adj(MeshView m, RansArgs<dou
       FlowCell<D, double> O;
       double xo[D];
       double apb_o;
-      const int ls = tile_slot(o, base);
+      const int ls = ex.slot(o);
       if (ls >= 0) {
         tile.get(ls, O, xo);
         apb_o = tapb[ls];
@@ -1139,15 +1222,18 @@ __global__ void __launch_bounds__(256, 3) k_rans_ap_adj(MeshView m, RansArgs<dou
   bar.nutw[c] += B.nutw;
 }
 
+template <int D, int ADV>
+__global__ void __launch_bounds__(256, 3) k_rans_ap_adj(MeshView m, RansArgs<double> a, RansBars bar) {
+  __shared__ RansApAdjSh<D> sh;
+  k_rans_ap_adj_body<D, ADV>(DevExec(m), sh, m, a, bar);
+}
+
 // reverse pass C': closure (seeded duals), wall shear rate, limiters; then the gradient cotangent
 // is pre-multiplied by the cell's reconstruction operator.
 template <int D, int ADVF, int ADVS, int GRAD>
-__global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<double> a,
-                                                          const int32_t* __restrict__ ext,
-                                                          const int32_t* __restrict__ kmin_pos, RansBars bar) {
+AFX_DEV void k_rans_closure_adj_body(int32_t c, MeshView m, RansArgs<double> a, const int32_t* __restrict__ ext, const int32_t* __restrict__ kmin_pos, RansBars bar) {
   constexpr int NA = D + 3, NL = D + 2;
   const SstConst& sc = a.sc;
-  const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   const bool wall = a.wallc[c] != 0;
   const double d = a.dwall[c];
@@ -1344,18 +1430,32 @@ __global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<d
   }
 }
 
+template <int D, int ADVF, int ADVS, int GRAD>
+__global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<double> a,
+                                                          const int32_t* __restrict__ ext,
+                                                          const int32_t* __restrict__ kmin_pos, RansBars bar) {
+  k_rans_closure_adj_body<D, ADVF, ADVS, GRAD>(xcd_thread_index(m), m, a, ext, kmin_pos, bar);
+}
+
 // reverse pass G'
-template <int D, int GRAD, bool LIMF, bool LIMS>
-__global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<double> a, const int32_t* __restrict__ ext,
-                                                       RansBars bar) {
+template <int D, int NE>
+struct RansGradAdjSh {
+  double tB[(D + 3) * D][kBlock];
+  double tcc[D][kBlock];
+  int32_t tex[2 * (NE > 0 ? NE : 1)][kBlock];
+  double tbe[2 * (NE > 0 ? NE : 1)][kBlock];
+};
+
+template <int D, int GRAD, bool LIMF, bool LIMS, class Exec, class Sh>
+AFX_DEV void k_rans_grad_adj_body(const Exec& ex, Sh& sh, MeshView m, RansArgs<double> a, const int32_t* __restrict__ ext, RansBars bar) {
   constexpr int NA = D + 3, NL = D + 2;
   constexpr int NE = (LIMF ? D : 0) + (LIMS ? 2 : 0);  // limited variables
-  __shared__ double tB[NA * D][kBlock];
-  __shared__ double tcc[D][kBlock];
-  __shared__ int32_t tex[2 * (NE > 0 ? NE : 1)][kBlock];
-  __shared__ double tbe[2 * (NE > 0 ? NE : 1)][kBlock];  // the limiter extremum cotangents
-  const int32_t base = block_base(m);
-  const int t = threadIdx.x;
+  auto& tB = sh.tB;
+  auto& tcc = sh.tcc;
+  auto& tex = sh.tex;
+  auto& tbe = sh.tbe;  // the limiter extremum cotangents
+  const int32_t base = ex.base;
+  const int t = ex.t;
   const int32_t c = base + t;
   const bool valid = c < m.C;
   double Bc[NA][D];
@@ -1379,7 +1479,7 @@ __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<doub
       ++e;
     }
   }
-  __syncthreads();
+  ex.sync();
   if (!valid) return;
   double xb[NA];
 #pragma unroll
@@ -1388,7 +1488,7 @@ __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<doub
     FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
-      const int ls = tile_slot(o, base);
+      const int ls = ex.slot(o);
       // the neighbour's gradient cotangent, centroid and limiter extrema (which cells attain them and
       // their cotangents), all fetched together: no load waits on a comparison
       double Bo[NA][D], cco[D];
@@ -1504,6 +1604,13 @@ __global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<doub
   }
 #pragma unroll
   for (int v = 0; v < NA; ++v) bar.x[(int64_t)v * m.C + c] += xb[v];
+}
+
+template <int D, int GRAD, bool LIMF, bool LIMS>
+__global__ void __launch_bounds__(256) k_rans_grad_adj(MeshView m, RansArgs<double> a, const int32_t* __restrict__ ext,
+                                                       RansBars bar) {
+  __shared__ RansGradAdjSh<D, (LIMF ? D : 0) + (LIMS ? 2 : 0)> sh;
+  k_rans_grad_adj_body<D, GRAD, LIMF, LIMS>(DevExec(m), sh, m, a, ext, bar);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2510,5 +2617,186 @@ extern "C" int afx_rans_fields(const afx_mesh* m, const afx_geometry* g, const a
                      return rans_forward<decltype(Dc)::value, double, decltype(Fc)::value, decltype(Sc)::value,
                                          decltype(Gc)::value>(m, g, a, Out<double>{nullptr, nullptr}, w, nullptr,
                                                               nullptr, nullptr, nullptr, nullptr, nullptr, false, s);
+                   });
+}
+
+// ---------------------------------------------------------------------------------------------
+// The OpenMP CPU line: afx_rans_residual_vjp on host arrays.  The passes are the HIP kernels' own
+// cell bodies, run one cell at a time by OpenMP threads (HostExec: no LDS tile, every neighbour
+// read from memory), in the caller's face order.  A measured baseline, not a fallback: nothing in
+// the device path calls it.
+namespace {
+
+// Cell -> face lists of a host mesh in face order: entry f where the cell owns f, ~f where it is
+// f's neighbour, with the other cell alongside (-1 on a boundary face).
+struct HostCells {
+  std::vector<int32_t> cptr, cface, cnbr;
+};
+
+HostCells host_cell_faces(int64_t C, int64_t F, const int32_t* owner, const int32_t* nb) {
+  HostCells h;
+  h.cptr.assign(C + 1, 0);
+  for (int64_t f = 0; f < F; ++f) {
+    ++h.cptr[owner[f] + 1];
+    if (nb[f] >= 0) ++h.cptr[nb[f] + 1];
+  }
+  for (int64_t c = 0; c < C; ++c) h.cptr[c + 1] += h.cptr[c];
+  h.cface.resize(h.cptr[C]);
+  h.cnbr.resize(h.cptr[C]);
+  std::vector<int32_t> fill(h.cptr.begin(), h.cptr.end() - 1);
+  for (int64_t f = 0; f < F; ++f) {
+    const int32_t P = owner[f], N = nb[f];
+    h.cface[fill[P]] = (int32_t)f;
+    h.cnbr[fill[P]++] = N;
+    if (N >= 0) {
+      h.cface[fill[N]] = ~(int32_t)f;
+      h.cnbr[fill[N]++] = P;
+    }
+  }
+  return h;
+}
+
+// Runs a staged body over every cell; each thread owns one staging block, written and never read.
+template <class Sh, class Fn>
+void host_staged(int64_t C, int threads, Fn&& fn) {
+#pragma omp parallel num_threads(threads)
+  {
+    std::unique_ptr<Sh> sh(new Sh);
+#pragma omp for schedule(static)
+    for (int64_t c = 0; c < C; ++c) fn(HostExec((int32_t)c), *sh);
+  }
+}
+
+template <class Fn>
+void host_cells(int64_t C, int threads, Fn&& fn) {
+#pragma omp parallel for num_threads(threads) schedule(static)
+  for (int64_t c = 0; c < C; ++c) fn((int32_t)c);
+}
+
+// The pass sequence of rans_vjp_impl with R requested.
+template <int D, int AF, int AS, int GR>
+int rans_residual_vjp_host_run(const MeshView& m, RansArgs<double> a, const double* lam, double* R, RansBars bar,
+                               const RWork& w, const afx_rans_config* cfg, int threads) {
+  constexpr bool LF = AF == kAdvLimited, LS = AS == kAdvLimited;
+  const int64_t C = m.C, F = m.F;
+  ClosureOut co{};
+  double** vv[7] = {&co.nut, &co.f1, &co.Sr, &co.cd, &co.nutw, &co.tau, &co.mueff};
+  for (int i = 0; i < 7; ++i) *vv[i] = w.cl + i * C;
+  const Out<double> oG{w.G, nullptr}, oP{w.psi, nullptr};
+  host_staged<RansGradSh<D, double>>(C, threads, [&](const HostExec& ex, RansGradSh<D, double>& sh) {
+    k_rans_grad_body<D, double, GR, LF, LS, true, false>(ex, sh, m, a, oG, oP, w.ext, w.kmin, co);
+  });
+  a.G.v = w.G;
+  a.psi.v = w.psi;
+  bind_closure<double>(a, w.cl, nullptr, C);
+  host_staged<RansApSh<D, double>>(C, threads, [&](const HostExec& ex, RansApSh<D, double>& sh) {
+    k_rans_ap_body<D, double, AF, false>(ex, sh, m, a, Out<double>{w.aP, nullptr});
+  });
+  a.aP.v = w.aP;
+  host_staged<RansFresSh<D>>(C, threads, [&](const HostExec& ex, RansFresSh<D>& sh) {
+    k_rans_fres_body<D, double, AF, false>(ex, sh, m, a, Out<double>{R, nullptr}, Out<double>{w.mdot, nullptr});
+  });
+  host_staged<RansSresAdjSh<D>>(C, threads, [&](const HostExec& ex, RansSresAdjSh<D>& sh) {
+    k_rans_sres_adj_body<D, AS>(ex, sh, m, a, w.mdot, lam, bar, w.mdotb, R);
+  });
+  std::fill(bar.ub, bar.ub + (int64_t)D * F, 0.0);
+  std::fill(bar.pb, bar.pb + F, 0.0);
+  host_staged<RansFresAdjSh<D>>(C, threads, [&](const HostExec& ex, RansFresAdjSh<D>& sh) {
+    k_rans_fres_adj_body<D, AF>(ex, sh, m, a, lam, w.mdotb, bar);
+  });
+  host_staged<RansApAdjSh<D>>(C, threads, [&](const HostExec& ex, RansApAdjSh<D>& sh) {
+    k_rans_ap_adj_body<D, AF>(ex, sh, m, a, bar);
+  });
+  host_cells(C, threads, [&](int32_t c) { k_rans_closure_adj_body<D, AF, AS, GR>(c, m, a, w.ext, w.kmin, bar); });
+  using GASh = RansGradAdjSh<D, (LF ? D : 0) + (LS ? 2 : 0)>;
+  host_staged<GASh>(C, threads, [&](const HostExec& ex, GASh& sh) {
+    k_rans_grad_adj_body<D, GR, LF, LS>(ex, sh, m, a, w.ext, bar);
+  });
+  for (int row = D + 1; row <= D + 2; ++row) {  // log-form rows: wbar = e^w phibar
+    const int tr = row == D + 1 ? cfg->k_transform : cfg->omega_transform;
+    if (tr != AFX_SCALAR_LOG) continue;
+    double* xb = bar.x + (int64_t)row * C;
+    const double* xs = a.x.v + (int64_t)row * C;
+    host_cells(C, threads, [&](int32_t c) { xb[c] *= xs[c]; });
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int afx_rans_residual_vjp_host(int dim, int64_t n_cells, int64_t n_faces, const int32_t* owner,
+                                          const int32_t* neighbour, const afx_geometry* g,
+                                          const afx_rans_config* cfg, const afx_rans_inputs* in,
+                                          const double* cotangent, double* residual, afx_rans_cotangents* out,
+                                          int threads) {
+  AFX_REQUIRE(owner && neighbour && g && cfg && in && cotangent && residual && out,
+              "afx_rans_residual_vjp_host: null argument");
+  AFX_REQUIRE(dim == 2 || dim == 3, "afx_rans_residual_vjp_host: dim must be 2 or 3");
+  AFX_REQUIRE(n_cells > 0 && n_cells < (int64_t)1 << 31 && n_faces > 0 && n_faces < (int64_t)1 << 31,
+              "afx_rans_residual_vjp_host: mesh size out of range");
+  AFX_REQUIRE(g->area && g->normal && g->face_centroid && g->volume && g->cell_centroid,
+              "afx_rans_residual_vjp_host: the geometry needs all five arrays");
+  AFX_REQUIRE(in->state && in->mu && in->rho && in->nu && in->u_b && in->p_b && in->k_value && in->omega_value,
+              "afx_rans_residual_vjp_host: a required input pointer is null");
+  AFX_REQUIRE(cfg->flow.face_kind && cfg->k_kind && cfg->omega_kind && cfg->wall_face && cfg->wall_cell &&
+                  cfg->wall_distance && cfg->wall_distance_gradient,
+              "afx_rans_residual_vjp_host: a required configuration pointer is null");
+  AFX_REQUIRE(out->state && out->mu && out->rho && out->nu && out->u_b && out->p_b,
+              "afx_rans_residual_vjp_host: null cotangent argument");
+  AFX_REQUIRE(cfg->flow.gradient == 0 || cfg->flow.gradient == 1,
+              "afx_rans_residual_vjp_host: compact Green-Gauss or least-squares gradients");
+  AFX_REQUIRE(cfg->density > 0.0, "afx_rans_residual_vjp_host: density must be positive");
+  AFX_REQUIRE(cfg->k_transform >= 0 && cfg->k_transform <= 1 && cfg->omega_transform >= 0 &&
+                  cfg->omega_transform <= 1,
+              "afx_rans_residual_vjp_host: k_transform / omega_transform must be AFX_SCALAR_DIRECT or "
+              "AFX_SCALAR_LOG");
+  if (threads <= 0) threads = omp_get_max_threads();
+  const int D = dim, NA = D + 3;
+  const int64_t C = n_cells, F = n_faces;
+  const HostCells hc = host_cell_faces(C, F, owner, neighbour);
+  MeshView m{};
+  m.C = (int32_t)C;
+  m.F = (int32_t)F;
+  m.owner = owner;
+  m.nb = neighbour;
+  m.cptr = hc.cptr.data();
+  m.cface = hc.cface.data();
+  m.cnbr = hc.cnbr.data();
+  m.area = g->area;
+  m.nrm = g->normal;
+  m.fc = g->face_centroid;
+  m.vol = g->volume;
+  m.cc = g->cell_centroid;
+  m.fu = nullptr;
+  m.bsched = nullptr;
+  m.nbs = 0;
+  // every workspace array is written by a pass before a later one reads it
+  std::unique_ptr<double[]> work(new double[(size_t)rans_work_doubles(D, C, F, 2)]);
+  RWork w = rcarve(work.get(), D, C, F, 2);
+  RansArgs<double> a = rargs<double>(cfg, in, nullptr);
+  if (rans_transformed(cfg)) {  // solved -> physical state
+    const int lk = cfg->k_transform == AFX_SCALAR_LOG ? D + 1 : -1;
+    const int lw = cfg->omega_transform == AFX_SCALAR_LOG ? D + 2 : -1;
+    const double* x = in->state;
+    double* xs = w.xs;
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int64_t i = 0; i < (int64_t)NA * C; ++i) {
+      const int v = (int)(i / C);
+      xs[i] = (v == lk || v == lw) ? exp(x[i]) : x[i];
+    }
+    a.x.v = xs;
+  }
+  RansBars bar{out->state, out->mu,  out->rho, out->nu, out->u_b, out->p_b, w.Gb,  w.aPb, w.psib,
+               w.extb,     w.mueffb, w.nutb,   w.f1b,   w.Srb,    w.cdb,    w.taub, w.nutwb};
+  return rdispatch(D, cfg->flow.advection, cfg->scalar_advection, cfg->flow.gradient,
+                   [&](auto Dc, auto Fc, auto Sc, auto Gc) {
+                     constexpr int GR = decltype(Gc)::value;
+                     if constexpr (GR == kGradCorrected) {
+                       return 2;  // refused above
+                     } else {
+                       return rans_residual_vjp_host_run<decltype(Dc)::value, decltype(Fc)::value,
+                                                         decltype(Sc)::value, GR>(m, a, cotangent, residual, bar, w,
+                                                                                  cfg, threads);
+                     }
                    });
 }

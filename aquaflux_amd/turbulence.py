@@ -475,6 +475,56 @@ class _RansRaw:
         )
         return R, bars
 
+    def host_line(self, primals, cot):
+        """The OpenMP CPU line (afx_rans_residual_vjp_host) on host copies of every array, made once:
+        returns run(threads) -> (R, cotangents) as host tensors, which runs the same cell bodies as
+        the device kernels in the caller's face order (threads <= 0: OpenMP's default)."""
+        t, fraw = self.keep
+        mesh = self.mesh
+        held = []
+
+        def hp(x):
+            h = torch.as_tensor(x).detach().to("cpu").contiguous()
+            held.append(h)
+            return h.data_ptr()
+
+        g = fraw.geometry
+        geo = _lib.AfxGeometry(hp(g.face.area), hp(g.face.normal_soa), hp(g.face.centroid_soa), hp(g.cell.volume),
+                               hp(g.cell.centroid_soa), None)
+        cfg = _lib.AfxRansConfig.from_buffer_copy(self.cfg)
+        cfg.flow.face_kind = hp(fraw.kind)
+        cfg.k_kind = hp(t._k_kind)
+        cfg.omega_kind = hp(t._omega_kind)
+        cfg.wall_face = hp(t._wall_face)
+        cfg.wall_cell = hp(t._wall_cell)
+        cfg.wall_distance = hp(t.wall_distance)
+        cfg.wall_distance_gradient = hp(t.wall_distance_gradient)
+        ins = _lib.AfxRansInputs(*[hp(x) for x in primals], hp(self.k_value), hp(self.omega_value))
+        lam = hp(cot)
+        R = torch.empty(primals[0].shape, dtype=F64)
+        bars = [torch.empty(p.shape, dtype=F64) for p in primals]
+        outs = _lib.AfxRansCotangents(*[b.data_ptr() for b in bars])
+        owner = np.ascontiguousarray(mesh.owner, dtype=np.int32)
+        nb = np.ascontiguousarray(mesh.neighbour, dtype=np.int32)
+        L = _lib.lib()
+
+        def run(threads=0):
+            _lib.check(
+                L.afx_rans_residual_vjp_host(mesh.dim, mesh.n_cells, mesh.n_faces, owner.ctypes.data, nb.ctypes.data,
+                                             ctypes.byref(geo), ctypes.byref(cfg), ctypes.byref(ins), lam,
+                                             R.data_ptr(), ctypes.byref(outs), int(threads)),
+                "afx_rans_residual_vjp_host",
+            )
+            _ = held  # the host copies live as long as run does
+            return R, bars
+
+        return run
+
+    def residual_vjp_host(self, primals, cot, threads=0):
+        """(R, cotangents) of residual_vjp from the OpenMP CPU line, as host tensors."""
+        R, bars = self.host_line(primals, cot)(threads)
+        return R.clone(), [b.clone() for b in bars]
+
     def fields(self, state, mu, rho, nu, ub, pb):
         d, C = self.mesh.dim, self.mesh.n_cells
         n = ((d + 3) * d + (d + 2) + 1 + 7) * C

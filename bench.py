@@ -311,6 +311,28 @@ def cpu_baseline(workload, nx, ny, nz, reps=10, warmups=3, nu=1e-5):
     }
 
 
+def cpu_openmp_line(op, prim, lam, reps=10, warmups=3):
+    """The OpenMP C++ FP64 line: afx_rans_residual_vjp_host runs the device kernels' own cell bodies
+    (gradient + limiter + closure, a_P, flow and k / omega face loops and their adjoints) one cell at
+    a time over the host threads, on host copies of the full workload made before timing; median of
+    ``reps`` runs after ``warmups`` untimed ones."""
+    threads = _cpu_threads()
+    run = op._raw.host_line(prim, lam)
+    tm, times = _time_cpu(lambda: run(threads), warmups, reps)
+    F, C = op._raw.mesh.n_faces, op._raw.mesh.n_cells
+    return {
+        "value": F / tm / 1e9,
+        "unit": "Gfaces/s",
+        "cores": threads,
+        "host_cpus": os.cpu_count(),
+        "kind": "port",
+        "sample": f"residual+VJP of the full workload ({C} cells, {F} faces) by the OpenMP C++ FP64 build of the "
+        f"same cell bodies as the HIP kernels (afx_rans_residual_vjp_host): median of {reps} runs after "
+        f"{warmups} warm-ups, {tm * 1e3:.1f} ms each (min {min(times) * 1e3:.1f}, max {max(times) * 1e3:.1f}), "
+        f"{threads} threads",
+    }
+
+
 TET_BCS = ("left", "right", "bottom", "top", "back", "front")
 
 
@@ -711,6 +733,8 @@ def main(argv=None):
         out.update(newton_lines())
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, *args.cpu_size, reps=args.cpu_reps)
+        if wl == "sst":
+            out["cpu_openmp"] = cpu_openmp_line(op, prim, lam, reps=args.cpu_reps)
     print(json.dumps(out), flush=True)
     return 0
 

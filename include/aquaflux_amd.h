@@ -458,6 +458,17 @@ int afx_rans_residual_vjp(const afx_mesh* mesh, const afx_geometry* geom, const 
                           const afx_rans_inputs* in, const double* cotangent, double* residual,
                           afx_rans_cotangents* out, void* work, int64_t work_bytes, hipStream_t stream);
 
+/* The OpenMP CPU line of afx_rans_residual_vjp: the same passes and per-cell code as the HIP
+ * kernels, run one cell at a time by `threads` host threads (<= 0: OpenMP's default) on HOST arrays
+ * (owner / neighbour, the five geometry arrays and every input and configuration array in the
+ * caller's face order); compact Green-Gauss or least-squares gradients.  No counterpart in the
+ * reference: it is the best-CPU baseline the GPU figures are reported against (the reference's own
+ * CPU path is JAX on XLA:CPU), and nothing in the device path calls it. */
+int afx_rans_residual_vjp_host(int dim, int64_t n_cells, int64_t n_faces, const int32_t* owner,
+                               const int32_t* neighbour, const afx_geometry* geom, const afx_rans_config* cfg,
+                               const afx_rans_inputs* in, const double* cotangent, double* residual,
+                               afx_rans_cotangents* out, int threads);
+
 /* Forward intermediates into `fields` (the mode-0 workspace layout): gradients [(d+3)d][C],
  * limiters [d+2][C], a_P [C], closure [7][C] = nu_t, F1, strain rate, cross-diffusion,
  * wall-function nu_t, wall shear rate, mu_eff.  Replaces SSTTurbulence.closure_fields (reference:

@@ -434,3 +434,43 @@ def test_rans_node_coordinate_adjoint_matches_oracle(case, cuda):
     a = float((dR.cpu() * lam).sum())
     b = float((got.cpu() * dX).sum())
     assert abs(a - b) <= 1e-11 * float((dR.cpu() * lam).abs().sum())
+
+
+# --- the OpenMP CPU line ------------------------------------------------------------------------
+HOST_CASES = [c for c in CASES if not c[3].startswith("cgg")]
+
+
+@pytest.mark.parametrize("case", HOST_CASES, ids=_ids)
+def test_cpu_line_matches_device(case, cuda):
+    """afx_rans_residual_vjp_host (the kernels' cell bodies run by OpenMP threads on host copies, in
+    the caller's face order) against the device path: R to 1e-12, every cotangent to 1e-11, per
+    field block."""
+    mesh, rans, oset, st = _setup(case, cuda, seed=9)
+    rng = np.random.default_rng(13)
+    lam = torch.tensor(rng.normal(size=st.numel()), device=cuda)
+    prim = rans._detached(st.to(cuda))
+    R, bars = rans._raw.residual_vjp(prim, lam)
+    Rh, barsh = rans._raw.residual_vjp_host(prim, lam, threads=4)
+    nb = mesh.dim + 3
+    assert max(RC.block_errors(Rh, R.cpu(), nb)) < TOL_R
+    assert max(RC.block_errors(barsh[0], bars[0].cpu(), nb)) < TOL_D
+    for name, h, d in zip(("mu", "rho", "nu", "u_b", "p_b"), barsh[1:], bars[1:]):
+        d = d.cpu()
+        err = float((h - d).abs().max() / d.abs().max().clamp_min(1e-300))
+        print(name, err)
+        assert err < TOL_D, name
+
+
+def test_cpu_line_log_scalars(cuda):
+    """The CPU line with k and omega solved in log form."""
+    mesh, rans, _ = RC.build("channel2d_sst", "limited", "limited", "lsq", nu=1e-4, device=cuda, k_log=True,
+                             omega_log=True)
+    st = torch.tensor(RC.seeded_state(mesh, 2, "channel"))
+    st = RC.to_solved(st, mesh.dim, True, True)
+    lam = torch.tensor(np.random.default_rng(4).normal(size=st.numel()), device=cuda)
+    prim = rans._detached(st.to(cuda))
+    R, bars = rans._raw.residual_vjp(prim, lam)
+    Rh, barsh = rans._raw.residual_vjp_host(prim, lam, threads=2)
+    nb = mesh.dim + 3
+    assert max(RC.block_errors(Rh, R.cpu(), nb)) < TOL_R
+    assert max(RC.block_errors(barsh[0], bars[0].cpu(), nb)) < TOL_D
