@@ -716,8 +716,9 @@ __global__ void __launch_bounds__(256) k_sc_lim_adj(MeshView m, ScArgs<double> a
   }
 }
 
-// reverse pass G'
-template <int D, int GRAD, bool LIM, bool GEO>
+// reverse pass G'.  CG: the gradient is corrected Green-Gauss (GRAD 0 here, the swept solve having
+// formed the cotangent of b), whose own geometry cotangents are folded in with the face sums.
+template <int D, int GRAD, bool LIM, bool GEO, bool CG = false>
 __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> a, const int32_t* __restrict__ ext,
                                                      ScBars bar) {
   const int32_t c = xcd_thread_index(m);
@@ -780,7 +781,7 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
           qb.A += phf * dn;
           qb.g += pb_ * (phN - phP);
           adj_face_geo<D>(q, true, qb);
-          if (bar.cggf)  // corrected Green-Gauss: the sweeps' own geometry dependence
+          if constexpr (CG)  // the sweeps' own geometry dependence
             cgg_face_geo_bar<D>(CggGeoBars{bar.cggf, bar.cggv}, m.F, r.f, q, qb.A, qb.n, qb.xf, qb.xP, qb.xN);
           have_geo = true;
         }
@@ -897,7 +898,7 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
   if (GEO) {
 #pragma unroll
     for (int j = 0; j < D; ++j) bar.cc[(int64_t)j * m.C + c] += ccb[j];
-    if (bar.cggv) bar.vol[c] += bar.cggv[c];
+    if constexpr (CG) bar.vol[c] += bar.cggv[c];
   }
 }
 
@@ -1734,8 +1735,8 @@ extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const af
                      : cgg_sweeps_adjoint(v, D, 1, bar.G, w.cgg, cfg->gradient_sweeps, cfg->gradient_relaxation, s);
         if (r3) return r3;
       }
-      AFX_LAUNCH(kKScalarGradAdj, (k_sc_grad_adj<D, GK, LIM, GEO>), grid_for(m->C), dim3(kBlock), 0, s, v, b,
-                 w.ext, bar);
+      AFX_LAUNCH(kKScalarGradAdj, (k_sc_grad_adj<D, GK, LIM, GEO, GEO && GR == kGradCorrected>), grid_for(m->C),
+                 dim3(kBlock), 0, s, v, b, w.ext, bar);
       AFX_CHECK_LAUNCH();
       return 0;
     };
