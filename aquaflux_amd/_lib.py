@@ -177,6 +177,7 @@ _SIGNATURES = {
                           c_void_p, ctypes.c_int32, ctypes.c_int32, c_void_p, ctypes.c_int, c_void_p, c_void_p],
     "afx_geometry_pack": [c_void_p, ctypes.POINTER(AfxGeometry), c_void_p, c_void_p],
     "afx_geometry_compute": [c_void_p, c_void_p, ctypes.POINTER(AfxGeometry), c_void_p],
+    "afx_geometry_compute_record": [c_void_p, c_void_p, ctypes.POINTER(AfxGeometry), c_void_p, c_void_p],
     "afx_flow_workspace_bytes": [c_void_p, ctypes.c_int, c_int64_p],
     "afx_flow_residual": [
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxFlowConfig), ctypes.POINTER(AfxFlowInputs),
@@ -265,6 +266,10 @@ _SIGNATURES = {
         c_void_p, c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxGeometryCotangents), c_void_p, c_void_p,
         ctypes.c_int64, c_void_p,
     ],
+    "afx_geometry_vjp_record": [
+        c_void_p, c_void_p, ctypes.POINTER(AfxGeometry), c_void_p, ctypes.POINTER(AfxGeometryCotangents), c_void_p,
+        c_void_p, ctypes.c_int64, c_void_p,
+    ],
     "afx_scalar_workspace_bytes": [c_void_p, ctypes.c_int, c_int64_p],
     "afx_scalar_residual": [
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxScalarConfig), ctypes.POINTER(AfxScalarInputs),
@@ -292,6 +297,15 @@ _SIGNATURES = {
     "afx_scalar_vjp": [
         c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxScalarConfig), ctypes.POINTER(AfxScalarInputs),
         c_void_p, ctypes.POINTER(AfxScalarCotangents), c_void_p, ctypes.c_int64, c_void_p,
+    ],
+    "afx_scalar_record_bytes": [c_void_p, c_int64_p],
+    "afx_scalar_residual_record": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxScalarConfig), ctypes.POINTER(AfxScalarInputs),
+        c_void_p, c_void_p, ctypes.c_int64, c_void_p,
+    ],
+    "afx_scalar_vjp_record": [
+        c_void_p, ctypes.POINTER(AfxGeometry), ctypes.POINTER(AfxScalarConfig), ctypes.POINTER(AfxScalarInputs),
+        c_void_p, c_void_p, ctypes.c_int64, ctypes.POINTER(AfxScalarCotangents), c_void_p, ctypes.c_int64, c_void_p,
     ],
     "afx_rans_workspace_bytes": [c_void_p, ctypes.c_int, c_int64_p],
     "afx_rans_residual": [

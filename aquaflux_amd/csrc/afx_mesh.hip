@@ -367,17 +367,39 @@ __global__ void __launch_bounds__(256) k_cell_volume(MeshView m, const double* _
 
 // Cell pass: cotangents of the divergence-theorem sums (csum [d], S = d V) and of the approximate
 // centroid, per cell: cellbar[(j)*C + c] for csum_j, [d*C + c] for S, [(d+1+j)*C + c] for approx_j.
-template <int D>
-__global__ void __launch_bounds__(256) k_geom_cell_adj(MeshView m, const double* __restrict__ approx, const double* __restrict__ vol_b,
+// FORM: the approximate centroid is re-formed here from the face centroids (the same sum in the
+// same order as k_approx_centroid, so the same value) and written to `approx` for the face pass;
+// otherwise `approx` holds the forward pass's.
+template <int D, bool FORM>
+__global__ void __launch_bounds__(256) k_geom_cell_adj(MeshView m, double* __restrict__ approx, const double* __restrict__ vol_b,
                                 const double* __restrict__ cc_b, double* __restrict__ cellbar) {
   int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   const double V = m.vol[c];
   double ap[D], csb[D], apb[D];
   double Vb = vol_b ? vol_b[c] : 0.0;
+  if (!FORM) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) ap[j] = approx[(int64_t)j * m.C + c];
+  } else {
+#pragma unroll
+    for (int j = 0; j < D; ++j) ap[j] = 0.0;
+    const int32_t b = m.cptr[c], e = m.cptr[c + 1];
+    for (int32_t k = b; k < e; ++k) {
+      const int32_t ent = m.cface[k];
+      const int32_t f = ent >= 0 ? ent : ~ent;
+#pragma unroll
+      for (int j = 0; j < D; ++j) ap[j] += m.fc[(int64_t)j * m.F + f];
+    }
+    const double inv = 1.0 / (double)(e - b);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      ap[j] = ap[j] * inv;
+      approx[(int64_t)j * m.C + c] = ap[j];
+    }
+  }
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    ap[j] = approx[(int64_t)j * m.C + c];
     const double ccb = cc_b ? cc_b[(int64_t)j * m.C + c] : 0.0;
     csb[j] = ccb / ((double)(D + 1) * V);
     Vb -= ccb * (m.cc[(int64_t)j * m.C + c] - ap[j]) / V;
@@ -882,8 +904,11 @@ extern "C" int afx_mesh_set_block_order(afx_mesh* m, const int32_t* order, int64
   return 0;
 }
 
-extern "C" int afx_geometry_compute(const afx_mesh* m, const double* node_coords, afx_geometry* g,
-                                    hipStream_t s) {
+namespace {
+
+// approx: where the approximate cell centroids go (the mesh's own buffer, or a caller's record).
+int geometry_compute(const afx_mesh* m, const double* node_coords, afx_geometry* g, double* approx,
+                     hipStream_t s) {
   AFX_REQUIRE(m && node_coords && g, "afx_geometry_compute: null argument");
   AFX_REQUIRE(g->area && g->normal && g->face_centroid && g->volume && g->cell_centroid,
               "afx_geometry_compute: geometry output pointer missing");
@@ -897,25 +922,39 @@ extern "C" int afx_geometry_compute(const afx_mesh* m, const double* node_coords
     AFX_LAUNCH(kKGeomFace, k_face_geometry<2>, grid_for(m->F), dim3(kBlock), 0, s, m->F, m->N, m->foff,
                        m->fidx, node_coords, area, m->normal_raw, fc);
     AFX_CHECK_LAUNCH();
-    AFX_LAUNCH(kKGeomApprox, k_approx_centroid<2>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx);
+    AFX_LAUNCH(kKGeomApprox, k_approx_centroid<2>, grid_for(m->C), dim3(kBlock), 0, s, v, approx);
     AFX_CHECK_LAUNCH();
-    AFX_LAUNCH(kKGeomOrient, k_orient<2>, grid_for(m->F), dim3(kBlock), 0, s, v, m->approx, m->normal_raw, nrm);
+    AFX_LAUNCH(kKGeomOrient, k_orient<2>, grid_for(m->F), dim3(kBlock), 0, s, v, approx, m->normal_raw, nrm);
     AFX_CHECK_LAUNCH();
-    AFX_LAUNCH(kKGeomVolume, k_cell_volume<2>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx, vol, cc);
+    AFX_LAUNCH(kKGeomVolume, k_cell_volume<2>, grid_for(m->C), dim3(kBlock), 0, s, v, approx, vol, cc);
     AFX_CHECK_LAUNCH();
   } else {
     AFX_LAUNCH(kKGeomFace, k_face_geometry<3>, grid_for(m->F), dim3(kBlock), 0, s, m->F, m->N, m->foff,
                        m->fidx, node_coords, area, m->normal_raw, fc);
     AFX_CHECK_LAUNCH();
-    AFX_LAUNCH(kKGeomApprox, k_approx_centroid<3>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx);
+    AFX_LAUNCH(kKGeomApprox, k_approx_centroid<3>, grid_for(m->C), dim3(kBlock), 0, s, v, approx);
     AFX_CHECK_LAUNCH();
-    AFX_LAUNCH(kKGeomOrient, k_orient<3>, grid_for(m->F), dim3(kBlock), 0, s, v, m->approx, m->normal_raw, nrm);
+    AFX_LAUNCH(kKGeomOrient, k_orient<3>, grid_for(m->F), dim3(kBlock), 0, s, v, approx, m->normal_raw, nrm);
     AFX_CHECK_LAUNCH();
-    AFX_LAUNCH(kKGeomVolume, k_cell_volume<3>, grid_for(m->C), dim3(kBlock), 0, s, v, m->approx, vol, cc);
+    AFX_LAUNCH(kKGeomVolume, k_cell_volume<3>, grid_for(m->C), dim3(kBlock), 0, s, v, approx, vol, cc);
     AFX_CHECK_LAUNCH();
   }
   if (g->face_pack && m->fu) return pack_faces(m, g, const_cast<double*>(g->face_pack), s);
   return 0;
+}
+
+}  // namespace
+
+extern "C" int afx_geometry_compute(const afx_mesh* m, const double* node_coords, afx_geometry* g,
+                                    hipStream_t s) {
+  AFX_REQUIRE(m, "afx_geometry_compute: null argument");
+  return geometry_compute(m, node_coords, g, m->approx, s);
+}
+
+extern "C" int afx_geometry_compute_record(const afx_mesh* m, const double* node_coords, afx_geometry* g,
+                                           double* approx, hipStream_t s) {
+  AFX_REQUIRE(m && approx, "afx_geometry_compute_record: null argument");
+  return geometry_compute(m, node_coords, g, approx, s);
 }
 
 extern "C" int afx_geometry_vjp_workspace_bytes(const afx_mesh* m, int64_t* bytes) {
@@ -925,9 +964,11 @@ extern "C" int afx_geometry_vjp_workspace_bytes(const afx_mesh* m, int64_t* byte
   return 0;
 }
 
-extern "C" int afx_geometry_vjp(const afx_mesh* m, const double* node_coords, const afx_geometry* g,
-                                const afx_geometry_cotangents* gb, double* node_b, void* work, int64_t work_bytes,
-                                hipStream_t s) {
+namespace {
+
+// record: null (the approximate centroids are re-formed into the workspace) or the forward's.
+int geometry_vjp(const afx_mesh* m, const double* node_coords, const afx_geometry* g, const double* record,
+                 const afx_geometry_cotangents* gb, double* node_b, void* work, int64_t work_bytes, hipStream_t s) {
   AFX_REQUIRE(m && node_coords && g && gb && node_b && work, "afx_geometry_vjp: null argument");
   AFX_REQUIRE(g->area && g->normal && g->face_centroid && g->volume && g->cell_centroid,
               "afx_geometry_vjp: geometry pointer missing");
@@ -936,15 +977,17 @@ extern "C" int afx_geometry_vjp(const afx_mesh* m, const double* node_coords, co
   AFX_REQUIRE(work_bytes >= need - 256, "afx_geometry_vjp: workspace too small");
   AFX_REQUIRE(m->C > 0 && m->F > 0 && m->N > 0, "afx_geometry_vjp: empty mesh");
   const int D = m->dim;
-  double* approx = (double*)work;
-  double* cellbar = approx + (int64_t)D * m->C;
+  double* approx = record ? const_cast<double*>(record) : (double*)work;
+  double* cellbar = (double*)work + (int64_t)D * m->C;
   double* inc_b = cellbar + (int64_t)(2 * D + 1) * m->C;
   afx::MeshView v = afx_view(m, g);
   if (D == 2) {
-    AFX_LAUNCH(kKGeomApprox, k_approx_centroid<2>, grid_for(m->C), dim3(kBlock), 0, s, v, approx);
-    AFX_CHECK_LAUNCH();
-    AFX_LAUNCH(kKGeomAdj, k_geom_cell_adj<2>, grid_for(m->C), dim3(kBlock), 0, s, v, approx, gb->volume,
-               gb->cell_centroid, cellbar);
+    if (record)
+      AFX_LAUNCH(kKGeomAdj, (k_geom_cell_adj<2, false>), grid_for(m->C), dim3(kBlock), 0, s, v, approx, gb->volume,
+                 gb->cell_centroid, cellbar);
+    else
+      AFX_LAUNCH(kKGeomAdj, (k_geom_cell_adj<2, true>), grid_for(m->C), dim3(kBlock), 0, s, v, approx, gb->volume,
+                 gb->cell_centroid, cellbar);
     AFX_CHECK_LAUNCH();
     AFX_LAUNCH(kKGeomAdjFace, k_geom_face_adj<2>, grid_for(m->F), dim3(kBlock), 0, s, v, m->N, m->foff, m->fidx,
                m->kpos, node_coords, approx, cellbar, gb->area, gb->normal, gb->face_centroid, inc_b);
@@ -953,10 +996,12 @@ extern "C" int afx_geometry_vjp(const afx_mesh* m, const double* node_coords, co
                node_b);
     AFX_CHECK_LAUNCH();
   } else {
-    AFX_LAUNCH(kKGeomApprox, k_approx_centroid<3>, grid_for(m->C), dim3(kBlock), 0, s, v, approx);
-    AFX_CHECK_LAUNCH();
-    AFX_LAUNCH(kKGeomAdj, k_geom_cell_adj<3>, grid_for(m->C), dim3(kBlock), 0, s, v, approx, gb->volume,
-               gb->cell_centroid, cellbar);
+    if (record)
+      AFX_LAUNCH(kKGeomAdj, (k_geom_cell_adj<3, false>), grid_for(m->C), dim3(kBlock), 0, s, v, approx, gb->volume,
+                 gb->cell_centroid, cellbar);
+    else
+      AFX_LAUNCH(kKGeomAdj, (k_geom_cell_adj<3, true>), grid_for(m->C), dim3(kBlock), 0, s, v, approx, gb->volume,
+                 gb->cell_centroid, cellbar);
     AFX_CHECK_LAUNCH();
     AFX_LAUNCH(kKGeomAdjFace, k_geom_face_adj<3>, grid_for(m->F), dim3(kBlock), 0, s, v, m->N, m->foff, m->fidx,
                m->kpos, node_coords, approx, cellbar, gb->area, gb->normal, gb->face_centroid, inc_b);
@@ -966,6 +1011,21 @@ extern "C" int afx_geometry_vjp(const afx_mesh* m, const double* node_coords, co
     AFX_CHECK_LAUNCH();
   }
   return 0;
+}
+
+}  // namespace
+
+extern "C" int afx_geometry_vjp(const afx_mesh* m, const double* node_coords, const afx_geometry* g,
+                                const afx_geometry_cotangents* gb, double* node_b, void* work, int64_t work_bytes,
+                                hipStream_t s) {
+  return geometry_vjp(m, node_coords, g, nullptr, gb, node_b, work, work_bytes, s);
+}
+
+extern "C" int afx_geometry_vjp_record(const afx_mesh* m, const double* node_coords, const afx_geometry* g,
+                                       const double* approx, const afx_geometry_cotangents* gb, double* node_b,
+                                       void* work, int64_t work_bytes, hipStream_t s) {
+  AFX_REQUIRE(approx, "afx_geometry_vjp_record: null record");
+  return geometry_vjp(m, node_coords, g, approx, gb, node_b, work, work_bytes, s);
 }
 
 // ---- face mass flux (u . n) A ------------------------------------------------------------------

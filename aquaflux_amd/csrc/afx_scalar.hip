@@ -1352,7 +1352,8 @@ int sc_check(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* 
   AFX_REQUIRE(cfg->transient >= 0 && cfg->transient <= 2, "scalar: transient must be 0 (steady), 1 or 2 (BDF order)");
   AFX_REQUIRE(cfg->transient == 0 || (cfg->dt > 0.0 && in->phi_old), "scalar: the transient term needs dt > 0 and phi_old");
   AFX_REQUIRE(cfg->transient != 2 || in->phi_older, "scalar: BDF2 needs phi_older");
-  AFX_REQUIRE(work && work_size >= sc_work_doubles(m->dim, m->C, mode) * 8, "scalar: workspace too small");
+  AFX_REQUIRE(mode < 0 || (work && work_size >= sc_work_doubles(m->dim, m->C, mode) * 8),
+              "scalar: workspace too small");
   return 0;
 }
 
@@ -1678,9 +1679,80 @@ extern "C" int afx_scalar_jvp_geometry(const afx_mesh* m, const afx_geometry* g,
   });
 }
 
+// ---------------------------------------------------------------------------------------------
+// Forward record: the gradient pass's outputs (gradient, limiter, the limiter's extremum cells) kept
+// from the residual evaluation, so the reverse pass at the same inputs does not run it again.
+namespace {
+
+int64_t sc_record_doubles(int D, int64_t C) { return (int64_t)(D + 2) * C; }
+
+ScWork sc_record_view(const void* record, int D, int64_t C) {
+  ScWork w{};
+  double* p = (double*)const_cast<void*>(record);
+  w.G = p;
+  w.psi = p + (int64_t)D * C;
+  w.ext = (int32_t*)(w.psi + C);
+  return w;
+}
+
+int sc_vjp(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cfg, const afx_scalar_inputs* in,
+           const double* cotangent, const void* record, afx_scalar_cotangents* out, void* work, int64_t work_size,
+           hipStream_t s);
+
+}  // namespace
+
+extern "C" int afx_scalar_record_bytes(const afx_mesh* m, int64_t* bytes) {
+  AFX_REQUIRE(m && bytes, "afx_scalar_record_bytes: null argument");
+  *bytes = sc_record_doubles(m->dim, m->C) * 8;
+  return 0;
+}
+
+extern "C" int afx_scalar_residual_record(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cfg,
+                                          const afx_scalar_inputs* in, double* residual, void* record,
+                                          int64_t record_size, hipStream_t s) {
+  AFX_REQUIRE(m && g && cfg && in, "afx_scalar_residual_record: null argument");
+  AFX_REQUIRE(cfg->gradient != 3,
+              "afx_scalar_residual_record: the corrected Green-Gauss reverse pass needs its sweep iterates; "
+              "use afx_scalar_residual and afx_scalar_vjp");
+  AFX_REQUIRE(residual, "afx_scalar_residual_record: null residual");
+  AFX_REQUIRE(record && record_size >= sc_record_doubles(m->dim, m->C) * 8,
+              "afx_scalar_residual_record: record too small");
+  // no workspace: the record holds the gradient pass's arrays, the only ones a forward pass without
+  // the corrected scheme writes
+  int rc = sc_check(m, g, cfg, in, nullptr, 0, -1);
+  if (rc) return rc;
+  ScWork w = sc_record_view(record, m->dim, m->C);
+  ScArgs<double> a = sc_args<double>(cfg, in, nullptr);
+  return sc_dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
+    return sc_forward<decltype(Dc)::value, double, decltype(Ac)::value, decltype(Gc)::value>(
+        m, g, a, Out<double>{residual, nullptr}, w, nullptr, nullptr, w.ext, cfg->gradient_sweeps,
+        cfg->gradient_relaxation, s);
+  });
+}
+
+extern "C" int afx_scalar_vjp_record(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cfg,
+                                     const afx_scalar_inputs* in, const double* cotangent, const void* record,
+                                     int64_t record_size, afx_scalar_cotangents* out, void* work,
+                                     int64_t work_size, hipStream_t s) {
+  AFX_REQUIRE(m && cfg, "afx_scalar_vjp_record: null argument");
+  AFX_REQUIRE(cfg->gradient != 3, "afx_scalar_vjp_record: the corrected Green-Gauss scheme has no record");
+  AFX_REQUIRE(record && record_size >= sc_record_doubles(m->dim, m->C) * 8,
+              "afx_scalar_vjp_record: record too small");
+  return sc_vjp(m, g, cfg, in, cotangent, record, out, work, work_size, s);
+}
+
 extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cfg,
                               const afx_scalar_inputs* in, const double* cotangent, afx_scalar_cotangents* out,
                               void* work, int64_t work_size, hipStream_t s) {
+  return sc_vjp(m, g, cfg, in, cotangent, nullptr, out, work, work_size, s);
+}
+
+namespace {
+
+// record: null (the gradient pass runs here) or what afx_scalar_residual_record kept at these inputs.
+int sc_vjp(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cfg, const afx_scalar_inputs* in,
+           const double* cotangent, const void* record, afx_scalar_cotangents* out, void* work, int64_t work_size,
+           hipStream_t s) {
   int rc = sc_check(m, g, cfg, in, work, work_size, 2);
   if (rc) return rc;
   AFX_REQUIRE(cotangent && out && out->phi && out->gamma && out->flux && out->value && out->t_inf,
@@ -1690,6 +1762,12 @@ extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const af
   AFX_REQUIRE(!geo || (gb.area && gb.normal && gb.face_centroid && gb.volume && gb.cell_centroid),
               "afx_scalar_vjp: geometry cotangents need all five arrays");
   ScWork w = sc_carve(work, m->dim, m->C, 2);
+  if (record) {
+    const ScWork r = sc_record_view(record, m->dim, m->C);
+    w.G = r.G;
+    w.psi = r.psi;
+    w.ext = r.ext;
+  }
   ScArgs<double> a = sc_args<double>(cfg, in, nullptr);
   AFX_REQUIRE(cfg->transient == 0 || out->phi_old, "afx_scalar_vjp: the transient term needs a phi_old cotangent");
   AFX_REQUIRE(cfg->transient != 2 || out->phi_older, "afx_scalar_vjp: BDF2 needs a phi_older cotangent");
@@ -1712,9 +1790,11 @@ extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const af
       bar.cggf = cb.face;
       bar.cggv = cb.vol;
     }
-    int r2 = sc_forward<D, double, ADV, GR>(m, g, a, Out<double>{nullptr, nullptr}, w, nullptr, nullptr, w.ext,
-                                            cfg->gradient_sweeps, cfg->gradient_relaxation, s, iters);
-    if (r2) return r2;
+    if (!record) {
+      int r2 = sc_forward<D, double, ADV, GR>(m, g, a, Out<double>{nullptr, nullptr}, w, nullptr, nullptr, w.ext,
+                                              cfg->gradient_sweeps, cfg->gradient_relaxation, s, iters);
+      if (r2) return r2;
+    }
     MeshView v;
     if (int rv_ = afx_rview(m, g, s, &v)) return rv_;
     ScArgs<double> b = a;
@@ -1743,3 +1823,5 @@ extern "C" int afx_scalar_vjp(const afx_mesh* m, const afx_geometry* g, const af
     return geo ? run(std::true_type{}) : run(std::false_type{});
   });
 }
+
+}  // namespace

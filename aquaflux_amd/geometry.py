@@ -105,10 +105,19 @@ class _GeometryFn(torch.autograd.Function):
         d, F, C = dm.dim, dm.n_faces, dm.n_cells
         area, nrm, fc, vol, cc = _empty_geometry(d, F, C, X.device)
         gs = _lib.AfxGeometry(area.data_ptr(), nrm.data_ptr(), fc.data_ptr(), vol.data_ptr(), cc.data_ptr())
-        _lib.check(
-            _lib.lib().afx_geometry_compute(dm.handle, X.data_ptr(), ctypes.byref(gs), _lib.stream_of(X.device)),
-            "afx_geometry_compute",
-        )
+        ctx.record = None
+        if ctx.needs_input_grad[1]:  # keep the approximate centroids for the reverse pass
+            ctx.record = torch.empty((d, C), dtype=torch.float64, device=X.device)
+            _lib.check(
+                _lib.lib().afx_geometry_compute_record(dm.handle, X.data_ptr(), ctypes.byref(gs),
+                                                       ctx.record.data_ptr(), _lib.stream_of(X.device)),
+                "afx_geometry_compute_record",
+            )
+        else:
+            _lib.check(
+                _lib.lib().afx_geometry_compute(dm.handle, X.data_ptr(), ctypes.byref(gs), _lib.stream_of(X.device)),
+                "afx_geometry_compute",
+            )
         ctx.dm = dm
         ctx.save_for_backward(X, area, nrm, fc, vol, cc)
         ctx.save_for_forward(X)
@@ -129,11 +138,14 @@ class _GeometryFn(torch.autograd.Function):
         _lib.check(L.afx_geometry_vjp_workspace_bytes(dm.handle, ctypes.byref(nbytes)), "afx_geometry_vjp_workspace_bytes")
         work = dm.workspace("geom_vjp", int(nbytes.value))
         out = torch.empty_like(X)
-        _lib.check(
-            L.afx_geometry_vjp(dm.handle, X.data_ptr(), ctypes.byref(gs), ctypes.byref(gb), out.data_ptr(),
-                               work.data_ptr(), work.numel(), _lib.stream_of(X.device)),
-            "afx_geometry_vjp",
-        )
+        if ctx.record is None:
+            rc = L.afx_geometry_vjp(dm.handle, X.data_ptr(), ctypes.byref(gs), ctypes.byref(gb), out.data_ptr(),
+                                    work.data_ptr(), work.numel(), _lib.stream_of(X.device))
+        else:
+            rc = L.afx_geometry_vjp_record(dm.handle, X.data_ptr(), ctypes.byref(gs), ctx.record.data_ptr(),
+                                           ctypes.byref(gb), out.data_ptr(), work.data_ptr(), work.numel(),
+                                           _lib.stream_of(X.device))
+        _lib.check(rc, "afx_geometry_vjp" if ctx.record is None else "afx_geometry_vjp_record")
         return None, out
 
     @staticmethod

@@ -175,6 +175,27 @@ class _ScalarRaw:
         )
         return R
 
+    @property
+    def records(self) -> bool:
+        """Whether the residual can keep a forward record for its VJP (not for corrected Green-Gauss)."""
+        return self.cfg.gradient != 3
+
+    def residual_record(self, prim, geo, step=None):
+        """(R, record): the residual and the gradient pass's outputs that vjp(record=...) reuses."""
+        R = torch.empty_like(prim[0])
+        n = ctypes.c_int64()
+        _lib.check(_lib.lib().afx_scalar_record_bytes(self.dm.handle, ctypes.byref(n)), "afx_scalar_record_bytes")
+        rec = torch.empty(int(n.value) // 8, dtype=F64, device=self.dm.device)
+        gs = self._gs(geo)
+        ins = _lib.AfxScalarInputs(*[_lib.ptr(t) for t in prim])
+        _lib.check(
+            _lib.lib().afx_scalar_residual_record(self.dm.handle, ctypes.byref(gs), ctypes.byref(self._cfg(step)),
+                                                  ctypes.byref(ins), R.data_ptr(), rec.data_ptr(), rec.numel() * 8,
+                                                  _lib.stream_of(self.dm.device)),
+            "afx_scalar_residual_record",
+        )
+        return R, rec
+
     def jvp(self, prim, tans, geo, step=None):
         R = torch.empty_like(prim[0])
         dR = torch.empty_like(prim[0])
@@ -215,7 +236,9 @@ class _ScalarRaw:
         )
         return R, dR
 
-    def vjp(self, prim, cot, geo, with_geometry, step=None):
+    def vjp(self, prim, cot, geo, with_geometry, step=None, record=None):
+        """Cotangents of the inputs (and of the geometry); ``record`` is what residual_record kept at
+        these inputs (the gradient pass is then not run again)."""
         bars = [None if t is None else torch.empty_like(t) for t in prim]
         gbars = [torch.empty_like(t) for t in geo] if with_geometry else [None] * 5
         w = self.dm.workspace("scalar2", self.work_bytes[2])
@@ -224,12 +247,15 @@ class _ScalarRaw:
         outs = _lib.AfxScalarCotangents(*[_lib.ptr(b) for b in bars[:5]],
                                         _lib.AfxGeometryCotangents(*[_lib.ptr(b) for b in gbars]),
                                         *[_lib.ptr(b) for b in bars[5:]])
-        _lib.check(
-            _lib.lib().afx_scalar_vjp(self.dm.handle, ctypes.byref(gs), ctypes.byref(self._cfg(step)),
-                                      ctypes.byref(ins), cot.data_ptr(), ctypes.byref(outs), w.data_ptr(), w.numel(),
-                                      _lib.stream_of(self.dm.device)),
-            "afx_scalar_vjp",
-        )
+        L, cfg, st = _lib.lib(), ctypes.byref(self._cfg(step)), _lib.stream_of(self.dm.device)
+        if record is None:
+            rc = L.afx_scalar_vjp(self.dm.handle, ctypes.byref(gs), cfg, ctypes.byref(ins), cot.data_ptr(),
+                                  ctypes.byref(outs), w.data_ptr(), w.numel(), st)
+        else:
+            rc = L.afx_scalar_vjp_record(self.dm.handle, ctypes.byref(gs), cfg, ctypes.byref(ins), cot.data_ptr(),
+                                         record.data_ptr(), record.numel() * 8, ctypes.byref(outs), w.data_ptr(),
+                                         w.numel(), st)
+        _lib.check(rc, "afx_scalar_vjp" if record is None else "afx_scalar_vjp_record")
         return bars, gbars
 
 
@@ -247,6 +273,10 @@ class _ScalarResidualFn(torch.autograd.Function):
         saved = [t for t in prim if t is not None] + list(geo)
         ctx.save_for_backward(*saved)
         ctx.save_for_forward(*saved)
+        ctx.record = None
+        if any(ctx.needs_input_grad) and raw.records:  # keep the gradient pass for the reverse pass
+            R, ctx.record = raw.residual_record(prim, geo, step)
+            return R
         return raw.residual(prim, geo, step)
 
     @staticmethod
@@ -262,7 +292,7 @@ class _ScalarResidualFn(torch.autograd.Function):
     def backward(ctx, gR):
         prim, geo = _ScalarResidualFn._unpack(ctx)
         with_geo = any(ctx.needs_input_grad[10:15])
-        bars, gbars = ctx.raw.vjp(prim, gR.contiguous(), geo, with_geo, ctx.step)
+        bars, gbars = ctx.raw.vjp(prim, gR.contiguous(), geo, with_geo, ctx.step, ctx.record)
         return (None, None, None, *bars, *gbars)
 
     @staticmethod

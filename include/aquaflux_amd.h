@@ -129,6 +129,18 @@ int afx_geometry_vjp(const afx_mesh* mesh, const double* node_coords, const afx_
                      const afx_geometry_cotangents* geom_cotangents, double* node_b, void* work,
                      int64_t work_bytes, hipStream_t stream);
 
+/* afx_geometry_compute / afx_geometry_vjp split at a forward record, as jax.vjp through
+ * Mesh.geometry keeps its linearisation point (reference: aquaflux/mesh/mesh.py:341-387): the
+ * forward also writes the approximate cell centroids approx[d][C] (the mean of each cell's face
+ * centroids, aquaflux/mesh/cell.py:84-112) into the caller's buffer, and the reverse pass reads them
+ * instead of forming them again.  afx_geometry_vjp_record returns what afx_geometry_vjp returns for
+ * the same nodes and geometry; workspace as afx_geometry_vjp. */
+int afx_geometry_compute_record(const afx_mesh* mesh, const double* node_coords, afx_geometry* out, double* approx,
+                                hipStream_t stream);
+int afx_geometry_vjp_record(const afx_mesh* mesh, const double* node_coords, const afx_geometry* geom,
+                            const double* approx, const afx_geometry_cotangents* geom_cotangents, double* node_b,
+                            void* work, int64_t work_bytes, hipStream_t stream);
+
 /* ------------------------------------------------------------------------------------------- */
 /* coupled (u, v[, w], p) flow residual                                                          */
 
@@ -355,6 +367,23 @@ int afx_scalar_jvp_geometry(const afx_mesh* mesh, const afx_geometry* geom, cons
 int afx_scalar_vjp(const afx_mesh* mesh, const afx_geometry* geom, const afx_scalar_config* cfg,
                    const afx_scalar_inputs* in, const double* cotangent, afx_scalar_cotangents* out, void* work,
                    int64_t work_bytes, hipStream_t stream);
+
+/* The residual and its VJP split at a forward record, as jax.vjp(f, ...) splits into the primal and
+ * a pullback that holds the linearisation point (reference call sites: jax.vjp of
+ * ScalarTransport.residual(flux)(phi), aquaflux/transport/scalar.py:185-223).  The record keeps the
+ * gradient pass's outputs (gradient, limiter value, the limiter's extremum cells:
+ * afx_scalar_record_bytes bytes), so afx_scalar_vjp_record does not run that pass again.  The
+ * record is valid for the inputs and geometry of the call that wrote it; afx_scalar_vjp_record
+ * returns what afx_scalar_vjp returns at those inputs.  Not for the corrected Green-Gauss scheme
+ * (status 2: its reverse pass needs the sweep iterates; use afx_scalar_vjp). */
+int afx_scalar_record_bytes(const afx_mesh* mesh, int64_t* bytes);
+int afx_scalar_residual_record(const afx_mesh* mesh, const afx_geometry* geom, const afx_scalar_config* cfg,
+                               const afx_scalar_inputs* in, double* residual, void* record, int64_t record_bytes,
+                               hipStream_t stream);
+int afx_scalar_vjp_record(const afx_mesh* mesh, const afx_geometry* geom, const afx_scalar_config* cfg,
+                          const afx_scalar_inputs* in, const double* cotangent, const void* record,
+                          int64_t record_bytes, afx_scalar_cotangents* out, void* work, int64_t work_bytes,
+                          hipStream_t stream);
 
 /* ------------------------------------------------------------------------------------------- */
 /* coupled RANS (u, v[, w], p, k, omega) residual with the k-omega SST closure                   */

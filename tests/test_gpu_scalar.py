@@ -414,3 +414,83 @@ def test_scalar_transient_node_adjoint_matches_oracle(first, cuda):
     err = rel(got, ref)
     print("transient node adjoint err", err)
     assert err < TOL_D
+
+
+RECORD_CASES = [c for c in CASES if c[2] != "cgg"]
+
+
+@pytest.mark.parametrize("with_geometry", [False, True], ids=["inputs", "geometry"])
+@pytest.mark.parametrize("case", RECORD_CASES, ids=_ids)
+def test_vjp_from_forward_record_is_bitwise_the_recomputing_vjp(case, with_geometry, cuda):
+    """afx_scalar_residual_record + afx_scalar_vjp_record (the reverse pass reusing the forward's
+    gradient pass) give bit for bit what afx_scalar_residual and afx_scalar_vjp give."""
+    mesh, u, bcs, phi, gamma, _, geom, st = _setup(case, cuda, seed=4)
+    lam = torch.tensor(np.random.default_rng(6).normal(size=mesh.n_cells), device=cuda)
+    raw = st._raw
+    prim = st._prim(SC.gpu_flux(geom, u), torch.tensor(phi, device=cuda), None, None)
+    geo = tuple(t.detach() for t in st._geometry())
+    R0 = raw.residual(prim, geo)
+    R1, rec = raw.residual_record(prim, geo)
+    assert torch.equal(R0, R1)
+    b0, g0 = raw.vjp(prim, lam, geo, with_geometry)
+    b1, g1 = raw.vjp(prim, lam, geo, with_geometry, record=rec)
+    for x, y in zip(b0 + g0, b1 + g1):
+        assert (x is None) == (y is None)
+        if x is not None:
+            assert torch.equal(x, y)
+
+
+def test_vjp_record_transient_and_autograd(cuda):
+    """The autograd backward takes the record path; its cotangents of phi and phi_old equal the
+    recomputing entry's bit for bit (BDF2, limited Green-Gauss on tetrahedra)."""
+    mesh, u, phi, gamma, old, older, first, _, geom, st = _transient_setup(("tet3", "limited", "gg", False), cuda, 3)
+    lam = torch.tensor(np.random.default_rng(13).normal(size=mesh.n_cells), device=cuda)
+    T = torch.tensor
+    xs = [T(x, device=cuda, requires_grad=True) for x in (phi, old, older)]
+    Rg = st.residual(SC.gpu_flux(geom, u).detach())(xs[0], xs[1], xs[2], DT, False)
+    got = torch.autograd.grad(Rg, xs, lam)
+    bars, _ = st.residual_vjp(SC.gpu_flux(geom, u), xs[0].detach(), lam, phi_old=xs[1].detach(),
+                              phi_older=xs[2].detach(), dt=DT, first_step=False)
+    for g, b in zip(got, (bars[0], bars[5], bars[6])):
+        assert torch.equal(g, b)
+
+
+def test_record_refused_for_corrected_green_gauss(cuda):
+    mesh, u, bcs, phi, gamma, _, geom, st = _setup(("quad_mixed", "limited", "cgg"), cuda)
+    assert not st._raw.records
+    prim = st._prim(SC.gpu_flux(geom, u), torch.tensor(phi, device=cuda), None, None)
+    geo = tuple(t.detach() for t in st._geometry())
+    with pytest.raises(ValueError, match="corrected Green-Gauss"):
+        st._raw.residual_record(prim, geo)
+
+
+def test_geometry_vjp_from_record_is_bitwise_the_recomputing_vjp(cuda):
+    """The autograd backward of the geometry reads the forward's approximate centroids
+    (afx_geometry_vjp_record); it equals afx_geometry_vjp, which forms them again, bit for bit."""
+    import ctypes
+
+    from aquaflux_amd import _lib
+
+    for name in ("tet3", "quad_mixed", "brick3d"):
+        mesh, _, _ = SC.make(name)
+        X = torch.tensor(mesh.node_coords.T, dtype=F64, device=cuda).contiguous().requires_grad_(True)
+        geom = mesh.geometry(node_coords=X.T, device=cuda)
+        dm = geom.device_mesh
+        soa = (geom.face.area, geom.face.normal_soa, geom.face.centroid_soa, geom.cell.volume,
+               geom.cell.centroid_soa)
+        rng = np.random.default_rng(10)
+        w = [torch.tensor(rng.normal(size=tuple(t.shape)), device=cuda) for t in soa]
+        (got,) = torch.autograd.grad(soa, X, w)
+        Xs = geom.node_coords_soa.detach()
+        gs = _lib.AfxGeometry(*[t.data_ptr() for t in soa])
+        gb = _lib.AfxGeometryCotangents(*[t.data_ptr() for t in w])
+        n = ctypes.c_int64()
+        L = _lib.lib()
+        _lib.check(L.afx_geometry_vjp_workspace_bytes(dm.handle, ctypes.byref(n)), "workspace")
+        work = torch.empty(int(n.value), dtype=torch.uint8, device=cuda)
+        ref = torch.empty_like(Xs)
+        _lib.check(L.afx_geometry_vjp(dm.handle, Xs.data_ptr(), ctypes.byref(gs), ctypes.byref(gb), ref.data_ptr(),
+                                      work.data_ptr(), work.numel(), _lib.stream_of(cuda)), "afx_geometry_vjp")
+        torch.cuda.synchronize()
+        print(name, float((got - ref).abs().max()))
+        assert torch.equal(got, ref)
