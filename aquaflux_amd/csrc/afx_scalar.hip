@@ -718,22 +718,61 @@ __global__ void __launch_bounds__(256) k_sc_lim_adj(MeshView m, ScArgs<double> a
 
 // reverse pass G'.  CG: the gradient is corrected Green-Gauss (GRAD 0 here, the swept solve having
 // formed the cotangent of b), whose own geometry cotangents are folded in with the face sums.
+// What a cell of the workgroup's block offers its neighbours in the reverse gradient pass: the
+// gradient cotangent, centroid and value, the forward gradient (least-squares geometry cotangents)
+// and the limiter extrema with their cotangents.
+template <int D, int GRAD, bool LIM, bool GEO>
+struct ScGradAdjTile {
+  static constexpr int NG = (GRAD == 1 && GEO) ? D : 1;
+  static constexpr int NE = LIM ? 2 : 1;
+  double B[D][kBlock];
+  double cc[D][kBlock];
+  double phi[kBlock];
+  double G[NG][kBlock];
+  double bex[NE][kBlock];
+  int32_t ex[NE][kBlock];
+};
+
 template <int D, int GRAD, bool LIM, bool GEO, bool CG = false>
 __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> a, const int32_t* __restrict__ ext,
                                                      ScBars bar) {
-  const int32_t c = xcd_thread_index(m);
-  if (c >= m.C) return;
-  double Bc[D];
+  __shared__ ScGradAdjTile<D, GRAD, LIM, GEO> tile;
+  const int32_t base = block_base(m);
+  const int t = threadIdx.x;
+  const int32_t c = base + t;
+  const bool valid = c < m.C;
+  double Bc[D], ccc[D];  // this cell's gradient cotangent and centroid, for every face
+  double Gc[D];          // the forward gradient (least-squares geometry cotangents)
+  double xc = 0.0;
 #pragma unroll
-  for (int j = 0; j < D; ++j) Bc[j] = bar.G[(int64_t)j * m.C + c];
-  const double xc = a.phi[c];
+  for (int j = 0; j < D; ++j) Gc[j] = 0.0;
+  if (valid) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      Bc[j] = bar.G[(int64_t)j * m.C + c];
+      ccc[j] = m.cc[(int64_t)j * m.C + c];
+      tile.B[j][t] = Bc[j];
+      tile.cc[j][t] = ccc[j];
+    }
+    xc = a.phi[c];
+    tile.phi[t] = xc;
+    if (GRAD == 1 && GEO) {
+      load_grad_row<D, double>(a.G, m.C, c, Gc);
+#pragma unroll
+      for (int j = 0; j < D; ++j) tile.G[j < ScGradAdjTile<D, GRAD, LIM, GEO>::NG ? j : 0][t] = Gc[j];
+    }
+    if (LIM) {
+      tile.ex[0][t] = ext[c];
+      tile.ex[LIM ? 1 : 0][t] = ext[(int64_t)m.C + c];
+      tile.bex[0][t] = bar.ext[c];
+      tile.bex[LIM ? 1 : 0][t] = bar.ext[(int64_t)m.C + c];
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
   double xb = 0.0, gamb = 0.0, ccb[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) ccb[j] = 0.0;
-  double Gc[D];  // the forward gradient (least-squares geometry cotangents)
-#pragma unroll
-  for (int j = 0; j < D; ++j) Gc[j] = 0.0;
-  if (GRAD == 1 && GEO) load_grad_row<D, double>(a.G, m.C, c, Gc);
   double zero[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) zero[j] = 0.0;
@@ -744,23 +783,59 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
     qb.zero();
     bool have_geo = false;
     if (o >= 0) {
-      // the other cell's limiter extrema (which cells attain them, and their cotangents), fetched
-      // with the rest of its data: no load waits on a comparison
+      // the other cell's record, from the tile when it is in this block: its gradient cotangent,
+      // centroid and value, forward gradient, and limiter extrema (which cells attain them, and
+      // their cotangents), all fetched before use, so no load waits on a comparison
       int32_t oamx = 0, oamn = 0;
-      double obmx = 0.0, obmn = 0.0;
-      if (LIM) {
-        oamx = ext[o];
-        oamn = ext[(int64_t)m.C + o];
-        obmx = bar.ext[o];
-        obmn = bar.ext[(int64_t)m.C + o];
-      }
-      if (GRAD == 0) {
-        FaceGeo<D> q;
-        load_face_geo<D>(m, r.f, r.P, r.N, q);
-        double Bo[D], BP[D], BN[D];
+      double obmx = 0.0, obmn = 0.0, po = 0.0;
+      double Bo[D], cco[D], Go[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) Go[j] = 0.0;
+      const int ls = tile_slot(o, base);
+      if (ls >= 0) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          Bo[j] = tile.B[j][ls];
+          cco[j] = tile.cc[j][ls];
+        }
+        po = tile.phi[ls];
+        if (GRAD == 1 && GEO) {
+#pragma unroll
+          for (int j = 0; j < D; ++j) Go[j] = tile.G[j < ScGradAdjTile<D, GRAD, LIM, GEO>::NG ? j : 0][ls];
+        }
+        if (LIM) {
+          oamx = tile.ex[0][ls];
+          oamn = tile.ex[LIM ? 1 : 0][ls];
+          obmx = tile.bex[0][ls];
+          obmn = tile.bex[LIM ? 1 : 0][ls];
+        }
+      } else {
 #pragma unroll
         for (int j = 0; j < D; ++j) {
           Bo[j] = bar.G[(int64_t)j * m.C + o];
+          cco[j] = m.cc[(int64_t)j * m.C + o];
+        }
+        po = a.phi[o];
+        if (GRAD == 1 && GEO) load_grad_row<D, double>(a.G, m.C, o, Go);
+        if (LIM) {
+          oamx = ext[o];
+          oamn = ext[(int64_t)m.C + o];
+          obmx = bar.ext[o];
+          obmn = bar.ext[(int64_t)m.C + o];
+        }
+      }
+      if (GRAD == 0) {
+        FaceGeo<D> q;
+        double xPa[D], xNa[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          xPa[j] = r.own ? ccc[j] : cco[j];
+          xNa[j] = r.own ? cco[j] : ccc[j];
+        }
+        load_face_geo_pts<D>(m, r.f, true, xPa, xNa, q);
+        double BP[D], BN[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
           BP[j] = r.own ? Bc[j] : Bo[j];
           BN[j] = r.own ? Bo[j] : Bc[j];
         }
@@ -769,7 +844,6 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
         for (int j = 0; j < D; ++j) pb_ += (q.n[j] * q.A) * (BP[j] - BN[j]);
         xb += (r.own ? 1.0 - q.g : q.g) * pb_;
         if (GEO) {
-          const double po = a.phi[o];
           const double phP = r.own ? xc : po, phN = r.own ? po : xc;
           const double phf = (1.0 - q.g) * phP + q.g * phN;
           double dn = 0.0;
@@ -788,12 +862,11 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
       } else if (GRAD == 1) {
         double dx[D];
 #pragma unroll
-        for (int j = 0; j < D; ++j) dx[j] = m.cc[(int64_t)j * m.C + o] - m.cc[(int64_t)j * m.C + c];
+        for (int j = 0; j < D; ++j) dx[j] = cco[j] - ccc[j];
         const double wgt = 1.0 / vdot<D>(dx, dx);
-        double own_dot = 0.0, oth_dot = 0.0, Bo[D];
+        double own_dot = 0.0, oth_dot = 0.0;
 #pragma unroll
         for (int j = 0; j < D; ++j) {
-          Bo[j] = bar.G[(int64_t)j * m.C + o];
           own_dot += dx[j] * Bc[j];
           oth_dot += dx[j] * Bo[j];
         }
@@ -803,9 +876,6 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
           // (bar.G here) and the fit residual r = dphi - G.d of the face, the cotangent of its d is
           // w (r y - (y.d) G) - 2 w^2 (y.d) r d.  c's own fit runs along d = x_o - x_c, o's along -d;
           // both ends are cell centroids, of which this thread owns x_c.
-          double Go[D];
-          load_grad_row<D, double>(a.G, m.C, o, Go);
-          const double po = a.phi[o];
           double gd_c = 0.0, gd_o = 0.0;
 #pragma unroll
           for (int j = 0; j < D; ++j) {
@@ -828,7 +898,7 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
       }
     } else if (GRAD != kGradNone) {
       FaceGeo<D> q;
-      load_face_geo<D>(m, r.f, r.P, r.N, q);
+      load_face_geo_pts<D>(m, r.f, false, ccc, ccc, q);
       double phib;
       if (GRAD == 0) {
         double s = 0.0;
