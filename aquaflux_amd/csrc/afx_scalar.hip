@@ -61,12 +61,36 @@ AFX_DEV void load_grad_row(const In<S>& G, int32_t C, int32_t c, S (&g)[D]) {
 
 // ---------------------------------------------------------------------------------------------
 // pass G
+// LDS copy of a workgroup's cells as the gradient pass reads its neighbours: value and centroid.
+template <int D, class S>
+struct ScGradTile {
+  S x[kBlock];
+  double cc[D][kBlock];
+};
+
 template <int D, class S, int GRAD, bool LIM>
 __global__ void __launch_bounds__(256) k_sc_grad(MeshView m, ScArgs<S> a, Out<S> G, Out<S> psi_out,
-                                                 int32_t* __restrict__ ext) {
-  const int32_t c = xcd_thread_index(m);
-  if (c >= m.C) return;
-  const S xc = a.phi[c];
+                                                 int32_t* __restrict__ ext, int32_t* __restrict__ kmin_out) {
+  __shared__ ScGradTile<D, S> tile;
+  const int32_t base = block_base(m);
+  const int t = threadIdx.x;
+  const int32_t c = base + t;
+  const bool valid = c < m.C;
+  S xc = sconst<S>(0.0);
+  double ccc[D];  // this cell's centroid, for every face
+#pragma unroll
+  for (int j = 0; j < D; ++j) ccc[j] = 0.0;
+  if (valid) {
+    xc = a.phi[c];
+    tile.x[t] = xc;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      ccc[j] = m.cc[(int64_t)j * m.C + c];
+      tile.cc[j][t] = ccc[j];
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
   S acc[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) acc[j] = sconst<S>(0.0);
@@ -85,7 +109,18 @@ __global__ void __launch_bounds__(256) k_sc_grad(MeshView m, ScArgs<S> a, Out<S>
     FaceRef r = walk.next(m, c);
     const int32_t o = r.own ? r.N : r.P;
     if (o >= 0) {
-      const S xo = a.phi[o];
+      S xo;
+      double cco[D];
+      const int ls = tile_slot(o, base);
+      if (ls >= 0) {
+        xo = tile.x[ls];
+#pragma unroll
+        for (int j = 0; j < D; ++j) cco[j] = tile.cc[j][ls];
+      } else {
+        xo = a.phi[o];
+#pragma unroll
+        for (int j = 0; j < D; ++j) cco[j] = m.cc[(int64_t)j * m.C + o];
+      }
       if (LIM) {
         if (val(xo) > val(mx)) {
           mx = xo;
@@ -98,7 +133,13 @@ __global__ void __launch_bounds__(256) k_sc_grad(MeshView m, ScArgs<S> a, Out<S>
       }
       if (GRAD == 0) {
         FaceGeo<D> q;
-        load_face_geo<D>(m, r.f, r.P, r.N, q);
+        double xPa[D], xNa[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          xPa[j] = r.own ? ccc[j] : cco[j];
+          xNa[j] = r.own ? cco[j] : ccc[j];
+        }
+        load_face_geo_pts<D>(m, r.f, true, xPa, xNa, q);
         const double w = 1.0 - q.g;
         const S pf = w * (r.own ? xc : xo) + q.g * (r.own ? xo : xc);
 #pragma unroll
@@ -109,7 +150,7 @@ __global__ void __launch_bounds__(256) k_sc_grad(MeshView m, ScArgs<S> a, Out<S>
       } else if (GRAD == 1) {
         double dx[D];
 #pragma unroll
-        for (int j = 0; j < D; ++j) dx[j] = m.cc[(int64_t)j * m.C + o] - m.cc[(int64_t)j * m.C + c];
+        for (int j = 0; j < D; ++j) dx[j] = cco[j] - ccc[j];
         const double wgt = 1.0 / vdot<D>(dx, dx);
 #pragma unroll
         for (int p = 0; p < D; ++p)
@@ -121,7 +162,7 @@ __global__ void __launch_bounds__(256) k_sc_grad(MeshView m, ScArgs<S> a, Out<S>
       }
     } else if (GRAD != kGradNone) {
       FaceGeo<D> q;
-      load_face_geo<D>(m, r.f, r.P, r.N, q);
+      load_face_geo_pts<D>(m, r.f, false, ccc, ccc, q);
       const S pb = scalar_bval<D, S>(a.kind[r.fu], q, xc, zero, a.val[r.fu], a.gam[c], a.tinf[r.fu]);
       if (GRAD == 0) {
 #pragma unroll
@@ -164,11 +205,10 @@ __global__ void __launch_bounds__(256) k_sc_grad(MeshView m, ScArgs<S> a, Out<S>
   for (int j = 0; j < D; ++j) G.set((int64_t)j * m.C + c, grad[j]);
   if (LIM) {
     const double e2 = m.vol[c] * (a.lim_k * a.lim_k * a.lim_k);
-    double xcen[D];
-#pragma unroll
-    for (int j = 0; j < D; ++j) xcen[j] = m.cc[(int64_t)j * m.C + c];
+    const double* xcen = ccc;
     S lo = sconst<S>(0.0);
     bool first = true;
+    int32_t ks = 0;  // position in the face list of the first face attaining the minimum
     for (int32_t k = kb; k < ke; ++k) {
       const int32_t e = m.cface[k];
       const int32_t f = e >= 0 ? e : ~e;
@@ -176,13 +216,17 @@ __global__ void __launch_bounds__(256) k_sc_grad(MeshView m, ScArgs<S> a, Out<S>
 #pragma unroll
       for (int j = 1; j < D; ++j) dm = dm + grad[j] * (m.fc[(int64_t)j * m.F + f] - xcen[j]);
       const S vf = venkat_face<S>(dm, mx - xc, mn - xc, e2);
-      if (first || val(vf) < val(lo)) lo = vf;
+      if (first || val(vf) < val(lo)) {
+        lo = vf;
+        ks = k - kb;
+      }
       first = false;
     }
     psi_out.set(c, dmin(dmax(lo, 0.0), 1.0));
     if (ext) {
       ext[c] = amx;
       ext[(int64_t)m.C + c] = amn;
+      if (kmin_out) kmin_out[c] = ks;
     }
   }
 }
@@ -595,7 +639,7 @@ AFX_DEV dual venkat_face_de2(double dm0, double hmax, double hmin, double e2) {
 // operator (GG: 1/V, LSQ: M^-1); GG also yields the volume cotangent -Gbar.G / V.
 template <int D, int GRAD, bool LIM, bool GEO>
 __global__ void __launch_bounds__(256) k_sc_lim_adj(MeshView m, ScArgs<double> a, const int32_t* __restrict__ ext,
-                                                    ScBars bar) {
+                                                    const int32_t* __restrict__ kmin_pos, ScBars bar) {
   const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   double Gb[D], Gc[D];
@@ -615,20 +659,17 @@ __global__ void __launch_bounds__(256) k_sc_lim_adj(MeshView m, ScArgs<double> a
     const double pbar = bar.psi[c];
     const int32_t amx = ext[c], amn = ext[(int64_t)m.C + c];
     const double xv = a.phi[c], mxv = a.phi[amx], mnv = a.phi[amn];
-    double lo = 0.0;
-    int32_t kmin = -1;
-    for (int32_t k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-      const int32_t e = m.cface[k];
-      const int32_t f = e >= 0 ? e : ~e;
-      double dmf = Gc[0] * (m.fc[f] - xcen[0]);
+    // the first face attaining the forward minimum, recorded by the forward limiter pass: only it
+    // is evaluated again
+    const int32_t e = m.cface[m.cptr[c] + kmin_pos[c]];
+    const int32_t f = e >= 0 ? e : ~e;
+    double dxf[D];
 #pragma unroll
-      for (int j = 1; j < D; ++j) dmf = dmf + Gc[j] * (m.fc[(int64_t)j * m.F + f] - xcen[j]);
-      const double vf = venkat_face<double>(dmf, mxv - xv, mnv - xv, e2);
-      if (kmin < 0 || vf < lo) {
-        lo = vf;
-        kmin = k;
-      }
-    }
+    for (int j = 0; j < D; ++j) dxf[j] = m.fc[(int64_t)j * m.F + f] - xcen[j];
+    double dmf = Gc[0] * dxf[0];
+#pragma unroll
+    for (int j = 1; j < D; ++j) dmf = dmf + Gc[j] * dxf[j];
+    const double lo = venkat_face<double>(dmf, mxv - xv, mnv - xv, e2);
     const double dclip = (lo > 0.0 && lo < 1.0) ? 1.0 : ((lo == 0.0 || lo == 1.0) ? 0.5 : 0.0);
     const double vbar = pbar * dclip;
     double mxb = 0.0, mnb = 0.0;
@@ -636,15 +677,7 @@ __global__ void __launch_bounds__(256) k_sc_lim_adj(MeshView m, ScArgs<double> a
     double xfb[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) xfb[j] = 0.0;
-    if (vbar != 0.0 && kmin >= 0) {
-      const int32_t e = m.cface[kmin];
-      const int32_t f = e >= 0 ? e : ~e;
-      double dxf[D];
-#pragma unroll
-      for (int j = 0; j < D; ++j) dxf[j] = m.fc[(int64_t)j * m.F + f] - xcen[j];
-      double dmf = Gc[0] * dxf[0];
-#pragma unroll
-      for (int j = 1; j < D; ++j) dmf = dmf + Gc[j] * dxf[j];
+    if (vbar != 0.0) {
       const dual r_dm = venkat_face<dual>(dual{dmf, 1.0}, dual{mxv - xv, 0.0}, dual{mnv - xv, 0.0}, e2);
       const dual r_hx = venkat_face<dual>(dual{dmf, 0.0}, dual{mxv - xv, 1.0}, dual{mnv - xv, 0.0}, e2);
       const dual r_hn = venkat_face<dual>(dual{dmf, 0.0}, dual{mxv - xv, 0.0}, dual{mnv - xv, 1.0}, e2);
@@ -1283,7 +1316,8 @@ __global__ void __launch_bounds__(256) k_lsq_solve_t(MeshView m, const double* _
 // host side
 
 // Workspace in doubles: mode 0 = G [D][C], psi [C], ext (2 int32 per cell); mode 1 adds tangents of
-// G and psi; mode 2 adds the adjoint scratch (G, psi, ext, limiter face cotangent, argmin face).
+// G and psi; mode 2 adds the adjoint scratch (G, psi, ext, limiter face cotangent, argmin face, and
+// the forward's minimum face).
 int64_t sc_work_doubles(int D, int64_t C, int mode) {
   int64_t n = (int64_t)(D + 1 + 1) * C + (int64_t)2 * D * C;  // ... + corrected Green-Gauss sweep scratch
   if (mode == 1) n += (int64_t)(D + 1) * C;
@@ -1295,7 +1329,7 @@ struct ScWork {
   double *G, *psi, *Gt, *psit;
   int32_t* ext;
   double *Gb, *psib, *extb, *limx;
-  int32_t* limf;
+  int32_t *limf, *kmin;  // kmin: the limiter's minimum face (position in the cell's face list)
   double* cgg;
 };
 
@@ -1324,6 +1358,7 @@ ScWork sc_carve(void* base, int D, int64_t C, int mode) {
     w.limx = p;
     p += (int64_t)D * C;
     w.limf = (int32_t*)p;
+    w.kmin = w.limf + C;
     p += C;
   }
   w.cgg = p;
@@ -1382,7 +1417,7 @@ int sc_forward(const afx_mesh* mh, const afx_geometry* g, ScArgs<S> a, Out<S> R,
   constexpr bool LIM = ADV == kAdvLimited;
   constexpr int GK = GRAD == kGradCorrected ? 0 : GRAD;
   AFX_LAUNCH(kKScalarGrad, (k_sc_grad<D, S, GK, LIM && GRAD != kGradCorrected>), grid_for(m.C), dim3(kBlock), 0, s,
-             m, a, Out<S>{w.G, Gt}, Out<S>{w.psi, psit}, ext);
+             m, a, Out<S>{w.G, Gt}, Out<S>{w.psi, psit}, ext, ext ? w.kmin : nullptr);
   AFX_CHECK_LAUNCH();
   a.G.v = w.G;
   a.G.t = Gt;
@@ -1398,6 +1433,7 @@ int sc_forward(const afx_mesh* mh, const afx_geometry* g, ScArgs<S> a, Out<S> R,
       LimiterSlots sl{};
       sl.nl = 1;
       sl.ext_stride = 1;
+      sl.kmin = ext ? w.kmin : nullptr;
       rc = venkat_pass(m, D, sl, a.phi, a.G, a.lim_k, Out<S>{w.psi, psit}, ext, s);
       if (rc) return rc;
     }
@@ -1754,7 +1790,7 @@ extern "C" int afx_scalar_jvp_geometry(const afx_mesh* m, const afx_geometry* g,
 // from the residual evaluation, so the reverse pass at the same inputs does not run it again.
 namespace {
 
-int64_t sc_record_doubles(int D, int64_t C) { return (int64_t)(D + 2) * C; }
+int64_t sc_record_doubles(int D, int64_t C) { return (int64_t)(D + 2) * C + (C + 1) / 2; }
 
 ScWork sc_record_view(const void* record, int D, int64_t C) {
   ScWork w{};
@@ -1762,6 +1798,7 @@ ScWork sc_record_view(const void* record, int D, int64_t C) {
   w.G = p;
   w.psi = p + (int64_t)D * C;
   w.ext = (int32_t*)(w.psi + C);
+  w.kmin = w.ext + 2 * C;
   return w;
 }
 
@@ -1837,6 +1874,7 @@ int sc_vjp(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cf
     w.G = r.G;
     w.psi = r.psi;
     w.ext = r.ext;
+    w.kmin = r.kmin;
   }
   ScArgs<double> a = sc_args<double>(cfg, in, nullptr);
   AFX_REQUIRE(cfg->transient == 0 || out->phi_old, "afx_scalar_vjp: the transient term needs a phi_old cotangent");
@@ -1877,7 +1915,7 @@ int sc_vjp(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cf
                  bar);
       AFX_CHECK_LAUNCH();
       AFX_LAUNCH(kKScalarLimAdj, (k_sc_lim_adj<D, GR, LIM, GEO>), grid_for(m->C), dim3(kBlock), 0, s, v, b, w.ext,
-                 bar);
+                 w.kmin, bar);
       AFX_CHECK_LAUNCH();
       if (GR == kGradCorrected) {
         int r3 = GEO ? cgg_sweeps_adjoint_geometry(v, D, 1, bar.G, w.G, iters, w.cgg, cfg->gradient_sweeps,
