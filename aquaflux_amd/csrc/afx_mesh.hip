@@ -743,20 +743,36 @@ __global__ void __launch_bounds__(256) k_geom_face_adj(MeshView m, int32_t Nn, c
 }
 
 // Node pass: node cotangent = sum over the node's incidences.
+constexpr int kNodeLanes = 8;  // lanes per node in the node pass of the geometry adjoint
+
 template <int D>
 __global__ void __launch_bounds__(256) k_geom_node_adj(int32_t Nn, const int32_t* __restrict__ nptr, const double* __restrict__ inc_b,
                                 double* __restrict__ node_b) {
-  int32_t p = xcd_thread_index();
-  if (p >= Nn) return;
+  // kNodeLanes consecutive lanes per node: lane l sums incidences l, l + kNodeLanes, ... of the
+  // node's contiguous range (so a wave's loads cover contiguous memory), then an xor butterfly adds
+  // the partial sums; every lane of the group forms the same total (each step adds the same two
+  // values), so the result does not depend on which lane stores it.
+  const int32_t tid = xcd_thread_index();
+  const int32_t p = tid / kNodeLanes;
+  const int l = tid % kNodeLanes;
   double s[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) s[j] = 0.0;
-  for (int32_t k = nptr[p]; k < nptr[p + 1]; ++k) {
+  if (p < Nn) {
+    for (int32_t k = nptr[p] + l; k < nptr[p + 1]; k += kNodeLanes) {
 #pragma unroll
-    for (int j = 0; j < D; ++j) s[j] += inc_b[(int64_t)k * D + j];
+      for (int j = 0; j < D; ++j) s[j] += inc_b[(int64_t)k * D + j];
+    }
   }
 #pragma unroll
-  for (int j = 0; j < D; ++j) node_b[(int64_t)j * Nn + p] = s[j];
+  for (int off = kNodeLanes / 2; off > 0; off >>= 1) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) s[j] += __shfl_xor(s[j], off, kNodeLanes);
+  }
+  if (p < Nn && l == 0) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) node_b[(int64_t)j * Nn + p] = s[j];
+  }
 }
 
 }  // namespace
@@ -976,6 +992,7 @@ int geometry_vjp(const afx_mesh* m, const double* node_coords, const afx_geometr
   afx_geometry_vjp_workspace_bytes(m, &need);
   AFX_REQUIRE(work_bytes >= need - 256, "afx_geometry_vjp: workspace too small");
   AFX_REQUIRE(m->C > 0 && m->F > 0 && m->N > 0, "afx_geometry_vjp: empty mesh");
+  AFX_REQUIRE((int64_t)m->N * kNodeLanes < INT32_MAX, "afx_geometry_vjp: too many nodes for the node pass");
   const int D = m->dim;
   double* approx = record ? const_cast<double*>(record) : (double*)work;
   double* cellbar = (double*)work + (int64_t)D * m->C;
@@ -992,7 +1009,7 @@ int geometry_vjp(const afx_mesh* m, const double* node_coords, const afx_geometr
     AFX_LAUNCH(kKGeomAdjFace, k_geom_face_adj<2>, grid_for(m->F), dim3(kBlock), 0, s, v, m->N, m->foff, m->fidx,
                m->kpos, node_coords, approx, cellbar, gb->area, gb->normal, gb->face_centroid, inc_b);
     AFX_CHECK_LAUNCH();
-    AFX_LAUNCH(kKGeomAdjNode, k_geom_node_adj<2>, grid_for(m->N), dim3(kBlock), 0, s, m->N, m->nptr, inc_b,
+    AFX_LAUNCH(kKGeomAdjNode, k_geom_node_adj<2>, grid_for((int64_t)m->N * kNodeLanes), dim3(kBlock), 0, s, m->N, m->nptr, inc_b,
                node_b);
     AFX_CHECK_LAUNCH();
   } else {
@@ -1006,7 +1023,7 @@ int geometry_vjp(const afx_mesh* m, const double* node_coords, const afx_geometr
     AFX_LAUNCH(kKGeomAdjFace, k_geom_face_adj<3>, grid_for(m->F), dim3(kBlock), 0, s, v, m->N, m->foff, m->fidx,
                m->kpos, node_coords, approx, cellbar, gb->area, gb->normal, gb->face_centroid, inc_b);
     AFX_CHECK_LAUNCH();
-    AFX_LAUNCH(kKGeomAdjNode, k_geom_node_adj<3>, grid_for(m->N), dim3(kBlock), 0, s, m->N, m->nptr, inc_b,
+    AFX_LAUNCH(kKGeomAdjNode, k_geom_node_adj<3>, grid_for((int64_t)m->N * kNodeLanes), dim3(kBlock), 0, s, m->N, m->nptr, inc_b,
                node_b);
     AFX_CHECK_LAUNCH();
   }
