@@ -60,6 +60,7 @@ struct RansArgs {
 struct ClosureOut {
   double *nut, *f1, *Sr, *cd, *nutw, *tau, *mueff;
   double *nut_t, *f1_t, *Sr_t, *cd_t, *nutw_t, *tau_t, *mueff_t;
+  double* lsqI;  // nullable [D][D][C]: the least-squares fit's inverse matrix, kept for the closure adjoint
 };
 
 template <int D, class S>
@@ -284,6 +285,14 @@ AFX_DEV void k_rans_grad_body(const Exec& ex, Sh& sh, MeshView m, RansArgs<S> a,
         for (int q2 = 1; q2 < D; ++q2) s = s + I[p][q2] * acc[v][q2];
         grad[v][p] = s;
       }
+    if constexpr (!GEO) {
+      if (co.lsqI) {
+#pragma unroll
+        for (int p = 0; p < D; ++p)
+#pragma unroll
+          for (int q2 = 0; q2 < D; ++q2) co.lsqI[(int64_t)(p * D + q2) * m.C + c] = I[p][q2];
+      }
+    }
   }
 #pragma unroll
   for (int v = 0; v < NV; ++v)
@@ -1231,7 +1240,7 @@ __global__ void __launch_bounds__(256, 3) k_rans_ap_adj(MeshView m, RansArgs<dou
 // reverse pass C': closure (seeded duals), wall shear rate, limiters; then the gradient cotangent
 // is pre-multiplied by the cell's reconstruction operator.
 template <int D, int ADVF, int ADVS, int GRAD>
-AFX_DEV void k_rans_closure_adj_body(int32_t c, MeshView m, RansArgs<double> a, const int32_t* __restrict__ ext, const int32_t* __restrict__ kmin_pos, RansBars bar) {
+AFX_DEV void k_rans_closure_adj_body(int32_t c, MeshView m, RansArgs<double> a, const int32_t* __restrict__ ext, const int32_t* __restrict__ kmin_pos, const double* __restrict__ lsqI, RansBars bar) {
   constexpr int NA = D + 3, NL = D + 2;
   const SstConst& sc = a.sc;
   if (c >= m.C) return;
@@ -1415,9 +1424,17 @@ AFX_DEV void k_rans_closure_adj_body(int32_t c, MeshView m, RansArgs<double> a, 
 #pragma unroll
       for (int j = 0; j < D; ++j) bar.G[(int64_t)(v * D + j) * m.C + c] = Gb[v][j] * iv;
   } else {
-    double M[D][D], I[D][D];
-    lsq_matrix<D>(m, c, M);
-    sym_inverse<D>(M, I);
+    double I[D][D];
+    if (lsqI) {  // the forward gradient pass's inverse
+#pragma unroll
+      for (int p = 0; p < D; ++p)
+#pragma unroll
+        for (int q2 = 0; q2 < D; ++q2) I[p][q2] = lsqI[(int64_t)(p * D + q2) * m.C + c];
+    } else {
+      double M[D][D];
+      lsq_matrix<D>(m, c, M);
+      sym_inverse<D>(M, I);
+    }
 #pragma unroll
     for (int v = 0; v < NA; ++v)
 #pragma unroll
@@ -1433,8 +1450,9 @@ AFX_DEV void k_rans_closure_adj_body(int32_t c, MeshView m, RansArgs<double> a, 
 template <int D, int ADVF, int ADVS, int GRAD>
 __global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<double> a,
                                                           const int32_t* __restrict__ ext,
-                                                          const int32_t* __restrict__ kmin_pos, RansBars bar) {
-  k_rans_closure_adj_body<D, ADVF, ADVS, GRAD>(xcd_thread_index(m), m, a, ext, kmin_pos, bar);
+                                                          const int32_t* __restrict__ kmin_pos,
+                                                          const double* __restrict__ lsqI, RansBars bar) {
+  k_rans_closure_adj_body<D, ADVF, ADVS, GRAD>(xcd_thread_index(m), m, a, ext, kmin_pos, lsqI, bar);
 }
 
 // reverse pass G'
@@ -2285,6 +2303,9 @@ int rans_forward(const afx_mesh* mh, const afx_geometry* g, RansArgs<S> a, Out<S
       *tt[i] = clt ? clt + i * C : nullptr;
     }
   }
+  // the reverse pass (ext given) keeps the least-squares inverse in the corrected scheme's sweep
+  // scratch, which that scheme alone uses
+  if constexpr (GRAD == 1 && !GEO) co.lsqI = ext ? w.cgg : nullptr;
   // the closure rides on the gradient pass unless corrected-GG sweeps still change the gradient
   // (the dual-number build keeps them apart: fused, it would drop to one wave per SIMD)
   constexpr bool FUSE = !CG && std::is_same<S, double>::value;
@@ -2527,7 +2548,7 @@ int rans_vjp_impl(const afx_mesh* m, const afx_geometry* g, const afx_rans_confi
                      AFX_LAUNCH(kKRansApAdj, (k_rans_ap_adj<D, AF>), grid_for(m->C), dim3(kBlock), 0, s, v, b, bar);
                      AFX_CHECK_LAUNCH();
                      AFX_LAUNCH(kKSstClosureAdj, (k_rans_closure_adj<D, AF, AS, GR>), grid_for(m->C), dim3(kBlock), 0,
-                                s, v, b, w.ext, w.kmin, bar);
+                                s, v, b, w.ext, w.kmin, GR == 1 ? w.cgg : nullptr, bar);
                      AFX_CHECK_LAUNCH();
                      constexpr int GK = GR == kGradCorrected ? 0 : GR;
                      if (GR == kGradCorrected) {
@@ -2682,6 +2703,7 @@ int rans_residual_vjp_host_run(const MeshView& m, RansArgs<double> a, const doub
   ClosureOut co{};
   double** vv[7] = {&co.nut, &co.f1, &co.Sr, &co.cd, &co.nutw, &co.tau, &co.mueff};
   for (int i = 0; i < 7; ++i) *vv[i] = w.cl + i * C;
+  co.lsqI = GR == 1 ? w.cgg : nullptr;
   const Out<double> oG{w.G, nullptr}, oP{w.psi, nullptr};
   host_staged<RansGradSh<D, double>>(C, threads, [&](const HostExec& ex, RansGradSh<D, double>& sh) {
     k_rans_grad_body<D, double, GR, LF, LS, true, false>(ex, sh, m, a, oG, oP, w.ext, w.kmin, co);
@@ -2707,7 +2729,7 @@ int rans_residual_vjp_host_run(const MeshView& m, RansArgs<double> a, const doub
   host_staged<RansApAdjSh<D>>(C, threads, [&](const HostExec& ex, RansApAdjSh<D>& sh) {
     k_rans_ap_adj_body<D, AF>(ex, sh, m, a, bar);
   });
-  host_cells(C, threads, [&](int32_t c) { k_rans_closure_adj_body<D, AF, AS, GR>(c, m, a, w.ext, w.kmin, bar); });
+  host_cells(C, threads, [&](int32_t c) { k_rans_closure_adj_body<D, AF, AS, GR>(c, m, a, w.ext, w.kmin, GR == 1 ? w.cgg : nullptr, bar); });
   using GASh = RansGradAdjSh<D, (LF ? D : 0) + (LS ? 2 : 0)>;
   host_staged<GASh>(C, threads, [&](const HostExec& ex, GASh& sh) {
     k_rans_grad_adj_body<D, GR, LF, LS>(ex, sh, m, a, w.ext, bar);
