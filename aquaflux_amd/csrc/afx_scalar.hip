@@ -576,9 +576,7 @@ __global__ void __launch_bounds__(256, 3) k_sc_res_adj(MeshView m, ScArgs<double
         Qb = adj_sc_interior<D, ADV, GEO>(q, Qf, xo, xc, Go, Gc, po, pc, go, gc, lf, 1, B, qb);
       if (GEO) adj_face_geo<D>(q, true, qb);
       if (r.own) {
-        bar.flux[r.fu] = Qb;
-        bar.val[r.fu] = 0.0;
-        bar.tinf[r.fu] = 0.0;
+        bar.flux[r.fu] = Qb;  // (the boundary-value cotangents of interior faces are zero-filled before)
         write_face_geo_bar<D, GEO>(bar, m.F, r.fu, qb, false);
       }
 #pragma unroll
@@ -1911,6 +1909,10 @@ int sc_vjp(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cf
     constexpr int GK = GR == kGradCorrected ? 0 : GR;
     auto run = [&](auto Geo) {
       constexpr bool GEO = decltype(Geo)::value;
+      // boundary-value cotangents: zero over all faces (one coalesced fill), boundary entries
+      // written by the face passes
+      if (int rz = afx_fill_zero(bar.val, m->F, s)) return rz;
+      if (int rz = afx_fill_zero(bar.tinf, m->F, s)) return rz;
       AFX_LAUNCH(kKScalarResAdj, (k_sc_res_adj<D, ADV, GEO>), grid_for(m->C), dim3(kBlock), 0, s, v, b, cotangent,
                  bar);
       AFX_CHECK_LAUNCH();
