@@ -492,26 +492,54 @@ struct ScBars {
   double *area, *nrm, *fc, *vol, *cc;
   // corrected Green-Gauss: the swept solve's geometry cotangents (null for the other schemes)
   double *cggf, *cggv;
+  // with geometry cotangents: per internal face one 64-byte record (area, normal, centroid and
+  // face-flux cotangents) that the residual adjoint fills and the gradient adjoint completes and
+  // writes out
+  double* frec;
 };
 
-template <int D, bool GEO>
-AFX_DEV void write_face_geo_bar(const ScBars& bar, int32_t F, int32_t f, const FaceGeoBar<D>& qb, bool accumulate) {
-  if (!GEO) return;
-  if (accumulate) {
-    bar.area[f] += qb.A;
+// Face record layout: [0] area, [1, 1 + D) normal, [1 + D, 1 + 2D) centroid, [7] face flux.  A
+// face's cotangents are stored as one contiguous record (four 16-byte stores) rather than one
+// double in each of 2D + 2 arrays, so a wave's stores fill whole lines at once.
+constexpr int kFaceRec = 8;
+
+template <int D>
+AFX_DEV void store_face_rec(double* __restrict__ frec, int32_t f, const FaceGeoBar<D>& qb, double flux_b) {
+  double r[kFaceRec];
+  r[0] = qb.A;
 #pragma unroll
-    for (int j = 0; j < D; ++j) {
-      bar.nrm[(int64_t)j * F + f] += qb.n[j];
-      bar.fc[(int64_t)j * F + f] += qb.xf[j];
-    }
-  } else {
-    bar.area[f] = qb.A;
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-      bar.nrm[(int64_t)j * F + f] = qb.n[j];
-      bar.fc[(int64_t)j * F + f] = qb.xf[j];
-    }
+  for (int j = 0; j < D; ++j) {
+    r[1 + j] = qb.n[j];
+    r[1 + D + j] = qb.xf[j];
   }
+#pragma unroll
+  for (int i = 1 + 2 * D; i < kFaceRec - 1; ++i) r[i] = 0.0;
+  r[kFaceRec - 1] = flux_b;
+  double2* q = reinterpret_cast<double2*>(frec + (int64_t)kFaceRec * f);
+#pragma unroll
+  for (int i = 0; i < kFaceRec / 2; ++i) q[i] = make_double2(r[2 * i], r[2 * i + 1]);
+}
+
+// The owner's last visit of a face (the gradient adjoint): the record plus this pass's part, out to
+// the caller's arrays at the caller's face index u.
+template <int D>
+AFX_DEV void finish_face_rec(const ScBars& bar, int32_t F, const double* __restrict__ frec, int32_t f, int32_t u,
+                             const FaceGeoBar<D>& qb) {
+  const double2* q = reinterpret_cast<const double2*>(frec + (int64_t)kFaceRec * f);
+  double r[kFaceRec];
+#pragma unroll
+  for (int i = 0; i < kFaceRec / 2; ++i) {
+    const double2 v = q[i];
+    r[2 * i] = v.x;
+    r[2 * i + 1] = v.y;
+  }
+  bar.area[u] = r[0] + qb.A;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    bar.nrm[(int64_t)j * F + u] = r[1 + j] + qb.n[j];
+    bar.fc[(int64_t)j * F + u] = r[1 + D + j] + qb.xf[j];
+  }
+  bar.flux[u] = r[kFaceRec - 1];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -576,8 +604,11 @@ __global__ void __launch_bounds__(256, 3) k_sc_res_adj(MeshView m, ScArgs<double
         Qb = adj_sc_interior<D, ADV, GEO>(q, Qf, xo, xc, Go, Gc, po, pc, go, gc, lf, 1, B, qb);
       if (GEO) adj_face_geo<D>(q, true, qb);
       if (r.own) {
-        bar.flux[r.fu] = Qb;  // (the boundary-value cotangents of interior faces are zero-filled before)
-        write_face_geo_bar<D, GEO>(bar, m.F, r.fu, qb, false);
+        // (the boundary-value cotangents of interior faces are zero-filled before)
+        if (GEO)
+          store_face_rec<D>(bar.frec, r.f, qb, Qb);
+        else
+          bar.flux[r.fu] = Qb;
       }
 #pragma unroll
       for (int j = 0; j < D; ++j) ccb[j] += (r.own ? qb.xP[j] : 0.0) + (r.own ? 0.0 : qb.xN[j]);
@@ -587,10 +618,12 @@ __global__ void __launch_bounds__(256, 3) k_sc_res_adj(MeshView m, ScArgs<double
       const double Qb = adj_sc_boundary<D, ADV, GEO>(a.kind[r.fu], q, Qf, xc, Gc, pc, gc, a.val[r.fu], a.tinf[r.fu],
                                                      lc, B, vb, tb, qb);
       if (GEO) adj_face_geo<D>(q, false, qb);
-      bar.flux[r.fu] = Qb;
       bar.val[r.fu] = vb;
       bar.tinf[r.fu] = tb;
-      write_face_geo_bar<D, GEO>(bar, m.F, r.fu, qb, false);
+      if (GEO)
+        store_face_rec<D>(bar.frec, r.f, qb, Qb);
+      else
+        bar.flux[r.fu] = Qb;
 #pragma unroll
       for (int j = 0; j < D; ++j) ccb[j] += qb.xP[j];
     }
@@ -986,7 +1019,7 @@ __global__ void __launch_bounds__(256) k_sc_grad_adj(MeshView m, ScArgs<double> 
 #pragma unroll
           for (int j = 0; j < D; ++j) qb.xf[j] += bar.limx[(int64_t)j * m.C + r.N];
         }
-        write_face_geo_bar<D, GEO>(bar, m.F, r.fu, qb, true);
+        if (GEO) finish_face_rec<D>(bar, m.F, bar.frec, r.f, r.fu, qb);
       }
       if (have_geo) {
 #pragma unroll
@@ -1316,10 +1349,10 @@ __global__ void __launch_bounds__(256) k_lsq_solve_t(MeshView m, const double* _
 // Workspace in doubles: mode 0 = G [D][C], psi [C], ext (2 int32 per cell); mode 1 adds tangents of
 // G and psi; mode 2 adds the adjoint scratch (G, psi, ext, limiter face cotangent, argmin face, and
 // the forward's minimum face).
-int64_t sc_work_doubles(int D, int64_t C, int mode) {
+int64_t sc_work_doubles(int D, int64_t C, int mode, int64_t F = 0) {
   int64_t n = (int64_t)(D + 1 + 1) * C + (int64_t)2 * D * C;  // ... + corrected Green-Gauss sweep scratch
   if (mode == 1) n += (int64_t)(D + 1) * C;
-  if (mode == 2) n += (int64_t)(D + 1 + 2 + D + 1) * C;
+  if (mode == 2) n += (int64_t)(D + 1 + 2 + D + 1) * C + kFaceRec * F + kFaceRec;  // + face records (aligned)
   return n;
 }
 
@@ -1328,10 +1361,11 @@ struct ScWork {
   int32_t* ext;
   double *Gb, *psib, *extb, *limx;
   int32_t *limf, *kmin;  // kmin: the limiter's minimum face (position in the cell's face list)
+  double* frec;          // [F][kFaceRec] face records, 64-byte aligned (mode 2)
   double* cgg;
 };
 
-ScWork sc_carve(void* base, int D, int64_t C, int mode) {
+ScWork sc_carve(void* base, int D, int64_t C, int mode, int64_t F = 0) {
   double* p = (double*)base;
   ScWork w{};
   w.G = p;
@@ -1358,6 +1392,8 @@ ScWork sc_carve(void* base, int D, int64_t C, int mode) {
     w.limf = (int32_t*)p;
     w.kmin = w.limf + C;
     p += C;
+    w.frec = (double*)(((uintptr_t)p + 63) & ~(uintptr_t)63);
+    p += kFaceRec * F + kFaceRec;
   }
   w.cgg = p;
   return w;
@@ -1456,7 +1492,7 @@ int sc_check(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* 
   AFX_REQUIRE(cfg->transient >= 0 && cfg->transient <= 2, "scalar: transient must be 0 (steady), 1 or 2 (BDF order)");
   AFX_REQUIRE(cfg->transient == 0 || (cfg->dt > 0.0 && in->phi_old), "scalar: the transient term needs dt > 0 and phi_old");
   AFX_REQUIRE(cfg->transient != 2 || in->phi_older, "scalar: BDF2 needs phi_older");
-  AFX_REQUIRE(mode < 0 || (work && work_size >= sc_work_doubles(m->dim, m->C, mode) * 8),
+  AFX_REQUIRE(mode < 0 || (work && work_size >= sc_work_doubles(m->dim, m->C, mode, m->F) * 8),
               "scalar: workspace too small");
   return 0;
 }
@@ -1630,7 +1666,7 @@ extern "C" int afx_scalar_gradient_vjp(const afx_mesh* mh, const afx_geometry* g
 
 extern "C" int afx_scalar_workspace_bytes(const afx_mesh* m, int mode, int64_t* bytes) {
   AFX_REQUIRE(m && bytes && mode >= 0 && mode <= 2, "afx_scalar_workspace_bytes: bad argument");
-  *bytes = sc_work_doubles(m->dim, m->C, mode) * 8 + 256;
+  *bytes = sc_work_doubles(m->dim, m->C, mode, m->F) * 8 + 256;
   return 0;
 }
 
@@ -1866,7 +1902,7 @@ int sc_vjp(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cf
   const bool geo = gb.area || gb.normal || gb.face_centroid || gb.volume || gb.cell_centroid;
   AFX_REQUIRE(!geo || (gb.area && gb.normal && gb.face_centroid && gb.volume && gb.cell_centroid),
               "afx_scalar_vjp: geometry cotangents need all five arrays");
-  ScWork w = sc_carve(work, m->dim, m->C, 2);
+  ScWork w = sc_carve(work, m->dim, m->C, 2, m->F);
   if (record) {
     const ScWork r = sc_record_view(record, m->dim, m->C);
     w.G = r.G;
@@ -1879,6 +1915,7 @@ int sc_vjp(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cf
   AFX_REQUIRE(cfg->transient != 2 || out->phi_older, "afx_scalar_vjp: BDF2 needs a phi_older cotangent");
   ScBars bar{out->phi_old, out->phi_older, out->phi, out->gamma, out->flux, out->value, out->t_inf, w.Gb, w.psib,
              w.extb, w.limx, w.limf, gb.area, gb.normal, gb.face_centroid, gb.volume, gb.cell_centroid};
+  bar.frec = geo ? w.frec : nullptr;
   return sc_dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
     constexpr int D = decltype(Dc)::value, ADV = decltype(Ac)::value, GR = decltype(Gc)::value;
     constexpr bool LIM = ADV == kAdvLimited;
