@@ -371,7 +371,7 @@ int afx_scalar_vjp(const afx_mesh* mesh, const afx_geometry* geom, const afx_sca
 /* The residual and its VJP split at a forward record, as jax.vjp(f, ...) splits into the primal and
  * a pullback that holds the linearisation point (reference call sites: jax.vjp of
  * ScalarTransport.residual(flux)(phi), aquaflux/transport/scalar.py:185-223).  The record keeps the
- * gradient pass's outputs (gradient, limiter value, the limiter's extremum cells:
+ * gradient pass's outputs (gradient, limiter value, the limiter's extremum cells and minimum face:
  * afx_scalar_record_bytes bytes), so afx_scalar_vjp_record does not run that pass again.  The
  * record is valid for the inputs and geometry of the call that wrote it; afx_scalar_vjp_record
  * returns what afx_scalar_vjp returns at those inputs.  Not for the corrected Green-Gauss scheme
