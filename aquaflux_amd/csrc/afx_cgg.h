@@ -33,8 +33,8 @@ struct LimiterSlots {
   int ext_stride;
   int var[8];
   int slot[8];
-  int32_t* kmin;  // nullable: kmin[slot[l] * C + c] = position in the cell's face list of the
-                  // first face attaining the minimum (the limiter adjoint re-evaluates only it)
+  int32_t* kmin;  // nullable: kmin[slot[l] * C + c] = the first face (internal index) attaining the
+                  // minimum (the limiter adjoint re-evaluates only it)
 };
 
 int venkat_pass(const MeshView& m, int dim, const LimiterSlots& sl, In<double> x, In<double> G, double lim_k,

@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(256) k_venkat(MeshView m, LimiterSlots sl, In<
       const S vf = venkat_face<S>(dm, mx - xc, mn - xc, e2);
       if (first || val(vf) < val(lo)) {
         lo = vf;
-        ks = k - m.cptr[c];
+        ks = f;
       }
       first = false;
     }

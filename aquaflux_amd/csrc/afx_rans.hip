@@ -322,7 +322,7 @@ AFX_DEV void k_rans_grad_body(const Exec& ex, Sh& sh, MeshView m, RansArgs<S> a,
         S vf = venkat_face<S, GT>(dm, mx[l] - xc[v], mn[l] - xc[v], e2);
         if (first || val(vf) < val(lo[l])) {
           lo[l] = vf;
-          ks[l] = k - kb;
+          ks[l] = f;
         }
       }
       first = false;
@@ -1240,7 +1240,7 @@ __global__ void __launch_bounds__(256, 3) k_rans_ap_adj(MeshView m, RansArgs<dou
 // reverse pass C': closure (seeded duals), wall shear rate, limiters; then the gradient cotangent
 // is pre-multiplied by the cell's reconstruction operator.
 template <int D, int ADVF, int ADVS, int GRAD>
-AFX_DEV void k_rans_closure_adj_body(int32_t c, MeshView m, RansArgs<double> a, const int32_t* __restrict__ ext, const int32_t* __restrict__ kmin_pos, const double* __restrict__ lsqI, RansBars bar) {
+AFX_DEV void k_rans_closure_adj_body(int32_t c, MeshView m, RansArgs<double> a, const int32_t* __restrict__ ext, const int32_t* __restrict__ kmin_face, const double* __restrict__ lsqI, RansBars bar) {
   constexpr int NA = D + 3, NL = D + 2;
   const SstConst& sc = a.sc;
   if (c >= m.C) return;
@@ -1364,8 +1364,7 @@ AFX_DEV void k_rans_closure_adj_body(int32_t c, MeshView m, RansArgs<double> a, 
 #pragma unroll
       for (int j = 0; j < D; ++j) Gv[j] = a.G[(int64_t)(v * D + j) * m.C + c];
       // the first face attaining the forward minimum, recorded by the forward limiter pass
-      const int32_t e = m.cface[m.cptr[c] + kmin_pos[(int64_t)l * m.C + c]];
-      const int32_t f = e >= 0 ? e : ~e;
+      const int32_t f = kmin_face[(int64_t)l * m.C + c];
       double dxf[D];
 #pragma unroll
       for (int j = 0; j < D; ++j) dxf[j] = m.fc[(int64_t)j * m.F + f] - xcen[j];
@@ -1450,9 +1449,9 @@ AFX_DEV void k_rans_closure_adj_body(int32_t c, MeshView m, RansArgs<double> a, 
 template <int D, int ADVF, int ADVS, int GRAD>
 __global__ void __launch_bounds__(256) k_rans_closure_adj(MeshView m, RansArgs<double> a,
                                                           const int32_t* __restrict__ ext,
-                                                          const int32_t* __restrict__ kmin_pos,
+                                                          const int32_t* __restrict__ kmin_face,
                                                           const double* __restrict__ lsqI, RansBars bar) {
-  k_rans_closure_adj_body<D, ADVF, ADVS, GRAD>(xcd_thread_index(m), m, a, ext, kmin_pos, lsqI, bar);
+  k_rans_closure_adj_body<D, ADVF, ADVS, GRAD>(xcd_thread_index(m), m, a, ext, kmin_face, lsqI, bar);
 }
 
 // reverse pass G'
@@ -1654,7 +1653,7 @@ struct RansGeoBars {
 template <int D, bool LFV, bool LSV, int GRAD>
 __global__ void __launch_bounds__(256) k_rans_geo_cell(MeshView m, RansArgs<double> a, const double* __restrict__ lam,
                                                        const int32_t* __restrict__ ext,
-                                                       const int32_t* __restrict__ kmin_pos, RansBars bar,
+                                                       const int32_t* __restrict__ kmin_face, RansBars bar,
                                                        RansGeoBars gb) {
   constexpr int NA = D + 3, NL = D + 2;
   const SstConst& sc = a.sc;
@@ -1749,8 +1748,7 @@ __global__ void __launch_bounds__(256) k_rans_geo_cell(MeshView m, RansArgs<doub
       double Gv[D];
 #pragma unroll
       for (int j = 0; j < D; ++j) Gv[j] = a.G[(int64_t)(v * D + j) * m.C + c];
-      const int32_t e = m.cface[m.cptr[c] + kmin_pos[(int64_t)l * m.C + c]];
-      const int32_t f = e >= 0 ? e : ~e;
+      const int32_t f = kmin_face[(int64_t)l * m.C + c];
       double dxf[D];
 #pragma unroll
       for (int j = 0; j < D; ++j) dxf[j] = m.fc[(int64_t)j * m.F + f] - xcen[j];
@@ -2105,7 +2103,7 @@ struct RWork {
   double *Gt, *psit, *aPt, *clt, *mdott;  // tangents
   double *Gb, *aPb, *psib, *extb, *mueffb, *nutb, *f1b, *Srb, *cdb, *taub, *nutwb, *mdotb;
   int32_t* ext;
-  int32_t* kmin;  // [NL][C] limiter argmin face positions (VJP only)
+  int32_t* kmin;  // [NL][C] limiter argmin faces (internal face index; VJP only)
   double* cgg;
   double *xs, *xst;  // physical state (and tangent) when k or omega is solved in log form
   double *limx, *tauA;  // mode 3

@@ -208,7 +208,7 @@ __global__ void __launch_bounds__(256) k_sc_grad(MeshView m, ScArgs<S> a, Out<S>
     const double* xcen = ccc;
     S lo = sconst<S>(0.0);
     bool first = true;
-    int32_t ks = 0;  // position in the face list of the first face attaining the minimum
+    int32_t ks = 0;  // the first face attaining the minimum
     for (int32_t k = kb; k < ke; ++k) {
       const int32_t e = m.cface[k];
       const int32_t f = e >= 0 ? e : ~e;
@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(256) k_sc_grad(MeshView m, ScArgs<S> a, Out<S>
       const S vf = venkat_face<S>(dm, mx - xc, mn - xc, e2);
       if (first || val(vf) < val(lo)) {
         lo = vf;
-        ks = k - kb;
+        ks = f;
       }
       first = false;
     }
@@ -670,7 +670,7 @@ AFX_DEV dual venkat_face_de2(double dm0, double hmax, double hmin, double e2) {
 // operator (GG: 1/V, LSQ: M^-1); GG also yields the volume cotangent -Gbar.G / V.
 template <int D, int GRAD, bool LIM, bool GEO>
 __global__ void __launch_bounds__(256) k_sc_lim_adj(MeshView m, ScArgs<double> a, const int32_t* __restrict__ ext,
-                                                    const int32_t* __restrict__ kmin_pos, ScBars bar) {
+                                                    const int32_t* __restrict__ kmin_face, ScBars bar) {
   const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   double Gb[D], Gc[D];
@@ -692,8 +692,7 @@ __global__ void __launch_bounds__(256) k_sc_lim_adj(MeshView m, ScArgs<double> a
     const double xv = a.phi[c], mxv = a.phi[amx], mnv = a.phi[amn];
     // the first face attaining the forward minimum, recorded by the forward limiter pass: only it
     // is evaluated again
-    const int32_t e = m.cface[m.cptr[c] + kmin_pos[c]];
-    const int32_t f = e >= 0 ? e : ~e;
+    const int32_t f = kmin_face[c];
     double dxf[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) dxf[j] = m.fc[(int64_t)j * m.F + f] - xcen[j];
@@ -1360,7 +1359,7 @@ struct ScWork {
   double *G, *psi, *Gt, *psit;
   int32_t* ext;
   double *Gb, *psib, *extb, *limx;
-  int32_t *limf, *kmin;  // kmin: the limiter's minimum face (position in the cell's face list)
+  int32_t *limf, *kmin;  // kmin: the limiter's minimum face (internal face index)
   double* frec;          // [F][kFaceRec] face records, 64-byte aligned (mode 2)
   double* cgg;
 };
