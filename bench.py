@@ -91,8 +91,8 @@ def _rans_kernel_bytes(D=3, limited=True):
     NV = D + 1
     flow_cell = NV * S8 + (D * D + D) * S8 + 2 * S8  # u, p, grad u, grad p, mu_eff, rho
     return {
-        # launched twice per step (residual, and the recompute inside the VJP, which also writes
-        # the extremum owners): the average of the two.  Unless corrected-GG sweeps follow, it also
+        # launched once per step by the fused residual + VJP call, which also has it write the
+        # extremum owners and the limiter's minimum faces.  Unless corrected-GG sweeps follow, it also
         # runs the cell-local closure from the gradient in registers: reads nu, mu, rho, d and the
         # wall flag (wall cells: the wall-distance gradient and their wall faces), writes the
         # seven closure fields
@@ -133,8 +133,8 @@ def _scalar_kernel_bytes(D=3):
     fgeo = (1 + 2 * D) * S8
     bnd = 1 + 2 * S8
     return {
-        # launched twice per step (residual, and the recompute inside the VJP that also writes the
-        # stencil-extremum owners)
+        # launched once per step: the residual's forward record (with the stencil-extremum owners and
+        # the limiter's minimum face) serves the VJP
         "scalar_grad": dict(c=4 + S8 + cgeo + D * S8 + S8 + 4, f=fp, b=bnd),
         "scalar_res": dict(c=4 + 3 * S8 + D * S8 + cgeo + S8, f=fp + S8, b=bnd),
         "scalar_res_adj": dict(c=4 + 4 * S8 + D * S8 + cgeo + 3 * S8 + D * S8 + D * S8 + S8, f=fp + 2 * S8 + fgeo,
