@@ -96,7 +96,7 @@ def _rans_kernel_bytes(D=3, limited=True):
         # runs the cell-local closure from the gradient in registers: reads nu, mu, rho, d and the
         # wall flag (wall cells: the wall-distance gradient and their wall faces), writes the
         # seven closure fields
-        "rans_grad": dict(c=4 + state + cgeo + G + psi + ext / 2 + 4 * S8 + 1 + 7 * S8, f=fp, b=bnd,
+        "rans_grad": dict(c=4 + state + cgeo + G + psi + ext + 4 * S8 + 1 + 7 * S8, f=fp, b=bnd,
                           w=D * S8 + D * S8, wf=1 + D * S8 + S8 + 4),
         "rans_closure": dict(c=4 + closure_in + 7 * S8, f=0, b=0, w=D * S8 + D * S8, wf=1 + D * S8 + S8 + 4),
         "rans_ap": dict(c=4 + flow_cell + cgeo + S8, f=fp, b=2, w=2 * S8),
