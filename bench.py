@@ -135,7 +135,7 @@ def _scalar_kernel_bytes(D=3):
     return {
         # launched once per step: the residual's forward record (with the stencil-extremum owners and
         # the limiter's minimum face) serves the VJP
-        "scalar_grad": dict(c=4 + S8 + cgeo + D * S8 + S8 + 4, f=fp, b=bnd),
+        "scalar_grad": dict(c=4 + S8 + cgeo + D * S8 + S8 + 2 * 4 + 4, f=fp, b=bnd),
         "scalar_res": dict(c=4 + 3 * S8 + D * S8 + cgeo + S8, f=fp + S8, b=bnd),
         "scalar_res_adj": dict(c=4 + 4 * S8 + D * S8 + cgeo + 3 * S8 + D * S8 + D * S8 + S8, f=fp + 2 * S8 + fgeo,
                                b=bnd + 2 * S8),
