@@ -1341,7 +1341,7 @@ extern "C" int afx_flow_vjp_geometry(const afx_mesh* m, const afx_geometry* g, c
   FlowArgs<double> a = make_args<double>(cfg, in, nullptr);
   FlowBars bar{out->state, w.Gb, w.aPb, w.psib, out->mu, out->rho, out->u_b, out->p_b, out->gamma_b, w.extb};
   FlowGeoBars gb{geom_out->area, geom_out->normal, geom_out->face_centroid, geom_out->volume,
-                 geom_out->cell_centroid, w.limx, w.limf};
+                 geom_out->cell_centroid, w.limx, w.limf, CggGeoBars{nullptr, nullptr}};
   return dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
     constexpr int D = decltype(Dc)::value, ADV = decltype(Ac)::value, GR = decltype(Gc)::value;
     return run_vjp_geometry<D, ADV, GR>(m, g, a, cotangent, bar, gb, w, cfg->gradient_sweeps,

@@ -486,12 +486,11 @@ template <int D, int ADV>
 AFX_DEV void adj_flux_boundary(const FaceGeo<D>& q, int kind, const FlowCell<D, double>& P, const double ub[D],
                                double pb, double gB, const double lam[D + 1], FlowCellBar<D>& B, double ub_bar[D],
                                double& pb_bar, double& gB_bar) {
-  double ubf[D], pf = 0.0, mdot = 0.0;
+  double ubf[D], mdot = 0.0;
   double dcP = P.V / P.aP, dh = 0.0, compact = 0.0, gn = 0.0, un = 0.0;
   if (kind == kFlowWall || kind == kFlowInlet) {
 #pragma unroll
     for (int i = 0; i < D; ++i) ubf[i] = ub[i];
-    pf = P.p;
     if (kind == kFlowInlet) {
       un = ub[0] * q.n[0];
 #pragma unroll
@@ -501,7 +500,6 @@ AFX_DEV void adj_flux_boundary(const FaceGeo<D>& q, int kind, const FlowCell<D, 
   } else if (kind == kFlowOutlet) {
 #pragma unroll
     for (int i = 0; i < D; ++i) ubf[i] = P.u[i];
-    pf = pb;
     un = P.u[0] * q.n[0];
     gn = P.Gp[0] * q.n[0];
 #pragma unroll

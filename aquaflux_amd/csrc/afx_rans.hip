@@ -1926,7 +1926,7 @@ __global__ void __launch_bounds__(256) k_rans_geo_face(MeshView m, RansArgs<doub
 // squares: the fit directions) and the limiters' face-centroid parts.  Adds to the arrays.
 template <int D, int GRAD, bool LFV, bool LSV>
 __global__ void __launch_bounds__(256) k_rans_geo_grad(MeshView m, RansArgs<double> a, RansBars bar, RansGeoBars gb) {
-  constexpr int NA = D + 3, NL = D + 2;
+  constexpr int NA = D + 3;
   const int32_t c = xcd_thread_index(m);
   if (c >= m.C) return;
   double ccb[D];

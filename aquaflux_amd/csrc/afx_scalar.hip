@@ -1913,8 +1913,8 @@ int sc_vjp(const afx_mesh* m, const afx_geometry* g, const afx_scalar_config* cf
   AFX_REQUIRE(cfg->transient == 0 || out->phi_old, "afx_scalar_vjp: the transient term needs a phi_old cotangent");
   AFX_REQUIRE(cfg->transient != 2 || out->phi_older, "afx_scalar_vjp: BDF2 needs a phi_older cotangent");
   ScBars bar{out->phi_old, out->phi_older, out->phi, out->gamma, out->flux, out->value, out->t_inf, w.Gb, w.psib,
-             w.extb, w.limx, w.limf, gb.area, gb.normal, gb.face_centroid, gb.volume, gb.cell_centroid};
-  bar.frec = geo ? w.frec : nullptr;
+             w.extb, w.limx, w.limf, gb.area, gb.normal, gb.face_centroid, gb.volume, gb.cell_centroid,
+             nullptr, nullptr, geo ? w.frec : nullptr};
   return sc_dispatch(m->dim, cfg->advection, cfg->gradient, [&](auto Dc, auto Ac, auto Gc) {
     constexpr int D = decltype(Dc)::value, ADV = decltype(Ac)::value, GR = decltype(Gc)::value;
     constexpr bool LIM = ADV == kAdvLimited;
